@@ -1,0 +1,180 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+The oracle is the checker for the HIP product path and the timed CPU
+baseline.  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+cpu_baseline leg import this module; the product never does.
+
+``build()`` compiles ``liboracle.so`` from ``wrfdyn_oracle.c`` with the
+Makefile beside it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+# Variant built with the reference's pre-v0.14 dycore constants (see Makefile).
+LEGACY_LIB_PATH = os.path.join(HERE, "liboracle_v013.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("c1h", _dp), ("c2h", _dp), ("c1f", _dp), ("c2f", _dp),
+        ("dnw", _dp), ("rdnw", _dp), ("dn", _dp), ("rdn", _dp),
+        ("fnm", _dp), ("fnp", _dp),
+        ("cf1", ctypes.c_double), ("cf2", ctypes.c_double), ("cf3", ctypes.c_double),
+        ("msftx", _dp), ("msfty", _dp), ("msfux", _dp), ("msfuy", _dp),
+        ("msfvx", _dp), ("msfvy", _dp),
+    ]
+
+
+class _Namelist(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("ns", ctypes.c_int), ("epssm", ctypes.c_double),
+        ("top_lid", ctypes.c_int), ("w_damping", ctypes.c_int), ("damp_opt", ctypes.c_int),
+        ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
+    ]
+
+
+class _State(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in (
+        "u", "v", "w", "theta", "p_total", "p_pert", "ph_total", "ph_pert",
+        "mu_total", "mu_pert", "mudf")]
+
+
+_STATE_KEYS = {
+    "u": "u", "v": "v", "w": "w", "theta": "theta", "p_total": "p_total",
+    "p_pert": "p_perturbation", "ph_total": "ph_total", "ph_pert": "ph_perturbation",
+    "mu_total": "mu_total", "mu_pert": "mu_perturbation", "mudf": "mudf",
+}
+
+_libs: dict = {}
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so (gcc, -ffp-contract=off) and return its path."""
+
+    if force or not (os.path.exists(LIB_PATH) and os.path.exists(LEGACY_LIB_PATH)):
+        cmd = ["make", "-s", "-C", HERE] + (["-B"] if force else [])
+        subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return LIB_PATH
+
+
+def lib(legacy_v013: bool = False) -> ctypes.CDLL:
+    path = LEGACY_LIB_PATH if legacy_v013 else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            build()
+        _lib = ctypes.CDLL(path)
+        _libs[path] = _lib
+        _lib.orc_run.restype = ctypes.c_int
+        _lib.orc_run.argtypes = [ctypes.POINTER(_Grid), ctypes.POINTER(_Namelist),
+                                 ctypes.POINTER(_State), ctypes.c_int]
+        _lib.orc_calc_coef_w.restype = ctypes.c_int
+        _lib.orc_thomas.restype = None
+        _lib.orc_alt_from_theta_p.restype = ctypes.c_double
+        _lib.orc_alt_from_theta_p.argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib.orc_p_from_theta_alt.restype = ctypes.c_double
+        _lib.orc_p_from_theta_alt.argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib.orc_advect_scalar.restype = ctypes.c_int
+        _lib.orc_set_threads.restype = None
+        _lib.orc_set_threads.argtypes = [ctypes.c_int]
+    return _libs[path]
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class _Keep:
+    """Holds the contiguous arrays referenced by a ctypes struct."""
+
+    def __init__(self):
+        self.arrays = []
+
+    def __call__(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.arrays.append(a)
+        return _ptr(a)
+
+
+def _grid(metrics, keep: _Keep) -> _Grid:
+    m = metrics
+    return _Grid(
+        m.nx, m.ny, m.nz, m.dx, m.dy,
+        keep(m.c1h), keep(m.c2h), keep(m.c1f), keep(m.c2f), keep(m.dnw), keep(m.rdnw),
+        keep(m.dn), keep(m.rdn), keep(m.fnm), keep(m.fnp), m.cf1, m.cf2, m.cf3,
+        keep(m.msftx), keep(m.msfty), keep(m.msfux), keep(m.msfuy), keep(m.msfvx), keep(m.msfvy),
+    )
+
+
+def run(metrics, namelist, state: dict, n_steps: int, threads: int | None = None,
+        legacy_v013: bool = False) -> dict:
+    """Advance ``n_steps`` large steps on copies of ``state``; returns the new state dict."""
+
+    L = lib(legacy_v013)
+    if threads is not None:
+        L.orc_set_threads(int(threads))
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    nl = _Namelist(float(namelist.dt_s), int(namelist.acoustic_substeps), float(namelist.epssm),
+                   int(bool(namelist.top_lid)), int(namelist.w_damping), int(namelist.damp_opt),
+                   float(namelist.dampcoef), float(namelist.zdamp))
+    out = {k: np.array(v, dtype=np.float64, copy=True, order="C") for k, v in state.items()}
+    if "mudf" not in out:
+        out["mudf"] = np.zeros((metrics.ny, metrics.nx))
+    s = _State(*[_ptr(out[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    rc = L.orc_run(ctypes.byref(g), ctypes.byref(nl), ctypes.byref(s), int(n_steps))
+    if rc != 0:
+        raise RuntimeError(f"orc_run failed with code {rc}")
+    return out
+
+
+def calc_coef_w(metrics, mut: np.ndarray, c2a: np.ndarray, dts: float, epssm: float, top_lid=True):
+    L = lib()
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    shape = (metrics.nz + 1, metrics.ny, metrics.nx)
+    a, alpha, gamma = (np.zeros(shape) for _ in range(3))
+    L.orc_calc_coef_w(ctypes.byref(g), keep(mut), keep(c2a), ctypes.c_double(dts),
+                      ctypes.c_double(epssm), ctypes.c_int(int(top_lid)), _ptr(a), _ptr(alpha),
+                      _ptr(gamma))
+    return a, alpha, gamma
+
+
+def thomas(a, alpha, gamma, rhs):
+    L = lib()
+    n = len(rhs)
+    out = np.zeros(n)
+    keep = _Keep()
+    L.orc_thomas(ctypes.c_int(n), keep(a), keep(alpha), keep(gamma), keep(rhs), _ptr(out))
+    return out
+
+
+def alt_from_theta_p(theta: float, p: float) -> float:
+    return float(lib().orc_alt_from_theta_p(theta, p))
+
+
+def p_from_theta_alt(theta: float, alt: float) -> float:
+    return float(lib().orc_p_from_theta_alt(theta, alt))
+
+
+def advect_scalar(metrics, u, v, mu_total, field) -> np.ndarray:
+    L = lib()
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    tend = np.zeros((metrics.nz, metrics.ny, metrics.nx))
+    rc = L.orc_advect_scalar(ctypes.byref(g), keep(u), keep(v), keep(mu_total), keep(field), _ptr(tend))
+    if rc != 0:
+        raise RuntimeError("orc_advect_scalar failed")
+    return tend

@@ -1,0 +1,78 @@
+/*
+ * CPU oracle for the split-explicit ARW dycore large step (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against; it is never linked into, loaded by or
+ * called from the product (wrf_gpu_amd/, include/).  Only tests/, the
+ * smoke() check in __graft_entry__.py and bench.py's cpu_baseline leg use it.
+ *
+ * It restates, function by function and in the same expression order, the
+ * reference's dry periodic idealized path (physics and lateral boundaries
+ * off, hypsometric_opt=1, unit or general map factors, f=e=0, q=0):
+ *   src/gpuwrf/runtime/operational_mode.py:3319-3534  _rk_scan_step
+ * Each function in wrfdyn_oracle.c cites the reference file:line it follows.
+ *
+ * Layout: (k, j, i) C order, i contiguous, exactly the reference's
+ *   u (nz, ny, nx+1), v (nz, ny+1, nx), w/ph (nz+1, ny, nx),
+ *   mass fields (nz, ny, nx), column fields (ny, nx).
+ */
+#ifndef WRFDYN_ORACLE_H
+#define WRFDYN_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy;
+    /* vertical metrics (contracts/grid.py:166-245) */
+    const double *c1h, *c2h;                 /* nz   */
+    const double *c1f, *c2f;                 /* nz+1 */
+    const double *dnw, *rdnw, *dn, *rdn;     /* nz   */
+    const double *fnm, *fnp;                 /* nz   */
+    double cf1, cf2, cf3;
+    /* map factors */
+    const double *msftx, *msfty;             /* ny*nx      */
+    const double *msfux, *msfuy;             /* ny*(nx+1)  */
+    const double *msfvx, *msfvy;             /* (ny+1)*nx  */
+} orc_grid;
+
+typedef struct {
+    double dt;          /* large step dt_s                          */
+    int    ns;          /* namelist acoustic_substeps               */
+    double epssm;
+    int    top_lid;     /* only 1 is supported                      */
+    int    w_damping;   /* 0 or 1                                   */
+    int    damp_opt;    /* 0 or 3                                   */
+    double dampcoef;
+    double zdamp;
+} orc_namelist;
+
+typedef struct {
+    double *u, *v, *w, *theta;
+    double *p_total, *p_pert;
+    double *ph_total, *ph_pert;
+    double *mu_total, *mu_pert;
+    double *mudf;
+} orc_state;
+
+/* Advance n_steps large steps in place.  Returns 0 or a negative error. */
+int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps);
+
+/* Single-operator entry points used by the oracle's own pinning tests. */
+int orc_calc_coef_w(const orc_grid *g, const double *mut, const double *c2a,
+                    double dts, double epssm, int top_lid,
+                    double *a, double *alpha, double *gamma);
+void orc_thomas(int nzp1, const double *a, const double *alpha, const double *gamma,
+                const double *rhs, double *out);
+double orc_alt_from_theta_p(double theta, double p);
+double orc_p_from_theta_alt(double theta, double alt);
+int orc_advect_scalar(const orc_grid *g, const double *u, const double *v,
+                      const double *mu_total, const double *field, double *tend);
+void orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
