@@ -86,6 +86,10 @@ def lib(legacy_v013: bool = False) -> ctypes.CDLL:
         _lib.orc_p_from_theta_alt.restype = ctypes.c_double
         _lib.orc_p_from_theta_alt.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_advect_scalar.restype = ctypes.c_int
+        _lib.orc_pow.restype = ctypes.c_double
+        _lib.orc_pow.argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib.orc_sin.restype = ctypes.c_double
+        _lib.orc_sin.argtypes = [ctypes.c_double]
         _lib.orc_set_threads.restype = None
         _lib.orc_set_threads.argtypes = [ctypes.c_int]
     return _libs[path]
@@ -167,6 +171,15 @@ def alt_from_theta_p(theta: float, p: float) -> float:
 
 def p_from_theta_alt(theta: float, alt: float) -> float:
     return float(lib().orc_p_from_theta_alt(theta, alt))
+
+
+def det_pow(x: float, y: float) -> float:
+    """The deterministic pow shared with the HIP kernels (include/wrfdyn_math.h)."""
+    return float(lib().orc_pow(x, y))
+
+
+def det_sin(x: float) -> float:
+    return float(lib().orc_sin(x))
 
 
 def advect_scalar(metrics, u, v, mu_total, field) -> np.ndarray:

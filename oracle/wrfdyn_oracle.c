@@ -19,6 +19,7 @@
  *   emdiv=0.01, smdiv=0.1 (core/acoustic.py:845-846), t0=300.
  */
 #include "wrfdyn_oracle.h"
+#include "wrfdyn_math.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -81,6 +82,11 @@ static inline double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x)
 /* jnp.where(|x|>floor, x, floor) */
 static inline double safe_pos(double x, double floor) { return fabs(x) > floor ? x : floor; }
 
+/* The deterministic elementary functions shared with the HIP kernels
+ * (include/wrfdyn_math.h), exported for their accuracy tests. */
+double orc_pow(double x, double y) { return wd_pow(x, y); }
+double orc_sin(double x) { return wd_sin(x); }
+
 void orc_set_threads(int n) {
 #ifdef _OPENMP
     if (n > 0) omp_set_num_threads(n);
@@ -97,7 +103,7 @@ void orc_set_threads(int n) {
  * (qvf = 1.0: state theta is theta_m). */
 double orc_alt_from_theta_p(double theta, double p) {
     double ps = p > MIN_PRESSURE ? p : MIN_PRESSURE;
-    return (((R_D / P0_PA) * theta) * 1.0) * pow(ps / P0_PA, CVPM);
+    return (((R_D / P0_PA) * theta) * 1.0) * wd_pow(ps / P0_PA, CVPM);
 }
 
 /* _pressure_from_theta_alt, dynamics/acoustic_wrf.py:198-211. */
@@ -105,7 +111,7 @@ double orc_p_from_theta_alt(double theta, double alt) {
     double as = alt > MIN_ALT ? alt : MIN_ALT;
     double arg = ((R_D * theta) * 1.0) / (P0_PA * as);
     double a2 = arg > 1.0e-12 ? arg : 1.0e-12;
-    return P0_PA * pow(a2, CPOVCV);
+    return P0_PA * wd_pow(a2, CPOVCV);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1242,7 +1248,7 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
                     double hbot = htop - hdepth;
                     for (int f = 1; f <= nz; ++f) {
                         double hk = PH1T(f) / G;
-                        double s = sin(((0.5 * PI_D) * (hk - hbot)) / hdepth);
+                        double s = wd_sin(((0.5 * PI_D) * (hk - hbot)) / hdepth);
                         double ramp = s * s;
                         double dampwt = hk >= hbot ? dampmag * ramp : 0.0;
                         double mass_f = g->c1f[f] * mut + g->c2f[f];

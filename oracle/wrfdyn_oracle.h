@@ -70,6 +70,8 @@ double orc_alt_from_theta_p(double theta, double p);
 double orc_p_from_theta_alt(double theta, double alt);
 int orc_advect_scalar(const orc_grid *g, const double *u, const double *v,
                       const double *mu_total, const double *field, double *tend);
+double orc_pow(double x, double y);
+double orc_sin(double x);
 void orc_set_threads(int n);
 
 #ifdef __cplusplus

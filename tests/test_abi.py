@@ -1,0 +1,92 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every symbol
+include/wrfdyn.h declares, the ctypes structs match the C layout, and the
+argument checks that run before any device call return the documented codes.
+"""
+
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from wrf_gpu_amd import _lib
+from wrf_gpu_amd.grid import Metrics, Namelist
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wrfdyn.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(wd_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _declared_functions()
+    assert set(declared) == set(_lib.EXPORTED_SYMBOLS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared:
+        assert re.search(rf"\bT {name}\b", out), name
+
+
+def test_ctypes_structs_match_c_layout():
+    src = r"""
+    #include <stdio.h>
+    #include <stddef.h>
+    #include "wrfdyn.h"
+    int main(void) {
+      printf("%zu %zu %zu\n", sizeof(wd_grid), sizeof(wd_namelist), sizeof(wd_state));
+      printf("%zu %zu %zu %zu\n", offsetof(wd_grid, cf1), offsetof(wd_grid, msfvy),
+             offsetof(wd_namelist, dampcoef), offsetof(wd_state, mudf));
+      return 0;
+    }
+    """
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        sizes, offs = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.splitlines()
+    s = [int(x) for x in sizes.split()]
+    o = [int(x) for x in offs.split()]
+    assert s == [ctypes.sizeof(_lib.WdGrid), ctypes.sizeof(_lib.WdNamelist), ctypes.sizeof(_lib.WdState)]
+    assert o == [_lib.WdGrid.cf1.offset, _lib.WdGrid.msfvy.offset, _lib.WdNamelist.dampcoef.offset,
+                 _lib.WdState.mudf.offset]
+
+
+def test_create_rejects_bad_arguments_without_touching_the_device():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.wd_create(None, None, ctypes.byref(h)) == _lib.WD_EINVAL
+    assert "NULL" in _lib.last_error(None)
+    grid = _lib.WdGrid()
+    grid.nx, grid.ny, grid.nz = 3, 1, 10  # nx < 4
+    nl = _lib.WdNamelist()
+    assert lib.wd_create(ctypes.byref(grid), ctypes.byref(nl), ctypes.byref(h)) == _lib.WD_EINVAL
+    assert "grid too small" in _lib.last_error(None)
+    assert lib.wd_step(None, None, 1, None) == _lib.WD_EINVAL
+    assert lib.wd_workspace_bytes(None) == 0
+    lib.wd_destroy(None)
+
+
+def test_namelist_validation_matches_supported_path():
+    nl = Namelist(dt_s=6.0, acoustic_substeps=6)
+    nl.validate()
+    for bad in (dict(top_lid=False), dict(hypsometric_opt=2), dict(damp_opt=1), dict(w_damping=2),
+                dict(acoustic_substeps=0), dict(run_physics=True), dict(h_sca_adv_order=5)):
+        with pytest.raises(ValueError):
+            nl.replace(**bad).validate()
+
+
+def test_metrics_shapes():
+    m = Metrics.flat(nx=5, ny=4, nz=6, dx=1.0, dy=1.0, eta=[1.0, 0.8, 0.6, 0.45, 0.3, 0.15, 0.0], p_top=5000.0)
+    sh = m.shapes()
+    assert sh["u"] == (6, 4, 6) and sh["v"] == (6, 5, 5) and sh["w"] == (7, 4, 5)
+    assert m.dnw[0] < 0 and m.rdnw[0] < 0  # WRF-signed metric
+    with pytest.raises(ValueError):
+        Metrics.flat(nx=5, ny=4, nz=3, dx=1.0, dy=1.0, eta=[1.0, 0.6, 0.3, 0.0], p_top=5000.0)

@@ -1,0 +1,90 @@
+"""ctypes binding of libwrfdyn.so (the C ABI in include/wrfdyn.h).
+
+There is no fallback: if the HIP library is missing this raises, so a GPU
+run can never silently take another path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwrfdyn.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+WD_OK = 0
+WD_EINVAL = -1
+WD_ENOMEM = -2
+WD_EDEVICE = -3
+
+EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error", "wd_destroy")
+
+
+class WdGrid(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("c1h", _dp), ("c2h", _dp), ("c1f", _dp), ("c2f", _dp),
+        ("dnw", _dp), ("rdnw", _dp), ("dn", _dp), ("rdn", _dp), ("fnm", _dp), ("fnp", _dp),
+        ("cf1", ctypes.c_double), ("cf2", ctypes.c_double), ("cf3", ctypes.c_double),
+        ("msftx", _dp), ("msfty", _dp), ("msfux", _dp), ("msfuy", _dp), ("msfvx", _dp), ("msfvy", _dp),
+    ]
+
+
+class WdNamelist(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("acoustic_substeps", ctypes.c_int), ("epssm", ctypes.c_double),
+        ("top_lid", ctypes.c_int), ("w_damping", ctypes.c_int), ("damp_opt", ctypes.c_int),
+        ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
+    ]
+
+
+class WdState(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "u", "v", "w", "theta", "p_total", "p_pert", "ph_total", "ph_pert", "mu_total", "mu_pert", "mudf")]
+
+
+_lib = None
+
+
+class WrfdynError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"wrfdyn error {code}: {message}")
+        self.code = code
+
+
+def load() -> ctypes.CDLL:
+    """Load libwrfdyn.so; raises if it has not been built."""
+
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP extension first "
+                "(python -c 'import __graft_entry__ as e; e.build()')")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.wd_create.restype = ctypes.c_int
+        lib.wd_create.argtypes = [ctypes.POINTER(WdGrid), ctypes.POINTER(WdNamelist),
+                                  ctypes.POINTER(ctypes.c_void_p)]
+        lib.wd_step.restype = ctypes.c_int
+        lib.wd_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(WdState), ctypes.c_int, ctypes.c_void_p]
+        lib.wd_workspace_bytes.restype = ctypes.c_size_t
+        lib.wd_workspace_bytes.argtypes = [ctypes.c_void_p]
+        lib.wd_last_error.restype = ctypes.c_char_p
+        lib.wd_last_error.argtypes = [ctypes.c_void_p]
+        lib.wd_destroy.restype = None
+        lib.wd_destroy.argtypes = [ctypes.c_void_p]
+        _lib = lib
+    return _lib
+
+
+def last_error(handle=None) -> str:
+    msg = load().wd_last_error(handle)
+    return msg.decode() if msg else ""
+
+
+def check(code: int, handle=None) -> None:
+    if code != WD_OK:
+        raise WrfdynError(code, last_error(handle))

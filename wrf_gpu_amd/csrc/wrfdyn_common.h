@@ -1,0 +1,136 @@
+// Shared device definitions for the wrfdyn HIP kernels (gfx950, FP64).
+//
+// Physical constants follow the reference dycore:
+//   src/gpuwrf/dynamics/acoustic_wrf.py:27-75 (R_D, CP_D=7R/2, P0, g=9.81,
+//   MIN_ACOUSTIC_ALT=0.1), core/advance_w.py:31 (g), core/acoustic.py:845-846
+//   (emdiv=0.01, smdiv=0.1), runtime/operational_mode.py:202 (1/6370e3).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "wrfdyn_math.h"
+
+namespace wd {
+
+constexpr double kRd = 287.0;
+constexpr double kCp = 7.0 * kRd / 2.0;
+constexpr double kP0 = 100000.0;
+constexpr double kG = 9.81;
+constexpr double kT0 = 300.0;
+constexpr double kEmdiv = 0.01;
+constexpr double kSmdiv = 0.1;
+constexpr double kMinAcousticAlt = 0.1;
+constexpr double kMinAlt = 1.0e-8;
+constexpr double kMinPressure = 1.0;
+constexpr double kWAlpha = 0.3;
+constexpr double kWBeta = 1.0;
+constexpr double kCpovcv = kCp / (kCp - kRd);
+constexpr double kCvpm = -(kCp - kRd) / kCp;
+constexpr double kReradius = 1.0 / 6370.0e3;
+constexpr double kPi = 3.141592653589793;
+
+// Grid + metrics as passed to every kernel (by value, kernarg segment).
+struct DGrid {
+  int nx, ny, nz;
+  double dx, dy, rdx, rdy;
+  const double *c1h, *c2h, *c1f, *c2f;
+  const double *dnw, *rdnw, *dn, *rdn, *fnm, *fnp;
+  double cf1, cf2, cf3, cfn, cfn1;
+  const double *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy;
+};
+
+struct DNamelist {
+  double dt;
+  int ns;
+  double epssm;
+  int top_lid, w_damping, damp_opt;
+  double dampcoef, zdamp;
+};
+
+// Caller state (device pointers).
+struct DState {
+  double *u, *v, *w, *theta, *p_total, *p_pert, *ph_total, *ph_pert, *mu_total, *mu_pert, *mudf;
+};
+
+// Per-handle device workspace.
+struct DWork {
+  // RK1 reference (step origin) copies
+  double *u0, *v0, *w0, *th0, *ph0, *mu0;
+  // stage transport velocities
+  double *ru, *rv, *rom;
+  // large-step tendencies
+  double *ut, *vt, *wt, *tht;
+  // stage diagnostics (large-step PGF)
+  double *alt_raw, *al_ls, *php_ls;
+  // prep (2-D)
+  double *mub, *mut, *muts0, *mu_work0, *mu, *muts, *muave;
+  double *muu, *muus, *muv, *muvs;
+  // prep (3-D)
+  double *u_work, *v_work, *w_work, *th_work, *ph_work;
+  double *c2a, *alt, *pb, *phb, *php;
+  double *p, *al, *pm1;
+  // stage column setup
+  double *rw_tend, *ph_tend, *ca, *calpha, *cgamma;
+  // acoustic carried
+  double *ww, *t2ave;
+  // column-kernel scratch
+  double *dvdxi, *rhs;
+};
+
+__host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {
+  return ((size_t)k * g.ny + (size_t)j) * g.nx + (size_t)i;
+}
+__host__ __device__ inline size_t iu(const DGrid& g, int k, int j, int i) {
+  return ((size_t)k * g.ny + (size_t)j) * (g.nx + 1) + (size_t)i;
+}
+__host__ __device__ inline size_t iv(const DGrid& g, int k, int j, int i) {
+  return ((size_t)k * (g.ny + 1) + (size_t)j) * g.nx + (size_t)i;
+}
+__host__ __device__ inline size_t i2(const DGrid& g, int j, int i) { return (size_t)j * g.nx + (size_t)i; }
+__host__ __device__ inline size_t i2u(const DGrid& g, int j, int i) { return (size_t)j * (g.nx + 1) + (size_t)i; }
+__host__ __device__ inline size_t i2v(const DGrid& g, int j, int i) { return (size_t)j * g.nx + (size_t)i; }
+
+__device__ inline int wrapi(int i, int n) {
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+__device__ inline int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
+// jnp.sign: -1, +1, and the argument itself for +-0 and NaN.
+__device__ inline double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+// jnp.where(|x| > floor, x, floor)
+__device__ inline double safe_pos(double x, double floor) { return fabs(x) > floor ? x : floor; }
+
+// _inverse_density_from_theta_pressure, dynamics/acoustic_wrf.py:178-195 (qvf=1).
+__device__ inline double alt_from_theta_p(double theta, double p) {
+  double ps = p > kMinPressure ? p : kMinPressure;
+  return (((kRd / kP0) * theta) * 1.0) * wd_pow(ps / kP0, kCvpm);
+}
+// _pressure_from_theta_alt, dynamics/acoustic_wrf.py:198-211.
+__device__ inline double p_from_theta_alt(double theta, double alt) {
+  double as = alt > kMinAlt ? alt : kMinAlt;
+  double arg = ((kRd * theta) * 1.0) / (kP0 * as);
+  double a2 = arg > 1.0e-12 ? arg : 1.0e-12;
+  return kP0 * wd_pow(a2, kCpovcv);
+}
+
+// flux5_face_periodic, dynamics/flux_advection.py:53-88.
+__device__ inline double flux5(double q_im3, double q_im2, double q_im1, double q_i, double q_ip1,
+                               double q_ip2, double vel) {
+  double f6 = ((37.0 * (q_i + q_im1) - 8.0 * (q_ip1 + q_im2)) + (q_ip2 + q_im3)) / 60.0;
+  double corr = (((q_ip2 - q_im3) - 5.0 * (q_ip1 - q_im2)) + 10.0 * (q_i - q_im1)) / 60.0;
+  return f6 - sgn(vel) * corr;
+}
+// flux3 with caller-chosen upwind velocity (flux_advection.py:537-547, 1388-1397, 1452-1467).
+__device__ inline double flux3(double q_km2, double q_km1, double q_k, double q_kp1, double velz) {
+  double f4 = (7.0 * (q_k + q_km1) - (q_kp1 + q_km2)) / 12.0;
+  double corr = ((q_kp1 - q_km2) - 3.0 * (q_k - q_km1)) / 12.0;
+  return f4 + sgn(velz) * corr;
+}
+
+// post-pg_buoy_w cqw with qtot=0: 0 at faces 0 and nz, 1 inside
+// (core/advance_w.py:266-290).
+__device__ inline double cqw_at(int kf, int nz) { return (kf >= 1 && kf < nz) ? 1.0 : 0.0; }
+
+}  // namespace wd

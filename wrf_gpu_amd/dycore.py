@@ -1,0 +1,160 @@
+"""Host API of the MI355X dycore large step.
+
+Mirrors the reference's step call: a state (u, v, w, theta, p, ph, mu and the
+carried mudf) plus namelist parameters go in, the advanced state comes out --
+``_physics_boundary_step(carry, namelist, step_index)`` with physics and
+boundaries off (src/gpuwrf/runtime/operational_mode.py:4996-5013), whose core
+is ``_rk_scan_step`` (:3319-3534).  The reference returns new arrays
+(functional JAX, donating the state at its forecast entry,
+operational_mode.py:5584); here the caller's device tensors are advanced in
+place and returned, which is the donated-buffer behaviour.
+
+Errors follow the reference: an unsupported namelist or mismatched state
+raises ``ValueError`` before any device work; nothing is raised from inside
+the step (NaNs propagate, as in the reference with guards off).
+
+PyTorch only provides device memory and streams; all arithmetic runs in the
+HIP kernels of libwrfdyn.so.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .grid import STATE_FIELDS, Metrics, Namelist
+
+# Reference State field names -> C-ABI wd_state member.
+_ABI_FIELDS = (
+    ("u", "u"), ("v", "v"), ("w", "w"), ("theta", "theta"), ("p_total", "p_total"),
+    ("p_perturbation", "p_pert"), ("ph_total", "ph_total"), ("ph_perturbation", "ph_pert"),
+    ("mu_total", "mu_total"), ("mu_perturbation", "mu_pert"), ("mudf", "mudf"),
+)
+
+
+def _c_array(a: np.ndarray, keep: list):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    keep.append(a)
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Dycore:
+    """A compiled dycore for one (grid, namelist): the jit-cache entry analogue.
+
+    ``step(state, n_steps)`` advances the state dict of float64 device
+    tensors in place.
+    """
+
+    def __init__(self, metrics: Metrics, namelist: Namelist):
+        namelist.validate()
+        self.metrics = metrics
+        self.namelist = namelist
+        lib = _lib.load()
+        keep: list = []
+        m = metrics
+        self._grid = _lib.WdGrid(
+            m.nx, m.ny, m.nz, m.dx, m.dy,
+            _c_array(m.c1h, keep), _c_array(m.c2h, keep), _c_array(m.c1f, keep), _c_array(m.c2f, keep),
+            _c_array(m.dnw, keep), _c_array(m.rdnw, keep), _c_array(m.dn, keep), _c_array(m.rdn, keep),
+            _c_array(m.fnm, keep), _c_array(m.fnp, keep), m.cf1, m.cf2, m.cf3,
+            _c_array(m.msftx, keep), _c_array(m.msfty, keep), _c_array(m.msfux, keep),
+            _c_array(m.msfuy, keep), _c_array(m.msfvx, keep), _c_array(m.msfvy, keep),
+        )
+        nl = namelist
+        self._nl = _lib.WdNamelist(float(nl.dt_s), int(nl.acoustic_substeps), float(nl.epssm),
+                                   int(bool(nl.top_lid)), int(nl.w_damping), int(nl.damp_opt),
+                                   float(nl.dampcoef), float(nl.zdamp))
+        handle = ctypes.c_void_p()
+        code = lib.wd_create(ctypes.byref(self._grid), ctypes.byref(self._nl), ctypes.byref(handle))
+        if code != _lib.WD_OK:
+            raise _lib.WrfdynError(code, _lib.last_error(None))
+        self._handle = handle
+        self._lib = lib
+
+    @property
+    def workspace_bytes(self) -> int:
+        return int(self._lib.wd_workspace_bytes(self._handle))
+
+    def check_state(self, state: dict) -> None:
+        import torch
+
+        shapes = self.metrics.shapes()
+        missing = [k for k in STATE_FIELDS if k not in state]
+        if missing:
+            raise ValueError(f"state is missing fields {missing}")
+        dev = None
+        for name in STATE_FIELDS:
+            t = state[name]
+            if not isinstance(t, torch.Tensor):
+                raise ValueError(f"state[{name!r}] must be a torch tensor")
+            if t.dtype != torch.float64:
+                raise ValueError(f"state[{name!r}] must be float64 (force_fp64 path), got {t.dtype}")
+            if tuple(t.shape) != shapes[name]:
+                raise ValueError(f"state[{name!r}] has shape {tuple(t.shape)}, expected {shapes[name]}")
+            if not t.is_cuda:
+                raise ValueError(f"state[{name!r}] must live on the GPU")
+            if not t.is_contiguous():
+                raise ValueError(f"state[{name!r}] must be contiguous")
+            dev = t.device if dev is None else dev
+            if t.device != dev:
+                raise ValueError("all state tensors must be on the same device")
+
+    def step(self, state: dict, n_steps: int = 1, stream=None) -> dict:
+        """Advance ``n_steps`` large steps in place on ``stream`` (default: torch's current)."""
+
+        import torch
+
+        self.check_state(state)
+        if n_steps < 0:
+            raise ValueError("n_steps must be >= 0")
+        if stream is None:
+            stream = torch.cuda.current_stream(state["u"].device)
+        ptrs = _lib.WdState(*[ctypes.c_void_p(state[k].data_ptr()) for k, _ in _ABI_FIELDS])
+        code = self._lib.wd_step(self._handle, ctypes.byref(ptrs), int(n_steps),
+                                 ctypes.c_void_p(stream.cuda_stream))
+        if code != _lib.WD_OK:
+            raise _lib.WrfdynError(code, _lib.last_error(self._handle))
+        return state
+
+    def close(self) -> None:
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            self._lib.wd_destroy(h)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def state_to_device(state: dict, device="cuda") -> dict:
+    """Copy a dict of numpy arrays (the IC builders' output) to float64 device tensors."""
+
+    import torch
+
+    return {k: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(device)
+            for k, v in state.items()}
+
+
+def state_to_host(state: dict) -> dict:
+    return {k: v.detach().cpu().numpy() for k, v in state.items()}
+
+
+def dycore_step(state: dict, namelist: Namelist, metrics: Metrics, n_steps: int = 1) -> dict:
+    """One-shot drop-in for the reference step call: state dict in, advanced state out.
+
+    For repeated calls build a :class:`Dycore` once (the reference's jit
+    cache) and call ``step``.
+    """
+
+    import torch
+
+    dyn = Dycore(metrics, namelist)
+    out = dyn.step(state, n_steps)
+    torch.cuda.synchronize(state["u"].device)
+    dyn.close()
+    return out

@@ -44,12 +44,13 @@ class Case:
     info: dict
 
 
-def alpha_dry(theta, pressure):
+def alpha_dry(theta, pressure, cp=CP_DRY):
     """``_alpha_dry``, idealized.py:61-69: alt = (R/p0)*theta*(p/p0)^(-cv/cp)."""
 
     theta = np.asarray(theta, dtype=np.float64)
     pressure = np.asarray(pressure, dtype=np.float64)
-    return (R_DRY / P0) * theta * (pressure / P0) ** (-CV_DRY / CP_DRY)
+    cv = cp - R_DRY
+    return (R_DRY / P0) * theta * (pressure / P0) ** (-cv / cp)
 
 
 def centered_cosine(radius):
@@ -80,7 +81,8 @@ def uniform_z_hydrostatic_base(z_face, theta0, p_surface=P0):
     return eta, p_mass, ph_face, mu
 
 
-def _balanced_state(metrics: Metrics, *, mub, pb, alb, theta_full, phb_surface, u0=0.0, v0=0.0):
+def _balanced_state(metrics: Metrics, *, mub, pb, alb, theta_full, phb_surface, u0=0.0, v0=0.0,
+                    cp=CP_DRY):
     """Per-column restatement of ``_make_state`` (idealized.py:468-563).
 
     ``mub`` (ny, nx) base dry mass, ``pb`` (nz, ny, nx) base pressure at mass
@@ -92,7 +94,7 @@ def _balanced_state(metrics: Metrics, *, mub, pb, alb, theta_full, phb_surface, 
     nz, ny, nx = metrics.nz, metrics.ny, metrics.nx
     wrf_dnw = metrics.eta[1:] - metrics.eta[:-1]
     mass_h = np.asarray(mub, dtype=np.float64)
-    alt_full = alpha_dry(theta_full, pb)
+    alt_full = alpha_dry(theta_full, pb, cp)
     al = alt_full - alb
     phb = np.zeros((nz + 1, ny, nx), dtype=np.float64)
     ph_pert = np.zeros((nz + 1, ny, nx), dtype=np.float64)
@@ -103,7 +105,7 @@ def _balanced_state(metrics: Metrics, *, mub, pb, alb, theta_full, phb_surface, 
     wrf_rdnw = 1.0 / wrf_dnw
     al_diag = -(wrf_rdnw[:, None, None] * (ph_pert[1:] - ph_pert[:-1])) / mass_h
     alt_diag = al_diag + alb
-    cpovcv = CP_DRY / CV_DRY
+    cpovcv = cp / (cp - R_DRY)
     p_full = P0 * ((R_DRY * theta_full) / (P0 * alt_diag)) ** cpovcv
     p_pert = p_full - pb
     state = {
@@ -151,7 +153,7 @@ def warm_bubble_2d() -> Case:
                 {"x_m": x, "z_m": z, "snapshot_steps": (1000, 2500, 5000)})
 
 
-def _column_base(metrics: Metrics, mub, phb_surface, theta_of_z, iterations=8):
+def _column_base(metrics: Metrics, mub, phb_surface, theta_of_z, iterations=8, cp=CP_DRY):
     """Hydrostatic base columns on the eta levels for a theta(z) profile.
 
     pb = eta_mass*mub + p_top (pure sigma), alb = alpha(theta(z_mass), pb),
@@ -167,18 +169,18 @@ def _column_base(metrics: Metrics, mub, phb_surface, theta_of_z, iterations=8):
     # first guess: isothermal-ish heights from the pressure
     z_mass = (phb_surface / GRAVITY)[None] + 7000.0 * np.log(pb[0:1] / pb)
     for _ in range(int(iterations)):
-        alb = alpha_dry(theta_of_z(z_mass), pb)
+        alb = alpha_dry(theta_of_z(z_mass), pb, cp)
         phb = np.empty((nz + 1,) + mub.shape)
         phb[0] = phb_surface
         for k in range(nz):
             phb[k + 1] = phb[k] - dnw[k] * mub * alb[k]
         z_mass = 0.5 * (phb[:-1] + phb[1:]) / GRAVITY
-    alb = alpha_dry(theta_of_z(z_mass), pb)
+    alb = alpha_dry(theta_of_z(z_mass), pb, cp)
     return pb, alb, z_mass
 
 
 def bubble_3d(nx=64, ny=64, nz=40, dx=1000.0, p_top=5000.0, u0=10.0, noise=0.01, seed=1,
-              dt=6.0, ns=6, zdamp=5000.0) -> Case:
+              dt=6.0, ns=6, zdamp=5000.0, amplitude=2.0, cp=CP_DRY, dy=None) -> Case:
     """Plain FP64 dry 3-D warm bubble (the 64x64x40 CPU-runnable workload).
 
     Flat, uniform eta, neutral theta0=300 K hydrostatic from 1e5 Pa (mu=95000 Pa),
@@ -186,23 +188,28 @@ def bubble_3d(nx=64, ny=64, nz=40, dx=1000.0, p_top=5000.0, u0=10.0, noise=0.01,
     uniform u, seeded uniform +-noise K theta perturbation.
     """
 
+    dy = dx if dy is None else dy
     eta = np.linspace(1.0, 0.0, nz + 1)
-    metrics = Metrics.flat(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, eta=eta, p_top=p_top)
+    metrics = Metrics.flat(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, eta=eta, p_top=p_top)
     mub = np.full((ny, nx), P0 - p_top)
-    pb, alb, z_mass = _column_base(metrics, mub, np.zeros((ny, nx)), lambda z: np.full_like(z, THETA0))
+    pb, alb, z_mass = _column_base(metrics, mub, np.zeros((ny, nx)), lambda z: np.full_like(z, THETA0),
+                                   cp=cp)
     x = (np.arange(nx) + 0.5) * dx
-    y = (np.arange(ny) + 0.5) * dx
-    xc, yc = 0.5 * nx * dx, 0.5 * ny * dx
+    y = (np.arange(ny) + 0.5) * dy
+    xc, yc = 0.5 * nx * dx, 0.5 * ny * dy
     r = np.sqrt(((x[None, None, :] - xc) / 2000.0) ** 2 + ((y[None, :, None] - yc) / 2000.0) ** 2
                 + ((z_mass - 2000.0) / 2000.0) ** 2)
     tp = 2.0 * centered_cosine(r)
-    if np.max(tp) > 0:
-        tp *= 2.0 / float(np.max(tp))
-    rng = np.random.default_rng(seed)
-    tp = tp + rng.uniform(-noise, noise, size=(nz, ny, nx))
+    if np.max(tp) > 0 and amplitude != 0.0:
+        tp *= amplitude / float(np.max(tp))
+    else:
+        tp *= 0.0
+    if noise > 0.0:
+        rng = np.random.default_rng(seed)
+        tp = tp + rng.uniform(-noise, noise, size=(nz, ny, nx))
     theta = THETA0 + tp
     state = _balanced_state(metrics, mub=mub, pb=pb, alb=alb, theta_full=theta,
-                            phb_surface=np.zeros((ny, nx)), u0=u0)
+                            phb_surface=np.zeros((ny, nx)), u0=u0, cp=cp)
     nl = Namelist(dt_s=float(dt), acoustic_substeps=int(ns), zdamp=float(zdamp))
     return Case(f"bubble_{nx}x{ny}x{nz}", metrics, nl, state, 100, {"seed": seed})
 
