@@ -88,6 +88,18 @@ int wd_create(const wd_grid *grid, const wd_namelist *nl, wd_handle **out);
  * (operational_mode.py:3319-3534 per step).  Asynchronous. */
 int wd_step(wd_handle *h, const wd_state *state, int n_large_steps, void *stream);
 
+/* Kernel-duration instrumentation for benchmarking (no counterpart in the
+ * reference, whose step is one XLA program).  Bit `id` of `kernel_mask`
+ * brackets every launch of kernel `id` in subsequent wd_step calls with HIP
+ * events on the launch stream; mask 0 disables it.  wd_profile_read waits
+ * for the recorded events, writes per-kernel summed milliseconds and launch
+ * counts for ids < n, clears the records and returns the number of kernel
+ * ids (or a negative WD_E* code). */
+int wd_profile_enable(wd_handle *h, unsigned long long kernel_mask);
+int wd_profile_read(wd_handle *h, double *total_ms, long long *launches, int n);
+int wd_kernel_count(void);
+const char *wd_kernel_name(int id);
+
 /* Device workspace bytes the handle holds (for capacity planning). */
 size_t wd_workspace_bytes(const wd_handle *h);
 

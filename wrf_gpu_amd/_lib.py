@@ -19,7 +19,8 @@ WD_EINVAL = -1
 WD_ENOMEM = -2
 WD_EDEVICE = -3
 
-EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error", "wd_destroy")
+EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error", "wd_destroy",
+                    "wd_profile_enable", "wd_profile_read", "wd_kernel_count", "wd_kernel_name")
 
 
 class WdGrid(ctypes.Structure):
@@ -76,8 +77,22 @@ def load() -> ctypes.CDLL:
         lib.wd_last_error.argtypes = [ctypes.c_void_p]
         lib.wd_destroy.restype = None
         lib.wd_destroy.argtypes = [ctypes.c_void_p]
+        lib.wd_profile_enable.restype = ctypes.c_int
+        lib.wd_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong]
+        lib.wd_profile_read.restype = ctypes.c_int
+        lib.wd_profile_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        lib.wd_kernel_count.restype = ctypes.c_int
+        lib.wd_kernel_count.argtypes = []
+        lib.wd_kernel_name.restype = ctypes.c_char_p
+        lib.wd_kernel_name.argtypes = [ctypes.c_int]
         _lib = lib
     return _lib
+
+
+def kernel_names() -> list:
+    lib = load()
+    return [lib.wd_kernel_name(i).decode() for i in range(lib.wd_kernel_count())]
 
 
 def last_error(handle=None) -> str:

@@ -415,28 +415,36 @@ inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 
 }  // namespace
 
 hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
-                                   double dts, hipStream_t st) {
+                                   double dts, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
   const size_t U = (size_t)g.nz * g.ny * (g.nx + 1);
   const size_t V = (size_t)g.nz * (g.ny + 1) * g.nx;
-  k_uv_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w, dts);
-  k_uv_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w, dts);
+  WD_TIMED(prof, KID_UV_U, st, (k_uv_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w, dts)));
+  WD_TIMED(prof, KID_UV_V, st, (k_uv_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w, dts)));
   dim3 colgrid((g.nx + 63) / 64, g.ny);
-  k_column<<<colgrid, 64, 0, st>>>(g, nl, s, w, dts);
+  WD_TIMED(prof, KID_COLUMN, st, (k_column<<<colgrid, 64, 0, st>>>(g, nl, s, w, dts)));
   return hipGetLastError();
 }
 
-hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st) {
+hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
   const size_t M = (size_t)g.nz * g.ny * g.nx;
   const size_t Wn = (size_t)(g.nz + 1) * g.ny * g.nx;
   const size_t UV = (size_t)g.nz * g.ny * (g.nx + 1) + (size_t)g.nz * (g.ny + 1) * g.nx;
   const size_t C = (size_t)g.ny * g.nx;
-  k_fin_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w);
-  k_fin_uv<<<blocks_for(UV, bs), bs, 0, st>>>(g, s, w);
-  k_fin_m<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w);
-  k_fin_mu<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w);
+  WD_TIMED(prof, KID_FIN_W, st, (k_fin_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_FIN_UV, st, (k_fin_uv<<<blocks_for(UV, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_FIN_M, st, (k_fin_m<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_FIN_MU, st, (k_fin_mu<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   return hipGetLastError();
+}
+
+const char* kernel_name(int id) {
+  static const char* names[KID_COUNT] = {
+      "origin_copy", "k_ruv", "k_rom", "k_diag", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t",
+      "k_prep_2d", "k_prep_faces", "k_prep_u", "k_prep_v", "k_prep_w", "k_prep_m", "k_col_setup",
+      "k_uv_u", "k_uv_v", "k_column", "k_fin_w", "k_fin_uv", "k_fin_m", "k_fin_mu"};
+  return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 
 }  // namespace wd

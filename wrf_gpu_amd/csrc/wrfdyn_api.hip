@@ -25,6 +25,7 @@ struct wd_handle {
   size_t metric_bytes = 0;
   int device = 0;
   std::string err;
+  wd::Prof prof;
 };
 
 namespace {
@@ -234,26 +235,68 @@ int wd_step(wd_handle* h, const wd_state* st, int n_large_steps, void* stream) {
   };
   for (int step = 0; step < n_large_steps; ++step) {
     hipError_t e;
+    wd::Prof* prof = h->prof.mask ? &h->prof : nullptr;
     const std::pair<double*, const double*> copies[] = {
         {w.u0, ds.u}, {w.v0, ds.v}, {w.w0, ds.w}, {w.th0, ds.theta}, {w.ph0, ds.ph_pert}, {w.mu0, ds.mu_pert}};
     const size_t sizes[] = {U, V, Wn, M, Wn, C};
+    size_t tok = prof ? prof->begin(wd::KID_ORIGIN_COPY, s) : 0;
     for (int q = 0; q < 6; ++q) {
       e = hipMemcpyAsync(copies[q].first, copies[q].second, sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, s);
       if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync(origin)");
     }
+    if (prof) prof->end(tok, s);
     for (const StageDesc& sd : stages) {
-      e = wd::launch_stage_setup(g, h->nl, ds, w, sd.rk_step, sd.dts, s);
+      e = wd::launch_stage_setup(g, h->nl, ds, w, sd.rk_step, sd.dts, s, prof);
       if (e != hipSuccess) return hip_fail(h, e, "stage setup launch");
       for (int it = 0; it < sd.nsub; ++it) {
-        e = wd::launch_acoustic_substep(g, h->nl, ds, w, sd.dts, s);
+        e = wd::launch_acoustic_substep(g, h->nl, ds, w, sd.dts, s, prof);
         if (e != hipSuccess) return hip_fail(h, e, "acoustic substep launch");
       }
-      e = wd::launch_stage_finish(g, ds, w, s);
+      e = wd::launch_stage_finish(g, ds, w, s, prof);
       if (e != hipSuccess) return hip_fail(h, e, "stage finish launch");
     }
   }
   return WD_OK;
 }
+
+int wd_profile_enable(wd_handle* h, unsigned long long kernel_mask) {
+  if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  h->prof.mask = kernel_mask;
+  h->prof.recs.clear();
+  h->prof.used = 0;
+  h->prof.failed = false;
+  return WD_OK;
+}
+
+int wd_profile_read(wd_handle* h, double* total_ms, long long* launches, int n) {
+  if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  if (n < 0 || (n > 0 && (!total_ms || !launches))) return fail(h, WD_EINVAL, "bad output arrays");
+  for (int i = 0; i < n; ++i) {
+    total_ms[i] = 0.0;
+    launches[i] = 0;
+  }
+  wd::Prof& p = h->prof;
+  if (p.failed) return fail(h, WD_EDEVICE, "event pool allocation failed while profiling");
+  for (const auto& r : p.recs) {
+    hipEvent_t a = p.pool[r.second], b = p.pool[r.second + 1];
+    hipError_t e = hipEventSynchronize(b);
+    if (e != hipSuccess) return hip_fail(h, e, "hipEventSynchronize");
+    float ms = 0.0f;
+    e = hipEventElapsedTime(&ms, a, b);
+    if (e != hipSuccess) return hip_fail(h, e, "hipEventElapsedTime");
+    if (r.first < n) {
+      total_ms[r.first] += (double)ms;
+      launches[r.first] += 1;
+    }
+  }
+  p.recs.clear();
+  p.used = 0;
+  return wd::KID_COUNT;
+}
+
+int wd_kernel_count(void) { return wd::KID_COUNT; }
+
+const char* wd_kernel_name(int id) { return wd::kernel_name(id); }
 
 size_t wd_workspace_bytes(const wd_handle* h) { return h ? h->work_bytes + h->metric_bytes : 0; }
 

@@ -1,20 +1,100 @@
-// Launchers for the wrfdyn kernels (one translation unit per phase).
+// Launchers for the wrfdyn kernels (one translation unit per phase) and the
+// optional per-kernel event timing used for benchmarking.
 #pragma once
+
+#include <utility>
+#include <vector>
 
 #include "wrfdyn_common.h"
 
 namespace wd {
 
+// Kernel ids for per-kernel timing (wd_profile_enable / wd_profile_read).
+enum KernelId {
+  KID_ORIGIN_COPY = 0,
+  KID_RUV,
+  KID_ROM,
+  KID_DIAG,
+  KID_TEND_U,
+  KID_TEND_V,
+  KID_TEND_W,
+  KID_TEND_T,
+  KID_PREP_2D,
+  KID_PREP_FACES,
+  KID_PREP_U,
+  KID_PREP_V,
+  KID_PREP_W,
+  KID_PREP_M,
+  KID_COL_SETUP,
+  KID_UV_U,
+  KID_UV_V,
+  KID_COLUMN,
+  KID_FIN_W,
+  KID_FIN_UV,
+  KID_FIN_M,
+  KID_FIN_MU,
+  KID_COUNT
+};
+
+const char* kernel_name(int id);
+
+// Brackets selected launches with HIP events on the launch stream; the
+// elapsed times are summed when read.  Disabled (mask 0) costs nothing.
+struct Prof {
+  unsigned long long mask = 0;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> recs;  // kernel id, index of the start event
+  bool failed = false;
+
+  bool on(int id) const { return (mask >> id) & 1ull; }
+  hipEvent_t next() {
+    if (used == pool.size()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) {
+        failed = true;
+        return nullptr;
+      }
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  // returns a token (start index + 1) or 0 when not recording
+  size_t begin(int id, hipStream_t st) {
+    if (!on(id) || failed) return 0;
+    size_t idx = used;
+    hipEvent_t a = next();
+    hipEvent_t b = next();
+    if (!a || !b) return 0;
+    hipEventRecord(a, st);
+    recs.emplace_back(id, idx);
+    return idx + 1;
+  }
+  void end(size_t tok, hipStream_t st) {
+    if (tok) hipEventRecord(pool[tok], st);
+  }
+  ~Prof() {
+    for (hipEvent_t e : pool) hipEventDestroy(e);
+  }
+};
+
+#define WD_TIMED(prof, id, st, launch)                      \
+  do {                                                      \
+    size_t _tok = (prof) ? (prof)->begin((id), (st)) : 0;  \
+    launch;                                                 \
+    if (prof) (prof)->end(_tok, (st));                      \
+  } while (0)
+
 // RK-stage setup: transport velocities, tendencies, small_step_prep,
 // calc_p_rho step 0, rw_tend/ph_tend and calc_coef_w.
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
-                              int rk_step, double dts, hipStream_t st);
+                              int rk_step, double dts, hipStream_t st, Prof* prof);
 
 // One acoustic substep: advance_uv, then the fused column kernel.
 hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
-                                   double dts, hipStream_t st);
+                                   double dts, hipStream_t st, Prof* prof);
 
 // small_step_finish + grid%p refresh.
-hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st);
+hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof);
 
 }  // namespace wd

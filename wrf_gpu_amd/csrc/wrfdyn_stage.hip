@@ -604,7 +604,7 @@ inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 
 }  // namespace
 
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
-                              int rk_step, double dts, hipStream_t st) {
+                              int rk_step, double dts, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
   const size_t M = (size_t)g.nz * g.ny * g.nx;
   const size_t Wn = (size_t)(g.nz + 1) * g.ny * g.nx;
@@ -613,20 +613,20 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const size_t C = (size_t)g.ny * g.nx;
   const size_t F = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;
   dim3 colgrid((g.nx + 63) / 64, g.ny);
-  k_ruv<<<blocks_for(M, bs), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv);
-  k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom);
-  k_diag<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w);
-  k_tend_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w);
-  k_tend_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w);
-  k_tend_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w);
-  k_tend_t<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w);
-  k_prep_2d<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w, rk_step);
-  k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w);
-  k_prep_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w);
-  k_prep_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w);
-  k_prep_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w);
-  k_prep_m<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w);
-  k_col_setup<<<colgrid, 64, 0, st>>>(g, nl, s, w, dts);
+  WD_TIMED(prof, KID_RUV, st, (k_ruv<<<blocks_for(M, bs), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
+  WD_TIMED(prof, KID_ROM, st, (k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom)));
+  WD_TIMED(prof, KID_DIAG, st, (k_diag<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_PREP_2D, st, (k_prep_2d<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w, rk_step)));
+  WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
+  WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_PREP_V, st, (k_prep_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_PREP_W, st, (k_prep_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_PREP_M, st, (k_prep_m<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<<<colgrid, 64, 0, st>>>(g, nl, s, w, dts)));
   return hipGetLastError();
 }
 

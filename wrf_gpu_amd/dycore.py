@@ -118,6 +118,27 @@ class Dycore:
             raise _lib.WrfdynError(code, _lib.last_error(self._handle))
         return state
 
+    def profile(self, kernels) -> None:
+        """Time the named kernels (see _lib.kernel_names()) with HIP events in later steps."""
+
+        names = _lib.kernel_names()
+        mask = 0
+        for k in kernels:
+            mask |= 1 << names.index(k)
+        _lib.check(self._lib.wd_profile_enable(self._handle, mask), self._handle)
+
+    def profile_read(self) -> dict:
+        """Summed milliseconds and launch counts per profiled kernel since the last read."""
+
+        names = _lib.kernel_names()
+        n = len(names)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        code = self._lib.wd_profile_read(self._handle, ms, cnt, n)
+        if code < 0:
+            raise _lib.WrfdynError(code, _lib.last_error(self._handle))
+        return {names[i]: (ms[i], cnt[i]) for i in range(n) if cnt[i] > 0}
+
     def close(self) -> None:
         h = getattr(self, "_handle", None)
         if h is not None and h.value:
