@@ -7,6 +7,7 @@
 //   stage setup -> acoustic substeps -> finish.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -24,6 +25,7 @@ struct wd_handle {
   size_t work_bytes = 0;
   size_t metric_bytes = 0;
   int device = 0;
+  int column_tile = 0;
   std::string err;
   wd::Prof prof;
 };
@@ -201,6 +203,15 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     delete h;
     return hip_fail(nullptr, e, "hipMemset(workspace)");
   }
+  // Column-kernel tile shape: WRFDYN_COLUMN_TILE="TXxKT" selects one of the
+  // compiled shapes (default 16x16).
+  if (const char* env = std::getenv("WRFDYN_COLUMN_TILE")) {
+    int tx = 0, kt = 0;
+    if (std::sscanf(env, "%dx%d", &tx, &kt) == 2) {
+      for (int t = 0; t < wd::kNumColumnTiles; ++t)
+        if (wd::kColumnTiles[t].tx == tx && wd::kColumnTiles[t].kt == kt) h->column_tile = t;
+    }
+  }
   *out = h;
   return WD_OK;
 }
@@ -249,7 +260,7 @@ int wd_step(wd_handle* h, const wd_state* st, int n_large_steps, void* stream) {
       e = wd::launch_stage_setup(g, h->nl, ds, w, sd.rk_step, sd.dts, s, prof);
       if (e != hipSuccess) return hip_fail(h, e, "stage setup launch");
       for (int it = 0; it < sd.nsub; ++it) {
-        e = wd::launch_acoustic_substep(g, h->nl, ds, w, sd.dts, s, prof);
+        e = wd::launch_acoustic_substep(g, h->nl, ds, w, sd.dts, h->column_tile, s, prof);
         if (e != hipSuccess) return hip_fail(h, e, "acoustic substep launch");
       }
       e = wd::launch_stage_finish(g, ds, w, s, prof);

@@ -26,8 +26,7 @@ enum KernelId {
   KID_PREP_W,
   KID_PREP_M,
   KID_COL_SETUP,
-  KID_UV_U,
-  KID_UV_V,
+  KID_UV,
   KID_COLUMN,
   KID_FIN_W,
   KID_FIN_UV,
@@ -90,9 +89,17 @@ struct Prof {
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                               int rk_step, double dts, hipStream_t st, Prof* prof);
 
-// One acoustic substep: advance_uv, then the fused column kernel.
+// Column-kernel tile shapes: TX columns x KT threads per column.
+struct ColumnTile {
+  int tx, kt;
+};
+extern const ColumnTile kColumnTiles[];
+extern const int kNumColumnTiles;
+
+// One acoustic substep: advance_uv, then the fused column kernel with tile
+// shape kColumnTiles[column_tile].
 hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
-                                   double dts, hipStream_t st, Prof* prof);
+                                   double dts, int column_tile, hipStream_t st, Prof* prof);
 
 // small_step_finish + grid%p refresh.
 hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof);
