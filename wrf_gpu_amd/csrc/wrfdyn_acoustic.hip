@@ -39,8 +39,10 @@ __device__ inline double dpn_ac(const DGrid& g, const double* __restrict__ p, in
 template <int BX, int BY>
 __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = blockIdx.y * BY + threadIdx.y;
+  const unsigned nbx = (nx + 1 + BX - 1) / BX;
+  const unsigned t = tile_id(g);
+  const int i = (int)(t % nbx) * BX + threadIdx.x;
+  const int j = (int)(t / nbx) * BY + threadIdx.y;
   if (i > nx || j > ny) return;
   const bool do_u = j < ny;
   const bool do_v = i < nx;
@@ -84,8 +86,61 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
   double sv_cur = sv0, sv_prev = 0.0, sv_prev2 = 0.0;
   double su_next = su1, sv_next = sv1;  // level k+1
   double phc = w.ph_work[cc], phw = w.ph_work[cw], phs = w.ph_work[cs];  // level k
+  // Per-level loads, double-buffered: level k+1's loads are issued before
+  // level k is computed and stored, so one level of loads is always in flight.
+  struct Lv {
+    double phc1, phw1, phs1;                  // ph work at level k+1
+    double altc, alc, pbc, phpc, pc;          // centre column, level k
+    double altw, alw, pbw, phpw, pw, uw, ut;  // west column and the u face
+    double alts, als, pbs, phps, ps, vw, vt;  // south column and the v face
+    double p2c, p2w, p2s;                     // p at level k+2 (pair sums)
+  };
+  const unsigned pl8 = (unsigned)plane * 8u;
+  const unsigned cc8 = (unsigned)cc * 8u, cw8 = (unsigned)cw * 8u, cs8 = (unsigned)cs * 8u;
+  const unsigned uq8 = (unsigned)iu(g, 0, j, i) * 8u, upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u;
+  const unsigned vq8 = (unsigned)iv(g, 0, j, i) * 8u, vpl8 = (unsigned)((size_t)(ny + 1) * nx) * 8u;
+  auto load = [&](int k, Lv& L) {
+    const unsigned ok = (unsigned)k * pl8, ok1 = ok + pl8;
+    L.phc1 = ldo(w.ph_work, ok1 + cc8);
+    L.altc = ldo(w.alt, ok + cc8);
+    L.alc = ldo(w.al, ok + cc8);
+    L.pbc = ldo(w.pb, ok + cc8);
+    L.phpc = ldo(w.php, ok + cc8);
+    L.pc = ldo(P, ok + cc8);
+    if (do_u) {
+      const unsigned q = uq8 + (unsigned)k * upl8;
+      L.phw1 = ldo(w.ph_work, ok1 + cw8);
+      L.altw = ldo(w.alt, ok + cw8);
+      L.alw = ldo(w.al, ok + cw8);
+      L.pbw = ldo(w.pb, ok + cw8);
+      L.phpw = ldo(w.php, ok + cw8);
+      L.pw = ldo(P, ok + cw8);
+      L.uw = ldo(w.u_work, q);
+      L.ut = ldo(w.ut, q);
+    }
+    if (do_v) {
+      const unsigned q = vq8 + (unsigned)k * vpl8;
+      L.phs1 = ldo(w.ph_work, ok1 + cs8);
+      L.alts = ldo(w.alt, ok + cs8);
+      L.als = ldo(w.al, ok + cs8);
+      L.pbs = ldo(w.pb, ok + cs8);
+      L.phps = ldo(w.php, ok + cs8);
+      L.ps = ldo(P, ok + cs8);
+      L.vw = ldo(w.v_work, q);
+      L.vt = ldo(w.vt, q);
+    }
+    if (k + 2 < nz) {
+      const unsigned o2 = ok + 2 * pl8;
+      L.p2c = ldo(P, o2 + cc8);
+      L.p2w = ldo(P, o2 + cw8);
+      L.p2s = ldo(P, o2 + cs8);
+    }
+  };
+  Lv cur{};
+  load(0, cur);
   for (int k = 0; k < nz; ++k) {
-    const size_t ok = (size_t)k * plane;
+    Lv nxt{};
+    if (k + 1 < nz) load(k + 1, nxt);
     // dpn(k+1)
     double dpn_u1, dpn_v1;
     if (k + 1 < nz) {
@@ -95,39 +150,39 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
       dpn_u1 = 0.5 * ((g.cf1 * su_cur + g.cf2 * su_prev) + g.cf3 * su_prev2);
       dpn_v1 = 0.5 * ((g.cf1 * sv_cur + g.cf2 * sv_prev) + g.cf3 * sv_prev2);
     }
-    const double phc1 = w.ph_work[ok + plane + cc];
-    const double altc = w.alt[ok + cc], alc = w.al[ok + cc], pbc = w.pb[ok + cc], phpc = w.php[ok + cc];
-    const double pck = P[ok + cc];
+    const double phc1 = cur.phc1;
+    const double altc = cur.altc, alc = cur.alc, pbc = cur.pbc, phpc = cur.phpc;
+    const double pck = cur.pc;
     if (do_u) {
-      const double phw1 = w.ph_work[ok + plane + cw];
-      size_t q = iu(g, k, j, i);
-      double u = w.u_work[q] + dts * w.ut[q];
+      const double phw1 = cur.phw1;
+      const unsigned q = uq8 + (unsigned)k * upl8;
+      double u = cur.uw + dts * cur.ut;
       double mass_x = g.c1h[k] * muu + g.c2h[k];
       double ph_term = (phc1 - phw1) + (phc - phw);
-      double p_term = (w.alt[ok + cw] + altc) * (pck - P[ok + cw]);
-      double pb_term = (w.al[ok + cw] + alc) * (pbc - w.pb[ok + cw]);
+      double p_term = (cur.altw + altc) * (pck - cur.pw);
+      double pb_term = (cur.alw + alc) * (pbc - cur.pbw);
       double dpx = (((msf_u * 0.5) * g.rdx) * mass_x) * ((ph_term + p_term) + pb_term);
       double bracket = g.rdnw[k] * (dpn_u1 - dpn_u) - 0.5 * (g.c1h[k] * muwsum_u);
-      dpx = dpx + ((msf_u * g.rdx) * (phpc - w.php[ok + cw])) * bracket;
+      dpx = dpx + ((msf_u * g.rdx) * (phpc - cur.phpw)) * bracket;
       u = u - (dts * 1.0) * dpx;
       u = u + g.c1h[k] * mudf_u;
-      w.u_work[q] = u;
+      sto(w.u_work, q, u);
       phw = phw1;
     }
     if (do_v) {
-      const double phs1 = w.ph_work[ok + plane + cs];
-      size_t q = iv(g, k, j, i);
-      double v = w.v_work[q] + dts * w.vt[q];
+      const double phs1 = cur.phs1;
+      const unsigned q = vq8 + (unsigned)k * vpl8;
+      double v = cur.vw + dts * cur.vt;
       double mass_y = g.c1h[k] * muv + g.c2h[k];
       double ph_term = (phc1 - phs1) + (phc - phs);
-      double p_term = (w.alt[ok + cs] + altc) * (pck - P[ok + cs]);
-      double pb_term = (w.al[ok + cs] + alc) * (pbc - w.pb[ok + cs]);
+      double p_term = (cur.alts + altc) * (pck - cur.ps);
+      double pb_term = (cur.als + alc) * (pbc - cur.pbs);
       double dpy = (((msf_v * 0.5) * g.rdy) * mass_y) * ((ph_term + p_term) + pb_term);
       double bracket = g.rdnw[k] * (dpn_v1 - dpn_v) - 0.5 * (g.c1h[k] * muwsum_v);
-      dpy = dpy + ((msf_v * g.rdy) * (phpc - w.php[ok + cs])) * bracket;
+      dpy = dpy + ((msf_v * g.rdy) * (phpc - cur.phps)) * bracket;
       v = v - (dts * 1.0) * dpy;
       v = v + g.c1h[k] * mudf_v;
-      w.v_work[q] = v;
+      sto(w.v_work, q, v);
       phs = phs1;
     }
     phc = phc1;
@@ -140,26 +195,34 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     sv_prev = sv_cur;
     sv_cur = sv_next;
     if (k + 2 < nz) {
-      const size_t o2 = (size_t)(k + 2) * plane;
-      double pc_n = P[o2 + cc];
-      su_next = P[o2 + cw] + pc_n;
-      sv_next = P[o2 + cs] + pc_n;
+      double pc_n = cur.p2c;
+      su_next = cur.p2w + pc_n;
+      sv_next = cur.p2s + pc_n;
     }
+    cur = nxt;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Fused column kernel (one workgroup = TX adjacent columns of one row j,
-// KT threads per column):
+// KT threads per column, each thread owning the R levels kt, kt+KT, ...):
 //   advance_mu_t periodic   dynamics/mu_t_advance.py:139-292
 //   advance_w_wrf           core/advance_w.py:293-627 (w_damp :92-132,
 //                           Thomas sweeps :558-592, damp_opt=3 :594-617)
 //   calc_p_rho_step         core/calc_p_rho.py:142-183 (smdiv = 0.1)
-// Level-local work runs in parallel over k (thread kt takes levels kt,
-// kt+KT, ...; loads coalesce along i) and is exchanged through LDS; the
+// Level-local work runs in parallel over k and is exchanged through LDS; the
 // vertical recurrences (the dmdt sum, the omega recurrence, the Thomas
-// sweeps) run one thread per column in the reference's order, so every
-// value is produced by the same operations as the sequential form.
+// sweeps) run one thread per column in the reference's order, so every value
+// is produced by the same operations as the sequential form.
+//
+// Every phase first issues all of its global loads for the thread's R levels
+// into registers and only then computes, so a wave keeps R levels of loads in
+// flight instead of one; values a later phase needs again (theta, c2a, alt,
+// ph_1 total, the old w) stay in registers instead of being re-read.
+//
+// Up to four levels per thread the kernel is held to 168 VGPRs (three waves
+// per SIMD) without spilling.  The serial sweeps read LDS and the metrics
+// eight levels ahead of their arithmetic chain.
 //
 // LDS: six (nz+1) x TX double planes plus three TX rows.
 //   A: dvdxi -> explicit w -> solved w      B: omega -> a -> new ph work
@@ -167,14 +230,17 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
 //   E: omega_save (rom) -> new coupled theta
 //   F: old ph work -> gamma
 // ---------------------------------------------------------------------------
-template <int TX, int KT>
-__global__ void __launch_bounds__(TX * KT) k_column(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
+template <int TX, int KT, int R>
+__global__ void __launch_bounds__(TX * KT, (R <= 4 ? 3 : 1))
+    k_column(DGrid g, DNamelist nl, DState s, DWork w, double dts, int first) {
   extern __shared__ double lds[];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int tx = threadIdx.x % TX;
   const int kt = threadIdx.x / TX;
-  const int i = blockIdx.x * TX + tx;
-  const int j = blockIdx.y;
+  const unsigned nbx = (nx + TX - 1) / TX;
+  const unsigned t = tile_id(g);
+  const int i = (int)(t % nbx) * TX + tx;
+  const int j = (int)(t / nbx);
   const bool valid = i < nx;
   const size_t pl = (size_t)(nz + 1) * TX;
   double* A = lds;
@@ -187,41 +253,88 @@ __global__ void __launch_bounds__(TX * KT) k_column(DGrid g, DNamelist nl, DStat
   double* s_muave = s_muts + TX;
   double* s_ph0 = s_muave + TX;
 #define L(arr, k) arr[(size_t)(k) * TX + tx]
+  // Flat offsets: mass/w level k of column (j, i) is c + k * plane.
+  const unsigned plane = (unsigned)((size_t)ny * nx) * 8u;  // byte strides (ldo/sto)
+  const unsigned uplane = (unsigned)((size_t)ny * (nx + 1)) * 8u, vplane = (unsigned)((size_t)(ny + 1) * nx) * 8u;
   const size_t c = valid ? i2(g, j, i) : 0;
+  const unsigned c8 = (unsigned)c * 8u;
   const double msftx = valid ? g.msftx[c] : 1.0;
   const double msfty = valid ? g.msfty[c] : 1.0;
   const double eps_p = 1.0 + nl.epssm, eps_m = 1.0 - nl.epssm;
   const double mut = valid ? w.mut[c] : 0.0;
+  auto lev = [&](int r) { return kt + r * KT; };
 
   // ---- P1: dvdxi per level (advance_mu_t), omega_save into LDS ----
   double p2_muts = 0.0, p2_mu = 0.0, p2_ww0 = 0.0, p2_wwtop = 0.0;
   if (valid && kt == 0) {  // column values the recurrence thread needs in P2
     p2_muts = w.muts[c];
     p2_mu = w.mu[c];
-    p2_ww0 = w.ww[im(g, 0, j, i)];
-    p2_wwtop = w.ww[im(g, nz, j, i)];
+    p2_ww0 = w.ww[c];
+    p2_wwtop = ldo(w.ww, c8 + (unsigned)nz * plane);
   }
-  if (valid) {
-    for (int k = kt; k < nz; k += KT) {
-      double c1 = g.c1h[k], c2 = g.c2h[k];
-      double v_n = w.v_work[iv(g, k, j + 1, i)] +
-                   ((c1 * w.muv[i2v(g, j + 1, i)] + c2) * s.v[iv(g, k, j + 1, i)]) * (1.0 / g.msfvx[i2v(g, j + 1, i)]);
-      double v_s = w.v_work[iv(g, k, j, i)] +
-                   ((c1 * w.muv[i2v(g, j, i)] + c2) * s.v[iv(g, k, j, i)]) * (1.0 / g.msfvx[i2v(g, j, i)]);
-      double u_e = w.u_work[iu(g, k, j, i + 1)] +
-                   ((c1 * w.muu[i2u(g, j, i + 1)] + c2) * s.u[iu(g, k, j, i + 1)]) / g.msfuy[i2u(g, j, i + 1)];
-      double u_w = w.u_work[iu(g, k, j, i)] +
-                   ((c1 * w.muu[i2u(g, j, i)] + c2) * s.u[iu(g, k, j, i)]) / g.msfuy[i2u(g, j, i)];
-      L(A, k) = (msftx * msfty) * (g.rdy * (v_n - v_s) + g.rdx * (u_e - u_w));
+  {
+    const size_t fvn = valid ? i2v(g, j + 1, i) : 0, fvs = valid ? i2v(g, j, i) : 0;
+    const size_t fue = valid ? i2u(g, j, i + 1) : 0, fuw = valid ? i2u(g, j, i) : 0;
+    const unsigned vn8 = (unsigned)fvn * 8u, vs8 = (unsigned)fvs * 8u, ue8 = (unsigned)fue * 8u, uw8 = (unsigned)fuw * 8u;
+    double muv_n = 0.0, muv_s = 0.0, rmvx_n = 0.0, rmvx_s = 0.0;
+    double muu_e = 0.0, muu_w = 0.0, msfuy_e = 1.0, msfuy_w = 1.0;
+    if (valid) {
+      muv_n = w.muv[fvn];
+      muv_s = w.muv[fvs];
+      rmvx_n = 1.0 / g.msfvx[fvn];
+      rmvx_s = 1.0 / g.msfvx[fvs];
+      muu_e = w.muu[fue];
+      muu_w = w.muu[fuw];
+      msfuy_e = g.msfuy[fue];
+      msfuy_w = g.msfuy[fuw];
     }
-    for (int k = kt; k <= nz; k += KT) L(E, k) = w.rom[im(g, k, j, i)];
+    double vwn[R], vws[R], vn[R], vs[R], uwe[R], uww[R], ue[R], uw[R], romr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = lev(r);
+      if (valid && k < nz) {
+        const unsigned kv = (unsigned)k * vplane, ku = (unsigned)k * uplane;
+        vwn[r] = ldo(w.v_work, vn8 + kv);
+        vws[r] = ldo(w.v_work, vs8 + kv);
+        vn[r] = ldo(s.v, vn8 + kv);
+        vs[r] = ldo(s.v, vs8 + kv);
+        uwe[r] = ldo(w.u_work, ue8 + ku);
+        uww[r] = ldo(w.u_work, uw8 + ku);
+        ue[r] = ldo(s.u, ue8 + ku);
+        uw[r] = ldo(s.u, uw8 + ku);
+      }
+      if (valid && k <= nz) romr[r] = ldo(w.rom, c8 + (unsigned)k * plane);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = lev(r);
+      if (valid && k < nz) {
+        const double c1 = g.c1h[k], c2 = g.c2h[k];
+        double v_n = vwn[r] + ((c1 * muv_n + c2) * vn[r]) * rmvx_n;
+        double v_s = vws[r] + ((c1 * muv_s + c2) * vs[r]) * rmvx_s;
+        double u_e = uwe[r] + ((c1 * muu_e + c2) * ue[r]) / msfuy_e;
+        double u_w = uww[r] + ((c1 * muu_w + c2) * uw[r]) / msfuy_w;
+        L(A, k) = (msftx * msfty) * (g.rdy * (v_n - v_s) + g.rdx * (u_e - u_w));
+      }
+      if (valid && k <= nz) L(E, k) = romr[r];
+    }
   }
   __syncthreads();
 
   // ---- P2 (one thread per column): dmdt, mass updates, omega recurrence ----
   if (valid && kt == 0) {
+    // LDS reads are batched eight levels ahead of the serial arithmetic
     double dmdt = 0.0;
-    for (int k = 0; k < nz; ++k) dmdt = (k == 0) ? g.dnw[0] * L(A, 0) : dmdt + g.dnw[k] * L(A, k);
+    for (int k0 = 0; k0 < nz; k0 += 8) {
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = k0 + u < nz ? L(A, k0 + u) : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u;
+        if (k < nz) dmdt = (k == 0) ? g.dnw[0] * a[u] : dmdt + g.dnw[k] * a[u];
+      }
+    }
     const double mu_work_old = p2_muts - mut;
     const double mu_save = p2_mu - mu_work_old;
     const double mu_tendency = dmdt + 0.0;
@@ -235,13 +348,28 @@ __global__ void __launch_bounds__(TX * KT) k_column(DGrid g, DNamelist nl, DStat
     s_muts[tx] = muts;
     s_muave[tx] = muave;
     double wprev = p2_ww0;
-    L(B, 0) = wprev - L(E, 0);
-    for (int kk = 1; kk < nz; ++kk) {
-      int k = kk - 1;
-      double incr = (g.dnw[kk - 1] * ((g.c1h[k] * dmdt + L(A, kk - 1)) + g.c1h[k] * 0.0)) / msfty;
-      double wk = wprev - incr;
-      L(B, kk) = wk - L(E, kk);
-      wprev = wk;
+    const double ww0 = wprev - L(E, 0);
+    L(B, 0) = ww0;
+    w.ww[c] = ww0;  // carried to the next substep; the interior is rebuilt each substep
+    for (int k0 = 1; k0 < nz; k0 += 8) {
+      double a[8], e[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int kk = k0 + u;
+        a[u] = kk < nz ? L(A, kk - 1) : 0.0;
+        e[u] = kk < nz ? L(E, kk) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int kk = k0 + u;
+        if (kk < nz) {
+          const int k = kk - 1;
+          double incr = (g.dnw[kk - 1] * ((g.c1h[k] * dmdt + a[u]) + g.c1h[k] * 0.0)) / msfty;
+          double wk = wprev - incr;
+          L(B, kk) = wk - e[u];
+          wprev = wk;
+        }
+      }
     }
     L(B, nz) = p2_wwtop;
   }
@@ -250,78 +378,156 @@ __global__ void __launch_bounds__(TX * KT) k_column(DGrid g, DNamelist nl, DStat
   const double muts = valid ? s_muts[tx] : 0.0;
   const double muave = valid ? s_muave[tx] : 0.0;
 
-  // ---- P3: coupled theta update, omega out, new t_2ave ----
-  if (valid) {
-    int ip = wrapi(i + 1, nx), imn = wrapi(i - 1, nx), jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny);
-    auto T1 = [&](int k, int jj, int ii) { return s.theta[im(g, k, jj, ii)] - kT0; };
-    for (int k = kt; k < nz; k += KT) {
-      double wd_hi = 0.0, wd_lo = 0.0;
-      if (k + 1 < nz) {
-        int kk = k + 1;
-        wd_hi = L(B, kk) * (g.fnm[kk] * T1(kk, j, i) + g.fnp[kk] * T1(kk - 1, j, i));
+  // ---- P3: coupled theta update, new t_2ave, old ph work into LDS ----
+  double thc[R];  // theta at the thread's levels, reused by calc_p_rho
+  {
+    const int ip = valid ? wrapi(i + 1, nx) : 0, imn = valid ? wrapi(i - 1, nx) : 0;
+    const int jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny);
+    const unsigned cjp = (unsigned)i2(g, jp, i) * 8u, cjm = (unsigned)i2(g, jm, i) * 8u;
+    const unsigned cip = (unsigned)i2(g, j, ip) * 8u, cim = (unsigned)i2(g, j, imn) * 8u;
+    const unsigned vn8 = valid ? (unsigned)i2v(g, j + 1, i) * 8u : 0u, vs8 = valid ? (unsigned)i2v(g, j, i) * 8u : 0u;
+    const unsigned ue8 = valid ? (unsigned)i2u(g, j, i + 1) * 8u : 0u, uw8 = valid ? (unsigned)i2u(g, j, i) * 8u : 0u;
+    double thm[R], thp[R], thn[R], ths[R], the[R], thw[R], vwn[R], vws[R], uwe[R], uww[R];
+    double thwk[R], tht[R], t2o[R], phw[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = lev(r);
+      if (valid && k < nz) {
+        const unsigned kp = (unsigned)k * plane, q = c8 + kp;
+        const unsigned kv = (unsigned)k * vplane, ku = (unsigned)k * uplane;
+        thc[r] = ldo(s.theta, q);
+        thm[r] = k >= 1 ? ldo(s.theta, q - plane) : 0.0;
+        thp[r] = k + 1 < nz ? ldo(s.theta, q + plane) : 0.0;
+        thn[r] = ldo(s.theta, cjp + kp);
+        ths[r] = ldo(s.theta, cjm + kp);
+        the[r] = ldo(s.theta, cip + kp);
+        thw[r] = ldo(s.theta, cim + kp);
+        vwn[r] = ldo(w.v_work, vn8 + kv);
+        vws[r] = ldo(w.v_work, vs8 + kv);
+        uwe[r] = ldo(w.u_work, ue8 + ku);
+        uww[r] = ldo(w.u_work, uw8 + ku);
+        thwk[r] = ldo(w.th_work, q);
+        tht[r] = ldo(w.tht, q);
+        t2o[r] = first ? 0.0 : ldo(w.t2ave, q);  // t_2ave starts each stage at 0
       }
-      if (k >= 1) wd_lo = L(B, k) * (g.fnm[k] * T1(k, j, i) + g.fnp[k] * T1(k - 1, j, i));
-      double th = T1(k, j, i);
-      double v_flux = w.v_work[iv(g, k, j + 1, i)] * (T1(k, jp, i) + th) - w.v_work[iv(g, k, j, i)] * (th + T1(k, jm, i));
-      double u_flux = w.u_work[iu(g, k, j, i + 1)] * (T1(k, j, ip) + th) - w.u_work[iu(g, k, j, i)] * (th + T1(k, j, imn));
-      double tendency = msftx * ((0.5 * g.rdy) * v_flux + (0.5 * g.rdx) * u_flux) + g.rdnw[k] * (wd_hi - wd_lo);
-      size_t q = im(g, k, j, i);
-      double theta_i = w.th_work[q] + ((msfty * dts) * w.tht[q]);
-      double th_new = theta_i - ((dts * msfty) * tendency);
-      w.th_work[q] = th_new;
-      L(E, k) = th_new;
-      // t_2ave (advance_w.py:448-458)
-      double mass_h = g.c1h[k] * muts + g.c2h[k];
-      double half = 0.5 * (eps_p * th_new + eps_m * w.t2ave[q]);
-      double ttr = kT0 + th;
-      double t2n = (half + (g.c1h[k] * muave) * kT0) / (mass_h * ttr);
-      L(C, k) = t2n;
-      w.t2ave[q] = t2n;
+      if (valid && k <= nz) phw[r] = ldo(w.ph_work, c8 + (unsigned)k * plane);
     }
-    for (int f = kt; f <= nz; f += KT) {
-      w.ww[im(g, f, j, i)] = L(B, f);
-      double phw = w.ph_work[im(g, f, j, i)];
-      L(F, f) = phw;
-      if (f == 0) s_ph0[tx] = phw;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = lev(r);
+      if (valid && k < nz) {
+        const double th = thc[r] - kT0;
+        double wd_hi = 0.0, wd_lo = 0.0;
+        if (k + 1 < nz) {
+          const int kk = k + 1;
+          wd_hi = L(B, kk) * (g.fnm[kk] * (thp[r] - kT0) + g.fnp[kk] * th);
+        }
+        if (k >= 1) wd_lo = L(B, k) * (g.fnm[k] * th + g.fnp[k] * (thm[r] - kT0));
+        double v_flux = vwn[r] * ((thn[r] - kT0) + th) - vws[r] * (th + (ths[r] - kT0));
+        double u_flux = uwe[r] * ((the[r] - kT0) + th) - uww[r] * (th + (thw[r] - kT0));
+        double tendency = msftx * ((0.5 * g.rdy) * v_flux + (0.5 * g.rdx) * u_flux) + g.rdnw[k] * (wd_hi - wd_lo);
+        const unsigned q = c8 + (unsigned)k * plane;
+        double theta_i = thwk[r] + ((msfty * dts) * tht[r]);
+        double th_new = theta_i - ((dts * msfty) * tendency);
+        sto(w.th_work, q, th_new);
+        L(E, k) = th_new;
+        // t_2ave (advance_w.py:448-458)
+        double mass_h = g.c1h[k] * muts + g.c2h[k];
+        double half = 0.5 * (eps_p * th_new + eps_m * t2o[r]);
+        double ttr = kT0 + th;
+        double t2n = (half + (g.c1h[k] * muave) * kT0) / (mass_h * ttr);
+        L(C, k) = t2n;
+        sto(w.t2ave, q, t2n);
+      }
+      if (valid && k <= nz) {
+        L(F, k) = phw[r];
+        if (k == 0) s_ph0[tx] = phw[r];
+      }
     }
   }
   __syncthreads();
 
   // ---- P4: rhs of the phi equation (advance_w.py:460-503) ----
-  auto PH1T = [&](int k) { return w.ph0[im(g, k, j, i)] + w.phb[im(g, k, j, i)]; };
-  if (valid) {
+  double ph1c[R], wold[R];  // ph_1 total and the old w at the thread's levels
+  {
+    double pht[R], php1[R], phm1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int f = lev(r);
+      if (valid && f <= nz) {
+        const unsigned q = c8 + (unsigned)f * plane;
+        ph1c[r] = ldo(w.ph1t, q);
+        wold[r] = ldo(w.w_work, q);
+        pht[r] = ldo(w.ph_tend, q);
+        php1[r] = f < nz ? ldo(w.ph1t, q + plane) : 0.0;
+        phm1[r] = f >= 1 ? ldo(w.ph1t, q - plane) : 0.0;
+      }
+    }
     const double c05geps = (0.5 * kG) * eps_m;
-    for (int f = kt; f <= nz; f += KT) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int f = lev(r);
+      if (!(valid && f <= nz)) continue;
       if (f == 0) {
         L(D, 0) = 0.0;
         continue;
       }
-      double r = dts * (w.ph_tend[im(g, f, j, i)] + c05geps * w.w_work[im(g, f, j, i)]);
+      double rr = dts * (pht[r] + c05geps * wold[r]);
       if (f < nz) {
-        double wd_f1 = ((0.5 * (L(B, f + 1) + L(B, f))) * g.rdnw[f]) * (PH1T(f + 1) - PH1T(f));
-        double wd_f = ((0.5 * (L(B, f) + L(B, f - 1))) * g.rdnw[f - 1]) * (PH1T(f) - PH1T(f - 1));
+        double wd_f1 = ((0.5 * (L(B, f + 1) + L(B, f))) * g.rdnw[f]) * (php1[r] - ph1c[r]);
+        double wd_f = ((0.5 * (L(B, f) + L(B, f - 1))) * g.rdnw[f - 1]) * (ph1c[r] - phm1[r]);
         double adv = dts * (g.fnm[f] * wd_f1 + g.fnp[f] * wd_f);
-        r = r + (-adv);
+        rr = rr + (-adv);
       }
       double mass_f = g.c1f[f] * mut + g.c2f[f];
-      r = L(F, f) + (msfty * r) / mass_f;
-      if (f == nz && nl.top_lid) r = 0.0;
-      L(D, f) = r;
+      rr = L(F, f) + (msfty * rr) / mass_f;
+      if (f == nz && nl.top_lid) rr = 0.0;
+      L(D, f) = rr;
     }
   }
   __syncthreads();
 
-  // ---- P5: explicit w (advance_w.py:505-556) and the terrain surface w ----
-  if (valid) {
+  // ---- P5: explicit w (advance_w.py:505-556) and the terrain surface w;
+  //      the tridiagonal coefficients are loaded here and staged after ----
+  double c2ar[R], altr[R];  // c2a and alt at the thread's mass levels, reused by calc_p_rho
+  double car[R], calr[R], cgar[R];
+  {
+    double rwt[R], c2am[R], altm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int f = lev(r);
+      if (valid && f <= nz) {
+        const unsigned q = c8 + (unsigned)f * plane;
+        car[r] = ldo(w.ca, q);
+        calr[r] = ldo(w.calpha, q);
+        cgar[r] = ldo(w.cgamma, q);
+        if (f < nz) {
+          c2ar[r] = ldo(w.c2a, q);
+          altr[r] = ldo(w.alt, q);
+        }
+        if (f >= 1 && f < nz) {
+          rwt[r] = ldo(w.rw_tend, q);
+          c2am[r] = ldo(w.c2a, q - plane);
+          altm[r] = ldo(w.alt, q - plane);
+        }
+      }
+    }
     const double msft_inv = 1.0 / msfty;
     const double c05dtg = (0.5 * dts) * kG;
-    for (int f = kt; f <= nz; f += KT) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int f = lev(r);
+      if (!(valid && f <= nz)) continue;
       if (f == 0) {
-        int jn = clampi(j + 1, 0, ny - 1), js = clampi(j - 1, 0, ny - 1);
-        int ie = clampi(i + 1, 0, nx - 1), iw = clampi(i - 1, 0, nx - 1);
-        double ht = w.phb[im(g, 0, j, i)] / kG;
-        double ht_n = w.phb[im(g, 0, jn, i)] / kG, ht_s = w.phb[im(g, 0, js, i)] / kG;
-        double ht_e = w.phb[im(g, 0, j, ie)] / kG, ht_w = w.phb[im(g, 0, j, iw)] / kG;
+        const int jn = clampi(j + 1, 0, ny - 1), js = clampi(j - 1, 0, ny - 1);
+        const int ie = clampi(i + 1, 0, nx - 1), iw = clampi(i - 1, 0, nx - 1);
+        auto PHB0 = [&](int jj, int ii) {
+          const size_t q = i2(g, jj, ii);
+          return s.ph_total[q] - s.ph_pert[q];
+        };
+        double ht = PHB0(j, i) / kG;
+        double ht_n = PHB0(jn, i) / kG, ht_s = PHB0(js, i) / kG;
+        double ht_e = PHB0(j, ie) / kG, ht_w = PHB0(j, iw) / kG;
         double dy_n = ht_n - ht, dy_s = ht - ht_s, dx_e = ht_e - ht, dx_w = ht - ht_w;
         double v_cf_n = (g.cf1 * s.v[iv(g, 0, j + 1, i)] + g.cf2 * s.v[iv(g, 1, j + 1, i)]) + g.cf3 * s.v[iv(g, 2, j + 1, i)];
         double v_cf_s = (g.cf1 * s.v[iv(g, 0, j, i)] + g.cf2 * s.v[iv(g, 1, j, i)]) + g.cf3 * s.v[iv(g, 2, j, i)];
@@ -335,29 +541,26 @@ __global__ void __launch_bounds__(TX * KT) k_column(DGrid g, DNamelist nl, DStat
         L(A, nz) = 0.0;  // rigid lid
         continue;
       }
-      size_t qf = im(g, f, j, i);
-      double wold = w.w_work[qf];
-      double rw = w.rw_tend[qf];
+      double rw = rwt[r];
       if (nl.w_damping == 1) {
         double mass_f = g.c1f[f] * mut + g.c2f[f];
         double safe_mf = safe_pos(mass_f, 1.0e-12);
         double vert_cfl = fabs(((L(B, f) / safe_mf) * g.rdnw[f]) * dts);
-        double damp = (((-sgn(wold)) * kWAlpha) * (vert_cfl - kWBeta)) * mass_f;
+        double damp = (((-sgn(wold[r])) * kWAlpha) * (vert_cfl - kWBeta)) * mass_f;
         rw = rw + (vert_cfl > kWBeta ? damp : 0.0);
       }
-      double wn = wold + dts * rw;
-      size_t qm = im(g, f - 1, j, i);
+      double wn = wold[r] + dts * rw;
       double mh_u = safe_pos(g.c1h[f] * mut + g.c2h[f], 1.0e-12);
       double mh_l = safe_pos(g.c1h[f - 1] * mut + g.c2h[f - 1], 1.0e-12);
-      double c2a_u = w.c2a[qf], c2a_l = w.c2a[qm];
+      double c2a_u = c2ar[r], c2a_l = c2am[r];
       double coef_u = (c2a_u * g.rdnw[f]) / mh_u;
       double coef_l = (c2a_l * g.rdnw[f - 1]) / mh_l;
       double phk1 = L(F, f + 1), phk = L(F, f), phkm = L(F, f - 1);
       double tA_u = coef_u * (eps_p * (L(D, f + 1) - L(D, f)) + eps_m * (phk1 - phk));
       double tA_l = coef_l * (eps_p * (L(D, f) - L(D, f - 1)) + eps_m * (phk - phkm));
       double termA = ((msft_inv * cqw_at(f, nz)) * (c05dtg * g.rdn[f])) * (tA_u - tA_l);
-      double bu = (c2a_u * w.alt[qf]) * L(C, f);
-      double bl = (c2a_l * w.alt[qm]) * L(C, f - 1);
+      double bu = (c2a_u * altr[r]) * L(C, f);
+      double bl = (c2a_l * altm[r]) * L(C, f - 1);
       double termB = ((dts * kG) * msft_inv) * (g.rdn[f] * (bu - bl) - (g.c1f[f] * muave));
       L(A, f) = wn + (termA + termB);
     }
@@ -365,12 +568,13 @@ __global__ void __launch_bounds__(TX * KT) k_column(DGrid g, DNamelist nl, DStat
   __syncthreads();
 
   // ---- stage the tridiagonal coefficients (B, C, F are free now) ----
-  if (valid) {
-    for (int f = kt; f <= nz; f += KT) {
-      size_t qf = im(g, f, j, i);
-      L(B, f) = w.ca[qf];
-      L(C, f) = w.calpha[qf];
-      L(F, f) = w.cgamma[qf];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int f = lev(r);
+    if (valid && f <= nz) {
+      L(B, f) = car[r];
+      L(C, f) = calr[r];
+      L(F, f) = cgar[r];
     }
   }
   __syncthreads();
@@ -378,71 +582,117 @@ __global__ void __launch_bounds__(TX * KT) k_column(DGrid g, DNamelist nl, DStat
   // ---- P6 (one thread per column): Thomas forward / back (advance_w.py:558-592) ----
   if (valid && kt == 0) {
     double prev = L(A, 0);
-    for (int f = 1; f <= nz; ++f) {
-      double wf = (L(A, f) - L(B, f) * prev) * L(C, f);
-      L(A, f) = wf;
-      prev = wf;
+    for (int f0 = 1; f0 <= nz; f0 += 8) {
+      double a[8], b[8], cc[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int f = f0 + u;
+        a[u] = f <= nz ? L(A, f) : 0.0;
+        b[u] = f <= nz ? L(B, f) : 0.0;
+        cc[u] = f <= nz ? L(C, f) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int f = f0 + u;
+        if (f <= nz) {
+          double wf = (a[u] - b[u] * prev) * cc[u];
+          L(A, f) = wf;
+          prev = wf;
+        }
+      }
     }
     double next = L(A, nz);
-    for (int f = nz - 1; f >= 1; --f) {
-      double wv = L(A, f) - L(F, f) * next;
-      L(A, f) = wv;
-      next = wv;
+    for (int f0 = nz - 1; f0 >= 1; f0 -= 8) {
+      double a[8], ff[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int f = f0 - u;
+        a[u] = f >= 1 ? L(A, f) : 0.0;
+        ff[u] = f >= 1 ? L(F, f) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int f = f0 - u;
+        if (f >= 1) {
+          double wv = a[u] - ff[u] * next;
+          L(A, f) = wv;
+          next = wv;
+        }
+      }
     }
   }
   __syncthreads();
 
   // ---- P7: damp_opt=3 Rayleigh damping and the geopotential finish ----
-  if (valid) {
+  {
     const bool damp = (nl.damp_opt == 3 && nl.dampcoef > 0.0);
     const double dampmag = dts * nl.dampcoef, hdepth = nl.zdamp;
-    const double htop = PH1T(nz) / kG;
+    // ph_1 total at the lid (thread-independent column value; the owner of
+    // level nz holds it, everyone reads it from global once)
+    const double htop = valid ? ldo(w.ph1t, c8 + (unsigned)nz * plane) / kG : 0.0;
     const double hbot = htop - hdepth;
-    for (int f = kt; f <= nz; f += KT) {
-      size_t qf = im(g, f, j, i);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int f = lev(r);
+      if (!(valid && f <= nz)) continue;
+      const unsigned qf = c8 + (unsigned)f * plane;
       if (f == 0) {
-        w.w_work[qf] = L(A, 0);
+        sto(w.w_work, qf, L(A, 0));
         L(B, 0) = s_ph0[tx];
         continue;
       }
       double wv = L(A, f);
       double wd = wv;
       if (damp) {
-        double hk = PH1T(f) / kG;
+        double hk = ph1c[r] / kG;
         double sn = wd_sin(((0.5 * kPi) * (hk - hbot)) / hdepth);
         double ramp = sn * sn;
         double dampwt = hk >= hbot ? dampmag * ramp : 0.0;
         double mass_f = g.c1f[f] * mut + g.c2f[f];
-        wd = (wv - (dampwt * mass_f) * s.w[qf]) / (1.0 + dampwt);
+        // below the damping layer dampwt = 0 and the saved w drops out
+        // exactly (wv - 0 * w_save) / 1, so it is only read inside the layer
+        double wsave = hk >= hbot ? ldo(s.w, qf) : 0.0;
+        wd = (wv - (dampwt * mass_f) * wsave) / (1.0 + dampwt);
       }
       double mfs = g.c1f[f] * muts + g.c2f[f];
       double phn = L(D, f) + (((((msfty * 0.5) * dts) * kG) * eps_p) * wd) / mfs;
-      w.ph_work[qf] = phn;
-      w.w_work[qf] = wd;
+      sto(w.ph_work, qf, phn);
+      sto(w.w_work, qf, wd);
       L(B, f) = phn;
     }
   }
   __syncthreads();
 
   // ---- P8: calc_p_rho_step ----
-  if (valid) {
+  {
+    double pm1r[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = lev(r);
+      // first substep of a stage: pm1 is the step-0 pressure (calc_p_rho_wrf
+      // step 0 sets pm1 = p), read from p before this kernel overwrites it
+      if (valid && k < nz) pm1r[r] = ldo(first ? w.p : w.pm1, c8 + (unsigned)k * plane);
+    }
     const double mu_work = muts - mut;
-    for (int k = kt; k < nz; k += KT) {
-      size_t q = im(g, k, j, i);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = lev(r);
+      if (!(valid && k < nz)) continue;
+      const unsigned q = c8 + (unsigned)k * plane;
       double mass_h = g.c1h[k] * muts + g.c2h[k];
       double safe_mass = safe_pos(mass_h, 1.0e-12);
       double mu_term = g.c1h[k] * mu_work;
-      double alt = w.alt[q];
+      double alt = altr[r];
       double al = -(alt * mu_term + g.rdnw[k] * (L(B, k + 1) - L(B, k))) / safe_mass;
-      double theta_1 = s.theta[q] - kT0;
+      double theta_1 = thc[r] - kT0;
       double ttr = kT0 + theta_1;
       double safe_t = safe_pos(ttr, 1.0e-6);
-      double p = w.c2a[q] * (((alt * (L(E, k) - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
+      double p = c2ar[r] * (((alt * (L(E, k) - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
       double ptmp = p;
-      p = p + kSmdiv * (p - w.pm1[q]);
-      w.al[q] = al;
-      w.p[q] = p;
-      w.pm1[q] = ptmp;
+      p = p + kSmdiv * (p - pm1r[r]);
+      sto(w.al, q, al);
+      sto(w.p, q, p);
+      sto(w.pm1, q, ptmp);
     }
   }
 #undef L
@@ -453,26 +703,10 @@ __global__ void __launch_bounds__(TX * KT) k_column(DGrid g, DNamelist nl, DStat
 // ---------------------------------------------------------------------------
 __device__ inline double safe_den(double ms) { return fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12); }
 
-__global__ void k_fin_w(DGrid g, DState s, DWork w) {
-  size_t n = (size_t)(g.nz + 1) * g.ny * g.nx;
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  int k, j, i;
-  decompose(q, g.nx, g.ny, k, j, i);
-  size_t c = i2(g, j, i);
-  double ms = g.c1f[k] * w.muts[c] + g.c2f[k];
-  double mc = g.c1f[k] * w.mut[c] + g.c2f[k];
-  s.w[q] = (g.msfty[c] * w.w_work[q] + s.w[q] * mc) / safe_den(ms);
-  double ph_pert = w.ph_work[q] + s.ph_pert[q];
-  double ph_base = s.ph_total[q] - s.ph_pert[q];
-  s.ph_total[q] = ph_base + ph_pert;
-  s.ph_pert[q] = ph_pert;
-}
-
 __global__ void k_fin_uv(DGrid g, DState s, DWork w) {
   size_t nu = (size_t)g.nz * g.ny * (g.nx + 1);
   size_t nv = (size_t)g.nz * (g.ny + 1) * g.nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
   int k, j, i;
   if (idx < nu) {
     decompose(idx, g.nx + 1, g.ny, k, j, i);
@@ -490,116 +724,170 @@ __global__ void k_fin_uv(DGrid g, DState s, DWork w) {
   }
 }
 
-// theta, p (finish then _refresh_grid_p_from_finished, operational_mode.py:2169-2231
-// with diagnose_pressure_al_alt's base branch, acoustic_wrf.py:360-438).
-// Requires k_fin_w (new ph_total/ph_pert) to have run.
-__global__ void k_fin_m(DGrid g, DState s, DWork w) {
-  size_t n = (size_t)g.nz * g.ny * g.nx;
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  int k, j, i;
-  decompose(q, g.nx, g.ny, k, j, i);
-  size_t c = i2(g, j, i), q1 = im(g, k + 1, j, i);
-  double ms = g.c1h[k] * w.muts[c] + g.c2h[k];
-  double mc = g.c1h[k] * w.mut[c] + g.c2h[k];
-  double t_save = s.theta[q] - kT0;
-  double theta_pert = (w.th_work[q] + t_save * mc) / safe_den(ms);
-  double theta = theta_pert + kT0;
-  s.theta[q] = theta;
-  double p_base = s.p_total[q] - s.p_pert[q];
-  double p_total = p_base + w.p[q];
-  double p_pert = w.p[q];
-  // refresh
-  double mub = w.mub[c];
-  double mu_pert_new = w.mu[c];
-  double muts = mub + mu_pert_new;
-  double phb_k = s.ph_total[q] - s.ph_pert[q];
-  double phb_k1 = s.ph_total[q1] - s.ph_pert[q1];
-  double mass_h_base = g.c1h[k] * mub + g.c2h[k];
-  double denom_alb = g.dnw[k] * mass_h_base;
-  double safe_da = safe_pos(denom_alb, 1.0e-12);
-  double alb = -(phb_k1 - phb_k) / safe_da;
-  double mass_w = g.c1h[k] * muts + g.c2h[k];
-  double safe_mass = safe_pos(mass_w, 1.0e-12);
-  double mu_term = (alb * g.c1h[k]) * mu_pert_new;
-  double geo_term = g.rdnw[k] * (s.ph_pert[q1] - s.ph_pert[q]);
-  double al = -(mu_term + geo_term) / safe_mass;
-  double alt = al + alb;
-  double ptot = p_from_theta_alt(theta, alt);
-  double p_pert_new = ptot - w.pb[q];
-  double p_base2 = p_total - p_pert;
-  s.p_total[q] = p_base2 + p_pert_new;
-  s.p_pert[q] = p_pert_new;
-}
-
-__global__ void k_fin_mu(DGrid g, DState s, DWork w) {
-  size_t n = (size_t)g.ny * g.nx;
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  double mu_base = s.mu_total[c] - s.mu_pert[c];
-  s.mu_pert[c] = w.mu[c];
-  s.mu_total[c] = mu_base + w.mu[c];
+// small_step_finish for w, ph, theta, p and mu (core/small_step_finish.py:22-101)
+// followed by _refresh_grid_p_from_finished (operational_mode.py:2169-2231,
+// diagnose_pressure_al_alt's base branch, acoustic_wrf.py:360-438).  One
+// thread per column marches up the w levels: the finished ph of levels f-1
+// and f stay in registers for mass level f-1, and the next level's loads are
+// issued before the current level is computed.
+__global__ void __launch_bounds__(256) k_fin_col(DGrid g, DState s, DWork w) {
+  const size_t ncol = (size_t)g.ny * g.nx;
+  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int nz = g.nz;
+  const unsigned pl8 = (unsigned)ncol * 8u, c8 = (unsigned)c * 8u;
+  const double msfty = g.msfty[c];
+  const double muts = w.muts[c], mut = w.mut[c];
+  const double mub = w.mub[c], mu_pert_new = w.mu[c];
+  {
+    double mu_base = s.mu_total[c] - s.mu_pert[c];
+    s.mu_pert[c] = mu_pert_new;
+    s.mu_total[c] = mu_base + mu_pert_new;
+  }
+  struct Lv {
+    double ww, wsave, phw, php, pht;           // w level f
+    double th, thw, ptot, ppert, pwork, pb;    // mass level f
+  };
+  auto load = [&](int f, Lv& L) {
+    const unsigned q = c8 + (unsigned)f * pl8;
+    L.ww = ldo(w.w_work, q);
+    L.wsave = ldo(s.w, q);
+    L.phw = ldo(w.ph_work, q);
+    L.php = ldo(s.ph_pert, q);
+    L.pht = ldo(s.ph_total, q);
+    if (f < nz) {
+      L.th = ldo(s.theta, q);
+      L.thw = ldo(w.th_work, q);
+      L.ptot = ldo(s.p_total, q);
+      L.ppert = ldo(s.p_pert, q);
+      L.pwork = ldo(w.p, q);
+      L.pb = ldo(w.pb, q);
+    }
+  };
+  Lv cur{}, prev{};
+  double pht_prev = 0.0, php_prev = 0.0;  // finished ph total / perturbation at f-1
+  load(0, cur);
+  for (int f = 0; f <= nz; ++f) {
+    Lv nxt{};
+    if (f < nz) load(f + 1, nxt);
+    const unsigned q = c8 + (unsigned)f * pl8;
+    // w and ph at level f
+    double ms = g.c1f[f] * muts + g.c2f[f];
+    double mc = g.c1f[f] * mut + g.c2f[f];
+    sto(s.w, q, (msfty * cur.ww + cur.wsave * mc) / safe_den(ms));
+    const double ph_pert = cur.phw + cur.php;
+    const double ph_base = cur.pht - cur.php;
+    const double ph_total = ph_base + ph_pert;
+    sto(s.ph_total, q, ph_total);
+    sto(s.ph_pert, q, ph_pert);
+    if (f >= 1) {  // theta and p at mass level k = f - 1
+      const int k = f - 1;
+      const unsigned qk = q - pl8;
+      double msh = g.c1h[k] * muts + g.c2h[k];
+      double mch = g.c1h[k] * mut + g.c2h[k];
+      double t_save = prev.th - kT0;
+      double theta_pert = (prev.thw + t_save * mch) / safe_den(msh);
+      double theta = theta_pert + kT0;
+      sto(s.theta, qk, theta);
+      double p_base = prev.ptot - prev.ppert;
+      double p_total = p_base + prev.pwork;
+      double p_pert = prev.pwork;
+      double muts_r = mub + mu_pert_new;
+      double phb_k = pht_prev - php_prev;
+      double phb_k1 = ph_total - ph_pert;
+      double mass_h_base = g.c1h[k] * mub + g.c2h[k];
+      double denom_alb = g.dnw[k] * mass_h_base;
+      double safe_da = safe_pos(denom_alb, 1.0e-12);
+      double alb = -(phb_k1 - phb_k) / safe_da;
+      double mass_w = g.c1h[k] * muts_r + g.c2h[k];
+      double safe_mass = safe_pos(mass_w, 1.0e-12);
+      double mu_term = (alb * g.c1h[k]) * mu_pert_new;
+      double geo_term = g.rdnw[k] * (ph_pert - php_prev);
+      double al = -(mu_term + geo_term) / safe_mass;
+      double alt = al + alb;
+      double ptot = p_from_theta_alt(theta, alt);
+      double p_pert_new = ptot - prev.pb;
+      double p_base2 = p_total - p_pert;
+      sto(s.p_total, qk, p_base2 + p_pert_new);
+      sto(s.p_pert, qk, p_pert_new);
+    }
+    pht_prev = ph_total;
+    php_prev = ph_pert;
+    prev = cur;
+    cur = nxt;
+  }
 }
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-template <int TX, int KT>
+template <int TX, int KT, int R>
 hipError_t launch_column(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, double dts,
-                         hipStream_t st, Prof* prof) {
-  dim3 colgrid((g.nx + TX - 1) / TX, g.ny);
+                         int first, hipStream_t st, Prof* prof) {
+  const unsigned colgrid = (unsigned)((g.nx + TX - 1) / TX) * (unsigned)g.ny;
   size_t lds = (6 * (size_t)(g.nz + 1) * TX + 3 * TX) * sizeof(double);
-  WD_TIMED(prof, KID_COLUMN, st, (k_column<TX, KT><<<colgrid, TX * KT, lds, st>>>(g, nl, s, w, dts)));
+  WD_TIMED(prof, KID_COLUMN, st, (k_column<TX, KT, R><<<colgrid, TX * KT, lds, st>>>(g, nl, s, w, dts, first)));
   return hipGetLastError();
+}
+
+// R = levels per thread, the smallest that covers the nz+1 w levels.
+template <int TX, int KT>
+hipError_t launch_column_levels(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
+                                double dts, int first, hipStream_t st, Prof* prof) {
+  switch ((g.nz + 1 + KT - 1) / KT) {
+    case 1: return launch_column<TX, KT, 1>(g, nl, s, w, dts, first, st, prof);
+    case 2: return launch_column<TX, KT, 2>(g, nl, s, w, dts, first, st, prof);
+    case 3: return launch_column<TX, KT, 3>(g, nl, s, w, dts, first, st, prof);
+    case 4: return launch_column<TX, KT, 4>(g, nl, s, w, dts, first, st, prof);
+    case 5: return launch_column<TX, KT, 5>(g, nl, s, w, dts, first, st, prof);
+    case 6: return launch_column<TX, KT, 6>(g, nl, s, w, dts, first, st, prof);
+    case 7: return launch_column<TX, KT, 7>(g, nl, s, w, dts, first, st, prof);
+    case 8: return launch_column<TX, KT, 8>(g, nl, s, w, dts, first, st, prof);
+    default: return hipErrorInvalidValue;  // rejected by wd_create
+  }
 }
 
 }  // namespace
 
 // Column-tile shapes (TX columns x KT threads per column) compiled in.
-const ColumnTile kColumnTiles[] = {{16, 16}, {32, 8}, {16, 8}, {32, 4}, {64, 4}, {8, 16}};
+const ColumnTile kColumnTiles[] = {{16, 16}, {8, 16}, {32, 8}, {16, 8}};
 const int kNumColumnTiles = sizeof(kColumnTiles) / sizeof(kColumnTiles[0]);
 
 namespace {
 
 hipError_t launch_column_tile(int tile, const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
-                              double dts, hipStream_t st, Prof* prof) {
+                              double dts, int first, hipStream_t st, Prof* prof) {
   switch (tile) {
-    case 1: return launch_column<32, 8>(g, nl, s, w, dts, st, prof);
-    case 2: return launch_column<16, 8>(g, nl, s, w, dts, st, prof);
-    case 3: return launch_column<32, 4>(g, nl, s, w, dts, st, prof);
-    case 4: return launch_column<64, 4>(g, nl, s, w, dts, st, prof);
-    case 5: return launch_column<8, 16>(g, nl, s, w, dts, st, prof);
-    default: return launch_column<16, 16>(g, nl, s, w, dts, st, prof);
+    case 1: return launch_column_levels<8, 16>(g, nl, s, w, dts, first, st, prof);
+    case 2: return launch_column_levels<32, 8>(g, nl, s, w, dts, first, st, prof);
+    case 3: return launch_column_levels<16, 8>(g, nl, s, w, dts, first, st, prof);
+    default: return launch_column_levels<16, 16>(g, nl, s, w, dts, first, st, prof);
   }
 }
 
 }  // namespace
 
 hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
-                                   double dts, int column_tile, hipStream_t st, Prof* prof) {
+                                   double dts, int first, int column_tile, hipStream_t st, Prof* prof) {
   constexpr int BX = 64, BY = 4;
-  dim3 uvgrid((g.nx + 1 + BX - 1) / BX, (g.ny + 1 + BY - 1) / BY);
+  const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 1) / BX) * (unsigned)((g.ny + 1 + BY - 1) / BY);
   WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
-  return launch_column_tile(column_tile, g, nl, s, w, dts, st, prof);
+  return launch_column_tile(column_tile, g, nl, s, w, dts, first, st, prof);
 }
 
 hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
-  const size_t M = (size_t)g.nz * g.ny * g.nx;
-  const size_t Wn = (size_t)(g.nz + 1) * g.ny * g.nx;
   const size_t UV = (size_t)g.nz * g.ny * (g.nx + 1) + (size_t)g.nz * (g.ny + 1) * g.nx;
   const size_t C = (size_t)g.ny * g.nx;
-  WD_TIMED(prof, KID_FIN_W, st, (k_fin_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_FIN_UV, st, (k_fin_uv<<<blocks_for(UV, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_FIN_M, st, (k_fin_m<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_FIN_MU, st, (k_fin_mu<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_FIN_COL, st, (k_fin_col<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   return hipGetLastError();
 }
 
 const char* kernel_name(int id) {
   static const char* names[KID_COUNT] = {
-      "origin_copy", "k_ruv", "k_rom", "k_diag", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t",
-      "k_prep_2d", "k_prep_faces", "k_prep_u", "k_prep_v", "k_prep_w", "k_prep_m", "k_col_setup",
-      "k_uv", "k_column", "k_fin_w", "k_fin_uv", "k_fin_m", "k_fin_mu"};
+      "origin_copy", "k_ruv", "k_rom", "k_prep_col", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t",
+      "k_prep_2d", "k_prep_faces", "k_prep_u", "k_prep_v", "k_col_setup",
+      "k_uv", "k_column", "k_fin_uv", "k_fin_col"};
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 

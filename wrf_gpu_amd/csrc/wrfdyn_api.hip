@@ -56,6 +56,16 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   const int nx = grid->nx, ny = grid->ny, nz = grid->nz;
   if (nx < 4 || ny < 1 || nz < 4)
     return fail(nullptr, WD_EINVAL, "grid too small: need nx >= 4, ny >= 1, nz >= 4");
+  {
+    // the k-marching kernels address each 3-D array with 32-bit byte offsets
+    const double cells_w = (double)(nz + 1) * (double)(ny + 1) * (double)(nx + 1);
+    if (cells_w * 8.0 > 4294967295.0)
+      return fail(nullptr, WD_EINVAL, "grid too large: each 3-D field must stay below 4 GiB");
+  }
+  if (nz + 1 > wd::kColumnTiles[0].kt * wd::kMaxLevelsPerThread)
+    return fail(nullptr, WD_EINVAL, "nz too large: the column kernel holds at most " +
+                                        std::to_string(wd::kColumnTiles[0].kt * wd::kMaxLevelsPerThread - 1) +
+                                        " levels");
   if (!(grid->dx > 0.0) || !(grid->dy > 0.0)) return fail(nullptr, WD_EINVAL, "dx and dy must be positive");
   const double* req[] = {grid->c1h, grid->c2h, grid->c1f, grid->c2f, grid->dnw, grid->rdnw, grid->dn,
                          grid->rdn, grid->fnm, grid->fnp, grid->msftx, grid->msfty, grid->msfux,
@@ -176,9 +186,9 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
       {&w.mu, c},       {&w.muts, c},      {&w.muave, c},    {&w.muu, cu},     {&w.muus, cu},
       {&w.muv, cv},     {&w.muvs, cv},     {&w.u_work, U},   {&w.v_work, V},   {&w.w_work, Wn},
       {&w.th_work, M},  {&w.ph_work, Wn},  {&w.c2a, M},      {&w.alt, M},      {&w.pb, M},
-      {&w.phb, Wn},     {&w.php, M},       {&w.p, M},        {&w.al, M},       {&w.pm1, M},
+      {&w.ph1t, Wn},     {&w.php, M},       {&w.p, M},        {&w.al, M},       {&w.pm1, M},
       {&w.rw_tend, Wn}, {&w.ph_tend, Wn},  {&w.ca, Wn},      {&w.calpha, Wn},  {&w.cgamma, Wn},
-      {&w.ww, Wn},      {&w.t2ave, M},     {&w.dvdxi, M},    {&w.rhs, Wn},
+      {&w.ww, Wn},      {&w.t2ave, M},
   };
   // 256-byte aligned sub-buffers
   size_t total = 0;
@@ -209,9 +219,15 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     int tx = 0, kt = 0;
     if (std::sscanf(env, "%dx%d", &tx, &kt) == 2) {
       for (int t = 0; t < wd::kNumColumnTiles; ++t)
-        if (wd::kColumnTiles[t].tx == tx && wd::kColumnTiles[t].kt == kt) h->column_tile = t;
+        if (wd::kColumnTiles[t].tx == tx && wd::kColumnTiles[t].kt == kt &&
+            nz + 1 <= kt * wd::kMaxLevelsPerThread)
+          h->column_tile = t;
     }
   }
+  // XCD-contiguous tile order (default on); WRFDYN_XCD_REMAP=0 keeps the
+  // hardware dispatch order, for measurement.
+  h->g.xcd_remap = 1;
+  if (const char* env = std::getenv("WRFDYN_XCD_REMAP")) h->g.xcd_remap = std::atoi(env) != 0;
   *out = h;
   return WD_OK;
 }
@@ -260,7 +276,7 @@ int wd_step(wd_handle* h, const wd_state* st, int n_large_steps, void* stream) {
       e = wd::launch_stage_setup(g, h->nl, ds, w, sd.rk_step, sd.dts, s, prof);
       if (e != hipSuccess) return hip_fail(h, e, "stage setup launch");
       for (int it = 0; it < sd.nsub; ++it) {
-        e = wd::launch_acoustic_substep(g, h->nl, ds, w, sd.dts, h->column_tile, s, prof);
+        e = wd::launch_acoustic_substep(g, h->nl, ds, w, sd.dts, it == 0, h->column_tile, s, prof);
         if (e != hipSuccess) return hip_fail(h, e, "acoustic substep launch");
       }
       e = wd::launch_stage_finish(g, ds, w, s, prof);

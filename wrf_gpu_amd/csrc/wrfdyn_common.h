@@ -38,6 +38,7 @@ struct DGrid {
   const double *dnw, *rdnw, *dn, *rdn, *fnm, *fnp;
   double cf1, cf2, cf3, cfn, cfn1;
   const double *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy;
+  int xcd_remap;  // 1: walk tiles XCD-contiguously (xcd_tile), 0: hardware order
 };
 
 struct DNamelist {
@@ -68,14 +69,12 @@ struct DWork {
   double *muu, *muus, *muv, *muvs;
   // prep (3-D)
   double *u_work, *v_work, *w_work, *th_work, *ph_work;
-  double *c2a, *alt, *pb, *phb, *php;
+  double *c2a, *alt, *pb, *ph1t, *php;  // ph1t = origin ph' + base ph (stage constant)
   double *p, *al, *pm1;
   // stage column setup
   double *rw_tend, *ph_tend, *ca, *calpha, *cgamma;
-  // acoustic carried
+  // acoustic carried (ww: only levels 0 and nz carry between substeps)
   double *ww, *t2ave;
-  // column-kernel scratch
-  double *dvdxi, *rhs;
 };
 
 __host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {
@@ -95,6 +94,33 @@ __device__ inline int wrapi(int i, int n) {
   int r = i % n;
   return r < 0 ? r + n : r;
 }
+// gfx950 dispatches the workgroups of a launch round-robin over its 8 XCDs
+// (linear workgroup id b runs on XCD b % 8), and each XCD has its own L2.
+// xcd_tile maps b to a logical tile id so that every XCD walks one contiguous
+// run of the n tiles: neighbouring tiles, which share halo rows and the cache
+// lines split between them, then meet in the same L2.  A bijection on [0, n).
+constexpr unsigned kNumXcd = 8;
+__device__ inline unsigned xcd_tile(unsigned b, unsigned n) {
+  const unsigned x = b % kNumXcd, s = b / kNumXcd, q = n / kNumXcd, r = n % kNumXcd;
+  return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
+__device__ inline unsigned tile_id(const DGrid& g) {
+  return g.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+}
+
+// Loads and stores at a 32-bit byte offset from a wave-uniform base pointer.
+// The compiler then addresses with the base in SGPRs plus one 32-bit VGPR
+// offset (the global_load/store saddr form) instead of a 64-bit VGPR address
+// per access, which halves address registers and address arithmetic in the
+// k-marching kernels.  Every 3-D array must stay below 4 GiB (wd_create
+// checks the grid).
+__device__ __forceinline__ double ldo(const double* base, unsigned byte_off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ void sto(double* base, unsigned byte_off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
 __device__ inline int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 // jnp.sign: -1, +1, and the argument itself for +-0 and NaN.

@@ -14,7 +14,7 @@ enum KernelId {
   KID_ORIGIN_COPY = 0,
   KID_RUV,
   KID_ROM,
-  KID_DIAG,
+  KID_PREP_COL,
   KID_TEND_U,
   KID_TEND_V,
   KID_TEND_W,
@@ -23,15 +23,11 @@ enum KernelId {
   KID_PREP_FACES,
   KID_PREP_U,
   KID_PREP_V,
-  KID_PREP_W,
-  KID_PREP_M,
   KID_COL_SETUP,
   KID_UV,
   KID_COLUMN,
-  KID_FIN_W,
   KID_FIN_UV,
-  KID_FIN_M,
-  KID_FIN_MU,
+  KID_FIN_COL,
   KID_COUNT
 };
 
@@ -89,17 +85,21 @@ struct Prof {
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                               int rk_step, double dts, hipStream_t st, Prof* prof);
 
-// Column-kernel tile shapes: TX columns x KT threads per column.
+// Column-kernel tile shapes: TX columns x KT threads per column; each thread
+// owns up to kMaxLevelsPerThread of the nz+1 levels, so a shape serves
+// nz + 1 <= kt * kMaxLevelsPerThread.
 struct ColumnTile {
   int tx, kt;
 };
+constexpr int kMaxLevelsPerThread = 8;
 extern const ColumnTile kColumnTiles[];
 extern const int kNumColumnTiles;
 
 // One acoustic substep: advance_uv, then the fused column kernel with tile
-// shape kColumnTiles[column_tile].
+// shape kColumnTiles[column_tile].  `first` marks a stage's first substep,
+// which starts from t_2ave = 0 and pm1 = the step-0 pressure.
 hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
-                                   double dts, int column_tile, hipStream_t st, Prof* prof);
+                                   double dts, int first, int column_tile, hipStream_t st, Prof* prof);
 
 // small_step_finish + grid%p refresh.
 hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof);

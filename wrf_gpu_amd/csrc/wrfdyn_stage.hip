@@ -23,7 +23,7 @@ __device__ inline void decompose(size_t idx, int nxx, int nyy, int& k, int& j, i
 __global__ void k_ruv(DGrid g, const double* __restrict__ u, const double* __restrict__ v,
                       const double* __restrict__ mu, double* __restrict__ ru, double* __restrict__ rv) {
   size_t n = (size_t)g.nz * g.ny * g.nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   int k, j, i;
   decompose(idx, g.nx, g.ny, k, j, i);
@@ -38,8 +38,10 @@ __global__ void k_ruv(DGrid g, const double* __restrict__ u, const double* __res
 // calc_ww_cp omega, dynamics/flux_advection.py:430-458 (one thread per column)
 __global__ void k_rom(DGrid g, const double* __restrict__ ru, const double* __restrict__ rv,
                       double* __restrict__ rom) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y;
+  const unsigned nbx = (g.nx + blockDim.x - 1) / blockDim.x;
+  const unsigned t = tile_id(g);
+  int i = (int)(t % nbx * blockDim.x + threadIdx.x);
+  int j = (int)(t / nbx);
   if (i >= g.nx) return;
   int ip = wrapi(i + 1, g.nx), jp = wrapi(j + 1, g.ny);
   double msftx = g.msftx[i2(g, j, i)];
@@ -61,266 +63,492 @@ __global__ void k_rom(DGrid g, const double* __restrict__ ru, const double* __re
   rom[im(g, g.nz, j, i)] = 0.0;
 }
 
-// ---------------------------------------------------------------------------
-// Stage-entry EOS + _absolute_diagnostics (core/rk_addtend_dry.py:127-229,
-// hypsometric_opt=1): alt (shared with small_step_prep's base-less
-// diagnose_pressure_al_alt, acoustic_wrf.py:354-358), large-step al and php.
-// ---------------------------------------------------------------------------
-__global__ void k_diag(DGrid g, DState s, DWork w) {
-  size_t n = (size_t)g.nz * g.ny * g.nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, g.nx, g.ny, k, j, i);
-  w.alt_raw[idx] = alt_from_theta_p(s.theta[idx], s.p_total[idx]);
-  size_t c = i2(g, j, i);
-  double mu_pert = s.mu_pert[c];
-  double mu_total = s.mu_total[c];
-  double mub = mu_total - mu_pert;
-  double muts = mu_total + mu_pert;
-  size_t q = idx, q1 = im(g, k + 1, j, i);
-  double phb_k = s.ph_total[q] - s.ph_pert[q];
-  double phb_k1 = s.ph_total[q1] - s.ph_pert[q1];
-  double mass_base = g.c1h[k] * mub + g.c2h[k];
-  double safe_base = safe_pos(mass_base, 1.0e-12);
-  double alb = ((-g.rdnw[k]) * (phb_k1 - phb_k)) / safe_base;
-  double mass_t = g.c1h[k] * muts + g.c2h[k];
-  double safe_t = safe_pos(mass_t, 1.0e-12);
-  double mu_term = g.c1h[k] * mu_pert;
-  w.al_ls[q] = -(alb * mu_term + g.rdnw[k] * (s.ph_pert[q1] - s.ph_pert[q])) / safe_t;
-  double pht_k = phb_k + s.ph_pert[q];
-  double pht_k1 = phb_k1 + s.ph_pert[q1];
-  w.php_ls[q] = 0.5 * (pht_k + pht_k1);
-}
-
-// dpn face value from pair sums of a mass-level pressure p at columns (a, b)
-// (large-step PGF form, rk_addtend_dry.py:314-327, top_lid with cfn/cfn1).
-__device__ inline double dpn_ls(const DGrid& g, const double* __restrict__ p, int kk, size_t ca,
-                                size_t cb, size_t plane) {
-#define PS(k) (p[(size_t)(k) * plane + ca] + p[(size_t)(k) * plane + cb])
-  if (kk == 0) return 0.5 * ((g.cf1 * PS(0) + g.cf2 * PS(1)) + g.cf3 * PS(2));
-  if (kk == g.nz) return 0.5 * (g.cfn * PS(g.nz - 1) + g.cfn1 * PS(g.nz - 2));
-  return 0.5 * (g.fnm[kk] * PS(kk) + g.fnp[kk] * PS(kk - 1));
-#undef PS
-}
-
-// _vertical_flux_div_3 face flux for one column (flux_advection.py:1362-1407);
-// q(k) and romq(k) are accessed through the functors.
-template <class Q, class R>
-__device__ inline double vflux3_face(const DGrid& g, int kf, Q q, R romq) {
+// _vertical_flux_div_3 face flux (flux_advection.py:1362-1407) from a column
+// window: qm2..qp1 = q at kf-2 .. kf+1, r = the transport at face kf.
+__device__ inline double vface3(const DGrid& g, int kf, double qm2, double qm1, double q0, double qp1, double r) {
   const int nz = g.nz;
   if (kf == 0 || kf == nz) return 0.0;
-  if (kf == 1) return romq(1) * (g.fnm[1] * q(1) + g.fnp[1] * q(0));
-  if (kf == nz - 1) return romq(nz - 1) * (g.fnm[nz - 1] * q(nz - 1) + g.fnp[nz - 1] * q(nz - 2));
-  double r = romq(kf);
-  return r * flux3(q(kf - 2), q(kf - 1), q(kf), q(kf + 1), -r);
+  if (kf == 1) return r * (g.fnm[1] * q0 + g.fnp[1] * qm1);
+  if (kf == nz - 1) return r * (g.fnm[nz - 1] * q0 + g.fnp[nz - 1] * qm1);
+  return r * flux3(qm2, qm1, q0, qp1, -r);
 }
 
-// ---------------------------------------------------------------------------
-// advect_u_flux (flux_advection.py:1131-1202) + large_step_horizontal_pgf x
-// (rk_addtend_dry.py:329-343, 357-361) on u faces (nz, ny, nx+1).
-// ---------------------------------------------------------------------------
-__global__ void k_tend_u(DGrid g, DState s, DWork w) {
+// dpn face value (large-step PGF, rk_addtend_dry.py:314-327, top_lid with
+// cfn/cfn1) from the pair sums ps(k) = p(k, a) + p(k, b) of levels kk-2..kk
+// (ps0 at kk, psm1 at kk-1, psm2 at kk-2); the bottom face takes levels 0..2.
+__device__ inline double dpn_ls_top(const DGrid& g, double psm1, double psm2) {
+  return 0.5 * (g.cfn * psm1 + g.cfn1 * psm2);
+}
+__device__ inline double dpn_ls_mid(const DGrid& g, int kk, double ps0, double psm1) {
+  return 0.5 * (g.fnm[kk] * ps0 + g.fnp[kk] * psm1);
+}
+__device__ inline double dpn_ls_bot(const DGrid& g, double ps0, double ps1, double ps2) {
+  return 0.5 * ((g.cf1 * ps0 + g.cf2 * ps1) + g.cf3 * ps2);
+}
+
+// The four tendency kernels below march up one column per thread: the
+// vertical stencil (flux3 over k-2..k+1, the PGF pair sums over k-1..k+1)
+// runs from register windows and the horizontal stencil (flux5 over +-3
+// columns and rows) reads level k, all of whose loads are issued together.
+// k_tend_w also issues level k+1's loads before computing level k; for the
+// other three the registers that takes cost more occupancy than it gains.
+// XCD-contiguous tile order keeps neighbouring rows in one L2.
+
+// advect_scalar_flux of theta-300 (flux_advection.py:482-568) on mass levels.
+__global__ void __launch_bounds__(256) k_tend_t(DGrid g, DState s, DWork w) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  size_t n = (size_t)nz * ny * (nx + 1);
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, nx + 1, ny, k, j, i);
-  const double* u = s.u;
-  // --- advection on the collapsed periodic face ia ---
-  int ia = (i == nx) ? 0 : i;
-  int ip = wrapi(ia + 1, nx), imn = wrapi(ia - 1, nx);
-  auto UX = [&](int sft) { return u[iu(g, k, j, wrapi(ia + sft, nx))]; };
-  double velx0 = 0.5 * (w.ru[im(g, k, j, ia)] + w.ru[im(g, k, j, imn)]);
-  double velx1 = 0.5 * (w.ru[im(g, k, j, ip)] + w.ru[im(g, k, j, ia)]);
-  double fq0 = velx0 * flux5(UX(-3), UX(-2), UX(-1), UX(0), UX(1), UX(2), velx0);
-  double fq1 = velx1 * flux5(UX(-2), UX(-1), UX(0), UX(1), UX(2), UX(3), velx1);
-  double msf = g.msfux[i2u(g, j, ia)];
-  double t = ((-msf) * g.rdx) * (fq1 - fq0);
-  if (ny > 1) {
-    int jp = wrapi(j + 1, ny);
-    auto UY = [&](int jj, int sft) { return u[iu(g, k, wrapi(jj + sft, ny), ia)]; };
-    double vy0 = 0.5 * (w.rv[im(g, k, j, ia)] + w.rv[im(g, k, j, imn)]);
-    double vy1 = 0.5 * (w.rv[im(g, k, jp, ia)] + w.rv[im(g, k, jp, imn)]);
-    double g0 = vy0 * flux5(UY(j, -3), UY(j, -2), UY(j, -1), UY(j, 0), UY(j, 1), UY(j, 2), vy0);
-    double g1 = vy1 * flux5(UY(jp, -3), UY(jp, -2), UY(jp, -1), UY(jp, 0), UY(jp, 1), UY(jp, 2), vy1);
-    t = t - (msf * g.rdy) * (g1 - g0);
+  const size_t ncol = (size_t)ny * nx;
+  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
+  const unsigned pl8 = (unsigned)ncol * 8u;
+  unsigned ox[7], oy[7];  // byte offsets of columns i-3..i+3 (row j) and rows j-3..j+3 (column i)
+#pragma unroll
+  for (int d = 0; d < 7; ++d) {
+    ox[d] = (unsigned)i2(g, j, wrapi(i + d - 3, nx)) * 8u;
+    oy[d] = (unsigned)i2(g, wrapi(j + d - 3, ny), i) * 8u;
   }
-  {
-    auto q = [&](int kk) { return u[iu(g, kk, j, ia)]; };
-    auto romq = [&](int kk) { return 0.5 * (w.rom[im(g, kk, j, ia)] + w.rom[im(g, kk, j, imn)]); };
-    double vlo = vflux3_face(g, k, q, romq);
-    double vhi = vflux3_face(g, k + 1, q, romq);
-    t = t + (-g.rdnw[k]) * (vhi - vlo);
+  const unsigned c8 = ox[3], cip8 = ox[4], cjp8 = oy[4];
+  const double msftx = g.msftx[c];
+  const double* th = s.theta;
+  struct Lv {
+    double x[7], y[7];  // theta at columns i-3..i+3 and rows j-3..j+3 (x[3], y[3]: centre)
+    double ru0, ru1, rv0, rv1;
+  };
+  auto load = [&](int k, Lv& L) {
+    const unsigned o = (unsigned)k * pl8;
+#pragma unroll
+    for (int d = 0; d < 7; ++d) L.x[d] = ldo(th, o + ox[d]);
+#pragma unroll
+    for (int d = 0; d < 7; ++d)
+      if (d != 3) L.y[d] = ldo(th, o + oy[d]);
+    L.y[3] = L.x[3];
+    L.ru0 = ldo(w.ru, o + c8);
+    L.ru1 = ldo(w.ru, o + cip8);
+    L.rv0 = ldo(w.rv, o + c8);
+    L.rv1 = ldo(w.rv, o + cjp8);
+  };
+  // centre-column window of theta-300 (k-1..k+3) and rom (k+1, k+2)
+  double qm1 = 0.0, q0 = ldo(th, c8) - kT0, qp1 = ldo(th, pl8 + c8) - kT0, qp2 = ldo(th, 2 * pl8 + c8) - kT0;
+  double rom1 = ldo(w.rom, pl8 + c8);
+  double vf_lo = 0.0;  // vertical face flux at k (0 at the surface)
+  Lv cur{};
+  for (int k = 0; k < nz; ++k) {
+    const double qp3 = k + 3 < nz ? ldo(th, (unsigned)(k + 3) * pl8 + c8) - kT0 : 0.0;
+    const double rom2 = k + 2 <= nz ? ldo(w.rom, (unsigned)(k + 2) * pl8 + c8) : 0.0;
+    load(k, cur);
+    auto QX = [&](int sft) { return cur.x[sft + 3] - kT0; };
+    auto QY = [&](int d) { return cur.y[d + 3] - kT0; };
+    double vel0 = cur.ru0, vel1 = cur.ru1;
+    double fq0 = vel0 * flux5(QX(-3), QX(-2), QX(-1), QX(0), QX(1), QX(2), vel0);
+    double fq1 = vel1 * flux5(QX(-2), QX(-1), QX(0), QX(1), QX(2), QX(3), vel1);
+    double t = ((-msftx) * g.rdx) * (fq1 - fq0);
+    double vy0 = cur.rv0, vy1 = cur.rv1;
+    double gy0 = vy0 * flux5(QY(-3), QY(-2), QY(-1), QY(0), QY(1), QY(2), vy0);
+    double gy1 = vy1 * flux5(QY(-2), QY(-1), QY(0), QY(1), QY(2), QY(3), vy1);
+    t = t - (msftx * g.rdy) * (gy1 - gy0);
+    const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, rom1);
+    t = t - g.rdnw[k] * (vf_hi - vf_lo);
+    sto(w.tht, (unsigned)k * pl8 + c8, t);
+    qm1 = q0;
+    q0 = qp1;
+    qp1 = qp2;
+    qp2 = qp3;
+    rom1 = rom2;
+    vf_lo = vf_hi;
   }
-  // --- large-step horizontal PGF at face i (edge-padded pairs) ---
-  int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
-  size_t cl = i2(g, j, il), cr = i2(g, j, ir);
-  size_t plane = (size_t)ny * nx;
-  double msf_u = g.msfux[i2u(g, j, i)] / g.msfuy[i2u(g, j, i)];
-  double mub_l = s.mu_total[cl] - s.mu_pert[cl];
-  double mub_r = s.mu_total[cr] - s.mu_pert[cr];
-  double mt_l = mub_l + s.mu_pert[cl];
-  double mt_r = mub_r + s.mu_pert[cr];
-  double muu = 0.5 * (mt_l + mt_r);
-  double mu_l = s.mu_pert[cl], mu_r = s.mu_pert[cr];
-  double dpn_k = dpn_ls(g, s.p_pert, k, cl, cr, plane);
-  double dpn_k1 = dpn_ls(g, s.p_pert, k + 1, cl, cr, plane);
-  size_t ql = im(g, k, j, il), qr = im(g, k, j, ir);
-  size_t ql1 = im(g, k + 1, j, il), qr1 = im(g, k + 1, j, ir);
-  double mass_u = g.c1h[k] * muu + g.c2h[k];
-  double ph_term = (s.ph_pert[qr1] - s.ph_pert[ql1]) + (s.ph_pert[qr] - s.ph_pert[ql]);
-  double p_term = (w.alt_raw[ql] + w.alt_raw[qr]) * (s.p_pert[qr] - s.p_pert[ql]);
-  double pb_l = s.p_total[ql] - s.p_pert[ql];
-  double pb_r = s.p_total[qr] - s.p_pert[qr];
-  double pb_term = (w.al_ls[ql] + w.al_ls[qr]) * (pb_r - pb_l);
-  double dpx = (((msf_u * 0.5) * g.rdx) * mass_u) * ((ph_term + p_term) + pb_term);
-  double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_l + mu_r));
-  dpx = dpx + ((msf_u * g.rdx) * (w.php_ls[qr] - w.php_ls[ql])) * bracket;
-  w.ut[idx] = t + (-1.0 * dpx);
-}
-
-// advect_v_flux (flux_advection.py:1205-1274) + large-step PGF y
-// (rk_addtend_dry.py:345-361) on v faces (nz, ny+1, nx).
-__global__ void k_tend_v(DGrid g, DState s, DWork w) {
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  size_t n = (size_t)nz * (ny + 1) * nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, nx, ny + 1, k, j, i);
-  const double* v = s.v;
-  int ja = (j == ny) ? 0 : j;
-  int jm = wrapi(ja - 1, ny), jp = wrapi(ja + 1, ny), ip = wrapi(i + 1, nx);
-  auto VX = [&](int sft) { return v[iv(g, k, ja, wrapi(i + sft, nx))]; };
-  auto VY = [&](int jj, int sft) { return v[iv(g, k, wrapi(jj + sft, ny), i)]; };
-  double vx0 = 0.5 * (w.ru[im(g, k, ja, i)] + w.ru[im(g, k, jm, i)]);
-  double vx1 = 0.5 * (w.ru[im(g, k, ja, ip)] + w.ru[im(g, k, jm, ip)]);
-  double fq0 = vx0 * flux5(VX(-3), VX(-2), VX(-1), VX(0), VX(1), VX(2), vx0);
-  double fq1 = vx1 * flux5(VX(-2), VX(-1), VX(0), VX(1), VX(2), VX(3), vx1);
-  double msfvy = g.msfvy[i2v(g, ja, i)];
-  double t = ((-msfvy) * g.rdx) * (fq1 - fq0);
-  double vy0 = 0.5 * (w.rv[im(g, k, ja, i)] + w.rv[im(g, k, jm, i)]);
-  double vy1 = 0.5 * (w.rv[im(g, k, jp, i)] + w.rv[im(g, k, ja, i)]);
-  double g0 = vy0 * flux5(VY(ja, -3), VY(ja, -2), VY(ja, -1), VY(ja, 0), VY(ja, 1), VY(ja, 2), vy0);
-  double g1 = vy1 * flux5(VY(jp, -3), VY(jp, -2), VY(jp, -1), VY(jp, 0), VY(jp, 1), VY(jp, 2), vy1);
-  t = t - (msfvy * g.rdy) * (g1 - g0);
-  {
-    auto q = [&](int kk) { return v[iv(g, kk, ja, i)]; };
-    auto romq = [&](int kk) { return 0.5 * (w.rom[im(g, kk, ja, i)] + w.rom[im(g, kk, jm, i)]); };
-    double ratio = g.msfvy[i2v(g, ja, i)] / g.msfvx[i2v(g, ja, i)];
-    double vlo = vflux3_face(g, k, q, romq);
-    double vhi = vflux3_face(g, k + 1, q, romq);
-    double vd = (-g.rdnw[k]) * (vhi - vlo);
-    t = t + ratio * vd;
-  }
-  // --- large-step PGF at v face j ---
-  int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
-  size_t cs = i2(g, js, i), cn = i2(g, jn, i);
-  size_t plane = (size_t)ny * nx;
-  double msf_v = g.msfvy[i2v(g, j, i)] / g.msfvx[i2v(g, j, i)];
-  double mub_s = s.mu_total[cs] - s.mu_pert[cs];
-  double mub_n = s.mu_total[cn] - s.mu_pert[cn];
-  double mt_s = mub_s + s.mu_pert[cs];
-  double mt_n = mub_n + s.mu_pert[cn];
-  double muv = 0.5 * (mt_s + mt_n);
-  double mu_s = s.mu_pert[cs], mu_n = s.mu_pert[cn];
-  double dpn_k = dpn_ls(g, s.p_pert, k, cs, cn, plane);
-  double dpn_k1 = dpn_ls(g, s.p_pert, k + 1, cs, cn, plane);
-  size_t qs = im(g, k, js, i), qn = im(g, k, jn, i);
-  size_t qs1 = im(g, k + 1, js, i), qn1 = im(g, k + 1, jn, i);
-  double mass_v = g.c1h[k] * muv + g.c2h[k];
-  double ph_term = (s.ph_pert[qn1] - s.ph_pert[qs1]) + (s.ph_pert[qn] - s.ph_pert[qs]);
-  double p_term = (w.alt_raw[qs] + w.alt_raw[qn]) * (s.p_pert[qn] - s.p_pert[qs]);
-  double pb_s = s.p_total[qs] - s.p_pert[qs];
-  double pb_n = s.p_total[qn] - s.p_pert[qn];
-  double pb_term = (w.al_ls[qs] + w.al_ls[qn]) * (pb_n - pb_s);
-  double dpy = (((msf_v * 0.5) * g.rdy) * mass_v) * ((ph_term + p_term) + pb_term);
-  double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_s + mu_n));
-  dpy = dpy + ((msf_v * g.rdy) * (w.php_ls[qn] - w.php_ls[qs])) * bracket;
-  w.vt[idx] = t + (-1.0 * dpy);
-}
-
-// _mass_to_full_levels (flux_advection.py:1332-1348)
-__device__ inline double mass_to_full(const DGrid& g, const double* __restrict__ r, int kf, int j, int i) {
-  if (kf == 0) return r[im(g, 0, j, i)];
-  if (kf == g.nz) return r[im(g, g.nz - 1, j, i)];
-  return g.fnm[kf] * r[im(g, kf, j, i)] + g.fnp[kf] * r[im(g, kf - 1, j, i)];
 }
 
 // advect_w_flux + _vertical_flux_div_w with a rigid lid
-// (flux_advection.py:1277-1329, 1410-1482) on w levels (nz+1, ny, nx).
-__global__ void k_tend_w(DGrid g, DState s, DWork w) {
+// (flux_advection.py:1277-1329, 1410-1482) on w levels (nz+1, ny, nx); the
+// transport is _mass_to_full_levels of ru/rv (flux_advection.py:1332-1348).
+__global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  size_t n = (size_t)(nz + 1) * ny * nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, nx, ny, k, j, i);
-  const double* ww = s.w;
-  int ip = wrapi(i + 1, nx);
-  auto WX = [&](int sft) { return ww[im(g, k, j, wrapi(i + sft, nx))]; };
-  double vx0 = mass_to_full(g, w.ru, k, j, i), vx1 = mass_to_full(g, w.ru, k, j, ip);
-  double fq0 = vx0 * flux5(WX(-3), WX(-2), WX(-1), WX(0), WX(1), WX(2), vx0);
-  double fq1 = vx1 * flux5(WX(-2), WX(-1), WX(0), WX(1), WX(2), WX(3), vx1);
-  double msf = g.msftx[i2(g, j, i)];
-  double t = ((-msf) * g.rdx) * (fq1 - fq0);
-  if (ny > 1) {
-    int jp = wrapi(j + 1, ny);
-    auto WY = [&](int jj, int sft) { return ww[im(g, k, wrapi(jj + sft, ny), i)]; };
-    double vy0 = mass_to_full(g, w.rv, k, j, i), vy1 = mass_to_full(g, w.rv, k, jp, i);
-    double g0 = vy0 * flux5(WY(j, -3), WY(j, -2), WY(j, -1), WY(j, 0), WY(j, 1), WY(j, 2), vy0);
-    double g1 = vy1 * flux5(WY(jp, -3), WY(jp, -2), WY(jp, -1), WY(jp, 0), WY(jp, 1), WY(jp, 2), vy1);
-    t = t - (msf * g.rdy) * (g1 - g0);
+  const size_t ncol = (size_t)ny * nx;
+  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
+  const unsigned pl8 = (unsigned)ncol * 8u;
+  unsigned ox[7], oy[7];
+#pragma unroll
+  for (int d = 0; d < 7; ++d) {
+    ox[d] = (unsigned)i2(g, j, wrapi(i + d - 3, nx)) * 8u;
+    oy[d] = (unsigned)i2(g, wrapi(j + d - 3, ny), i) * 8u;
   }
-  if (k >= 1 && k < nz) {
-    auto VF = [&](int kk) { return 0.5 * (w.rom[im(g, kk, j, i)] + w.rom[im(g, kk - 1, j, i)]); };
-    auto W = [&](int kk) { return ww[im(g, kk, j, i)]; };
-    auto vflux = [&](int kk) -> double {
-      if (kk <= 0 || kk >= nz) return 0.0;
-      if (kk == 1) return (VF(1) * 0.5) * (W(1) + W(0));
-      double vel = VF(kk);
-      return vel * flux3(W(kk - 2), W(kk - 1), W(kk), W(kk + 1), -vel);
+  const unsigned c8 = ox[3], cip8 = ox[4], cjp8 = oy[4];
+  const double msf = g.msftx[c];
+  const bool do_y = ny > 1;
+  const double* W = s.w;
+  struct Lv {
+    double x[7], y[7];
+    double ru0, ru1, rv0, rv1;  // mass level f (f < nz)
+  };
+  auto load = [&](int f, Lv& L) {
+    const unsigned o = (unsigned)f * pl8;
+#pragma unroll
+    for (int d = 0; d < 7; ++d) L.x[d] = ldo(W, o + ox[d]);
+    if (do_y) {
+#pragma unroll
+      for (int d = 0; d < 7; ++d)
+        if (d != 3) L.y[d] = ldo(W, o + oy[d]);
+    }
+    L.y[3] = L.x[3];
+    if (f < nz) {
+      L.ru0 = ldo(w.ru, o + c8);
+      L.ru1 = ldo(w.ru, o + cip8);
+      L.rv0 = ldo(w.rv, o + c8);
+      L.rv1 = ldo(w.rv, o + cjp8);
+    }
+  };
+  // centre-column window of w (f-1..f+3) and rom (f, f+1, f+2)
+  double wm1 = 0.0, w0 = ldo(W, c8), wp1 = ldo(W, pl8 + c8), wp2 = ldo(W, 2 * pl8 + c8);
+  double rom0 = ldo(w.rom, c8), rom1 = ldo(w.rom, pl8 + c8);
+  double vfl_lo = 0.0;  // vflux(f); vflux(0) = 0
+  double ru0m = 0.0, ru1m = 0.0, rv0m = 0.0, rv1m = 0.0;  // ru/rv at mass level f-1
+  Lv cur{};
+  load(0, cur);
+  for (int f = 0; f <= nz; ++f) {
+    const double wp3 = f + 3 <= nz ? ldo(W, (unsigned)(f + 3) * pl8 + c8) : 0.0;
+    const double rom2 = f + 2 <= nz ? ldo(w.rom, (unsigned)(f + 2) * pl8 + c8) : 0.0;
+    Lv nxt{};
+    if (f + 1 <= nz) load(f + 1, nxt);
+    auto m2f = [&](double r_f, double r_fm1) {
+      if (f == 0) return r_f;
+      if (f == nz) return r_fm1;
+      return g.fnm[f] * r_f + g.fnp[f] * r_fm1;
     };
-    t = t + (-g.rdn[k]) * (vflux(k + 1) - vflux(k));
-  } else {
-    t = t + 0.0;
+    auto WX = [&](int sft) { return cur.x[sft + 3]; };
+    auto WY = [&](int d) { return cur.y[d + 3]; };
+    double vx0 = m2f(cur.ru0, ru0m), vx1 = m2f(cur.ru1, ru1m);
+    double fq0 = vx0 * flux5(WX(-3), WX(-2), WX(-1), WX(0), WX(1), WX(2), vx0);
+    double fq1 = vx1 * flux5(WX(-2), WX(-1), WX(0), WX(1), WX(2), WX(3), vx1);
+    double t = ((-msf) * g.rdx) * (fq1 - fq0);
+    if (do_y) {
+      double vy0 = m2f(cur.rv0, rv0m), vy1 = m2f(cur.rv1, rv1m);
+      double g0 = vy0 * flux5(WY(-3), WY(-2), WY(-1), WY(0), WY(1), WY(2), vy0);
+      double g1 = vy1 * flux5(WY(-2), WY(-1), WY(0), WY(1), WY(2), WY(3), vy1);
+      t = t - (msf * g.rdy) * (g1 - g0);
+    }
+    // vflux(f+1) from w(f-1..f+2) and VF(f+1) = 0.5 (rom(f+1) + rom(f))
+    double vfl_hi = 0.0;
+    const int kk = f + 1;
+    if (kk >= 1 && kk < nz) {
+      const double vf = 0.5 * (rom1 + rom0);
+      if (kk == 1) vfl_hi = (vf * 0.5) * (wp1 + w0);
+      else vfl_hi = vf * flux3(wm1, w0, wp1, wp2, -vf);
+    }
+    if (f >= 1 && f < nz) t = t + (-g.rdn[f]) * (vfl_hi - vfl_lo);
+    else t = t + 0.0;
+    sto(w.wt, (unsigned)f * pl8 + c8, t);
+    wm1 = w0;
+    w0 = wp1;
+    wp1 = wp2;
+    wp2 = wp3;
+    rom0 = rom1;
+    rom1 = rom2;
+    vfl_lo = vfl_hi;
+    if (f < nz) {
+      ru0m = cur.ru0;
+      ru1m = cur.ru1;
+      rv0m = cur.rv0;
+      rv1m = cur.rv1;
+    }
+    cur = nxt;
   }
-  w.wt[idx] = t;
 }
 
-// advect_scalar_flux of theta-300 (flux_advection.py:482-568) on mass levels.
-__global__ void k_tend_t(DGrid g, DState s, DWork w) {
+// advect_u_flux (flux_advection.py:1131-1202) + large_step_horizontal_pgf x
+// (rk_addtend_dry.py:329-343, 357-361) on u faces (nz, ny, nx+1).  Face nx
+// repeats face 0's advection (periodic); the PGF uses edge-padded pairs.
+__global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  size_t n = (size_t)nz * ny * nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, nx, ny, k, j, i);
-  const double* th = s.theta;
-  auto QX = [&](int sft) { return th[im(g, k, j, wrapi(i + sft, nx))] - kT0; };
-  auto QY = [&](int jj, int sft) { return th[im(g, k, wrapi(jj + sft, ny), i)] - kT0; };
-  int ip = wrapi(i + 1, nx), jp = wrapi(j + 1, ny);
-  double vel0 = w.ru[im(g, k, j, i)], vel1 = w.ru[im(g, k, j, ip)];
-  double fq0 = vel0 * flux5(QX(-3), QX(-2), QX(-1), QX(0), QX(1), QX(2), vel0);
-  double fq1 = vel1 * flux5(QX(-2), QX(-1), QX(0), QX(1), QX(2), QX(3), vel1);
-  double msftx = g.msftx[i2(g, j, i)];
-  double t = ((-msftx) * g.rdx) * (fq1 - fq0);
-  double vy0 = w.rv[im(g, k, j, i)], vy1 = w.rv[im(g, k, jp, i)];
-  double gy0 = vy0 * flux5(QY(j, -3), QY(j, -2), QY(j, -1), QY(j, 0), QY(j, 1), QY(j, 2), vy0);
-  double gy1 = vy1 * flux5(QY(jp, -3), QY(jp, -2), QY(jp, -1), QY(jp, 0), QY(jp, 1), QY(jp, 2), vy1);
-  t = t - (msftx * g.rdy) * (gy1 - gy0);
-  auto q = [&](int kk) { return th[im(g, kk, j, i)] - kT0; };
-  auto vf = [&](int kf) -> double {
-    if (kf == nz || kf == 0) return 0.0;
-    if (kf == 1) return w.rom[im(g, 1, j, i)] * (g.fnm[1] * q(1) + g.fnp[1] * q(0));
-    if (kf == nz - 1) return w.rom[im(g, nz - 1, j, i)] * (g.fnm[nz - 1] * q(nz - 1) + g.fnp[nz - 1] * q(nz - 2));
-    double r = w.rom[im(g, kf, j, i)];
-    return r * flux3(q(kf - 2), q(kf - 1), q(kf), q(kf + 1), -r);
+  const size_t nface = (size_t)ny * (nx + 1);
+  const size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (idx >= nface) return;
+  const int j = (int)(idx / (size_t)(nx + 1)), i = (int)(idx % (size_t)(nx + 1));
+  const int ia = (i == nx) ? 0 : i;
+  const int ip = wrapi(ia + 1, nx), imn = wrapi(ia - 1, nx);
+  const unsigned pl8 = (unsigned)((size_t)ny * nx) * 8u, upl8 = (unsigned)nface * 8u;
+  const bool do_y = ny > 1;
+  const int jp = wrapi(j + 1, ny);
+  unsigned ux[7], uy[7];  // u faces ia-3..ia+3 on row j; rows j-3..j+3 at face ia
+#pragma unroll
+  for (int d = 0; d < 7; ++d) {
+    ux[d] = (unsigned)i2u(g, j, wrapi(ia + d - 3, nx)) * 8u;
+    uy[d] = (unsigned)i2u(g, wrapi(j + d - 3, ny), ia) * 8u;
+  }
+  const unsigned ca8 = (unsigned)i2(g, j, ia) * 8u, cim8 = (unsigned)i2(g, j, imn) * 8u;
+  const unsigned cip8 = (unsigned)i2(g, j, ip) * 8u;
+  const unsigned jpa8 = (unsigned)i2(g, jp, ia) * 8u, jpm8 = (unsigned)i2(g, jp, imn) * 8u;
+  const double msf = g.msfux[i2u(g, j, ia)];
+  const unsigned fo8 = (unsigned)i2u(g, j, i) * 8u;  // the output face
+  // PGF columns (edge padded)
+  const int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
+  const size_t cl = i2(g, j, il), cr = i2(g, j, ir);
+  const unsigned cl8 = (unsigned)cl * 8u, cr8 = (unsigned)cr * 8u;
+  const double msf_u = g.msfux[i2u(g, j, i)] / g.msfuy[i2u(g, j, i)];
+  const double mub_l = s.mu_total[cl] - s.mu_pert[cl];
+  const double mub_r = s.mu_total[cr] - s.mu_pert[cr];
+  const double mt_l = mub_l + s.mu_pert[cl];
+  const double mt_r = mub_r + s.mu_pert[cr];
+  const double muu = 0.5 * (mt_l + mt_r);
+  const double mu_l = s.mu_pert[cl], mu_r = s.mu_pert[cr];
+  const double* U = s.u;
+  const double* PP = s.p_pert;
+  struct Lv {
+    double x[7], y[7];                          // u at faces ia-3..ia+3 / rows j-3..j+3
+    double ru_a, ru_m, ru_p, rv_a, rv_m, rv_pa, rv_pm;  // transport pieces
+    double pl1, pr1, phl1, phr1;                // p' and ph' at level k+1 (l, r)
+    double altl, altr, ptl, ptr, all, alr, phpl, phpr;  // level k diagnostics (l, r)
   };
-  t = t - g.rdnw[k] * (vf(k + 1) - vf(k));
-  w.tht[idx] = t;
+  auto load = [&](int k, Lv& L) {
+    const unsigned o = (unsigned)k * pl8, ou = (unsigned)k * upl8;
+#pragma unroll
+    for (int d = 0; d < 7; ++d) L.x[d] = ldo(U, ou + ux[d]);
+    if (do_y) {
+#pragma unroll
+      for (int d = 0; d < 7; ++d)
+        if (d != 3) L.y[d] = ldo(U, ou + uy[d]);
+    }
+    L.y[3] = L.x[3];
+    L.ru_a = ldo(w.ru, o + ca8);
+    L.ru_m = ldo(w.ru, o + cim8);
+    L.ru_p = ldo(w.ru, o + cip8);
+    if (do_y) {
+      L.rv_a = ldo(w.rv, o + ca8);
+      L.rv_m = ldo(w.rv, o + cim8);
+      L.rv_pa = ldo(w.rv, o + jpa8);
+      L.rv_pm = ldo(w.rv, o + jpm8);
+    }
+    if (k + 1 < nz) {
+      L.pl1 = ldo(PP, o + pl8 + cl8);
+      L.pr1 = ldo(PP, o + pl8 + cr8);
+    }
+    L.phl1 = ldo(s.ph_pert, o + pl8 + cl8);
+    L.phr1 = ldo(s.ph_pert, o + pl8 + cr8);
+    L.altl = ldo(w.alt_raw, o + cl8);
+    L.altr = ldo(w.alt_raw, o + cr8);
+    L.ptl = ldo(s.p_total, o + cl8);
+    L.ptr = ldo(s.p_total, o + cr8);
+    L.all = ldo(w.al_ls, o + cl8);
+    L.alr = ldo(w.al_ls, o + cr8);
+    L.phpl = ldo(w.php_ls, o + cl8);
+    L.phpr = ldo(w.php_ls, o + cr8);
+  };
+  // centre-face window of u (k-1..k+3) and the face omega (k+1, k+2)
+  auto romq = [&](int kk) {
+    const unsigned o = (unsigned)kk * pl8;
+    return 0.5 * (ldo(w.rom, o + ca8) + ldo(w.rom, o + cim8));
+  };
+  const unsigned ua8 = ux[3];
+  double qm1 = 0.0, q0 = ldo(U, ua8), qp1 = ldo(U, upl8 + ua8), qp2 = ldo(U, 2 * upl8 + ua8);
+  double r1 = romq(1);
+  double vf_lo = 0.0;
+  // p' window at (l, r): level k (p0), k-1 (pm) ; ph' at level k
+  double pl0 = ldo(PP, cl8), pr0 = ldo(PP, cr8), plm = 0.0, prm = 0.0;
+  double phl0 = ldo(s.ph_pert, cl8), phr0 = ldo(s.ph_pert, cr8);
+  double dpn_k;
+  {
+    const double ps0 = pl0 + pr0;
+    const double ps1 = ldo(PP, pl8 + cl8) + ldo(PP, pl8 + cr8);
+    const double ps2 = ldo(PP, 2 * pl8 + cl8) + ldo(PP, 2 * pl8 + cr8);
+    dpn_k = dpn_ls_bot(g, ps0, ps1, ps2);
+  }
+  Lv cur{};
+  for (int k = 0; k < nz; ++k) {
+    const double qp3 = k + 3 < nz ? ldo(U, (unsigned)(k + 3) * upl8 + ua8) : 0.0;
+    const double r2 = k + 2 <= nz ? romq(k + 2) : 0.0;
+    load(k, cur);
+    // --- advection on the collapsed periodic face ia ---
+    auto UX = [&](int sft) { return cur.x[sft + 3]; };
+    auto UY = [&](int d) { return cur.y[d + 3]; };
+    double velx0 = 0.5 * (cur.ru_a + cur.ru_m);
+    double velx1 = 0.5 * (cur.ru_p + cur.ru_a);
+    double fq0 = velx0 * flux5(UX(-3), UX(-2), UX(-1), UX(0), UX(1), UX(2), velx0);
+    double fq1 = velx1 * flux5(UX(-2), UX(-1), UX(0), UX(1), UX(2), UX(3), velx1);
+    double t = ((-msf) * g.rdx) * (fq1 - fq0);
+    if (do_y) {
+      double vy0 = 0.5 * (cur.rv_a + cur.rv_m);
+      double vy1 = 0.5 * (cur.rv_pa + cur.rv_pm);
+      double g0 = vy0 * flux5(UY(-3), UY(-2), UY(-1), UY(0), UY(1), UY(2), vy0);
+      double g1 = vy1 * flux5(UY(-2), UY(-1), UY(0), UY(1), UY(2), UY(3), vy1);
+      t = t - (msf * g.rdy) * (g1 - g0);
+    }
+    const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, r1);
+    t = t + (-g.rdnw[k]) * (vf_hi - vf_lo);
+    // --- large-step horizontal PGF at face i ---
+    const double pl1 = cur.pl1, pr1 = cur.pr1;
+    double dpn_k1;
+    if (k + 1 == nz) dpn_k1 = dpn_ls_top(g, pl0 + pr0, plm + prm);
+    else dpn_k1 = dpn_ls_mid(g, k + 1, pl1 + pr1, pl0 + pr0);
+    double mass_u = g.c1h[k] * muu + g.c2h[k];
+    double ph_term = (cur.phr1 - cur.phl1) + (phr0 - phl0);
+    double p_term = (cur.altl + cur.altr) * (pr0 - pl0);
+    double pb_l = cur.ptl - pl0;
+    double pb_r = cur.ptr - pr0;
+    double pb_term = (cur.all + cur.alr) * (pb_r - pb_l);
+    double dpx = (((msf_u * 0.5) * g.rdx) * mass_u) * ((ph_term + p_term) + pb_term);
+    double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_l + mu_r));
+    dpx = dpx + ((msf_u * g.rdx) * (cur.phpr - cur.phpl)) * bracket;
+    sto(w.ut, (unsigned)k * upl8 + fo8, t + (-1.0 * dpx));
+    qm1 = q0;
+    q0 = qp1;
+    qp1 = qp2;
+    qp2 = qp3;
+    r1 = r2;
+    vf_lo = vf_hi;
+    plm = pl0;
+    prm = pr0;
+    pl0 = pl1;
+    pr0 = pr1;
+    phl0 = cur.phl1;
+    phr0 = cur.phr1;
+    dpn_k = dpn_k1;
+  }
+}
+
+// advect_v_flux (flux_advection.py:1205-1274) + large_step_horizontal_pgf y
+// (rk_addtend_dry.py:345-361) on v faces (nz, ny+1, nx).  Face ny repeats
+// face 0's advection (periodic); the PGF uses edge-padded pairs.
+__global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t nface = (size_t)(ny + 1) * nx;
+  const size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (idx >= nface) return;
+  const int j = (int)(idx / (size_t)nx), i = (int)(idx % (size_t)nx);
+  const int ja = (j == ny) ? 0 : j;
+  const int jm = wrapi(ja - 1, ny), jp = wrapi(ja + 1, ny), ip = wrapi(i + 1, nx);
+  const unsigned pl8 = (unsigned)((size_t)ny * nx) * 8u, upl8 = (unsigned)nface * 8u;
+  unsigned ux[7], uy[7];  // v faces i-3..i+3 on row ja; rows ja-3..ja+3 at column i
+#pragma unroll
+  for (int d = 0; d < 7; ++d) {
+    ux[d] = (unsigned)i2v(g, ja, wrapi(i + d - 3, nx)) * 8u;
+    uy[d] = (unsigned)i2v(g, wrapi(ja + d - 3, ny), i) * 8u;
+  }
+  const unsigned ca8 = (unsigned)i2(g, ja, i) * 8u, cjm8 = (unsigned)i2(g, jm, i) * 8u;
+  const unsigned cip8 = (unsigned)i2(g, ja, ip) * 8u, cjmp8 = (unsigned)i2(g, jm, ip) * 8u;
+  const unsigned cjp8 = (unsigned)i2(g, jp, i) * 8u;
+  const double msfvy = g.msfvy[i2v(g, ja, i)];
+  const double ratio = g.msfvy[i2v(g, ja, i)] / g.msfvx[i2v(g, ja, i)];
+  const unsigned fo8 = (unsigned)i2v(g, j, i) * 8u;  // the output face
+  // PGF rows (edge padded)
+  const int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
+  const size_t cl = i2(g, js, i), cr = i2(g, jn, i);
+  const unsigned cl8 = (unsigned)cl * 8u, cr8 = (unsigned)cr * 8u;
+  const double msf_v = g.msfvy[i2v(g, j, i)] / g.msfvx[i2v(g, j, i)];
+  const double mub_s = s.mu_total[cl] - s.mu_pert[cl];
+  const double mub_n = s.mu_total[cr] - s.mu_pert[cr];
+  const double mt_s = mub_s + s.mu_pert[cl];
+  const double mt_n = mub_n + s.mu_pert[cr];
+  const double muv = 0.5 * (mt_s + mt_n);
+  const double mu_s = s.mu_pert[cl], mu_n = s.mu_pert[cr];
+  const double* U = s.v;
+  const double* PP = s.p_pert;
+  struct Lv {
+    double x[7], y[7];                          // v at columns i-3..i+3 / rows ja-3..ja+3
+    double ru_a, ru_m, ru_p, ru_mp, rv_a, rv_m, rv_p;   // transport pieces
+    double pl1, pr1, phl1, phr1;                // p' and ph' at level k+1 (l, r)
+    double altl, altr, ptl, ptr, all, alr, phpl, phpr;  // level k diagnostics (l, r)
+  };
+  auto load = [&](int k, Lv& L) {
+    const unsigned o = (unsigned)k * pl8, ou = (unsigned)k * upl8;
+#pragma unroll
+    for (int d = 0; d < 7; ++d) L.x[d] = ldo(U, ou + ux[d]);
+#pragma unroll
+    for (int d = 0; d < 7; ++d)
+      if (d != 3) L.y[d] = ldo(U, ou + uy[d]);
+    L.y[3] = L.x[3];
+    L.ru_a = ldo(w.ru, o + ca8);
+    L.ru_m = ldo(w.ru, o + cjm8);
+    L.ru_p = ldo(w.ru, o + cip8);
+    L.ru_mp = ldo(w.ru, o + cjmp8);
+    L.rv_a = ldo(w.rv, o + ca8);
+    L.rv_m = ldo(w.rv, o + cjm8);
+    L.rv_p = ldo(w.rv, o + cjp8);
+    if (k + 1 < nz) {
+      L.pl1 = ldo(PP, o + pl8 + cl8);
+      L.pr1 = ldo(PP, o + pl8 + cr8);
+    }
+    L.phl1 = ldo(s.ph_pert, o + pl8 + cl8);
+    L.phr1 = ldo(s.ph_pert, o + pl8 + cr8);
+    L.altl = ldo(w.alt_raw, o + cl8);
+    L.altr = ldo(w.alt_raw, o + cr8);
+    L.ptl = ldo(s.p_total, o + cl8);
+    L.ptr = ldo(s.p_total, o + cr8);
+    L.all = ldo(w.al_ls, o + cl8);
+    L.alr = ldo(w.al_ls, o + cr8);
+    L.phpl = ldo(w.php_ls, o + cl8);
+    L.phpr = ldo(w.php_ls, o + cr8);
+  };
+  // centre-face window of v (k-1..k+3) and the face omega (k+1, k+2)
+  auto romq = [&](int kk) {
+    const unsigned o = (unsigned)kk * pl8;
+    return 0.5 * (ldo(w.rom, o + ca8) + ldo(w.rom, o + cjm8));
+  };
+  const unsigned ua8 = ux[3];
+  double qm1 = 0.0, q0 = ldo(U, ua8), qp1 = ldo(U, upl8 + ua8), qp2 = ldo(U, 2 * upl8 + ua8);
+  double r1 = romq(1);
+  double vf_lo = 0.0;
+  // p' window at (l, r): level k (p0), k-1 (pm) ; ph' at level k
+  double pl0 = ldo(PP, cl8), pr0 = ldo(PP, cr8), plm = 0.0, prm = 0.0;
+  double phl0 = ldo(s.ph_pert, cl8), phr0 = ldo(s.ph_pert, cr8);
+  double dpn_k;
+  {
+    const double ps0 = pl0 + pr0;
+    const double ps1 = ldo(PP, pl8 + cl8) + ldo(PP, pl8 + cr8);
+    const double ps2 = ldo(PP, 2 * pl8 + cl8) + ldo(PP, 2 * pl8 + cr8);
+    dpn_k = dpn_ls_bot(g, ps0, ps1, ps2);
+  }
+  Lv cur{};
+  for (int k = 0; k < nz; ++k) {
+    const double qp3 = k + 3 < nz ? ldo(U, (unsigned)(k + 3) * upl8 + ua8) : 0.0;
+    const double r2 = k + 2 <= nz ? romq(k + 2) : 0.0;
+    load(k, cur);
+    // --- advection on the collapsed periodic face ja ---
+    auto VX = [&](int sft) { return cur.x[sft + 3]; };
+    auto VY = [&](int d) { return cur.y[d + 3]; };
+    double vx0 = 0.5 * (cur.ru_a + cur.ru_m);
+    double vx1 = 0.5 * (cur.ru_p + cur.ru_mp);
+    double fq0 = vx0 * flux5(VX(-3), VX(-2), VX(-1), VX(0), VX(1), VX(2), vx0);
+    double fq1 = vx1 * flux5(VX(-2), VX(-1), VX(0), VX(1), VX(2), VX(3), vx1);
+    double t = ((-msfvy) * g.rdx) * (fq1 - fq0);
+    double vy0 = 0.5 * (cur.rv_a + cur.rv_m);
+    double vy1 = 0.5 * (cur.rv_p + cur.rv_a);
+    double g0 = vy0 * flux5(VY(-3), VY(-2), VY(-1), VY(0), VY(1), VY(2), vy0);
+    double g1 = vy1 * flux5(VY(-2), VY(-1), VY(0), VY(1), VY(2), VY(3), vy1);
+    t = t - (msfvy * g.rdy) * (g1 - g0);
+    const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, r1);
+    double vd = (-g.rdnw[k]) * (vf_hi - vf_lo);
+    t = t + ratio * vd;
+    // --- large-step PGF at face j (pairs south = l, north = r) ---
+    const double pl1 = cur.pl1, pr1 = cur.pr1;
+    double dpn_k1;
+    if (k + 1 == nz) dpn_k1 = dpn_ls_top(g, pl0 + pr0, plm + prm);
+    else dpn_k1 = dpn_ls_mid(g, k + 1, pl1 + pr1, pl0 + pr0);
+    double mass_v = g.c1h[k] * muv + g.c2h[k];
+    double ph_term = (cur.phr1 - cur.phl1) + (phr0 - phl0);
+    double p_term = (cur.altl + cur.altr) * (pr0 - pl0);
+    double pb_s = cur.ptl - pl0;
+    double pb_n = cur.ptr - pr0;
+    double pb_term = (cur.all + cur.alr) * (pb_n - pb_s);
+    double dpy = (((msf_v * 0.5) * g.rdy) * mass_v) * ((ph_term + p_term) + pb_term);
+    double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_s + mu_n));
+    dpy = dpy + ((msf_v * g.rdy) * (cur.phpr - cur.phpl)) * bracket;
+    sto(w.vt, (unsigned)k * upl8 + fo8, t + (-1.0 * dpy));
+    qm1 = q0;
+    q0 = qp1;
+    qp1 = qp2;
+    qp2 = qp3;
+    r1 = r2;
+    vf_lo = vf_hi;
+    plm = pl0;
+    prm = pr0;
+    pl0 = pl1;
+    pr0 = pr1;
+    phl0 = cur.phl1;
+    phr0 = cur.phr1;
+    dpn_k = dpn_k1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -328,7 +556,7 @@ __global__ void k_tend_t(DGrid g, DState s, DWork w) {
 // ---------------------------------------------------------------------------
 __global__ void k_prep_2d(DGrid g, DState s, DWork w, int rk_step) {
   size_t n = (size_t)g.ny * g.nx;
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   double mub = s.mu_total[c] - s.mu_pert[c];
   double mu_cur = s.mu_pert[c];
@@ -349,7 +577,7 @@ __global__ void k_prep_2d(DGrid g, DState s, DWork w, int rk_step) {
 __global__ void k_prep_faces(DGrid g, DWork w) {
   const int nx = g.nx, ny = g.ny;
   size_t nu = (size_t)ny * (nx + 1), nv = (size_t)(ny + 1) * nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
   if (idx < nu) {
     int i = (int)(idx % (size_t)(nx + 1)), j = (int)(idx / (size_t)(nx + 1));
     double a, b;
@@ -386,7 +614,7 @@ __global__ void k_prep_faces(DGrid g, DWork w) {
 
 __global__ void k_prep_u(DGrid g, DState s, DWork w) {
   size_t n = (size_t)g.nz * g.ny * (g.nx + 1);
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   int k, j, i;
   decompose(idx, g.nx + 1, g.ny, k, j, i);
@@ -398,7 +626,7 @@ __global__ void k_prep_u(DGrid g, DState s, DWork w) {
 
 __global__ void k_prep_v(DGrid g, DState s, DWork w) {
   size_t n = (size_t)g.nz * (g.ny + 1) * g.nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   int k, j, i;
   decompose(idx, g.nx, g.ny + 1, k, j, i);
@@ -408,195 +636,312 @@ __global__ void k_prep_v(DGrid g, DState s, DWork w) {
   w.v_work[idx] = (mref * w.v0[idx] - mcur * s.v[idx]) / g.msfvx[f];
 }
 
-// w-level prep: w_work, ph_work, phb; the acoustic omega starts at ww_save.
-__global__ void k_prep_w(DGrid g, DState s, DWork w) {
-  size_t n = (size_t)(g.nz + 1) * g.ny * g.nx;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, g.nx, g.ny, k, j, i);
-  size_t c = i2(g, j, i);
-  double mref = g.c1f[k] * w.muts0[c] + g.c2f[k];
-  double mcur = g.c1f[k] * w.mut[c] + g.c2f[k];
-  w.w_work[idx] = (mref * w.w0[idx] - mcur * s.w[idx]) / g.msfty[c];
-  w.ph_work[idx] = w.ph0[idx] - s.ph_pert[idx];
-  w.phb[idx] = s.ph_total[idx] - s.ph_pert[idx];
-  w.ww[idx] = w.rom[idx];
-}
-
-// mass-level prep + calc_p_rho_wrf step 0 (core/calc_p_rho.py:48-139).
-__global__ void k_prep_m(DGrid g, DState s, DWork w) {
-  size_t n = (size_t)g.nz * g.ny * g.nx;
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  int k, j, i;
-  decompose(q, g.nx, g.ny, k, j, i);
-  size_t c = i2(g, j, i), q1 = im(g, k + 1, j, i);
-  double mref = g.c1h[k] * w.muts0[c] + g.c2h[k];
-  double mcur = g.c1h[k] * w.mut[c] + g.c2h[k];
-  double th_ref = w.th0[q] - kT0, th_cur = s.theta[q] - kT0;
-  double th_work = mref * th_ref - mcur * th_cur;
-  w.th_work[q] = th_work;
-  double pb = s.p_total[q] - s.p_pert[q];
-  w.pb[q] = pb;
-  double alt_raw = w.alt_raw[q];
-  double alt_ac = alt_raw > kMinAcousticAlt ? alt_raw : kMinAcousticAlt;
-  w.alt[q] = alt_ac;
-  double c2a = (kCpovcv * (pb + s.p_pert[q])) / alt_ac;
-  w.c2a[q] = c2a;
-  double phb_k = s.ph_total[q] - s.ph_pert[q];
-  double phb_k1 = s.ph_total[q1] - s.ph_pert[q1];
-  w.php[q] = 0.5 * (((phb_k + phb_k1) + s.ph_pert[q]) + s.ph_pert[q1]);
-  // calc_p_rho step 0
-  double mass_h = g.c1h[k] * w.muts0[c] + g.c2h[k];
-  double safe_mass = safe_pos(mass_h, 1.0e-12);
-  double mu_term = g.c1h[k] * w.mu_work0[c];
-  double phw_k = w.ph0[q] - s.ph_pert[q];
-  double phw_k1 = w.ph0[q1] - s.ph_pert[q1];
-  double al = -(alt_ac * mu_term + g.rdnw[k] * (phw_k1 - phw_k)) / safe_mass;
-  double ttr = kT0 + th_ref;
-  double safe_t = safe_pos(ttr, 1.0e-6);
-  double p = c2a * (((alt_ac * (th_work - mu_term * th_ref)) / (safe_mass * safe_t)) - al);
-  w.al[q] = al;
-  w.p[q] = p;
-  w.pm1[q] = p;
-  w.t2ave[q] = 0.0;
+// ---------------------------------------------------------------------------
+// Column prep, one thread per column marching up the w levels f = 0..nz; the
+// mass level k = f - 1 is done once level f is loaded, so every state field is
+// read once and the ph values of level k+1 come from registers:
+//   w level f:   small_step_prep w_work, ph_work, ph_1 + phb (ph1t) and the
+//                carried omega ends (core/small_step_prep.py:216-394)
+//   mass level k: stage-entry EOS alt and _absolute_diagnostics al, php
+//                (core/rk_addtend_dry.py:127-229, hypsometric_opt=1;
+//                acoustic_wrf.py:354-358), small_step_prep th_work, pb, alt,
+//                c2a, php and calc_p_rho_wrf step 0 al, p
+//                (core/calc_p_rho.py:48-139)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
+  const size_t ncol = (size_t)g.ny * g.nx;
+  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int nz = g.nz;
+  const unsigned pl8 = (unsigned)ncol * 8u, c8 = (unsigned)c * 8u;
+  const double msfty = g.msfty[c];
+  const double muts0 = w.muts0[c], mut = w.mut[c], mu_work0 = w.mu_work0[c];
+  const double mu_pert = s.mu_pert[c], mu_total = s.mu_total[c];
+  const double mub_d = mu_total - mu_pert;
+  const double muts_d = mu_total + mu_pert;
+  struct Lv {
+    double w0, wc, ph0, php, pht;   // w level f
+    double th, ptot, ppert, th0;    // mass level f
+  };
+  auto load = [&](int f, Lv& L) {
+    const unsigned q = c8 + (unsigned)f * pl8;
+    L.w0 = ldo(w.w0, q);
+    L.wc = ldo(s.w, q);
+    L.ph0 = ldo(w.ph0, q);
+    L.php = ldo(s.ph_pert, q);
+    L.pht = ldo(s.ph_total, q);
+    if (f < nz) {
+      L.th = ldo(s.theta, q);
+      L.ptot = ldo(s.p_total, q);
+      L.ppert = ldo(s.p_pert, q);
+      L.th0 = ldo(w.th0, q);
+    }
+  };
+  Lv cur{}, prev{};
+  load(0, cur);
+  for (int f = 0; f <= nz; ++f) {
+    Lv nxt{};
+    if (f < nz) load(f + 1, nxt);
+    const unsigned q = c8 + (unsigned)f * pl8;
+    {  // w level f
+      double mref = g.c1f[f] * muts0 + g.c2f[f];
+      double mcur = g.c1f[f] * mut + g.c2f[f];
+      sto(w.w_work, q, (mref * cur.w0 - mcur * cur.wc) / msfty);
+      sto(w.ph_work, q, cur.ph0 - cur.php);
+      const double phb = cur.pht - cur.php;
+      sto(w.ph1t, q, cur.ph0 + phb);
+      if (f == 0 || f == nz) sto(w.ww, q, ldo(w.rom, q));
+    }
+    if (f >= 1) {  // mass level k = f - 1 (level k in prev, level k+1 in cur)
+      const int k = f - 1;
+      const unsigned qk = q - pl8;
+      // stage-entry EOS and _absolute_diagnostics
+      const double alt_raw = alt_from_theta_p(prev.th, prev.ptot);
+      sto(w.alt_raw, qk, alt_raw);
+      {
+        double phb_k = prev.pht - prev.php;
+        double phb_k1 = cur.pht - cur.php;
+        double mass_base = g.c1h[k] * mub_d + g.c2h[k];
+        double safe_base = safe_pos(mass_base, 1.0e-12);
+        double alb = ((-g.rdnw[k]) * (phb_k1 - phb_k)) / safe_base;
+        double mass_t = g.c1h[k] * muts_d + g.c2h[k];
+        double safe_t = safe_pos(mass_t, 1.0e-12);
+        double mu_term = g.c1h[k] * mu_pert;
+        sto(w.al_ls, qk, -(alb * mu_term + g.rdnw[k] * (cur.php - prev.php)) / safe_t);
+        double pht_k = phb_k + prev.php;
+        double pht_k1 = phb_k1 + cur.php;
+        sto(w.php_ls, qk, 0.5 * (pht_k + pht_k1));
+      }
+      // small_step_prep (mass level) and calc_p_rho step 0
+      double mref = g.c1h[k] * muts0 + g.c2h[k];
+      double mcur = g.c1h[k] * mut + g.c2h[k];
+      double th_ref = prev.th0 - kT0, th_cur = prev.th - kT0;
+      double th_work = mref * th_ref - mcur * th_cur;
+      sto(w.th_work, qk, th_work);
+      double pb = prev.ptot - prev.ppert;
+      sto(w.pb, qk, pb);
+      double alt_ac = alt_raw > kMinAcousticAlt ? alt_raw : kMinAcousticAlt;
+      sto(w.alt, qk, alt_ac);
+      double c2a = (kCpovcv * (pb + prev.ppert)) / alt_ac;
+      sto(w.c2a, qk, c2a);
+      double phb_k = prev.pht - prev.php;
+      double phb_k1 = cur.pht - cur.php;
+      sto(w.php, qk, 0.5 * (((phb_k + phb_k1) + prev.php) + cur.php));
+      double mass_h = g.c1h[k] * muts0 + g.c2h[k];
+      double safe_mass = safe_pos(mass_h, 1.0e-12);
+      double mu_term = g.c1h[k] * mu_work0;
+      double phw_k = prev.ph0 - prev.php;
+      double phw_k1 = cur.ph0 - cur.php;
+      double al = -(alt_ac * mu_term + g.rdnw[k] * (phw_k1 - phw_k)) / safe_mass;
+      double ttr = kT0 + th_ref;
+      double safe_t = safe_pos(ttr, 1.0e-6);
+      double p = c2a * (((alt_ac * (th_work - mu_term * th_ref)) / (safe_mass * safe_t)) - al);
+      sto(w.al, qk, al);
+      sto(w.p, qk, p);
+      // pm1 = p and t_2ave = 0 (calc_p_rho_wrf step 0, small_step_prep) are
+      // not stored: the stage's first substep reads p for pm1 and starts
+      // t_2ave at 0.
+    }
+    prev = cur;
+    cur = nxt;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Stage column setup (one thread per column):
+// Stage column setup, one thread per column marching up the faces:
 //  rw_tend = pg_buoy_w_moist(qtot=0) + advect_w + curvature
 //    (core/advance_w.py:209-290; operational_mode.py:1730-1822)
 //  ph_tend = rhs_ph_wrf order 2 periodic, rigid lid (core/rhs_ph.py:175-398)
 //  a/alpha/gamma = calc_coef_w_wrf_coefficients (dynamics/acoustic_wrf.py:783-852)
+// Level f-1's coupled face velocities and ph_1 totals come from registers;
+// level f+1's loads are issued before level f is computed.
 // ---------------------------------------------------------------------------
-__global__ void k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
+__global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y;
-  if (i >= nx) return;
-  size_t c = i2(g, j, i);
+  const size_t ncol = (size_t)ny * nx;
+  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
+  const unsigned pl8 = (unsigned)ncol * 8u;
+  const unsigned upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u, vpl8 = (unsigned)((size_t)(ny + 1) * nx) * 8u;
+  const unsigned c8 = (unsigned)c * 8u;
+  const unsigned uw8 = (unsigned)i2u(g, j, i) * 8u, ue8 = (unsigned)i2u(g, j, i + 1) * 8u;
+  const unsigned vs8 = (unsigned)i2v(g, j, i) * 8u, vn8 = (unsigned)i2v(g, j + 1, i) * 8u;
+  const int jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny), ip = wrapi(i + 1, nx), imn = wrapi(i - 1, nx);
+  const unsigned cjp8 = (unsigned)i2(g, jp, i) * 8u, cjm8 = (unsigned)i2(g, jm, i) * 8u;
+  const unsigned cip8 = (unsigned)i2(g, j, ip) * 8u, cim8 = (unsigned)i2(g, j, imn) * 8u;
   const double msfty = g.msfty[c];
   const double msft_inv = 1.0 / msfty;
   const double mut = w.mut[c], mub = w.mub[c];
   const double mu_p = mut - mub;
-  // ---- rw_tend ----
-  w.rw_tend[im(g, 0, j, i)] = 0.0 + w.wt[im(g, 0, j, i)];
-  {
-    double mxy = g.msftx[c] / msfty;
-    for (int k = 1; k < nz; ++k) {
-      double cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
+  const double mxy = g.msftx[c] / msfty;
+  const double muu_w = w.muu[i2u(g, j, i)], muu_e = w.muu[i2u(g, j, i + 1)];
+  const double msfuy_w = g.msfuy[i2u(g, j, i)], msfuy_e = g.msfuy[i2u(g, j, i + 1)];
+  const double muv_s = w.muv[i2v(g, j, i)], muv_n = w.muv[i2v(g, j + 1, i)];
+  const double msfvx_s = g.msfvx[i2v(g, j, i)], msfvx_n = g.msfvx[i2v(g, j + 1, i)];
+  const double cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
+  double cof = (0.5 * dts * kG * (1.0 + nl.epssm));
+  cof = cof * cof;
+  const double lid_flag = nl.top_lid ? 0.0 : 1.0;
+  auto MH = [&](int k) { return g.c1h[k] * mut + g.c2h[k]; };
+  auto MF = [&](int k) { return g.c1f[k] * mut + g.c2f[k]; };
+  auto PHT = [&](double pert, double total) { return pert + (total - pert); };
+  // curvature pieces at mass level kk: ru, u, rv, v averaged to the column
+  struct Cv {
+    double ru_m, u_m, rv_m, v_m;
+  };
+  auto curv_at = [&](int kk, double uw, double ue, double vs, double vn) {
+    Cv r;
+    double mu_l = g.c1h[kk] * muu_w + g.c2h[kk];
+    double mu_r = g.c1h[kk] * muu_e + g.c2h[kk];
+    double ru_l = (mu_l * uw) / msfuy_w;
+    double ru_r = (mu_r * ue) / msfuy_e;
+    r.ru_m = 0.5 * (ru_l + ru_r);
+    r.u_m = 0.5 * (uw + ue);
+    double mv_s = g.c1h[kk] * muv_s + g.c2h[kk];
+    double mv_n = g.c1h[kk] * muv_n + g.c2h[kk];
+    double rv_s = (mv_s * vs) / msfvx_s;
+    double rv_n = (mv_n * vn) / msfvx_n;
+    r.rv_m = 0.5 * (rv_s + rv_n);
+    r.v_m = 0.5 * (vs + vn);
+    return r;
+  };
+  struct Lv {
+    double pp, wt, uw, ue, vs, vn, romp, phpp, phtp, ww, c2a;
+    double pnp, pnt, psp, pst, pep, pet, pwp, pwt;  // ph' and ph total at f on rows j+-1, columns i+-1
+  };
+  auto load = [&](int f, Lv& L) {
+    const unsigned o = (unsigned)f * pl8, ou = (unsigned)f * upl8, ov = (unsigned)f * vpl8;
+    L.pp = ldo(s.p_pert, o + c8);
+    L.wt = ldo(w.wt, o + c8);
+    L.uw = ldo(s.u, ou + uw8);
+    L.ue = ldo(s.u, ou + ue8);
+    L.vs = ldo(s.v, ov + vs8);
+    L.vn = ldo(s.v, ov + vn8);
+    L.romp = ldo(w.rom, o + pl8 + c8);
+    L.phpp = ldo(s.ph_pert, o + pl8 + c8);
+    L.phtp = ldo(s.ph_total, o + pl8 + c8);
+    L.ww = ldo(s.w, o + c8);
+    L.c2a = ldo(w.c2a, o + c8);
+    L.pnp = ldo(s.ph_pert, o + cjp8);
+    L.pnt = ldo(s.ph_total, o + cjp8);
+    L.psp = ldo(s.ph_pert, o + cjm8);
+    L.pst = ldo(s.ph_total, o + cjm8);
+    L.pep = ldo(s.ph_pert, o + cip8);
+    L.pet = ldo(s.ph_total, o + cip8);
+    L.pwp = ldo(s.ph_pert, o + cim8);
+    L.pwt = ldo(s.ph_total, o + cim8);
+  };
+  // level 0
+  double p_m = ldo(s.p_pert, c8);  // p' at f-1
+  double uw_m = ldo(s.u, uw8), ue_m = ldo(s.u, ue8), vs_m = ldo(s.v, vs8), vn_m = ldo(s.v, vn8);
+  Cv cv_m = curv_at(0, uw_m, ue_m, vs_m, vn_m);
+  double rom_m = ldo(w.rom, c8), rom_0 = ldo(w.rom, pl8 + c8);                 // rom at f-1, f
+  double pht_m = PHT(ldo(s.ph_pert, c8), ldo(s.ph_total, c8));                // ph_1 total at f-1
+  double pht_0 = PHT(ldo(s.ph_pert, pl8 + c8), ldo(s.ph_total, pl8 + c8));    // at f
+  double c2a_m = ldo(w.c2a, c8);
+  sto(w.rw_tend, c8, 0.0 + ldo(w.wt, c8));
+  sto(w.ph_tend, c8, 0.0);
+  sto(w.ca, c8, 0.0);
+  sto(w.calpha, c8, 1.0);
+  sto(w.cgamma, c8, 0.0);
+  double gamma_prev = 0.0;
+  Lv cur{};
+  load(1, cur);
+  for (int f = 1; f < nz; ++f) {
+    Lv nxt{};
+    if (f + 1 < nz) load(f + 1, nxt);
+    const unsigned q = c8 + (unsigned)f * pl8;
+    // ---- rw_tend ----
+    {
       double rw = (msft_inv * kG) *
-                  (((cq1 * g.rdn[k]) * (s.p_pert[im(g, k, j, i)] - s.p_pert[im(g, k - 1, j, i)]) -
-                    g.c1f[k] * mu_p) -
-                   cq2 * (g.c1f[k] * mub + g.c2f[k]));
-      rw = rw + w.wt[im(g, k, j, i)];
-      double ru_m[2], u_m[2], rv_m[2], v_m[2];
-      for (int sft = 0; sft < 2; ++sft) {
-        int kk = k - sft;
-        double mu_l = g.c1h[kk] * w.muu[i2u(g, j, i)] + g.c2h[kk];
-        double mu_r = g.c1h[kk] * w.muu[i2u(g, j, i + 1)] + g.c2h[kk];
-        double ru_l = (mu_l * s.u[iu(g, kk, j, i)]) / g.msfuy[i2u(g, j, i)];
-        double ru_r = (mu_r * s.u[iu(g, kk, j, i + 1)]) / g.msfuy[i2u(g, j, i + 1)];
-        ru_m[sft] = 0.5 * (ru_l + ru_r);
-        u_m[sft] = 0.5 * (s.u[iu(g, kk, j, i)] + s.u[iu(g, kk, j, i + 1)]);
-        double mv_s = g.c1h[kk] * w.muv[i2v(g, j, i)] + g.c2h[kk];
-        double mv_n = g.c1h[kk] * w.muv[i2v(g, j + 1, i)] + g.c2h[kk];
-        double rv_s = (mv_s * s.v[iv(g, kk, j, i)]) / g.msfvx[i2v(g, j, i)];
-        double rv_n = (mv_n * s.v[iv(g, kk, j + 1, i)]) / g.msfvx[i2v(g, j + 1, i)];
-        rv_m[sft] = 0.5 * (rv_s + rv_n);
-        v_m[sft] = 0.5 * (s.v[iv(g, kk, j, i)] + s.v[iv(g, kk, j + 1, i)]);
-      }
-      double ru_f = g.fnm[k] * ru_m[0] + g.fnp[k] * ru_m[1];
-      double u_f = g.fnm[k] * u_m[0] + g.fnp[k] * u_m[1];
-      double rv_f = g.fnm[k] * rv_m[0] + g.fnp[k] * rv_m[1];
-      double v_f = g.fnm[k] * v_m[0] + g.fnp[k] * v_m[1];
+                  (((cq1 * g.rdn[f]) * (cur.pp - p_m) - g.c1f[f] * mu_p) - cq2 * (g.c1f[f] * mub + g.c2f[f]));
+      rw = rw + cur.wt;
+      const Cv cv0 = curv_at(f, cur.uw, cur.ue, cur.vs, cur.vn);
+      double ru_f = g.fnm[f] * cv0.ru_m + g.fnp[f] * cv_m.ru_m;
+      double u_f = g.fnm[f] * cv0.u_m + g.fnp[f] * cv_m.u_m;
+      double rv_f = g.fnm[f] * cv0.rv_m + g.fnp[f] * cv_m.rv_m;
+      double v_f = g.fnm[f] * cv0.v_m + g.fnp[f] * cv_m.v_m;
       double curv = kReradius * (ru_f * u_f + (mxy * rv_f) * v_f);
-      w.rw_tend[im(g, k, j, i)] = rw + (0.0 + curv);
+      sto(w.rw_tend, q, rw + (0.0 + curv));
+      cv_m = cv0;
     }
-    double cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
-    double rwt = (msft_inv * kG) *
-                 (((((cq1 * 2.0) * g.rdnw[nz - 1]) * (-s.p_pert[im(g, nz - 1, j, i)])) - g.c1f[nz] * mu_p) -
-                  cq2 * (g.c1f[nz] * mub + g.c2f[nz]));
-    w.rw_tend[im(g, nz, j, i)] = rwt + w.wt[im(g, nz, j, i)];
-  }
-  // ---- ph_tend (rhs_ph) ----
-  {
-    int jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny), ip = wrapi(i + 1, nx), imn = wrapi(i - 1, nx);
-    auto PHT = [&](int k, int jj, int ii) { return s.ph_pert[im(g, k, jj, ii)] + w.phb[im(g, k, jj, ii)]; };
-    w.ph_tend[im(g, 0, j, i)] = 0.0;
-    for (int f = 1; f < nz; ++f) {
-      double wd_f = ((0.5 * (w.rom[im(g, f + 1, j, i)] + w.rom[im(g, f, j, i)])) * g.rdnw[f]) *
-                    (PHT(f + 1, j, i) - PHT(f, j, i));
-      double wd_fm = ((0.5 * (w.rom[im(g, f, j, i)] + w.rom[im(g, f - 1, j, i)])) * g.rdnw[f - 1]) *
-                     (PHT(f, j, i) - PHT(f - 1, j, i));
+    // ---- ph_tend (rhs_ph) ----
+    const double pht_p = PHT(cur.phpp, cur.phtp);  // ph_1 total at f+1
+    {
+      double wd_f = ((0.5 * (cur.romp + rom_0)) * g.rdnw[f]) * (pht_p - pht_0);
+      double wd_fm = ((0.5 * (rom_0 + rom_m)) * g.rdnw[f - 1]) * (pht_0 - pht_m);
       double term3 = g.fnm[f] * wd_f + g.fnp[f] * wd_fm;
       double t = -term3;
       double mass_f = g.c1f[f] * mut + g.c2f[f];
-      double gw = ((mass_f * kG) * s.w[im(g, f, j, i)]) * msft_inv;
+      double gw = ((mass_f * kG) * cur.ww) * msft_inv;
       t = t + gw;
-      double pti = PHT(f, j, i);
-      double dn_ = PHT(f, jp, i) - pti, ds_ = pti - PHT(f, jm, i);
-      double vn = s.v[iv(g, f, j + 1, i)] + s.v[iv(g, f - 1, j + 1, i)];
-      double vs = s.v[iv(g, f, j, i)] + s.v[iv(g, f - 1, j, i)];
-      double mvn = g.c1f[f] * w.muv[i2v(g, j + 1, i)] + g.c2f[f];
-      double mvs = g.c1f[f] * w.muv[i2v(g, j, i)] + g.c2f[f];
+      double pti = pht_0;
+      double dn_ = PHT(cur.pnp, cur.pnt) - pti, ds_ = pti - PHT(cur.psp, cur.pst);
+      double vn = cur.vn + vn_m;
+      double vs = cur.vs + vs_m;
+      double mvn = g.c1f[f] * muv_n + g.c2f[f];
+      double mvs = g.c1f[f] * muv_s + g.c2f[f];
       double fy = (mvn * vn) * dn_ + (mvs * vs) * ds_;
       t = t + (-((0.25 * g.rdy) * msft_inv)) * fy;
-      double de_ = PHT(f, j, ip) - pti, dw_ = pti - PHT(f, j, imn);
-      double ue = s.u[iu(g, f, j, i + 1)] + s.u[iu(g, f - 1, j, i + 1)];
-      double uw = s.u[iu(g, f, j, i)] + s.u[iu(g, f - 1, j, i)];
-      double mue = g.c1f[f] * w.muu[i2u(g, j, i + 1)] + g.c2f[f];
-      double muw = g.c1f[f] * w.muu[i2u(g, j, i)] + g.c2f[f];
+      double de_ = PHT(cur.pep, cur.pet) - pti, dw_ = pti - PHT(cur.pwp, cur.pwt);
+      double ue = cur.ue + ue_m;
+      double uw = cur.uw + uw_m;
+      double mue = g.c1f[f] * muu_e + g.c2f[f];
+      double muw = g.c1f[f] * muu_w + g.c2f[f];
       double fx = (mue * ue) * de_ + (muw * uw) * dw_;
       t = t + (-((0.25 * g.rdx) * msft_inv)) * fx;
-      w.ph_tend[im(g, f, j, i)] = t;
+      sto(w.ph_tend, q, t);
     }
-    w.ph_tend[im(g, nz, j, i)] = 0.0;
-  }
-  // ---- calc_coef_w ----
-  {
-    double cof = (0.5 * dts * kG * (1.0 + nl.epssm));
-    cof = cof * cof;
-    double lid_flag = nl.top_lid ? 0.0 : 1.0;
-    auto MH = [&](int k) { return g.c1h[k] * mut + g.c2h[k]; };
-    auto MF = [&](int k) { return g.c1f[k] * mut + g.c2f[k]; };
-    auto C2A = [&](int k) { return w.c2a[im(g, k, j, i)]; };
-    double top_denom_a = MH(nz - 1) * MF(nz - 1);
-    double top_denom_b = MH(nz - 1) * MF(nz);
-    double r2 = g.rdnw[nz - 1] * g.rdnw[nz - 1];
-    double a_top = ((((-2.0 * cof) * r2) * C2A(nz - 1)) * lid_flag) / top_denom_a;
-    w.ca[im(g, 0, j, i)] = 0.0;
-    w.calpha[im(g, 0, j, i)] = 1.0;
-    w.cgamma[im(g, 0, j, i)] = 0.0;
-    double gamma_prev = 0.0;
-    for (int k = 1; k < nz; ++k) {
+    // ---- calc_coef_w ----
+    {
+      const int k = f;
       double ak;
       if (k == 1) {
         ak = 0.0;
       } else {
         int kb = k - 1;
         double denom = MH(kb) * MF(kb);
-        ak = (((((-cqw_at(k, nz)) * cof) * g.rdn[k]) * g.rdnw[k - 1]) * C2A(k - 1)) / denom;
+        ak = (((((-cqw_at(k, nz)) * cof) * g.rdn[k]) * g.rdnw[k - 1]) * c2a_m) / denom;
       }
       double du = MH(k) * MF(k);
       double dl = MH(k - 1) * MF(k);
       double dc = MH(k) * MF(k + 1);
-      double b = 1.0 + ((cqw_at(k, nz) * cof) * g.rdn[k]) *
-                           ((g.rdnw[k] * C2A(k)) / du + (g.rdnw[k - 1] * C2A(k - 1)) / dl);
-      double cc = (((((-cqw_at(k, nz)) * cof) * g.rdn[k]) * g.rdnw[k]) * C2A(k)) / dc;
+      double b = 1.0 + ((cqw_at(k, nz) * cof) * g.rdn[k]) * ((g.rdnw[k] * cur.c2a) / du + (g.rdnw[k - 1] * c2a_m) / dl);
+      double cc = (((((-cqw_at(k, nz)) * cof) * g.rdn[k]) * g.rdnw[k]) * cur.c2a) / dc;
       double al = 1.0 / (b - ak * gamma_prev);
       double gm = cc * al;
-      w.ca[im(g, k, j, i)] = ak;
-      w.calpha[im(g, k, j, i)] = al;
-      w.cgamma[im(g, k, j, i)] = gm;
+      sto(w.ca, q, ak);
+      sto(w.calpha, q, al);
+      sto(w.cgamma, q, gm);
       gamma_prev = gm;
     }
-    double b_top = 1.0 + (((2.0 * cof) * r2) * C2A(nz - 1)) / top_denom_b;
-    w.ca[im(g, nz, j, i)] = a_top;
-    w.calpha[im(g, nz, j, i)] = 1.0 / (b_top - a_top * gamma_prev);
-    w.cgamma[im(g, nz, j, i)] = 0.0;
+    p_m = cur.pp;
+    uw_m = cur.uw;
+    ue_m = cur.ue;
+    vs_m = cur.vs;
+    vn_m = cur.vn;
+    rom_m = rom_0;
+    rom_0 = cur.romp;
+    pht_m = pht_0;
+    pht_0 = pht_p;
+    c2a_m = cur.c2a;
+    cur = nxt;
   }
+  // top face nz (p_m, c2a_m now hold level nz-1)
+  const unsigned qt = c8 + (unsigned)nz * pl8;
+  double rwt = (msft_inv * kG) *
+               (((((cq1 * 2.0) * g.rdnw[nz - 1]) * (-p_m)) - g.c1f[nz] * mu_p) - cq2 * (g.c1f[nz] * mub + g.c2f[nz]));
+  sto(w.rw_tend, qt, rwt + ldo(w.wt, qt));
+  sto(w.ph_tend, qt, 0.0);
+  double top_denom_a = MH(nz - 1) * MF(nz - 1);
+  double top_denom_b = MH(nz - 1) * MF(nz);
+  double r2 = g.rdnw[nz - 1] * g.rdnw[nz - 1];
+  double a_top = ((((-2.0 * cof) * r2) * c2a_m) * lid_flag) / top_denom_a;
+  double b_top = 1.0 + (((2.0 * cof) * r2) * c2a_m) / top_denom_b;
+  sto(w.ca, qt, a_top);
+  sto(w.calpha, qt, 1.0 / (b_top - a_top * gamma_prev));
+  sto(w.cgamma, qt, 0.0);
 }
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -607,26 +952,25 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
                               int rk_step, double dts, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
   const size_t M = (size_t)g.nz * g.ny * g.nx;
-  const size_t Wn = (size_t)(g.nz + 1) * g.ny * g.nx;
   const size_t U = (size_t)g.nz * g.ny * (g.nx + 1);
   const size_t V = (size_t)g.nz * (g.ny + 1) * g.nx;
   const size_t C = (size_t)g.ny * g.nx;
   const size_t F = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;
-  dim3 colgrid((g.nx + 63) / 64, g.ny);
+  const unsigned colgrid = (unsigned)((g.nx + 63) / 64) * (unsigned)g.ny;
   WD_TIMED(prof, KID_RUV, st, (k_ruv<<<blocks_for(M, bs), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
   WD_TIMED(prof, KID_ROM, st, (k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom)));
-  WD_TIMED(prof, KID_DIAG, st, (k_diag<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_PREP_2D, st, (k_prep_2d<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w, rk_step)));
+  // k_prep_col writes the large-step diagnostics the tendency kernels read
+  WD_TIMED(prof, KID_PREP_COL, st, (k_prep_col<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+  const size_t CU = (size_t)g.ny * (g.nx + 1), CV = (size_t)(g.ny + 1) * g.nx;
+  WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<<<blocks_for(CU, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<<<blocks_for(CV, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
   WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_PREP_V, st, (k_prep_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_PREP_W, st, (k_prep_w<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_PREP_M, st, (k_prep_m<<<blocks_for(M, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<<<colgrid, 64, 0, st>>>(g, nl, s, w, dts)));
+  WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts)));
   return hipGetLastError();
 }
 
