@@ -95,3 +95,18 @@ def test_bubble_64x64x40_100_steps(oracle_lib):
     case = ic.bubble_3d()
     got, ref = _run_both(case, 100, oracle_lib)
     _assert_parity(got, ref, case.state)
+
+
+@pytest.mark.parametrize("column", ["16x16", "8x16", "32x8", "16x8", "march"])
+def test_column_kernel_variants(oracle_lib, monkeypatch, column):
+    """Every compiled form of the acoustic column kernel (tile shapes of the
+    LDS-tiled kernel and the column-march kernel) on a terrain case with the
+    damping layer active, at the 50 levels of the 433x433x50 workload (the
+    levels-per-thread instantiation it runs); the handle reads
+    WRFDYN_COLUMN_TILE at creation."""
+    from wrf_gpu_amd import ic
+
+    monkeypatch.setenv("WRFDYN_COLUMN_TILE", column)
+    case = ic.stratified_terrain_3d(nx=20, ny=18, nz=50, dx=3000.0, u_sigma=1.0, sigma=0.5)
+    got, ref = _run_both(case, 6, oracle_lib)
+    _assert_parity(got, ref, case.state)

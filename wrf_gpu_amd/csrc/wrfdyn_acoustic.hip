@@ -724,6 +724,345 @@ __global__ void k_fin_uv(DGrid g, DState s, DWork w) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Column-march form of the acoustic column work, split in two kernels; each
+// value is computed with the same operations, in the same order, as k_column,
+// by one thread per column with no LDS and no barriers.
+//
+// k_colmarch1 (advance_mu_t): sweep up for dvdxi and the dmdt sum, the mu /
+// muts / muave updates, then a second sweep up for the omega recurrence at
+// level f (stored, all levels, to w.ww) and the theta update and t_2ave at
+// k = f-1.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_colmarch1(DGrid g, DNamelist nl, DState s, DWork w, double dts,
+                                                   int first) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t ncol = (size_t)ny * nx;
+  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
+  const unsigned plane = (unsigned)ncol * 8u;
+  const unsigned uplane = (unsigned)((size_t)ny * (nx + 1)) * 8u, vplane = (unsigned)((size_t)(ny + 1) * nx) * 8u;
+  const unsigned c8 = (unsigned)c * 8u;
+  const double msftx = g.msftx[c], msfty = g.msfty[c];
+  const double msft_inv = 1.0 / msfty;
+  const double eps_p = 1.0 + nl.epssm, eps_m = 1.0 - nl.epssm;
+  const double mut = w.mut[c];
+  const size_t fvn = i2v(g, j + 1, i), fvs = i2v(g, j, i), fue = i2u(g, j, i + 1), fuw = i2u(g, j, i);
+  const unsigned vn8 = (unsigned)fvn * 8u, vs8 = (unsigned)fvs * 8u, ue8 = (unsigned)fue * 8u, uw8 = (unsigned)fuw * 8u;
+  const double muv_n = w.muv[fvn], muv_s = w.muv[fvs];
+  const double rmvx_n = 1.0 / g.msfvx[fvn], rmvx_s = 1.0 / g.msfvx[fvs];
+  const double muu_e = w.muu[fue], muu_w = w.muu[fuw];
+  const double msfuy_e = g.msfuy[fue], msfuy_w = g.msfuy[fuw];
+  auto dvdxi_at = [&](int k, double vwn, double vws, double vn, double vs, double uwe, double uww, double ue,
+                      double uw) {
+    const double c1 = g.c1h[k], c2 = g.c2h[k];
+    double v_n = vwn + ((c1 * muv_n + c2) * vn) * rmvx_n;
+    double v_s = vws + ((c1 * muv_s + c2) * vs) * rmvx_s;
+    double u_e = uwe + ((c1 * muu_e + c2) * ue) / msfuy_e;
+    double u_w = uww + ((c1 * muu_w + c2) * uw) / msfuy_w;
+    return (msftx * msfty) * (g.rdy * (v_n - v_s) + g.rdx * (u_e - u_w));
+  };
+
+  // ---- sweep 1: dmdt (advance_mu_t), then the mass updates ----
+  double dmdt = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < nz; ++k) {
+    const unsigned kv = (unsigned)k * vplane, ku = (unsigned)k * uplane;
+    const double dv = dvdxi_at(k, ldo(w.v_work, vn8 + kv), ldo(w.v_work, vs8 + kv), ldo(s.v, vn8 + kv),
+                               ldo(s.v, vs8 + kv), ldo(w.u_work, ue8 + ku), ldo(w.u_work, uw8 + ku),
+                               ldo(s.u, ue8 + ku), ldo(s.u, uw8 + ku));
+    dmdt = (k == 0) ? g.dnw[0] * dv : dmdt + g.dnw[k] * dv;
+  }
+  const double p2_ww0 = w.ww[c];
+  const double p2_wwtop = ldo(w.ww, c8 + (unsigned)nz * plane);
+  double muts, muave;
+  {
+    const double mu_work_old = w.muts[c] - mut;
+    const double mu_save = w.mu[c] - mu_work_old;
+    const double mu_tendency = dmdt + 0.0;
+    const double mu_work_new = mu_work_old + dts * mu_tendency;
+    w.mu[c] = mu_save + mu_work_new;
+    s.mudf[c] = mu_tendency;
+    muts = mut + mu_work_new;
+    w.muts[c] = muts;
+    muave = 0.5 * (eps_p * mu_work_new + eps_m * mu_work_old);
+    w.muave[c] = muave;
+  }
+
+  const int ip = wrapi(i + 1, nx), imn = wrapi(i - 1, nx), jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny);
+  const unsigned cjp = (unsigned)i2(g, jp, i) * 8u, cjm = (unsigned)i2(g, jm, i) * 8u;
+  const unsigned cip = (unsigned)i2(g, j, ip) * 8u, cim = (unsigned)i2(g, j, imn) * 8u;
+  struct Mv {  // mass level data
+    double vwn, vws, uwe, uww, vn, vs, ue, uw;
+    double thc, thn, ths, the, thw, thwk, tht, t2o;
+  };
+  Mv m_1{};                         // level f-1
+  double thc_2 = 0.0;               // theta at f-2
+  double b_1 = 0.0;                 // omega at f-1
+  double wprev = p2_ww0, dv_1 = 0.0;  // omega prefix, dvdxi at f-1
+  for (int f = 0; f <= nz; ++f) {
+    Mv m{};
+    if (f < nz) {
+      const unsigned kp = (unsigned)f * plane, q = c8 + kp;
+      const unsigned kv = (unsigned)f * vplane, ku = (unsigned)f * uplane;
+      m.vwn = ldo(w.v_work, vn8 + kv);
+      m.vws = ldo(w.v_work, vs8 + kv);
+      m.uwe = ldo(w.u_work, ue8 + ku);
+      m.uww = ldo(w.u_work, uw8 + ku);
+      m.vn = ldo(s.v, vn8 + kv);
+      m.vs = ldo(s.v, vs8 + kv);
+      m.ue = ldo(s.u, ue8 + ku);
+      m.uw = ldo(s.u, uw8 + ku);
+      m.thc = ldo(s.theta, q);
+      m.thn = ldo(s.theta, cjp + kp);
+      m.ths = ldo(s.theta, cjm + kp);
+      m.the = ldo(s.theta, cip + kp);
+      m.thw = ldo(s.theta, cim + kp);
+      m.thwk = ldo(w.th_work, q);
+      m.tht = ldo(w.tht, q);
+      m.t2o = first ? 0.0 : ldo(w.t2ave, q);  // t_2ave starts each stage at 0
+    }
+    const unsigned qf = c8 + (unsigned)f * plane;
+    const double rom = f < nz ? ldo(w.rom, qf) : 0.0;
+    const double dv = f < nz ? dvdxi_at(f, m.vwn, m.vws, m.vn, m.vs, m.uwe, m.uww, m.ue, m.uw) : 0.0;
+    // omega at f (levels 0 and nz carry to the next substep; the interior is
+    // rebuilt each substep and read by k_colmarch2)
+    double b0;
+    if (f == 0) {
+      b0 = wprev - rom;
+      sto(w.ww, qf, b0);
+    } else if (f < nz) {
+      const int k = f - 1;
+      double incr = (g.dnw[f - 1] * ((g.c1h[k] * dmdt + dv_1) + g.c1h[k] * 0.0)) / msfty;
+      double wk = wprev - incr;
+      b0 = wk - rom;
+      wprev = wk;
+      sto(w.ww, qf, b0);
+    } else {
+      b0 = p2_wwtop;
+    }
+    // theta update and t_2ave at k = f-1
+    if (f >= 1) {
+      const int k = f - 1;
+      const double th = m_1.thc - kT0;
+      double wd_hi = 0.0, wd_lo = 0.0;
+      if (k + 1 < nz) {
+        const int kk = k + 1;
+        wd_hi = b0 * (g.fnm[kk] * (m.thc - kT0) + g.fnp[kk] * th);
+      }
+      if (k >= 1) wd_lo = b_1 * (g.fnm[k] * th + g.fnp[k] * (thc_2 - kT0));
+      double v_flux = m_1.vwn * ((m_1.thn - kT0) + th) - m_1.vws * (th + (m_1.ths - kT0));
+      double u_flux = m_1.uwe * ((m_1.the - kT0) + th) - m_1.uww * (th + (m_1.thw - kT0));
+      double tendency = msftx * ((0.5 * g.rdy) * v_flux + (0.5 * g.rdx) * u_flux) + g.rdnw[k] * (wd_hi - wd_lo);
+      const unsigned q = c8 + (unsigned)k * plane;
+      double theta_i = m_1.thwk + ((msfty * dts) * m_1.tht);
+      double th_new = theta_i - ((dts * msfty) * tendency);
+      sto(w.th_work, q, th_new);
+      double mass_h = g.c1h[k] * muts + g.c2h[k];
+      double half = 0.5 * (eps_p * th_new + eps_m * m_1.t2o);
+      double ttr = kT0 + th;
+      double t2n = (half + (g.c1h[k] * muave) * kT0) / (mass_h * ttr);
+      sto(w.t2ave, q, t2n);
+    }
+    b_1 = b0;
+    thc_2 = m_1.thc;
+    dv_1 = dv;
+    m_1 = m;
+  }
+}
+
+// k_colmarch2 (advance_w + calc_p_rho): sweep up for the phi-equation rhs at
+// f-1 (stored to w.rhs) and the explicit w and Thomas forward sweep at f-2
+// (stored to w.wfwd), reading omega and the new t_2ave from k_colmarch1; then
+// sweep down for the back substitution, damping, the phi finish and
+// calc_p_rho (which needs the finished phi of levels k and k+1).
+__global__ void __launch_bounds__(256) k_colmarch2(DGrid g, DNamelist nl, DState s, DWork w, double dts,
+                                                   int first) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t ncol = (size_t)ny * nx;
+  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
+  const unsigned plane = (unsigned)ncol * 8u;
+  const unsigned c8 = (unsigned)c * 8u;
+  const double msftx = g.msftx[c], msfty = g.msfty[c];
+  const double msft_inv = 1.0 / msfty;
+  const double eps_p = 1.0 + nl.epssm, eps_m = 1.0 - nl.epssm;
+  const double mut = w.mut[c];
+  const double muts = w.muts[c], muave = w.muave[c];
+  // terrain surface w (advance_w.py:505-556, level 0 of the explicit w)
+  double a0;
+  {
+    const int jn = clampi(j + 1, 0, ny - 1), js = clampi(j - 1, 0, ny - 1);
+    const int ie = clampi(i + 1, 0, nx - 1), iw = clampi(i - 1, 0, nx - 1);
+    auto PHB0 = [&](int jj, int ii) {
+      const size_t q = i2(g, jj, ii);
+      return s.ph_total[q] - s.ph_pert[q];
+    };
+    double ht = PHB0(j, i) / kG;
+    double ht_n = PHB0(jn, i) / kG, ht_s = PHB0(js, i) / kG;
+    double ht_e = PHB0(j, ie) / kG, ht_w = PHB0(j, iw) / kG;
+    double dy_n = ht_n - ht, dy_s = ht - ht_s, dx_e = ht_e - ht, dx_w = ht - ht_w;
+    double v_cf_n = (g.cf1 * s.v[iv(g, 0, j + 1, i)] + g.cf2 * s.v[iv(g, 1, j + 1, i)]) + g.cf3 * s.v[iv(g, 2, j + 1, i)];
+    double v_cf_s = (g.cf1 * s.v[iv(g, 0, j, i)] + g.cf2 * s.v[iv(g, 1, j, i)]) + g.cf3 * s.v[iv(g, 2, j, i)];
+    double u_cf_e = (g.cf1 * s.u[iu(g, 0, j, i + 1)] + g.cf2 * s.u[iu(g, 1, j, i + 1)]) + g.cf3 * s.u[iu(g, 2, j, i + 1)];
+    double u_cf_w = (g.cf1 * s.u[iu(g, 0, j, i)] + g.cf2 * s.u[iu(g, 1, j, i)]) + g.cf3 * s.u[iu(g, 2, j, i)];
+    a0 = ((msfty * 0.5) * g.rdy) * (dy_n * v_cf_n + dy_s * v_cf_s) +
+         ((msftx * 0.5) * g.rdx) * (dx_e * u_cf_e + dx_w * u_cf_w);
+  }
+
+  const double c05geps = (0.5 * kG) * eps_m;
+  const double c05dtg = (0.5 * dts) * kG;
+  struct Wv {  // w level data (c2a, alt, t2n: mass level f)
+    double ph1t, pht, wold, phw, rwt, ca, cal, ww, c2a, alt, t2n;
+  };
+  Wv w_1{}, w_2{}, w_3{};                   // levels f-1, f-2, f-3
+  double d_1 = 0.0, d_2 = 0.0;              // phi rhs at f-2, f-3
+  double ph1t_2 = 0.0;                      // ph1t at f-2
+  double wf_1 = 0.0;                        // Thomas forward value at the previous explicit level
+  for (int f = 0; f <= nz + 2; ++f) {
+    Wv wv{};
+    if (f <= nz) {
+      const unsigned q = c8 + (unsigned)f * plane;
+      wv.ph1t = ldo(w.ph1t, q);
+      wv.pht = ldo(w.ph_tend, q);
+      wv.wold = ldo(w.w_work, q);
+      wv.phw = ldo(w.ph_work, q);
+      wv.rwt = ldo(w.rw_tend, q);
+      wv.ca = ldo(w.ca, q);
+      wv.cal = ldo(w.calpha, q);
+      wv.ww = ldo(w.ww, q);
+      if (f < nz) {
+        wv.c2a = ldo(w.c2a, q);
+        wv.alt = ldo(w.alt, q);
+        wv.t2n = ldo(w.t2ave, q);
+      }
+    }
+    // phi rhs at fd = f-1 (advance_w.py:460-503)
+    double d0 = 0.0;
+    if (f >= 1 && f <= nz + 1) {
+      const int fd = f - 1;
+      if (fd > 0) {
+        double rr = dts * (w_1.pht + c05geps * w_1.wold);
+        if (fd < nz) {
+          double wd_f1 = ((0.5 * (wv.ww + w_1.ww)) * g.rdnw[fd]) * (wv.ph1t - w_1.ph1t);
+          double wd_f = ((0.5 * (w_1.ww + w_2.ww)) * g.rdnw[fd - 1]) * (w_1.ph1t - ph1t_2);
+          double adv = dts * (g.fnm[fd] * wd_f1 + g.fnp[fd] * wd_f);
+          rr = rr + (-adv);
+        }
+        double mass_f = g.c1f[fd] * mut + g.c2f[fd];
+        rr = w_1.phw + (msfty * rr) / mass_f;
+        if (fd == nz && nl.top_lid) rr = 0.0;
+        d0 = rr;
+      }
+      sto(w.rhs, c8 + (unsigned)fd * plane, d0);
+    }
+    // explicit w (advance_w.py:505-556) and the Thomas forward sweep at fa = f-2
+    if (f >= 2) {
+      const int fa = f - 2;
+      double a;
+      if (fa == 0) {
+        a = a0;
+      } else if (fa == nz) {
+        a = 0.0;  // rigid lid
+      } else {
+        double rw = w_2.rwt;
+        if (nl.w_damping == 1) {
+          double mass_f = g.c1f[fa] * mut + g.c2f[fa];
+          double safe_mf = safe_pos(mass_f, 1.0e-12);
+          double vert_cfl = fabs(((w_2.ww / safe_mf) * g.rdnw[fa]) * dts);
+          double damp = (((-sgn(w_2.wold)) * kWAlpha) * (vert_cfl - kWBeta)) * mass_f;
+          rw = rw + (vert_cfl > kWBeta ? damp : 0.0);
+        }
+        double wn = w_2.wold + dts * rw;
+        double mh_u = safe_pos(g.c1h[fa] * mut + g.c2h[fa], 1.0e-12);
+        double mh_l = safe_pos(g.c1h[fa - 1] * mut + g.c2h[fa - 1], 1.0e-12);
+        double c2a_u = w_2.c2a, c2a_l = w_3.c2a;
+        double coef_u = (c2a_u * g.rdnw[fa]) / mh_u;
+        double coef_l = (c2a_l * g.rdnw[fa - 1]) / mh_l;
+        double phk1 = w_1.phw, phk = w_2.phw, phkm = w_3.phw;
+        double tA_u = coef_u * (eps_p * (d0 - d_1) + eps_m * (phk1 - phk));
+        double tA_l = coef_l * (eps_p * (d_1 - d_2) + eps_m * (phk - phkm));
+        double termA = ((msft_inv * cqw_at(fa, nz)) * (c05dtg * g.rdn[fa])) * (tA_u - tA_l);
+        double bu = (c2a_u * w_2.alt) * w_2.t2n;
+        double bl = (c2a_l * w_3.alt) * w_3.t2n;
+        double termB = ((dts * kG) * msft_inv) * (g.rdn[fa] * (bu - bl) - (g.c1f[fa] * muave));
+        a = wn + (termA + termB);
+      }
+      double wf = fa == 0 ? a : (a - w_2.ca * wf_1) * w_2.cal;
+      sto(w.wfwd, c8 + (unsigned)fa * plane, wf);
+      wf_1 = wf;
+    }
+    d_2 = d_1;
+    d_1 = d0;
+    ph1t_2 = w_1.ph1t;
+    w_3 = w_2;
+    w_2 = w_1;
+    w_1 = wv;
+  }
+
+  // ---- sweep 3: back substitution, damping, phi finish, calc_p_rho ----
+  {
+    const bool damp = (nl.damp_opt == 3 && nl.dampcoef > 0.0);
+    const double dampmag = dts * nl.dampcoef, hdepth = nl.zdamp;
+    const double htop = ldo(w.ph1t, c8 + (unsigned)nz * plane) / kG;
+    const double hbot = htop - hdepth;
+    const double mu_work = muts - mut;
+    double w_next = 0.0, phn_next = 0.0;
+    for (int f = nz; f >= 0; --f) {
+      const unsigned q = c8 + (unsigned)f * plane;
+      const double wf = ldo(w.wfwd, q);
+      double wv;
+      if (f == nz || f == 0) wv = wf;
+      else wv = wf - ldo(w.cgamma, q) * w_next;
+      w_next = wv;
+      double phn;
+      if (f == 0) {
+        sto(w.w_work, q, wv);
+        phn = ldo(w.ph_work, q);  // the surface ph work is unchanged
+      } else {
+        double wd = wv;
+        if (damp) {
+          double hk = ldo(w.ph1t, q) / kG;
+          double sn = wd_sin(((0.5 * kPi) * (hk - hbot)) / hdepth);
+          double ramp = sn * sn;
+          double dampwt = hk >= hbot ? dampmag * ramp : 0.0;
+          double mass_f = g.c1f[f] * mut + g.c2f[f];
+          // below the damping layer dampwt = 0 and the saved w drops out
+          // exactly ((wv - 0 * w_save) / 1), so it is only read inside the layer
+          double wsave = hk >= hbot ? ldo(s.w, q) : 0.0;
+          wd = (wv - (dampwt * mass_f) * wsave) / (1.0 + dampwt);
+        }
+        double mfs = g.c1f[f] * muts + g.c2f[f];
+        phn = ldo(w.rhs, q) + (((((msfty * 0.5) * dts) * kG) * eps_p) * wd) / mfs;
+        sto(w.ph_work, q, phn);
+        sto(w.w_work, q, wd);
+      }
+      if (f < nz) {  // calc_p_rho_step at mass level k = f
+        const int k = f;
+        double mass_h = g.c1h[k] * muts + g.c2h[k];
+        double safe_mass = safe_pos(mass_h, 1.0e-12);
+        double mu_term = g.c1h[k] * mu_work;
+        double alt = ldo(w.alt, q);
+        double al = -(alt * mu_term + g.rdnw[k] * (phn_next - phn)) / safe_mass;
+        double theta_1 = ldo(s.theta, q) - kT0;
+        double ttr = kT0 + theta_1;
+        double safe_t = safe_pos(ttr, 1.0e-6);
+        double p = ldo(w.c2a, q) * (((alt * (ldo(w.th_work, q) - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
+        double ptmp = p;
+        // first substep of a stage: pm1 is the step-0 pressure (calc_p_rho_wrf
+        // step 0 sets pm1 = p), read from p before it is overwritten here
+        p = p + kSmdiv * (p - ldo(first ? w.p : w.pm1, q));
+        sto(w.al, q, al);
+        sto(w.p, q, p);
+        sto(w.pm1, q, ptmp);
+      }
+      phn_next = phn;
+    }
+  }
+}
+
 // small_step_finish for w, ph, theta, p and mu (core/small_step_finish.py:22-101)
 // followed by _refresh_grid_p_from_finished (operational_mode.py:2169-2231,
 // diagnose_pressure_al_alt's base branch, acoustic_wrf.py:360-438).  One
@@ -856,6 +1195,13 @@ namespace {
 
 hipError_t launch_column_tile(int tile, const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                               double dts, int first, hipStream_t st, Prof* prof) {
+  if (tile == kColumnMarch) {
+    const unsigned bs = 256;
+    const size_t C = (size_t)g.ny * g.nx;
+    WD_TIMED(prof, KID_COLUMN, st, (k_colmarch1<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+    WD_TIMED(prof, KID_COLUMN2, st, (k_colmarch2<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+    return hipGetLastError();
+  }
   switch (tile) {
     case 1: return launch_column_levels<8, 16>(g, nl, s, w, dts, first, st, prof);
     case 2: return launch_column_levels<32, 8>(g, nl, s, w, dts, first, st, prof);
@@ -887,7 +1233,7 @@ const char* kernel_name(int id) {
   static const char* names[KID_COUNT] = {
       "origin_copy", "k_ruv", "k_rom", "k_prep_col", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t",
       "k_prep_2d", "k_prep_faces", "k_prep_u", "k_prep_v", "k_col_setup",
-      "k_uv", "k_column", "k_fin_uv", "k_fin_col"};
+      "k_uv", "k_column", "k_column2", "k_fin_uv", "k_fin_col"};
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 

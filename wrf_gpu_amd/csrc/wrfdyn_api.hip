@@ -62,10 +62,6 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     if (cells_w * 8.0 > 4294967295.0)
       return fail(nullptr, WD_EINVAL, "grid too large: each 3-D field must stay below 4 GiB");
   }
-  if (nz + 1 > wd::kColumnTiles[0].kt * wd::kMaxLevelsPerThread)
-    return fail(nullptr, WD_EINVAL, "nz too large: the column kernel holds at most " +
-                                        std::to_string(wd::kColumnTiles[0].kt * wd::kMaxLevelsPerThread - 1) +
-                                        " levels");
   if (!(grid->dx > 0.0) || !(grid->dy > 0.0)) return fail(nullptr, WD_EINVAL, "dx and dy must be positive");
   const double* req[] = {grid->c1h, grid->c2h, grid->c1f, grid->c2f, grid->dnw, grid->rdnw, grid->dn,
                          grid->rdn, grid->fnm, grid->fnp, grid->msftx, grid->msfty, grid->msfux,
@@ -188,7 +184,7 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
       {&w.th_work, M},  {&w.ph_work, Wn},  {&w.c2a, M},      {&w.alt, M},      {&w.pb, M},
       {&w.ph1t, Wn},     {&w.php, M},       {&w.p, M},        {&w.al, M},       {&w.pm1, M},
       {&w.rw_tend, Wn}, {&w.ph_tend, Wn},  {&w.ca, Wn},      {&w.calpha, Wn},  {&w.cgamma, Wn},
-      {&w.ww, Wn},      {&w.t2ave, M},
+      {&w.ww, Wn},      {&w.t2ave, M},     {&w.rhs, Wn},     {&w.wfwd, Wn},
   };
   // 256-byte aligned sub-buffers
   size_t total = 0;
@@ -213,11 +209,14 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     delete h;
     return hip_fail(nullptr, e, "hipMemset(workspace)");
   }
-  // Column-kernel tile shape: WRFDYN_COLUMN_TILE="TXxKT" selects one of the
-  // compiled shapes (default 16x16).
+  // Acoustic column kernel: the column-march pair (default) or, with
+  // WRFDYN_COLUMN_TILE="TXxKT", the LDS-tiled kernel in one of its compiled
+  // tile shapes (when that shape holds nz + 1 levels).
+  h->column_tile = wd::kColumnMarch;
   if (const char* env = std::getenv("WRFDYN_COLUMN_TILE")) {
     int tx = 0, kt = 0;
-    if (std::sscanf(env, "%dx%d", &tx, &kt) == 2) {
+    if (std::strcmp(env, "march") == 0) h->column_tile = wd::kColumnMarch;
+    else if (std::sscanf(env, "%dx%d", &tx, &kt) == 2) {
       for (int t = 0; t < wd::kNumColumnTiles; ++t)
         if (wd::kColumnTiles[t].tx == tx && wd::kColumnTiles[t].kt == kt &&
             nz + 1 <= kt * wd::kMaxLevelsPerThread)

@@ -75,6 +75,8 @@ struct DWork {
   double *rw_tend, *ph_tend, *ca, *calpha, *cgamma;
   // acoustic carried (ww: only levels 0 and nz carry between substeps)
   double *ww, *t2ave;
+  // column-march scratch: phi-equation rhs and the Thomas forward sweep
+  double *rhs, *wfwd;
 };
 
 __host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {

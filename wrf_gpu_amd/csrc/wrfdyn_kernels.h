@@ -25,7 +25,8 @@ enum KernelId {
   KID_PREP_V,
   KID_COL_SETUP,
   KID_UV,
-  KID_COLUMN,
+  KID_COLUMN,   // the column kernel (k_column, or k_colmarch1 in the march form)
+  KID_COLUMN2,  // k_colmarch2 (march form only)
   KID_FIN_UV,
   KID_FIN_COL,
   KID_COUNT
@@ -92,6 +93,8 @@ struct ColumnTile {
   int tx, kt;
 };
 constexpr int kMaxLevelsPerThread = 8;
+// column_tile value that selects the column-march kernel instead of a tile shape
+constexpr int kColumnMarch = -1;
 extern const ColumnTile kColumnTiles[];
 extern const int kNumColumnTiles;
 
