@@ -11,9 +11,15 @@ Throughput counts namelist substeps: cell-substeps/s = N * ns * steps / wall.
 
 Prints ONE JSON line with the metric, the dominant kernel's roofline (its
 algorithmic bytes per launch divided by its average launch duration, timed
-with HIP events on the launch stream over the timed region) and a CPU
-baseline (the oracle on a bounded sample of the same workload, this host's
-cores).
+with HIP events on the launch stream over the timed region), its measured HBM
+traffic per launch (from the committed rocprofv3 counter summary, when
+present) and a CPU baseline (the oracle on a bounded sample of the same
+workload, this host's cores).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+  python bench.py --pmc-summary FETCH_DIR WRITE_DIR --calibration-copy-mib 1024 > summary.json
+      (summarise two rocprofv3 --pmc runs, FETCH_SIZE then WRITE_SIZE, of
+       `bench.py --steps 1 --warmup 0 --cpu-baseline-steps 0 --calibration-copy-mib 1024`)
 """
 
 from __future__ import annotations
@@ -30,18 +36,26 @@ if ROOT not in sys.path:
 
 METRIC = "cell-substeps/s, chosen form vs plain FP64 at stated err; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_terrain_433x433x50.json")
 
-# Algorithmic HBM bytes per mass cell for one launch of each kernel: every
-# 3-D array the kernel must read or write once (FP64), neighbour re-reads and
-# scratch excluded.
+# Algorithmic HBM bytes per mass cell for one launch of each kernel: every 3-D
+# array the kernel must read or write, counted once at 8 B per mass cell
+# (staggering extras, 2-D fields, neighbour re-reads and kernel-internal
+# scratch excluded).
 ALG_ARRAYS_PER_CELL = {
-    # reads u_work, v_work, u, v, th_work, t_tend, theta, ww, rom, w_work,
-    # rw_tend, ph_tend, ph_1, phb, ph_work, c2a, alt, a, alpha, gamma, t_2ave,
-    # w_save, pm1 (23); writes th_work, ww, w_work, ph_work, t_2ave, pm1, p, al (8)
-    "k_column": 31,
+    # column march, advance_mu_t kernel: reads u_work, v_work, u, v, theta,
+    # th_work, t_tend, t_2ave, rom (9); writes th_work, t_2ave, ww (3)
+    "k_column_mu": 12,
+    # column march, advance_w + calc_p_rho kernel: reads ph_1 total, ph_tend,
+    # w_work, ph_work, rw_tend, a, alpha, gamma, ww, c2a, alt, t_2ave, theta,
+    # th_work, pm1, w_save (16); writes ph_work, w_work, al, p, pm1 (5)
+    "k_column_w": 21,
     # advance_uv for u and v: reads u_work, v_work, u_tend, v_tend, p, ph_work,
     # pb, al, alt, php; writes u_work, v_work
     "k_uv": 12,
+    # LDS-tiled column kernel (WRFDYN_COLUMN_TILE=TXxKT): the work of both march
+    # kernels in one launch, 23 reads and 8 writes
+    "k_column": 31,
 }
 # Per-large-step compulsory traffic model (cells x 8 B x passes):
 # 3 stages x 55 stage passes + S(ns) x 45 substep passes, S(ns) = 1 + ns//2 + ns.
@@ -60,26 +74,85 @@ def parse():
     ap.add_argument("--nx", type=int, default=433)
     ap.add_argument("--ny", type=int, default=433)
     ap.add_argument("--nz", type=int, default=50)
-    ap.add_argument("--cpu-baseline-steps", type=int, default=1,
+    ap.add_argument("--cpu-baseline-steps", type=int, default=8,
                     help="large steps of the oracle timed on the host (0 skips)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all cores in this process's affinity")
+    ap.add_argument("--breakdown-steps", type=int, default=3,
+                    help="untimed steps after the timed region with every kernel timed (0 skips)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE output directories and exit")
     return ap.parse_args()
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, copy_mib: int) -> dict:
+    """Per-kernel average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
+    rocprofv3 --pmc runs, plus the same counters on the calibration copy (the
+    largest copy dispatch, `copy_mib` MiB read and written)."""
+
+    import glob
+    import re
+    import sqlite3
+
+    def read(d, counter):
+        db = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)
+        if not db:
+            raise SystemExit(f"no rocprofv3 database under {d}")
+        con = sqlite3.connect(db[0])
+        per, copies = {}, []
+        for name, cname, val in con.execute("select name, counter_name, counter_value from pmc_events"):
+            if cname != counter:
+                continue
+            m = re.search(r"::(k_\w+)", name)
+            if "copyBuffer" in name:
+                copies.append(val)
+            if m:
+                per.setdefault(m.group(1), []).append(val)
+        return {k: sum(v) / len(v) for k, v in per.items()}, (max(copies) if copies else None)
+
+    fetch, fetch_copy = read(fetch_dir, "FETCH_SIZE")
+    write, write_copy = read(write_dir, "WRITE_SIZE")
+    return {
+        "unit": "KiB per dispatch (rocprofv3 FETCH_SIZE / WRITE_SIZE)",
+        "calibration": {"copy_bytes": copy_mib << 20, "fetch_kib": fetch_copy, "write_kib": write_copy},
+        "kernels": {k: {"fetch_kib": fetch[k], "write_kib": write.get(k)} for k in sorted(fetch)},
+    }
+
+
+def traffic_for(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary:
+    each counter scaled by what it reads for the plain calibration copy."""
+
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    with open(TRAFFIC_FILE) as f:
+        summ = json.load(f)
+    cal = summ["calibration"]
+    k = summ["kernels"].get(kernel)
+    if not k or not cal.get("fetch_kib") or not cal.get("write_kib") or k.get("write_kib") is None:
+        return None, None
+    f_scale = cal["copy_bytes"] / (cal["fetch_kib"] * 1024.0)
+    w_scale = cal["copy_bytes"] / (cal["write_kib"] * 1024.0)
+    bytes_ = k["fetch_kib"] * 1024.0 * f_scale + k["write_kib"] * 1024.0 * w_scale
+    return bytes_, {"fetch_scale": round(f_scale, 3), "write_scale": round(w_scale, 3),
+                    "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
 
 
 def main():
     args = parse()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary, args.calibration_copy_mib or 1024), indent=1))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the dycore shards as described in DESIGN.md "
-                                   "but multi-GPU runs are not part of it", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU: sharding across GPUs is designed but not "
+                                   "built, so only --gpus 1 is measured", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import numpy as np
     import torch
 
-    from wrf_gpu_amd import ic
+    from wrf_gpu_amd import _lib, ic
     from wrf_gpu_amd.dycore import Dycore, state_to_device
 
     if not torch.cuda.is_available():
@@ -112,8 +185,9 @@ def main():
     torch.cuda.synchronize()
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
-    # timed region, dominant-kernel events recorded on the launch stream
-    profiled = ("k_column", "k_uv")
+    # timed region; the candidate dominant kernels carry HIP events on the launch stream
+    names = _lib.kernel_names()
+    profiled = [k for k in ("k_column", "k_column_mu", "k_column_w", "k_uv") if k in names]
     dyn.profile(profiled)
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -141,13 +215,29 @@ def main():
                             "alg_bytes": alg, "achieved_GBs": alg / (avg_ms * 1e-3) / 1e9}
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"] * per_kernel[k]["launches"])
     d = per_kernel[dom]
+    traffic, traffic_info = traffic_for(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["achieved_GBs"], 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": None if traffic is None else round(traffic),
                 "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": round(d["avg_ms"], 4)}
+    if traffic_info:
+        roofline["traffic_calibration"] = traffic_info
+        roofline["traffic_GBs"] = round(traffic / (d["avg_ms"] * 1e-3) / 1e9, 1)
     step_bytes = step_bytes_per_cell(ns) * n_cells
     step_roofline = {"model_bytes_per_step": step_bytes,
                      "achieved": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1), "unit": "GB/s",
                      "frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    # untimed per-kernel breakdown: every kernel bracketed by events
+    breakdown = None
+    if args.breakdown_steps > 0:
+        dyn.profile(names)
+        dyn.step(state, args.breakdown_steps)
+        torch.cuda.synchronize()
+        br = dyn.profile_read()
+        dyn.profile(())
+        breakdown = {k: {"avg_ms": round(v[0] / v[1], 4), "ms_per_step": round(v[0] / args.breakdown_steps, 4)}
+                     for k, v in sorted(br.items(), key=lambda kv: -kv[1][0])}
 
     # CPU baseline: the oracle (a C restatement of the reference path) on a
     # bounded sample of the same workload
@@ -187,6 +277,7 @@ def main():
         "step_roofline": step_roofline,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in per_kernel.items()},
+        "kernel_breakdown": breakdown,
         "cpu_baseline": cpu,
         "finite": finite,
     }
