@@ -729,12 +729,12 @@ __global__ void k_fin_uv(DGrid g, DState s, DWork w) {
 // value is computed with the same operations, in the same order, as k_column,
 // by one thread per column with no LDS and no barriers.
 //
-// k_colmarch1 (advance_mu_t): sweep up for dvdxi and the dmdt sum, the mu /
+// k_column_mu (advance_mu_t): sweep up for dvdxi and the dmdt sum, the mu /
 // muts / muave updates, then a second sweep up for the omega recurrence at
 // level f (stored, all levels, to w.ww) and the theta update and t_2ave at
 // k = f-1.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_colmarch1(DGrid g, DNamelist nl, DState s, DWork w, double dts,
+__global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState s, DWork w, double dts,
                                                    int first) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t ncol = (size_t)ny * nx;
@@ -827,7 +827,7 @@ __global__ void __launch_bounds__(256) k_colmarch1(DGrid g, DNamelist nl, DState
     const double rom = f < nz ? ldo(w.rom, qf) : 0.0;
     const double dv = f < nz ? dvdxi_at(f, m.vwn, m.vws, m.vn, m.vs, m.uwe, m.uww, m.ue, m.uw) : 0.0;
     // omega at f (levels 0 and nz carry to the next substep; the interior is
-    // rebuilt each substep and read by k_colmarch2)
+    // rebuilt each substep and read by k_column_w)
     double b0;
     if (f == 0) {
       b0 = wprev - rom;
@@ -872,12 +872,12 @@ __global__ void __launch_bounds__(256) k_colmarch1(DGrid g, DNamelist nl, DState
   }
 }
 
-// k_colmarch2 (advance_w + calc_p_rho): sweep up for the phi-equation rhs at
+// k_column_w (advance_w + calc_p_rho): sweep up for the phi-equation rhs at
 // f-1 (stored to w.rhs) and the explicit w and Thomas forward sweep at f-2
-// (stored to w.wfwd), reading omega and the new t_2ave from k_colmarch1; then
+// (stored to w.wfwd), reading omega and the new t_2ave from k_column_mu; then
 // sweep down for the back substitution, damping, the phi finish and
 // calc_p_rho (which needs the finished phi of levels k and k+1).
-__global__ void __launch_bounds__(256) k_colmarch2(DGrid g, DNamelist nl, DState s, DWork w, double dts,
+__global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState s, DWork w, double dts,
                                                    int first) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t ncol = (size_t)ny * nx;
@@ -921,24 +921,31 @@ __global__ void __launch_bounds__(256) k_colmarch2(DGrid g, DNamelist nl, DState
   double d_1 = 0.0, d_2 = 0.0;              // phi rhs at f-2, f-3
   double ph1t_2 = 0.0;                      // ph1t at f-2
   double wf_1 = 0.0;                        // Thomas forward value at the previous explicit level
-  for (int f = 0; f <= nz + 2; ++f) {
-    Wv wv{};
+  // level f's loads are issued one iteration ahead, before level f-1's stores
+  auto load_w = [&](int f, Wv& L) {
     if (f <= nz) {
       const unsigned q = c8 + (unsigned)f * plane;
-      wv.ph1t = ldo(w.ph1t, q);
-      wv.pht = ldo(w.ph_tend, q);
-      wv.wold = ldo(w.w_work, q);
-      wv.phw = ldo(w.ph_work, q);
-      wv.rwt = ldo(w.rw_tend, q);
-      wv.ca = ldo(w.ca, q);
-      wv.cal = ldo(w.calpha, q);
-      wv.ww = ldo(w.ww, q);
+      L.ph1t = ldo(w.ph1t, q);
+      L.pht = ldo(w.ph_tend, q);
+      L.wold = ldo(w.w_work, q);
+      L.phw = ldo(w.ph_work, q);
+      L.rwt = ldo(w.rw_tend, q);
+      L.ca = ldo(w.ca, q);
+      L.cal = ldo(w.calpha, q);
+      L.ww = ldo(w.ww, q);
       if (f < nz) {
-        wv.c2a = ldo(w.c2a, q);
-        wv.alt = ldo(w.alt, q);
-        wv.t2n = ldo(w.t2ave, q);
+        L.c2a = ldo(w.c2a, q);
+        L.alt = ldo(w.alt, q);
+        L.t2n = ldo(w.t2ave, q);
       }
     }
+  };
+  Wv wv_next{};
+  load_w(0, wv_next);
+  for (int f = 0; f <= nz + 2; ++f) {
+    const Wv wv = wv_next;
+    wv_next = Wv{};
+    load_w(f + 1, wv_next);
     // phi rhs at fd = f-1 (advance_w.py:460-503)
     double d0 = 0.0;
     if (f >= 1 && f <= nz + 1) {
@@ -1010,21 +1017,50 @@ __global__ void __launch_bounds__(256) k_colmarch2(DGrid g, DNamelist nl, DState
     const double hbot = htop - hdepth;
     const double mu_work = muts - mut;
     double w_next = 0.0, phn_next = 0.0;
-    for (int f = nz; f >= 0; --f) {
+    struct Dv {  // level f loads of the down sweep (mass-level fields for f < nz)
+      double wf, cg, ph1t, rhs, phw0, alt, th, c2a, thw, pm1;
+    };
+    // level f-1's loads are issued before level f's stores
+    auto load_d = [&](int f, Dv& L) {
+      if (f < 0) return;
       const unsigned q = c8 + (unsigned)f * plane;
-      const double wf = ldo(w.wfwd, q);
+      L.wf = ldo(w.wfwd, q);
+      if (f >= 1 && f < nz) L.cg = ldo(w.cgamma, q);
+      if (f >= 1) {
+        L.ph1t = ldo(w.ph1t, q);
+        L.rhs = ldo(w.rhs, q);
+      } else {
+        L.phw0 = ldo(w.ph_work, q);  // the surface ph work is unchanged
+      }
+      if (f < nz) {
+        L.alt = ldo(w.alt, q);
+        L.th = ldo(s.theta, q);
+        L.c2a = ldo(w.c2a, q);
+        L.thw = ldo(w.th_work, q);
+        // first substep of a stage: pm1 is the step-0 pressure (calc_p_rho_wrf
+        // step 0 sets pm1 = p), read from p before it is overwritten here
+        L.pm1 = ldo(first ? w.p : w.pm1, q);
+      }
+    };
+    Dv dn_next{};
+    load_d(nz, dn_next);
+    for (int f = nz; f >= 0; --f) {
+      const Dv dv = dn_next;
+      dn_next = Dv{};
+      load_d(f - 1, dn_next);
+      const unsigned q = c8 + (unsigned)f * plane;
       double wv;
-      if (f == nz || f == 0) wv = wf;
-      else wv = wf - ldo(w.cgamma, q) * w_next;
+      if (f == nz || f == 0) wv = dv.wf;
+      else wv = dv.wf - dv.cg * w_next;
       w_next = wv;
       double phn;
       if (f == 0) {
         sto(w.w_work, q, wv);
-        phn = ldo(w.ph_work, q);  // the surface ph work is unchanged
+        phn = dv.phw0;
       } else {
         double wd = wv;
         if (damp) {
-          double hk = ldo(w.ph1t, q) / kG;
+          double hk = dv.ph1t / kG;
           double sn = wd_sin(((0.5 * kPi) * (hk - hbot)) / hdepth);
           double ramp = sn * sn;
           double dampwt = hk >= hbot ? dampmag * ramp : 0.0;
@@ -1035,7 +1071,7 @@ __global__ void __launch_bounds__(256) k_colmarch2(DGrid g, DNamelist nl, DState
           wd = (wv - (dampwt * mass_f) * wsave) / (1.0 + dampwt);
         }
         double mfs = g.c1f[f] * muts + g.c2f[f];
-        phn = ldo(w.rhs, q) + (((((msfty * 0.5) * dts) * kG) * eps_p) * wd) / mfs;
+        phn = dv.rhs + (((((msfty * 0.5) * dts) * kG) * eps_p) * wd) / mfs;
         sto(w.ph_work, q, phn);
         sto(w.w_work, q, wd);
       }
@@ -1044,16 +1080,14 @@ __global__ void __launch_bounds__(256) k_colmarch2(DGrid g, DNamelist nl, DState
         double mass_h = g.c1h[k] * muts + g.c2h[k];
         double safe_mass = safe_pos(mass_h, 1.0e-12);
         double mu_term = g.c1h[k] * mu_work;
-        double alt = ldo(w.alt, q);
+        double alt = dv.alt;
         double al = -(alt * mu_term + g.rdnw[k] * (phn_next - phn)) / safe_mass;
-        double theta_1 = ldo(s.theta, q) - kT0;
+        double theta_1 = dv.th - kT0;
         double ttr = kT0 + theta_1;
         double safe_t = safe_pos(ttr, 1.0e-6);
-        double p = ldo(w.c2a, q) * (((alt * (ldo(w.th_work, q) - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
+        double p = dv.c2a * (((alt * (dv.thw - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
         double ptmp = p;
-        // first substep of a stage: pm1 is the step-0 pressure (calc_p_rho_wrf
-        // step 0 sets pm1 = p), read from p before it is overwritten here
-        p = p + kSmdiv * (p - ldo(first ? w.p : w.pm1, q));
+        p = p + kSmdiv * (p - dv.pm1);
         sto(w.al, q, al);
         sto(w.p, q, p);
         sto(w.pm1, q, ptmp);
@@ -1198,8 +1232,8 @@ hipError_t launch_column_tile(int tile, const DGrid& g, const DNamelist& nl, con
   if (tile == kColumnMarch) {
     const unsigned bs = 256;
     const size_t C = (size_t)g.ny * g.nx;
-    WD_TIMED(prof, KID_COLUMN, st, (k_colmarch1<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
-    WD_TIMED(prof, KID_COLUMN2, st, (k_colmarch2<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+    WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+    WD_TIMED(prof, KID_COLUMN_W, st, (k_column_w<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
     return hipGetLastError();
   }
   switch (tile) {
@@ -1233,7 +1267,7 @@ const char* kernel_name(int id) {
   static const char* names[KID_COUNT] = {
       "origin_copy", "k_ruv", "k_rom", "k_prep_col", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t",
       "k_prep_2d", "k_prep_faces", "k_prep_u", "k_prep_v", "k_col_setup",
-      "k_uv", "k_column", "k_column2", "k_fin_uv", "k_fin_col"};
+      "k_uv", "k_column", "k_column_mu", "k_column_w", "k_fin_uv", "k_fin_col"};
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 
