@@ -25,8 +25,9 @@ enum KernelId {
   KID_PREP_V,
   KID_COL_SETUP,
   KID_UV,
-  KID_COLUMN,   // the column kernel (k_column, or k_colmarch1 in the march form)
-  KID_COLUMN2,  // k_colmarch2 (march form only)
+  KID_COLUMN,     // LDS-tiled column kernel
+  KID_COLUMN_MU,  // column march, advance_mu_t kernel
+  KID_COLUMN_W,   // column march, advance_w + calc_p_rho kernel
   KID_FIN_UV,
   KID_FIN_COL,
   KID_COUNT
