@@ -17,6 +17,22 @@
  *   R_D=287, CP_D=7*R_D/2, P0=1e5, CPOVCV=CP_D/(CP_D-R_D),
  *   CVPM=-(CP_D-R_D)/CP_D, g=9.81, MIN_ACOUSTIC_ALT=0.1, re=6370e3,
  *   emdiv=0.01, smdiv=0.1 (core/acoustic.py:845-846), t0=300.
+ *
+ * How it is pinned (tests/test_oracle_golden.py, test_oracle_invariants.py):
+ * built with the reference's v0.13 constants (ORC_LEGACY_V013) it reproduces
+ * the reference-held warm-bubble golden scalars (2-D, 80x40, 5000 steps) to
+ * <= 6e-8, which pins the whole x-z large step incl. advection, PGF, prep,
+ * advance_uv/mu_t/w, Thomas, damp_opt=3, calc_p_rho and finish; the y path
+ * is pinned against it by x<->y transposition; Thomas, calc_coef_w, the EOS,
+ * flux5 and the shared pow/sin have closed-form or library checks.
+ *
+ * Parity unpinned (restated from the cited lines, checked only for internal
+ * consistency -- symmetry, transposition -- because no reference vector or
+ * closed form covers them): the terrain-dependent terms (terrain surface w
+ * in advance_w, horizontally varying base geopotential in the large-step PGF
+ * and rhs_ph), stretched-eta metrics, the w curvature term and cp = 7R/2
+ * (both newer than the golden runs), and the w_damping branch for a vertical
+ * CFL above 1.
  */
 #include "wrfdyn_oracle.h"
 #include "wrfdyn_math.h"
