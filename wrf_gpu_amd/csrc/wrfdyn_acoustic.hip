@@ -10,6 +10,17 @@ namespace wd {
 
 namespace {
 
+// Value of the lane one below (lane - 1) in the wavefront; lane 0 gets `old`.
+// DPP wave_shr:1 with bound_ctrl off leaves a destination whose source lane
+// is outside the wave unwritten, so lane 0 keeps `old`.
+__device__ __forceinline__ double wave_shr1(double old, double src) {
+  constexpr int kWaveShr1 = 0x138;
+  const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, kWaveShr1, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), kWaveShr1, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ inline void decompose(size_t idx, int nxx, int nyy, int& k, int& j, int& i) {
   i = (int)(idx % (size_t)nxx);
   size_t t = idx / (size_t)nxx;
@@ -97,6 +108,9 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
   };
   const unsigned pl8 = (unsigned)plane * 8u;
   const unsigned cc8 = (unsigned)cc * 8u, cw8 = (unsigned)cw * 8u, cs8 = (unsigned)cs * 8u;
+  // a wave is one row of BX faces (blockDim.x == BX == wavefront size)
+  static_assert(BX == 64, "k_uv maps one wavefront to one row of faces");
+  const bool row_start = threadIdx.x == 0;
   const unsigned uq8 = (unsigned)iu(g, 0, j, i) * 8u, upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u;
   const unsigned vq8 = (unsigned)iv(g, 0, j, i) * 8u, vpl8 = (unsigned)((size_t)(ny + 1) * nx) * 8u;
   auto load = [&](int k, Lv& L) {
@@ -109,12 +123,14 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     L.pc = ldo(P, ok + cc8);
     if (do_u) {
       const unsigned q = uq8 + (unsigned)k * upl8;
-      L.phw1 = ldo(w.ph_work, ok1 + cw8);
-      L.altw = ldo(w.alt, ok + cw8);
-      L.alw = ldo(w.al, ok + cw8);
-      L.pbw = ldo(w.pb, ok + cw8);
-      L.phpw = ldo(w.php, ok + cw8);
-      L.pw = ldo(P, ok + cw8);
+      if (row_start) {  // the west column is not in this wave
+        L.phw1 = ldo(w.ph_work, ok1 + cw8);
+        L.altw = ldo(w.alt, ok + cw8);
+        L.alw = ldo(w.al, ok + cw8);
+        L.pbw = ldo(w.pb, ok + cw8);
+        L.phpw = ldo(w.php, ok + cw8);
+        L.pw = ldo(P, ok + cw8);
+      }
       L.uw = ldo(w.u_work, q);
       L.ut = ldo(w.ut, q);
     }
@@ -132,15 +148,30 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     if (k + 2 < nz) {
       const unsigned o2 = ok + 2 * pl8;
       L.p2c = ldo(P, o2 + cc8);
-      L.p2w = ldo(P, o2 + cw8);
+      if (row_start) L.p2w = ldo(P, o2 + cw8);
       L.p2s = ldo(P, o2 + cs8);
     }
+  };
+  // The west column of face i is the centre column of face i-1, which the
+  // next lane down holds: each west value is that lane's centre value moved
+  // up one lane by a DPP wave shift (no memory access).  Lane 0 of the row has
+  // no lower lane; the shift leaves its destination untouched, so it keeps
+  // the value it loaded itself.
+  auto west = [&](Lv& L) {
+    L.phw1 = wave_shr1(L.phw1, L.phc1);
+    L.altw = wave_shr1(L.altw, L.altc);
+    L.alw = wave_shr1(L.alw, L.alc);
+    L.pbw = wave_shr1(L.pbw, L.pbc);
+    L.phpw = wave_shr1(L.phpw, L.phpc);
+    L.pw = wave_shr1(L.pw, L.pc);
+    L.p2w = wave_shr1(L.p2w, L.p2c);
   };
   Lv cur{};
   load(0, cur);
   for (int k = 0; k < nz; ++k) {
     Lv nxt{};
     if (k + 1 < nz) load(k + 1, nxt);
+    west(cur);
     // dpn(k+1)
     double dpn_u1, dpn_v1;
     if (k + 1 < nz) {
