@@ -88,6 +88,47 @@ int wd_create(const wd_grid *grid, const wd_namelist *nl, wd_handle **out);
  * (operational_mode.py:3319-3534 per step).  Asynchronous. */
 int wd_step(wd_handle *h, const wd_state *state, int n_large_steps, void *stream);
 
+/* Graph-captured large step.  The reference compiles its step once per
+ * (grid, namelist) and replays the XLA executable (operational_mode.py:807-905
+ * jit cache, :3319 _rk_scan_step); the analogue here records the ~80 launches
+ * of one large step into a HIP graph and replays that graph.
+ * wd_capture_graph records one large step for the buffers in `state`
+ * (replacing an earlier capture; profiling must be off).  wd_graph_step
+ * replays it n_large_steps times on `stream`, with results identical to
+ * wd_step on the same buffers.  The buffers must stay allocated while the
+ * graph is in use; wd_autotune drops the captured graph. */
+int wd_capture_graph(wd_handle *h, const wd_state *state);
+int wd_graph_step(wd_handle *h, int n_large_steps, void *stream);
+
+/* Tile-shape autotuner (no counterpart in the reference, which leaves launch
+ * shapes to XLA).  Times every compiled form of the acoustic column kernel
+ * that fits the grid's levels, in XCD-contiguous and in hardware workgroup
+ * order, over `trial_steps` large steps each on a scratch copy of `state`
+ * (allocated and freed here; the caller's state is not changed), and keeps
+ * the fastest for later steps.  Synchronises `stream`.  All forms give
+ * identical results.  wd_tuned_config reports the choice: column_tile -1 is
+ * the column-march kernel pair, t >= 0 the t-th LDS tile shape. */
+int wd_autotune(wd_handle *h, const wd_state *state, int trial_steps, void *stream);
+int wd_tuned_config(const wd_handle *h, int *column_tile, int *xcd_remap);
+
+/* Ensembles of independent members with differing grids (no counterpart in
+ * the reference, which steps one domain per call).  wd_batch_create captures
+ * one large step of every member into one HIP graph: members are spread over
+ * `branches` parallel graph branches (largest first, each to the branch with
+ * the fewest cells so far) and run back to back within a branch.
+ * wd_step_batched launches that graph n_large_steps times on `stream`; each
+ * member's result is identical to wd_step on that member alone.  Member
+ * handles must be distinct, outlive the batch and not be stepped by other
+ * calls while a batched step runs; the state buffers must stay allocated.
+ * wd_batch_branch returns the branch a member runs on. */
+typedef struct wd_batch wd_batch;
+int wd_batch_create(wd_handle *const *members, const wd_state *states, int n_members, int branches,
+                    wd_batch **out);
+int wd_step_batched(wd_batch *b, int n_large_steps, void *stream);
+int wd_batch_branch(const wd_batch *b, int member);
+const char *wd_batch_last_error(const wd_batch *b);
+void wd_batch_destroy(wd_batch *b);
+
 /* Kernel-duration instrumentation for benchmarking (no counterpart in the
  * reference, whose step is one XLA program).  Bit `id` of `kernel_mask`
  * brackets every launch of kernel `id` in subsequent wd_step calls with HIP

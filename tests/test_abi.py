@@ -74,6 +74,26 @@ def test_create_rejects_bad_arguments_without_touching_the_device():
     lib.wd_destroy(None)
 
 
+def test_graph_batch_and_autotune_entry_points_reject_bad_arguments():
+    lib = _lib.load()
+    assert lib.wd_capture_graph(None, None) == _lib.WD_EINVAL
+    assert lib.wd_graph_step(None, 1, None) == _lib.WD_EINVAL
+    assert lib.wd_autotune(None, None, 1, None) == _lib.WD_EINVAL
+    t, r = ctypes.c_int(), ctypes.c_int()
+    assert lib.wd_tuned_config(None, ctypes.byref(t), ctypes.byref(r)) == _lib.WD_EINVAL
+    b = ctypes.c_void_p()
+    assert lib.wd_batch_create(None, None, 0, 1, ctypes.byref(b)) == _lib.WD_EINVAL
+    assert "member" in _lib.last_error(None)
+    handles = (ctypes.c_void_p * 1)(None)
+    states = (_lib.WdState * 1)()
+    assert lib.wd_batch_create(handles, states, 1, 0, ctypes.byref(b)) == _lib.WD_EINVAL
+    assert "branches" in _lib.last_error(None)
+    assert lib.wd_step_batched(None, 1, None) == _lib.WD_EINVAL
+    assert lib.wd_batch_branch(None, 0) == _lib.WD_EINVAL
+    assert lib.wd_batch_last_error(None) is not None
+    lib.wd_batch_destroy(None)
+
+
 def test_namelist_validation_matches_supported_path():
     nl = Namelist(dt_s=6.0, acoustic_substeps=6)
     nl.validate()

@@ -20,7 +20,10 @@ WD_ENOMEM = -2
 WD_EDEVICE = -3
 
 EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error", "wd_destroy",
-                    "wd_profile_enable", "wd_profile_read", "wd_kernel_count", "wd_kernel_name")
+                    "wd_profile_enable", "wd_profile_read", "wd_kernel_count", "wd_kernel_name",
+                    "wd_capture_graph", "wd_graph_step", "wd_autotune", "wd_tuned_config",
+                    "wd_batch_create", "wd_step_batched", "wd_batch_branch", "wd_batch_last_error",
+                    "wd_batch_destroy")
 
 
 class WdGrid(ctypes.Structure):
@@ -86,6 +89,25 @@ def load() -> ctypes.CDLL:
         lib.wd_kernel_count.argtypes = []
         lib.wd_kernel_name.restype = ctypes.c_char_p
         lib.wd_kernel_name.argtypes = [ctypes.c_int]
+        lib.wd_capture_graph.restype = ctypes.c_int
+        lib.wd_capture_graph.argtypes = [ctypes.c_void_p, ctypes.POINTER(WdState)]
+        lib.wd_graph_step.restype = ctypes.c_int
+        lib.wd_graph_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.wd_autotune.restype = ctypes.c_int
+        lib.wd_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(WdState), ctypes.c_int, ctypes.c_void_p]
+        lib.wd_tuned_config.restype = ctypes.c_int
+        lib.wd_tuned_config.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        lib.wd_batch_create.restype = ctypes.c_int
+        lib.wd_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(WdState), ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        lib.wd_step_batched.restype = ctypes.c_int
+        lib.wd_step_batched.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.wd_batch_branch.restype = ctypes.c_int
+        lib.wd_batch_branch.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.wd_batch_last_error.restype = ctypes.c_char_p
+        lib.wd_batch_last_error.argtypes = [ctypes.c_void_p]
+        lib.wd_batch_destroy.restype = None
+        lib.wd_batch_destroy.argtypes = [ctypes.c_void_p]
         _lib = lib
     return _lib
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -28,6 +29,23 @@ struct wd_handle {
   int column_tile = 0;
   std::string err;
   wd::Prof prof;
+  // wd_capture_graph: one captured large step and the stream it is captured on
+  hipStream_t capture_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+};
+
+// An ensemble: member handles (each its own grid, namelist and workspace)
+// whose large steps are captured together into one graph, spread over
+// `branches` parallel branches (one capture stream each).
+struct wd_batch {
+  std::vector<wd_handle*> members;
+  std::vector<wd::DState> states;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;
+  std::vector<int> branch_of;  // member -> branch
+  hipGraphExec_t graph = nullptr;
+  int device = 0;
+  std::string err;
 };
 
 namespace {
@@ -44,6 +62,89 @@ int hip_fail(wd_handle* h, hipError_t e, const char* what) {
   std::string m = std::string(what) + ": " + hipGetErrorString(e);
   return fail(h, WD_EDEVICE, m);
 }
+
+int check_state(wd_handle* h, const wd_state* st, wd::DState* out) {
+  if (!st) return fail(h, WD_EINVAL, "state is NULL");
+  double* ptrs[] = {st->u, st->v, st->w, st->theta, st->p_total, st->p_pert, st->ph_total,
+                    st->ph_pert, st->mu_total, st->mu_pert, st->mudf};
+  for (double* p : ptrs)
+    if (!p) return fail(h, WD_EINVAL, "a state pointer is NULL");
+  *out = wd::DState{st->u, st->v, st->w, st->theta, st->p_total, st->p_pert,
+                    st->ph_total, st->ph_pert, st->mu_total, st->mu_pert, st->mudf};
+  return WD_OK;
+}
+
+// Enqueue n large steps of _rk_scan_step (operational_mode.py:3319-3534) on
+// stream s: no host synchronisation, allocation or memset, so the sequence
+// can also be captured into a graph (profiling records events, so it is
+// refused during capture).
+int enqueue_steps(wd_handle* h, const wd::DState& ds, int n_large_steps, hipStream_t s) {
+  const wd::DGrid& g = h->g;
+  const wd::DWork& w = h->w;
+  const size_t M = (size_t)g.nz * g.ny * g.nx, Wn = (size_t)(g.nz + 1) * g.ny * g.nx;
+  const size_t U = (size_t)g.nz * g.ny * (g.nx + 1), V = (size_t)g.nz * (g.ny + 1) * g.nx;
+  const size_t C = (size_t)g.ny * g.nx;
+  const double dt = h->nl.dt;
+  const int ns = h->nl.ns;
+  struct StageDesc {
+    int rk_step;
+    double dts;
+    int nsub;
+  };
+  const StageDesc stages[3] = {
+      {1, dt / 3.0, 1},
+      {2, dt / (double)ns, ns / 2 > 1 ? ns / 2 : 1},
+      {3, dt / (double)ns, ns},
+  };
+  for (int step = 0; step < n_large_steps; ++step) {
+    hipError_t e;
+    wd::Prof* prof = h->prof.mask ? &h->prof : nullptr;
+    const std::pair<double*, const double*> copies[] = {
+        {w.u0, ds.u}, {w.v0, ds.v}, {w.w0, ds.w}, {w.th0, ds.theta}, {w.ph0, ds.ph_pert}, {w.mu0, ds.mu_pert}};
+    const size_t sizes[] = {U, V, Wn, M, Wn, C};
+    size_t tok = prof ? prof->begin(wd::KID_ORIGIN_COPY, s) : 0;
+    for (int q = 0; q < 6; ++q) {
+      e = hipMemcpyAsync(copies[q].first, copies[q].second, sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync(origin)");
+    }
+    if (prof) prof->end(tok, s);
+    for (const StageDesc& sd : stages) {
+      e = wd::launch_stage_setup(g, h->nl, ds, w, sd.rk_step, sd.dts, s, prof);
+      if (e != hipSuccess) return hip_fail(h, e, "stage setup launch");
+      for (int it = 0; it < sd.nsub; ++it) {
+        e = wd::launch_acoustic_substep(g, h->nl, ds, w, sd.dts, it == 0, h->column_tile, s, prof);
+        if (e != hipSuccess) return hip_fail(h, e, "acoustic substep launch");
+      }
+      e = wd::launch_stage_finish(g, ds, w, s, prof);
+      if (e != hipSuccess) return hip_fail(h, e, "stage finish launch");
+    }
+  }
+  return WD_OK;
+}
+
+// Abandon an open capture on s (after a failed enqueue) without leaking the graph.
+void abort_capture(hipStream_t s) {
+  hipGraph_t partial = nullptr;
+  if (hipStreamEndCapture(s, &partial) == hipSuccess && partial) hipGraphDestroy(partial);
+}
+
+size_t state_doubles(const wd::DGrid& g, int field) {
+  const size_t nx = g.nx, ny = g.ny, nz = g.nz;
+  switch (field) {
+    case 0: return nz * ny * (nx + 1);                  // u
+    case 1: return nz * (ny + 1) * nx;                  // v
+    case 2: case 6: case 7: return (nz + 1) * ny * nx;  // w, ph_total, ph_pert
+    case 3: case 4: case 5: return nz * ny * nx;        // theta, p_total, p_pert
+    default: return ny * nx;                            // mu_total, mu_pert, mudf
+  }
+}
+
+double** state_field(wd::DState& s, int field) {
+  double** f[] = {&s.u, &s.v, &s.w, &s.theta, &s.p_total, &s.p_pert,
+                  &s.ph_total, &s.ph_pert, &s.mu_total, &s.mu_pert, &s.mudf};
+  return f[field];
+}
+constexpr int kStateFields = 11;
 
 }  // namespace
 
@@ -233,56 +334,286 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
 
 int wd_step(wd_handle* h, const wd_state* st, int n_large_steps, void* stream) {
   if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
-  if (!st) return fail(h, WD_EINVAL, "state is NULL");
   if (n_large_steps < 0) return fail(h, WD_EINVAL, "n_large_steps must be >= 0");
-  double* ptrs[] = {st->u, st->v, st->w, st->theta, st->p_total, st->p_pert, st->ph_total,
-                    st->ph_pert, st->mu_total, st->mu_pert, st->mudf};
-  for (double* p : ptrs)
-    if (!p) return fail(h, WD_EINVAL, "a state pointer is NULL");
+  wd::DState ds;
+  int rc = check_state(h, st, &ds);
+  if (rc != WD_OK) return rc;
+  return enqueue_steps(h, ds, n_large_steps, static_cast<hipStream_t>(stream));
+}
+
+int wd_capture_graph(wd_handle* h, const wd_state* st) {
+  if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  if (h->prof.mask) return fail(h, WD_EINVAL, "disable profiling (wd_profile_enable(h, 0)) before capturing");
+  wd::DState ds;
+  int rc = check_state(h, st, &ds);
+  if (rc != WD_OK) return rc;
+  hipError_t e;
+  if (!h->capture_stream) {
+    e = hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(h, e, "hipStreamCreateWithFlags");
+  }
+  e = hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(h, e, "hipStreamBeginCapture");
+  rc = enqueue_steps(h, ds, 1, h->capture_stream);
+  if (rc != WD_OK) {
+    abort_capture(h->capture_stream);
+    return rc;
+  }
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(h->capture_stream, &graph);
+  if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  h->graph = exec;
+  return WD_OK;
+}
+
+int wd_graph_step(wd_handle* h, int n_large_steps, void* stream) {
+  if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  if (!h->graph) return fail(h, WD_EINVAL, "no captured graph: call wd_capture_graph first");
+  if (n_large_steps < 0) return fail(h, WD_EINVAL, "n_large_steps must be >= 0");
+  for (int i = 0; i < n_large_steps; ++i) {
+    hipError_t e = hipGraphLaunch(h->graph, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(h, e, "hipGraphLaunch");
+  }
+  return WD_OK;
+}
+
+int wd_autotune(wd_handle* h, const wd_state* st, int trial_steps, void* stream) {
+  if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  if (trial_steps < 1) return fail(h, WD_EINVAL, "trial_steps must be >= 1");
+  if (h->prof.mask) return fail(h, WD_EINVAL, "disable profiling before autotuning");
+  wd::DState src;
+  int rc = check_state(h, st, &src);
+  if (rc != WD_OK) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  wd::DState ds{st->u, st->v, st->w, st->theta, st->p_total, st->p_pert,
-                st->ph_total, st->ph_pert, st->mu_total, st->mu_pert, st->mudf};
-  const wd::DGrid& g = h->g;
-  const wd::DWork& w = h->w;
-  const size_t M = (size_t)g.nz * g.ny * g.nx, Wn = (size_t)(g.nz + 1) * g.ny * g.nx;
-  const size_t U = (size_t)g.nz * g.ny * (g.nx + 1), V = (size_t)g.nz * (g.ny + 1) * g.nx;
-  const size_t C = (size_t)g.ny * g.nx;
-  const double dt = h->nl.dt;
-  const int ns = h->nl.ns;
-  struct StageDesc {
-    int rk_step;
-    double dts;
-    int nsub;
-  };
-  const StageDesc stages[3] = {
-      {1, dt / 3.0, 1},
-      {2, dt / (double)ns, ns / 2 > 1 ? ns / 2 : 1},
-      {3, dt / (double)ns, ns},
-  };
-  for (int step = 0; step < n_large_steps; ++step) {
-    hipError_t e;
-    wd::Prof* prof = h->prof.mask ? &h->prof : nullptr;
-    const std::pair<double*, const double*> copies[] = {
-        {w.u0, ds.u}, {w.v0, ds.v}, {w.w0, ds.w}, {w.th0, ds.theta}, {w.ph0, ds.ph_pert}, {w.mu0, ds.mu_pert}};
-    const size_t sizes[] = {U, V, Wn, M, Wn, C};
-    size_t tok = prof ? prof->begin(wd::KID_ORIGIN_COPY, s) : 0;
-    for (int q = 0; q < 6; ++q) {
-      e = hipMemcpyAsync(copies[q].first, copies[q].second, sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync(origin)");
+  // scratch copy of the state: trials advance it, the caller's state is untouched
+  size_t total = 0;
+  for (int f = 0; f < kStateFields; ++f) total += (state_doubles(h->g, f) * sizeof(double) + 255) / 256 * 256;
+  void* buf = nullptr;
+  hipError_t e = hipMalloc(&buf, total);
+  if (e != hipSuccess) return fail(h, WD_ENOMEM, std::string("hipMalloc(autotune state): ") + hipGetErrorString(e));
+  wd::DState scratch = src;
+  {
+    size_t off = 0;
+    for (int f = 0; f < kStateFields; ++f) {
+      *state_field(scratch, f) = reinterpret_cast<double*>(static_cast<char*>(buf) + off);
+      off += (state_doubles(h->g, f) * sizeof(double) + 255) / 256 * 256;
     }
-    if (prof) prof->end(tok, s);
-    for (const StageDesc& sd : stages) {
-      e = wd::launch_stage_setup(g, h->nl, ds, w, sd.rk_step, sd.dts, s, prof);
-      if (e != hipSuccess) return hip_fail(h, e, "stage setup launch");
-      for (int it = 0; it < sd.nsub; ++it) {
-        e = wd::launch_acoustic_substep(g, h->nl, ds, w, sd.dts, it == 0, h->column_tile, s, prof);
-        if (e != hipSuccess) return hip_fail(h, e, "acoustic substep launch");
+  }
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  const int keep_tile = h->column_tile, keep_remap = h->g.xcd_remap;
+  // candidates: the column-march pair and every LDS tile shape that holds nz + 1 levels, each in
+  // both workgroup orders
+  std::vector<std::pair<int, int>> cand;
+  for (int remap = 1; remap >= 0; --remap) {
+    cand.emplace_back(wd::kColumnMarch, remap);
+    for (int t = 0; t < wd::kNumColumnTiles; ++t)
+      if (h->g.nz + 1 <= wd::kColumnTiles[t].kt * wd::kMaxLevelsPerThread) cand.emplace_back(t, remap);
+  }
+  int best = -1;
+  float best_ms = 0.0f;
+  auto reset = [&]() -> hipError_t {
+    for (int f = 0; f < kStateFields; ++f) {
+      hipError_t r = hipMemcpyAsync(*state_field(scratch, f), *state_field(src, f),
+                                    state_doubles(h->g, f) * sizeof(double), hipMemcpyDeviceToDevice, s);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  };
+  e = hipEventCreate(&ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+  for (size_t c = 0; e == hipSuccess && rc == WD_OK && c < cand.size(); ++c) {
+    h->column_tile = cand[c].first;
+    h->g.xcd_remap = cand[c].second;
+    e = reset();
+    if (e != hipSuccess) break;
+    rc = enqueue_steps(h, scratch, 1, s);  // warm-up (first touch, code load)
+    if (rc != WD_OK) break;
+    e = reset();
+    if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+    if (e != hipSuccess) break;
+    rc = enqueue_steps(h, scratch, trial_steps, s);
+    if (rc != WD_OK) break;
+    e = hipEventRecord(ev[1], s);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (e == hipSuccess && (best < 0 || ms < best_ms)) {
+      best = (int)c;
+      best_ms = ms;
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  for (hipEvent_t x : ev)
+    if (x) hipEventDestroy(x);
+  hipFree(buf);
+  if (e != hipSuccess || rc != WD_OK || best < 0) {
+    h->column_tile = keep_tile;
+    h->g.xcd_remap = keep_remap;
+    if (rc != WD_OK) return rc;
+    return hip_fail(h, e, "autotune trial");
+  }
+  h->column_tile = cand[best].first;
+  h->g.xcd_remap = cand[best].second;
+  if (h->graph) {  // a captured graph holds the old launch shapes
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return WD_OK;
+}
+
+int wd_tuned_config(const wd_handle* h, int* column_tile, int* xcd_remap) {
+  if (!h || !column_tile || !xcd_remap) return WD_EINVAL;
+  *column_tile = h->column_tile;
+  *xcd_remap = h->g.xcd_remap;
+  return WD_OK;
+}
+
+void wd_batch_destroy(wd_batch* b);
+
+int wd_batch_create(wd_handle* const* members, const wd_state* states, int n_members, int branches,
+                    wd_batch** out) {
+  if (!out) return fail(nullptr, WD_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!members || !states || n_members < 1) return fail(nullptr, WD_EINVAL, "need at least one member");
+  if (branches < 1) return fail(nullptr, WD_EINVAL, "branches must be >= 1");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDevice");
+  for (int m = 0; m < n_members; ++m) {
+    if (!members[m]) return fail(nullptr, WD_EINVAL, "a member handle is NULL");
+    if (members[m]->device != dev) return fail(nullptr, WD_EINVAL, "members must live on the current device");
+    if (members[m]->prof.mask) return fail(nullptr, WD_EINVAL, "disable profiling on members before batching");
+    for (int q = 0; q < m; ++q)
+      if (members[q] == members[m]) return fail(nullptr, WD_EINVAL, "a handle appears twice (members share no workspace)");
+  }
+  wd_batch* b = new (std::nothrow) wd_batch();
+  if (!b) return fail(nullptr, WD_ENOMEM, "host allocation failed");
+  b->device = dev;
+  b->members.assign(members, members + n_members);
+  b->states.resize(n_members);
+  for (int m = 0; m < n_members; ++m) {
+    int rc = check_state(nullptr, &states[m], &b->states[m]);
+    if (rc != WD_OK) {
+      delete b;
+      return rc;
+    }
+  }
+  // Longest-processing-time assignment: members by decreasing cell count,
+  // each to the branch with the fewest cells so far.
+  const int nb = branches < n_members ? branches : n_members;
+  std::vector<int> order(n_members);
+  for (int m = 0; m < n_members; ++m) order[m] = m;
+  auto cells = [&](int m) {
+    const wd::DGrid& g = b->members[m]->g;
+    return (double)g.nx * g.ny * g.nz;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cells(a) > cells(c); });
+  std::vector<double> load(nb, 0.0);
+  b->branch_of.assign(n_members, 0);
+  for (int m : order) {
+    int best = 0;
+    for (int k = 1; k < nb; ++k)
+      if (load[k] < load[best]) best = k;
+    b->branch_of[m] = best;
+    load[best] += cells(m);
+  }
+  b->streams.assign(nb, nullptr);
+  b->events.assign(nb + 1, nullptr);  // [0] fork, [1..nb-1] joins
+  for (int k = 0; k < nb; ++k) {
+    e = hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking);
+    if (e != hipSuccess) break;
+  }
+  for (int k = 0; e == hipSuccess && k <= nb; ++k) e = hipEventCreateWithFlags(&b->events[k], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    wd_batch_destroy(b);
+    return hip_fail(nullptr, e, "batch stream/event creation");
+  }
+  // Capture one large step of every member: branch 0 forks to the others,
+  // each branch runs its members back to back, and branch 0 joins them.
+  hipStream_t s0 = b->streams[0];
+  auto capture = [&]() -> int {
+    hipError_t r = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+    if (r != hipSuccess) return hip_fail(nullptr, r, "hipStreamBeginCapture");
+    r = hipEventRecord(b->events[0], s0);
+    for (int k = 1; r == hipSuccess && k < nb; ++k) r = hipStreamWaitEvent(b->streams[k], b->events[0], 0);
+    if (r != hipSuccess) {
+      abort_capture(s0);
+      return hip_fail(nullptr, r, "capture fork");
+    }
+    for (int k = 0; k < nb; ++k)
+      for (int m = 0; m < n_members; ++m) {
+        if (b->branch_of[m] != k) continue;
+        int rc = enqueue_steps(b->members[m], b->states[m], 1, b->streams[k]);
+        if (rc != WD_OK) {
+          abort_capture(s0);
+          return fail(nullptr, rc, "member " + std::to_string(m) + ": " + b->members[m]->err);
+        }
       }
-      e = wd::launch_stage_finish(g, ds, w, s, prof);
-      if (e != hipSuccess) return hip_fail(h, e, "stage finish launch");
+    for (int k = 1; r == hipSuccess && k < nb; ++k) {
+      r = hipEventRecord(b->events[k], b->streams[k]);
+      if (r == hipSuccess) r = hipStreamWaitEvent(s0, b->events[k], 0);
+    }
+    if (r != hipSuccess) {
+      abort_capture(s0);
+      return hip_fail(nullptr, r, "capture join");
+    }
+    hipGraph_t graph = nullptr;
+    r = hipStreamEndCapture(s0, &graph);
+    if (r != hipSuccess) return hip_fail(nullptr, r, "hipStreamEndCapture");
+    r = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (r != hipSuccess) return hip_fail(nullptr, r, "hipGraphInstantiate");
+    return WD_OK;
+  };
+  int rc = capture();
+  if (rc != WD_OK) {
+    wd_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return WD_OK;
+}
+
+int wd_step_batched(wd_batch* b, int n_large_steps, void* stream) {
+  if (!b) return fail(nullptr, WD_EINVAL, "batch is NULL");
+  if (n_large_steps < 0) {
+    b->err = "n_large_steps must be >= 0";
+    return WD_EINVAL;
+  }
+  for (int i = 0; i < n_large_steps; ++i) {
+    hipError_t e = hipGraphLaunch(b->graph, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+      b->err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+      return WD_EDEVICE;
     }
   }
   return WD_OK;
+}
+
+int wd_batch_branch(const wd_batch* b, int member) {
+  if (!b || member < 0 || member >= (int)b->members.size()) return WD_EINVAL;
+  return b->branch_of[member];
+}
+
+const char* wd_batch_last_error(const wd_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+void wd_batch_destroy(wd_batch* b) {
+  if (!b) return;
+  int cur = 0;
+  if (hipGetDevice(&cur) == hipSuccess && cur != b->device) hipSetDevice(b->device);
+  (void)hipDeviceSynchronize();
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  for (hipEvent_t x : b->events)
+    if (x) hipEventDestroy(x);
+  for (hipStream_t x : b->streams)
+    if (x) hipStreamDestroy(x);
+  if (cur != b->device) hipSetDevice(cur);
+  delete b;
 }
 
 int wd_profile_enable(wd_handle* h, unsigned long long kernel_mask) {
@@ -337,6 +668,8 @@ void wd_destroy(wd_handle* h) {
   if (hipGetDevice(&cur) == hipSuccess && cur != h->device) hipSetDevice(h->device);
   // Work queued on any stream may still use the workspace.
   (void)hipDeviceSynchronize();
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  if (h->capture_stream) hipStreamDestroy(h->capture_stream);
   if (h->work_buf) hipFree(h->work_buf);
   if (h->metric_buf) hipFree(h->metric_buf);
   if (cur != h->device) hipSetDevice(cur);

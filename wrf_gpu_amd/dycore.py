@@ -104,19 +104,59 @@ class Dycore:
     def step(self, state: dict, n_steps: int = 1, stream=None) -> dict:
         """Advance ``n_steps`` large steps in place on ``stream`` (default: torch's current)."""
 
-        import torch
-
         self.check_state(state)
         if n_steps < 0:
             raise ValueError("n_steps must be >= 0")
-        if stream is None:
-            stream = torch.cuda.current_stream(state["u"].device)
-        ptrs = _lib.WdState(*[ctypes.c_void_p(state[k].data_ptr()) for k, _ in _ABI_FIELDS])
+        stream = _stream_of(state, stream)
+        ptrs = _state_ptrs(state)
         code = self._lib.wd_step(self._handle, ctypes.byref(ptrs), int(n_steps),
                                  ctypes.c_void_p(stream.cuda_stream))
         if code != _lib.WD_OK:
             raise _lib.WrfdynError(code, _lib.last_error(self._handle))
         return state
+
+    def capture_graph(self, state: dict) -> None:
+        """Record one large step on ``state``'s buffers into a HIP graph (see graph_step).
+
+        The tensors must stay alive, and keep their storage, while the graph is used.
+        """
+
+        self.check_state(state)
+        ptrs = _state_ptrs(state)
+        _lib.check(self._lib.wd_capture_graph(self._handle, ctypes.byref(ptrs)), self._handle)
+        self._graph_state = state
+
+    def graph_step(self, n_steps: int = 1, stream=None) -> dict:
+        """Replay the captured large step ``n_steps`` times; same results as ``step``."""
+
+        state = getattr(self, "_graph_state", None)
+        if state is None:
+            raise ValueError("no captured graph: call capture_graph(state) first")
+        if n_steps < 0:
+            raise ValueError("n_steps must be >= 0")
+        stream = _stream_of(state, stream)
+        _lib.check(self._lib.wd_graph_step(self._handle, int(n_steps), ctypes.c_void_p(stream.cuda_stream)),
+                   self._handle)
+        return state
+
+    def autotune(self, state: dict, trial_steps: int = 2, stream=None) -> dict:
+        """Time the column-kernel forms and workgroup orders on a copy of ``state``; keep the fastest.
+
+        Returns ``{"column_tile": t, "xcd_remap": r}`` (t = -1: column-march kernels).
+        """
+
+        self.check_state(state)
+        stream = _stream_of(state, stream)
+        ptrs = _state_ptrs(state)
+        _lib.check(self._lib.wd_autotune(self._handle, ctypes.byref(ptrs), int(trial_steps),
+                                         ctypes.c_void_p(stream.cuda_stream)), self._handle)
+        self._graph_state = None
+        return self.tuned_config()
+
+    def tuned_config(self) -> dict:
+        t, r = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.wd_tuned_config(self._handle, ctypes.byref(t), ctypes.byref(r)), self._handle)
+        return {"column_tile": t.value, "xcd_remap": r.value}
 
     def profile(self, kernels) -> None:
         """Time the named kernels (see _lib.kernel_names()) with HIP events in later steps."""
@@ -150,6 +190,79 @@ class Dycore:
             self.close()
         except Exception:
             pass
+
+
+class Ensemble:
+    """Independent members with differing grids, advanced together.
+
+    One large step of every member is captured into one HIP graph, the
+    members spread over ``branches`` parallel branches; ``step`` launches it
+    once per large step.  ``members`` is a list of (Dycore, state) pairs; the
+    handles and tensors must outlive the ensemble.  Each member's result is
+    identical to ``Dycore.step`` on that member alone.
+    """
+
+    def __init__(self, members, branches: int = 4):
+        if not members:
+            raise ValueError("an ensemble needs at least one member")
+        self.members = list(members)
+        lib = _lib.load()
+        dev = None
+        for dyn, state in self.members:
+            dyn.check_state(state)
+            dev = state["u"].device if dev is None else dev
+            if state["u"].device != dev:
+                raise ValueError("all members must live on the same device")
+        n = len(self.members)
+        handles = (ctypes.c_void_p * n)(*[d._handle for d, _ in self.members])
+        states = (_lib.WdState * n)(*[_state_ptrs(s) for _, s in self.members])
+        out = ctypes.c_void_p()
+        code = lib.wd_batch_create(handles, states, n, int(branches), ctypes.byref(out))
+        if code != _lib.WD_OK:
+            raise _lib.WrfdynError(code, _lib.last_error(None))
+        self._batch = out
+        self._lib = lib
+        self.device = dev
+
+    def branch(self, member: int) -> int:
+        return int(self._lib.wd_batch_branch(self._batch, int(member)))
+
+    def step(self, n_steps: int = 1, stream=None) -> None:
+        import torch
+
+        if n_steps < 0:
+            raise ValueError("n_steps must be >= 0")
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        code = self._lib.wd_step_batched(self._batch, int(n_steps), ctypes.c_void_p(stream.cuda_stream))
+        if code != _lib.WD_OK:
+            raise _lib.WrfdynError(code, self._lib.wd_batch_last_error(self._batch).decode())
+
+    @property
+    def cells(self) -> int:
+        return sum(d.metrics.nx * d.metrics.ny * d.metrics.nz for d, _ in self.members)
+
+    def close(self) -> None:
+        b = getattr(self, "_batch", None)
+        if b is not None and b.value:
+            self._lib.wd_batch_destroy(b)
+            self._batch = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _state_ptrs(state: dict):
+    return _lib.WdState(*[ctypes.c_void_p(state[k].data_ptr()) for k, _ in _ABI_FIELDS])
+
+
+def _stream_of(state: dict, stream):
+    import torch
+
+    return torch.cuda.current_stream(state["u"].device) if stream is None else stream
 
 
 def state_to_device(state: dict, device="cuda") -> dict:

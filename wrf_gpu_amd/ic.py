@@ -267,3 +267,38 @@ def stratified_terrain_3d(nx=433, ny=433, nz=50, dx=3000.0, p_top=5000.0, u0=10.
     nl = Namelist(dt_s=float(dt), acoustic_substeps=int(ns), zdamp=float(zdamp))
     return Case(f"terrain_{nx}x{ny}x{nz}", metrics, nl, state, 200,
                 {"seed": seed, "eta_gamma": gamma})
+
+
+def ensemble_member(nx, ny, nz=50, member=0, dx=1000.0, p_top=5000.0, u0=10.0, sigma=0.5,
+                    u_sigma=1.0, length_m=20000.0, dt=6.0, ns=6, zdamp=5000.0) -> Case:
+    """One member of the uneven-size ensemble: the flat, uniform-eta, neutral
+    theta0=300 K base state of the warm-bubble workload with seeded, 20 km
+    correlated theta' (sigma 0.5 K) and u' (sigma 1 m/s) perturbations; the
+    member's seed is 100 + member."""
+
+    seed = 100 + int(member)
+    eta = np.linspace(1.0, 0.0, nz + 1)
+    metrics = Metrics.flat(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, eta=eta, p_top=p_top)
+    mub = np.full((ny, nx), P0 - p_top)
+    pb, alb, _ = _column_base(metrics, mub, np.zeros((ny, nx)), lambda z: np.full_like(z, THETA0))
+    rng = np.random.default_rng(seed)
+    tp = _correlated_field((nz, ny, nx), sigma, length_m, dx, rng)
+    state = _balanced_state(metrics, mub=mub, pb=pb, alb=alb, theta_full=THETA0 + tp,
+                            phb_surface=np.zeros((ny, nx)), u0=u0)
+    state["u"] = state["u"] + _correlated_field(state["u"].shape, u_sigma, length_m, dx, rng)
+    nl = Namelist(dt_s=float(dt), acoustic_substeps=int(ns), zdamp=float(zdamp))
+    return Case(f"member{member}_{nx}x{ny}x{nz}", metrics, nl, state, 200, {"seed": seed})
+
+
+ENSEMBLE_SIZES = ((129, 129), (201, 201), (257, 257), (321, 321))
+
+
+def ensemble_32(nz=50, per_size=8):
+    """The 32-member uneven ensemble: ``per_size`` members each of 129^2,
+    201^2, 257^2 and 321^2 columns, ``nz`` levels (members numbered in that order)."""
+
+    out = []
+    for nx, ny in ENSEMBLE_SIZES:
+        for _ in range(per_size):
+            out.append(ensemble_member(nx, ny, nz, member=len(out)))
+    return out
