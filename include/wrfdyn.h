@@ -126,6 +126,8 @@ int wd_batch_create(wd_handle *const *members, const wd_state *states, int n_mem
                     wd_batch **out);
 int wd_step_batched(wd_batch *b, int n_large_steps, void *stream);
 int wd_batch_branch(const wd_batch *b, int member);
+int wd_batch_branches(const wd_batch *b);
+
 const char *wd_batch_last_error(const wd_batch *b);
 void wd_batch_destroy(wd_batch *b);
 

@@ -90,6 +90,7 @@ def test_graph_batch_and_autotune_entry_points_reject_bad_arguments():
     assert "branches" in _lib.last_error(None)
     assert lib.wd_step_batched(None, 1, None) == _lib.WD_EINVAL
     assert lib.wd_batch_branch(None, 0) == _lib.WD_EINVAL
+    assert lib.wd_batch_branches(None) == _lib.WD_EINVAL
     assert lib.wd_batch_last_error(None) is not None
     lib.wd_batch_destroy(None)
 

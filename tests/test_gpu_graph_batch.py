@@ -143,3 +143,4 @@ def test_autotune_leaves_state_and_results_unchanged(oracle_lib):
     _assert_bitwise(state_to_host(st), state_to_host(sp), "tuned vs default")
     tuned.close()
     plain.close()
+

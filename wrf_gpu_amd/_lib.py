@@ -22,8 +22,8 @@ WD_EDEVICE = -3
 EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error", "wd_destroy",
                     "wd_profile_enable", "wd_profile_read", "wd_kernel_count", "wd_kernel_name",
                     "wd_capture_graph", "wd_graph_step", "wd_autotune", "wd_tuned_config",
-                    "wd_batch_create", "wd_step_batched", "wd_batch_branch", "wd_batch_last_error",
-                    "wd_batch_destroy")
+                    "wd_batch_create", "wd_step_batched", "wd_batch_branch", "wd_batch_branches",
+                    "wd_batch_last_error", "wd_batch_destroy")
 
 
 class WdGrid(ctypes.Structure):
@@ -104,6 +104,8 @@ def load() -> ctypes.CDLL:
         lib.wd_step_batched.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         lib.wd_batch_branch.restype = ctypes.c_int
         lib.wd_batch_branch.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.wd_batch_branches.restype = ctypes.c_int
+        lib.wd_batch_branches.argtypes = [ctypes.c_void_p]
         lib.wd_batch_last_error.restype = ctypes.c_char_p
         lib.wd_batch_last_error.argtypes = [ctypes.c_void_p]
         lib.wd_batch_destroy.restype = None

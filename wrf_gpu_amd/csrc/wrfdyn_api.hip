@@ -43,6 +43,7 @@ struct wd_batch {
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
   std::vector<int> branch_of;  // member -> branch
+  int branches = 1;            // branches in use (<= streams.size())
   hipGraphExec_t graph = nullptr;
   int device = 0;
   std::string err;
@@ -145,6 +146,73 @@ double** state_field(wd::DState& s, int field) {
   return f[field];
 }
 constexpr int kStateFields = 11;
+
+}  // namespace
+
+namespace {
+
+// Longest-processing-time assignment of members to nb branches: members by
+// decreasing cell count, each to the branch with the fewest cells so far.
+std::vector<int> assign_branches(const std::vector<wd_handle*>& members, int nb) {
+  const int n = (int)members.size();
+  std::vector<int> order(n), branch_of(n, 0);
+  for (int m = 0; m < n; ++m) order[m] = m;
+  auto cells = [&](int m) {
+    const wd::DGrid& g = members[m]->g;
+    return (double)g.nx * g.ny * g.nz;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cells(a) > cells(c); });
+  std::vector<double> load(nb, 0.0);
+  for (int m : order) {
+    int best = 0;
+    for (int k = 1; k < nb; ++k)
+      if (load[k] < load[best]) best = k;
+    branch_of[m] = best;
+    load[best] += cells(m);
+  }
+  return branch_of;
+}
+
+// Capture one large step of every member on `states` into *out: branch 0
+// forks to the others, each branch runs its members back to back, and
+// branch 0 joins them.  Errors are reported through g_create_error.
+int capture_batch(wd_batch* b, const std::vector<wd::DState>& states, const std::vector<int>& branch_of, int nb,
+                  hipGraphExec_t* out) {
+  hipStream_t s0 = b->streams[0];
+  hipError_t r = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+  if (r != hipSuccess) return hip_fail(nullptr, r, "hipStreamBeginCapture");
+  r = hipEventRecord(b->events[0], s0);
+  for (int k = 1; r == hipSuccess && k < nb; ++k) r = hipStreamWaitEvent(b->streams[k], b->events[0], 0);
+  if (r != hipSuccess) {
+    abort_capture(s0);
+    return hip_fail(nullptr, r, "capture fork");
+  }
+  const int n = (int)b->members.size();
+  for (int k = 0; k < nb; ++k)
+    for (int m = 0; m < n; ++m) {
+      if (branch_of[m] != k) continue;
+      int rc = enqueue_steps(b->members[m], states[m], 1, b->streams[k]);
+      if (rc != WD_OK) {
+        abort_capture(s0);
+        return fail(nullptr, rc, "member " + std::to_string(m) + ": " + b->members[m]->err);
+      }
+    }
+  for (int k = 1; r == hipSuccess && k < nb; ++k) {
+    r = hipEventRecord(b->events[k], b->streams[k]);
+    if (r == hipSuccess) r = hipStreamWaitEvent(s0, b->events[k], 0);
+  }
+  if (r != hipSuccess) {
+    abort_capture(s0);
+    return hip_fail(nullptr, r, "capture join");
+  }
+  hipGraph_t graph = nullptr;
+  r = hipStreamEndCapture(s0, &graph);
+  if (r != hipSuccess) return hip_fail(nullptr, r, "hipStreamEndCapture");
+  r = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (r != hipSuccess) return hip_fail(nullptr, r, "hipGraphInstantiate");
+  return WD_OK;
+}
 
 }  // namespace
 
@@ -503,74 +571,21 @@ int wd_batch_create(wd_handle* const* members, const wd_state* states, int n_mem
       return rc;
     }
   }
-  // Longest-processing-time assignment: members by decreasing cell count,
-  // each to the branch with the fewest cells so far.
   const int nb = branches < n_members ? branches : n_members;
-  std::vector<int> order(n_members);
-  for (int m = 0; m < n_members; ++m) order[m] = m;
-  auto cells = [&](int m) {
-    const wd::DGrid& g = b->members[m]->g;
-    return (double)g.nx * g.ny * g.nz;
-  };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cells(a) > cells(c); });
-  std::vector<double> load(nb, 0.0);
-  b->branch_of.assign(n_members, 0);
-  for (int m : order) {
-    int best = 0;
-    for (int k = 1; k < nb; ++k)
-      if (load[k] < load[best]) best = k;
-    b->branch_of[m] = best;
-    load[best] += cells(m);
-  }
   b->streams.assign(nb, nullptr);
-  b->events.assign(nb + 1, nullptr);  // [0] fork, [1..nb-1] joins
+  b->events.assign(nb, nullptr);  // [0] fork, [k >= 1] join of branch k
   for (int k = 0; k < nb; ++k) {
     e = hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking);
     if (e != hipSuccess) break;
   }
-  for (int k = 0; e == hipSuccess && k <= nb; ++k) e = hipEventCreateWithFlags(&b->events[k], hipEventDisableTiming);
+  for (int k = 0; e == hipSuccess && k < nb; ++k) e = hipEventCreateWithFlags(&b->events[k], hipEventDisableTiming);
   if (e != hipSuccess) {
     wd_batch_destroy(b);
     return hip_fail(nullptr, e, "batch stream/event creation");
   }
-  // Capture one large step of every member: branch 0 forks to the others,
-  // each branch runs its members back to back, and branch 0 joins them.
-  hipStream_t s0 = b->streams[0];
-  auto capture = [&]() -> int {
-    hipError_t r = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
-    if (r != hipSuccess) return hip_fail(nullptr, r, "hipStreamBeginCapture");
-    r = hipEventRecord(b->events[0], s0);
-    for (int k = 1; r == hipSuccess && k < nb; ++k) r = hipStreamWaitEvent(b->streams[k], b->events[0], 0);
-    if (r != hipSuccess) {
-      abort_capture(s0);
-      return hip_fail(nullptr, r, "capture fork");
-    }
-    for (int k = 0; k < nb; ++k)
-      for (int m = 0; m < n_members; ++m) {
-        if (b->branch_of[m] != k) continue;
-        int rc = enqueue_steps(b->members[m], b->states[m], 1, b->streams[k]);
-        if (rc != WD_OK) {
-          abort_capture(s0);
-          return fail(nullptr, rc, "member " + std::to_string(m) + ": " + b->members[m]->err);
-        }
-      }
-    for (int k = 1; r == hipSuccess && k < nb; ++k) {
-      r = hipEventRecord(b->events[k], b->streams[k]);
-      if (r == hipSuccess) r = hipStreamWaitEvent(s0, b->events[k], 0);
-    }
-    if (r != hipSuccess) {
-      abort_capture(s0);
-      return hip_fail(nullptr, r, "capture join");
-    }
-    hipGraph_t graph = nullptr;
-    r = hipStreamEndCapture(s0, &graph);
-    if (r != hipSuccess) return hip_fail(nullptr, r, "hipStreamEndCapture");
-    r = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
-    if (r != hipSuccess) return hip_fail(nullptr, r, "hipGraphInstantiate");
-    return WD_OK;
-  };
-  int rc = capture();
+  b->branches = nb;
+  b->branch_of = assign_branches(b->members, nb);
+  int rc = capture_batch(b, b->states, b->branch_of, nb, &b->graph);
   if (rc != WD_OK) {
     wd_batch_destroy(b);
     return rc;
@@ -599,6 +614,8 @@ int wd_batch_branch(const wd_batch* b, int member) {
   if (!b || member < 0 || member >= (int)b->members.size()) return WD_EINVAL;
   return b->branch_of[member];
 }
+
+int wd_batch_branches(const wd_batch* b) { return b ? b->branches : WD_EINVAL; }
 
 const char* wd_batch_last_error(const wd_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
 

@@ -227,16 +227,24 @@ class Ensemble:
     def branch(self, member: int) -> int:
         return int(self._lib.wd_batch_branch(self._batch, int(member)))
 
-    def step(self, n_steps: int = 1, stream=None) -> None:
-        import torch
+    @property
+    def branches(self) -> int:
+        return int(self._lib.wd_batch_branches(self._batch))
 
-        if n_steps < 0:
-            raise ValueError("n_steps must be >= 0")
-        if stream is None:
-            stream = torch.cuda.current_stream(self.device)
-        code = self._lib.wd_step_batched(self._batch, int(n_steps), ctypes.c_void_p(stream.cuda_stream))
+    def _check(self, code: int) -> None:
         if code != _lib.WD_OK:
             raise _lib.WrfdynError(code, self._lib.wd_batch_last_error(self._batch).decode())
+
+    def step(self, n_steps: int = 1, stream=None) -> None:
+        if n_steps < 0:
+            raise ValueError("n_steps must be >= 0")
+        stream = self._stream(stream)
+        self._check(self._lib.wd_step_batched(self._batch, int(n_steps), ctypes.c_void_p(stream.cuda_stream)))
+
+    def _stream(self, stream):
+        import torch
+
+        return torch.cuda.current_stream(self.device) if stream is None else stream
 
     @property
     def cells(self) -> int:
