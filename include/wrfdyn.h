@@ -100,6 +100,22 @@ int wd_step(wd_handle *h, const wd_state *state, int n_large_steps, void *stream
 int wd_capture_graph(wd_handle *h, const wd_state *state);
 int wd_graph_step(wd_handle *h, int n_large_steps, void *stream);
 
+/* Arithmetic form of the step.  The reference's force_fp64 path is FP64
+ * throughout (operational_mode.py:412-683); it is WD_PREC_FP64 here, the
+ * default, bit-identical to the CPU oracle.  WD_PREC_MIXED evaluates the
+ * flux5/flux3 advection interpolants in FP32 on deviations from the cell's
+ * own value, with the mean part (value x transport divergence) added in FP64,
+ * and stores the RK tendencies of u, v, w, theta and ph in FP32.  FP64 stays
+ * in the large-step pressure gradient, calc_p_rho, advance_w's pressure and
+ * buoyancy terms and rw_tend, the tridiagonal solve and the mu/theta
+ * accumulation.  Results then differ from the FP64 form by rounding-level
+ * amounts; they are not bit-identical.  The mixed form runs the column-march
+ * kernels.  Changing the form drops a captured graph. */
+#define WD_PREC_FP64 0
+#define WD_PREC_MIXED 1
+int wd_set_precision(wd_handle *h, int mode);
+int wd_get_precision(const wd_handle *h);
+
 /* Tile-shape autotuner (no counterpart in the reference, which leaves launch
  * shapes to XLA).  Times every compiled form of the acoustic column kernel
  * that fits the grid's levels, in XCD-contiguous and in hardware workgroup
@@ -107,7 +123,8 @@ int wd_graph_step(wd_handle *h, int n_large_steps, void *stream);
  * (allocated and freed here; the caller's state is not changed), and keeps
  * the fastest for later steps.  Synchronises `stream`.  All forms give
  * identical results.  wd_tuned_config reports the choice: column_tile -1 is
- * the column-march kernel pair, t >= 0 the t-th LDS tile shape. */
+ * the column-march kernel pair, t >= 0 the t-th LDS tile shape.  In the mixed
+ * form only the workgroup order is tuned. */
 int wd_autotune(wd_handle *h, const wd_state *state, int trial_steps, void *stream);
 int wd_tuned_config(const wd_handle *h, int *column_tile, int *xcd_remap);
 

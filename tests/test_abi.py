@@ -81,6 +81,8 @@ def test_graph_batch_and_autotune_entry_points_reject_bad_arguments():
     assert lib.wd_autotune(None, None, 1, None) == _lib.WD_EINVAL
     t, r = ctypes.c_int(), ctypes.c_int()
     assert lib.wd_tuned_config(None, ctypes.byref(t), ctypes.byref(r)) == _lib.WD_EINVAL
+    assert lib.wd_set_precision(None, _lib.WD_PREC_MIXED) == _lib.WD_EINVAL
+    assert lib.wd_get_precision(None) == _lib.WD_EINVAL
     b = ctypes.c_void_p()
     assert lib.wd_batch_create(None, None, 0, 1, ctypes.byref(b)) == _lib.WD_EINVAL
     assert "member" in _lib.last_error(None)

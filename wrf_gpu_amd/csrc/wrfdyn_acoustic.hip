@@ -47,7 +47,7 @@ __device__ inline double dpn_ac(const DGrid& g, const double* __restrict__ p, in
 // sums of the work pressure are carried up the column (interior dpn(k) uses
 // levels k and k-1; the bottom and rigid-lid top faces use three levels).
 // ---------------------------------------------------------------------------
-template <int BX, int BY>
+template <int BX, int BY, class TT>
 __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const unsigned nbx = (nx + 1 + BX - 1) / BX;
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
         L.pw = ldo(P, ok + cw8);
       }
       L.uw = ldo(w.u_work, q);
-      L.ut = ldo(w.ut, q);
+      L.ut = ldt<TT>(w.ut, q);
     }
     if (do_v) {
       const unsigned q = vq8 + (unsigned)k * vpl8;
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
       L.phps = ldo(w.php, ok + cs8);
       L.ps = ldo(P, ok + cs8);
       L.vw = ldo(w.v_work, q);
-      L.vt = ldo(w.vt, q);
+      L.vt = ldt<TT>(w.vt, q);
     }
     if (k + 2 < nz) {
       const unsigned o2 = ok + 2 * pl8;
@@ -765,6 +765,7 @@ __global__ void k_fin_uv(DGrid g, DState s, DWork w) {
 // level f (stored, all levels, to w.ww) and the theta update and t_2ave at
 // k = f-1.
 // ---------------------------------------------------------------------------
+template <class TT>
 __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState s, DWork w, double dts,
                                                    int first) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -851,7 +852,7 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
       m.the = ldo(s.theta, cip + kp);
       m.thw = ldo(s.theta, cim + kp);
       m.thwk = ldo(w.th_work, q);
-      m.tht = ldo(w.tht, q);
+      m.tht = ldt<TT>(w.tht, q);
       m.t2o = first ? 0.0 : ldo(w.t2ave, q);  // t_2ave starts each stage at 0
     }
     const unsigned qf = c8 + (unsigned)f * plane;
@@ -908,6 +909,7 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
 // (stored to w.wfwd), reading omega and the new t_2ave from k_column_mu; then
 // sweep down for the back substitution, damping, the phi finish and
 // calc_p_rho (which needs the finished phi of levels k and k+1).
+template <class TT>
 __global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState s, DWork w, double dts,
                                                    int first) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -957,7 +959,7 @@ __global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState 
     if (f <= nz) {
       const unsigned q = c8 + (unsigned)f * plane;
       L.ph1t = ldo(w.ph1t, q);
-      L.pht = ldo(w.ph_tend, q);
+      L.pht = ldt<TT>(w.ph_tend, q);
       L.wold = ldo(w.w_work, q);
       L.phw = ldo(w.ph_work, q);
       L.rwt = ldo(w.rw_tend, q);
@@ -1263,10 +1265,17 @@ hipError_t launch_column_tile(int tile, const DGrid& g, const DNamelist& nl, con
   if (tile == kColumnMarch) {
     const unsigned bs = 256;
     const size_t C = (size_t)g.ny * g.nx;
-    WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
-    WD_TIMED(prof, KID_COLUMN_W, st, (k_column_w<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+    if (nl.mixed) {
+      WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<float><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+      WD_TIMED(prof, KID_COLUMN_W, st, (k_column_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+    } else {
+      WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<double><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+      WD_TIMED(prof, KID_COLUMN_W, st, (k_column_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+    }
     return hipGetLastError();
   }
+  // the LDS-tiled kernel reads FP64 tendencies only
+  if (nl.mixed) return hipErrorInvalidValue;
   switch (tile) {
     case 1: return launch_column_levels<8, 16>(g, nl, s, w, dts, first, st, prof);
     case 2: return launch_column_levels<32, 8>(g, nl, s, w, dts, first, st, prof);
@@ -1281,7 +1290,10 @@ hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DS
                                    double dts, int first, int column_tile, hipStream_t st, Prof* prof) {
   constexpr int BX = 64, BY = 4;
   const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 1) / BX) * (unsigned)((g.ny + 1 + BY - 1) / BY);
-  WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
+  if (nl.mixed)
+    WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, float><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
+  else
+    WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, double><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
   return launch_column_tile(column_tile, g, nl, s, w, dts, first, st, prof);
 }
 

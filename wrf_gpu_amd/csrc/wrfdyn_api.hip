@@ -378,6 +378,7 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     delete h;
     return hip_fail(nullptr, e, "hipMemset(workspace)");
   }
+  h->nl.mixed = 0;
   // Acoustic column kernel: the column-march pair (default) or, with
   // WRFDYN_COLUMN_TILE="TXxKT", the LDS-tiled kernel in one of its compiled
   // tile shapes (when that shape holds nz + 1 levels).
@@ -479,7 +480,7 @@ int wd_autotune(wd_handle* h, const wd_state* st, int trial_steps, void* stream)
   std::vector<std::pair<int, int>> cand;
   for (int remap = 1; remap >= 0; --remap) {
     cand.emplace_back(wd::kColumnMarch, remap);
-    for (int t = 0; t < wd::kNumColumnTiles; ++t)
+    for (int t = 0; t < wd::kNumColumnTiles && !h->nl.mixed; ++t)
       if (h->g.nz + 1 <= wd::kColumnTiles[t].kt * wd::kMaxLevelsPerThread) cand.emplace_back(t, remap);
   }
   int best = -1;
@@ -533,6 +534,21 @@ int wd_autotune(wd_handle* h, const wd_state* st, int trial_steps, void* stream)
   }
   return WD_OK;
 }
+
+int wd_set_precision(wd_handle* h, int mode) {
+  if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  if (mode != WD_PREC_FP64 && mode != WD_PREC_MIXED) return fail(h, WD_EINVAL, "precision must be WD_PREC_FP64 or WD_PREC_MIXED");
+  h->nl.mixed = mode == WD_PREC_MIXED;
+  // the mixed form runs the column-march kernels (the LDS-tiled kernel reads FP64 tendencies)
+  if (h->nl.mixed) h->column_tile = wd::kColumnMarch;
+  if (h->graph) {  // a captured graph holds the previous form's kernels
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return WD_OK;
+}
+
+int wd_get_precision(const wd_handle* h) { return h ? (h->nl.mixed ? WD_PREC_MIXED : WD_PREC_FP64) : WD_EINVAL; }
 
 int wd_tuned_config(const wd_handle* h, int* column_tile, int* xcd_remap) {
   if (!h || !column_tile || !xcd_remap) return WD_EINVAL;

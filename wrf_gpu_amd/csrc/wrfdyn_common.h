@@ -47,6 +47,7 @@ struct DNamelist {
   double epssm;
   int top_lid, w_damping, damp_opt;
   double dampcoef, zdamp;
+  int mixed;  // 1: mixed-precision form (FP32 tendencies and flux interpolants), 0: plain FP64
 };
 
 // Caller state (device pointers).
@@ -123,6 +124,27 @@ __device__ __forceinline__ void sto(double* base, unsigned byte_off, double v) {
   *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 
+// Tendency arrays are stored as T (double, or float in the mixed-precision
+// form) in slots sized for doubles.  ldt/stt take the byte offset the element
+// would have in a double array (off8 = 8 * index) and access element `index`
+// of the T array, converting to and from double.
+template <class T>
+__device__ __forceinline__ double ldt(const double* base, unsigned off8) {
+  if constexpr (sizeof(T) == sizeof(double)) {
+    return ldo(base, off8);
+  } else {
+    return (double)*reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (off8 >> 1));
+  }
+}
+template <class T>
+__device__ __forceinline__ void stt(double* base, unsigned off8, double v) {
+  if constexpr (sizeof(T) == sizeof(double)) {
+    sto(base, off8, v);
+  } else {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (off8 >> 1)) = (float)v;
+  }
+}
+
 __device__ inline int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 // jnp.sign: -1, +1, and the argument itself for +-0 and NaN.
@@ -155,6 +177,38 @@ __device__ inline double flux3(double q_km2, double q_km1, double q_k, double q_
   double f4 = (7.0 * (q_k + q_km1) - (q_kp1 + q_km2)) / 12.0;
   double corr = ((q_kp1 - q_km2) - 3.0 * (q_k - q_km1)) / 12.0;
   return f4 + sgn(velz) * corr;
+}
+
+// ---- mixed-precision flux interpolation ----
+// The flux5 and flux3 interpolants are affine with weights summing to one,
+// so interpolating q - qc and adding qc back is exact algebra.  The mixed
+// form evaluates the interpolant of the deviations q - qc (qc: the value at
+// the cell the divergence is taken for) in FP32 and keeps the mean part in
+// FP64: for the pair of faces of a cell,
+//   v1 F(q)_1 - v0 F(q)_0 = [v1 F(q-qc)_1 - v0 F(q-qc)_0]_FP32 + [qc (v1 - v0)]_FP64,
+// so FP32 rounding acts on the small deviations only, and the transport
+// divergence, which the mass flux nearly cancels, stays FP64.
+__device__ inline float sgnf(double x) { return x > 0.0 ? 1.0f : (x < 0.0 ? -1.0f : (float)x); }
+__device__ inline float flux5f(float q_im3, float q_im2, float q_im1, float q_i, float q_ip1, float q_ip2, float s) {
+  float f6 = ((37.0f * (q_i + q_im1) - 8.0f * (q_ip1 + q_im2)) + (q_ip2 + q_im3)) / 60.0f;
+  float corr = (((q_ip2 - q_im3) - 5.0f * (q_ip1 - q_im2)) + 10.0f * (q_i - q_im1)) / 60.0f;
+  return f6 - s * corr;
+}
+__device__ inline float flux3f(float q_km2, float q_km1, float q_k, float q_kp1, float s) {
+  float f4 = (7.0f * (q_k + q_km1) - (q_kp1 + q_km2)) / 12.0f;
+  float corr = ((q_kp1 - q_km2) - 3.0f * (q_k - q_km1)) / 12.0f;
+  return f4 + s * corr;
+}
+// v1 * flux5(x[1..6], v1) - v0 * flux5(x[0..5], v0) for the cell whose value
+// is x[3]; qc is x[3] in the units the interpolated quantity uses.
+__device__ inline double hdiv5_mixed(const double* x, double qc, double v0, double v1) {
+  float d[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) d[k] = (float)(x[k] - x[3]);
+  const float f0 = flux5f(d[0], d[1], d[2], d[3], d[4], d[5], sgnf(v0));
+  const float f1 = flux5f(d[1], d[2], d[3], d[4], d[5], d[6], sgnf(v1));
+  const float dev = (float)v1 * f1 - (float)v0 * f0;
+  return (double)dev + qc * (v1 - v0);
 }
 
 // post-pg_buoy_w cqw with qtot=0: 0 at faces 0 and nz, 1 inside

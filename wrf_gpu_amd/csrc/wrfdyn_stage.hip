@@ -73,6 +73,24 @@ __device__ inline double vface3(const DGrid& g, int kf, double qm2, double qm1, 
   return r * flux3(qm2, qm1, q0, qp1, -r);
 }
 
+// Mixed form: the face interpolant of vface3 applied to deviations d = q - qc
+// (the face flux is r times it); 0 at the boundary faces, where r is 0.
+__device__ inline float vface3_dev(const DGrid& g, int kf, float dm2, float dm1, float d0, float dp1, double r) {
+  const int nz = g.nz;
+  if (kf == 0 || kf == nz) return 0.0f;
+  if (kf == 1) return (float)g.fnm[1] * d0 + (float)g.fnp[1] * dm1;
+  if (kf == nz - 1) return (float)g.fnm[nz - 1] * d0 + (float)g.fnp[nz - 1] * dm1;
+  return flux3f(dm2, dm1, d0, dp1, sgnf(-r));
+}
+// Mixed form: r_hi F_hi - r_lo F_lo for a cell of value qc, from the face
+// interpolants of deviations (f_hi relative to qc, f_lo relative to the cell
+// below, whose value is qb).
+__device__ inline double vdiv_mixed(double qc, double qb, double r_lo, double r_hi, float f_lo_below, float f_hi) {
+  const float f_lo = f_lo_below + (float)(qb - qc);
+  const float dev = (float)r_hi * f_hi - (float)r_lo * f_lo;
+  return (double)dev + qc * (r_hi - r_lo);
+}
+
 // dpn face value (large-step PGF, rk_addtend_dry.py:314-327, top_lid with
 // cfn/cfn1) from the pair sums ps(k) = p(k, a) + p(k, b) of levels kk-2..kk
 // (ps0 at kk, psm1 at kk-1, psm2 at kk-2); the bottom face takes levels 0..2.
@@ -95,7 +113,10 @@ __device__ inline double dpn_ls_bot(const DGrid& g, double ps0, double ps1, doub
 // XCD-contiguous tile order keeps neighbouring rows in one L2.
 
 // advect_scalar_flux of theta-300 (flux_advection.py:482-568) on mass levels.
+// TT: the tendency storage type; float selects the mixed-precision form.
+template <class TT>
 __global__ void __launch_bounds__(256) k_tend_t(DGrid g, DState s, DWork w) {
+  constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t ncol = (size_t)ny * nx;
   const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
@@ -132,6 +153,8 @@ __global__ void __launch_bounds__(256) k_tend_t(DGrid g, DState s, DWork w) {
   double qm1 = 0.0, q0 = ldo(th, c8) - kT0, qp1 = ldo(th, pl8 + c8) - kT0, qp2 = ldo(th, 2 * pl8 + c8) - kT0;
   double rom1 = ldo(w.rom, pl8 + c8);
   double vf_lo = 0.0;  // vertical face flux at k (0 at the surface)
+  double rom0 = 0.0;   // mixed form: rom at face k, and its face interpolant (deviation from level k-1)
+  float fdev_lo = 0.0f;
   Lv cur{};
   for (int k = 0; k < nz; ++k) {
     const double qp3 = k + 3 < nz ? ldo(th, (unsigned)(k + 3) * pl8 + c8) - kT0 : 0.0;
@@ -139,30 +162,42 @@ __global__ void __launch_bounds__(256) k_tend_t(DGrid g, DState s, DWork w) {
     load(k, cur);
     auto QX = [&](int sft) { return cur.x[sft + 3] - kT0; };
     auto QY = [&](int d) { return cur.y[d + 3] - kT0; };
-    double vel0 = cur.ru0, vel1 = cur.ru1;
-    double fq0 = vel0 * flux5(QX(-3), QX(-2), QX(-1), QX(0), QX(1), QX(2), vel0);
-    double fq1 = vel1 * flux5(QX(-2), QX(-1), QX(0), QX(1), QX(2), QX(3), vel1);
-    double t = ((-msftx) * g.rdx) * (fq1 - fq0);
-    double vy0 = cur.rv0, vy1 = cur.rv1;
-    double gy0 = vy0 * flux5(QY(-3), QY(-2), QY(-1), QY(0), QY(1), QY(2), vy0);
-    double gy1 = vy1 * flux5(QY(-2), QY(-1), QY(0), QY(1), QY(2), QY(3), vy1);
-    t = t - (msftx * g.rdy) * (gy1 - gy0);
-    const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, rom1);
-    t = t - g.rdnw[k] * (vf_hi - vf_lo);
-    sto(w.tht, (unsigned)k * pl8 + c8, t);
+    double t;
+    if constexpr (MX) {
+      t = ((-msftx) * g.rdx) * hdiv5_mixed(cur.x, QX(0), cur.ru0, cur.ru1);
+      t = t - (msftx * g.rdy) * hdiv5_mixed(cur.y, QY(0), cur.rv0, cur.rv1);
+      const float fdev_hi = vface3_dev(g, k + 1, (float)(qm1 - q0), 0.0f, (float)(qp1 - q0), (float)(qp2 - q0), rom1);
+      t = t - g.rdnw[k] * vdiv_mixed(q0, qm1, rom0, rom1, fdev_lo, fdev_hi);
+      fdev_lo = fdev_hi;
+      rom0 = rom1;
+    } else {
+      double vel0 = cur.ru0, vel1 = cur.ru1;
+      double fq0 = vel0 * flux5(QX(-3), QX(-2), QX(-1), QX(0), QX(1), QX(2), vel0);
+      double fq1 = vel1 * flux5(QX(-2), QX(-1), QX(0), QX(1), QX(2), QX(3), vel1);
+      t = ((-msftx) * g.rdx) * (fq1 - fq0);
+      double vy0 = cur.rv0, vy1 = cur.rv1;
+      double gy0 = vy0 * flux5(QY(-3), QY(-2), QY(-1), QY(0), QY(1), QY(2), vy0);
+      double gy1 = vy1 * flux5(QY(-2), QY(-1), QY(0), QY(1), QY(2), QY(3), vy1);
+      t = t - (msftx * g.rdy) * (gy1 - gy0);
+      const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, rom1);
+      t = t - g.rdnw[k] * (vf_hi - vf_lo);
+      vf_lo = vf_hi;
+    }
+    stt<TT>(w.tht, (unsigned)k * pl8 + c8, t);
     qm1 = q0;
     q0 = qp1;
     qp1 = qp2;
     qp2 = qp3;
     rom1 = rom2;
-    vf_lo = vf_hi;
   }
 }
 
 // advect_w_flux + _vertical_flux_div_w with a rigid lid
 // (flux_advection.py:1277-1329, 1410-1482) on w levels (nz+1, ny, nx); the
 // transport is _mass_to_full_levels of ru/rv (flux_advection.py:1332-1348).
+template <class TT>
 __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
+  constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t ncol = (size_t)ny * nx;
   const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
@@ -204,6 +239,8 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
   double wm1 = 0.0, w0 = ldo(W, c8), wp1 = ldo(W, pl8 + c8), wp2 = ldo(W, 2 * pl8 + c8);
   double rom0 = ldo(w.rom, c8), rom1 = ldo(w.rom, pl8 + c8);
   double vfl_lo = 0.0;  // vflux(f); vflux(0) = 0
+  double vf_lo = 0.0;   // mixed form: transport at face f and its interpolant (deviation from w(f-1))
+  float fdev_lo = 0.0f;
   double ru0m = 0.0, ru1m = 0.0, rv0m = 0.0, rv1m = 0.0;  // ru/rv at mass level f-1
   Lv cur{};
   load(0, cur);
@@ -220,26 +257,49 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
     auto WX = [&](int sft) { return cur.x[sft + 3]; };
     auto WY = [&](int d) { return cur.y[d + 3]; };
     double vx0 = m2f(cur.ru0, ru0m), vx1 = m2f(cur.ru1, ru1m);
-    double fq0 = vx0 * flux5(WX(-3), WX(-2), WX(-1), WX(0), WX(1), WX(2), vx0);
-    double fq1 = vx1 * flux5(WX(-2), WX(-1), WX(0), WX(1), WX(2), WX(3), vx1);
-    double t = ((-msf) * g.rdx) * (fq1 - fq0);
+    double t;
+    if constexpr (MX) {
+      t = ((-msf) * g.rdx) * hdiv5_mixed(cur.x, WX(0), vx0, vx1);
+    } else {
+      double fq0 = vx0 * flux5(WX(-3), WX(-2), WX(-1), WX(0), WX(1), WX(2), vx0);
+      double fq1 = vx1 * flux5(WX(-2), WX(-1), WX(0), WX(1), WX(2), WX(3), vx1);
+      t = ((-msf) * g.rdx) * (fq1 - fq0);
+    }
     if (do_y) {
       double vy0 = m2f(cur.rv0, rv0m), vy1 = m2f(cur.rv1, rv1m);
-      double g0 = vy0 * flux5(WY(-3), WY(-2), WY(-1), WY(0), WY(1), WY(2), vy0);
-      double g1 = vy1 * flux5(WY(-2), WY(-1), WY(0), WY(1), WY(2), WY(3), vy1);
-      t = t - (msf * g.rdy) * (g1 - g0);
+      if constexpr (MX) {
+        t = t - (msf * g.rdy) * hdiv5_mixed(cur.y, WY(0), vy0, vy1);
+      } else {
+        double g0 = vy0 * flux5(WY(-3), WY(-2), WY(-1), WY(0), WY(1), WY(2), vy0);
+        double g1 = vy1 * flux5(WY(-2), WY(-1), WY(0), WY(1), WY(2), WY(3), vy1);
+        t = t - (msf * g.rdy) * (g1 - g0);
+      }
     }
     // vflux(f+1) from w(f-1..f+2) and VF(f+1) = 0.5 (rom(f+1) + rom(f))
     double vfl_hi = 0.0;
     const int kk = f + 1;
-    if (kk >= 1 && kk < nz) {
-      const double vf = 0.5 * (rom1 + rom0);
-      if (kk == 1) vfl_hi = (vf * 0.5) * (wp1 + w0);
-      else vfl_hi = vf * flux3(wm1, w0, wp1, wp2, -vf);
+    if constexpr (MX) {
+      double vf_hi = 0.0;
+      float fdev_hi = 0.0f;
+      if (kk >= 1 && kk < nz) {
+        vf_hi = 0.5 * (rom1 + rom0);
+        const float dp1 = (float)(wp1 - w0);
+        if (kk == 1) fdev_hi = 0.5f * dp1;
+        else fdev_hi = flux3f((float)(wm1 - w0), 0.0f, dp1, (float)(wp2 - w0), sgnf(-vf_hi));
+      }
+      if (f >= 1 && f < nz) t = t + (-g.rdn[f]) * vdiv_mixed(w0, wm1, vf_lo, vf_hi, fdev_lo, fdev_hi);
+      vf_lo = vf_hi;
+      fdev_lo = fdev_hi;
+    } else {
+      if (kk >= 1 && kk < nz) {
+        const double vf = 0.5 * (rom1 + rom0);
+        if (kk == 1) vfl_hi = (vf * 0.5) * (wp1 + w0);
+        else vfl_hi = vf * flux3(wm1, w0, wp1, wp2, -vf);
+      }
+      if (f >= 1 && f < nz) t = t + (-g.rdn[f]) * (vfl_hi - vfl_lo);
+      else t = t + 0.0;
     }
-    if (f >= 1 && f < nz) t = t + (-g.rdn[f]) * (vfl_hi - vfl_lo);
-    else t = t + 0.0;
-    sto(w.wt, (unsigned)f * pl8 + c8, t);
+    stt<TT>(w.wt, (unsigned)f * pl8 + c8, t);
     wm1 = w0;
     w0 = wp1;
     wp1 = wp2;
@@ -260,7 +320,9 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
 // advect_u_flux (flux_advection.py:1131-1202) + large_step_horizontal_pgf x
 // (rk_addtend_dry.py:329-343, 357-361) on u faces (nz, ny, nx+1).  Face nx
 // repeats face 0's advection (periodic); the PGF uses edge-padded pairs.
+template <class TT>
 __global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
+  constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t nface = (size_t)ny * (nx + 1);
   const size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
@@ -344,6 +406,8 @@ __global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
   double qm1 = 0.0, q0 = ldo(U, ua8), qp1 = ldo(U, upl8 + ua8), qp2 = ldo(U, 2 * upl8 + ua8);
   double r1 = romq(1);
   double vf_lo = 0.0;
+  double r0 = 0.0;  // mixed form: face-k transport and interpolant (deviation from level k-1)
+  float fdev_lo = 0.0f;
   // p' window at (l, r): level k (p0), k-1 (pm) ; ph' at level k
   double pl0 = ldo(PP, cl8), pr0 = ldo(PP, cr8), plm = 0.0, prm = 0.0;
   double phl0 = ldo(s.ph_pert, cl8), phr0 = ldo(s.ph_pert, cr8);
@@ -364,18 +428,33 @@ __global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
     auto UY = [&](int d) { return cur.y[d + 3]; };
     double velx0 = 0.5 * (cur.ru_a + cur.ru_m);
     double velx1 = 0.5 * (cur.ru_p + cur.ru_a);
-    double fq0 = velx0 * flux5(UX(-3), UX(-2), UX(-1), UX(0), UX(1), UX(2), velx0);
-    double fq1 = velx1 * flux5(UX(-2), UX(-1), UX(0), UX(1), UX(2), UX(3), velx1);
-    double t = ((-msf) * g.rdx) * (fq1 - fq0);
-    if (do_y) {
-      double vy0 = 0.5 * (cur.rv_a + cur.rv_m);
-      double vy1 = 0.5 * (cur.rv_pa + cur.rv_pm);
-      double g0 = vy0 * flux5(UY(-3), UY(-2), UY(-1), UY(0), UY(1), UY(2), vy0);
-      double g1 = vy1 * flux5(UY(-2), UY(-1), UY(0), UY(1), UY(2), UY(3), vy1);
-      t = t - (msf * g.rdy) * (g1 - g0);
+    double t;
+    if constexpr (MX) {
+      t = ((-msf) * g.rdx) * hdiv5_mixed(cur.x, UX(0), velx0, velx1);
+      if (do_y) {
+        double vy0 = 0.5 * (cur.rv_a + cur.rv_m);
+        double vy1 = 0.5 * (cur.rv_pa + cur.rv_pm);
+        t = t - (msf * g.rdy) * hdiv5_mixed(cur.y, UY(0), vy0, vy1);
+      }
+      const float fdev_hi = vface3_dev(g, k + 1, (float)(qm1 - q0), 0.0f, (float)(qp1 - q0), (float)(qp2 - q0), r1);
+      t = t + (-g.rdnw[k]) * vdiv_mixed(q0, qm1, r0, r1, fdev_lo, fdev_hi);
+      fdev_lo = fdev_hi;
+      r0 = r1;
+    } else {
+      double fq0 = velx0 * flux5(UX(-3), UX(-2), UX(-1), UX(0), UX(1), UX(2), velx0);
+      double fq1 = velx1 * flux5(UX(-2), UX(-1), UX(0), UX(1), UX(2), UX(3), velx1);
+      t = ((-msf) * g.rdx) * (fq1 - fq0);
+      if (do_y) {
+        double vy0 = 0.5 * (cur.rv_a + cur.rv_m);
+        double vy1 = 0.5 * (cur.rv_pa + cur.rv_pm);
+        double g0 = vy0 * flux5(UY(-3), UY(-2), UY(-1), UY(0), UY(1), UY(2), vy0);
+        double g1 = vy1 * flux5(UY(-2), UY(-1), UY(0), UY(1), UY(2), UY(3), vy1);
+        t = t - (msf * g.rdy) * (g1 - g0);
+      }
+      const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, r1);
+      t = t + (-g.rdnw[k]) * (vf_hi - vf_lo);
+      vf_lo = vf_hi;
     }
-    const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, r1);
-    t = t + (-g.rdnw[k]) * (vf_hi - vf_lo);
     // --- large-step horizontal PGF at face i ---
     const double pl1 = cur.pl1, pr1 = cur.pr1;
     double dpn_k1;
@@ -390,13 +469,12 @@ __global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
     double dpx = (((msf_u * 0.5) * g.rdx) * mass_u) * ((ph_term + p_term) + pb_term);
     double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_l + mu_r));
     dpx = dpx + ((msf_u * g.rdx) * (cur.phpr - cur.phpl)) * bracket;
-    sto(w.ut, (unsigned)k * upl8 + fo8, t + (-1.0 * dpx));
+    stt<TT>(w.ut, (unsigned)k * upl8 + fo8, t + (-1.0 * dpx));
     qm1 = q0;
     q0 = qp1;
     qp1 = qp2;
     qp2 = qp3;
     r1 = r2;
-    vf_lo = vf_hi;
     plm = pl0;
     prm = pr0;
     pl0 = pl1;
@@ -410,7 +488,9 @@ __global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
 // advect_v_flux (flux_advection.py:1205-1274) + large_step_horizontal_pgf y
 // (rk_addtend_dry.py:345-361) on v faces (nz, ny+1, nx).  Face ny repeats
 // face 0's advection (periodic); the PGF uses edge-padded pairs.
+template <class TT>
 __global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
+  constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t nface = (size_t)(ny + 1) * nx;
   const size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
@@ -489,6 +569,8 @@ __global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
   double qm1 = 0.0, q0 = ldo(U, ua8), qp1 = ldo(U, upl8 + ua8), qp2 = ldo(U, 2 * upl8 + ua8);
   double r1 = romq(1);
   double vf_lo = 0.0;
+  double r0 = 0.0;  // mixed form: face-k transport and interpolant (deviation from level k-1)
+  float fdev_lo = 0.0f;
   // p' window at (l, r): level k (p0), k-1 (pm) ; ph' at level k
   double pl0 = ldo(PP, cl8), pr0 = ldo(PP, cr8), plm = 0.0, prm = 0.0;
   double phl0 = ldo(s.ph_pert, cl8), phr0 = ldo(s.ph_pert, cr8);
@@ -509,16 +591,27 @@ __global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
     auto VY = [&](int d) { return cur.y[d + 3]; };
     double vx0 = 0.5 * (cur.ru_a + cur.ru_m);
     double vx1 = 0.5 * (cur.ru_p + cur.ru_mp);
-    double fq0 = vx0 * flux5(VX(-3), VX(-2), VX(-1), VX(0), VX(1), VX(2), vx0);
-    double fq1 = vx1 * flux5(VX(-2), VX(-1), VX(0), VX(1), VX(2), VX(3), vx1);
-    double t = ((-msfvy) * g.rdx) * (fq1 - fq0);
     double vy0 = 0.5 * (cur.rv_a + cur.rv_m);
     double vy1 = 0.5 * (cur.rv_p + cur.rv_a);
-    double g0 = vy0 * flux5(VY(-3), VY(-2), VY(-1), VY(0), VY(1), VY(2), vy0);
-    double g1 = vy1 * flux5(VY(-2), VY(-1), VY(0), VY(1), VY(2), VY(3), vy1);
-    t = t - (msfvy * g.rdy) * (g1 - g0);
-    const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, r1);
-    double vd = (-g.rdnw[k]) * (vf_hi - vf_lo);
+    double t, vd;
+    if constexpr (MX) {
+      t = ((-msfvy) * g.rdx) * hdiv5_mixed(cur.x, VX(0), vx0, vx1);
+      t = t - (msfvy * g.rdy) * hdiv5_mixed(cur.y, VY(0), vy0, vy1);
+      const float fdev_hi = vface3_dev(g, k + 1, (float)(qm1 - q0), 0.0f, (float)(qp1 - q0), (float)(qp2 - q0), r1);
+      vd = (-g.rdnw[k]) * vdiv_mixed(q0, qm1, r0, r1, fdev_lo, fdev_hi);
+      fdev_lo = fdev_hi;
+      r0 = r1;
+    } else {
+      double fq0 = vx0 * flux5(VX(-3), VX(-2), VX(-1), VX(0), VX(1), VX(2), vx0);
+      double fq1 = vx1 * flux5(VX(-2), VX(-1), VX(0), VX(1), VX(2), VX(3), vx1);
+      t = ((-msfvy) * g.rdx) * (fq1 - fq0);
+      double g0 = vy0 * flux5(VY(-3), VY(-2), VY(-1), VY(0), VY(1), VY(2), vy0);
+      double g1 = vy1 * flux5(VY(-2), VY(-1), VY(0), VY(1), VY(2), VY(3), vy1);
+      t = t - (msfvy * g.rdy) * (g1 - g0);
+      const double vf_hi = vface3(g, k + 1, qm1, q0, qp1, qp2, r1);
+      vd = (-g.rdnw[k]) * (vf_hi - vf_lo);
+      vf_lo = vf_hi;
+    }
     t = t + ratio * vd;
     // --- large-step PGF at face j (pairs south = l, north = r) ---
     const double pl1 = cur.pl1, pr1 = cur.pr1;
@@ -534,13 +627,12 @@ __global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
     double dpy = (((msf_v * 0.5) * g.rdy) * mass_v) * ((ph_term + p_term) + pb_term);
     double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_s + mu_n));
     dpy = dpy + ((msf_v * g.rdy) * (cur.phpr - cur.phpl)) * bracket;
-    sto(w.vt, (unsigned)k * upl8 + fo8, t + (-1.0 * dpy));
+    stt<TT>(w.vt, (unsigned)k * upl8 + fo8, t + (-1.0 * dpy));
     qm1 = q0;
     q0 = qp1;
     qp1 = qp2;
     qp2 = qp3;
     r1 = r2;
-    vf_lo = vf_hi;
     plm = pl0;
     prm = pr0;
     pl0 = pl1;
@@ -756,6 +848,7 @@ __global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
 // Level f-1's coupled face velocities and ph_1 totals come from registers;
 // level f+1's loads are issued before level f is computed.
 // ---------------------------------------------------------------------------
+template <class TT>
 __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t ncol = (size_t)ny * nx;
@@ -813,7 +906,7 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
   auto load = [&](int f, Lv& L) {
     const unsigned o = (unsigned)f * pl8, ou = (unsigned)f * upl8, ov = (unsigned)f * vpl8;
     L.pp = ldo(s.p_pert, o + c8);
-    L.wt = ldo(w.wt, o + c8);
+    L.wt = ldt<TT>(w.wt, o + c8);
     L.uw = ldo(s.u, ou + uw8);
     L.ue = ldo(s.u, ou + ue8);
     L.vs = ldo(s.v, ov + vs8);
@@ -840,8 +933,8 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
   double pht_m = PHT(ldo(s.ph_pert, c8), ldo(s.ph_total, c8));                // ph_1 total at f-1
   double pht_0 = PHT(ldo(s.ph_pert, pl8 + c8), ldo(s.ph_total, pl8 + c8));    // at f
   double c2a_m = ldo(w.c2a, c8);
-  sto(w.rw_tend, c8, 0.0 + ldo(w.wt, c8));
-  sto(w.ph_tend, c8, 0.0);
+  sto(w.rw_tend, c8, 0.0 + ldt<TT>(w.wt, c8));
+  stt<TT>(w.ph_tend, c8, 0.0);
   sto(w.ca, c8, 0.0);
   sto(w.calpha, c8, 1.0);
   sto(w.cgamma, c8, 0.0);
@@ -891,7 +984,7 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
       double muw = g.c1f[f] * muu_w + g.c2f[f];
       double fx = (mue * ue) * de_ + (muw * uw) * dw_;
       t = t + (-((0.25 * g.rdx) * msft_inv)) * fx;
-      sto(w.ph_tend, q, t);
+      stt<TT>(w.ph_tend, q, t);
     }
     // ---- calc_coef_w ----
     {
@@ -932,8 +1025,8 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
   const unsigned qt = c8 + (unsigned)nz * pl8;
   double rwt = (msft_inv * kG) *
                (((((cq1 * 2.0) * g.rdnw[nz - 1]) * (-p_m)) - g.c1f[nz] * mu_p) - cq2 * (g.c1f[nz] * mub + g.c2f[nz]));
-  sto(w.rw_tend, qt, rwt + ldo(w.wt, qt));
-  sto(w.ph_tend, qt, 0.0);
+  sto(w.rw_tend, qt, rwt + ldt<TT>(w.wt, qt));
+  stt<TT>(w.ph_tend, qt, 0.0);
   double top_denom_a = MH(nz - 1) * MF(nz - 1);
   double top_denom_b = MH(nz - 1) * MF(nz);
   double r2 = g.rdnw[nz - 1] * g.rdnw[nz - 1];
@@ -963,14 +1056,24 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   // k_prep_col writes the large-step diagnostics the tendency kernels read
   WD_TIMED(prof, KID_PREP_COL, st, (k_prep_col<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   const size_t CU = (size_t)g.ny * (g.nx + 1), CV = (size_t)(g.ny + 1) * g.nx;
-  WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<<<blocks_for(CU, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<<<blocks_for(CV, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+  if (nl.mixed) {
+    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float><<<blocks_for(CU, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float><<<blocks_for(CV, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+  } else {
+    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double><<<blocks_for(CU, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double><<<blocks_for(CV, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+  }
   WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
   WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_PREP_V, st, (k_prep_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts)));
+  if (nl.mixed)
+    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<float><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts)));
+  else
+    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<double><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts)));
   return hipGetLastError();
 }
 

@@ -44,10 +44,17 @@ class Dycore:
     """A compiled dycore for one (grid, namelist): the jit-cache entry analogue.
 
     ``step(state, n_steps)`` advances the state dict of float64 device
-    tensors in place.
+    tensors in place.  ``precision`` is ``"fp64"`` (the plain form,
+    bit-identical to the CPU oracle) or ``"mixed"`` (FP32 advection
+    interpolants and RK tendency storage, FP64 elsewhere; see
+    ``wd_set_precision`` in include/wrfdyn.h).
     """
 
-    def __init__(self, metrics: Metrics, namelist: Namelist):
+    PRECISIONS = {"fp64": _lib.WD_PREC_FP64, "mixed": _lib.WD_PREC_MIXED}
+
+    def __init__(self, metrics: Metrics, namelist: Namelist, precision: str = "fp64"):
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(self.PRECISIONS)}, got {precision!r}")
         namelist.validate()
         self.metrics = metrics
         self.namelist = namelist
@@ -72,6 +79,9 @@ class Dycore:
             raise _lib.WrfdynError(code, _lib.last_error(None))
         self._handle = handle
         self._lib = lib
+        if precision != "fp64":
+            _lib.check(lib.wd_set_precision(handle, self.PRECISIONS[precision]), handle)
+        self.precision = precision
 
     @property
     def workspace_bytes(self) -> int:
