@@ -14,7 +14,11 @@ algorithmic bytes per launch divided by its average launch duration, timed
 with HIP events on the launch stream over the timed region), its measured HBM
 traffic per launch (from the committed rocprofv3 counter summary, when
 present) and a CPU baseline (the oracle on a bounded sample of the same
-workload, this host's cores).
+workload, this host's cores).  The headline value is the plain FP64 form;
+"forms" adds the mixed-precision form on the same workload (throughput and
+per-field error against FP64 after --form-steps large steps, 200 = one
+forecast hour) and the 32-member uneven ensemble (members stepped one after
+another, and the batched graph in FP64 and in the mixed form).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
   python bench.py --pmc-summary FETCH_DIR WRITE_DIR --calibration-copy-mib 1024 > summary.json
@@ -77,6 +81,10 @@ def parse():
     ap.add_argument("--cpu-baseline-steps", type=int, default=8,
                     help="large steps of the oracle timed on the host (0 skips)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all cores in this process's affinity")
+    ap.add_argument("--form-steps", type=int, default=200,
+                    help="large steps of the mixed-precision form leg (error measured after them); 0 skips it")
+    ap.add_argument("--ensemble-steps", type=int, default=3,
+                    help="timed large steps per form of the 32-member ensemble leg; 0 skips it")
     ap.add_argument("--breakdown-steps", type=int, default=3,
                     help="untimed steps after the timed region with every kernel timed (0 skips)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -137,6 +145,113 @@ def traffic_for(kernel: str):
     bytes_ = k["fetch_kib"] * 1024.0 * f_scale + k["write_kib"] * 1024.0 * w_scale
     return bytes_, {"fetch_scale": round(f_scale, 3), "write_scale": round(w_scale, 3),
                     "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
+
+
+def field_errors(got: dict, ref: dict) -> dict:
+    """Per field: [max|got-ref| / max|ref|, rms(got-ref) / rms(ref)]."""
+
+    import torch
+
+    out = {}
+    for k in ref:
+        d = (got[k] - ref[k]).abs()
+        mx = float(ref[k].abs().max())
+        rms = float(ref[k].pow(2).mean().sqrt())
+        out[k] = [float(d.max()) / mx if mx > 0 else float(d.max()),
+                  float(d.pow(2).mean().sqrt()) / rms if rms > 0 else float(d.pow(2).mean().sqrt())]
+    return out
+
+
+def timed_steps(step, n: int, stream) -> float:
+    """Wall milliseconds of `step(n)` between HIP events on `stream`."""
+
+    import torch
+
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    step(n)
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b)
+
+
+def mixed_form_leg(case, steps: int, fp64_ms_per_step: float, dev) -> dict:
+    """The mixed-precision form on the same workload: throughput over `steps`
+    large steps from the initial state, and the per-field error against the
+    plain FP64 form after the same steps from the same state."""
+
+    import torch
+
+    from wrf_gpu_amd.dycore import Dycore, state_to_device
+
+    m, nl = case.metrics, case.namelist
+    stream = torch.cuda.current_stream(dev)
+    runs = {}
+    for prec in ("mixed", "fp64"):
+        dyn = Dycore(m, nl, precision=prec)
+        st = state_to_device(case.state, dev)
+        ms = timed_steps(lambda n: dyn.step(st, n), steps, stream)
+        runs[prec] = (st, ms / steps)
+        dyn.close()
+    ms_mixed = runs["mixed"][1]
+    value = m.n_cells * nl.acoustic_substeps / (ms_mixed * 1e-3)
+    errs = field_errors(runs["mixed"][0], runs["fp64"][0])
+    return {"form": "mixed", "value": value, "unit": "cell-substeps/s", "ms_per_step": ms_mixed, "steps": steps,
+            "speedup_vs_fp64": fp64_ms_per_step / ms_mixed,
+            "fp64_ms_per_step_same_run": runs["fp64"][1],
+            "error_vs_fp64_after_steps": {k: [float(f"{a:.3e}"), float(f"{b:.3e}")] for k, (a, b) in errs.items()},
+            "error_columns": ["max|mixed-fp64|/max|fp64|", "rms(mixed-fp64)/rms(fp64)"],
+            "max_rel_err": max(a for a, _ in errs.values()),
+            "note": "the plain FP64 form is bit-identical to the CPU oracle (GPU parity tests), so this is the "
+                    "error against the oracle"}
+
+
+def ensemble_leg(steps: int, dev) -> dict:
+    """32 members (8 each of 129^2, 201^2, 257^2, 321^2 columns, 50 levels):
+    members stepped one after another with the plain step, then the batched
+    graph (4 branches) in FP64 and in the mixed form."""
+
+    import torch
+
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device
+
+    t0 = time.time()
+    cases = ic.ensemble_32()
+    members = [(Dycore(c.metrics, c.namelist), state_to_device(c.state, dev)) for c in cases]
+    del cases
+    cells = sum(d.metrics.n_cells for d, _ in members)
+    ns = members[0][0].namelist.acoustic_substeps
+    print(f"[bench] ensemble of {len(members)} members built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream(dev)
+
+    def seq(n):
+        for _ in range(n):
+            for d, s in members:
+                d.step(s, 1)
+
+    out = {"workload": "32 members, 8 each of 129x129, 201x201, 257x257, 321x321, 50 levels, flat neutral base of "
+                       "the 64x64x40 warm-bubble workload, correlated theta' (0.5 K) and u' (1 m/s), seeds 100+m",
+           "n_cells": cells, "members": len(members), "steps": steps}
+    seq(1)
+    ms = timed_steps(seq, steps, stream) / steps
+    out["sequential_fp64"] = {"ms_per_step": ms, "value": cells * ns / (ms * 1e-3)}
+    for prec in ("fp64", "mixed"):
+        for d, _ in members:
+            d.set_precision(prec)
+        ens = Ensemble(members, branches=4)
+        ens.step(1)
+        ms = timed_steps(ens.step, steps, stream) / steps
+        out[f"batched_graph_{prec}"] = {"ms_per_step": ms, "value": cells * ns / (ms * 1e-3),
+                                        "branches": ens.branches}
+        ens.close()
+    base = out["sequential_fp64"]["ms_per_step"]
+    out["speedup_vs_sequential_fp64"] = {k: base / out[k]["ms_per_step"]
+                                         for k in ("batched_graph_fp64", "batched_graph_mixed")}
+    for d, _ in members:
+        d.close()
+    return out
 
 
 def main():
@@ -256,6 +371,18 @@ def main():
                          f"from the initial state, CPU oracle (C, OpenMP), {cpu_s:.1f} s"}
         print(f"[bench] cpu baseline {cpu_s:.1f}s on {threads} threads", file=sys.stderr, flush=True)
 
+    # the layered forms: mixed precision on this workload, and the batched ensemble
+    forms = {}
+    if args.form_steps > 0:
+        forms["mixed"] = mixed_form_leg(case, args.form_steps, ms_per_step, dev)
+        print(f"[bench] mixed form leg done", file=sys.stderr, flush=True)
+    if args.ensemble_steps > 0:
+        dyn.close()
+        del state
+        torch.cuda.empty_cache()
+        forms["batched_ensemble"] = ensemble_leg(args.ensemble_steps, dev)
+        print(f"[bench] ensemble leg done", file=sys.stderr, flush=True)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -279,6 +406,7 @@ def main():
                     for k, v in per_kernel.items()},
         "kernel_breakdown": breakdown,
         "cpu_baseline": cpu,
+        "forms": forms or None,
         "finite": finite,
     }
     print(json.dumps(out))

@@ -125,6 +125,15 @@ class Dycore:
             raise _lib.WrfdynError(code, _lib.last_error(self._handle))
         return state
 
+    def set_precision(self, precision: str) -> None:
+        """Switch the arithmetic form ("fp64" or "mixed") for later steps; drops a captured graph."""
+
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(self.PRECISIONS)}, got {precision!r}")
+        _lib.check(self._lib.wd_set_precision(self._handle, self.PRECISIONS[precision]), self._handle)
+        self.precision = precision
+        self._graph_state = None
+
     def capture_graph(self, state: dict) -> None:
         """Record one large step on ``state``'s buffers into a HIP graph (see graph_step).
 
