@@ -734,24 +734,30 @@ __global__ void __launch_bounds__(TX * KT, (R <= 4 ? 3 : 1))
 // ---------------------------------------------------------------------------
 __device__ inline double safe_den(double ms) { return fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12); }
 
-__global__ void k_fin_uv(DGrid g, DState s, DWork w) {
-  size_t nu = (size_t)g.nz * g.ny * (g.nx + 1);
-  size_t nv = (size_t)g.nz * (g.ny + 1) * g.nx;
-  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  int k, j, i;
-  if (idx < nu) {
-    decompose(idx, g.nx + 1, g.ny, k, j, i);
-    size_t f = i2u(g, j, i);
-    double ms = g.c1h[k] * w.muus[f] + g.c2h[k];
-    double mc = g.c1h[k] * w.muu[f] + g.c2h[k];
-    s.u[idx] = (g.msfuy[f] * w.u_work[idx] + s.u[idx] * mc) / safe_den(ms);
-  } else if (idx < nu + nv) {
-    size_t q = idx - nu;
-    decompose(q, g.nx, g.ny + 1, k, j, i);
-    size_t f = i2v(g, j, i);
-    double ms = g.c1h[k] * w.muvs[f] + g.c2h[k];
-    double mc = g.c1h[k] * w.muv[f] + g.c2h[k];
-    s.v[q] = (g.msfvx[f] * w.v_work[q] + s.v[q] * mc) / safe_den(ms);
+// One thread per face column (u faces first, then v faces) and a chunk of
+// kFinChunk levels (blockIdx.y); the 2-D factors are loaded once per chunk.
+constexpr int kFinChunk = 8;
+
+__device__ inline void fin_face_column(const DGrid& g, unsigned f, unsigned fplane8, double mus, double mu,
+                                       double msf, const double* __restrict__ xw, double* __restrict__ x) {
+  const unsigned f8 = f * 8u;
+  const int k0 = blockIdx.y * kFinChunk, k1 = k0 + kFinChunk < g.nz ? k0 + kFinChunk : g.nz;
+#pragma unroll 4
+  for (int k = k0; k < k1; ++k) {
+    const unsigned q = (unsigned)k * fplane8 + f8;
+    const double ms = g.c1h[k] * mus + g.c2h[k];
+    const double mc = g.c1h[k] * mu + g.c2h[k];
+    sto(x, q, (msf * ldo(xw, q) + ldo(x, q) * mc) / safe_den(ms));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fin_uv(DGrid g, DState s, DWork w) {
+  const unsigned ncu = (unsigned)(g.ny * (g.nx + 1)), ncv = (unsigned)((g.ny + 1) * g.nx);
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ncu) fin_face_column(g, t, ncu * 8u, w.muus[t], w.muu[t], g.msfuy[t], w.u_work, s.u);
+  else if (t < ncu + ncv) {
+    const unsigned f = t - ncu;
+    fin_face_column(g, f, ncv * 8u, w.muvs[f], w.muv[f], g.msfvx[f], w.v_work, s.v);
   }
 }
 
@@ -1299,9 +1305,10 @@ hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DS
 
 hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
-  const size_t UV = (size_t)g.nz * g.ny * (g.nx + 1) + (size_t)g.nz * (g.ny + 1) * g.nx;
+  const size_t UVcols = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;
   const size_t C = (size_t)g.ny * g.nx;
-  WD_TIMED(prof, KID_FIN_UV, st, (k_fin_uv<<<blocks_for(UV, bs), bs, 0, st>>>(g, s, w)));
+  const unsigned kchunks = (unsigned)((g.nz + kFinChunk - 1) / kFinChunk);
+  WD_TIMED(prof, KID_FIN_UV, st, (k_fin_uv<<<dim3(blocks_for(UVcols, bs), kchunks), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_FIN_COL, st, (k_fin_col<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   return hipGetLastError();
 }

@@ -20,19 +20,32 @@ __device__ inline void decompose(size_t idx, int nxx, int nyy, int& k, int& j, i
 // ---------------------------------------------------------------------------
 // couple_velocities_periodic ru/rv, dynamics/flux_advection.py:394-428
 // ---------------------------------------------------------------------------
-__global__ void k_ruv(DGrid g, const double* __restrict__ u, const double* __restrict__ v,
-                      const double* __restrict__ mu, double* __restrict__ ru, double* __restrict__ rv) {
-  size_t n = (size_t)g.nz * g.ny * g.nx;
-  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, g.nx, g.ny, k, j, i);
-  double muu = 0.5 * (mu[i2(g, j, i)] + mu[i2(g, j, wrapi(i - 1, g.nx))]);
-  double muv = 0.5 * (mu[i2(g, j, i)] + mu[i2(g, wrapi(j - 1, g.ny), i)]);
-  double mass_u = g.c1h[k] * muu + g.c2h[k];
-  double mass_v = g.c1h[k] * muv + g.c2h[k];
-  ru[idx] = mass_u * u[iu(g, k, j, i)] / g.msfuy[i2u(g, j, i)];
-  rv[idx] = mass_v * v[iv(g, k, j, i)] / g.msfvx[i2v(g, j, i)];
+// The element-wise kernels below give each thread one column (mass, u-face
+// or v-face) and a chunk of kChunk levels (blockIdx.y): the 2-D factors are
+// loaded once per chunk, the 3-D index is k * plane + column, and a wave's
+// loads cover 64 consecutive elements of one level.
+constexpr int kChunk = 8;
+
+__global__ void __launch_bounds__(256) k_ruv(DGrid g, const double* __restrict__ u, const double* __restrict__ v,
+                                             const double* __restrict__ mu, double* __restrict__ ru,
+                                             double* __restrict__ rv) {
+  const unsigned ncol = (unsigned)(g.ny * g.nx);
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int j = (int)(c / (unsigned)g.nx), i = (int)(c - (unsigned)j * (unsigned)g.nx);
+  const double muu = 0.5 * (mu[c] + mu[i2(g, j, wrapi(i - 1, g.nx))]);
+  const double muv = 0.5 * (mu[c] + mu[i2(g, wrapi(j - 1, g.ny), i)]);
+  const double msfuy = g.msfuy[i2u(g, j, i)], msfvx = g.msfvx[i2v(g, j, i)];
+  const unsigned pl8 = ncol * 8u, upl8 = (unsigned)(g.ny * (g.nx + 1)) * 8u, vpl8 = (unsigned)((g.ny + 1) * g.nx) * 8u;
+  const unsigned c8 = c * 8u, cu8 = (unsigned)i2u(g, j, i) * 8u;
+  const int k0 = blockIdx.y * kChunk, k1 = k0 + kChunk < g.nz ? k0 + kChunk : g.nz;
+#pragma unroll 4
+  for (int k = k0; k < k1; ++k) {
+    const double mass_u = g.c1h[k] * muu + g.c2h[k];
+    const double mass_v = g.c1h[k] * muv + g.c2h[k];
+    sto(ru, (unsigned)k * pl8 + c8, mass_u * ldo(u, (unsigned)k * upl8 + cu8) / msfuy);
+    sto(rv, (unsigned)k * pl8 + c8, mass_v * ldo(v, (unsigned)k * vpl8 + c8) / msfvx);
+  }
 }
 
 // calc_ww_cp omega, dynamics/flux_advection.py:430-458 (one thread per column)
@@ -704,28 +717,34 @@ __global__ void k_prep_faces(DGrid g, DWork w) {
   }
 }
 
-__global__ void k_prep_u(DGrid g, DState s, DWork w) {
-  size_t n = (size_t)g.nz * g.ny * (g.nx + 1);
-  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, g.nx + 1, g.ny, k, j, i);
-  size_t f = i2u(g, j, i);
-  double mref = g.c1h[k] * w.muus[f] + g.c2h[k];
-  double mcur = g.c1h[k] * w.muu[f] + g.c2h[k];
-  w.u_work[idx] = (mref * w.u0[idx] - mcur * s.u[idx]) / g.msfuy[f];
+// coupled work velocity (small_step_prep.py:216-394) on one face column
+// (u faces: f = j * (nx + 1) + i; v faces: f = j * nx + i)
+__device__ inline void prep_face_column(const DGrid& g, unsigned f, unsigned fplane8, double mus, double mu,
+                                        double msf, const double* __restrict__ x0, const double* __restrict__ x,
+                                        double* __restrict__ xw) {
+  const unsigned f8 = f * 8u;
+  const int k0 = blockIdx.y * kChunk, k1 = k0 + kChunk < g.nz ? k0 + kChunk : g.nz;
+#pragma unroll 4
+  for (int k = k0; k < k1; ++k) {
+    const unsigned q = (unsigned)k * fplane8 + f8;
+    const double mref = g.c1h[k] * mus + g.c2h[k];
+    const double mcur = g.c1h[k] * mu + g.c2h[k];
+    sto(xw, q, (mref * ldo(x0, q) - mcur * ldo(x, q)) / msf);
+  }
 }
 
-__global__ void k_prep_v(DGrid g, DState s, DWork w) {
-  size_t n = (size_t)g.nz * (g.ny + 1) * g.nx;
-  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  int k, j, i;
-  decompose(idx, g.nx, g.ny + 1, k, j, i);
-  size_t f = i2v(g, j, i);
-  double mref = g.c1h[k] * w.muvs[f] + g.c2h[k];
-  double mcur = g.c1h[k] * w.muv[f] + g.c2h[k];
-  w.v_work[idx] = (mref * w.v0[idx] - mcur * s.v[idx]) / g.msfvx[f];
+__global__ void __launch_bounds__(256) k_prep_u(DGrid g, DState s, DWork w) {
+  const unsigned ncol = (unsigned)(g.ny * (g.nx + 1));
+  const unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= ncol) return;
+  prep_face_column(g, f, ncol * 8u, w.muus[f], w.muu[f], g.msfuy[f], w.u0, s.u, w.u_work);
+}
+
+__global__ void __launch_bounds__(256) k_prep_v(DGrid g, DState s, DWork w) {
+  const unsigned ncol = (unsigned)((g.ny + 1) * g.nx);
+  const unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= ncol) return;
+  prep_face_column(g, f, ncol * 8u, w.muvs[f], w.muv[f], g.msfvx[f], w.v0, s.v, w.v_work);
 }
 
 // ---------------------------------------------------------------------------
@@ -1050,7 +1069,9 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const size_t C = (size_t)g.ny * g.nx;
   const size_t F = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;
   const unsigned colgrid = (unsigned)((g.nx + 63) / 64) * (unsigned)g.ny;
-  WD_TIMED(prof, KID_RUV, st, (k_ruv<<<blocks_for(M, bs), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
+  const unsigned kchunks = (unsigned)((g.nz + kChunk - 1) / kChunk);
+  WD_TIMED(prof, KID_RUV, st,
+           (k_ruv<<<dim3(blocks_for(C, bs), kchunks), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
   WD_TIMED(prof, KID_ROM, st, (k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom)));
   WD_TIMED(prof, KID_PREP_2D, st, (k_prep_2d<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w, rk_step)));
   // k_prep_col writes the large-step diagnostics the tendency kernels read
@@ -1068,8 +1089,8 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
     WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   }
   WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
-  WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<blocks_for(U, bs), bs, 0, st>>>(g, s, w)));
-  WD_TIMED(prof, KID_PREP_V, st, (k_prep_v<<<blocks_for(V, bs), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<dim3(blocks_for(CU, bs), kchunks), bs, 0, st>>>(g, s, w)));
+  WD_TIMED(prof, KID_PREP_V, st, (k_prep_v<<<dim3(blocks_for(CV, bs), kchunks), bs, 0, st>>>(g, s, w)));
   if (nl.mixed)
     WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<float><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts)));
   else
