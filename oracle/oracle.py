@@ -86,6 +86,7 @@ def lib(legacy_v013: bool = False) -> ctypes.CDLL:
         _lib.orc_p_from_theta_alt.restype = ctypes.c_double
         _lib.orc_p_from_theta_alt.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_advect_scalar.restype = ctypes.c_int
+        _lib.orc_surface_w.restype = ctypes.c_int
         _lib.orc_pow.restype = ctypes.c_double
         _lib.orc_pow.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_sin.restype = ctypes.c_double
@@ -191,3 +192,15 @@ def advect_scalar(metrics, u, v, mu_total, field) -> np.ndarray:
     if rc != 0:
         raise RuntimeError("orc_advect_scalar failed")
     return tend
+
+
+def surface_w(metrics, phb_surface, u, v) -> np.ndarray:
+    """Terrain surface w of advance_w for every column (advance_w.py:505-556)."""
+    L = lib()
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    out = np.zeros((metrics.ny, metrics.nx))
+    rc = L.orc_surface_w(ctypes.byref(g), keep(phb_surface), keep(u), keep(v), _ptr(out))
+    if rc != 0:
+        raise RuntimeError("orc_surface_w failed")
+    return out

@@ -24,15 +24,17 @@
  * <= 6e-8, which pins the whole x-z large step incl. advection, PGF, prep,
  * advance_uv/mu_t/w, Thomas, damp_opt=3, calc_p_rho and finish; the y path
  * is pinned against it by x<->y transposition; Thomas, calc_coef_w, the EOS,
- * flux5 and the shared pow/sin have closed-form or library checks.
+ * flux5 and the shared pow/sin have closed-form or library checks; the
+ * terrain surface w (surface_w) converges at second order to the kinematic
+ * condition u dh/dx + v dh/dy on stretched eta, which also pins the cf1..cf3
+ * surface extrapolation.
  *
  * Parity unpinned (restated from the cited lines, checked only for internal
  * consistency -- symmetry, transposition -- because no reference vector or
- * closed form covers them): the terrain-dependent terms (terrain surface w
- * in advance_w, horizontally varying base geopotential in the large-step PGF
- * and rhs_ph), stretched-eta metrics, the w curvature term and cp = 7R/2
- * (both newer than the golden runs), and the w_damping branch for a vertical
- * CFL above 1.
+ * closed form covers them): horizontally varying base geopotential in the
+ * large-step PGF and rhs_ph, the stretched-eta metrics other than cf1..cf3
+ * inside the large step, the w curvature term and cp = 7R/2 (both newer than
+ * the golden runs), and the w_damping branch for a vertical CFL above 1.
  */
 #include "wrfdyn_oracle.h"
 #include "wrfdyn_math.h"
@@ -701,6 +703,37 @@ typedef struct {
     double *tmp_ph_total;
 } ws_t;
 
+/* Terrain surface w of advance_w (core/advance_w.py:505-556): the kinematic
+ * condition from u and v extrapolated to the surface (cf1..cf3 over mass
+ * levels 0..2) and the edge-clamped terrain height differences, the height
+ * being the surface base geopotential phb0 / g.  phb0 is the level-0 plane
+ * (ny*nx). */
+static double surface_w(const orc_grid *g, const double *phb0, const double *u, const double *v, int j, int i) {
+    const int nx = g->nx, ny = g->ny;
+    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const double msftx = g->msftx[I2(j, i)], msfty = g->msfty[I2(j, i)];
+    int jn = clampi(j + 1, 0, ny - 1), js = clampi(j - 1, 0, ny - 1);
+    int ie = clampi(i + 1, 0, nx - 1), iw = clampi(i - 1, 0, nx - 1);
+    double ht = phb0[I2(j, i)] / G;
+    double ht_n = phb0[I2(jn, i)] / G, ht_s = phb0[I2(js, i)] / G;
+    double ht_e = phb0[I2(j, ie)] / G, ht_w = phb0[I2(j, iw)] / G;
+    double dy_n = ht_n - ht, dy_s = ht - ht_s, dx_e = ht_e - ht, dx_w = ht - ht_w;
+    double v_cf_n = (g->cf1 * v[IV(0, j + 1, i)] + g->cf2 * v[IV(1, j + 1, i)]) + g->cf3 * v[IV(2, j + 1, i)];
+    double v_cf_s = (g->cf1 * v[IV(0, j, i)] + g->cf2 * v[IV(1, j, i)]) + g->cf3 * v[IV(2, j, i)];
+    double u_cf_e = (g->cf1 * u[IU(0, j, i + 1)] + g->cf2 * u[IU(1, j, i + 1)]) + g->cf3 * u[IU(2, j, i + 1)];
+    double u_cf_w = (g->cf1 * u[IU(0, j, i)] + g->cf2 * u[IU(1, j, i)]) + g->cf3 * u[IU(2, j, i)];
+    return ((msfty * 0.5) * rdy) * (dy_n * v_cf_n + dy_s * v_cf_s) +
+           ((msftx * 0.5) * rdx) * (dx_e * u_cf_e + dx_w * u_cf_w);
+}
+
+int orc_surface_w(const orc_grid *g, const double *phb0, const double *u, const double *v, double *out) {
+    if (!g || !phb0 || !u || !v || !out || g->nz < 3) return -1;
+    const int nx = g->nx;
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) out[I2(j, i)] = surface_w(g, phb0, u, v, j, i);
+    return 0;
+}
+
 static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 
 static int ws_alloc(ws_t *w, int nx, int ny, int nz) {
@@ -1205,23 +1238,7 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
                 }
                 if (nl->top_lid) rhs[nz] = 0.0;
                 /* surface w from terrain */
-                double w_surface;
-                {
-                    int jn = clampi(j + 1, 0, ny - 1), js = clampi(j - 1, 0, ny - 1);
-                    int ie = clampi(i + 1, 0, nx - 1), iw = clampi(i - 1, 0, nx - 1);
-                    double ht = W->phb[IM(0, j, i)] / G;
-                    double ht_n = W->phb[IM(0, jn, i)] / G, ht_s = W->phb[IM(0, js, i)] / G;
-                    double ht_e = W->phb[IM(0, j, ie)] / G, ht_w = W->phb[IM(0, j, iw)] / G;
-                    double dy_n = ht_n - ht, dy_s = ht - ht_s, dx_e = ht_e - ht, dx_w = ht - ht_w;
-                    double v_cf_n = (g->cf1 * S->v[IV(0, j + 1, i)] + g->cf2 * S->v[IV(1, j + 1, i)]) +
-                                    g->cf3 * S->v[IV(2, j + 1, i)];
-                    double v_cf_s = (g->cf1 * S->v[IV(0, j, i)] + g->cf2 * S->v[IV(1, j, i)]) + g->cf3 * S->v[IV(2, j, i)];
-                    double u_cf_e = (g->cf1 * S->u[IU(0, j, i + 1)] + g->cf2 * S->u[IU(1, j, i + 1)]) +
-                                    g->cf3 * S->u[IU(2, j, i + 1)];
-                    double u_cf_w = (g->cf1 * S->u[IU(0, j, i)] + g->cf2 * S->u[IU(1, j, i)]) + g->cf3 * S->u[IU(2, j, i)];
-                    w_surface = ((msfty * 0.5) * rdy) * (dy_n * v_cf_n + dy_s * v_cf_s) +
-                                ((msftx * 0.5) * rdx) * (dx_e * u_cf_e + dx_w * u_cf_w);
-                }
+                double w_surface = surface_w(g, W->phb, S->u, S->v, j, i);
                 /* explicit w */
                 for (int f = 0; f <= nz; ++f) {
                     double rw = W->rw_tend[IM(f, j, i)];

@@ -70,6 +70,9 @@ double orc_alt_from_theta_p(double theta, double p);
 double orc_p_from_theta_alt(double theta, double alt);
 int orc_advect_scalar(const orc_grid *g, const double *u, const double *v,
                       const double *mu_total, const double *field, double *tend);
+/* Terrain surface w (advance_w.py:505-556) for every column: phb0 is the
+ * surface base geopotential (ny*nx), u and v the staggered 3-D winds. */
+int orc_surface_w(const orc_grid *g, const double *phb0, const double *u, const double *v, double *out);
 double orc_pow(double x, double y);
 double orc_sin(double x);
 void orc_set_threads(int n);

@@ -262,3 +262,57 @@ def test_deterministic_pow_sin_within_libm_accuracy(oracle_lib):
             continue
         worst = max(worst, abs(got - ref) / np.spacing(abs(ref)))
     assert worst <= 2.0, worst
+
+
+def _surface_case(n, a_u=8.0, b_u=-3.0, a_v=-2.0, b_v=5.0, nz=10, gamma=1.4, length=100000.0, height=500.0):
+    """Smooth terrain h = H sin(2 pi x/L) cos(2 pi y/L) on stretched eta, with
+    u and v linear in eta at the mass levels (horizontally uniform)."""
+
+    eta = 1.0 - (np.arange(nz + 1) / nz) ** gamma
+    d = length / n
+    m = Metrics.flat(nx=n, ny=n, nz=nz, dx=d, dy=d, eta=eta, p_top=5000.0)
+    x = (np.arange(n) + 0.5) * d
+    kx = 2.0 * np.pi / length
+    h = height * np.sin(kx * x)[None, :] * np.cos(kx * x)[:, None]
+    dhdx = height * kx * np.cos(kx * x)[None, :] * np.cos(kx * x)[:, None]
+    dhdy = -height * kx * np.sin(kx * x)[None, :] * np.sin(kx * x)[:, None]
+    eta_m = 0.5 * (eta[:-1] + eta[1:])
+    u = np.broadcast_to((a_u + b_u * eta_m)[:, None, None], (nz, n, n + 1)).copy()
+    v = np.broadcast_to((a_v + b_v * eta_m)[:, None, None], (nz, n + 1, n)).copy()
+    us, vs = a_u + b_u * eta[0], a_v + b_v * eta[0]  # the surface is eta = 1
+    return m, 9.81 * h, u, v, us * dhdx + vs * dhdy
+
+
+def test_terrain_surface_w_converges_to_kinematic_condition(oracle_lib):
+    """advance_w's terrain surface w (advance_w.py:505-556) is the kinematic
+    condition w = u dh/dx + v dh/dy: with u, v extrapolated to the surface by
+    cf1..cf3 (exact for profiles linear in eta, stretched eta here) and
+    centred terrain differences, it converges to the analytic value at second
+    order away from the edge-clamped rows and columns.  A wrong sign, index,
+    factor 1/2, swapped direction or extrapolation level breaks the order or
+    the limit."""
+
+    errs = []
+    for n in (24, 48, 96):
+        m, phb0, u, v, exact = _surface_case(n)
+        w = oracle_lib.surface_w(m, phb0, u, v)
+        inner = (slice(1, n - 1), slice(1, n - 1))
+        errs.append(float(np.max(np.abs(w[inner] - exact[inner]))) / float(np.max(np.abs(exact))))
+    print("surface w rel errors", errs)
+    assert errs[-1] < 2e-3
+    for coarse, fine in zip(errs, errs[1:]):
+        assert 3.6 < coarse / fine < 4.4
+
+
+def test_terrain_surface_w_edges_are_one_sided(oracle_lib):
+    """The reference clamps the terrain neighbours at the domain edges, so the
+    edge columns use a one-sided difference: (h(1) - h(0)) / (2 dx) times the
+    surface wind."""
+
+    n = 24
+    m, phb0, u, v, _ = _surface_case(n, a_v=0.0, b_v=0.0)
+    w = oracle_lib.surface_w(m, phb0, u, v)
+    h = phb0 / 9.81
+    us = 8.0 - 3.0 * 1.0
+    np.testing.assert_allclose(w[:, 0], us * (h[:, 1] - h[:, 0]) / (2.0 * m.dx), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(w[:, -1], us * (h[:, -1] - h[:, -2]) / (2.0 * m.dx), rtol=1e-12, atol=1e-15)
