@@ -110,3 +110,69 @@ def test_column_kernel_variants(oracle_lib, monkeypatch, column):
     case = ic.stratified_terrain_3d(nx=20, ny=18, nz=50, dx=3000.0, u_sigma=1.0, sigma=0.5)
     got, ref = _run_both(case, 6, oracle_lib)
     _assert_parity(got, ref, case.state)
+
+
+def test_zero_steps_leave_state_untouched():
+    import torch
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import Dycore, state_to_device, state_to_host
+
+    case = ic.bubble_3d(nx=12, ny=10, nz=8, dx=1000.0)
+    dyn = Dycore(case.metrics, case.namelist)
+    st = state_to_device(case.state)
+    dyn.step(st, 0)
+    torch.cuda.synchronize()
+    got = state_to_host(st)
+    for k in case.state:
+        assert np.array_equal(got[k], case.state[k]), k
+    dyn.close()
+
+
+def test_streams_and_concurrent_handles(oracle_lib):
+    """Steps enqueued on non-default streams, two handles at once on two
+    streams, equal the oracle; the handles share nothing."""
+
+    import torch
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import Dycore, state_to_device, state_to_host
+
+    cases = [ic.stratified_terrain_3d(nx=24, ny=20, nz=12, dx=3000.0, u_sigma=1.0, sigma=0.5),
+             ic.bubble_3d(nx=20, ny=16, nz=10, dx=1000.0, u0=8.0)]
+    print("[streams] two handles on two streams", flush=True)
+    runs = []
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for c, s in zip(cases, streams):
+        d = Dycore(c.metrics, c.namelist)
+        st = state_to_device(c.state)
+        torch.cuda.synchronize()
+        d.step(st, 3, stream=s)
+        runs.append((d, st))
+    torch.cuda.synchronize()
+    for (d, st), c in zip(runs, cases):
+        ref = oracle_lib.run(c.metrics, c.namelist, c.state, 3)
+        got = state_to_host(st)
+        for k in FIELDS:
+            assert np.array_equal(got[k], ref[k]), k
+        d.close()
+
+
+def test_levels_beyond_lds_tile_limit(oracle_lib):
+    """130 levels: more than any LDS tile shape holds (128), so the column
+    march must serve it; the autotuner may then only pick the march."""
+
+    import torch
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import Dycore, state_to_device, state_to_host
+
+    case = ic.stratified_terrain_3d(nx=12, ny=10, nz=130, dx=3000.0, u_sigma=1.0, sigma=0.5)
+    print("[levels] 12x10x130", flush=True)
+    dyn = Dycore(case.metrics, case.namelist)
+    st = state_to_device(case.state)
+    cfg = dyn.autotune(st, trial_steps=1)
+    assert cfg["column_tile"] == -1
+    dyn.step(st, 2)
+    torch.cuda.synchronize()
+    got = state_to_host(st)
+    dyn.close()
+    ref = oracle_lib.run(case.metrics, case.namelist, case.state, 2)
+    _assert_parity(got, ref, case.state)
