@@ -769,7 +769,9 @@ __global__ void __launch_bounds__(256) k_fin_uv(DGrid g, DState s, DWork w) {
 // k_column_mu (advance_mu_t): sweep up for dvdxi and the dmdt sum, the mu /
 // muts / muave updates, then a second sweep up for the omega recurrence at
 // level f (stored, all levels, to w.ww) and the theta update and t_2ave at
-// k = f-1.
+// k = f-1.  Sweep 1 stores dvdxi per level in w.rhs, which k_column_w
+// overwrites at every level before it reads it, and sweep 2 reads it back
+// instead of recomputing it from four u and v faces.
 // ---------------------------------------------------------------------------
 template <class TT>
 __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState s, DWork w, double dts,
@@ -811,6 +813,7 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
                                ldo(s.v, vs8 + kv), ldo(w.u_work, ue8 + ku), ldo(w.u_work, uw8 + ku),
                                ldo(s.u, ue8 + ku), ldo(s.u, uw8 + ku));
     dmdt = (k == 0) ? g.dnw[0] * dv : dmdt + g.dnw[k] * dv;
+    sto(w.rhs, c8 + (unsigned)k * plane, dv);  // read back by sweep 2
   }
   const double p2_ww0 = w.ww[c];
   const double p2_wwtop = ldo(w.ww, c8 + (unsigned)nz * plane);
@@ -832,7 +835,7 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
   const unsigned cjp = (unsigned)i2(g, jp, i) * 8u, cjm = (unsigned)i2(g, jm, i) * 8u;
   const unsigned cip = (unsigned)i2(g, j, ip) * 8u, cim = (unsigned)i2(g, j, imn) * 8u;
   struct Mv {  // mass level data
-    double vwn, vws, uwe, uww, vn, vs, ue, uw;
+    double vwn, vws, uwe, uww, dv;
     double thc, thn, ths, the, thw, thwk, tht, t2o;
   };
   Mv m_1{};                         // level f-1
@@ -848,10 +851,7 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
       m.vws = ldo(w.v_work, vs8 + kv);
       m.uwe = ldo(w.u_work, ue8 + ku);
       m.uww = ldo(w.u_work, uw8 + ku);
-      m.vn = ldo(s.v, vn8 + kv);
-      m.vs = ldo(s.v, vs8 + kv);
-      m.ue = ldo(s.u, ue8 + ku);
-      m.uw = ldo(s.u, uw8 + ku);
+      m.dv = ldo(w.rhs, q);
       m.thc = ldo(s.theta, q);
       m.thn = ldo(s.theta, cjp + kp);
       m.ths = ldo(s.theta, cjm + kp);
@@ -863,7 +863,7 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
     }
     const unsigned qf = c8 + (unsigned)f * plane;
     const double rom = f < nz ? ldo(w.rom, qf) : 0.0;
-    const double dv = f < nz ? dvdxi_at(f, m.vwn, m.vws, m.vn, m.vs, m.uwe, m.uww, m.ue, m.uw) : 0.0;
+    const double dv = f < nz ? m.dv : 0.0;
     // omega at f (levels 0 and nz carry to the next substep; the interior is
     // rebuilt each substep and read by k_column_w)
     double b0;

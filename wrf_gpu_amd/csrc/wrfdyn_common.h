@@ -77,6 +77,7 @@ struct DWork {
   // acoustic carried (ww: only levels 0 and nz carry between substeps)
   double *ww, *t2ave;
   // column-march scratch: phi-equation rhs and the Thomas forward sweep
+  // (k_column_w); rhs also carries dvdxi between k_column_mu's two sweeps
   double *rhs, *wfwd;
 };
 
