@@ -43,7 +43,9 @@ __device__ inline double dpn_ac(const DGrid& g, const double* __restrict__ p, in
 // advance_uv_wrf, core/acoustic.py:626-786 (emdiv = 0.01).
 // One thread per (j, i) face column of the (ny+1) x (nx+1) face grid marches
 // up k and advances u at face (j, i) when j < ny and v at face (j, i) when
-// i < nx; the mass column (j, i) is shared by both stencils.  The dpn pair
+// i < nx; the mass column (j, i) is shared by both stencils.  A block is 64
+// faces by BY rows, one wave per row: the west column comes from the lane
+// below (DPP) and the south column from the wave below (LDS).  The dpn pair
 // sums of the work pressure are carried up the column (interior dpn(k) uses
 // levels k and k-1; the bottom and rigid-lid top faces use three levels).
 // ---------------------------------------------------------------------------
@@ -54,9 +56,11 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
   const unsigned t = tile_id(g);
   const int i = (int)(t % nbx) * BX + threadIdx.x;
   const int j = (int)(t / nbx) * BY + threadIdx.y;
-  if (i > nx || j > ny) return;
-  const bool do_u = j < ny;
-  const bool do_v = i < nx;
+  // threads past the face grid stay resident for the per-level barrier; their
+  // mass-column indices are clamped and their face work is switched off
+  const bool active = i <= nx && j <= ny;
+  const bool do_u = active && j < ny;
+  const bool do_v = active && i < nx;
   const size_t plane = (size_t)ny * nx;
   // mass columns: centre (clamped), west and south neighbours (edge padded)
   const int ic = clampi(i, 0, nx - 1), iw = clampi(i - 1, 0, nx - 1);
@@ -111,6 +115,9 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
   // a wave is one row of BX faces (blockDim.x == BX == wavefront size)
   static_assert(BX == 64, "k_uv maps one wavefront to one row of faces");
   const bool row_start = threadIdx.x == 0;
+  // row j - 1 is in this block unless this is its first row or j = 0 (whose
+  // south column is clamped to row 0 itself)
+  const bool row_below_outside = threadIdx.y == 0 || j == 0;
   const unsigned uq8 = (unsigned)iu(g, 0, j, i) * 8u, upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u;
   const unsigned vq8 = (unsigned)iv(g, 0, j, i) * 8u, vpl8 = (unsigned)((size_t)(ny + 1) * nx) * 8u;
   auto load = [&](int k, Lv& L) {
@@ -136,12 +143,14 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     }
     if (do_v) {
       const unsigned q = vq8 + (unsigned)k * vpl8;
-      L.phs1 = ldo(w.ph_work, ok1 + cs8);
-      L.alts = ldo(w.alt, ok + cs8);
-      L.als = ldo(w.al, ok + cs8);
-      L.pbs = ldo(w.pb, ok + cs8);
-      L.phps = ldo(w.php, ok + cs8);
-      L.ps = ldo(P, ok + cs8);
+      if (row_below_outside) {  // rows 1.. of the block take the south column from LDS
+        L.phs1 = ldo(w.ph_work, ok1 + cs8);
+        L.alts = ldo(w.alt, ok + cs8);
+        L.als = ldo(w.al, ok + cs8);
+        L.pbs = ldo(w.pb, ok + cs8);
+        L.phps = ldo(w.php, ok + cs8);
+        L.ps = ldo(P, ok + cs8);
+      }
       L.vw = ldo(w.v_work, q);
       L.vt = ldt<TT>(w.vt, q);
     }
@@ -149,7 +158,32 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
       const unsigned o2 = ok + 2 * pl8;
       L.p2c = ldo(P, o2 + cc8);
       if (row_start) L.p2w = ldo(P, o2 + cw8);
-      L.p2s = ldo(P, o2 + cs8);
+      if (do_v && row_below_outside) L.p2s = ldo(P, o2 + cs8);
+    }
+  };
+  // The south column of row j is the centre column of row j - 1, which the
+  // wave below holds (rows j and j - 1 of one block): each level's centre
+  // values go through LDS, double-buffered, with one barrier per level.
+  __shared__ double sh[2][BY][7][BX];
+  auto south = [&](int k, Lv& L) {
+    double* mine = &sh[k & 1][threadIdx.y][0][threadIdx.x];
+    mine[0 * BX] = L.phc1;
+    mine[1 * BX] = L.altc;
+    mine[2 * BX] = L.alc;
+    mine[3 * BX] = L.pbc;
+    mine[4 * BX] = L.phpc;
+    mine[5 * BX] = L.pc;
+    mine[6 * BX] = L.p2c;
+    __syncthreads();
+    if (!row_below_outside) {
+      const double* below = &sh[k & 1][threadIdx.y - 1][0][threadIdx.x];
+      L.phs1 = below[0 * BX];
+      L.alts = below[1 * BX];
+      L.als = below[2 * BX];
+      L.pbs = below[3 * BX];
+      L.phps = below[4 * BX];
+      L.ps = below[5 * BX];
+      L.p2s = below[6 * BX];
     }
   };
   // The west column of face i is the centre column of face i-1, which the
@@ -172,6 +206,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     Lv nxt{};
     if (k + 1 < nz) load(k + 1, nxt);
     west(cur);
+    south(k, cur);
     // dpn(k+1)
     double dpn_u1, dpn_v1;
     if (k + 1 < nz) {
