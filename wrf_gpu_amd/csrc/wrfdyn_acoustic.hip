@@ -807,24 +807,56 @@ __global__ void __launch_bounds__(256) k_fin_uv(DGrid g, DState s, DWork w) {
 // k = f-1.  Sweep 1 stores dvdxi per level in w.rhs, which k_column_w
 // overwrites at every level before it reads it, and sweep 2 reads it back
 // instead of recomputing it from four u and v faces.
+//
+// A block is kMuRows rows of one 64-lane wave each.  Waves overlap by two
+// lanes: lanes 1..62 own columns and lanes 0 and 63 load the columns either
+// side, so every east/west neighbour value (theta, the east u face) comes
+// from the adjacent lane by a DPP wave shift.  North/south values of sweep 2
+// (theta, the north v face) come from the waves above and below through LDS,
+// exchanged one level behind the loads; a wave at the block edge, or whose
+// neighbour row wraps periodically, loads them itself.
 // ---------------------------------------------------------------------------
+constexpr int kMuRows = 4;
+constexpr int kMuOwned = 62;  // columns owned per wave
+
+// value of lane + 1 (DPP wave_shl:1); lane 63 keeps `old`
+__device__ __forceinline__ double wave_shl1(double old, double src) {
+  constexpr int kWaveShl1 = 0x130;
+  const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, kWaveShl1, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), kWaveShl1, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <class TT>
-__global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState s, DWork w, double dts,
-                                                   int first) {
+__global__ void __launch_bounds__(64 * kMuRows) k_column_mu(DGrid g, DNamelist nl, DState s, DWork w, double dts,
+                                                           int first) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t ncol = (size_t)ny * nx;
-  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  if (c >= ncol) return;
-  const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
-  const unsigned plane = (unsigned)ncol * 8u;
+  const unsigned nbx = (unsigned)(nx + kMuOwned - 1) / kMuOwned;
+  const unsigned t = tile_id(g);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = (int)(t % nbx) * kMuOwned + tx - 1;  // -1 and >= nx on the halo lanes
+  const int j = (int)(t / nbx) * kMuRows + ty;
+  const bool owned = tx >= 1 && tx <= kMuOwned && i >= 0 && i < nx && j < ny;
+  // every lane loads in-bounds: its mass column (wrapped in i, row clamped),
+  // its west u face (face index clamped to 0..nx) and its south v face
+  const int ic = wrapi(i, nx), jr = j < ny ? j : ny - 1;
+  const unsigned plane = (unsigned)((size_t)ny * nx) * 8u;
   const unsigned uplane = (unsigned)((size_t)ny * (nx + 1)) * 8u, vplane = (unsigned)((size_t)(ny + 1) * nx) * 8u;
-  const unsigned c8 = (unsigned)c * 8u;
+  const unsigned c8 = (unsigned)i2(g, jr, ic) * 8u;
+  const unsigned uw8 = (unsigned)i2u(g, jr, clampi(i, 0, nx)) * 8u;
+  const unsigned vs8 = (unsigned)i2v(g, j <= ny ? j : ny, ic) * 8u;
+  const unsigned vn8 = (unsigned)i2v(g, j + 1 <= ny ? j + 1 : ny, ic) * 8u;
+  // wave-uniform: which north/south values this wave loads itself
+  const bool load_vn = ty == kMuRows - 1;
+  const bool load_thn = !(ty < kMuRows - 1 && j + 1 < ny);
+  const bool load_ths = !(ty > 0 && j >= 1);
+  const size_t c = c8 / 8u;
   const double msftx = g.msftx[c], msfty = g.msfty[c];
-  const double msft_inv = 1.0 / msfty;
   const double eps_p = 1.0 + nl.epssm, eps_m = 1.0 - nl.epssm;
   const double mut = w.mut[c];
-  const size_t fvn = i2v(g, j + 1, i), fvs = i2v(g, j, i), fue = i2u(g, j, i + 1), fuw = i2u(g, j, i);
-  const unsigned vn8 = (unsigned)fvn * 8u, vs8 = (unsigned)fvs * 8u, ue8 = (unsigned)fue * 8u, uw8 = (unsigned)fuw * 8u;
+  const size_t fvn = vn8 / 8u, fvs = vs8 / 8u, fuw = uw8 / 8u;
+  const size_t fue = i2u(g, jr, clampi(i + 1, 0, nx));
   const double muv_n = w.muv[fvn], muv_s = w.muv[fvs];
   const double rmvx_n = 1.0 / g.msfvx[fvn], rmvx_s = 1.0 / g.msfvx[fvs];
   const double muu_e = w.muu[fue], muu_w = w.muu[fuw];
@@ -844,11 +876,13 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
 #pragma unroll 4
   for (int k = 0; k < nz; ++k) {
     const unsigned kv = (unsigned)k * vplane, ku = (unsigned)k * uplane;
+    const double uww = ldo(w.u_work, uw8 + ku), uw = ldo(s.u, uw8 + ku);
+    // the east face of column i is the west face of column i + 1 (lane + 1)
+    const double uwe = wave_shl1(0.0, uww), ue = wave_shl1(0.0, uw);
     const double dv = dvdxi_at(k, ldo(w.v_work, vn8 + kv), ldo(w.v_work, vs8 + kv), ldo(s.v, vn8 + kv),
-                               ldo(s.v, vs8 + kv), ldo(w.u_work, ue8 + ku), ldo(w.u_work, uw8 + ku),
-                               ldo(s.u, ue8 + ku), ldo(s.u, uw8 + ku));
+                               ldo(s.v, vs8 + kv), uwe, uww, ue, uw);
     dmdt = (k == 0) ? g.dnw[0] * dv : dmdt + g.dnw[k] * dv;
-    sto(w.rhs, c8 + (unsigned)k * plane, dv);  // read back by sweep 2
+    if (owned) sto(w.rhs, c8 + (unsigned)k * plane, dv);  // read back by sweep 2
   }
   const double p2_ww0 = w.ww[c];
   const double p2_wwtop = ldo(w.ww, c8 + (unsigned)nz * plane);
@@ -858,43 +892,55 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
     const double mu_save = w.mu[c] - mu_work_old;
     const double mu_tendency = dmdt + 0.0;
     const double mu_work_new = mu_work_old + dts * mu_tendency;
-    w.mu[c] = mu_save + mu_work_new;
-    s.mudf[c] = mu_tendency;
     muts = mut + mu_work_new;
-    w.muts[c] = muts;
     muave = 0.5 * (eps_p * mu_work_new + eps_m * mu_work_old);
-    w.muave[c] = muave;
+    if (owned) {
+      w.mu[c] = mu_save + mu_work_new;
+      s.mudf[c] = mu_tendency;
+      w.muts[c] = muts;
+      w.muave[c] = muave;
+    }
   }
 
-  const int ip = wrapi(i + 1, nx), imn = wrapi(i - 1, nx), jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny);
-  const unsigned cjp = (unsigned)i2(g, jp, i) * 8u, cjm = (unsigned)i2(g, jm, i) * 8u;
-  const unsigned cip = (unsigned)i2(g, j, ip) * 8u, cim = (unsigned)i2(g, j, imn) * 8u;
+  const unsigned cjp = (unsigned)i2(g, wrapi(jr + 1, ny), ic) * 8u, cjm = (unsigned)i2(g, wrapi(jr - 1, ny), ic) * 8u;
   struct Mv {  // mass level data
     double vwn, vws, uwe, uww, dv;
     double thc, thn, ths, the, thw, thwk, tht, t2o;
   };
-  Mv m_1{};                         // level f-1
-  double thc_2 = 0.0;               // theta at f-2
-  double b_1 = 0.0;                 // omega at f-1
+  __shared__ double sh[2][kMuRows][2][64];
+  Mv m_1{};                           // level f-1
+  double thc_2 = 0.0;                 // theta at f-2
+  double b_1 = 0.0;                   // omega at f-1
   double wprev = p2_ww0, dv_1 = 0.0;  // omega prefix, dvdxi at f-1
   for (int f = 0; f <= nz; ++f) {
     Mv m{};
     if (f < nz) {
       const unsigned kp = (unsigned)f * plane, q = c8 + kp;
       const unsigned kv = (unsigned)f * vplane, ku = (unsigned)f * uplane;
-      m.vwn = ldo(w.v_work, vn8 + kv);
+      if (load_vn) m.vwn = ldo(w.v_work, vn8 + kv);
       m.vws = ldo(w.v_work, vs8 + kv);
-      m.uwe = ldo(w.u_work, ue8 + ku);
       m.uww = ldo(w.u_work, uw8 + ku);
       m.dv = ldo(w.rhs, q);
       m.thc = ldo(s.theta, q);
-      m.thn = ldo(s.theta, cjp + kp);
-      m.ths = ldo(s.theta, cjm + kp);
-      m.the = ldo(s.theta, cip + kp);
-      m.thw = ldo(s.theta, cim + kp);
+      if (load_thn) m.thn = ldo(s.theta, cjp + kp);
+      if (load_ths) m.ths = ldo(s.theta, cjm + kp);
       m.thwk = ldo(w.th_work, q);
       m.tht = ldt<TT>(w.tht, q);
       m.t2o = first ? 0.0 : ldo(w.t2ave, q);  // t_2ave starts each stage at 0
+    }
+    if (f >= 1) {
+      // level f-1's neighbours: east/west from the adjacent lanes, north/south
+      // from the adjacent waves (LDS, double-buffered by level)
+      m_1.uwe = wave_shl1(0.0, m_1.uww);
+      m_1.the = wave_shl1(0.0, m_1.thc);
+      m_1.thw = wave_shr1(0.0, m_1.thc);
+      double* mine = &sh[f & 1][ty][0][tx];
+      mine[0] = m_1.thc;
+      mine[64] = m_1.vws;
+      __syncthreads();
+      if (!load_thn) m_1.thn = sh[f & 1][ty + 1][0][tx];
+      if (!load_vn) m_1.vwn = sh[f & 1][ty + 1][1][tx];
+      if (!load_ths) m_1.ths = sh[f & 1][ty - 1][0][tx];
     }
     const unsigned qf = c8 + (unsigned)f * plane;
     const double rom = f < nz ? ldo(w.rom, qf) : 0.0;
@@ -904,19 +950,19 @@ __global__ void __launch_bounds__(256) k_column_mu(DGrid g, DNamelist nl, DState
     double b0;
     if (f == 0) {
       b0 = wprev - rom;
-      sto(w.ww, qf, b0);
+      if (owned) sto(w.ww, qf, b0);
     } else if (f < nz) {
       const int k = f - 1;
       double incr = (g.dnw[f - 1] * ((g.c1h[k] * dmdt + dv_1) + g.c1h[k] * 0.0)) / msfty;
       double wk = wprev - incr;
       b0 = wk - rom;
       wprev = wk;
-      sto(w.ww, qf, b0);
+      if (owned) sto(w.ww, qf, b0);
     } else {
       b0 = p2_wwtop;
     }
     // theta update and t_2ave at k = f-1
-    if (f >= 1) {
+    if (f >= 1 && owned) {
       const int k = f - 1;
       const double th = m_1.thc - kT0;
       double wd_hi = 0.0, wd_lo = 0.0;
@@ -1306,11 +1352,13 @@ hipError_t launch_column_tile(int tile, const DGrid& g, const DNamelist& nl, con
   if (tile == kColumnMarch) {
     const unsigned bs = 256;
     const size_t C = (size_t)g.ny * g.nx;
+    const unsigned mu_tiles = (unsigned)((g.nx + kMuOwned - 1) / kMuOwned) * (unsigned)((g.ny + kMuRows - 1) / kMuRows);
+    const dim3 mu_block(64, kMuRows);
     if (nl.mixed) {
-      WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<float><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+      WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<float><<<mu_tiles, mu_block, 0, st>>>(g, nl, s, w, dts, first)));
       WD_TIMED(prof, KID_COLUMN_W, st, (k_column_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
     } else {
-      WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<double><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
+      WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<double><<<mu_tiles, mu_block, 0, st>>>(g, nl, s, w, dts, first)));
       WD_TIMED(prof, KID_COLUMN_W, st, (k_column_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first)));
     }
     return hipGetLastError();
