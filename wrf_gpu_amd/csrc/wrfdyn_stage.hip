@@ -866,31 +866,63 @@ __global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
 //  a/alpha/gamma = calc_coef_w_wrf_coefficients (dynamics/acoustic_wrf.py:783-852)
 // Level f-1's coupled face velocities and ph_1 totals come from registers;
 // level f+1's loads are issued before level f is computed.
+//
+// A block is kColRows rows of one 64-lane wave each; waves overlap by two
+// lanes (lanes 1..62 own columns, lanes 0 and 63 load the columns either
+// side).  At level f each lane's ph_1 total, west u face and south v face are
+// exchanged: east/west from the adjacent lanes (DPP), north/south from the
+// adjacent waves (LDS); a wave at the block edge, or whose neighbour row
+// wraps periodically, loads them itself.
 // ---------------------------------------------------------------------------
+constexpr int kColRows = 4;
+constexpr int kColOwned = 62;
+
+// value of lane + 1 (DPP wave_shl:1) / lane - 1 (wave_shr:1); the lane with no
+// source keeps `old`
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double old, double src) {
+  const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+
 template <class TT>
-__global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
+__global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const unsigned nbx = (unsigned)(nx + kColOwned - 1) / kColOwned;
+  const unsigned tile = tile_id(g);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = (int)(tile % nbx) * kColOwned + tx - 1;  // -1 and >= nx on the halo lanes
+  const int j = (int)(tile / nbx) * kColRows + ty;
+  const bool owned = tx >= 1 && tx <= kColOwned && i >= 0 && i < nx && j < ny;
+  // every lane loads in-bounds: its mass column (wrapped in i, row clamped),
+  // its west u face (face index clamped to 0..nx) and its south v face
+  const int ic = wrapi(i, nx), jr = j < ny ? j : ny - 1;
   const size_t ncol = (size_t)ny * nx;
-  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  if (c >= ncol) return;
-  const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
+  const size_t c = i2(g, jr, ic);
   const unsigned pl8 = (unsigned)ncol * 8u;
   const unsigned upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u, vpl8 = (unsigned)((size_t)(ny + 1) * nx) * 8u;
   const unsigned c8 = (unsigned)c * 8u;
-  const unsigned uw8 = (unsigned)i2u(g, j, i) * 8u, ue8 = (unsigned)i2u(g, j, i + 1) * 8u;
-  const unsigned vs8 = (unsigned)i2v(g, j, i) * 8u, vn8 = (unsigned)i2v(g, j + 1, i) * 8u;
-  const int jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny), ip = wrapi(i + 1, nx), imn = wrapi(i - 1, nx);
-  const unsigned cjp8 = (unsigned)i2(g, jp, i) * 8u, cjm8 = (unsigned)i2(g, jm, i) * 8u;
-  const unsigned cip8 = (unsigned)i2(g, j, ip) * 8u, cim8 = (unsigned)i2(g, j, imn) * 8u;
+  const size_t fuw = i2u(g, jr, clampi(i, 0, nx)), fue = i2u(g, jr, clampi(i + 1, 0, nx));
+  const size_t fvs = i2v(g, j <= ny ? j : ny, ic), fvn = i2v(g, j + 1 <= ny ? j + 1 : ny, ic);
+  const unsigned uw8 = (unsigned)fuw * 8u, ue8 = (unsigned)fue * 8u;
+  const unsigned vs8 = (unsigned)fvs * 8u, vn8 = (unsigned)fvn * 8u;
+  const unsigned cjp8 = (unsigned)i2(g, wrapi(jr + 1, ny), ic) * 8u, cjm8 = (unsigned)i2(g, wrapi(jr - 1, ny), ic) * 8u;
+  // wave-uniform: which north/south values this wave loads itself
+  const bool load_vn = ty == kColRows - 1;
+  const bool load_n = !(ty < kColRows - 1 && j + 1 < ny);
+  const bool load_s = !(ty > 0 && j >= 1);
   const double msfty = g.msfty[c];
   const double msft_inv = 1.0 / msfty;
   const double mut = w.mut[c], mub = w.mub[c];
   const double mu_p = mut - mub;
   const double mxy = g.msftx[c] / msfty;
-  const double muu_w = w.muu[i2u(g, j, i)], muu_e = w.muu[i2u(g, j, i + 1)];
-  const double msfuy_w = g.msfuy[i2u(g, j, i)], msfuy_e = g.msfuy[i2u(g, j, i + 1)];
-  const double muv_s = w.muv[i2v(g, j, i)], muv_n = w.muv[i2v(g, j + 1, i)];
-  const double msfvx_s = g.msfvx[i2v(g, j, i)], msfvx_n = g.msfvx[i2v(g, j + 1, i)];
+  const double muu_w = w.muu[fuw], muu_e = w.muu[fue];
+  const double msfuy_w = g.msfuy[fuw], msfuy_e = g.msfuy[fue];
+  const double muv_s = w.muv[fvs], muv_n = w.muv[fvn];
+  const double msfvx_s = g.msfvx[fvs], msfvx_n = g.msfvx[fvn];
   const double cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
   double cof = (0.5 * dts * kG * (1.0 + nl.epssm));
   cof = cof * cof;
@@ -920,30 +952,30 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
   };
   struct Lv {
     double pp, wt, uw, ue, vs, vn, romp, phpp, phtp, ww, c2a;
-    double pnp, pnt, psp, pst, pep, pet, pwp, pwt;  // ph' and ph total at f on rows j+-1, columns i+-1
+    double pnp, pnt, psp, pst;  // ph' and ph total at f on rows j+1, j-1 (edge waves only)
   };
   auto load = [&](int f, Lv& L) {
     const unsigned o = (unsigned)f * pl8, ou = (unsigned)f * upl8, ov = (unsigned)f * vpl8;
     L.pp = ldo(s.p_pert, o + c8);
     L.wt = ldt<TT>(w.wt, o + c8);
     L.uw = ldo(s.u, ou + uw8);
-    L.ue = ldo(s.u, ou + ue8);
     L.vs = ldo(s.v, ov + vs8);
-    L.vn = ldo(s.v, ov + vn8);
+    if (load_vn) L.vn = ldo(s.v, ov + vn8);
     L.romp = ldo(w.rom, o + pl8 + c8);
     L.phpp = ldo(s.ph_pert, o + pl8 + c8);
     L.phtp = ldo(s.ph_total, o + pl8 + c8);
     L.ww = ldo(s.w, o + c8);
     L.c2a = ldo(w.c2a, o + c8);
-    L.pnp = ldo(s.ph_pert, o + cjp8);
-    L.pnt = ldo(s.ph_total, o + cjp8);
-    L.psp = ldo(s.ph_pert, o + cjm8);
-    L.pst = ldo(s.ph_total, o + cjm8);
-    L.pep = ldo(s.ph_pert, o + cip8);
-    L.pet = ldo(s.ph_total, o + cip8);
-    L.pwp = ldo(s.ph_pert, o + cim8);
-    L.pwt = ldo(s.ph_total, o + cim8);
+    if (load_n) {
+      L.pnp = ldo(s.ph_pert, o + cjp8);
+      L.pnt = ldo(s.ph_total, o + cjp8);
+    }
+    if (load_s) {
+      L.psp = ldo(s.ph_pert, o + cjm8);
+      L.pst = ldo(s.ph_total, o + cjm8);
+    }
   };
+  __shared__ double sh[2][kColRows][2][64];
   // level 0
   double p_m = ldo(s.p_pert, c8);  // p' at f-1
   double uw_m = ldo(s.u, uw8), ue_m = ldo(s.u, ue8), vs_m = ldo(s.v, vs8), vn_m = ldo(s.v, vn8);
@@ -952,11 +984,13 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
   double pht_m = PHT(ldo(s.ph_pert, c8), ldo(s.ph_total, c8));                // ph_1 total at f-1
   double pht_0 = PHT(ldo(s.ph_pert, pl8 + c8), ldo(s.ph_total, pl8 + c8));    // at f
   double c2a_m = ldo(w.c2a, c8);
-  sto(w.rw_tend, c8, 0.0 + ldt<TT>(w.wt, c8));
-  stt<TT>(w.ph_tend, c8, 0.0);
-  sto(w.ca, c8, 0.0);
-  sto(w.calpha, c8, 1.0);
-  sto(w.cgamma, c8, 0.0);
+  if (owned) {
+    sto(w.rw_tend, c8, 0.0 + ldt<TT>(w.wt, c8));
+    stt<TT>(w.ph_tend, c8, 0.0);
+    sto(w.ca, c8, 0.0);
+    sto(w.calpha, c8, 1.0);
+    sto(w.cgamma, c8, 0.0);
+  }
   double gamma_prev = 0.0;
   Lv cur{};
   load(1, cur);
@@ -964,6 +998,20 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
     Lv nxt{};
     if (f + 1 < nz) load(f + 1, nxt);
     const unsigned q = c8 + (unsigned)f * pl8;
+    // level f neighbours: east/west from the adjacent lanes, north/south from
+    // the adjacent waves (LDS, double-buffered by level)
+    double pht_n, pht_s;
+    const double pht_e = dpp_move<kDppWaveShl1>(0.0, pht_0), pht_w = dpp_move<kDppWaveShr1>(0.0, pht_0);
+    cur.ue = dpp_move<kDppWaveShl1>(0.0, cur.uw);
+    {
+      double* mine = &sh[f & 1][ty][0][tx];
+      mine[0] = pht_0;
+      mine[64] = cur.vs;
+      __syncthreads();
+      pht_n = load_n ? PHT(cur.pnp, cur.pnt) : sh[f & 1][ty + 1][0][tx];
+      pht_s = load_s ? PHT(cur.psp, cur.pst) : sh[f & 1][ty - 1][0][tx];
+      if (!load_vn) cur.vn = sh[f & 1][ty + 1][1][tx];
+    }
     // ---- rw_tend ----
     {
       double rw = (msft_inv * kG) *
@@ -975,7 +1023,7 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
       double rv_f = g.fnm[f] * cv0.rv_m + g.fnp[f] * cv_m.rv_m;
       double v_f = g.fnm[f] * cv0.v_m + g.fnp[f] * cv_m.v_m;
       double curv = kReradius * (ru_f * u_f + (mxy * rv_f) * v_f);
-      sto(w.rw_tend, q, rw + (0.0 + curv));
+      if (owned) sto(w.rw_tend, q, rw + (0.0 + curv));
       cv_m = cv0;
     }
     // ---- ph_tend (rhs_ph) ----
@@ -989,21 +1037,21 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
       double gw = ((mass_f * kG) * cur.ww) * msft_inv;
       t = t + gw;
       double pti = pht_0;
-      double dn_ = PHT(cur.pnp, cur.pnt) - pti, ds_ = pti - PHT(cur.psp, cur.pst);
+      double dn_ = pht_n - pti, ds_ = pti - pht_s;
       double vn = cur.vn + vn_m;
       double vs = cur.vs + vs_m;
       double mvn = g.c1f[f] * muv_n + g.c2f[f];
       double mvs = g.c1f[f] * muv_s + g.c2f[f];
       double fy = (mvn * vn) * dn_ + (mvs * vs) * ds_;
       t = t + (-((0.25 * g.rdy) * msft_inv)) * fy;
-      double de_ = PHT(cur.pep, cur.pet) - pti, dw_ = pti - PHT(cur.pwp, cur.pwt);
+      double de_ = pht_e - pti, dw_ = pti - pht_w;
       double ue = cur.ue + ue_m;
       double uw = cur.uw + uw_m;
       double mue = g.c1f[f] * muu_e + g.c2f[f];
       double muw = g.c1f[f] * muu_w + g.c2f[f];
       double fx = (mue * ue) * de_ + (muw * uw) * dw_;
       t = t + (-((0.25 * g.rdx) * msft_inv)) * fx;
-      stt<TT>(w.ph_tend, q, t);
+      if (owned) stt<TT>(w.ph_tend, q, t);
     }
     // ---- calc_coef_w ----
     {
@@ -1023,9 +1071,11 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
       double cc = (((((-cqw_at(k, nz)) * cof) * g.rdn[k]) * g.rdnw[k]) * cur.c2a) / dc;
       double al = 1.0 / (b - ak * gamma_prev);
       double gm = cc * al;
-      sto(w.ca, q, ak);
-      sto(w.calpha, q, al);
-      sto(w.cgamma, q, gm);
+      if (owned) {
+        sto(w.ca, q, ak);
+        sto(w.calpha, q, al);
+        sto(w.cgamma, q, gm);
+      }
       gamma_prev = gm;
     }
     p_m = cur.pp;
@@ -1041,6 +1091,7 @@ __global__ void __launch_bounds__(256) k_col_setup(DGrid g, DNamelist nl, DState
     cur = nxt;
   }
   // top face nz (p_m, c2a_m now hold level nz-1)
+  if (!owned) return;  // no barrier follows
   const unsigned qt = c8 + (unsigned)nz * pl8;
   double rwt = (msft_inv * kG) *
                (((((cq1 * 2.0) * g.rdnw[nz - 1]) * (-p_m)) - g.c1f[nz] * mu_p) - cq2 * (g.c1f[nz] * mub + g.c2f[nz]));
@@ -1091,10 +1142,13 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
   WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<dim3(blocks_for(CU, bs), kchunks), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_PREP_V, st, (k_prep_v<<<dim3(blocks_for(CV, bs), kchunks), bs, 0, st>>>(g, s, w)));
+  const unsigned col_tiles =
+      (unsigned)((g.nx + kColOwned - 1) / kColOwned) * (unsigned)((g.ny + kColRows - 1) / kColRows);
+  const dim3 col_block(64, kColRows);
   if (nl.mixed)
-    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<float><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts)));
+    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<float><<<col_tiles, col_block, 0, st>>>(g, nl, s, w, dts)));
   else
-    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<double><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts)));
+    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<double><<<col_tiles, col_block, 0, st>>>(g, nl, s, w, dts)));
   return hipGetLastError();
 }
 
