@@ -10,16 +10,9 @@ namespace wd {
 
 namespace {
 
-// Value of the lane one below (lane - 1) in the wavefront; lane 0 gets `old`.
-// DPP wave_shr:1 with bound_ctrl off leaves a destination whose source lane
-// is outside the wave unwritten, so lane 0 keeps `old`.
-__device__ __forceinline__ double wave_shr1(double old, double src) {
-  constexpr int kWaveShr1 = 0x138;
-  const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
-  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, kWaveShr1, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), kWaveShr1, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
+// Value of lane - 1 / lane + 1 in the wavefront; the lane with no source keeps `old`.
+__device__ __forceinline__ double wave_shr1(double old, double src) { return dpp_move<kDppWaveShr1>(old, src); }
+__device__ __forceinline__ double wave_shl1(double old, double src) { return dpp_move<kDppWaveShl1>(old, src); }
 
 __device__ inline void decompose(size_t idx, int nxx, int nyy, int& k, int& j, int& i) {
   i = (int)(idx % (size_t)nxx);
@@ -818,15 +811,6 @@ __global__ void __launch_bounds__(256) k_fin_uv(DGrid g, DState s, DWork w) {
 // ---------------------------------------------------------------------------
 constexpr int kMuRows = 4;
 constexpr int kMuOwned = 62;  // columns owned per wave
-
-// value of lane + 1 (DPP wave_shl:1); lane 63 keeps `old`
-__device__ __forceinline__ double wave_shl1(double old, double src) {
-  constexpr int kWaveShl1 = 0x130;
-  const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
-  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, kWaveShl1, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), kWaveShl1, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 template <class TT>
 __global__ void __launch_bounds__(64 * kMuRows) k_column_mu(DGrid g, DNamelist nl, DState s, DWork w, double dts,

@@ -146,6 +146,19 @@ __device__ __forceinline__ void stt(double* base, unsigned off8, double v) {
   }
 }
 
+// Cross-lane moves within a 64-lane wavefront (DPP, no memory access):
+// kDppWaveShl1 gives each lane the value of lane + 1, kDppWaveShr1 that of
+// lane - 1.  With bound_ctrl off, the lane whose source lies outside the wave
+// (63, or 0) keeps `old`.
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double old, double src) {
+  const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ inline int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 // jnp.sign: -1, +1, and the argument itself for +-0 and NaN.

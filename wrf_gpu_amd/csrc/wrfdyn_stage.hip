@@ -877,17 +877,6 @@ __global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
 constexpr int kColRows = 4;
 constexpr int kColOwned = 62;
 
-// value of lane + 1 (DPP wave_shl:1) / lane - 1 (wave_shr:1); the lane with no
-// source keeps `old`
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double old, double src) {
-  const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
-  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
-
 template <class TT>
 __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
