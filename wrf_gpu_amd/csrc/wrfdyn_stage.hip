@@ -330,38 +330,52 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
   }
 }
 
+constexpr int kTendURows = 4;
+
 // advect_u_flux (flux_advection.py:1131-1202) + large_step_horizontal_pgf x
 // (rk_addtend_dry.py:329-343, 357-361) on u faces (nz, ny, nx+1).  Face nx
 // repeats face 0's advection (periodic); the PGF uses edge-padded pairs.
 template <class TT>
-__global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
+__global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, DWork w) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t nface = (size_t)ny * (nx + 1);
-  const size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  if (idx >= nface) return;
-  const int j = (int)(idx / (size_t)(nx + 1)), i = (int)(idx % (size_t)(nx + 1));
-  const int ia = (i == nx) ? 0 : i;
-  const int ip = wrapi(ia + 1, nx), imn = wrapi(ia - 1, nx);
-  const unsigned pl8 = (unsigned)((size_t)ny * nx) * 8u, upl8 = (unsigned)nface * 8u;
+  // Blocks of kTendURows rows of one 64-lane wave each; waves overlap by one
+  // lane either side (lanes 1..62 own u faces).  Each lane loads its own face
+  // (wrapped), its own mass column ia and the PGF's right column; the +-1
+  // face, ru/rv/rom at ia-1 and ia+1 and the PGF's left column come from the
+  // adjacent lanes (DPP), the rows j-3..j+3 inside the block and rv at j+1
+  // from the adjacent waves (LDS); the +-2, +-3 faces and rows outside the
+  // block (or wrapping periodically) are loaded.
+  const unsigned nbx = (unsigned)(nx + 1 + 61) / 62;
+  const unsigned tile = tile_id(g);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = (int)(tile % nbx) * 62 + tx - 1;  // -1 and > nx on the halo lanes
+  const int j = (int)(tile / nbx) * kTendURows + ty;
+  const bool owned = tx >= 1 && tx <= 62 && i <= nx && j < ny;
+  const int jr = j < ny ? j : ny - 1;
+  const int ia = wrapi(i, nx);  // the collapsed periodic face (face nx repeats face 0)
+  const unsigned pl8 = (unsigned)((size_t)ny * nx) * 8u, upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u;
   const bool do_y = ny > 1;
-  const int jp = wrapi(j + 1, ny);
   unsigned ux[7], uy[7];  // u faces ia-3..ia+3 on row j; rows j-3..j+3 at face ia
+  bool lds_y[7];
 #pragma unroll
   for (int d = 0; d < 7; ++d) {
-    ux[d] = (unsigned)i2u(g, j, wrapi(ia + d - 3, nx)) * 8u;
-    uy[d] = (unsigned)i2u(g, wrapi(j + d - 3, ny), ia) * 8u;
+    ux[d] = (unsigned)i2u(g, jr, wrapi(ia + d - 3, nx)) * 8u;
+    uy[d] = (unsigned)i2u(g, wrapi(jr + d - 3, ny), ia) * 8u;
+    const int r = ty + d - 3, jj = j + d - 3;
+    lds_y[d] = r >= 0 && r < kTendURows && jj >= 0 && jj < ny;
   }
-  const unsigned ca8 = (unsigned)i2(g, j, ia) * 8u, cim8 = (unsigned)i2(g, j, imn) * 8u;
-  const unsigned cip8 = (unsigned)i2(g, j, ip) * 8u;
-  const unsigned jpa8 = (unsigned)i2(g, jp, ia) * 8u, jpm8 = (unsigned)i2(g, jp, imn) * 8u;
-  const double msf = g.msfux[i2u(g, j, ia)];
-  const unsigned fo8 = (unsigned)i2u(g, j, i) * 8u;  // the output face
-  // PGF columns (edge padded)
+  const unsigned ca8 = (unsigned)i2(g, jr, ia) * 8u;
+  const unsigned jpa8 = (unsigned)i2(g, wrapi(jr + 1, ny), ia) * 8u;
+  const double msf = g.msfux[i2u(g, jr, ia)];
+  const int io = clampi(i, 0, nx);
+  const unsigned fo8 = (unsigned)i2u(g, jr, io) * 8u;  // the output face
+  // PGF columns (edge padded): this lane's right column; the left one is the
+  // right column of lane - 1
   const int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
-  const size_t cl = i2(g, j, il), cr = i2(g, j, ir);
+  const size_t cl = i2(g, jr, il), cr = i2(g, jr, ir);
   const unsigned cl8 = (unsigned)cl * 8u, cr8 = (unsigned)cr * 8u;
-  const double msf_u = g.msfux[i2u(g, j, i)] / g.msfuy[i2u(g, j, i)];
+  const double msf_u = g.msfux[i2u(g, jr, io)] / g.msfuy[i2u(g, jr, io)];
   const double mub_l = s.mu_total[cl] - s.mu_pert[cl];
   const double mub_r = s.mu_total[cr] - s.mu_pert[cr];
   const double mt_l = mub_l + s.mu_pert[cl];
@@ -370,50 +384,73 @@ __global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
   const double mu_l = s.mu_pert[cl], mu_r = s.mu_pert[cr];
   const double* U = s.u;
   const double* PP = s.p_pert;
+  __shared__ double sh[2][kTendURows][2][64];  // u at face ia and rv at column ia, level k
   struct Lv {
     double x[7], y[7];                          // u at faces ia-3..ia+3 / rows j-3..j+3
     double ru_a, ru_m, ru_p, rv_a, rv_m, rv_pa, rv_pm;  // transport pieces
     double pl1, pr1, phl1, phr1;                // p' and ph' at level k+1 (l, r)
     double altl, altr, ptl, ptr, all, alr, phpl, phpr;  // level k diagnostics (l, r)
   };
+  auto shr = [](double v) { return dpp_move<kDppWaveShr1>(0.0, v); };
+  auto shl = [](double v) { return dpp_move<kDppWaveShl1>(0.0, v); };
+  // ru, rv at the own column, one level ahead (rv goes to LDS before it is used)
+  double ru_n = ldo(w.ru, ca8), rv_n = do_y ? ldo(w.rv, ca8) : 0.0;
   auto load = [&](int k, Lv& L) {
     const unsigned o = (unsigned)k * pl8, ou = (unsigned)k * upl8;
-#pragma unroll
-    for (int d = 0; d < 7; ++d) L.x[d] = ldo(U, ou + ux[d]);
+    L.x[0] = ldo(U, ou + ux[0]);
+    L.x[1] = ldo(U, ou + ux[1]);
+    L.x[5] = ldo(U, ou + ux[5]);
+    L.x[6] = ldo(U, ou + ux[6]);
     if (do_y) {
 #pragma unroll
       for (int d = 0; d < 7; ++d)
-        if (d != 3) L.y[d] = ldo(U, ou + uy[d]);
+        if (d != 3 && !lds_y[d]) L.y[d] = ldo(U, ou + uy[d]);
+      if (!lds_y[4]) L.rv_pa = ldo(w.rv, o + jpa8);
     }
-    L.y[3] = L.x[3];
-    L.ru_a = ldo(w.ru, o + ca8);
-    L.ru_m = ldo(w.ru, o + cim8);
-    L.ru_p = ldo(w.ru, o + cip8);
-    if (do_y) {
-      L.rv_a = ldo(w.rv, o + ca8);
-      L.rv_m = ldo(w.rv, o + cim8);
-      L.rv_pa = ldo(w.rv, o + jpa8);
-      L.rv_pm = ldo(w.rv, o + jpm8);
-    }
+    L.ru_a = ru_n;
+    L.rv_a = rv_n;
     if (k + 1 < nz) {
-      L.pl1 = ldo(PP, o + pl8 + cl8);
+      ru_n = ldo(w.ru, o + pl8 + ca8);
+      if (do_y) rv_n = ldo(w.rv, o + pl8 + ca8);
       L.pr1 = ldo(PP, o + pl8 + cr8);
     }
-    L.phl1 = ldo(s.ph_pert, o + pl8 + cl8);
     L.phr1 = ldo(s.ph_pert, o + pl8 + cr8);
-    L.altl = ldo(w.alt_raw, o + cl8);
     L.altr = ldo(w.alt_raw, o + cr8);
-    L.ptl = ldo(s.p_total, o + cl8);
     L.ptr = ldo(s.p_total, o + cr8);
-    L.all = ldo(w.al_ls, o + cl8);
     L.alr = ldo(w.al_ls, o + cr8);
-    L.phpl = ldo(w.php_ls, o + cl8);
     L.phpr = ldo(w.php_ls, o + cr8);
+  };
+  // level k's values from the adjacent lanes and waves
+  auto exchange = [&](int k, Lv& L, double uc) {
+    L.x[3] = uc;
+    L.y[3] = uc;
+    L.x[2] = shr(uc);
+    L.x[4] = shl(uc);
+    L.ru_m = shr(L.ru_a);
+    L.ru_p = shl(L.ru_a);
+    L.pl1 = shr(L.pr1);
+    L.phl1 = shr(L.phr1);
+    L.altl = shr(L.altr);
+    L.ptl = shr(L.ptr);
+    L.all = shr(L.alr);
+    L.phpl = shr(L.phpr);
+    double* mine = &sh[k & 1][ty][0][tx];
+    mine[0] = uc;
+    mine[64] = L.rv_a;
+    __syncthreads();
+    if (do_y) {
+#pragma unroll
+      for (int d = 0; d < 7; ++d)
+        if (d != 3 && lds_y[d]) L.y[d] = sh[k & 1][ty + d - 3][0][tx];
+      if (lds_y[4]) L.rv_pa = sh[k & 1][ty + 1][1][tx];
+      L.rv_m = shr(L.rv_a);
+      L.rv_pm = shr(L.rv_pa);
+    }
   };
   // centre-face window of u (k-1..k+3) and the face omega (k+1, k+2)
   auto romq = [&](int kk) {
-    const unsigned o = (unsigned)kk * pl8;
-    return 0.5 * (ldo(w.rom, o + ca8) + ldo(w.rom, o + cim8));
+    const double ra = ldo(w.rom, (unsigned)kk * pl8 + ca8);
+    return 0.5 * (ra + shr(ra));
   };
   const unsigned ua8 = ux[3];
   double qm1 = 0.0, q0 = ldo(U, ua8), qp1 = ldo(U, upl8 + ua8), qp2 = ldo(U, 2 * upl8 + ua8);
@@ -436,6 +473,7 @@ __global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
     const double qp3 = k + 3 < nz ? ldo(U, (unsigned)(k + 3) * upl8 + ua8) : 0.0;
     const double r2 = k + 2 <= nz ? romq(k + 2) : 0.0;
     load(k, cur);
+    exchange(k, cur, q0);
     // --- advection on the collapsed periodic face ia ---
     auto UX = [&](int sft) { return cur.x[sft + 3]; };
     auto UY = [&](int d) { return cur.y[d + 3]; };
@@ -482,7 +520,7 @@ __global__ void __launch_bounds__(256) k_tend_u(DGrid g, DState s, DWork w) {
     double dpx = (((msf_u * 0.5) * g.rdx) * mass_u) * ((ph_term + p_term) + pb_term);
     double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_l + mu_r));
     dpx = dpx + ((msf_u * g.rdx) * (cur.phpr - cur.phpl)) * bracket;
-    stt<TT>(w.ut, (unsigned)k * upl8 + fo8, t + (-1.0 * dpx));
+    if (owned) stt<TT>(w.ut, (unsigned)k * upl8 + fo8, t + (-1.0 * dpx));
     qm1 = q0;
     q0 = qp1;
     qp1 = qp2;
@@ -1117,13 +1155,15 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   // k_prep_col writes the large-step diagnostics the tendency kernels read
   WD_TIMED(prof, KID_PREP_COL, st, (k_prep_col<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   const size_t CU = (size_t)g.ny * (g.nx + 1), CV = (size_t)(g.ny + 1) * g.nx;
+  const unsigned tu_tiles = (unsigned)((g.nx + 1 + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
+  const dim3 tu_block(64, kTendURows);
   if (nl.mixed) {
-    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float><<<blocks_for(CU, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float><<<blocks_for(CV, bs), bs, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   } else {
-    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double><<<blocks_for(CU, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double><<<blocks_for(CV, bs), bs, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
