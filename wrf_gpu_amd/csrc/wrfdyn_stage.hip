@@ -540,33 +540,51 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
 // (rk_addtend_dry.py:345-361) on v faces (nz, ny+1, nx).  Face ny repeats
 // face 0's advection (periodic); the PGF uses edge-padded pairs.
 template <class TT>
-__global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
+__global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, DWork w) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t nface = (size_t)(ny + 1) * nx;
-  const size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  if (idx >= nface) return;
-  const int j = (int)(idx / (size_t)nx), i = (int)(idx % (size_t)nx);
-  const int ja = (j == ny) ? 0 : j;
-  const int jm = wrapi(ja - 1, ny), jp = wrapi(ja + 1, ny), ip = wrapi(i + 1, nx);
-  const unsigned pl8 = (unsigned)((size_t)ny * nx) * 8u, upl8 = (unsigned)nface * 8u;
+  // Blocks of kTendURows v-face rows of one 64-lane wave each; waves overlap
+  // by one lane either side (lanes 1..62 own columns).  Each lane loads its
+  // own face (collapsed row ja, column wrapped), its own mass column and the
+  // PGF's north row; the +-1 columns come from the adjacent lanes (DPP), the
+  // rows ja-3..ja+3 inside the block, ru/rv on row ja-1, rv on row ja+1 and
+  // the PGF's south row from the adjacent waves (LDS); the +-2, +-3 columns
+  // and rows outside the block (or wrapping differently) are loaded.
+  const unsigned nbx = (unsigned)(nx + 61) / 62;
+  const unsigned tile = tile_id(g);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = (int)(tile % nbx) * 62 + tx - 1;  // outside 0..nx-1 on the halo lanes
+  const int j = (int)(tile / nbx) * kTendURows + ty;
+  const bool owned = tx >= 1 && tx <= 62 && i < nx && j <= ny;
+  const int jv = j <= ny ? j : ny;    // face row, clamped
+  const int ja = (jv == ny) ? 0 : jv;  // the collapsed periodic face row (face ny repeats face 0)
+  const int ic = wrapi(i, nx);
+  const int jm = wrapi(ja - 1, ny), jp = wrapi(ja + 1, ny);
+  const unsigned pl8 = (unsigned)((size_t)ny * nx) * 8u, upl8 = (unsigned)((size_t)(ny + 1) * nx) * 8u;
   unsigned ux[7], uy[7];  // v faces i-3..i+3 on row ja; rows ja-3..ja+3 at column i
+  bool lds_y[7];
 #pragma unroll
   for (int d = 0; d < 7; ++d) {
-    ux[d] = (unsigned)i2v(g, ja, wrapi(i + d - 3, nx)) * 8u;
-    uy[d] = (unsigned)i2v(g, wrapi(ja + d - 3, ny), i) * 8u;
+    ux[d] = (unsigned)i2v(g, ja, wrapi(ic + d - 3, nx)) * 8u;
+    uy[d] = (unsigned)i2v(g, wrapi(ja + d - 3, ny), ic) * 8u;
+    const int r = ty + d - 3, jj = j + d - 3;
+    lds_y[d] = r >= 0 && r < kTendURows && jj >= 0 && jj < ny;
   }
-  const unsigned ca8 = (unsigned)i2(g, ja, i) * 8u, cjm8 = (unsigned)i2(g, jm, i) * 8u;
-  const unsigned cip8 = (unsigned)i2(g, ja, ip) * 8u, cjmp8 = (unsigned)i2(g, jm, ip) * 8u;
-  const unsigned cjp8 = (unsigned)i2(g, jp, i) * 8u;
-  const double msfvy = g.msfvy[i2v(g, ja, i)];
-  const double ratio = g.msfvy[i2v(g, ja, i)] / g.msfvx[i2v(g, ja, i)];
-  const unsigned fo8 = (unsigned)i2v(g, j, i) * 8u;  // the output face
+  // wave-uniform: row ja-1 (ru, rv) from the wave below, row ja+1 (rv) from
+  // the wave above, the PGF's south row from the wave below
+  const bool lds_m = ty > 0 && j >= 1 && j <= ny;
+  const bool lds_p = ty < kTendURows - 1 && j + 1 <= ny;
+  const bool lds_l = ty > 0;
+  const unsigned ca8 = (unsigned)i2(g, ja, ic) * 8u, cjm8 = (unsigned)i2(g, jm, ic) * 8u;
+  const unsigned cjp8 = (unsigned)i2(g, jp, ic) * 8u;
+  const double msfvy = g.msfvy[i2v(g, ja, ic)];
+  const double ratio = g.msfvy[i2v(g, ja, ic)] / g.msfvx[i2v(g, ja, ic)];
+  const unsigned fo8 = (unsigned)i2v(g, jv, ic) * 8u;  // the output face
   // PGF rows (edge padded)
-  const int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
-  const size_t cl = i2(g, js, i), cr = i2(g, jn, i);
+  const int js = clampi(jv - 1, 0, ny - 1), jn = clampi(jv, 0, ny - 1);
+  const size_t cl = i2(g, js, ic), cr = i2(g, jn, ic);
   const unsigned cl8 = (unsigned)cl * 8u, cr8 = (unsigned)cr * 8u;
-  const double msf_v = g.msfvy[i2v(g, j, i)] / g.msfvx[i2v(g, j, i)];
+  const double msf_v = g.msfvy[i2v(g, jv, ic)] / g.msfvx[i2v(g, jv, ic)];
   const double mub_s = s.mu_total[cl] - s.mu_pert[cl];
   const double mub_n = s.mu_total[cr] - s.mu_pert[cr];
   const double mt_s = mub_s + s.mu_pert[cl];
@@ -575,41 +593,90 @@ __global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
   const double mu_s = s.mu_pert[cl], mu_n = s.mu_pert[cr];
   const double* U = s.v;
   const double* PP = s.p_pert;
+  constexpr int kF = 9;  // LDS fields: v, ru, rv, and the six PGF north-row values
+  __shared__ double sh[2][kTendURows][kF][64];
   struct Lv {
     double x[7], y[7];                          // v at columns i-3..i+3 / rows ja-3..ja+3
     double ru_a, ru_m, ru_p, ru_mp, rv_a, rv_m, rv_p;   // transport pieces
     double pl1, pr1, phl1, phr1;                // p' and ph' at level k+1 (l, r)
     double altl, altr, ptl, ptr, all, alr, phpl, phpr;  // level k diagnostics (l, r)
   };
+  auto shl = [](double v) { return dpp_move<kDppWaveShl1>(0.0, v); };
+  auto shr = [](double v) { return dpp_move<kDppWaveShr1>(0.0, v); };
+  double ru_n = ldo(w.ru, ca8), rv_n = ldo(w.rv, ca8);  // own column, one level ahead
   auto load = [&](int k, Lv& L) {
     const unsigned o = (unsigned)k * pl8, ou = (unsigned)k * upl8;
-#pragma unroll
-    for (int d = 0; d < 7; ++d) L.x[d] = ldo(U, ou + ux[d]);
+    L.x[0] = ldo(U, ou + ux[0]);
+    L.x[1] = ldo(U, ou + ux[1]);
+    L.x[5] = ldo(U, ou + ux[5]);
+    L.x[6] = ldo(U, ou + ux[6]);
 #pragma unroll
     for (int d = 0; d < 7; ++d)
-      if (d != 3) L.y[d] = ldo(U, ou + uy[d]);
-    L.y[3] = L.x[3];
-    L.ru_a = ldo(w.ru, o + ca8);
-    L.ru_m = ldo(w.ru, o + cjm8);
-    L.ru_p = ldo(w.ru, o + cip8);
-    L.ru_mp = ldo(w.ru, o + cjmp8);
-    L.rv_a = ldo(w.rv, o + ca8);
-    L.rv_m = ldo(w.rv, o + cjm8);
-    L.rv_p = ldo(w.rv, o + cjp8);
+      if (d != 3 && !lds_y[d]) L.y[d] = ldo(U, ou + uy[d]);
+    L.ru_a = ru_n;
+    L.rv_a = rv_n;
     if (k + 1 < nz) {
-      L.pl1 = ldo(PP, o + pl8 + cl8);
-      L.pr1 = ldo(PP, o + pl8 + cr8);
+      ru_n = ldo(w.ru, o + pl8 + ca8);
+      rv_n = ldo(w.rv, o + pl8 + ca8);
     }
-    L.phl1 = ldo(s.ph_pert, o + pl8 + cl8);
+    if (!lds_m) {
+      L.ru_m = ldo(w.ru, o + cjm8);
+      L.rv_m = ldo(w.rv, o + cjm8);
+    }
+    if (!lds_p) L.rv_p = ldo(w.rv, o + cjp8);
+    if (k + 1 < nz) {
+      L.pr1 = ldo(PP, o + pl8 + cr8);
+      if (!lds_l) L.pl1 = ldo(PP, o + pl8 + cl8);
+    }
     L.phr1 = ldo(s.ph_pert, o + pl8 + cr8);
-    L.altl = ldo(w.alt_raw, o + cl8);
     L.altr = ldo(w.alt_raw, o + cr8);
-    L.ptl = ldo(s.p_total, o + cl8);
     L.ptr = ldo(s.p_total, o + cr8);
-    L.all = ldo(w.al_ls, o + cl8);
     L.alr = ldo(w.al_ls, o + cr8);
-    L.phpl = ldo(w.php_ls, o + cl8);
     L.phpr = ldo(w.php_ls, o + cr8);
+    if (!lds_l) {
+      L.phl1 = ldo(s.ph_pert, o + pl8 + cl8);
+      L.altl = ldo(w.alt_raw, o + cl8);
+      L.ptl = ldo(s.p_total, o + cl8);
+      L.all = ldo(w.al_ls, o + cl8);
+      L.phpl = ldo(w.php_ls, o + cl8);
+    }
+  };
+  // level k's values from the adjacent lanes and waves
+  auto exchange = [&](int k, Lv& L, double vc) {
+    L.x[3] = vc;
+    L.y[3] = vc;
+    L.x[2] = shr(vc);
+    L.x[4] = shl(vc);
+    double* mine = &sh[k & 1][ty][0][tx];
+    mine[0 * 64] = vc;
+    mine[1 * 64] = L.ru_a;
+    mine[2 * 64] = L.rv_a;
+    mine[3 * 64] = L.pr1;
+    mine[4 * 64] = L.phr1;
+    mine[5 * 64] = L.altr;
+    mine[6 * 64] = L.ptr;
+    mine[7 * 64] = L.alr;
+    mine[8 * 64] = L.phpr;
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 7; ++d)
+      if (d != 3 && lds_y[d]) L.y[d] = sh[k & 1][ty + d - 3][0][tx];
+    if (lds_m) {
+      L.ru_m = sh[k & 1][ty - 1][1][tx];
+      L.rv_m = sh[k & 1][ty - 1][2][tx];
+    }
+    if (lds_p) L.rv_p = sh[k & 1][ty + 1][2][tx];
+    if (lds_l) {
+      const double* below = &sh[k & 1][ty - 1][0][tx];
+      L.pl1 = below[3 * 64];
+      L.phl1 = below[4 * 64];
+      L.altl = below[5 * 64];
+      L.ptl = below[6 * 64];
+      L.all = below[7 * 64];
+      L.phpl = below[8 * 64];
+    }
+    L.ru_p = shl(L.ru_a);
+    L.ru_mp = shl(L.ru_m);
   };
   // centre-face window of v (k-1..k+3) and the face omega (k+1, k+2)
   auto romq = [&](int kk) {
@@ -637,6 +704,7 @@ __global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
     const double qp3 = k + 3 < nz ? ldo(U, (unsigned)(k + 3) * upl8 + ua8) : 0.0;
     const double r2 = k + 2 <= nz ? romq(k + 2) : 0.0;
     load(k, cur);
+    exchange(k, cur, q0);
     // --- advection on the collapsed periodic face ja ---
     auto VX = [&](int sft) { return cur.x[sft + 3]; };
     auto VY = [&](int d) { return cur.y[d + 3]; };
@@ -678,7 +746,7 @@ __global__ void __launch_bounds__(256) k_tend_v(DGrid g, DState s, DWork w) {
     double dpy = (((msf_v * 0.5) * g.rdy) * mass_v) * ((ph_term + p_term) + pb_term);
     double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_s + mu_n));
     dpy = dpy + ((msf_v * g.rdy) * (cur.phpr - cur.phpl)) * bracket;
-    stt<TT>(w.vt, (unsigned)k * upl8 + fo8, t + (-1.0 * dpy));
+    if (owned) stt<TT>(w.vt, (unsigned)k * upl8 + fo8, t + (-1.0 * dpy));
     qm1 = q0;
     q0 = qp1;
     qp1 = qp2;
@@ -1157,14 +1225,15 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const size_t CU = (size_t)g.ny * (g.nx + 1), CV = (size_t)(g.ny + 1) * g.nx;
   const unsigned tu_tiles = (unsigned)((g.nx + 1 + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
   const dim3 tu_block(64, kTendURows);
+  const unsigned tv_tiles = (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + 1 + kTendURows - 1) / kTendURows);
   if (nl.mixed) {
     WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
-    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float><<<blocks_for(CV, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   } else {
     WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
-    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double><<<blocks_for(CV, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   }
