@@ -125,54 +125,86 @@ __device__ inline double dpn_ls_bot(const DGrid& g, double ps0, double ps1, doub
 // other three the registers that takes cost more occupancy than it gains.
 // XCD-contiguous tile order keeps neighbouring rows in one L2.
 
+constexpr int kTendURows = 4;
+
 // advect_scalar_flux of theta-300 (flux_advection.py:482-568) on mass levels.
 // TT: the tendency storage type; float selects the mixed-precision form.
+// Blocks of kTendURows rows of one 64-lane wave each, waves overlapping by one
+// lane either side (lanes 1..62 own columns): the +-1 columns and ru at i+1
+// come from the adjacent lanes (DPP), the rows j-3..j+3 inside the block and
+// rv at j+1 from the adjacent waves (LDS); the +-2, +-3 columns and rows
+// outside the block (or wrapping periodically) are loaded.
 template <class TT>
-__global__ void __launch_bounds__(256) k_tend_t(DGrid g, DState s, DWork w) {
+__global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DState s, DWork w) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t ncol = (size_t)ny * nx;
-  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
-  if (c >= ncol) return;
-  const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
-  const unsigned pl8 = (unsigned)ncol * 8u;
+  const unsigned nbx = (unsigned)(nx + 61) / 62;
+  const unsigned tile = tile_id(g);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = (int)(tile % nbx) * 62 + tx - 1;  // outside 0..nx-1 on the halo lanes
+  const int j = (int)(tile / nbx) * kTendURows + ty;
+  const bool owned = tx >= 1 && tx <= 62 && i < nx && j < ny;
+  const int ic = wrapi(i, nx), jr = j < ny ? j : ny - 1;
+  const unsigned pl8 = (unsigned)((size_t)ny * nx) * 8u;
+  const unsigned c8 = (unsigned)i2(g, jr, ic) * 8u;
   unsigned ox[7], oy[7];  // byte offsets of columns i-3..i+3 (row j) and rows j-3..j+3 (column i)
+  bool lds_y[7];
 #pragma unroll
   for (int d = 0; d < 7; ++d) {
-    ox[d] = (unsigned)i2(g, j, wrapi(i + d - 3, nx)) * 8u;
-    oy[d] = (unsigned)i2(g, wrapi(j + d - 3, ny), i) * 8u;
+    ox[d] = (unsigned)i2(g, jr, wrapi(ic + d - 3, nx)) * 8u;
+    oy[d] = (unsigned)i2(g, wrapi(jr + d - 3, ny), ic) * 8u;
+    const int r = ty + d - 3, jj = j + d - 3;
+    lds_y[d] = r >= 0 && r < kTendURows && jj >= 0 && jj < ny;
   }
-  const unsigned c8 = ox[3], cip8 = ox[4], cjp8 = oy[4];
-  const double msftx = g.msftx[c];
+  const double msftx = g.msftx[c8 / 8u];
   const double* th = s.theta;
+  __shared__ double sh[2][kTendURows][2][64];  // theta and rv at level k
   struct Lv {
     double x[7], y[7];  // theta at columns i-3..i+3 and rows j-3..j+3 (x[3], y[3]: centre)
     double ru0, ru1, rv0, rv1;
   };
-  auto load = [&](int k, Lv& L) {
-    const unsigned o = (unsigned)k * pl8;
-#pragma unroll
-    for (int d = 0; d < 7; ++d) L.x[d] = ldo(th, o + ox[d]);
-#pragma unroll
-    for (int d = 0; d < 7; ++d)
-      if (d != 3) L.y[d] = ldo(th, o + oy[d]);
-    L.y[3] = L.x[3];
-    L.ru0 = ldo(w.ru, o + c8);
-    L.ru1 = ldo(w.ru, o + cip8);
-    L.rv0 = ldo(w.rv, o + c8);
-    L.rv1 = ldo(w.rv, o + cjp8);
-  };
-  // centre-column window of theta-300 (k-1..k+3) and rom (k+1, k+2)
-  double qm1 = 0.0, q0 = ldo(th, c8) - kT0, qp1 = ldo(th, pl8 + c8) - kT0, qp2 = ldo(th, 2 * pl8 + c8) - kT0;
+  // centre-column window of theta (k-1..k+3, raw) and rom (k+1, k+2)
+  double tm1 = 0.0, t0 = ldo(th, c8), tp1 = ldo(th, pl8 + c8), tp2 = ldo(th, 2 * pl8 + c8);
   double rom1 = ldo(w.rom, pl8 + c8);
+  double ru_n = ldo(w.ru, c8), rv_n = ldo(w.rv, c8);  // own column, one level ahead
   double vf_lo = 0.0;  // vertical face flux at k (0 at the surface)
   double rom0 = 0.0;   // mixed form: rom at face k, and its face interpolant (deviation from level k-1)
   float fdev_lo = 0.0f;
   Lv cur{};
   for (int k = 0; k < nz; ++k) {
-    const double qp3 = k + 3 < nz ? ldo(th, (unsigned)(k + 3) * pl8 + c8) - kT0 : 0.0;
+    const unsigned o = (unsigned)k * pl8;
+    const double tp3 = k + 3 < nz ? ldo(th, (unsigned)(k + 3) * pl8 + c8) : 0.0;
     const double rom2 = k + 2 <= nz ? ldo(w.rom, (unsigned)(k + 2) * pl8 + c8) : 0.0;
-    load(k, cur);
+    cur.x[0] = ldo(th, o + ox[0]);
+    cur.x[1] = ldo(th, o + ox[1]);
+    cur.x[5] = ldo(th, o + ox[5]);
+    cur.x[6] = ldo(th, o + ox[6]);
+#pragma unroll
+    for (int d = 0; d < 7; ++d)
+      if (d != 3 && !lds_y[d]) cur.y[d] = ldo(th, o + oy[d]);
+    if (!lds_y[4]) cur.rv1 = ldo(w.rv, o + oy[4]);
+    cur.ru0 = ru_n;
+    cur.rv0 = rv_n;
+    if (k + 1 < nz) {
+      ru_n = ldo(w.ru, o + pl8 + c8);
+      rv_n = ldo(w.rv, o + pl8 + c8);
+    }
+    cur.x[3] = t0;
+    cur.y[3] = t0;
+    cur.x[2] = dpp_move<kDppWaveShr1>(0.0, t0);
+    cur.x[4] = dpp_move<kDppWaveShl1>(0.0, t0);
+    cur.ru1 = dpp_move<kDppWaveShl1>(0.0, cur.ru0);
+    {
+      double* mine = &sh[k & 1][ty][0][tx];
+      mine[0] = t0;
+      mine[64] = cur.rv0;
+      __syncthreads();
+#pragma unroll
+      for (int d = 0; d < 7; ++d)
+        if (d != 3 && lds_y[d]) cur.y[d] = sh[k & 1][ty + d - 3][0][tx];
+      if (lds_y[4]) cur.rv1 = sh[k & 1][ty + 1][1][tx];
+    }
+    const double qm1 = tm1 - kT0, q0 = t0 - kT0, qp1 = tp1 - kT0, qp2 = tp2 - kT0;
     auto QX = [&](int sft) { return cur.x[sft + 3] - kT0; };
     auto QY = [&](int d) { return cur.y[d + 3] - kT0; };
     double t;
@@ -196,11 +228,11 @@ __global__ void __launch_bounds__(256) k_tend_t(DGrid g, DState s, DWork w) {
       t = t - g.rdnw[k] * (vf_hi - vf_lo);
       vf_lo = vf_hi;
     }
-    stt<TT>(w.tht, (unsigned)k * pl8 + c8, t);
-    qm1 = q0;
-    q0 = qp1;
-    qp1 = qp2;
-    qp2 = qp3;
+    if (owned) stt<TT>(w.tht, o + c8, t);
+    tm1 = t0;
+    t0 = tp1;
+    tp1 = tp2;
+    tp2 = tp3;
     rom1 = rom2;
   }
 }
@@ -329,8 +361,6 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
     cur = nxt;
   }
 }
-
-constexpr int kTendURows = 4;
 
 // advect_u_flux (flux_advection.py:1131-1202) + large_step_horizontal_pgf x
 // (rk_addtend_dry.py:329-343, 357-361) on u faces (nz, ny, nx+1).  Face nx
@@ -1225,17 +1255,18 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const size_t CU = (size_t)g.ny * (g.nx + 1), CV = (size_t)(g.ny + 1) * g.nx;
   const unsigned tu_tiles = (unsigned)((g.nx + 1 + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
   const dim3 tu_block(64, kTendURows);
+  const unsigned tt_tiles = (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
   const unsigned tv_tiles = (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + 1 + kTendURows - 1) / kTendURows);
   if (nl.mixed) {
     WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
-    WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<float><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
   } else {
     WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
-    WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
   }
   WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
   WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<dim3(blocks_for(CU, bs), kchunks), bs, 0, st>>>(g, s, w)));
