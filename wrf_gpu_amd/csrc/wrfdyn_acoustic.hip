@@ -45,13 +45,16 @@ __device__ inline double dpn_ac(const DGrid& g, const double* __restrict__ p, in
 template <int BX, int BY, class TT>
 __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const unsigned nbx = (nx + 1 + BX - 1) / BX;
+  // waves overlap by one lane: lanes 1..BX-1 own faces, lane 0 holds the
+  // column west of the first owned face
+  const unsigned nbx = (nx + 1 + BX - 2) / (BX - 1);
   const unsigned t = tile_id(g);
-  const int i = (int)(t % nbx) * BX + threadIdx.x;
+  const int i = (int)(t % nbx) * (BX - 1) + (int)threadIdx.x - 1;
   const int j = (int)(t / nbx) * BY + threadIdx.y;
-  // threads past the face grid stay resident for the per-level barrier; their
-  // mass-column indices are clamped and their face work is switched off
-  const bool active = i <= nx && j <= ny;
+  // threads past the face grid, and lane 0, stay resident for the per-level
+  // barrier; their mass-column indices are clamped and their face work is
+  // switched off
+  const bool active = threadIdx.x >= 1 && i <= nx && j <= ny;
   const bool do_u = active && j < ny;
   const bool do_v = active && i < nx;
   const size_t plane = (size_t)ny * nx;
@@ -104,15 +107,16 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     double p2c, p2w, p2s;                     // p at level k+2 (pair sums)
   };
   const unsigned pl8 = (unsigned)plane * 8u;
-  const unsigned cc8 = (unsigned)cc * 8u, cw8 = (unsigned)cw * 8u, cs8 = (unsigned)cs * 8u;
+  const unsigned cc8 = (unsigned)cc * 8u, cs8 = (unsigned)cs * 8u;
   // a wave is one row of BX faces (blockDim.x == BX == wavefront size)
   static_assert(BX == 64, "k_uv maps one wavefront to one row of faces");
-  const bool row_start = threadIdx.x == 0;
   // row j - 1 is in this block unless this is its first row or j = 0 (whose
   // south column is clamped to row 0 itself)
   const bool row_below_outside = threadIdx.y == 0 || j == 0;
-  const unsigned uq8 = (unsigned)iu(g, 0, j, i) * 8u, upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u;
-  const unsigned vq8 = (unsigned)iv(g, 0, j, i) * 8u, vpl8 = (unsigned)((size_t)(ny + 1) * nx) * 8u;
+  // face offsets (used only where do_u / do_v hold, i.e. for in-range faces)
+  const int ifc = clampi(i, 0, nx), jfc = clampi(j, 0, ny);
+  const unsigned uq8 = (unsigned)iu(g, 0, jfc, ifc) * 8u, upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u;
+  const unsigned vq8 = (unsigned)iv(g, 0, jfc, ifc) * 8u, vpl8 = (unsigned)((size_t)(ny + 1) * nx) * 8u;
   auto load = [&](int k, Lv& L) {
     const unsigned ok = (unsigned)k * pl8, ok1 = ok + pl8;
     L.phc1 = ldo(w.ph_work, ok1 + cc8);
@@ -123,14 +127,6 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     L.pc = ldo(P, ok + cc8);
     if (do_u) {
       const unsigned q = uq8 + (unsigned)k * upl8;
-      if (row_start) {  // the west column is not in this wave
-        L.phw1 = ldo(w.ph_work, ok1 + cw8);
-        L.altw = ldo(w.alt, ok + cw8);
-        L.alw = ldo(w.al, ok + cw8);
-        L.pbw = ldo(w.pb, ok + cw8);
-        L.phpw = ldo(w.php, ok + cw8);
-        L.pw = ldo(P, ok + cw8);
-      }
       L.uw = ldo(w.u_work, q);
       L.ut = ldt<TT>(w.ut, q);
     }
@@ -150,7 +146,6 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     if (k + 2 < nz) {
       const unsigned o2 = ok + 2 * pl8;
       L.p2c = ldo(P, o2 + cc8);
-      if (row_start) L.p2w = ldo(P, o2 + cw8);
       if (do_v && row_below_outside) L.p2s = ldo(P, o2 + cs8);
     }
   };
@@ -181,9 +176,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
   };
   // The west column of face i is the centre column of face i-1, which the
   // next lane down holds: each west value is that lane's centre value moved
-  // up one lane by a DPP wave shift (no memory access).  Lane 0 of the row has
-  // no lower lane; the shift leaves its destination untouched, so it keeps
-  // the value it loaded itself.
+  // up one lane by a DPP wave shift (no memory access).  Lane 0 owns no face.
   auto west = [&](Lv& L) {
     L.phw1 = wave_shr1(L.phw1, L.phc1);
     L.altw = wave_shr1(L.altw, L.altc);
@@ -1362,7 +1355,7 @@ hipError_t launch_column_tile(int tile, const DGrid& g, const DNamelist& nl, con
 hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                                    double dts, int first, int column_tile, hipStream_t st, Prof* prof) {
   constexpr int BX = 64, BY = 4;
-  const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 1) / BX) * (unsigned)((g.ny + 1 + BY - 1) / BY);
+  const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 2) / (BX - 1)) * (unsigned)((g.ny + 1 + BY - 1) / BY);
   if (nl.mixed)
     WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, float><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
   else

@@ -63,15 +63,18 @@ struct Prof {
     hipEvent_t a = next();
     hipEvent_t b = next();
     if (!a || !b) return 0;
-    hipEventRecord(a, st);
+    if (hipEventRecord(a, st) != hipSuccess) {
+      failed = true;
+      return 0;
+    }
     recs.emplace_back(id, idx);
     return idx + 1;
   }
   void end(size_t tok, hipStream_t st) {
-    if (tok) hipEventRecord(pool[tok], st);
+    if (tok && hipEventRecord(pool[tok], st) != hipSuccess) failed = true;
   }
   ~Prof() {
-    for (hipEvent_t e : pool) hipEventDestroy(e);
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
   }
 };
 
