@@ -17,7 +17,8 @@ present) and a CPU baseline (the oracle on a bounded sample of the same
 workload, this host's cores).  The headline value is the plain FP64 form;
 "forms" adds the mixed-precision form on the same workload (throughput and
 per-field error against FP64 after --form-steps large steps, 200 = one
-forecast hour) and the 32-member uneven ensemble (members stepped one after
+forecast hour), the launch-bound 64x64x40 warm bubble stepped eagerly and by
+graph replay, and the 32-member uneven ensemble (members stepped one after
 another, and the batched graph in FP64 and in the mixed form).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -83,6 +84,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all cores in this process's affinity")
     ap.add_argument("--form-steps", type=int, default=200,
                     help="large steps of the mixed-precision form leg (error measured after them); 0 skips it")
+    ap.add_argument("--small-steps", type=int, default=100,
+                    help="large steps of the 64x64x40 warm-bubble leg (eager and graph-replayed); 0 skips it")
     ap.add_argument("--ensemble-steps", type=int, default=3,
                     help="timed large steps per form of the 32-member ensemble leg; 0 skips it")
     ap.add_argument("--breakdown-steps", type=int, default=3,
@@ -205,6 +208,41 @@ def mixed_form_leg(case, steps: int, fp64_ms_per_step: float, dev) -> dict:
             "max_rel_err": max(a for a, _ in errs.values()),
             "note": "the plain FP64 form is bit-identical to the CPU oracle (GPU parity tests), so this is the "
                     "error against the oracle"}
+
+
+def small_workload_leg(steps: int, dev) -> dict:
+    """The 64x64x40 warm bubble (dt 6 s, ns 6): a launch-bound size, stepped
+    eagerly and by replaying the graph-captured large step."""
+
+    import torch
+
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import Dycore, state_to_device
+
+    case = ic.bubble_3d()
+    m, nl = case.metrics, case.namelist
+    stream = torch.cuda.current_stream(dev)
+    out = {"workload": "plain FP64 dry 3-D warm bubble 64x64x40, dx=1 km, dt=6 s, ns=6", "n_cells": m.n_cells,
+           "steps": steps}
+    eager, graphed = Dycore(m, nl), Dycore(m, nl)
+    se, sg = state_to_device(case.state, dev), state_to_device(case.state, dev)
+    eager.step(se, 1)
+    graphed.capture_graph(sg)
+    graphed.graph_step(1)
+    for name, fn in (("eager", lambda n: eager.step(se, n)), ("graph", lambda n: graphed.graph_step(n))):
+        ms = timed_steps(fn, steps, stream) / steps
+        out[name] = {"ms_per_step": ms, "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3)}
+    out["graph_speedup"] = out["eager"]["ms_per_step"] / out["graph"]["ms_per_step"]
+    out["graph_equals_eager"] = all(bool(torch.equal(se[k], sg[k])) for k in se)
+    # the autotuner's pick for this size (k-marching kernels have one wave per 64 columns here)
+    out["autotuned_config"] = graphed.autotune(sg, trial_steps=2)
+    graphed.capture_graph(sg)
+    graphed.graph_step(1)
+    ms = timed_steps(lambda n: graphed.graph_step(n), steps, stream) / steps
+    out["autotuned_graph"] = {"ms_per_step": ms, "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3)}
+    eager.close()
+    graphed.close()
+    return out
 
 
 def ensemble_leg(steps: int, dev) -> dict:
@@ -376,6 +414,9 @@ def main():
     if args.form_steps > 0:
         forms["mixed"] = mixed_form_leg(case, args.form_steps, ms_per_step, dev)
         print(f"[bench] mixed form leg done", file=sys.stderr, flush=True)
+    if args.small_steps > 0:
+        forms["small_workload"] = small_workload_leg(args.small_steps, dev)
+        print(f"[bench] small workload leg done", file=sys.stderr, flush=True)
     if args.ensemble_steps > 0:
         dyn.close()
         del state
