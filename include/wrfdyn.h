@@ -21,8 +21,12 @@
  *      theta, p_* (nz, ny, nx)   mu_*, mudf (ny, nx)
  *    They are advanced in place.
  *  - Grid metrics are HOST pointers read once by wd_create.
- *  - A handle owns its device workspace and is not thread-safe; one stream
- *    per call.  `stream` is a hipStream_t passed as void* (NULL = default).
+ *  - A handle owns its device workspace, on the device that was current at
+ *    wd_create; calls that launch work need that device current (WD_EINVAL
+ *    otherwise).  A handle is not thread-safe and its calls must not overlap
+ *    (including a graph or batch replay still running on another stream when
+ *    wd_autotune or wd_set_precision drops the graph); one stream per call.
+ *    `stream` is a hipStream_t passed as void* (NULL = default).
  */
 #ifndef WRFDYN_H
 #define WRFDYN_H

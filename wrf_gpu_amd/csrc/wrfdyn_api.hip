@@ -64,6 +64,18 @@ int hip_fail(wd_handle* h, hipError_t e, const char* what) {
   return fail(h, WD_EDEVICE, m);
 }
 
+// Launches go to the current device; a handle's workspace lives on the device
+// it was created on.
+int check_device(wd_handle* h) {
+  int cur = 0;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return hip_fail(h, e, "hipGetDevice");
+  if (cur != h->device)
+    return fail(h, WD_EINVAL, "the current HIP device (" + std::to_string(cur) + ") is not the handle's device (" +
+                                  std::to_string(h->device) + ")");
+  return WD_OK;
+}
+
 int check_state(wd_handle* h, const wd_state* st, wd::DState* out) {
   if (!st) return fail(h, WD_EINVAL, "state is NULL");
   double* ptrs[] = {st->u, st->v, st->w, st->theta, st->p_total, st->p_pert, st->ph_total,
@@ -406,6 +418,7 @@ int wd_step(wd_handle* h, const wd_state* st, int n_large_steps, void* stream) {
   if (n_large_steps < 0) return fail(h, WD_EINVAL, "n_large_steps must be >= 0");
   wd::DState ds;
   int rc = check_state(h, st, &ds);
+  if (rc == WD_OK) rc = check_device(h);
   if (rc != WD_OK) return rc;
   return enqueue_steps(h, ds, n_large_steps, static_cast<hipStream_t>(stream));
 }
@@ -415,6 +428,7 @@ int wd_capture_graph(wd_handle* h, const wd_state* st) {
   if (h->prof.mask) return fail(h, WD_EINVAL, "disable profiling (wd_profile_enable(h, 0)) before capturing");
   wd::DState ds;
   int rc = check_state(h, st, &ds);
+  if (rc == WD_OK) rc = check_device(h);
   if (rc != WD_OK) return rc;
   hipError_t e;
   if (!h->capture_stream) {
@@ -444,6 +458,7 @@ int wd_graph_step(wd_handle* h, int n_large_steps, void* stream) {
   if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
   if (!h->graph) return fail(h, WD_EINVAL, "no captured graph: call wd_capture_graph first");
   if (n_large_steps < 0) return fail(h, WD_EINVAL, "n_large_steps must be >= 0");
+  if (int rc = check_device(h); rc != WD_OK) return rc;
   for (int i = 0; i < n_large_steps; ++i) {
     hipError_t e = hipGraphLaunch(h->graph, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(h, e, "hipGraphLaunch");
@@ -457,6 +472,7 @@ int wd_autotune(wd_handle* h, const wd_state* st, int trial_steps, void* stream)
   if (h->prof.mask) return fail(h, WD_EINVAL, "disable profiling before autotuning");
   wd::DState src;
   int rc = check_state(h, st, &src);
+  if (rc == WD_OK) rc = check_device(h);
   if (rc != WD_OK) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   // scratch copy of the state: trials advance it, the caller's state is untouched
@@ -614,6 +630,11 @@ int wd_step_batched(wd_batch* b, int n_large_steps, void* stream) {
   if (!b) return fail(nullptr, WD_EINVAL, "batch is NULL");
   if (n_large_steps < 0) {
     b->err = "n_large_steps must be >= 0";
+    return WD_EINVAL;
+  }
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess || cur != b->device) {
+    b->err = "the current HIP device is not the batch's device (" + std::to_string(b->device) + ")";
     return WD_EINVAL;
   }
   for (int i = 0; i < n_large_steps; ++i) {
