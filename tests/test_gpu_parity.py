@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle.
 
 Bar: per prognostic field, max|gpu - oracle| / max|oracle| <= 1e-12 (FP64,
-the north-star tolerance for the plain path), on seeded inputs at sizes the
+the tolerance set for the plain FP64 path), on seeded inputs at sizes the
 oracle finishes in seconds; the 64x64x40 warm bubble is checked over the full
 100 large steps.
 """
