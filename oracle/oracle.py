@@ -5,7 +5,9 @@ baseline.  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
 cpu_baseline leg import this module; the product never does.
 
 ``build()`` compiles ``liboracle.so`` from ``wrfdyn_oracle.c`` with the
-Makefile beside it.
+Makefile beside it, and ``liboracle_grad.so``: the same source compiled as
+C++ with a recording scalar (``wrfdyn_gradient.cpp``), whose ``orc_vjp`` is
+the vector-Jacobian product of the oracle's large step (``vjp`` below).
 """
 
 from __future__ import annotations
@@ -20,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 # Variant built with the reference's pre-v0.14 dycore constants (see Makefile).
 LEGACY_LIB_PATH = os.path.join(HERE, "liboracle_v013.so")
+GRAD_LIB_PATH = os.path.join(HERE, "liboracle_grad.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -63,7 +66,7 @@ _libs: dict = {}
 def build(force: bool = False) -> str:
     """Compile liboracle.so (gcc, -ffp-contract=off) and return its path."""
 
-    if force or not (os.path.exists(LIB_PATH) and os.path.exists(LEGACY_LIB_PATH)):
+    if force or not all(os.path.exists(p) for p in (LIB_PATH, LEGACY_LIB_PATH, GRAD_LIB_PATH)):
         cmd = ["make", "-s", "-C", HERE] + (["-B"] if force else [])
         subprocess.run(cmd, check=True, capture_output=True, text=True)
     return LIB_PATH
@@ -132,17 +135,68 @@ def run(metrics, namelist, state: dict, n_steps: int, threads: int | None = None
         L.orc_set_threads(int(threads))
     keep = _Keep()
     g = _grid(metrics, keep)
-    nl = _Namelist(float(namelist.dt_s), int(namelist.acoustic_substeps), float(namelist.epssm),
-                   int(bool(namelist.top_lid)), int(namelist.w_damping), int(namelist.damp_opt),
-                   float(namelist.dampcoef), float(namelist.zdamp))
-    out = {k: np.array(v, dtype=np.float64, copy=True, order="C") for k, v in state.items()}
-    if "mudf" not in out:
-        out["mudf"] = np.zeros((metrics.ny, metrics.nx))
+    nl = _namelist(namelist)
+    out = _state_copy(metrics, state)
     s = _State(*[_ptr(out[_STATE_KEYS[n]]) for n, _ in _State._fields_])
     rc = L.orc_run(ctypes.byref(g), ctypes.byref(nl), ctypes.byref(s), int(n_steps))
     if rc != 0:
         raise RuntimeError(f"orc_run failed with code {rc}")
     return out
+
+
+def _namelist(namelist) -> _Namelist:
+    return _Namelist(float(namelist.dt_s), int(namelist.acoustic_substeps), float(namelist.epssm),
+                     int(bool(namelist.top_lid)), int(namelist.w_damping), int(namelist.damp_opt),
+                     float(namelist.dampcoef), float(namelist.zdamp))
+
+
+def _state_copy(metrics, state: dict) -> dict:
+    out = {k: np.array(v, dtype=np.float64, copy=True, order="C") for k, v in state.items()}
+    if "mudf" not in out:
+        out["mudf"] = np.zeros((metrics.ny, metrics.nx))
+    return out
+
+
+def grad_lib() -> ctypes.CDLL:
+    if GRAD_LIB_PATH not in _libs:
+        if not os.path.exists(GRAD_LIB_PATH):
+            build()
+        L = ctypes.CDLL(GRAD_LIB_PATH)
+        L.orc_vjp.restype = ctypes.c_int
+        L.orc_vjp.argtypes = [ctypes.POINTER(_Grid), ctypes.POINTER(_Namelist), ctypes.POINTER(_State),
+                              ctypes.c_int, ctypes.POINTER(_State), ctypes.POINTER(_State),
+                              ctypes.POINTER(_State)]
+        _libs[GRAD_LIB_PATH] = L
+    return _libs[GRAD_LIB_PATH]
+
+
+def vjp(metrics, namelist, state: dict, n_steps: int, bar: dict) -> tuple:
+    """Vector-Jacobian product of ``n_steps`` large steps of the oracle.
+
+    ``bar`` holds the adjoint of each field of the final state (missing keys
+    count as zero).  Returns ``(grad, final)``: ``grad`` the adjoint of each
+    field of the starting ``state`` (every state array an independent input)
+    and ``final`` the state after ``n_steps``, which equals ``run``'s.
+    Single-threaded and memory-hungry (one tape node per recorded operation
+    of a large step): meant for small grids.
+    """
+
+    L = grad_lib()
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    nl = _namelist(namelist)
+    start = _state_copy(metrics, state)
+    seed = {k: np.ascontiguousarray(bar[k], dtype=np.float64) if k in bar else np.zeros_like(v)
+            for k, v in start.items()}
+    grad = {k: np.zeros_like(v) for k, v in start.items()}
+    final = {k: np.zeros_like(v) for k, v in start.items()}
+    mk = lambda d: _State(*[_ptr(d[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    s_in, s_bar, s_grad, s_out = mk(start), mk(seed), mk(grad), mk(final)
+    rc = L.orc_vjp(ctypes.byref(g), ctypes.byref(nl), ctypes.byref(s_in), int(n_steps), ctypes.byref(s_bar),
+                   ctypes.byref(s_grad), ctypes.byref(s_out))
+    if rc != 0:
+        raise RuntimeError(f"orc_vjp failed with code {rc}")
+    return grad, final
 
 
 def calc_coef_w(metrics, mut: np.ndarray, c2a: np.ndarray, dts: float, epssm: float, top_lid=True):

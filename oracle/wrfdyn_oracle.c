@@ -46,6 +46,16 @@
 #include <omp.h>
 #endif
 
+/* Scalar type of every value the step computes.  The library build uses
+ * double.  wrfdyn_gradient.cpp compiles this same file as C++ with ORC_AD
+ * defined and `real` a recording scalar, so that its reverse sweep is the
+ * derivative of exactly these expressions; that build supplies its own sgn
+ * (with a zero derivative, as jnp.sign's) and leaves out the double entry
+ * points below. */
+#ifndef ORC_AD
+typedef double real;
+#endif
+
 #define R_D 287.0
 /* ORC_LEGACY_V013 rebuilds the oracle with the two dycore constants the
  * reference changed after its v0.13 golden warm-bubble runs (cp 1004 -> 7R/2,
@@ -94,11 +104,13 @@ static inline int wrap(int i, int n) {
 }
 static inline int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
+#ifndef ORC_AD
 /* jnp.sign: -1, +1, and the argument itself for +-0 and NaN. */
-static inline double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+static inline real sgn(real x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+#endif
 
 /* jnp.where(|x|>floor, x, floor) */
-static inline double safe_pos(double x, double floor) { return fabs(x) > floor ? x : floor; }
+static inline real safe_pos(real x, double floor) { return fabs(x) > floor ? x : floor; }
 
 /* The deterministic elementary functions shared with the HIP kernels
  * (include/wrfdyn_math.h), exported for their accuracy tests. */
@@ -119,37 +131,51 @@ void orc_set_threads(int n) {
 
 /* _inverse_density_from_theta_pressure, dynamics/acoustic_wrf.py:178-195
  * (qvf = 1.0: state theta is theta_m). */
-double orc_alt_from_theta_p(double theta, double p) {
-    double ps = p > MIN_PRESSURE ? p : MIN_PRESSURE;
+static real alt_from_theta_p(real theta, real p) {
+    real ps = p > MIN_PRESSURE ? p : MIN_PRESSURE;
     return (((R_D / P0_PA) * theta) * 1.0) * wd_pow(ps / P0_PA, CVPM);
 }
 
 /* _pressure_from_theta_alt, dynamics/acoustic_wrf.py:198-211. */
-double orc_p_from_theta_alt(double theta, double alt) {
-    double as = alt > MIN_ALT ? alt : MIN_ALT;
-    double arg = ((R_D * theta) * 1.0) / (P0_PA * as);
-    double a2 = arg > 1.0e-12 ? arg : 1.0e-12;
+static real p_from_theta_alt(real theta, real alt) {
+    real as = alt > MIN_ALT ? alt : MIN_ALT;
+    real arg = ((R_D * theta) * 1.0) / (P0_PA * as);
+    real a2 = arg > 1.0e-12 ? arg : 1.0e-12;
     return P0_PA * wd_pow(a2, CPOVCV);
 }
+
+#ifndef ORC_AD
+double orc_alt_from_theta_p(double theta, double p) { return alt_from_theta_p(theta, p); }
+double orc_p_from_theta_alt(double theta, double alt) { return p_from_theta_alt(theta, alt); }
+#endif
 
 /* ------------------------------------------------------------------ */
 /* WRF flux operators, dynamics/flux_advection.py:53-88                 */
 /* ------------------------------------------------------------------ */
 
+/* The prognostic state as the step computes on it (orc_state's fields). */
+typedef struct {
+    real *u, *v, *w, *theta;
+    real *p_total, *p_pert;
+    real *ph_total, *ph_pert;
+    real *mu_total, *mu_pert;
+    real *mudf;
+} rstate;
+
 /* flux5_face_periodic: face value at the left face of cell i. */
-static inline double flux5(double q_im3, double q_im2, double q_im1, double q_i, double q_ip1,
-                           double q_ip2, double vel) {
-    double f6 = ((37.0 * (q_i + q_im1) - 8.0 * (q_ip1 + q_im2)) + (q_ip2 + q_im3)) / 60.0;
-    double corr = (((q_ip2 - q_im3) - 5.0 * (q_ip1 - q_im2)) + 10.0 * (q_i - q_im1)) / 60.0;
+static inline real flux5(real q_im3, real q_im2, real q_im1, real q_i, real q_ip1,
+                           real q_ip2, real vel) {
+    real f6 = ((37.0 * (q_i + q_im1) - 8.0 * (q_ip1 + q_im2)) + (q_ip2 + q_im3)) / 60.0;
+    real corr = (((q_ip2 - q_im3) - 5.0 * (q_ip1 - q_im2)) + 10.0 * (q_i - q_im1)) / 60.0;
     return f6 - sgn(vel) * corr;
 }
 
 /* flux3 as written in advect_scalar_flux / _vertical_flux_div_3 /
  * _vertical_flux_div_w (flux_advection.py:537-547,1388-1397,1452-1467):
  * flux4 + sign(velz)*corr with the caller choosing velz. */
-static inline double flux3(double q_km2, double q_km1, double q_k, double q_kp1, double velz) {
-    double f4 = (7.0 * (q_k + q_km1) - (q_kp1 + q_km2)) / 12.0;
-    double corr = ((q_kp1 - q_km2) - 3.0 * (q_k - q_km1)) / 12.0;
+static inline real flux3(real q_km2, real q_km1, real q_k, real q_kp1, real velz) {
+    real f4 = (7.0 * (q_k + q_km1) - (q_kp1 + q_km2)) / 12.0;
+    real corr = ((q_kp1 - q_km2) - 3.0 * (q_k - q_km1)) / 12.0;
     return f4 + sgn(velz) * corr;
 }
 
@@ -157,19 +183,19 @@ static inline double flux3(double q_km2, double q_km1, double q_k, double q_kp1,
 /* couple_velocities_periodic, dynamics/flux_advection.py:394-470       */
 /* ru, rv are collapsed to (nz, ny, nx); rom is (nz+1, ny, nx).         */
 /* ------------------------------------------------------------------ */
-static void couple_velocities(const orc_grid *g, const double *u, const double *v,
-                              const double *mu, double *ru, double *rv, double *rom,
-                              double *divv) {
+static void couple_velocities(const orc_grid *g, const real *u, const real *v,
+                              const real *mu, real *ru, real *rv, real *rom,
+                              real *divv) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         for (int k = 0; k < nz; ++k) {
             for (int i = 0; i < nx; ++i) {
-                double muu = 0.5 * (mu[I2(j, i)] + mu[I2(j, wrap(i - 1, nx))]);
-                double muv = 0.5 * (mu[I2(j, i)] + mu[I2(wrap(j - 1, ny), i)]);
-                double mass_u = g->c1h[k] * muu + g->c2h[k];
-                double mass_v = g->c1h[k] * muv + g->c2h[k];
+                real muu = 0.5 * (mu[I2(j, i)] + mu[I2(j, wrap(i - 1, nx))]);
+                real muv = 0.5 * (mu[I2(j, i)] + mu[I2(wrap(j - 1, ny), i)]);
+                real mass_u = g->c1h[k] * muu + g->c2h[k];
+                real mass_v = g->c1h[k] * muv + g->c2h[k];
                 ru[IM(k, j, i)] = mass_u * u[IU(k, j, i)] / g->msfuy[I2U(j, i)];
                 rv[IM(k, j, i)] = mass_v * v[IV(k, j, i)] / g->msfvx[I2V(j, i)];
             }
@@ -178,18 +204,18 @@ static void couple_velocities(const orc_grid *g, const double *u, const double *
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) {
-            double dmdt = 0.0;
+            real dmdt = 0.0;
             for (int k = 0; k < nz; ++k) {
-                double d = (g->dnw[k] * g->msftx[I2(j, i)]) *
+                real d = (g->dnw[k] * g->msftx[I2(j, i)]) *
                            (rdx * (ru[IM(k, j, wrap(i + 1, nx))] - ru[IM(k, j, i)]) +
                             rdy * (rv[IM(k, wrap(j + 1, ny), i)] - rv[IM(k, j, i)]));
                 divv[IM(k, j, i)] = d;
                 dmdt += d;
             }
-            double cum = 0.0;
+            real cum = 0.0;
             rom[IM(0, j, i)] = 0.0;
             for (int k = 0; k < nz; ++k) {
-                double incr = -((g->dnw[k] * g->c1h[k]) * dmdt) - divv[IM(k, j, i)];
+                real incr = -((g->dnw[k] * g->c1h[k]) * dmdt) - divv[IM(k, j, i)];
                 cum = (k == 0) ? incr : cum + incr;
                 if (k + 1 < nz) rom[IM(k + 1, j, i)] = cum;
             }
@@ -199,15 +225,16 @@ static void couple_velocities(const orc_grid *g, const double *u, const double *
 }
 
 /* Public helper: advect_scalar_flux of `field` with the velocities of (u,v,mu). */
-static void advect_scalar_flux(const orc_grid *g, const double *field, const double *ru,
-                               const double *rv, const double *rom, double *tend);
+static void advect_scalar_flux(const orc_grid *g, const real *field, const real *ru,
+                               const real *rv, const real *rom, real *tend);
 
+#ifndef ORC_AD
 int orc_advect_scalar(const orc_grid *g, const double *u, const double *v,
                       const double *mu_total, const double *field, double *tend) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     size_t m = (size_t)nz * ny * nx, w = (size_t)(nz + 1) * ny * nx;
-    double *ru = malloc(m * sizeof(double)), *rv = malloc(m * sizeof(double));
-    double *rom = malloc(w * sizeof(double)), *dv = malloc(m * sizeof(double));
+    real *ru = (real *)malloc(m * sizeof(real)), *rv = (real *)malloc(m * sizeof(real));
+    real *rom = (real *)malloc(w * sizeof(real)), *dv = (real *)malloc(m * sizeof(real));
     if (!ru || !rv || !rom || !dv) {
         free(ru); free(rv); free(rom); free(dv);
         return -1;
@@ -217,12 +244,13 @@ int orc_advect_scalar(const orc_grid *g, const double *u, const double *v,
     free(ru); free(rv); free(rom); free(dv);
     return 0;
 }
+#endif
 
 /* advect_scalar_flux, dynamics/flux_advection.py:482-568 (periodic branch). */
-static void advect_scalar_flux(const orc_grid *g, const double *q, const double *ru,
-                               const double *rv, const double *rom, double *tend) {
+static void advect_scalar_flux(const orc_grid *g, const real *q, const real *ru,
+                               const real *rv, const real *rom, real *tend) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         for (int k = 0; k < nz; ++k) {
@@ -230,14 +258,14 @@ static void advect_scalar_flux(const orc_grid *g, const double *q, const double 
 #define QX(s) q[IM(k, j, wrap(i + (s), nx))]
 #define QY(jj, s) q[IM(k, wrap((jj) + (s), ny), i)]
                 int ip = wrap(i + 1, nx);
-                double vel0 = ru[IM(k, j, i)], vel1 = ru[IM(k, j, ip)];
-                double fq0 = vel0 * flux5(QX(-3), QX(-2), QX(-1), QX(0), QX(1), QX(2), vel0);
-                double fq1 = vel1 * flux5(QX(-2), QX(-1), QX(0), QX(1), QX(2), QX(3), vel1);
-                double t = ((-g->msftx[I2(j, i)]) * rdx) * (fq1 - fq0);
+                real vel0 = ru[IM(k, j, i)], vel1 = ru[IM(k, j, ip)];
+                real fq0 = vel0 * flux5(QX(-3), QX(-2), QX(-1), QX(0), QX(1), QX(2), vel0);
+                real fq1 = vel1 * flux5(QX(-2), QX(-1), QX(0), QX(1), QX(2), QX(3), vel1);
+                real t = ((-g->msftx[I2(j, i)]) * rdx) * (fq1 - fq0);
                 int jp = wrap(j + 1, ny);
-                double vy0 = rv[IM(k, j, i)], vy1 = rv[IM(k, jp, i)];
-                double gy0 = vy0 * flux5(QY(j, -3), QY(j, -2), QY(j, -1), QY(j, 0), QY(j, 1), QY(j, 2), vy0);
-                double gy1 = vy1 * flux5(QY(jp, -3), QY(jp, -2), QY(jp, -1), QY(jp, 0), QY(jp, 1), QY(jp, 2), vy1);
+                real vy0 = rv[IM(k, j, i)], vy1 = rv[IM(k, jp, i)];
+                real gy0 = vy0 * flux5(QY(j, -3), QY(j, -2), QY(j, -1), QY(j, 0), QY(j, 1), QY(j, 2), vy0);
+                real gy1 = vy1 * flux5(QY(jp, -3), QY(jp, -2), QY(jp, -1), QY(jp, 0), QY(jp, 1), QY(jp, 2), vy1);
                 t = t - (g->msftx[I2(j, i)] * rdy) * (gy1 - gy0);
                 tend[IM(k, j, i)] = t;
 #undef QX
@@ -249,10 +277,10 @@ static void advect_scalar_flux(const orc_grid *g, const double *q, const double 
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) {
-            double vprev = 0.0; /* vflux[0] */
+            real vprev = 0.0; /* vflux[0] */
             for (int k = 0; k < nz; ++k) {
                 int kf = k + 1; /* upper face */
-                double vf;
+                real vf;
                 if (kf == nz) {
                     vf = 0.0;
                 } else if (kf == 1) {
@@ -261,7 +289,7 @@ static void advect_scalar_flux(const orc_grid *g, const double *q, const double 
                     vf = rom[IM(nz - 1, j, i)] *
                          (g->fnm[nz - 1] * q[IM(nz - 1, j, i)] + g->fnp[nz - 1] * q[IM(nz - 2, j, i)]);
                 } else {
-                    double r = rom[IM(kf, j, i)];
+                    real r = rom[IM(kf, j, i)];
                     vf = r * flux3(q[IM(kf - 2, j, i)], q[IM(kf - 1, j, i)], q[IM(kf, j, i)],
                                    q[IM(kf + 1, j, i)], -r);
                 }
@@ -274,25 +302,25 @@ static void advect_scalar_flux(const orc_grid *g, const double *q, const double 
 
 /* _vertical_flux_div_3, dynamics/flux_advection.py:1362-1407 for one column of a
  * mass-level field q (stride sq between levels) with transport romq (stride sr). */
-static inline double vflux3_face(const orc_grid *g, int kf, const double *q, size_t sq,
-                                 const double *romq, size_t sr) {
+static inline real vflux3_face(const orc_grid *g, int kf, const real *q, size_t sq,
+                                 const real *romq, size_t sr) {
     const int nz = g->nz;
     if (kf == 0 || kf == nz) return 0.0;
     if (kf == 1) return romq[1 * sr] * (g->fnm[1] * q[1 * sq] + g->fnp[1] * q[0]);
     if (kf == nz - 1)
         return romq[(size_t)(nz - 1) * sr] *
                (g->fnm[nz - 1] * q[(size_t)(nz - 1) * sq] + g->fnp[nz - 1] * q[(size_t)(nz - 2) * sq]);
-    double r = romq[(size_t)kf * sr];
+    real r = romq[(size_t)kf * sr];
     return r * flux3(q[(size_t)(kf - 2) * sq], q[(size_t)(kf - 1) * sq], q[(size_t)kf * sq],
                      q[(size_t)(kf + 1) * sq], -r);
 }
 
 /* advect_u_flux, dynamics/flux_advection.py:1131-1202 (periodic branch).
  * u_t is (nz, ny, nx+1) with face nx restored from face 0. */
-static void advect_u_flux(const orc_grid *g, const double *u, const double *ru, const double *rv,
-                          const double *rom, double *ut) {
+static void advect_u_flux(const orc_grid *g, const real *u, const real *ru, const real *rv,
+                          const real *rom, real *ut) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         for (int k = 0; k < nz; ++k) {
@@ -300,18 +328,18 @@ static void advect_u_flux(const orc_grid *g, const double *u, const double *ru, 
 #define UX(s) u[IU(k, j, wrap(i + (s), nx))]
 #define UY(jj, s) u[IU(k, wrap((jj) + (s), ny), i)]
                 int ip = wrap(i + 1, nx), im = wrap(i - 1, nx);
-                double velx0 = 0.5 * (ru[IM(k, j, i)] + ru[IM(k, j, im)]);
-                double velx1 = 0.5 * (ru[IM(k, j, ip)] + ru[IM(k, j, i)]);
-                double fq0 = velx0 * flux5(UX(-3), UX(-2), UX(-1), UX(0), UX(1), UX(2), velx0);
-                double fq1 = velx1 * flux5(UX(-2), UX(-1), UX(0), UX(1), UX(2), UX(3), velx1);
-                double msf = g->msfux[I2U(j, i)];
-                double t = ((-msf) * rdx) * (fq1 - fq0);
+                real velx0 = 0.5 * (ru[IM(k, j, i)] + ru[IM(k, j, im)]);
+                real velx1 = 0.5 * (ru[IM(k, j, ip)] + ru[IM(k, j, i)]);
+                real fq0 = velx0 * flux5(UX(-3), UX(-2), UX(-1), UX(0), UX(1), UX(2), velx0);
+                real fq1 = velx1 * flux5(UX(-2), UX(-1), UX(0), UX(1), UX(2), UX(3), velx1);
+                real msf = g->msfux[I2U(j, i)];
+                real t = ((-msf) * rdx) * (fq1 - fq0);
                 if (ny > 1) {
                     int jp = wrap(j + 1, ny);
-                    double vy0 = 0.5 * (rv[IM(k, j, i)] + rv[IM(k, j, im)]);
-                    double vy1 = 0.5 * (rv[IM(k, jp, i)] + rv[IM(k, jp, im)]);
-                    double g0 = vy0 * flux5(UY(j, -3), UY(j, -2), UY(j, -1), UY(j, 0), UY(j, 1), UY(j, 2), vy0);
-                    double g1 = vy1 * flux5(UY(jp, -3), UY(jp, -2), UY(jp, -1), UY(jp, 0), UY(jp, 1), UY(jp, 2), vy1);
+                    real vy0 = 0.5 * (rv[IM(k, j, i)] + rv[IM(k, j, im)]);
+                    real vy1 = 0.5 * (rv[IM(k, jp, i)] + rv[IM(k, jp, im)]);
+                    real g0 = vy0 * flux5(UY(j, -3), UY(j, -2), UY(j, -1), UY(j, 0), UY(j, 1), UY(j, 2), vy0);
+                    real g1 = vy1 * flux5(UY(jp, -3), UY(jp, -2), UY(jp, -1), UY(jp, 0), UY(jp, 1), UY(jp, 2), vy1);
                     t = t - (msf * rdy) * (g1 - g0);
                 }
                 ut[IU(k, j, i)] = t;
@@ -322,16 +350,16 @@ static void advect_u_flux(const orc_grid *g, const double *u, const double *ru, 
     }
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
-        double *romq = malloc((size_t)(nz + 1) * sizeof(double));
-        double *col = malloc((size_t)nz * sizeof(double));
+        real *romq = (real *)malloc((size_t)(nz + 1) * sizeof(real));
+        real *col = (real *)malloc((size_t)nz * sizeof(real));
         for (int i = 0; i < nx; ++i) {
             int im = wrap(i - 1, nx);
             for (int k = 0; k <= nz; ++k) romq[k] = 0.5 * (rom[IM(k, j, i)] + rom[IM(k, j, im)]);
             for (int k = 0; k < nz; ++k) col[k] = u[IU(k, j, i)];
-            double vprev = vflux3_face(g, 0, col, 1, romq, 1);
+            real vprev = vflux3_face(g, 0, col, 1, romq, 1);
             for (int k = 0; k < nz; ++k) {
-                double vf = vflux3_face(g, k + 1, col, 1, romq, 1);
-                double vd = (-g->rdnw[k]) * (vf - vprev);
+                real vf = vflux3_face(g, k + 1, col, 1, romq, 1);
+                real vd = (-g->rdnw[k]) * (vf - vprev);
                 ut[IU(k, j, i)] = ut[IU(k, j, i)] + vd;
                 vprev = vf;
             }
@@ -345,10 +373,10 @@ static void advect_u_flux(const orc_grid *g, const double *u, const double *ru, 
 
 /* advect_v_flux, dynamics/flux_advection.py:1205-1274 (periodic branch).
  * v_t is (nz, ny+1, nx) with row ny restored from row 0. */
-static void advect_v_flux(const orc_grid *g, const double *v, const double *ru, const double *rv,
-                          const double *rom, double *vt) {
+static void advect_v_flux(const orc_grid *g, const real *v, const real *ru, const real *rv,
+                          const real *rom, real *vt) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         int jm = wrap(j - 1, ny), jp = wrap(j + 1, ny), jpm = j; /* row jp's south neighbour is j */
@@ -358,16 +386,16 @@ static void advect_v_flux(const orc_grid *g, const double *v, const double *ru, 
 #define VX(s) v[IV(k, j, wrap(i + (s), nx))]
 #define VY(jj, s) v[IV(k, wrap((jj) + (s), ny), i)]
                 int ip = wrap(i + 1, nx);
-                double vx0 = 0.5 * (ru[IM(k, j, i)] + ru[IM(k, jm, i)]);
-                double vx1 = 0.5 * (ru[IM(k, j, ip)] + ru[IM(k, jm, ip)]);
-                double fq0 = vx0 * flux5(VX(-3), VX(-2), VX(-1), VX(0), VX(1), VX(2), vx0);
-                double fq1 = vx1 * flux5(VX(-2), VX(-1), VX(0), VX(1), VX(2), VX(3), vx1);
-                double msfvy = g->msfvy[I2V(j, i)];
-                double t = ((-msfvy) * rdx) * (fq1 - fq0);
-                double vy0 = 0.5 * (rv[IM(k, j, i)] + rv[IM(k, jm, i)]);
-                double vy1 = 0.5 * (rv[IM(k, jp, i)] + rv[IM(k, j, i)]);
-                double g0 = vy0 * flux5(VY(j, -3), VY(j, -2), VY(j, -1), VY(j, 0), VY(j, 1), VY(j, 2), vy0);
-                double g1 = vy1 * flux5(VY(jp, -3), VY(jp, -2), VY(jp, -1), VY(jp, 0), VY(jp, 1), VY(jp, 2), vy1);
+                real vx0 = 0.5 * (ru[IM(k, j, i)] + ru[IM(k, jm, i)]);
+                real vx1 = 0.5 * (ru[IM(k, j, ip)] + ru[IM(k, jm, ip)]);
+                real fq0 = vx0 * flux5(VX(-3), VX(-2), VX(-1), VX(0), VX(1), VX(2), vx0);
+                real fq1 = vx1 * flux5(VX(-2), VX(-1), VX(0), VX(1), VX(2), VX(3), vx1);
+                real msfvy = g->msfvy[I2V(j, i)];
+                real t = ((-msfvy) * rdx) * (fq1 - fq0);
+                real vy0 = 0.5 * (rv[IM(k, j, i)] + rv[IM(k, jm, i)]);
+                real vy1 = 0.5 * (rv[IM(k, jp, i)] + rv[IM(k, j, i)]);
+                real g0 = vy0 * flux5(VY(j, -3), VY(j, -2), VY(j, -1), VY(j, 0), VY(j, 1), VY(j, 2), vy0);
+                real g1 = vy1 * flux5(VY(jp, -3), VY(jp, -2), VY(jp, -1), VY(jp, 0), VY(jp, 1), VY(jp, 2), vy1);
                 t = t - (msfvy * rdy) * (g1 - g0);
                 vt[IV(k, j, i)] = t;
 #undef VX
@@ -378,16 +406,16 @@ static void advect_v_flux(const orc_grid *g, const double *v, const double *ru, 
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         int jm = wrap(j - 1, ny);
-        double *romq = malloc((size_t)(nz + 1) * sizeof(double));
-        double *col = malloc((size_t)nz * sizeof(double));
+        real *romq = (real *)malloc((size_t)(nz + 1) * sizeof(real));
+        real *col = (real *)malloc((size_t)nz * sizeof(real));
         for (int i = 0; i < nx; ++i) {
             for (int k = 0; k <= nz; ++k) romq[k] = 0.5 * (rom[IM(k, j, i)] + rom[IM(k, jm, i)]);
             for (int k = 0; k < nz; ++k) col[k] = v[IV(k, j, i)];
-            double ratio = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
-            double vprev = vflux3_face(g, 0, col, 1, romq, 1);
+            real ratio = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
+            real vprev = vflux3_face(g, 0, col, 1, romq, 1);
             for (int k = 0; k < nz; ++k) {
-                double vf = vflux3_face(g, k + 1, col, 1, romq, 1);
-                double vd = (-g->rdnw[k]) * (vf - vprev);
+                real vf = vflux3_face(g, k + 1, col, 1, romq, 1);
+                real vd = (-g->rdnw[k]) * (vf - vprev);
                 vt[IV(k, j, i)] = vt[IV(k, j, i)] + ratio * vd;
                 vprev = vf;
             }
@@ -400,7 +428,7 @@ static void advect_v_flux(const orc_grid *g, const double *v, const double *ru, 
 }
 
 /* _mass_to_full_levels, dynamics/flux_advection.py:1332-1348 */
-static inline double mass_to_full(const orc_grid *g, const double *r, int kf, int j, int i) {
+static inline real mass_to_full(const orc_grid *g, const real *r, int kf, int j, int i) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     if (kf == 0) return r[IM(0, j, i)];
     if (kf == nz) return r[IM(nz - 1, j, i)];
@@ -409,10 +437,10 @@ static inline double mass_to_full(const orc_grid *g, const double *r, int kf, in
 
 /* advect_w_flux + _vertical_flux_div_w (top_lid=True),
  * dynamics/flux_advection.py:1277-1329, :1410-1482. */
-static void advect_w_flux(const orc_grid *g, const double *w, const double *ru, const double *rv,
-                          const double *rom, double *wt) {
+static void advect_w_flux(const orc_grid *g, const real *w, const real *ru, const real *rv,
+                          const real *rom, real *wt) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         for (int k = 0; k <= nz; ++k) {
@@ -420,16 +448,16 @@ static void advect_w_flux(const orc_grid *g, const double *w, const double *ru, 
 #define WX(s) w[IM(k, j, wrap(i + (s), nx))]
 #define WY(jj, s) w[IM(k, wrap((jj) + (s), ny), i)]
                 int ip = wrap(i + 1, nx);
-                double vx0 = mass_to_full(g, ru, k, j, i), vx1 = mass_to_full(g, ru, k, j, ip);
-                double fq0 = vx0 * flux5(WX(-3), WX(-2), WX(-1), WX(0), WX(1), WX(2), vx0);
-                double fq1 = vx1 * flux5(WX(-2), WX(-1), WX(0), WX(1), WX(2), WX(3), vx1);
-                double msf = g->msftx[I2(j, i)];
-                double t = ((-msf) * rdx) * (fq1 - fq0);
+                real vx0 = mass_to_full(g, ru, k, j, i), vx1 = mass_to_full(g, ru, k, j, ip);
+                real fq0 = vx0 * flux5(WX(-3), WX(-2), WX(-1), WX(0), WX(1), WX(2), vx0);
+                real fq1 = vx1 * flux5(WX(-2), WX(-1), WX(0), WX(1), WX(2), WX(3), vx1);
+                real msf = g->msftx[I2(j, i)];
+                real t = ((-msf) * rdx) * (fq1 - fq0);
                 if (ny > 1) {
                     int jp = wrap(j + 1, ny);
-                    double vy0 = mass_to_full(g, rv, k, j, i), vy1 = mass_to_full(g, rv, k, jp, i);
-                    double g0 = vy0 * flux5(WY(j, -3), WY(j, -2), WY(j, -1), WY(j, 0), WY(j, 1), WY(j, 2), vy0);
-                    double g1 = vy1 * flux5(WY(jp, -3), WY(jp, -2), WY(jp, -1), WY(jp, 0), WY(jp, 1), WY(jp, 2), vy1);
+                    real vy0 = mass_to_full(g, rv, k, j, i), vy1 = mass_to_full(g, rv, k, jp, i);
+                    real g0 = vy0 * flux5(WY(j, -3), WY(j, -2), WY(j, -1), WY(j, 0), WY(j, 1), WY(j, 2), vy0);
+                    real g1 = vy1 * flux5(WY(jp, -3), WY(jp, -2), WY(jp, -1), WY(jp, 0), WY(jp, 1), WY(jp, 2), vy1);
                     t = t - (msf * rdy) * (g1 - g0);
                 }
                 wt[IM(k, j, i)] = t;
@@ -440,25 +468,25 @@ static void advect_w_flux(const orc_grid *g, const double *w, const double *ru, 
     }
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
-        double *vflux = malloc((size_t)(nz + 1) * sizeof(double));
+        real *vflux = (real *)malloc((size_t)(nz + 1) * sizeof(real));
         for (int i = 0; i < nx; ++i) {
 #define VF(k) (0.5 * (rom[IM((k), j, i)] + rom[IM((k) - 1, j, i)]))
 #define WW(k) w[IM((k), j, i)]
             for (int k = 0; k <= nz; ++k) vflux[k] = 0.0;
             for (int k = 3; k < nz; ++k) {
-                double vel = VF(k);
+                real vel = VF(k);
                 vflux[k] = vel * flux3(WW(k - 2), WW(k - 1), WW(k), WW(k + 1), -vel);
             }
             vflux[1] = (VF(1) * 0.5) * (WW(1) + WW(0));
             {
-                double vel = VF(2);
+                real vel = VF(2);
                 vflux[2] = vel * flux3(WW(0), WW(1), WW(2), WW(3), -vel);
                 vel = VF(nz - 1);
                 vflux[nz - 1] = vel * flux3(WW(nz - 3), WW(nz - 2), WW(nz - 1), WW(nz), -vel);
             }
             /* top_lid: vflux[nz] = 0, no pickup */
             for (int k = 1; k < nz; ++k) {
-                double vd = (-g->rdn[k]) * (vflux[k + 1] - vflux[k]);
+                real vd = (-g->rdn[k]) * (vflux[k + 1] - vflux[k]);
                 wt[IM(k, j, i)] = wt[IM(k, j, i)] + vd;
             }
             wt[IM(0, j, i)] = wt[IM(0, j, i)] + 0.0;
@@ -475,38 +503,38 @@ static void advect_w_flux(const orc_grid *g, const double *w, const double *ru, 
 /* core/rk_addtend_dry.py:127-229 (_absolute_diagnostics, hyps opt 1)   */
 /* core/rk_addtend_dry.py:232-361 (large_step_horizontal_pgf)          */
 /* ------------------------------------------------------------------ */
-static void large_step_pgf(const orc_grid *g, const orc_state *S, const double *alt,
-                           double *al, double *php, double *ut, double *vt, double *dpn_col) {
+static void large_step_pgf(const orc_grid *g, const rstate *S, const real *alt,
+                           real *al, real *php, real *ut, real *vt, real *dpn_col) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
-    double dn_top = g->dn[nz - 1];
-    double dn_safe = fabs(dn_top) > 1.0e-30 ? dn_top : 1.0;
-    double cfn = (0.5 * g->dnw[nz - 1] + g->dn[nz - 1]) / dn_safe;
-    double cfn1 = -0.5 * g->dnw[nz - 1] / dn_safe;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    real dn_top = g->dn[nz - 1];
+    real dn_safe = fabs(dn_top) > 1.0e-30 ? dn_top : 1.0;
+    real cfn = (0.5 * g->dnw[nz - 1] + g->dn[nz - 1]) / dn_safe;
+    real cfn1 = -0.5 * g->dnw[nz - 1] / dn_safe;
     (void)dpn_col;
 
     /* _absolute_diagnostics: al, php */
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) {
-            double mu_pert = S->mu_pert[I2(j, i)];
-            double mu_total = S->mu_total[I2(j, i)];
-            double mub = mu_total - mu_pert;
-            double muts = mu_total + mu_pert;
+            real mu_pert = S->mu_pert[I2(j, i)];
+            real mu_total = S->mu_total[I2(j, i)];
+            real mub = mu_total - mu_pert;
+            real muts = mu_total + mu_pert;
             for (int k = 0; k < nz; ++k) {
-                double phb_k = S->ph_total[IM(k, j, i)] - S->ph_pert[IM(k, j, i)];
-                double phb_k1 = S->ph_total[IM(k + 1, j, i)] - S->ph_pert[IM(k + 1, j, i)];
-                double mass_base = g->c1h[k] * mub + g->c2h[k];
-                double safe_base = safe_pos(mass_base, 1.0e-12);
-                double alb = ((-g->rdnw[k]) * (phb_k1 - phb_k)) / safe_base;
-                double mass_t = g->c1h[k] * muts + g->c2h[k];
-                double safe_t = safe_pos(mass_t, 1.0e-12);
-                double mu_term = g->c1h[k] * mu_pert;
+                real phb_k = S->ph_total[IM(k, j, i)] - S->ph_pert[IM(k, j, i)];
+                real phb_k1 = S->ph_total[IM(k + 1, j, i)] - S->ph_pert[IM(k + 1, j, i)];
+                real mass_base = g->c1h[k] * mub + g->c2h[k];
+                real safe_base = safe_pos(mass_base, 1.0e-12);
+                real alb = ((-g->rdnw[k]) * (phb_k1 - phb_k)) / safe_base;
+                real mass_t = g->c1h[k] * muts + g->c2h[k];
+                real safe_t = safe_pos(mass_t, 1.0e-12);
+                real mu_term = g->c1h[k] * mu_pert;
                 al[IM(k, j, i)] =
                     -(alb * mu_term + g->rdnw[k] * (S->ph_pert[IM(k + 1, j, i)] - S->ph_pert[IM(k, j, i)])) /
                     safe_t;
-                double pht_k = phb_k + S->ph_pert[IM(k, j, i)];
-                double pht_k1 = phb_k1 + S->ph_pert[IM(k + 1, j, i)];
+                real pht_k = phb_k + S->ph_pert[IM(k, j, i)];
+                real pht_k1 = phb_k1 + S->ph_pert[IM(k + 1, j, i)];
                 php[IM(k, j, i)] = 0.5 * (pht_k + pht_k1);
             }
         }
@@ -515,42 +543,42 @@ static void large_step_pgf(const orc_grid *g, const orc_state *S, const double *
     /* x PGF on u faces (edge-padded face pairs) */
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
-        double *dpn = malloc((size_t)(nz + 1) * sizeof(double));
+        real *dpn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
         for (int i = 0; i <= nx; ++i) {
             int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
-            double msf_u = g->msfux[I2U(j, i)] / g->msfuy[I2U(j, i)];
-            double mub_l = S->mu_total[I2(j, il)] - S->mu_pert[I2(j, il)];
-            double mub_r = S->mu_total[I2(j, ir)] - S->mu_pert[I2(j, ir)];
-            double mt_l = mub_l + S->mu_pert[I2(j, il)];
-            double mt_r = mub_r + S->mu_pert[I2(j, ir)];
-            double muu = 0.5 * (mt_l + mt_r);
-            double mu_l = S->mu_pert[I2(j, il)], mu_r = S->mu_pert[I2(j, ir)];
+            real msf_u = g->msfux[I2U(j, i)] / g->msfuy[I2U(j, i)];
+            real mub_l = S->mu_total[I2(j, il)] - S->mu_pert[I2(j, il)];
+            real mub_r = S->mu_total[I2(j, ir)] - S->mu_pert[I2(j, ir)];
+            real mt_l = mub_l + S->mu_pert[I2(j, il)];
+            real mt_r = mub_r + S->mu_pert[I2(j, ir)];
+            real muu = 0.5 * (mt_l + mt_r);
+            real mu_l = S->mu_pert[I2(j, il)], mu_r = S->mu_pert[I2(j, ir)];
             /* dpn faces from pair sums of p' */
             {
-                double ps0 = S->p_pert[IM(0, j, il)] + S->p_pert[IM(0, j, ir)];
-                double ps1 = S->p_pert[IM(1, j, il)] + S->p_pert[IM(1, j, ir)];
-                double ps2 = S->p_pert[IM(2, j, il)] + S->p_pert[IM(2, j, ir)];
+                real ps0 = S->p_pert[IM(0, j, il)] + S->p_pert[IM(0, j, ir)];
+                real ps1 = S->p_pert[IM(1, j, il)] + S->p_pert[IM(1, j, ir)];
+                real ps2 = S->p_pert[IM(2, j, il)] + S->p_pert[IM(2, j, ir)];
                 dpn[0] = 0.5 * ((g->cf1 * ps0 + g->cf2 * ps1) + g->cf3 * ps2);
                 for (int k = 1; k < nz; ++k) {
-                    double psk = S->p_pert[IM(k, j, il)] + S->p_pert[IM(k, j, ir)];
-                    double pskm = S->p_pert[IM(k - 1, j, il)] + S->p_pert[IM(k - 1, j, ir)];
+                    real psk = S->p_pert[IM(k, j, il)] + S->p_pert[IM(k, j, ir)];
+                    real pskm = S->p_pert[IM(k - 1, j, il)] + S->p_pert[IM(k - 1, j, ir)];
                     dpn[k] = 0.5 * (g->fnm[k] * psk + g->fnp[k] * pskm);
                 }
-                double pst = S->p_pert[IM(nz - 1, j, il)] + S->p_pert[IM(nz - 1, j, ir)];
-                double pstm = S->p_pert[IM(nz - 2, j, il)] + S->p_pert[IM(nz - 2, j, ir)];
+                real pst = S->p_pert[IM(nz - 1, j, il)] + S->p_pert[IM(nz - 1, j, ir)];
+                real pstm = S->p_pert[IM(nz - 2, j, il)] + S->p_pert[IM(nz - 2, j, ir)];
                 dpn[nz] = 0.5 * (cfn * pst + cfn1 * pstm);
             }
             for (int k = 0; k < nz; ++k) {
-                double mass_u = g->c1h[k] * muu + g->c2h[k];
-                double ph_term = (S->ph_pert[IM(k + 1, j, ir)] - S->ph_pert[IM(k + 1, j, il)]) +
+                real mass_u = g->c1h[k] * muu + g->c2h[k];
+                real ph_term = (S->ph_pert[IM(k + 1, j, ir)] - S->ph_pert[IM(k + 1, j, il)]) +
                                  (S->ph_pert[IM(k, j, ir)] - S->ph_pert[IM(k, j, il)]);
-                double p_term = (alt[IM(k, j, il)] + alt[IM(k, j, ir)]) *
+                real p_term = (alt[IM(k, j, il)] + alt[IM(k, j, ir)]) *
                                 (S->p_pert[IM(k, j, ir)] - S->p_pert[IM(k, j, il)]);
-                double pb_l = S->p_total[IM(k, j, il)] - S->p_pert[IM(k, j, il)];
-                double pb_r = S->p_total[IM(k, j, ir)] - S->p_pert[IM(k, j, ir)];
-                double pb_term = (al[IM(k, j, il)] + al[IM(k, j, ir)]) * (pb_r - pb_l);
-                double dpx = (((msf_u * 0.5) * rdx) * mass_u) * ((ph_term + p_term) + pb_term);
-                double bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (mu_l + mu_r));
+                real pb_l = S->p_total[IM(k, j, il)] - S->p_pert[IM(k, j, il)];
+                real pb_r = S->p_total[IM(k, j, ir)] - S->p_pert[IM(k, j, ir)];
+                real pb_term = (al[IM(k, j, il)] + al[IM(k, j, ir)]) * (pb_r - pb_l);
+                real dpx = (((msf_u * 0.5) * rdx) * mass_u) * ((ph_term + p_term) + pb_term);
+                real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (mu_l + mu_r));
                 dpx = dpx + ((msf_u * rdx) * (php[IM(k, j, ir)] - php[IM(k, j, il)])) * bracket;
                 ut[IU(k, j, i)] = ut[IU(k, j, i)] + (-1.0 * dpx);
             }
@@ -561,41 +589,41 @@ static void large_step_pgf(const orc_grid *g, const orc_state *S, const double *
     /* y PGF on v faces */
 #pragma omp parallel for schedule(static)
     for (int j = 0; j <= ny; ++j) {
-        double *dpn = malloc((size_t)(nz + 1) * sizeof(double));
+        real *dpn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
         int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
         for (int i = 0; i < nx; ++i) {
-            double msf_v = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
-            double mub_s = S->mu_total[I2(js, i)] - S->mu_pert[I2(js, i)];
-            double mub_n = S->mu_total[I2(jn, i)] - S->mu_pert[I2(jn, i)];
-            double mt_s = mub_s + S->mu_pert[I2(js, i)];
-            double mt_n = mub_n + S->mu_pert[I2(jn, i)];
-            double muv = 0.5 * (mt_s + mt_n);
-            double mu_s = S->mu_pert[I2(js, i)], mu_n = S->mu_pert[I2(jn, i)];
+            real msf_v = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
+            real mub_s = S->mu_total[I2(js, i)] - S->mu_pert[I2(js, i)];
+            real mub_n = S->mu_total[I2(jn, i)] - S->mu_pert[I2(jn, i)];
+            real mt_s = mub_s + S->mu_pert[I2(js, i)];
+            real mt_n = mub_n + S->mu_pert[I2(jn, i)];
+            real muv = 0.5 * (mt_s + mt_n);
+            real mu_s = S->mu_pert[I2(js, i)], mu_n = S->mu_pert[I2(jn, i)];
             {
-                double ps0 = S->p_pert[IM(0, js, i)] + S->p_pert[IM(0, jn, i)];
-                double ps1 = S->p_pert[IM(1, js, i)] + S->p_pert[IM(1, jn, i)];
-                double ps2 = S->p_pert[IM(2, js, i)] + S->p_pert[IM(2, jn, i)];
+                real ps0 = S->p_pert[IM(0, js, i)] + S->p_pert[IM(0, jn, i)];
+                real ps1 = S->p_pert[IM(1, js, i)] + S->p_pert[IM(1, jn, i)];
+                real ps2 = S->p_pert[IM(2, js, i)] + S->p_pert[IM(2, jn, i)];
                 dpn[0] = 0.5 * ((g->cf1 * ps0 + g->cf2 * ps1) + g->cf3 * ps2);
                 for (int k = 1; k < nz; ++k) {
-                    double psk = S->p_pert[IM(k, js, i)] + S->p_pert[IM(k, jn, i)];
-                    double pskm = S->p_pert[IM(k - 1, js, i)] + S->p_pert[IM(k - 1, jn, i)];
+                    real psk = S->p_pert[IM(k, js, i)] + S->p_pert[IM(k, jn, i)];
+                    real pskm = S->p_pert[IM(k - 1, js, i)] + S->p_pert[IM(k - 1, jn, i)];
                     dpn[k] = 0.5 * (g->fnm[k] * psk + g->fnp[k] * pskm);
                 }
-                double pst = S->p_pert[IM(nz - 1, js, i)] + S->p_pert[IM(nz - 1, jn, i)];
-                double pstm = S->p_pert[IM(nz - 2, js, i)] + S->p_pert[IM(nz - 2, jn, i)];
+                real pst = S->p_pert[IM(nz - 1, js, i)] + S->p_pert[IM(nz - 1, jn, i)];
+                real pstm = S->p_pert[IM(nz - 2, js, i)] + S->p_pert[IM(nz - 2, jn, i)];
                 dpn[nz] = 0.5 * (cfn * pst + cfn1 * pstm);
             }
             for (int k = 0; k < nz; ++k) {
-                double mass_v = g->c1h[k] * muv + g->c2h[k];
-                double ph_term = (S->ph_pert[IM(k + 1, jn, i)] - S->ph_pert[IM(k + 1, js, i)]) +
+                real mass_v = g->c1h[k] * muv + g->c2h[k];
+                real ph_term = (S->ph_pert[IM(k + 1, jn, i)] - S->ph_pert[IM(k + 1, js, i)]) +
                                  (S->ph_pert[IM(k, jn, i)] - S->ph_pert[IM(k, js, i)]);
-                double p_term = (alt[IM(k, js, i)] + alt[IM(k, jn, i)]) *
+                real p_term = (alt[IM(k, js, i)] + alt[IM(k, jn, i)]) *
                                 (S->p_pert[IM(k, jn, i)] - S->p_pert[IM(k, js, i)]);
-                double pb_s = S->p_total[IM(k, js, i)] - S->p_pert[IM(k, js, i)];
-                double pb_n = S->p_total[IM(k, jn, i)] - S->p_pert[IM(k, jn, i)];
-                double pb_term = (al[IM(k, js, i)] + al[IM(k, jn, i)]) * (pb_n - pb_s);
-                double dpy = (((msf_v * 0.5) * rdy) * mass_v) * ((ph_term + p_term) + pb_term);
-                double bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (mu_s + mu_n));
+                real pb_s = S->p_total[IM(k, js, i)] - S->p_pert[IM(k, js, i)];
+                real pb_n = S->p_total[IM(k, jn, i)] - S->p_pert[IM(k, jn, i)];
+                real pb_term = (al[IM(k, js, i)] + al[IM(k, jn, i)]) * (pb_n - pb_s);
+                real dpy = (((msf_v * 0.5) * rdy) * mass_v) * ((ph_term + p_term) + pb_term);
+                real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (mu_s + mu_n));
                 dpy = dpy + ((msf_v * rdy) * (php[IM(k, jn, i)] - php[IM(k, js, i)])) * bracket;
                 vt[IV(k, j, i)] = vt[IV(k, j, i)] + (-1.0 * dpy);
             }
@@ -609,15 +637,15 @@ static void large_step_pgf(const orc_grid *g, const orc_state *S, const double *
 /* cqw is the post-pg_buoy_w field: 0 at faces 0 and nz, 1 inside       */
 /* (core/advance_w.py:266-290 with qtot = 0).                           */
 /* ------------------------------------------------------------------ */
-static inline double cqw_at(int kf, int nz) { return (kf >= 1 && kf < nz) ? 1.0 : 0.0; }
+static inline real cqw_at(int kf, int nz) { return (kf >= 1 && kf < nz) ? 1.0 : 0.0; }
 
-static void calc_coef_w_column(const orc_grid *g, double mut, const double *c2a, size_t sc,
-                               double dts, double epssm, int top_lid, double *a, double *alpha,
-                               double *gamma, size_t sw) {
+static void calc_coef_w_column(const orc_grid *g, real mut, const real *c2a, size_t sc,
+                               double dts, double epssm, int top_lid, real *a, real *alpha,
+                               real *gamma, size_t sw) {
     const int nz = g->nz;
-    double cof = (0.5 * dts * G * (1.0 + epssm));
+    real cof = (0.5 * dts * G * (1.0 + epssm));
     cof = cof * cof;
-    double lid_flag = top_lid ? 0.0 : 1.0;
+    real lid_flag = top_lid ? 0.0 : 1.0;
 #define MH(k) (g->c1h[(k)] * mut + g->c2h[(k)])
 #define MF(k) (g->c1f[(k)] * mut + g->c2f[(k)])
 #define C2A(k) c2a[(size_t)(k) * sc]
@@ -626,30 +654,30 @@ static void calc_coef_w_column(const orc_grid *g, double mut, const double *c2a,
         alpha[(size_t)k * sw] = 1.0;
         gamma[(size_t)k * sw] = 0.0;
     }
-    double top_denom_a = MH(nz - 1) * MF(nz - 1);
-    double top_denom_b = MH(nz - 1) * MF(nz);
-    double r2 = g->rdnw[nz - 1] * g->rdnw[nz - 1];
+    real top_denom_a = MH(nz - 1) * MF(nz - 1);
+    real top_denom_b = MH(nz - 1) * MF(nz);
+    real r2 = g->rdnw[nz - 1] * g->rdnw[nz - 1];
     a[1 * sw] = 0.0;
     a[(size_t)nz * sw] = ((((-2.0 * cof) * r2) * C2A(nz - 1)) * lid_flag) / top_denom_a;
     gamma[0] = 0.0;
     for (int kk = 2; kk < nz; ++kk) {
         int k = kk - 1;
-        double denom = MH(k) * MF(k);
+        real denom = MH(k) * MF(k);
         a[(size_t)kk * sw] =
             (((((-cqw_at(kk, nz)) * cof) * g->rdn[kk]) * g->rdnw[kk - 1]) * C2A(kk - 1)) / denom;
     }
     for (int k = 1; k < nz; ++k) {
-        double du = MH(k) * MF(k);
-        double dl = MH(k - 1) * MF(k);
-        double dc = MH(k) * MF(k + 1);
-        double b = 1.0 + ((cqw_at(k, nz) * cof) * g->rdn[k]) *
+        real du = MH(k) * MF(k);
+        real dl = MH(k - 1) * MF(k);
+        real dc = MH(k) * MF(k + 1);
+        real b = 1.0 + ((cqw_at(k, nz) * cof) * g->rdn[k]) *
                              ((g->rdnw[k] * C2A(k)) / du + (g->rdnw[k - 1] * C2A(k - 1)) / dl);
-        double c = (((((-cqw_at(k, nz)) * cof) * g->rdn[k]) * g->rdnw[k]) * C2A(k)) / dc;
-        double al = 1.0 / (b - a[(size_t)k * sw] * gamma[(size_t)(k - 1) * sw]);
+        real c = (((((-cqw_at(k, nz)) * cof) * g->rdn[k]) * g->rdnw[k]) * C2A(k)) / dc;
+        real al = 1.0 / (b - a[(size_t)k * sw] * gamma[(size_t)(k - 1) * sw]);
         alpha[(size_t)k * sw] = al;
         gamma[(size_t)k * sw] = c * al;
     }
-    double b_top = 1.0 + (((2.0 * cof) * r2) * C2A(nz - 1)) / top_denom_b;
+    real b_top = 1.0 + (((2.0 * cof) * r2) * C2A(nz - 1)) / top_denom_b;
     alpha[(size_t)nz * sw] = 1.0 / (b_top - a[(size_t)nz * sw] * gamma[(size_t)(nz - 1) * sw]);
     gamma[(size_t)nz * sw] = 0.0;
 #undef MH
@@ -657,6 +685,7 @@ static void calc_coef_w_column(const orc_grid *g, double mut, const double *c2a,
 #undef C2A
 }
 
+#ifndef ORC_AD
 int orc_calc_coef_w(const orc_grid *g, const double *mut, const double *c2a, double dts,
                     double epssm, int top_lid, double *a, double *alpha, double *gamma) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
@@ -676,31 +705,32 @@ void orc_thomas(int nzp1, const double *a, const double *alpha, const double *ga
     for (int k = 1; k <= nz; ++k) out[k] = (rhs[k] - a[k] * out[k - 1]) * alpha[k];
     for (int k = nz - 1; k >= 1; --k) out[k] = out[k] - gamma[k] * out[k + 1];
 }
+#endif
 
 /* ------------------------------------------------------------------ */
 /* Workspace                                                           */
 /* ------------------------------------------------------------------ */
 typedef struct {
     /* origin (rk1 reference) copies */
-    double *u0, *v0, *w0, *th0, *php0_unused, *ph0, *mu0;
+    real *u0, *v0, *w0, *th0, *php0_unused, *ph0, *mu0;
     /* stage velocities */
-    double *ru, *rv, *rom, *divv;
+    real *ru, *rv, *rom, *divv;
     /* tendencies */
-    double *ut, *vt, *wt, *tht;
+    real *ut, *vt, *wt, *tht;
     /* stage diagnostics */
-    double *alt_raw, *al_ls, *php_ls;
+    real *alt_raw, *al_ls, *php_ls;
     /* prep */
-    double *mub, *mut, *muts0, *mu_work0, *muu, *muus, *muv, *muvs;
-    double *u_work, *v_work, *w_work, *th_work, *ph_work;
-    double *c2a, *alt, *pb, *phb, *php;
-    double *p, *al, *pm1;
-    double *rw_tend, *ph_tend, *ca, *calpha, *cgamma;
+    real *mub, *mut, *muts0, *mu_work0, *muu, *muus, *muv, *muvs;
+    real *u_work, *v_work, *w_work, *th_work, *ph_work;
+    real *c2a, *alt, *pb, *phb, *php;
+    real *p, *al, *pm1;
+    real *rw_tend, *ph_tend, *ca, *calpha, *cgamma;
     /* acoustic carried */
-    double *mu, *muts, *muave, *ww, *t2ave;
+    real *mu, *muts, *muave, *ww, *t2ave;
     /* substep scratch */
-    double *dvdxi, *mudf_new, *ww_new, *wsolve, *rhs;
+    real *dvdxi, *mudf_new, *ww_new, *wsolve, *rhs;
     /* finish */
-    double *tmp_ph_total;
+    real *tmp_ph_total;
 } ws_t;
 
 /* Terrain surface w of advance_w (core/advance_w.py:505-556): the kinematic
@@ -708,24 +738,25 @@ typedef struct {
  * levels 0..2) and the edge-clamped terrain height differences, the height
  * being the surface base geopotential phb0 / g.  phb0 is the level-0 plane
  * (ny*nx). */
-static double surface_w(const orc_grid *g, const double *phb0, const double *u, const double *v, int j, int i) {
+static real surface_w(const orc_grid *g, const real *phb0, const real *u, const real *v, int j, int i) {
     const int nx = g->nx, ny = g->ny;
-    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
-    const double msftx = g->msftx[I2(j, i)], msfty = g->msfty[I2(j, i)];
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const real msftx = g->msftx[I2(j, i)], msfty = g->msfty[I2(j, i)];
     int jn = clampi(j + 1, 0, ny - 1), js = clampi(j - 1, 0, ny - 1);
     int ie = clampi(i + 1, 0, nx - 1), iw = clampi(i - 1, 0, nx - 1);
-    double ht = phb0[I2(j, i)] / G;
-    double ht_n = phb0[I2(jn, i)] / G, ht_s = phb0[I2(js, i)] / G;
-    double ht_e = phb0[I2(j, ie)] / G, ht_w = phb0[I2(j, iw)] / G;
-    double dy_n = ht_n - ht, dy_s = ht - ht_s, dx_e = ht_e - ht, dx_w = ht - ht_w;
-    double v_cf_n = (g->cf1 * v[IV(0, j + 1, i)] + g->cf2 * v[IV(1, j + 1, i)]) + g->cf3 * v[IV(2, j + 1, i)];
-    double v_cf_s = (g->cf1 * v[IV(0, j, i)] + g->cf2 * v[IV(1, j, i)]) + g->cf3 * v[IV(2, j, i)];
-    double u_cf_e = (g->cf1 * u[IU(0, j, i + 1)] + g->cf2 * u[IU(1, j, i + 1)]) + g->cf3 * u[IU(2, j, i + 1)];
-    double u_cf_w = (g->cf1 * u[IU(0, j, i)] + g->cf2 * u[IU(1, j, i)]) + g->cf3 * u[IU(2, j, i)];
+    real ht = phb0[I2(j, i)] / G;
+    real ht_n = phb0[I2(jn, i)] / G, ht_s = phb0[I2(js, i)] / G;
+    real ht_e = phb0[I2(j, ie)] / G, ht_w = phb0[I2(j, iw)] / G;
+    real dy_n = ht_n - ht, dy_s = ht - ht_s, dx_e = ht_e - ht, dx_w = ht - ht_w;
+    real v_cf_n = (g->cf1 * v[IV(0, j + 1, i)] + g->cf2 * v[IV(1, j + 1, i)]) + g->cf3 * v[IV(2, j + 1, i)];
+    real v_cf_s = (g->cf1 * v[IV(0, j, i)] + g->cf2 * v[IV(1, j, i)]) + g->cf3 * v[IV(2, j, i)];
+    real u_cf_e = (g->cf1 * u[IU(0, j, i + 1)] + g->cf2 * u[IU(1, j, i + 1)]) + g->cf3 * u[IU(2, j, i + 1)];
+    real u_cf_w = (g->cf1 * u[IU(0, j, i)] + g->cf2 * u[IU(1, j, i)]) + g->cf3 * u[IU(2, j, i)];
     return ((msfty * 0.5) * rdy) * (dy_n * v_cf_n + dy_s * v_cf_s) +
            ((msftx * 0.5) * rdx) * (dx_e * u_cf_e + dx_w * u_cf_w);
 }
 
+#ifndef ORC_AD
 int orc_surface_w(const orc_grid *g, const double *phb0, const double *u, const double *v, double *out) {
     if (!g || !phb0 || !u || !v || !out || g->nz < 3) return -1;
     const int nx = g->nx;
@@ -733,15 +764,16 @@ int orc_surface_w(const orc_grid *g, const double *phb0, const double *u, const 
         for (int i = 0; i < g->nx; ++i) out[I2(j, i)] = surface_w(g, phb0, u, v, j, i);
     return 0;
 }
+#endif
 
-static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+static real *zalloc(size_t n) { return (real *)calloc(n ? n : 1, sizeof(real)); }
 
 static int ws_alloc(ws_t *w, int nx, int ny, int nz) {
     size_t M = (size_t)nz * ny * nx, W = (size_t)(nz + 1) * ny * nx;
     size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx;
     size_t C = (size_t)ny * nx, CU = (size_t)ny * (nx + 1), CV = (size_t)(ny + 1) * nx;
     memset(w, 0, sizeof(*w));
-    double **all[] = {&w->u0, &w->v0, &w->w0, &w->th0, &w->ph0, &w->mu0, &w->ru, &w->rv, &w->rom,
+    real **all[] = {&w->u0, &w->v0, &w->w0, &w->th0, &w->ph0, &w->mu0, &w->ru, &w->rv, &w->rom,
                       &w->divv, &w->ut, &w->vt, &w->wt, &w->tht, &w->alt_raw, &w->al_ls,
                       &w->php_ls, &w->mub, &w->mut, &w->muts0, &w->mu_work0, &w->muu, &w->muus,
                       &w->muv, &w->muvs, &w->u_work, &w->v_work, &w->w_work, &w->th_work,
@@ -761,7 +793,7 @@ static int ws_alloc(ws_t *w, int nx, int ny, int nz) {
 }
 
 static void ws_free(ws_t *w) {
-    double **all[] = {&w->u0, &w->v0, &w->w0, &w->th0, &w->ph0, &w->mu0, &w->ru, &w->rv, &w->rom,
+    real **all[] = {&w->u0, &w->v0, &w->w0, &w->th0, &w->ph0, &w->mu0, &w->ru, &w->rv, &w->rom,
                       &w->divv, &w->ut, &w->vt, &w->wt, &w->tht, &w->alt_raw, &w->al_ls,
                       &w->php_ls, &w->mub, &w->mut, &w->muts0, &w->mu_work0, &w->muu, &w->muus,
                       &w->muv, &w->muvs, &w->u_work, &w->v_work, &w->w_work, &w->th_work,
@@ -774,14 +806,14 @@ static void ws_free(ws_t *w) {
 
 /* _u_face_average_2d / _v_face_average_2d, core/small_step_prep.py:77-88:
  * edge faces copy the edge cell, interior faces average. */
-static void u_face_avg(int nx, int ny, const double *f, double *out) {
+static void u_face_avg(int nx, int ny, const real *f, real *out) {
     for (int j = 0; j < ny; ++j) {
         out[I2U(j, 0)] = f[I2(j, 0)];
         for (int i = 1; i < nx; ++i) out[I2U(j, i)] = 0.5 * (f[I2(j, i - 1)] + f[I2(j, i)]);
         out[I2U(j, nx)] = f[I2(j, nx - 1)];
     }
 }
-static void v_face_avg(int nx, int ny, const double *f, double *out) {
+static void v_face_avg(int nx, int ny, const real *f, real *out) {
     for (int i = 0; i < nx; ++i) {
         out[I2V(0, i)] = f[I2(0, i)];
         for (int j = 1; j < ny; ++j) out[I2V(j, i)] = 0.5 * (f[I2(j - 1, i)] + f[I2(j, i)]);
@@ -792,10 +824,10 @@ static void v_face_avg(int nx, int ny, const double *f, double *out) {
 /* ------------------------------------------------------------------ */
 /* One RK stage: operational_mode.py:3331-3515 advance_stage            */
 /* ------------------------------------------------------------------ */
-static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws_t *W, int rk_step,
+static void rk_stage(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W, int rk_step,
                      double dts, int nsub) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
     const size_t Mn = (size_t)nz * ny * nx;
 
     /* --- transport velocities (operational_mode.py:2429-2475) --- */
@@ -807,7 +839,7 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
     advect_w_flux(g, S->w, W->ru, W->rv, W->rom, W->wt);
     /* theta - 300 advected scalar */
     {
-        double *thp = W->th_work; /* temporary use before prep overwrites it */
+        real *thp = W->th_work; /* temporary use before prep overwrites it */
         for (size_t q = 0; q < Mn; ++q) thp[q] = S->theta[q] - T0;
         advect_scalar_flux(g, thp, W->ru, W->rv, W->rom, W->tht);
     }
@@ -815,18 +847,18 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
      * (rk_addtend_dry.py:184-186) and diagnose_pressure_al_alt's base_state=None
      * branch used by small_step_prep (acoustic_wrf.py:354-358). */
 #pragma omp parallel for schedule(static)
-    for (size_t q = 0; q < Mn; ++q) W->alt_raw[q] = orc_alt_from_theta_p(S->theta[q], S->p_total[q]);
+    for (size_t q = 0; q < Mn; ++q) W->alt_raw[q] = alt_from_theta_p(S->theta[q], S->p_total[q]);
     large_step_pgf(g, S, W->alt_raw, W->al_ls, W->php_ls, W->ut, W->vt, NULL);
 
     /* --- small_step_prep_wrf, core/small_step_prep.py:216-394 --- */
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             size_t c = I2(j, i);
-            double mub = S->mu_total[c] - S->mu_pert[c];
-            double mu_cur = S->mu_pert[c];
-            double mu_ref = W->mu0[c];
-            double mu_work = rk_step == 1 ? 0.0 : mu_ref - mu_cur;
-            double mut = mub + mu_cur;
+            real mub = S->mu_total[c] - S->mu_pert[c];
+            real mu_cur = S->mu_pert[c];
+            real mu_ref = W->mu0[c];
+            real mu_work = rk_step == 1 ? 0.0 : mu_ref - mu_cur;
+            real mut = mub + mu_cur;
             W->mub[c] = mub;
             W->mut[c] = mut;
             W->mu_work0[c] = mu_work;
@@ -841,16 +873,16 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
     for (int k = 0; k < nz; ++k) {
         for (int j = 0; j < ny; ++j) {
             for (int i = 0; i <= nx; ++i) {
-                double mref = g->c1h[k] * W->muus[I2U(j, i)] + g->c2h[k];
-                double mcur = g->c1h[k] * W->muu[I2U(j, i)] + g->c2h[k];
+                real mref = g->c1h[k] * W->muus[I2U(j, i)] + g->c2h[k];
+                real mcur = g->c1h[k] * W->muu[I2U(j, i)] + g->c2h[k];
                 W->u_work[IU(k, j, i)] =
                     (mref * W->u0[IU(k, j, i)] - mcur * S->u[IU(k, j, i)]) / g->msfuy[I2U(j, i)];
             }
         }
         for (int j = 0; j <= ny; ++j) {
             for (int i = 0; i < nx; ++i) {
-                double mref = g->c1h[k] * W->muvs[I2V(j, i)] + g->c2h[k];
-                double mcur = g->c1h[k] * W->muv[I2V(j, i)] + g->c2h[k];
+                real mref = g->c1h[k] * W->muvs[I2V(j, i)] + g->c2h[k];
+                real mcur = g->c1h[k] * W->muv[I2V(j, i)] + g->c2h[k];
                 W->v_work[IV(k, j, i)] =
                     (mref * W->v0[IV(k, j, i)] - mcur * S->v[IV(k, j, i)]) / g->msfvx[I2V(j, i)];
             }
@@ -858,13 +890,13 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         for (int j = 0; j < ny; ++j) {
             for (int i = 0; i < nx; ++i) {
                 size_t c = I2(j, i), q = IM(k, j, i);
-                double mref = g->c1h[k] * W->muts0[c] + g->c2h[k];
-                double mcur = g->c1h[k] * W->mut[c] + g->c2h[k];
-                double th_ref = W->th0[q] - T0, th_cur = S->theta[q] - T0;
+                real mref = g->c1h[k] * W->muts0[c] + g->c2h[k];
+                real mcur = g->c1h[k] * W->mut[c] + g->c2h[k];
+                real th_ref = W->th0[q] - T0, th_cur = S->theta[q] - T0;
                 W->th_work[q] = mref * th_ref - mcur * th_cur;
-                double pb = S->p_total[q] - S->p_pert[q];
+                real pb = S->p_total[q] - S->p_pert[q];
                 W->pb[q] = pb;
-                double alt_ac = W->alt_raw[q] > MIN_ACOUSTIC_ALT ? W->alt_raw[q] : MIN_ACOUSTIC_ALT;
+                real alt_ac = W->alt_raw[q] > MIN_ACOUSTIC_ALT ? W->alt_raw[q] : MIN_ACOUSTIC_ALT;
                 W->alt[q] = alt_ac;
                 W->c2a[q] = (CPOVCV * (pb + S->p_pert[q])) / alt_ac;
             }
@@ -875,8 +907,8 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         for (int j = 0; j < ny; ++j) {
             for (int i = 0; i < nx; ++i) {
                 size_t c = I2(j, i), q = IM(k, j, i);
-                double mref = g->c1f[k] * W->muts0[c] + g->c2f[k];
-                double mcur = g->c1f[k] * W->mut[c] + g->c2f[k];
+                real mref = g->c1f[k] * W->muts0[c] + g->c2f[k];
+                real mcur = g->c1f[k] * W->mut[c] + g->c2f[k];
                 W->w_work[q] = (mref * W->w0[q] - mcur * S->w[q]) / g->msfty[c];
                 W->ph_work[q] = W->ph0[q] - S->ph_pert[q];
                 W->phb[q] = S->ph_total[q] - S->ph_pert[q];
@@ -897,15 +929,15 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 size_t c = I2(j, i), q = IM(k, j, i);
-                double mass_h = g->c1h[k] * W->muts0[c] + g->c2h[k];
-                double safe_mass = safe_pos(mass_h, 1.0e-12);
-                double mu_term = g->c1h[k] * W->mu_work0[c];
-                double al = -(W->alt[q] * mu_term + g->rdnw[k] * (W->ph_work[IM(k + 1, j, i)] - W->ph_work[q])) /
+                real mass_h = g->c1h[k] * W->muts0[c] + g->c2h[k];
+                real safe_mass = safe_pos(mass_h, 1.0e-12);
+                real mu_term = g->c1h[k] * W->mu_work0[c];
+                real al = -(W->alt[q] * mu_term + g->rdnw[k] * (W->ph_work[IM(k + 1, j, i)] - W->ph_work[q])) /
                             safe_mass;
-                double theta_1 = W->th0[q] - T0;
-                double ttr = T0 + theta_1;
-                double safe_t = safe_pos(ttr, 1.0e-6);
-                double p = W->c2a[q] * (((W->alt[q] * (W->th_work[q] - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
+                real theta_1 = W->th0[q] - T0;
+                real ttr = T0 + theta_1;
+                real safe_t = safe_pos(ttr, 1.0e-6);
+                real p = W->c2a[q] * (((W->alt[q] * (W->th_work[q] - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
                 W->al[q] = al;
                 W->p[q] = p;
                 W->pm1[q] = p;
@@ -917,21 +949,21 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             size_t c = I2(j, i);
-            double msft_inv = 1.0 / g->msfty[c];
-            double mu_p = W->mut[c] - W->mub[c];
-            double mub = W->mub[c];
+            real msft_inv = 1.0 / g->msfty[c];
+            real mu_p = W->mut[c] - W->mub[c];
+            real mub = W->mub[c];
             W->rw_tend[IM(0, j, i)] = 0.0;
             for (int k = 1; k < nz; ++k) {
-                double cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
-                double rw = (msft_inv * G) *
+                real cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
+                real rw = (msft_inv * G) *
                             (((cq1 * g->rdn[k]) * (S->p_pert[IM(k, j, i)] - S->p_pert[IM(k - 1, j, i)]) -
                               g->c1f[k] * mu_p) -
                              cq2 * (g->c1f[k] * mub + g->c2f[k]));
                 W->rw_tend[IM(k, j, i)] = rw;
             }
             {
-                double cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
-                double rw = (msft_inv * G) *
+                real cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
+                real rw = (msft_inv * G) *
                             (((((cq1 * 2.0) * g->rdnw[nz - 1]) * (-S->p_pert[IM(nz - 1, j, i)])) -
                               g->c1f[nz] * mu_p) -
                              cq2 * (g->c1f[nz] * mub + g->c2f[nz]));
@@ -944,29 +976,29 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         for (int i = 0; i < nx; ++i) {
             size_t c = I2(j, i);
             for (int k = 0; k <= nz; ++k) W->rw_tend[IM(k, j, i)] = W->rw_tend[IM(k, j, i)] + W->wt[IM(k, j, i)];
-            double mxy = g->msftx[c] / g->msfty[c];
-            double ru_m[2], u_m[2], rv_m[2], v_m[2];
+            real mxy = g->msftx[c] / g->msfty[c];
+            real ru_m[2], u_m[2], rv_m[2], v_m[2];
             for (int k = 1; k < nz; ++k) {
                 for (int s = 0; s < 2; ++s) {
                     int kk = k - s; /* s=0: level k, s=1: level k-1 */
-                    double mu_l = g->c1h[kk] * W->muu[I2U(j, i)] + g->c2h[kk];
-                    double mu_r = g->c1h[kk] * W->muu[I2U(j, i + 1)] + g->c2h[kk];
-                    double ru_l = (mu_l * S->u[IU(kk, j, i)]) / g->msfuy[I2U(j, i)];
-                    double ru_r = (mu_r * S->u[IU(kk, j, i + 1)]) / g->msfuy[I2U(j, i + 1)];
+                    real mu_l = g->c1h[kk] * W->muu[I2U(j, i)] + g->c2h[kk];
+                    real mu_r = g->c1h[kk] * W->muu[I2U(j, i + 1)] + g->c2h[kk];
+                    real ru_l = (mu_l * S->u[IU(kk, j, i)]) / g->msfuy[I2U(j, i)];
+                    real ru_r = (mu_r * S->u[IU(kk, j, i + 1)]) / g->msfuy[I2U(j, i + 1)];
                     ru_m[s] = 0.5 * (ru_l + ru_r);
                     u_m[s] = 0.5 * (S->u[IU(kk, j, i)] + S->u[IU(kk, j, i + 1)]);
-                    double mv_s = g->c1h[kk] * W->muv[I2V(j, i)] + g->c2h[kk];
-                    double mv_n = g->c1h[kk] * W->muv[I2V(j + 1, i)] + g->c2h[kk];
-                    double rv_s = (mv_s * S->v[IV(kk, j, i)]) / g->msfvx[I2V(j, i)];
-                    double rv_n = (mv_n * S->v[IV(kk, j + 1, i)]) / g->msfvx[I2V(j + 1, i)];
+                    real mv_s = g->c1h[kk] * W->muv[I2V(j, i)] + g->c2h[kk];
+                    real mv_n = g->c1h[kk] * W->muv[I2V(j + 1, i)] + g->c2h[kk];
+                    real rv_s = (mv_s * S->v[IV(kk, j, i)]) / g->msfvx[I2V(j, i)];
+                    real rv_n = (mv_n * S->v[IV(kk, j + 1, i)]) / g->msfvx[I2V(j + 1, i)];
                     rv_m[s] = 0.5 * (rv_s + rv_n);
                     v_m[s] = 0.5 * (S->v[IV(kk, j, i)] + S->v[IV(kk, j + 1, i)]);
                 }
-                double ru_f = g->fnm[k] * ru_m[0] + g->fnp[k] * ru_m[1];
-                double u_f = g->fnm[k] * u_m[0] + g->fnp[k] * u_m[1];
-                double rv_f = g->fnm[k] * rv_m[0] + g->fnp[k] * rv_m[1];
-                double v_f = g->fnm[k] * v_m[0] + g->fnp[k] * v_m[1];
-                double curv = RERADIUS * (ru_f * u_f + (mxy * rv_f) * v_f);
+                real ru_f = g->fnm[k] * ru_m[0] + g->fnp[k] * ru_m[1];
+                real u_f = g->fnm[k] * u_m[0] + g->fnp[k] * u_m[1];
+                real rv_f = g->fnm[k] * rv_m[0] + g->fnp[k] * rv_m[1];
+                real v_f = g->fnm[k] * v_m[0] + g->fnp[k] * v_m[1];
+                real curv = RERADIUS * (ru_f * u_f + (mxy * rv_f) * v_f);
                 W->rw_tend[IM(k, j, i)] = W->rw_tend[IM(k, j, i)] + (0.0 + curv);
             }
         }
@@ -975,35 +1007,35 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             size_t c = I2(j, i);
-            double msft_inv = 1.0 / g->msfty[c];
+            real msft_inv = 1.0 / g->msfty[c];
             int jp = wrap(j + 1, ny), jm = wrap(j - 1, ny), ip = wrap(i + 1, nx), im = wrap(i - 1, nx);
 #define PHT(k, jj, ii) (S->ph_pert[IM((k), (jj), (ii))] + W->phb[IM((k), (jj), (ii))])
             W->ph_tend[IM(0, j, i)] = 0.0;
             for (int f = 1; f < nz; ++f) {
-                double wd_f = ((0.5 * (W->rom[IM(f + 1, j, i)] + W->rom[IM(f, j, i)])) * g->rdnw[f]) *
+                real wd_f = ((0.5 * (W->rom[IM(f + 1, j, i)] + W->rom[IM(f, j, i)])) * g->rdnw[f]) *
                               (PHT(f + 1, j, i) - PHT(f, j, i));
-                double wd_fm = ((0.5 * (W->rom[IM(f, j, i)] + W->rom[IM(f - 1, j, i)])) * g->rdnw[f - 1]) *
+                real wd_fm = ((0.5 * (W->rom[IM(f, j, i)] + W->rom[IM(f - 1, j, i)])) * g->rdnw[f - 1]) *
                                (PHT(f, j, i) - PHT(f - 1, j, i));
-                double term3 = g->fnm[f] * wd_f + g->fnp[f] * wd_fm;
-                double t = -term3;
-                double mass_f = g->c1f[f] * W->mut[c] + g->c2f[f];
-                double gw = ((mass_f * G) * S->w[IM(f, j, i)]) * msft_inv;
+                real term3 = g->fnm[f] * wd_f + g->fnp[f] * wd_fm;
+                real t = -term3;
+                real mass_f = g->c1f[f] * W->mut[c] + g->c2f[f];
+                real gw = ((mass_f * G) * S->w[IM(f, j, i)]) * msft_inv;
                 t = t + gw;
                 /* y advection */
-                double pti = PHT(f, j, i);
-                double dn_ = PHT(f, jp, i) - pti, ds_ = pti - PHT(f, jm, i);
-                double vn = S->v[IV(f, j + 1, i)] + S->v[IV(f - 1, j + 1, i)];
-                double vs = S->v[IV(f, j, i)] + S->v[IV(f - 1, j, i)];
-                double mvn = g->c1f[f] * W->muv[I2V(j + 1, i)] + g->c2f[f];
-                double mvs = g->c1f[f] * W->muv[I2V(j, i)] + g->c2f[f];
-                double fy = (mvn * vn) * dn_ + (mvs * vs) * ds_;
+                real pti = PHT(f, j, i);
+                real dn_ = PHT(f, jp, i) - pti, ds_ = pti - PHT(f, jm, i);
+                real vn = S->v[IV(f, j + 1, i)] + S->v[IV(f - 1, j + 1, i)];
+                real vs = S->v[IV(f, j, i)] + S->v[IV(f - 1, j, i)];
+                real mvn = g->c1f[f] * W->muv[I2V(j + 1, i)] + g->c2f[f];
+                real mvs = g->c1f[f] * W->muv[I2V(j, i)] + g->c2f[f];
+                real fy = (mvn * vn) * dn_ + (mvs * vs) * ds_;
                 t = t + (-((0.25 * rdy) * msft_inv)) * fy;
-                double de_ = PHT(f, j, ip) - pti, dw_ = pti - PHT(f, j, im);
-                double ue = S->u[IU(f, j, i + 1)] + S->u[IU(f - 1, j, i + 1)];
-                double uw = S->u[IU(f, j, i)] + S->u[IU(f - 1, j, i)];
-                double mue = g->c1f[f] * W->muu[I2U(j, i + 1)] + g->c2f[f];
-                double muw = g->c1f[f] * W->muu[I2U(j, i)] + g->c2f[f];
-                double fx = (mue * ue) * de_ + (muw * uw) * dw_;
+                real de_ = PHT(f, j, ip) - pti, dw_ = pti - PHT(f, j, im);
+                real ue = S->u[IU(f, j, i + 1)] + S->u[IU(f - 1, j, i + 1)];
+                real uw = S->u[IU(f, j, i)] + S->u[IU(f - 1, j, i)];
+                real mue = g->c1f[f] * W->muu[I2U(j, i + 1)] + g->c2f[f];
+                real muw = g->c1f[f] * W->muu[I2U(j, i)] + g->c2f[f];
+                real fx = (mue * ue) * de_ + (muw * uw) * dw_;
                 t = t + (-((0.25 * rdx) * msft_inv)) * fx;
                 W->ph_tend[IM(f, j, i)] = t;
             }
@@ -1019,52 +1051,52 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
                                (size_t)ny * nx);
 
     /* acoustic carried state initialisation */
-    memcpy(W->muts, W->muts0, sizeof(double) * (size_t)ny * nx);
-    memset(W->muave, 0, sizeof(double) * (size_t)ny * nx);
-    memcpy(W->ww, W->rom, sizeof(double) * (size_t)(nz + 1) * ny * nx);
-    memset(W->t2ave, 0, sizeof(double) * Mn);
+    memcpy(W->muts, W->muts0, sizeof(real) * (size_t)ny * nx);
+    memset(W->muave, 0, sizeof(real) * (size_t)ny * nx);
+    memcpy(W->ww, W->rom, sizeof(real) * (size_t)(nz + 1) * ny * nx);
+    memset(W->t2ave, 0, sizeof(real) * Mn);
 
-    const double eps_p = 1.0 + nl->epssm, eps_m = 1.0 - nl->epssm;
-    double *mudf = S->mudf;
+    const real eps_p = 1.0 + nl->epssm, eps_m = 1.0 - nl->epssm;
+    real *mudf = S->mudf;
 
     for (int it = 0; it < nsub; ++it) {
         /* ---- advance_uv_wrf, core/acoustic.py:626-786 (emdiv=0.01) ---- */
 #pragma omp parallel for schedule(static)
         for (int j = 0; j < ny; ++j) {
-            double *dpn = malloc((size_t)(nz + 1) * sizeof(double));
+            real *dpn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
             for (int i = 0; i <= nx; ++i) {
                 int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
-                double msf = g->msfux[I2U(j, i)] / g->msfuy[I2U(j, i)];
-                double muw_l = W->muts[I2(j, il)] - W->mut[I2(j, il)];
-                double muw_r = W->muts[I2(j, ir)] - W->mut[I2(j, ir)];
+                real msf = g->msfux[I2U(j, i)] / g->msfuy[I2U(j, i)];
+                real muw_l = W->muts[I2(j, il)] - W->mut[I2(j, il)];
+                real muw_r = W->muts[I2(j, ir)] - W->mut[I2(j, ir)];
                 {
-                    double ps0 = W->p[IM(0, j, il)] + W->p[IM(0, j, ir)];
-                    double ps1 = W->p[IM(1, j, il)] + W->p[IM(1, j, ir)];
-                    double ps2 = W->p[IM(2, j, il)] + W->p[IM(2, j, ir)];
+                    real ps0 = W->p[IM(0, j, il)] + W->p[IM(0, j, ir)];
+                    real ps1 = W->p[IM(1, j, il)] + W->p[IM(1, j, ir)];
+                    real ps2 = W->p[IM(2, j, il)] + W->p[IM(2, j, ir)];
                     dpn[0] = 0.5 * ((g->cf1 * ps0 + g->cf2 * ps1) + g->cf3 * ps2);
                     for (int k = 1; k < nz; ++k) {
-                        double psk = W->p[IM(k, j, il)] + W->p[IM(k, j, ir)];
-                        double pskm = W->p[IM(k - 1, j, il)] + W->p[IM(k - 1, j, ir)];
+                        real psk = W->p[IM(k, j, il)] + W->p[IM(k, j, ir)];
+                        real pskm = W->p[IM(k - 1, j, il)] + W->p[IM(k - 1, j, ir)];
                         dpn[k] = 0.5 * (g->fnm[k] * psk + g->fnp[k] * pskm);
                     }
-                    double pt = W->p[IM(nz - 1, j, il)] + W->p[IM(nz - 1, j, ir)];
-                    double pt2 = W->p[IM(nz - 2, j, il)] + W->p[IM(nz - 2, j, ir)];
-                    double pt3 = W->p[IM(nz - 3, j, il)] + W->p[IM(nz - 3, j, ir)];
+                    real pt = W->p[IM(nz - 1, j, il)] + W->p[IM(nz - 1, j, ir)];
+                    real pt2 = W->p[IM(nz - 2, j, il)] + W->p[IM(nz - 2, j, ir)];
+                    real pt3 = W->p[IM(nz - 3, j, il)] + W->p[IM(nz - 3, j, ir)];
                     dpn[nz] = 0.5 * ((g->cf1 * pt + g->cf2 * pt2) + g->cf3 * pt3);
                 }
-                double mudf_xy = ((-EMDIV * g->dx) * (mudf[I2(j, ir)] - mudf[I2(j, il)])) / g->msfuy[I2U(j, i)];
+                real mudf_xy = ((-EMDIV * g->dx) * (mudf[I2(j, ir)] - mudf[I2(j, il)])) / g->msfuy[I2U(j, i)];
                 for (int k = 0; k < nz; ++k) {
                     size_t q = IU(k, j, i);
-                    double u = W->u_work[q] + dts * W->ut[q];
-                    double mass_x = g->c1h[k] * W->muu[I2U(j, i)] + g->c2h[k];
-                    double ph_term = (W->ph_work[IM(k + 1, j, ir)] - W->ph_work[IM(k + 1, j, il)]) +
+                    real u = W->u_work[q] + dts * W->ut[q];
+                    real mass_x = g->c1h[k] * W->muu[I2U(j, i)] + g->c2h[k];
+                    real ph_term = (W->ph_work[IM(k + 1, j, ir)] - W->ph_work[IM(k + 1, j, il)]) +
                                      (W->ph_work[IM(k, j, ir)] - W->ph_work[IM(k, j, il)]);
-                    double p_term = (W->alt[IM(k, j, il)] + W->alt[IM(k, j, ir)]) *
+                    real p_term = (W->alt[IM(k, j, il)] + W->alt[IM(k, j, ir)]) *
                                     (W->p[IM(k, j, ir)] - W->p[IM(k, j, il)]);
-                    double pb_term = (W->al[IM(k, j, il)] + W->al[IM(k, j, ir)]) *
+                    real pb_term = (W->al[IM(k, j, il)] + W->al[IM(k, j, ir)]) *
                                      (W->pb[IM(k, j, ir)] - W->pb[IM(k, j, il)]);
-                    double dpx = (((msf * 0.5) * rdx) * mass_x) * ((ph_term + p_term) + pb_term);
-                    double bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_l + muw_r));
+                    real dpx = (((msf * 0.5) * rdx) * mass_x) * ((ph_term + p_term) + pb_term);
+                    real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_l + muw_r));
                     dpx = dpx + ((msf * rdx) * (W->php[IM(k, j, ir)] - W->php[IM(k, j, il)])) * bracket;
                     u = u - (dts * 1.0) * dpx;
                     u = u + g->c1h[k] * mudf_xy;
@@ -1075,41 +1107,41 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         }
 #pragma omp parallel for schedule(static)
         for (int j = 0; j <= ny; ++j) {
-            double *dpn = malloc((size_t)(nz + 1) * sizeof(double));
+            real *dpn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
             int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
             for (int i = 0; i < nx; ++i) {
-                double msf = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
-                double muw_s = W->muts[I2(js, i)] - W->mut[I2(js, i)];
-                double muw_n = W->muts[I2(jn, i)] - W->mut[I2(jn, i)];
+                real msf = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
+                real muw_s = W->muts[I2(js, i)] - W->mut[I2(js, i)];
+                real muw_n = W->muts[I2(jn, i)] - W->mut[I2(jn, i)];
                 {
-                    double ps0 = W->p[IM(0, js, i)] + W->p[IM(0, jn, i)];
-                    double ps1 = W->p[IM(1, js, i)] + W->p[IM(1, jn, i)];
-                    double ps2 = W->p[IM(2, js, i)] + W->p[IM(2, jn, i)];
+                    real ps0 = W->p[IM(0, js, i)] + W->p[IM(0, jn, i)];
+                    real ps1 = W->p[IM(1, js, i)] + W->p[IM(1, jn, i)];
+                    real ps2 = W->p[IM(2, js, i)] + W->p[IM(2, jn, i)];
                     dpn[0] = 0.5 * ((g->cf1 * ps0 + g->cf2 * ps1) + g->cf3 * ps2);
                     for (int k = 1; k < nz; ++k) {
-                        double psk = W->p[IM(k, js, i)] + W->p[IM(k, jn, i)];
-                        double pskm = W->p[IM(k - 1, js, i)] + W->p[IM(k - 1, jn, i)];
+                        real psk = W->p[IM(k, js, i)] + W->p[IM(k, jn, i)];
+                        real pskm = W->p[IM(k - 1, js, i)] + W->p[IM(k - 1, jn, i)];
                         dpn[k] = 0.5 * (g->fnm[k] * psk + g->fnp[k] * pskm);
                     }
-                    double pt = W->p[IM(nz - 1, js, i)] + W->p[IM(nz - 1, jn, i)];
-                    double pt2 = W->p[IM(nz - 2, js, i)] + W->p[IM(nz - 2, jn, i)];
-                    double pt3 = W->p[IM(nz - 3, js, i)] + W->p[IM(nz - 3, jn, i)];
+                    real pt = W->p[IM(nz - 1, js, i)] + W->p[IM(nz - 1, jn, i)];
+                    real pt2 = W->p[IM(nz - 2, js, i)] + W->p[IM(nz - 2, jn, i)];
+                    real pt3 = W->p[IM(nz - 3, js, i)] + W->p[IM(nz - 3, jn, i)];
                     dpn[nz] = 0.5 * ((g->cf1 * pt + g->cf2 * pt2) + g->cf3 * pt3);
                 }
-                double msfvx_inv = 1.0 / g->msfvx[I2V(j, i)];
-                double mudf_xy = ((-EMDIV * g->dy) * (mudf[I2(jn, i)] - mudf[I2(js, i)])) * msfvx_inv;
+                real msfvx_inv = 1.0 / g->msfvx[I2V(j, i)];
+                real mudf_xy = ((-EMDIV * g->dy) * (mudf[I2(jn, i)] - mudf[I2(js, i)])) * msfvx_inv;
                 for (int k = 0; k < nz; ++k) {
                     size_t q = IV(k, j, i);
-                    double v = W->v_work[q] + dts * W->vt[q];
-                    double mass_y = g->c1h[k] * W->muv[I2V(j, i)] + g->c2h[k];
-                    double ph_term = (W->ph_work[IM(k + 1, jn, i)] - W->ph_work[IM(k + 1, js, i)]) +
+                    real v = W->v_work[q] + dts * W->vt[q];
+                    real mass_y = g->c1h[k] * W->muv[I2V(j, i)] + g->c2h[k];
+                    real ph_term = (W->ph_work[IM(k + 1, jn, i)] - W->ph_work[IM(k + 1, js, i)]) +
                                      (W->ph_work[IM(k, jn, i)] - W->ph_work[IM(k, js, i)]);
-                    double p_term = (W->alt[IM(k, js, i)] + W->alt[IM(k, jn, i)]) *
+                    real p_term = (W->alt[IM(k, js, i)] + W->alt[IM(k, jn, i)]) *
                                     (W->p[IM(k, jn, i)] - W->p[IM(k, js, i)]);
-                    double pb_term = (W->al[IM(k, js, i)] + W->al[IM(k, jn, i)]) *
+                    real pb_term = (W->al[IM(k, js, i)] + W->al[IM(k, jn, i)]) *
                                      (W->pb[IM(k, jn, i)] - W->pb[IM(k, js, i)]);
-                    double dpy = (((msf * 0.5) * rdy) * mass_y) * ((ph_term + p_term) + pb_term);
-                    double bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_s + muw_n));
+                    real dpy = (((msf * 0.5) * rdy) * mass_y) * ((ph_term + p_term) + pb_term);
+                    real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_s + muw_n));
                     dpy = dpy + ((msf * rdy) * (W->php[IM(k, jn, i)] - W->php[IM(k, js, i)])) * bracket;
                     v = v - (dts * 1.0) * dpy;
                     v = v + g->c1h[k] * mudf_xy;
@@ -1124,40 +1156,40 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 size_t c = I2(j, i);
-                double msftx = g->msftx[c], msfty = g->msfty[c];
-                double dmdt = 0.0;
+                real msftx = g->msftx[c], msfty = g->msfty[c];
+                real dmdt = 0.0;
                 for (int k = 0; k < nz; ++k) {
-                    double c1 = g->c1h[k], c2 = g->c2h[k];
-                    double v_n = W->v_work[IV(k, j + 1, i)] +
+                    real c1 = g->c1h[k], c2 = g->c2h[k];
+                    real v_n = W->v_work[IV(k, j + 1, i)] +
                                  ((c1 * W->muv[I2V(j + 1, i)] + c2) * S->v[IV(k, j + 1, i)]) *
                                      (1.0 / g->msfvx[I2V(j + 1, i)]);
-                    double v_s = W->v_work[IV(k, j, i)] +
+                    real v_s = W->v_work[IV(k, j, i)] +
                                  ((c1 * W->muv[I2V(j, i)] + c2) * S->v[IV(k, j, i)]) * (1.0 / g->msfvx[I2V(j, i)]);
-                    double u_e = W->u_work[IU(k, j, i + 1)] +
+                    real u_e = W->u_work[IU(k, j, i + 1)] +
                                  ((c1 * W->muu[I2U(j, i + 1)] + c2) * S->u[IU(k, j, i + 1)]) / g->msfuy[I2U(j, i + 1)];
-                    double u_w = W->u_work[IU(k, j, i)] +
+                    real u_w = W->u_work[IU(k, j, i)] +
                                  ((c1 * W->muu[I2U(j, i)] + c2) * S->u[IU(k, j, i)]) / g->msfuy[I2U(j, i)];
-                    double d = (msftx * msfty) * (rdy * (v_n - v_s) + rdx * (u_e - u_w));
+                    real d = (msftx * msfty) * (rdy * (v_n - v_s) + rdx * (u_e - u_w));
                     W->dvdxi[IM(k, j, i)] = d;
                     dmdt = (k == 0) ? g->dnw[0] * d : dmdt + g->dnw[k] * d;
                 }
-                double mu_work_old = W->muts[c] - W->mut[c];
-                double mu_save = W->mu[c] - mu_work_old;
-                double mu_tendency = dmdt + 0.0;
-                double mu_work_new = mu_work_old + dts * mu_tendency;
+                real mu_work_old = W->muts[c] - W->mut[c];
+                real mu_save = W->mu[c] - mu_work_old;
+                real mu_tendency = dmdt + 0.0;
+                real mu_work_new = mu_work_old + dts * mu_tendency;
                 W->mu[c] = mu_save + mu_work_new;
                 W->mudf_new[c] = mu_tendency;
                 W->muts[c] = W->mut[c] + mu_work_new;
                 W->muave[c] = 0.5 * (eps_p * mu_work_new + eps_m * mu_work_old);
                 /* ww recurrence */
-                double wprev = W->ww[IM(0, j, i)];
+                real wprev = W->ww[IM(0, j, i)];
                 W->ww_new[IM(0, j, i)] = wprev - W->rom[IM(0, j, i)];
                 for (int kk = 1; kk < nz; ++kk) {
                     int k = kk - 1;
-                    double incr = (g->dnw[kk - 1] *
+                    real incr = (g->dnw[kk - 1] *
                                    ((g->c1h[k] * dmdt + W->dvdxi[IM(kk - 1, j, i)]) + g->c1h[k] * 0.0)) /
                                   msfty;
-                    double wk = wprev - incr;
+                    real wk = wprev - incr;
                     W->ww_new[IM(kk, j, i)] = wk - W->rom[IM(kk, j, i)];
                     wprev = wk;
                 }
@@ -1169,102 +1201,102 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
             for (int i = 0; i < nx; ++i) {
                 size_t c = I2(j, i);
                 int ip = wrap(i + 1, nx), im = wrap(i - 1, nx), jp = wrap(j + 1, ny), jm = wrap(j - 1, ny);
-                double msftx = g->msftx[c], msfty = g->msfty[c];
+                real msftx = g->msftx[c], msfty = g->msfty[c];
 #define T1(k, jj, ii) (S->theta[IM((k), (jj), (ii))] - T0)
-                double wd_lo = 0.0;
+                real wd_lo = 0.0;
                 for (int k = 0; k < nz; ++k) {
-                    double wd_hi;
+                    real wd_hi;
                     if (k + 1 < nz) {
                         int f = k + 1;
-                        double face = g->fnm[f] * T1(f, j, i) + g->fnp[f] * T1(f - 1, j, i);
+                        real face = g->fnm[f] * T1(f, j, i) + g->fnp[f] * T1(f - 1, j, i);
                         wd_hi = W->ww_new[IM(f, j, i)] * face;
                     } else {
                         wd_hi = 0.0;
                     }
-                    double th = T1(k, j, i);
-                    double v_flux = W->v_work[IV(k, j + 1, i)] * (T1(k, jp, i) + th) -
+                    real th = T1(k, j, i);
+                    real v_flux = W->v_work[IV(k, j + 1, i)] * (T1(k, jp, i) + th) -
                                     W->v_work[IV(k, j, i)] * (th + T1(k, jm, i));
-                    double u_flux = W->u_work[IU(k, j, i + 1)] * (T1(k, j, ip) + th) -
+                    real u_flux = W->u_work[IU(k, j, i + 1)] * (T1(k, j, ip) + th) -
                                     W->u_work[IU(k, j, i)] * (th + T1(k, j, im));
-                    double tendency = msftx * ((0.5 * rdy) * v_flux + (0.5 * rdx) * u_flux) +
+                    real tendency = msftx * ((0.5 * rdy) * v_flux + (0.5 * rdx) * u_flux) +
                                       g->rdnw[k] * (wd_hi - wd_lo);
                     size_t q = IM(k, j, i);
-                    double theta_i = W->th_work[q] + ((msfty * dts) * W->tht[q]);
+                    real theta_i = W->th_work[q] + ((msfty * dts) * W->tht[q]);
                     W->th_work[q] = theta_i - ((dts * msfty) * tendency);
                     wd_lo = wd_hi;
                 }
 #undef T1
             }
-        memcpy(W->ww, W->ww_new, sizeof(double) * (size_t)(nz + 1) * ny * nx);
-        memcpy(mudf, W->mudf_new, sizeof(double) * (size_t)ny * nx);
+        memcpy(W->ww, W->ww_new, sizeof(real) * (size_t)(nz + 1) * ny * nx);
+        memcpy(mudf, W->mudf_new, sizeof(real) * (size_t)ny * nx);
 
         /* ---- advance_w_wrf, core/advance_w.py:293-627 ---- */
 #pragma omp parallel for schedule(static)
         for (int j = 0; j < ny; ++j) {
-            double *rhs = malloc((size_t)(nz + 1) * sizeof(double));
-            double *wn = malloc((size_t)(nz + 1) * sizeof(double));
-            double *t2n = malloc((size_t)nz * sizeof(double));
+            real *rhs = (real *)malloc((size_t)(nz + 1) * sizeof(real));
+            real *wn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
+            real *t2n = (real *)malloc((size_t)nz * sizeof(real));
             for (int i = 0; i < nx; ++i) {
                 size_t c = I2(j, i);
-                double msfty = g->msfty[c], msftx = g->msftx[c];
-                double msft_inv = 1.0 / msfty;
-                double mut = W->mut[c], muts = W->muts[c], muave = W->muave[c];
+                real msfty = g->msfty[c];
+                real msft_inv = 1.0 / msfty;
+                real mut = W->mut[c], muts = W->muts[c], muave = W->muave[c];
 #define WQ(k) W->w_work[IM((k), j, i)]
                 /* w_damp_vertical_cfl on rw_tend (interior faces), into a local copy */
                 /* t_2ave */
                 for (int k = 0; k < nz; ++k) {
                     size_t q = IM(k, j, i);
-                    double mass_h = g->c1h[k] * muts + g->c2h[k];
-                    double half = 0.5 * (eps_p * W->th_work[q] + eps_m * W->t2ave[q]);
-                    double ttr = T0 + (S->theta[q] - T0);
+                    real mass_h = g->c1h[k] * muts + g->c2h[k];
+                    real half = 0.5 * (eps_p * W->th_work[q] + eps_m * W->t2ave[q]);
+                    real ttr = T0 + (S->theta[q] - T0);
                     t2n[k] = (half + (g->c1h[k] * muave) * T0) / (mass_h * ttr);
                 }
                 /* rhs of the phi equation */
                 rhs[0] = 0.0;
-                const double c05geps = (0.5 * G) * eps_m;
+                const real c05geps = (0.5 * G) * eps_m;
                 for (int f = 1; f <= nz; ++f) rhs[f] = dts * (W->ph_tend[IM(f, j, i)] + c05geps * WQ(f));
                 for (int f = 1; f < nz; ++f) {
 #define PH1T(k) (W->ph0[IM((k), j, i)] + W->phb[IM((k), j, i)])
-                    double wd_f1 = ((0.5 * (W->ww[IM(f + 1, j, i)] + W->ww[IM(f, j, i)])) * g->rdnw[f]) *
+                    real wd_f1 = ((0.5 * (W->ww[IM(f + 1, j, i)] + W->ww[IM(f, j, i)])) * g->rdnw[f]) *
                                    (PH1T(f + 1) - PH1T(f));
-                    double wd_f = ((0.5 * (W->ww[IM(f, j, i)] + W->ww[IM(f - 1, j, i)])) * g->rdnw[f - 1]) *
+                    real wd_f = ((0.5 * (W->ww[IM(f, j, i)] + W->ww[IM(f - 1, j, i)])) * g->rdnw[f - 1]) *
                                   (PH1T(f) - PH1T(f - 1));
-                    double adv = dts * (g->fnm[f] * wd_f1 + g->fnp[f] * wd_f);
+                    real adv = dts * (g->fnm[f] * wd_f1 + g->fnp[f] * wd_f);
                     rhs[f] = rhs[f] + (-adv);
                 }
                 for (int f = 1; f <= nz; ++f) {
-                    double mass_f = g->c1f[f] * mut + g->c2f[f];
+                    real mass_f = g->c1f[f] * mut + g->c2f[f];
                     rhs[f] = W->ph_work[IM(f, j, i)] + (msfty * rhs[f]) / mass_f;
                 }
                 if (nl->top_lid) rhs[nz] = 0.0;
                 /* surface w from terrain */
-                double w_surface = surface_w(g, W->phb, S->u, S->v, j, i);
+                real w_surface = surface_w(g, W->phb, S->u, S->v, j, i);
                 /* explicit w */
                 for (int f = 0; f <= nz; ++f) {
-                    double rw = W->rw_tend[IM(f, j, i)];
+                    real rw = W->rw_tend[IM(f, j, i)];
                     if (nl->w_damping == 1 && f >= 1 && f < nz) {
-                        double mass_f = g->c1f[f] * mut + g->c2f[f];
-                        double safe_mf = safe_pos(mass_f, 1.0e-12);
-                        double vert_cfl = fabs(((W->ww[IM(f, j, i)] / safe_mf) * g->rdnw[f]) * dts);
-                        double damp = (((-sgn(WQ(f))) * W_ALPHA) * (vert_cfl - W_BETA)) * mass_f;
+                        real mass_f = g->c1f[f] * mut + g->c2f[f];
+                        real safe_mf = safe_pos(mass_f, 1.0e-12);
+                        real vert_cfl = fabs(((W->ww[IM(f, j, i)] / safe_mf) * g->rdnw[f]) * dts);
+                        real damp = (((-sgn(WQ(f))) * W_ALPHA) * (vert_cfl - W_BETA)) * mass_f;
                         rw = rw + (vert_cfl > W_BETA ? damp : 0.0);
                     }
                     wn[f] = WQ(f) + dts * rw;
                 }
-                const double c05dtg = (0.5 * dts) * G;
+                const real c05dtg = (0.5 * dts) * G;
                 for (int f = 1; f < nz; ++f) {
-                    double mh_u = safe_pos(g->c1h[f] * mut + g->c2h[f], 1.0e-12);
-                    double mh_l = safe_pos(g->c1h[f - 1] * mut + g->c2h[f - 1], 1.0e-12);
-                    double coef_u = (W->c2a[IM(f, j, i)] * g->rdnw[f]) / mh_u;
-                    double coef_l = (W->c2a[IM(f - 1, j, i)] * g->rdnw[f - 1]) / mh_l;
-                    double phk1 = W->ph_work[IM(f + 1, j, i)], phk = W->ph_work[IM(f, j, i)],
+                    real mh_u = safe_pos(g->c1h[f] * mut + g->c2h[f], 1.0e-12);
+                    real mh_l = safe_pos(g->c1h[f - 1] * mut + g->c2h[f - 1], 1.0e-12);
+                    real coef_u = (W->c2a[IM(f, j, i)] * g->rdnw[f]) / mh_u;
+                    real coef_l = (W->c2a[IM(f - 1, j, i)] * g->rdnw[f - 1]) / mh_l;
+                    real phk1 = W->ph_work[IM(f + 1, j, i)], phk = W->ph_work[IM(f, j, i)],
                            phkm = W->ph_work[IM(f - 1, j, i)];
-                    double tA_u = coef_u * (eps_p * (rhs[f + 1] - rhs[f]) + eps_m * (phk1 - phk));
-                    double tA_l = coef_l * (eps_p * (rhs[f] - rhs[f - 1]) + eps_m * (phk - phkm));
-                    double termA = ((msft_inv * cqw_at(f, nz)) * (c05dtg * g->rdn[f])) * (tA_u - tA_l);
-                    double bu = (W->c2a[IM(f, j, i)] * W->alt[IM(f, j, i)]) * t2n[f];
-                    double bl = (W->c2a[IM(f - 1, j, i)] * W->alt[IM(f - 1, j, i)]) * t2n[f - 1];
-                    double termB = ((dts * G) * msft_inv) * (g->rdn[f] * (bu - bl) - (g->c1f[f] * muave));
+                    real tA_u = coef_u * (eps_p * (rhs[f + 1] - rhs[f]) + eps_m * (phk1 - phk));
+                    real tA_l = coef_l * (eps_p * (rhs[f] - rhs[f - 1]) + eps_m * (phk - phkm));
+                    real termA = ((msft_inv * cqw_at(f, nz)) * (c05dtg * g->rdn[f])) * (tA_u - tA_l);
+                    real bu = (W->c2a[IM(f, j, i)] * W->alt[IM(f, j, i)]) * t2n[f];
+                    real bl = (W->c2a[IM(f - 1, j, i)] * W->alt[IM(f - 1, j, i)]) * t2n[f - 1];
+                    real termB = ((dts * G) * msft_inv) * (g->rdn[f] * (bu - bl) - (g->c1f[f] * muave));
                     wn[f] = wn[f] + (termA + termB);
                 }
                 /* top face: top_lid -> 0 */
@@ -1276,21 +1308,21 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
                 for (int f = nz - 1; f >= 1; --f) wn[f] = wn[f] - W->cgamma[IM(f, j, i)] * wn[f + 1];
                 /* damp_opt = 3 */
                 if (nl->damp_opt == 3 && nl->dampcoef > 0.0) {
-                    double dampmag = dts * nl->dampcoef, hdepth = nl->zdamp;
-                    double htop = PH1T(nz) / G;
-                    double hbot = htop - hdepth;
+                    real dampmag = dts * nl->dampcoef, hdepth = nl->zdamp;
+                    real htop = PH1T(nz) / G;
+                    real hbot = htop - hdepth;
                     for (int f = 1; f <= nz; ++f) {
-                        double hk = PH1T(f) / G;
-                        double s = wd_sin(((0.5 * PI_D) * (hk - hbot)) / hdepth);
-                        double ramp = s * s;
-                        double dampwt = hk >= hbot ? dampmag * ramp : 0.0;
-                        double mass_f = g->c1f[f] * mut + g->c2f[f];
+                        real hk = PH1T(f) / G;
+                        real s = wd_sin(((0.5 * PI_D) * (hk - hbot)) / hdepth);
+                        real ramp = s * s;
+                        real dampwt = hk >= hbot ? dampmag * ramp : 0.0;
+                        real mass_f = g->c1f[f] * mut + g->c2f[f];
                         wn[f] = (wn[f] - (dampwt * mass_f) * S->w[IM(f, j, i)]) / (1.0 + dampwt);
                     }
                 }
                 /* geopotential finish */
                 for (int f = 1; f <= nz; ++f) {
-                    double mfs = g->c1f[f] * muts + g->c2f[f];
+                    real mfs = g->c1f[f] * muts + g->c2f[f];
                     W->ph_work[IM(f, j, i)] = rhs[f] + (((((msfty * 0.5) * dts) * G) * eps_p) * wn[f]) / mfs;
                 }
                 for (int f = 0; f <= nz; ++f) W->w_work[IM(f, j, i)] = wn[f];
@@ -1309,19 +1341,19 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
             for (int j = 0; j < ny; ++j)
                 for (int i = 0; i < nx; ++i) {
                     size_t c = I2(j, i), q = IM(k, j, i);
-                    double mu_work = W->muts[c] - W->mut[c];
-                    double mass_h = g->c1h[k] * W->muts[c] + g->c2h[k];
-                    double safe_mass = safe_pos(mass_h, 1.0e-12);
-                    double mu_term = g->c1h[k] * mu_work;
-                    double al = -(W->alt[q] * mu_term +
+                    real mu_work = W->muts[c] - W->mut[c];
+                    real mass_h = g->c1h[k] * W->muts[c] + g->c2h[k];
+                    real safe_mass = safe_pos(mass_h, 1.0e-12);
+                    real mu_term = g->c1h[k] * mu_work;
+                    real al = -(W->alt[q] * mu_term +
                                   g->rdnw[k] * (W->ph_work[IM(k + 1, j, i)] - W->ph_work[q])) /
                                 safe_mass;
-                    double theta_1 = S->theta[q] - T0;
-                    double ttr = T0 + theta_1;
-                    double safe_t = safe_pos(ttr, 1.0e-6);
-                    double p = W->c2a[q] *
+                    real theta_1 = S->theta[q] - T0;
+                    real ttr = T0 + theta_1;
+                    real safe_t = safe_pos(ttr, 1.0e-6);
+                    real p = W->c2a[q] *
                                (((W->alt[q] * (W->th_work[q] - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
-                    double ptmp = p;
+                    real ptmp = p;
                     p = p + SMDIV * (p - W->pm1[q]);
                     W->al[q] = al;
                     W->p[q] = p;
@@ -1335,17 +1367,17 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i <= nx; ++i) {
                 size_t q = IU(k, j, i);
-                double ms = g->c1h[k] * W->muus[I2U(j, i)] + g->c2h[k];
-                double mc = g->c1h[k] * W->muu[I2U(j, i)] + g->c2h[k];
-                double den = fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12);
+                real ms = g->c1h[k] * W->muus[I2U(j, i)] + g->c2h[k];
+                real mc = g->c1h[k] * W->muu[I2U(j, i)] + g->c2h[k];
+                real den = fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12);
                 S->u[q] = (g->msfuy[I2U(j, i)] * W->u_work[q] + S->u[q] * mc) / den;
             }
         for (int j = 0; j <= ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 size_t q = IV(k, j, i);
-                double ms = g->c1h[k] * W->muvs[I2V(j, i)] + g->c2h[k];
-                double mc = g->c1h[k] * W->muv[I2V(j, i)] + g->c2h[k];
-                double den = fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12);
+                real ms = g->c1h[k] * W->muvs[I2V(j, i)] + g->c2h[k];
+                real mc = g->c1h[k] * W->muv[I2V(j, i)] + g->c2h[k];
+                real den = fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12);
                 S->v[q] = (g->msfvx[I2V(j, i)] * W->v_work[q] + S->v[q] * mc) / den;
             }
     }
@@ -1354,12 +1386,12 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 size_t c = I2(j, i), q = IM(k, j, i);
-                double ms = g->c1f[k] * W->muts[c] + g->c2f[k];
-                double mc = g->c1f[k] * W->mut[c] + g->c2f[k];
-                double den = fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12);
+                real ms = g->c1f[k] * W->muts[c] + g->c2f[k];
+                real mc = g->c1f[k] * W->mut[c] + g->c2f[k];
+                real den = fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12);
                 S->w[q] = (g->msfty[c] * W->w_work[q] + S->w[q] * mc) / den;
-                double ph_pert = W->ph_work[q] + S->ph_pert[q];
-                double ph_base = S->ph_total[q] - S->ph_pert[q];
+                real ph_pert = W->ph_work[q] + S->ph_pert[q];
+                real ph_base = S->ph_total[q] - S->ph_pert[q];
                 S->ph_total[q] = ph_base + ph_pert;
                 S->ph_pert[q] = ph_pert;
             }
@@ -1368,21 +1400,21 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 size_t c = I2(j, i), q = IM(k, j, i);
-                double ms = g->c1h[k] * W->muts[c] + g->c2h[k];
-                double mc = g->c1h[k] * W->mut[c] + g->c2h[k];
-                double den = fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12);
-                double t_save = S->theta[q] - T0;
-                double theta_pert = (W->th_work[q] + t_save * mc) / den;
+                real ms = g->c1h[k] * W->muts[c] + g->c2h[k];
+                real mc = g->c1h[k] * W->mut[c] + g->c2h[k];
+                real den = fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12);
+                real t_save = S->theta[q] - T0;
+                real theta_pert = (W->th_work[q] + t_save * mc) / den;
                 S->theta[q] = theta_pert + T0;
                 /* p from the small step, then refreshed below */
-                double p_base = S->p_total[q] - S->p_pert[q];
+                real p_base = S->p_total[q] - S->p_pert[q];
                 S->p_total[q] = p_base + W->p[q];
                 S->p_pert[q] = W->p[q];
             }
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             size_t c = I2(j, i);
-            double mu_base = S->mu_total[c] - S->mu_pert[c];
+            real mu_base = S->mu_total[c] - S->mu_pert[c];
             S->mu_pert[c] = W->mu[c];
             S->mu_total[c] = mu_base + W->mu[c];
         }
@@ -1393,31 +1425,51 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, orc_state *S, ws
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             size_t c = I2(j, i);
-            double mub = W->mub[c];
-            double muts = mub + S->mu_pert[c];
+            real mub = W->mub[c];
+            real muts = mub + S->mu_pert[c];
             for (int k = 0; k < nz; ++k) {
                 size_t q = IM(k, j, i), q1 = IM(k + 1, j, i);
-                double phb_k = S->ph_total[q] - S->ph_pert[q];
-                double phb_k1 = S->ph_total[q1] - S->ph_pert[q1];
-                double mass_h_base = g->c1h[k] * mub + g->c2h[k];
-                double denom_alb = g->dnw[k] * mass_h_base;
-                double safe_da = safe_pos(denom_alb, 1.0e-12);
-                double alb = -(phb_k1 - phb_k) / safe_da;
-                double mass_w = g->c1h[k] * muts + g->c2h[k];
-                double safe_mass = safe_pos(mass_w, 1.0e-12);
-                double mu_term = (alb * g->c1h[k]) * S->mu_pert[c];
-                double geo_term = g->rdnw[k] * (S->ph_pert[q1] - S->ph_pert[q]);
-                double al = -(mu_term + geo_term) / safe_mass;
-                double alt = al + alb;
-                double ptot = orc_p_from_theta_alt(S->theta[q], alt);
-                double p_pert = ptot - W->pb[q];
-                double p_base = S->p_total[q] - S->p_pert[q];
+                real phb_k = S->ph_total[q] - S->ph_pert[q];
+                real phb_k1 = S->ph_total[q1] - S->ph_pert[q1];
+                real mass_h_base = g->c1h[k] * mub + g->c2h[k];
+                real denom_alb = g->dnw[k] * mass_h_base;
+                real safe_da = safe_pos(denom_alb, 1.0e-12);
+                real alb = -(phb_k1 - phb_k) / safe_da;
+                real mass_w = g->c1h[k] * muts + g->c2h[k];
+                real safe_mass = safe_pos(mass_w, 1.0e-12);
+                real mu_term = (alb * g->c1h[k]) * S->mu_pert[c];
+                real geo_term = g->rdnw[k] * (S->ph_pert[q1] - S->ph_pert[q]);
+                real al = -(mu_term + geo_term) / safe_mass;
+                real alt = al + alb;
+                real ptot = p_from_theta_alt(S->theta[q], alt);
+                real p_pert = ptot - W->pb[q];
+                real p_base = S->p_total[q] - S->p_pert[q];
                 S->p_total[q] = p_base + p_pert;
                 S->p_pert[q] = p_pert;
             }
         }
 }
 
+/* One large step in place: origin = rk1 reference (operational_mode.py:3328-3329),
+ * then the solve_em.F cadence of substeps 1, ns/2, ns (operational_mode.py:3524-3534). */
+static void large_step(const orc_grid *g, const orc_namelist *nl, rstate *s, ws_t *W) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
+    size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx, C = (size_t)ny * nx;
+    const double dt = nl->dt;
+    const int ns = nl->ns;
+    memcpy(W->u0, s->u, U * sizeof(real));
+    memcpy(W->v0, s->v, V * sizeof(real));
+    memcpy(W->w0, s->w, Wn * sizeof(real));
+    memcpy(W->th0, s->theta, M * sizeof(real));
+    memcpy(W->ph0, s->ph_pert, Wn * sizeof(real));
+    memcpy(W->mu0, s->mu_pert, C * sizeof(real));
+    rk_stage(g, nl, s, W, 1, dt / 3.0, 1);
+    rk_stage(g, nl, s, W, 2, dt / (double)ns, ns / 2 > 1 ? ns / 2 : 1);
+    rk_stage(g, nl, s, W, 3, dt / (double)ns, ns);
+}
+
+#ifndef ORC_AD
 int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     if (nx < 4 || ny < 1 || nz < 4) return -2;
@@ -1427,23 +1479,10 @@ int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps
         ws_free(&W);
         return -1;
     }
-    size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
-    size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx, C = (size_t)ny * nx;
-    const double dt = nl->dt;
-    const int ns = nl->ns;
-    for (int step = 0; step < n_steps; ++step) {
-        /* origin = rk1 reference (operational_mode.py:3328-3329) */
-        memcpy(W.u0, s->u, U * sizeof(double));
-        memcpy(W.v0, s->v, V * sizeof(double));
-        memcpy(W.w0, s->w, Wn * sizeof(double));
-        memcpy(W.th0, s->theta, M * sizeof(double));
-        memcpy(W.ph0, s->ph_pert, Wn * sizeof(double));
-        memcpy(W.mu0, s->mu_pert, C * sizeof(double));
-        /* solve_em.F cadence, operational_mode.py:3524-3534 */
-        rk_stage(g, nl, s, &W, 1, dt / 3.0, 1);
-        rk_stage(g, nl, s, &W, 2, dt / (double)ns, ns / 2 > 1 ? ns / 2 : 1);
-        rk_stage(g, nl, s, &W, 3, dt / (double)ns, ns);
-    }
+    rstate rs = {s->u, s->v, s->w, s->theta, s->p_total, s->p_pert,
+                 s->ph_total, s->ph_pert, s->mu_total, s->mu_pert, s->mudf};
+    for (int step = 0; step < n_steps; ++step) large_step(g, nl, &rs, &W);
     ws_free(&W);
     return 0;
 }
+#endif
