@@ -258,3 +258,93 @@ def surface_w(metrics, phb_surface, u, v) -> np.ndarray:
     if rc != 0:
         raise RuntimeError("orc_surface_w failed")
     return out
+
+
+# ---------------------------------------------------------------------------
+# A large step as a sequence of pieces over named arrays (wrfdyn_oracle.h)
+# ---------------------------------------------------------------------------
+
+PIECE_ORIGIN, PIECE_SETUP, PIECE_UV, PIECE_MU_THETA, PIECE_W, PIECE_P, PIECE_FINISH = range(7)
+_STATE_ARRAY = {"u": "u", "v": "v", "w": "w", "theta": "theta", "p_total": "p_total",
+                "p_perturbation": "p_pert", "ph_total": "ph_total", "ph_perturbation": "ph_pert",
+                "mu_total": "mu_total", "mu_perturbation": "mu_pert", "mudf": "mudf"}
+
+
+def array_names() -> list:
+    L = lib()
+    L.orc_array_name.restype = ctypes.c_char_p
+    return [L.orc_array_name(a).decode() for a in range(L.orc_array_count())]
+
+
+def _array_shapes(metrics) -> dict:
+    nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
+    shape = {0: (nz, ny, nx), 1: (nz + 1, ny, nx), 2: (nz, ny, nx + 1), 3: (nz, ny + 1, nx),
+             4: (ny, nx), 5: (ny, nx + 1), 6: (ny + 1, nx)}
+    L = lib()
+    return {name: shape[L.orc_array_extent(a)] for a, name in enumerate(array_names())}
+
+
+def new_arrays(metrics, state: dict) -> dict:
+    """The arrays of a large step: the state's fields (copied) and a zero workspace."""
+
+    arrays = {name: np.zeros(shp) for name, shp in _array_shapes(metrics).items()}
+    for key, name in _STATE_ARRAY.items():
+        if key in state:
+            arrays[name][...] = state[key]
+    return arrays
+
+
+def step_pieces(namelist) -> list:
+    """(piece, rk_step, dts) for every piece of one large step, in order."""
+
+    dt, ns = float(namelist.dt_s), int(namelist.acoustic_substeps)
+    out = [(PIECE_ORIGIN, 0, 0.0)]
+    for rk, dts, nsub in ((1, dt / 3.0, 1), (2, dt / float(ns), max(ns // 2, 1)), (3, dt / float(ns), ns)):
+        out.append((PIECE_SETUP, rk, dts))
+        for _ in range(nsub):
+            out += [(PIECE_UV, rk, dts), (PIECE_MU_THETA, rk, dts), (PIECE_W, rk, dts), (PIECE_P, rk, dts)]
+        out.append((PIECE_FINISH, rk, dts))
+    return out
+
+
+def _pointer_table(names, arrays: dict):
+    for n in names:
+        a = arrays[n]
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], n
+    return (_dp * len(names))(*[_ptr(arrays[n]) for n in names])
+
+
+def piece(metrics, namelist, arrays: dict, piece_id: int, rk_step: int, dts: float) -> None:
+    """Run one piece of a large step in place on ``arrays`` (from ``new_arrays``)."""
+
+    L = lib()
+    L.orc_piece.restype = ctypes.c_int
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    nl = _namelist(namelist)
+    table = _pointer_table(array_names(), arrays)
+    rc = L.orc_piece(ctypes.byref(g), ctypes.byref(nl), ctypes.c_int(piece_id), ctypes.c_int(rk_step),
+                     ctypes.c_double(dts), table)
+    if rc != 0:
+        raise RuntimeError(f"orc_piece failed with code {rc}")
+
+
+def piece_vjp(metrics, namelist, arrays: dict, piece_id: int, rk_step: int, dts: float, bar: dict) -> dict:
+    """Adjoint of every array before one piece, from ``bar`` (the adjoint of
+    the arrays after it; missing names count as zero).  ``arrays`` holds the
+    values before the piece and is not modified."""
+
+    L = grad_lib()
+    L.orc_piece_vjp.restype = ctypes.c_int
+    names = array_names()
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    nl = _namelist(namelist)
+    before = {n: np.ascontiguousarray(arrays[n], dtype=np.float64).copy() for n in names}
+    out = {n: (np.array(bar[n], dtype=np.float64, copy=True, order="C") if n in bar else np.zeros_like(before[n]))
+           for n in names}
+    rc = L.orc_piece_vjp(ctypes.byref(g), ctypes.byref(nl), ctypes.c_int(piece_id), ctypes.c_int(rk_step),
+                         ctypes.c_double(dts), _pointer_table(names, before), _pointer_table(names, out))
+    if rc != 0:
+        raise RuntimeError(f"orc_piece_vjp failed with code {rc}")
+    return out

@@ -163,4 +163,45 @@ int orc_vjp(const orc_grid *g, const orc_namelist *nl, const orc_state *in, int 
     return 0;
 }
 
+/* Vector-Jacobian product of one piece of a large step (ORC_PIECE_*).
+ *   arrays  the orc_array_count() arrays before the piece (not modified)
+ *   bar     in: the adjoint of every array after the piece;
+ *           out: the adjoint of every array before it
+ * Returns 0 or a negative error. */
+int orc_piece_vjp(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,
+                  double *const *arrays, double *const *bar) {
+    if (g->nx < 4 || g->ny < 1 || g->nz < 4) return -2;
+    std::vector<std::vector<ad>> a(N_ARRAYS);
+    std::vector<std::vector<int>> leaf(N_ARRAYS);
+    std::vector<real *> ptr(N_ARRAYS);
+    g_ad_tape.reset();
+    g_ad_tape.recording = true;
+    for (int k = 0; k < N_ARRAYS; ++k) {
+        const size_t n = extent_len(g->nx, g->ny, g->nz, array_extent[k]);
+        a[k].resize(n);
+        leaf[k].resize(n);
+        for (size_t q = 0; q < n; ++q) {
+            a[k][q] = ad_leaf(arrays[k][q]);
+            leaf[k][q] = a[k][q].id;
+        }
+        ptr[k] = a[k].data();
+    }
+    rstate S;
+    ws_t W;
+    bind_arrays(ptr.data(), &S, &W);
+    int rc = run_piece(g, nl, piece, rk_step, dts, &S, &W);
+    g_ad_tape.recording = false;
+    if (rc != 0) return rc;
+    std::vector<double> adj(g_ad_tape.nodes.size(), 0.0);
+    for (int k = 0; k < N_ARRAYS; ++k)
+        for (size_t q = 0; q < a[k].size(); ++q)
+            if (a[k][q].id) adj[a[k][q].id] += bar[k][q];
+    ad_reverse(adj);
+    for (int k = 0; k < N_ARRAYS; ++k)
+        for (size_t q = 0; q < a[k].size(); ++q) bar[k][q] = adj[leaf[k][q]];
+    g_ad_tape.nodes.clear();
+    g_ad_tape.nodes.shrink_to_fit();
+    return 0;
+}
+
 }  // extern "C"

@@ -768,40 +768,72 @@ int orc_surface_w(const orc_grid *g, const double *phb0, const double *u, const 
 
 static real *zalloc(size_t n) { return (real *)calloc(n ? n : 1, sizeof(real)); }
 
+/* Extents of the arrays a large step computes on. */
+enum { EX_M, EX_W, EX_U, EX_V, EX_C, EX_CU, EX_CV };
+
+static size_t extent_len(int nx, int ny, int nz, int ex) {
+    switch (ex) {
+        case EX_M: return (size_t)nz * ny * nx;
+        case EX_W: return (size_t)(nz + 1) * ny * nx;
+        case EX_U: return (size_t)nz * ny * (nx + 1);
+        case EX_V: return (size_t)nz * (ny + 1) * nx;
+        case EX_C: return (size_t)ny * nx;
+        case EX_CU: return (size_t)ny * (nx + 1);
+        default: return (size_t)(ny + 1) * nx;
+    }
+}
+
+/* The state fields, then the workspace arrays, in one fixed order. */
+#define N_STATE 11
+static const char *const array_names[] = {
+    "u", "v", "w", "theta", "p_total", "p_pert", "ph_total", "ph_pert", "mu_total", "mu_pert", "mudf",
+    "u0", "v0", "w0", "th0", "ph0", "mu0", "ru", "rv", "rom", "divv", "ut", "vt", "wt", "tht",
+    "alt_raw", "al_ls", "php_ls", "mub", "mut", "muts0", "mu_work0", "muu", "muus", "muv", "muvs",
+    "u_work", "v_work", "w_work", "th_work", "ph_work", "c2a", "alt", "pb", "phb", "php", "p", "al",
+    "pm1", "rw_tend", "ph_tend", "ca", "calpha", "cgamma", "mu", "muts", "muave", "ww", "t2ave",
+    "dvdxi", "mudf_new", "ww_new", "wsolve", "rhs", "tmp_ph_total"};
+static const int array_extent[] = {
+    EX_U, EX_V, EX_W, EX_M, EX_M, EX_M, EX_W, EX_W, EX_C, EX_C, EX_C,
+    EX_U, EX_V, EX_W, EX_M, EX_W, EX_C, EX_M, EX_M, EX_W, EX_M, EX_U, EX_V, EX_W, EX_M,
+    EX_M, EX_M, EX_M, EX_C, EX_C, EX_C, EX_C, EX_CU, EX_CU, EX_CV, EX_CV,
+    EX_U, EX_V, EX_W, EX_M, EX_W, EX_M, EX_M, EX_M, EX_W, EX_M, EX_M, EX_M,
+    EX_M, EX_W, EX_W, EX_W, EX_W, EX_W, EX_C, EX_C, EX_C, EX_W, EX_M,
+    EX_M, EX_C, EX_W, EX_W, EX_W, EX_W};
+#define N_ARRAYS ((int)(sizeof(array_names) / sizeof(array_names[0])))
+#define N_WS (N_ARRAYS - N_STATE)
+
+/* Addresses of the state and workspace array pointers, in the order above. */
+static void array_slots(rstate *s, ws_t *w, real **slot[N_ARRAYS]) {
+    real **all[] = {&s->u, &s->v, &s->w, &s->theta, &s->p_total, &s->p_pert, &s->ph_total,
+                    &s->ph_pert, &s->mu_total, &s->mu_pert, &s->mudf,
+                    &w->u0, &w->v0, &w->w0, &w->th0, &w->ph0, &w->mu0, &w->ru, &w->rv, &w->rom,
+                    &w->divv, &w->ut, &w->vt, &w->wt, &w->tht, &w->alt_raw, &w->al_ls,
+                    &w->php_ls, &w->mub, &w->mut, &w->muts0, &w->mu_work0, &w->muu, &w->muus,
+                    &w->muv, &w->muvs, &w->u_work, &w->v_work, &w->w_work, &w->th_work,
+                    &w->ph_work, &w->c2a, &w->alt, &w->pb, &w->phb, &w->php, &w->p, &w->al,
+                    &w->pm1, &w->rw_tend, &w->ph_tend, &w->ca, &w->calpha, &w->cgamma, &w->mu,
+                    &w->muts, &w->muave, &w->ww, &w->t2ave, &w->dvdxi, &w->mudf_new,
+                    &w->ww_new, &w->wsolve, &w->rhs, &w->tmp_ph_total};
+    for (int a = 0; a < N_ARRAYS; ++a) slot[a] = all[a];
+}
+
 static int ws_alloc(ws_t *w, int nx, int ny, int nz) {
-    size_t M = (size_t)nz * ny * nx, W = (size_t)(nz + 1) * ny * nx;
-    size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx;
-    size_t C = (size_t)ny * nx, CU = (size_t)ny * (nx + 1), CV = (size_t)(ny + 1) * nx;
+    rstate unused;
+    real **slot[N_ARRAYS];
     memset(w, 0, sizeof(*w));
-    real **all[] = {&w->u0, &w->v0, &w->w0, &w->th0, &w->ph0, &w->mu0, &w->ru, &w->rv, &w->rom,
-                      &w->divv, &w->ut, &w->vt, &w->wt, &w->tht, &w->alt_raw, &w->al_ls,
-                      &w->php_ls, &w->mub, &w->mut, &w->muts0, &w->mu_work0, &w->muu, &w->muus,
-                      &w->muv, &w->muvs, &w->u_work, &w->v_work, &w->w_work, &w->th_work,
-                      &w->ph_work, &w->c2a, &w->alt, &w->pb, &w->phb, &w->php, &w->p, &w->al,
-                      &w->pm1, &w->rw_tend, &w->ph_tend, &w->ca, &w->calpha, &w->cgamma, &w->mu,
-                      &w->muts, &w->muave, &w->ww, &w->t2ave, &w->dvdxi, &w->mudf_new,
-                      &w->ww_new, &w->wsolve, &w->rhs, &w->tmp_ph_total};
-    size_t sizes[] = {U, V, W, M, W, C, M, M, W, M, U, V, W, M, M, M, M, C, C, C, C, CU, CU, CV,
-                      CV, U, V, W, M, W, M, M, M, W, M, M, M, M, W, W, W, W, W, C, C, C, W, M,
-                      M, C, W, W, W, W};
-    size_t n = sizeof(sizes) / sizeof(sizes[0]);
-    for (size_t q = 0; q < n; ++q) {
-        *all[q] = zalloc(sizes[q]);
-        if (!*all[q]) return -1;
+    array_slots(&unused, w, slot);
+    for (int a = N_STATE; a < N_ARRAYS; ++a) {
+        *slot[a] = zalloc(extent_len(nx, ny, nz, array_extent[a]));
+        if (!*slot[a]) return -1;
     }
     return 0;
 }
 
 static void ws_free(ws_t *w) {
-    real **all[] = {&w->u0, &w->v0, &w->w0, &w->th0, &w->ph0, &w->mu0, &w->ru, &w->rv, &w->rom,
-                      &w->divv, &w->ut, &w->vt, &w->wt, &w->tht, &w->alt_raw, &w->al_ls,
-                      &w->php_ls, &w->mub, &w->mut, &w->muts0, &w->mu_work0, &w->muu, &w->muus,
-                      &w->muv, &w->muvs, &w->u_work, &w->v_work, &w->w_work, &w->th_work,
-                      &w->ph_work, &w->c2a, &w->alt, &w->pb, &w->phb, &w->php, &w->p, &w->al,
-                      &w->pm1, &w->rw_tend, &w->ph_tend, &w->ca, &w->calpha, &w->cgamma, &w->mu,
-                      &w->muts, &w->muave, &w->ww, &w->t2ave, &w->dvdxi, &w->mudf_new,
-                      &w->ww_new, &w->wsolve, &w->rhs, &w->tmp_ph_total};
-    for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(*all[q]);
+    rstate unused;
+    real **slot[N_ARRAYS];
+    array_slots(&unused, w, slot);
+    for (int a = N_STATE; a < N_ARRAYS; ++a) free(*slot[a]);
 }
 
 /* _u_face_average_2d / _v_face_average_2d, core/small_step_prep.py:77-88:
@@ -822,10 +854,13 @@ static void v_face_avg(int nx, int ny, const real *f, real *out) {
 }
 
 /* ------------------------------------------------------------------ */
-/* One RK stage: operational_mode.py:3331-3515 advance_stage            */
+/* One RK stage: operational_mode.py:3331-3515 advance_stage, as its    */
+/* pieces in order: stage_setup, then nsub times (substep_uv,           */
+/* substep_mu_theta, substep_w, substep_p), then stage_finish.  Each    */
+/* piece reads and writes only the state S and the workspace W.         */
 /* ------------------------------------------------------------------ */
-static void rk_stage(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W, int rk_step,
-                     double dts, int nsub) {
+static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W, int rk_step,
+                        double dts) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
     const size_t Mn = (size_t)nz * ny * nx;
@@ -1055,311 +1090,336 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t 
     memset(W->muave, 0, sizeof(real) * (size_t)ny * nx);
     memcpy(W->ww, W->rom, sizeof(real) * (size_t)(nz + 1) * ny * nx);
     memset(W->t2ave, 0, sizeof(real) * Mn);
+}
 
+static void substep_uv(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W, double dts) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const real *mudf = S->mudf;
+    (void)nl;
+
+    /* ---- advance_uv_wrf, core/acoustic.py:626-786 (emdiv=0.01) ---- */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j) {
+        real *dpn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
+        for (int i = 0; i <= nx; ++i) {
+            int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
+            real msf = g->msfux[I2U(j, i)] / g->msfuy[I2U(j, i)];
+            real muw_l = W->muts[I2(j, il)] - W->mut[I2(j, il)];
+            real muw_r = W->muts[I2(j, ir)] - W->mut[I2(j, ir)];
+            {
+                real ps0 = W->p[IM(0, j, il)] + W->p[IM(0, j, ir)];
+                real ps1 = W->p[IM(1, j, il)] + W->p[IM(1, j, ir)];
+                real ps2 = W->p[IM(2, j, il)] + W->p[IM(2, j, ir)];
+                dpn[0] = 0.5 * ((g->cf1 * ps0 + g->cf2 * ps1) + g->cf3 * ps2);
+                for (int k = 1; k < nz; ++k) {
+                    real psk = W->p[IM(k, j, il)] + W->p[IM(k, j, ir)];
+                    real pskm = W->p[IM(k - 1, j, il)] + W->p[IM(k - 1, j, ir)];
+                    dpn[k] = 0.5 * (g->fnm[k] * psk + g->fnp[k] * pskm);
+                }
+                real pt = W->p[IM(nz - 1, j, il)] + W->p[IM(nz - 1, j, ir)];
+                real pt2 = W->p[IM(nz - 2, j, il)] + W->p[IM(nz - 2, j, ir)];
+                real pt3 = W->p[IM(nz - 3, j, il)] + W->p[IM(nz - 3, j, ir)];
+                dpn[nz] = 0.5 * ((g->cf1 * pt + g->cf2 * pt2) + g->cf3 * pt3);
+            }
+            real mudf_xy = ((-EMDIV * g->dx) * (mudf[I2(j, ir)] - mudf[I2(j, il)])) / g->msfuy[I2U(j, i)];
+            for (int k = 0; k < nz; ++k) {
+                size_t q = IU(k, j, i);
+                real u = W->u_work[q] + dts * W->ut[q];
+                real mass_x = g->c1h[k] * W->muu[I2U(j, i)] + g->c2h[k];
+                real ph_term = (W->ph_work[IM(k + 1, j, ir)] - W->ph_work[IM(k + 1, j, il)]) +
+                                 (W->ph_work[IM(k, j, ir)] - W->ph_work[IM(k, j, il)]);
+                real p_term = (W->alt[IM(k, j, il)] + W->alt[IM(k, j, ir)]) *
+                                (W->p[IM(k, j, ir)] - W->p[IM(k, j, il)]);
+                real pb_term = (W->al[IM(k, j, il)] + W->al[IM(k, j, ir)]) *
+                                 (W->pb[IM(k, j, ir)] - W->pb[IM(k, j, il)]);
+                real dpx = (((msf * 0.5) * rdx) * mass_x) * ((ph_term + p_term) + pb_term);
+                real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_l + muw_r));
+                dpx = dpx + ((msf * rdx) * (W->php[IM(k, j, ir)] - W->php[IM(k, j, il)])) * bracket;
+                u = u - (dts * 1.0) * dpx;
+                u = u + g->c1h[k] * mudf_xy;
+                W->u_work[q] = u;
+            }
+        }
+        free(dpn);
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j <= ny; ++j) {
+        real *dpn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
+        int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
+        for (int i = 0; i < nx; ++i) {
+            real msf = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
+            real muw_s = W->muts[I2(js, i)] - W->mut[I2(js, i)];
+            real muw_n = W->muts[I2(jn, i)] - W->mut[I2(jn, i)];
+            {
+                real ps0 = W->p[IM(0, js, i)] + W->p[IM(0, jn, i)];
+                real ps1 = W->p[IM(1, js, i)] + W->p[IM(1, jn, i)];
+                real ps2 = W->p[IM(2, js, i)] + W->p[IM(2, jn, i)];
+                dpn[0] = 0.5 * ((g->cf1 * ps0 + g->cf2 * ps1) + g->cf3 * ps2);
+                for (int k = 1; k < nz; ++k) {
+                    real psk = W->p[IM(k, js, i)] + W->p[IM(k, jn, i)];
+                    real pskm = W->p[IM(k - 1, js, i)] + W->p[IM(k - 1, jn, i)];
+                    dpn[k] = 0.5 * (g->fnm[k] * psk + g->fnp[k] * pskm);
+                }
+                real pt = W->p[IM(nz - 1, js, i)] + W->p[IM(nz - 1, jn, i)];
+                real pt2 = W->p[IM(nz - 2, js, i)] + W->p[IM(nz - 2, jn, i)];
+                real pt3 = W->p[IM(nz - 3, js, i)] + W->p[IM(nz - 3, jn, i)];
+                dpn[nz] = 0.5 * ((g->cf1 * pt + g->cf2 * pt2) + g->cf3 * pt3);
+            }
+            real msfvx_inv = 1.0 / g->msfvx[I2V(j, i)];
+            real mudf_xy = ((-EMDIV * g->dy) * (mudf[I2(jn, i)] - mudf[I2(js, i)])) * msfvx_inv;
+            for (int k = 0; k < nz; ++k) {
+                size_t q = IV(k, j, i);
+                real v = W->v_work[q] + dts * W->vt[q];
+                real mass_y = g->c1h[k] * W->muv[I2V(j, i)] + g->c2h[k];
+                real ph_term = (W->ph_work[IM(k + 1, jn, i)] - W->ph_work[IM(k + 1, js, i)]) +
+                                 (W->ph_work[IM(k, jn, i)] - W->ph_work[IM(k, js, i)]);
+                real p_term = (W->alt[IM(k, js, i)] + W->alt[IM(k, jn, i)]) *
+                                (W->p[IM(k, jn, i)] - W->p[IM(k, js, i)]);
+                real pb_term = (W->al[IM(k, js, i)] + W->al[IM(k, jn, i)]) *
+                                 (W->pb[IM(k, jn, i)] - W->pb[IM(k, js, i)]);
+                real dpy = (((msf * 0.5) * rdy) * mass_y) * ((ph_term + p_term) + pb_term);
+                real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_s + muw_n));
+                dpy = dpy + ((msf * rdy) * (W->php[IM(k, jn, i)] - W->php[IM(k, js, i)])) * bracket;
+                v = v - (dts * 1.0) * dpy;
+                v = v + g->c1h[k] * mudf_xy;
+                W->v_work[q] = v;
+            }
+        }
+        free(dpn);
+    }
+}
+
+static void substep_mu_theta(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W, double dts) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
     const real eps_p = 1.0 + nl->epssm, eps_m = 1.0 - nl->epssm;
     real *mudf = S->mudf;
+    (void)nl;
 
-    for (int it = 0; it < nsub; ++it) {
-        /* ---- advance_uv_wrf, core/acoustic.py:626-786 (emdiv=0.01) ---- */
+    /* ---- advance_mu_t_wrf periodic, dynamics/mu_t_advance.py:139-292 ---- */
 #pragma omp parallel for schedule(static)
-        for (int j = 0; j < ny; ++j) {
-            real *dpn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
-            for (int i = 0; i <= nx; ++i) {
-                int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
-                real msf = g->msfux[I2U(j, i)] / g->msfuy[I2U(j, i)];
-                real muw_l = W->muts[I2(j, il)] - W->mut[I2(j, il)];
-                real muw_r = W->muts[I2(j, ir)] - W->mut[I2(j, ir)];
-                {
-                    real ps0 = W->p[IM(0, j, il)] + W->p[IM(0, j, ir)];
-                    real ps1 = W->p[IM(1, j, il)] + W->p[IM(1, j, ir)];
-                    real ps2 = W->p[IM(2, j, il)] + W->p[IM(2, j, ir)];
-                    dpn[0] = 0.5 * ((g->cf1 * ps0 + g->cf2 * ps1) + g->cf3 * ps2);
-                    for (int k = 1; k < nz; ++k) {
-                        real psk = W->p[IM(k, j, il)] + W->p[IM(k, j, ir)];
-                        real pskm = W->p[IM(k - 1, j, il)] + W->p[IM(k - 1, j, ir)];
-                        dpn[k] = 0.5 * (g->fnm[k] * psk + g->fnp[k] * pskm);
-                    }
-                    real pt = W->p[IM(nz - 1, j, il)] + W->p[IM(nz - 1, j, ir)];
-                    real pt2 = W->p[IM(nz - 2, j, il)] + W->p[IM(nz - 2, j, ir)];
-                    real pt3 = W->p[IM(nz - 3, j, il)] + W->p[IM(nz - 3, j, ir)];
-                    dpn[nz] = 0.5 * ((g->cf1 * pt + g->cf2 * pt2) + g->cf3 * pt3);
-                }
-                real mudf_xy = ((-EMDIV * g->dx) * (mudf[I2(j, ir)] - mudf[I2(j, il)])) / g->msfuy[I2U(j, i)];
-                for (int k = 0; k < nz; ++k) {
-                    size_t q = IU(k, j, i);
-                    real u = W->u_work[q] + dts * W->ut[q];
-                    real mass_x = g->c1h[k] * W->muu[I2U(j, i)] + g->c2h[k];
-                    real ph_term = (W->ph_work[IM(k + 1, j, ir)] - W->ph_work[IM(k + 1, j, il)]) +
-                                     (W->ph_work[IM(k, j, ir)] - W->ph_work[IM(k, j, il)]);
-                    real p_term = (W->alt[IM(k, j, il)] + W->alt[IM(k, j, ir)]) *
-                                    (W->p[IM(k, j, ir)] - W->p[IM(k, j, il)]);
-                    real pb_term = (W->al[IM(k, j, il)] + W->al[IM(k, j, ir)]) *
-                                     (W->pb[IM(k, j, ir)] - W->pb[IM(k, j, il)]);
-                    real dpx = (((msf * 0.5) * rdx) * mass_x) * ((ph_term + p_term) + pb_term);
-                    real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_l + muw_r));
-                    dpx = dpx + ((msf * rdx) * (W->php[IM(k, j, ir)] - W->php[IM(k, j, il)])) * bracket;
-                    u = u - (dts * 1.0) * dpx;
-                    u = u + g->c1h[k] * mudf_xy;
-                    W->u_work[q] = u;
-                }
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t c = I2(j, i);
+            real msftx = g->msftx[c], msfty = g->msfty[c];
+            real dmdt = 0.0;
+            for (int k = 0; k < nz; ++k) {
+                real c1 = g->c1h[k], c2 = g->c2h[k];
+                real v_n = W->v_work[IV(k, j + 1, i)] +
+                             ((c1 * W->muv[I2V(j + 1, i)] + c2) * S->v[IV(k, j + 1, i)]) *
+                                 (1.0 / g->msfvx[I2V(j + 1, i)]);
+                real v_s = W->v_work[IV(k, j, i)] +
+                             ((c1 * W->muv[I2V(j, i)] + c2) * S->v[IV(k, j, i)]) * (1.0 / g->msfvx[I2V(j, i)]);
+                real u_e = W->u_work[IU(k, j, i + 1)] +
+                             ((c1 * W->muu[I2U(j, i + 1)] + c2) * S->u[IU(k, j, i + 1)]) / g->msfuy[I2U(j, i + 1)];
+                real u_w = W->u_work[IU(k, j, i)] +
+                             ((c1 * W->muu[I2U(j, i)] + c2) * S->u[IU(k, j, i)]) / g->msfuy[I2U(j, i)];
+                real d = (msftx * msfty) * (rdy * (v_n - v_s) + rdx * (u_e - u_w));
+                W->dvdxi[IM(k, j, i)] = d;
+                dmdt = (k == 0) ? g->dnw[0] * d : dmdt + g->dnw[k] * d;
             }
-            free(dpn);
+            real mu_work_old = W->muts[c] - W->mut[c];
+            real mu_save = W->mu[c] - mu_work_old;
+            real mu_tendency = dmdt + 0.0;
+            real mu_work_new = mu_work_old + dts * mu_tendency;
+            W->mu[c] = mu_save + mu_work_new;
+            W->mudf_new[c] = mu_tendency;
+            W->muts[c] = W->mut[c] + mu_work_new;
+            W->muave[c] = 0.5 * (eps_p * mu_work_new + eps_m * mu_work_old);
+            /* ww recurrence */
+            real wprev = W->ww[IM(0, j, i)];
+            W->ww_new[IM(0, j, i)] = wprev - W->rom[IM(0, j, i)];
+            for (int kk = 1; kk < nz; ++kk) {
+                int k = kk - 1;
+                real incr = (g->dnw[kk - 1] *
+                               ((g->c1h[k] * dmdt + W->dvdxi[IM(kk - 1, j, i)]) + g->c1h[k] * 0.0)) /
+                              msfty;
+                real wk = wprev - incr;
+                W->ww_new[IM(kk, j, i)] = wk - W->rom[IM(kk, j, i)];
+                wprev = wk;
+            }
+            W->ww_new[IM(nz, j, i)] = W->ww[IM(nz, j, i)];
         }
+    /* theta update (uses theta_1 = t_save neighbours, ww_updated) */
 #pragma omp parallel for schedule(static)
-        for (int j = 0; j <= ny; ++j) {
-            real *dpn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
-            int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
-            for (int i = 0; i < nx; ++i) {
-                real msf = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
-                real muw_s = W->muts[I2(js, i)] - W->mut[I2(js, i)];
-                real muw_n = W->muts[I2(jn, i)] - W->mut[I2(jn, i)];
-                {
-                    real ps0 = W->p[IM(0, js, i)] + W->p[IM(0, jn, i)];
-                    real ps1 = W->p[IM(1, js, i)] + W->p[IM(1, jn, i)];
-                    real ps2 = W->p[IM(2, js, i)] + W->p[IM(2, jn, i)];
-                    dpn[0] = 0.5 * ((g->cf1 * ps0 + g->cf2 * ps1) + g->cf3 * ps2);
-                    for (int k = 1; k < nz; ++k) {
-                        real psk = W->p[IM(k, js, i)] + W->p[IM(k, jn, i)];
-                        real pskm = W->p[IM(k - 1, js, i)] + W->p[IM(k - 1, jn, i)];
-                        dpn[k] = 0.5 * (g->fnm[k] * psk + g->fnp[k] * pskm);
-                    }
-                    real pt = W->p[IM(nz - 1, js, i)] + W->p[IM(nz - 1, jn, i)];
-                    real pt2 = W->p[IM(nz - 2, js, i)] + W->p[IM(nz - 2, jn, i)];
-                    real pt3 = W->p[IM(nz - 3, js, i)] + W->p[IM(nz - 3, jn, i)];
-                    dpn[nz] = 0.5 * ((g->cf1 * pt + g->cf2 * pt2) + g->cf3 * pt3);
-                }
-                real msfvx_inv = 1.0 / g->msfvx[I2V(j, i)];
-                real mudf_xy = ((-EMDIV * g->dy) * (mudf[I2(jn, i)] - mudf[I2(js, i)])) * msfvx_inv;
-                for (int k = 0; k < nz; ++k) {
-                    size_t q = IV(k, j, i);
-                    real v = W->v_work[q] + dts * W->vt[q];
-                    real mass_y = g->c1h[k] * W->muv[I2V(j, i)] + g->c2h[k];
-                    real ph_term = (W->ph_work[IM(k + 1, jn, i)] - W->ph_work[IM(k + 1, js, i)]) +
-                                     (W->ph_work[IM(k, jn, i)] - W->ph_work[IM(k, js, i)]);
-                    real p_term = (W->alt[IM(k, js, i)] + W->alt[IM(k, jn, i)]) *
-                                    (W->p[IM(k, jn, i)] - W->p[IM(k, js, i)]);
-                    real pb_term = (W->al[IM(k, js, i)] + W->al[IM(k, jn, i)]) *
-                                     (W->pb[IM(k, jn, i)] - W->pb[IM(k, js, i)]);
-                    real dpy = (((msf * 0.5) * rdy) * mass_y) * ((ph_term + p_term) + pb_term);
-                    real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_s + muw_n));
-                    dpy = dpy + ((msf * rdy) * (W->php[IM(k, jn, i)] - W->php[IM(k, js, i)])) * bracket;
-                    v = v - (dts * 1.0) * dpy;
-                    v = v + g->c1h[k] * mudf_xy;
-                    W->v_work[q] = v;
-                }
-            }
-            free(dpn);
-        }
-
-        /* ---- advance_mu_t_wrf periodic, dynamics/mu_t_advance.py:139-292 ---- */
-#pragma omp parallel for schedule(static)
-        for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i) {
-                size_t c = I2(j, i);
-                real msftx = g->msftx[c], msfty = g->msfty[c];
-                real dmdt = 0.0;
-                for (int k = 0; k < nz; ++k) {
-                    real c1 = g->c1h[k], c2 = g->c2h[k];
-                    real v_n = W->v_work[IV(k, j + 1, i)] +
-                                 ((c1 * W->muv[I2V(j + 1, i)] + c2) * S->v[IV(k, j + 1, i)]) *
-                                     (1.0 / g->msfvx[I2V(j + 1, i)]);
-                    real v_s = W->v_work[IV(k, j, i)] +
-                                 ((c1 * W->muv[I2V(j, i)] + c2) * S->v[IV(k, j, i)]) * (1.0 / g->msfvx[I2V(j, i)]);
-                    real u_e = W->u_work[IU(k, j, i + 1)] +
-                                 ((c1 * W->muu[I2U(j, i + 1)] + c2) * S->u[IU(k, j, i + 1)]) / g->msfuy[I2U(j, i + 1)];
-                    real u_w = W->u_work[IU(k, j, i)] +
-                                 ((c1 * W->muu[I2U(j, i)] + c2) * S->u[IU(k, j, i)]) / g->msfuy[I2U(j, i)];
-                    real d = (msftx * msfty) * (rdy * (v_n - v_s) + rdx * (u_e - u_w));
-                    W->dvdxi[IM(k, j, i)] = d;
-                    dmdt = (k == 0) ? g->dnw[0] * d : dmdt + g->dnw[k] * d;
-                }
-                real mu_work_old = W->muts[c] - W->mut[c];
-                real mu_save = W->mu[c] - mu_work_old;
-                real mu_tendency = dmdt + 0.0;
-                real mu_work_new = mu_work_old + dts * mu_tendency;
-                W->mu[c] = mu_save + mu_work_new;
-                W->mudf_new[c] = mu_tendency;
-                W->muts[c] = W->mut[c] + mu_work_new;
-                W->muave[c] = 0.5 * (eps_p * mu_work_new + eps_m * mu_work_old);
-                /* ww recurrence */
-                real wprev = W->ww[IM(0, j, i)];
-                W->ww_new[IM(0, j, i)] = wprev - W->rom[IM(0, j, i)];
-                for (int kk = 1; kk < nz; ++kk) {
-                    int k = kk - 1;
-                    real incr = (g->dnw[kk - 1] *
-                                   ((g->c1h[k] * dmdt + W->dvdxi[IM(kk - 1, j, i)]) + g->c1h[k] * 0.0)) /
-                                  msfty;
-                    real wk = wprev - incr;
-                    W->ww_new[IM(kk, j, i)] = wk - W->rom[IM(kk, j, i)];
-                    wprev = wk;
-                }
-                W->ww_new[IM(nz, j, i)] = W->ww[IM(nz, j, i)];
-            }
-        /* theta update (uses theta_1 = t_save neighbours, ww_updated) */
-#pragma omp parallel for schedule(static)
-        for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i) {
-                size_t c = I2(j, i);
-                int ip = wrap(i + 1, nx), im = wrap(i - 1, nx), jp = wrap(j + 1, ny), jm = wrap(j - 1, ny);
-                real msftx = g->msftx[c], msfty = g->msfty[c];
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t c = I2(j, i);
+            int ip = wrap(i + 1, nx), im = wrap(i - 1, nx), jp = wrap(j + 1, ny), jm = wrap(j - 1, ny);
+            real msftx = g->msftx[c], msfty = g->msfty[c];
 #define T1(k, jj, ii) (S->theta[IM((k), (jj), (ii))] - T0)
-                real wd_lo = 0.0;
-                for (int k = 0; k < nz; ++k) {
-                    real wd_hi;
-                    if (k + 1 < nz) {
-                        int f = k + 1;
-                        real face = g->fnm[f] * T1(f, j, i) + g->fnp[f] * T1(f - 1, j, i);
-                        wd_hi = W->ww_new[IM(f, j, i)] * face;
-                    } else {
-                        wd_hi = 0.0;
-                    }
-                    real th = T1(k, j, i);
-                    real v_flux = W->v_work[IV(k, j + 1, i)] * (T1(k, jp, i) + th) -
-                                    W->v_work[IV(k, j, i)] * (th + T1(k, jm, i));
-                    real u_flux = W->u_work[IU(k, j, i + 1)] * (T1(k, j, ip) + th) -
-                                    W->u_work[IU(k, j, i)] * (th + T1(k, j, im));
-                    real tendency = msftx * ((0.5 * rdy) * v_flux + (0.5 * rdx) * u_flux) +
-                                      g->rdnw[k] * (wd_hi - wd_lo);
-                    size_t q = IM(k, j, i);
-                    real theta_i = W->th_work[q] + ((msfty * dts) * W->tht[q]);
-                    W->th_work[q] = theta_i - ((dts * msfty) * tendency);
-                    wd_lo = wd_hi;
+            real wd_lo = 0.0;
+            for (int k = 0; k < nz; ++k) {
+                real wd_hi;
+                if (k + 1 < nz) {
+                    int f = k + 1;
+                    real face = g->fnm[f] * T1(f, j, i) + g->fnp[f] * T1(f - 1, j, i);
+                    wd_hi = W->ww_new[IM(f, j, i)] * face;
+                } else {
+                    wd_hi = 0.0;
                 }
-#undef T1
+                real th = T1(k, j, i);
+                real v_flux = W->v_work[IV(k, j + 1, i)] * (T1(k, jp, i) + th) -
+                                W->v_work[IV(k, j, i)] * (th + T1(k, jm, i));
+                real u_flux = W->u_work[IU(k, j, i + 1)] * (T1(k, j, ip) + th) -
+                                W->u_work[IU(k, j, i)] * (th + T1(k, j, im));
+                real tendency = msftx * ((0.5 * rdy) * v_flux + (0.5 * rdx) * u_flux) +
+                                  g->rdnw[k] * (wd_hi - wd_lo);
+                size_t q = IM(k, j, i);
+                real theta_i = W->th_work[q] + ((msfty * dts) * W->tht[q]);
+                W->th_work[q] = theta_i - ((dts * msfty) * tendency);
+                wd_lo = wd_hi;
             }
-        memcpy(W->ww, W->ww_new, sizeof(real) * (size_t)(nz + 1) * ny * nx);
-        memcpy(mudf, W->mudf_new, sizeof(real) * (size_t)ny * nx);
+#undef T1
+        }
+    memcpy(W->ww, W->ww_new, sizeof(real) * (size_t)(nz + 1) * ny * nx);
+    memcpy(mudf, W->mudf_new, sizeof(real) * (size_t)ny * nx);
+}
 
-        /* ---- advance_w_wrf, core/advance_w.py:293-627 ---- */
+static void substep_w(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W, double dts) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const real eps_p = 1.0 + nl->epssm, eps_m = 1.0 - nl->epssm;
+
+    /* ---- advance_w_wrf, core/advance_w.py:293-627 ---- */
 #pragma omp parallel for schedule(static)
-        for (int j = 0; j < ny; ++j) {
-            real *rhs = (real *)malloc((size_t)(nz + 1) * sizeof(real));
-            real *wn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
-            real *t2n = (real *)malloc((size_t)nz * sizeof(real));
-            for (int i = 0; i < nx; ++i) {
-                size_t c = I2(j, i);
-                real msfty = g->msfty[c];
-                real msft_inv = 1.0 / msfty;
-                real mut = W->mut[c], muts = W->muts[c], muave = W->muave[c];
+    for (int j = 0; j < ny; ++j) {
+        real *rhs = (real *)malloc((size_t)(nz + 1) * sizeof(real));
+        real *wn = (real *)malloc((size_t)(nz + 1) * sizeof(real));
+        real *t2n = (real *)malloc((size_t)nz * sizeof(real));
+        for (int i = 0; i < nx; ++i) {
+            size_t c = I2(j, i);
+            real msfty = g->msfty[c];
+            real msft_inv = 1.0 / msfty;
+            real mut = W->mut[c], muts = W->muts[c], muave = W->muave[c];
 #define WQ(k) W->w_work[IM((k), j, i)]
-                /* w_damp_vertical_cfl on rw_tend (interior faces), into a local copy */
-                /* t_2ave */
-                for (int k = 0; k < nz; ++k) {
-                    size_t q = IM(k, j, i);
-                    real mass_h = g->c1h[k] * muts + g->c2h[k];
-                    real half = 0.5 * (eps_p * W->th_work[q] + eps_m * W->t2ave[q]);
-                    real ttr = T0 + (S->theta[q] - T0);
-                    t2n[k] = (half + (g->c1h[k] * muave) * T0) / (mass_h * ttr);
-                }
-                /* rhs of the phi equation */
-                rhs[0] = 0.0;
-                const real c05geps = (0.5 * G) * eps_m;
-                for (int f = 1; f <= nz; ++f) rhs[f] = dts * (W->ph_tend[IM(f, j, i)] + c05geps * WQ(f));
-                for (int f = 1; f < nz; ++f) {
+            /* w_damp_vertical_cfl on rw_tend (interior faces), into a local copy */
+            /* t_2ave */
+            for (int k = 0; k < nz; ++k) {
+                size_t q = IM(k, j, i);
+                real mass_h = g->c1h[k] * muts + g->c2h[k];
+                real half = 0.5 * (eps_p * W->th_work[q] + eps_m * W->t2ave[q]);
+                real ttr = T0 + (S->theta[q] - T0);
+                t2n[k] = (half + (g->c1h[k] * muave) * T0) / (mass_h * ttr);
+            }
+            /* rhs of the phi equation */
+            rhs[0] = 0.0;
+            const real c05geps = (0.5 * G) * eps_m;
+            for (int f = 1; f <= nz; ++f) rhs[f] = dts * (W->ph_tend[IM(f, j, i)] + c05geps * WQ(f));
+            for (int f = 1; f < nz; ++f) {
 #define PH1T(k) (W->ph0[IM((k), j, i)] + W->phb[IM((k), j, i)])
-                    real wd_f1 = ((0.5 * (W->ww[IM(f + 1, j, i)] + W->ww[IM(f, j, i)])) * g->rdnw[f]) *
-                                   (PH1T(f + 1) - PH1T(f));
-                    real wd_f = ((0.5 * (W->ww[IM(f, j, i)] + W->ww[IM(f - 1, j, i)])) * g->rdnw[f - 1]) *
-                                  (PH1T(f) - PH1T(f - 1));
-                    real adv = dts * (g->fnm[f] * wd_f1 + g->fnp[f] * wd_f);
-                    rhs[f] = rhs[f] + (-adv);
-                }
-                for (int f = 1; f <= nz; ++f) {
+                real wd_f1 = ((0.5 * (W->ww[IM(f + 1, j, i)] + W->ww[IM(f, j, i)])) * g->rdnw[f]) *
+                               (PH1T(f + 1) - PH1T(f));
+                real wd_f = ((0.5 * (W->ww[IM(f, j, i)] + W->ww[IM(f - 1, j, i)])) * g->rdnw[f - 1]) *
+                              (PH1T(f) - PH1T(f - 1));
+                real adv = dts * (g->fnm[f] * wd_f1 + g->fnp[f] * wd_f);
+                rhs[f] = rhs[f] + (-adv);
+            }
+            for (int f = 1; f <= nz; ++f) {
+                real mass_f = g->c1f[f] * mut + g->c2f[f];
+                rhs[f] = W->ph_work[IM(f, j, i)] + (msfty * rhs[f]) / mass_f;
+            }
+            if (nl->top_lid) rhs[nz] = 0.0;
+            /* surface w from terrain */
+            real w_surface = surface_w(g, W->phb, S->u, S->v, j, i);
+            /* explicit w */
+            for (int f = 0; f <= nz; ++f) {
+                real rw = W->rw_tend[IM(f, j, i)];
+                if (nl->w_damping == 1 && f >= 1 && f < nz) {
                     real mass_f = g->c1f[f] * mut + g->c2f[f];
-                    rhs[f] = W->ph_work[IM(f, j, i)] + (msfty * rhs[f]) / mass_f;
+                    real safe_mf = safe_pos(mass_f, 1.0e-12);
+                    real vert_cfl = fabs(((W->ww[IM(f, j, i)] / safe_mf) * g->rdnw[f]) * dts);
+                    real damp = (((-sgn(WQ(f))) * W_ALPHA) * (vert_cfl - W_BETA)) * mass_f;
+                    rw = rw + (vert_cfl > W_BETA ? damp : 0.0);
                 }
-                if (nl->top_lid) rhs[nz] = 0.0;
-                /* surface w from terrain */
-                real w_surface = surface_w(g, W->phb, S->u, S->v, j, i);
-                /* explicit w */
-                for (int f = 0; f <= nz; ++f) {
-                    real rw = W->rw_tend[IM(f, j, i)];
-                    if (nl->w_damping == 1 && f >= 1 && f < nz) {
-                        real mass_f = g->c1f[f] * mut + g->c2f[f];
-                        real safe_mf = safe_pos(mass_f, 1.0e-12);
-                        real vert_cfl = fabs(((W->ww[IM(f, j, i)] / safe_mf) * g->rdnw[f]) * dts);
-                        real damp = (((-sgn(WQ(f))) * W_ALPHA) * (vert_cfl - W_BETA)) * mass_f;
-                        rw = rw + (vert_cfl > W_BETA ? damp : 0.0);
-                    }
-                    wn[f] = WQ(f) + dts * rw;
-                }
-                const real c05dtg = (0.5 * dts) * G;
-                for (int f = 1; f < nz; ++f) {
-                    real mh_u = safe_pos(g->c1h[f] * mut + g->c2h[f], 1.0e-12);
-                    real mh_l = safe_pos(g->c1h[f - 1] * mut + g->c2h[f - 1], 1.0e-12);
-                    real coef_u = (W->c2a[IM(f, j, i)] * g->rdnw[f]) / mh_u;
-                    real coef_l = (W->c2a[IM(f - 1, j, i)] * g->rdnw[f - 1]) / mh_l;
-                    real phk1 = W->ph_work[IM(f + 1, j, i)], phk = W->ph_work[IM(f, j, i)],
-                           phkm = W->ph_work[IM(f - 1, j, i)];
-                    real tA_u = coef_u * (eps_p * (rhs[f + 1] - rhs[f]) + eps_m * (phk1 - phk));
-                    real tA_l = coef_l * (eps_p * (rhs[f] - rhs[f - 1]) + eps_m * (phk - phkm));
-                    real termA = ((msft_inv * cqw_at(f, nz)) * (c05dtg * g->rdn[f])) * (tA_u - tA_l);
-                    real bu = (W->c2a[IM(f, j, i)] * W->alt[IM(f, j, i)]) * t2n[f];
-                    real bl = (W->c2a[IM(f - 1, j, i)] * W->alt[IM(f - 1, j, i)]) * t2n[f - 1];
-                    real termB = ((dts * G) * msft_inv) * (g->rdn[f] * (bu - bl) - (g->c1f[f] * muave));
-                    wn[f] = wn[f] + (termA + termB);
-                }
-                /* top face: top_lid -> 0 */
-                wn[nz] = 0.0;
-                wn[0] = w_surface;
-                /* Thomas */
-                for (int f = 1; f <= nz; ++f)
-                    wn[f] = (wn[f] - W->ca[IM(f, j, i)] * wn[f - 1]) * W->calpha[IM(f, j, i)];
-                for (int f = nz - 1; f >= 1; --f) wn[f] = wn[f] - W->cgamma[IM(f, j, i)] * wn[f + 1];
-                /* damp_opt = 3 */
-                if (nl->damp_opt == 3 && nl->dampcoef > 0.0) {
-                    real dampmag = dts * nl->dampcoef, hdepth = nl->zdamp;
-                    real htop = PH1T(nz) / G;
-                    real hbot = htop - hdepth;
-                    for (int f = 1; f <= nz; ++f) {
-                        real hk = PH1T(f) / G;
-                        real s = wd_sin(((0.5 * PI_D) * (hk - hbot)) / hdepth);
-                        real ramp = s * s;
-                        real dampwt = hk >= hbot ? dampmag * ramp : 0.0;
-                        real mass_f = g->c1f[f] * mut + g->c2f[f];
-                        wn[f] = (wn[f] - (dampwt * mass_f) * S->w[IM(f, j, i)]) / (1.0 + dampwt);
-                    }
-                }
-                /* geopotential finish */
+                wn[f] = WQ(f) + dts * rw;
+            }
+            const real c05dtg = (0.5 * dts) * G;
+            for (int f = 1; f < nz; ++f) {
+                real mh_u = safe_pos(g->c1h[f] * mut + g->c2h[f], 1.0e-12);
+                real mh_l = safe_pos(g->c1h[f - 1] * mut + g->c2h[f - 1], 1.0e-12);
+                real coef_u = (W->c2a[IM(f, j, i)] * g->rdnw[f]) / mh_u;
+                real coef_l = (W->c2a[IM(f - 1, j, i)] * g->rdnw[f - 1]) / mh_l;
+                real phk1 = W->ph_work[IM(f + 1, j, i)], phk = W->ph_work[IM(f, j, i)],
+                       phkm = W->ph_work[IM(f - 1, j, i)];
+                real tA_u = coef_u * (eps_p * (rhs[f + 1] - rhs[f]) + eps_m * (phk1 - phk));
+                real tA_l = coef_l * (eps_p * (rhs[f] - rhs[f - 1]) + eps_m * (phk - phkm));
+                real termA = ((msft_inv * cqw_at(f, nz)) * (c05dtg * g->rdn[f])) * (tA_u - tA_l);
+                real bu = (W->c2a[IM(f, j, i)] * W->alt[IM(f, j, i)]) * t2n[f];
+                real bl = (W->c2a[IM(f - 1, j, i)] * W->alt[IM(f - 1, j, i)]) * t2n[f - 1];
+                real termB = ((dts * G) * msft_inv) * (g->rdn[f] * (bu - bl) - (g->c1f[f] * muave));
+                wn[f] = wn[f] + (termA + termB);
+            }
+            /* top face: top_lid -> 0 */
+            wn[nz] = 0.0;
+            wn[0] = w_surface;
+            /* Thomas */
+            for (int f = 1; f <= nz; ++f)
+                wn[f] = (wn[f] - W->ca[IM(f, j, i)] * wn[f - 1]) * W->calpha[IM(f, j, i)];
+            for (int f = nz - 1; f >= 1; --f) wn[f] = wn[f] - W->cgamma[IM(f, j, i)] * wn[f + 1];
+            /* damp_opt = 3 */
+            if (nl->damp_opt == 3 && nl->dampcoef > 0.0) {
+                real dampmag = dts * nl->dampcoef, hdepth = nl->zdamp;
+                real htop = PH1T(nz) / G;
+                real hbot = htop - hdepth;
                 for (int f = 1; f <= nz; ++f) {
-                    real mfs = g->c1f[f] * muts + g->c2f[f];
-                    W->ph_work[IM(f, j, i)] = rhs[f] + (((((msfty * 0.5) * dts) * G) * eps_p) * wn[f]) / mfs;
+                    real hk = PH1T(f) / G;
+                    real s = wd_sin(((0.5 * PI_D) * (hk - hbot)) / hdepth);
+                    real ramp = s * s;
+                    real dampwt = hk >= hbot ? dampmag * ramp : 0.0;
+                    real mass_f = g->c1f[f] * mut + g->c2f[f];
+                    wn[f] = (wn[f] - (dampwt * mass_f) * S->w[IM(f, j, i)]) / (1.0 + dampwt);
                 }
-                for (int f = 0; f <= nz; ++f) W->w_work[IM(f, j, i)] = wn[f];
-                for (int k = 0; k < nz; ++k) W->t2ave[IM(k, j, i)] = t2n[k];
+            }
+            /* geopotential finish */
+            for (int f = 1; f <= nz; ++f) {
+                real mfs = g->c1f[f] * muts + g->c2f[f];
+                W->ph_work[IM(f, j, i)] = rhs[f] + (((((msfty * 0.5) * dts) * G) * eps_p) * wn[f]) / mfs;
+            }
+            for (int f = 0; f <= nz; ++f) W->w_work[IM(f, j, i)] = wn[f];
+            for (int k = 0; k < nz; ++k) W->t2ave[IM(k, j, i)] = t2n[k];
 #undef WQ
 #undef PH1T
-            }
-            free(rhs);
-            free(wn);
-            free(t2n);
         }
-
-        /* ---- calc_p_rho_step (smdiv), core/calc_p_rho.py:142-183 ---- */
-#pragma omp parallel for schedule(static)
-        for (int k = 0; k < nz; ++k)
-            for (int j = 0; j < ny; ++j)
-                for (int i = 0; i < nx; ++i) {
-                    size_t c = I2(j, i), q = IM(k, j, i);
-                    real mu_work = W->muts[c] - W->mut[c];
-                    real mass_h = g->c1h[k] * W->muts[c] + g->c2h[k];
-                    real safe_mass = safe_pos(mass_h, 1.0e-12);
-                    real mu_term = g->c1h[k] * mu_work;
-                    real al = -(W->alt[q] * mu_term +
-                                  g->rdnw[k] * (W->ph_work[IM(k + 1, j, i)] - W->ph_work[q])) /
-                                safe_mass;
-                    real theta_1 = S->theta[q] - T0;
-                    real ttr = T0 + theta_1;
-                    real safe_t = safe_pos(ttr, 1.0e-6);
-                    real p = W->c2a[q] *
-                               (((W->alt[q] * (W->th_work[q] - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
-                    real ptmp = p;
-                    p = p + SMDIV * (p - W->pm1[q]);
-                    W->al[q] = al;
-                    W->p[q] = p;
-                    W->pm1[q] = ptmp;
-                }
+        free(rhs);
+        free(wn);
+        free(t2n);
     }
+}
+
+static void substep_p(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    (void)nl;
+
+    /* ---- calc_p_rho_step (smdiv), core/calc_p_rho.py:142-183 ---- */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t c = I2(j, i), q = IM(k, j, i);
+                real mu_work = W->muts[c] - W->mut[c];
+                real mass_h = g->c1h[k] * W->muts[c] + g->c2h[k];
+                real safe_mass = safe_pos(mass_h, 1.0e-12);
+                real mu_term = g->c1h[k] * mu_work;
+                real al = -(W->alt[q] * mu_term +
+                              g->rdnw[k] * (W->ph_work[IM(k + 1, j, i)] - W->ph_work[q])) /
+                            safe_mass;
+                real theta_1 = S->theta[q] - T0;
+                real ttr = T0 + theta_1;
+                real safe_t = safe_pos(ttr, 1.0e-6);
+                real p = W->c2a[q] *
+                           (((W->alt[q] * (W->th_work[q] - mu_term * theta_1)) / (safe_mass * safe_t)) - al);
+                real ptmp = p;
+                p = p + SMDIV * (p - W->pm1[q]);
+                W->al[q] = al;
+                W->p[q] = p;
+                W->pm1[q] = ptmp;
+            }
+}
+
+static void stage_finish(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    (void)nl;
 
     /* ---- small_step_finish_wrf, core/small_step_finish.py:22-101 ---- */
 #pragma omp parallel for schedule(static)
@@ -1450,24 +1510,70 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t 
         }
 }
 
-/* One large step in place: origin = rk1 reference (operational_mode.py:3328-3329),
- * then the solve_em.F cadence of substeps 1, ns/2, ns (operational_mode.py:3524-3534). */
-static void large_step(const orc_grid *g, const orc_namelist *nl, rstate *s, ws_t *W) {
+static void rk_stage(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W, int rk_step,
+                     double dts, int nsub) {
+    stage_setup(g, nl, S, W, rk_step, dts);
+    for (int it = 0; it < nsub; ++it) {
+        substep_uv(g, nl, S, W, dts);
+        substep_mu_theta(g, nl, S, W, dts);
+        substep_w(g, nl, S, W, dts);
+        substep_p(g, nl, S, W);
+    }
+    stage_finish(g, nl, S, W);
+}
+
+/* The rk1 reference of a large step: copies of the state it starts from
+ * (operational_mode.py:3328-3329). */
+static void step_origin(const orc_grid *g, const orc_namelist *nl, rstate *s, ws_t *W) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
     size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx, C = (size_t)ny * nx;
-    const double dt = nl->dt;
-    const int ns = nl->ns;
+    (void)nl;
     memcpy(W->u0, s->u, U * sizeof(real));
     memcpy(W->v0, s->v, V * sizeof(real));
     memcpy(W->w0, s->w, Wn * sizeof(real));
     memcpy(W->th0, s->theta, M * sizeof(real));
     memcpy(W->ph0, s->ph_pert, Wn * sizeof(real));
     memcpy(W->mu0, s->mu_pert, C * sizeof(real));
+}
+
+/* One large step in place: the origin, then the solve_em.F cadence of
+ * substeps 1, ns/2, ns (operational_mode.py:3524-3534). */
+static void large_step(const orc_grid *g, const orc_namelist *nl, rstate *s, ws_t *W) {
+    const double dt = nl->dt;
+    const int ns = nl->ns;
+    step_origin(g, nl, s, W);
     rk_stage(g, nl, s, W, 1, dt / 3.0, 1);
     rk_stage(g, nl, s, W, 2, dt / (double)ns, ns / 2 > 1 ? ns / 2 : 1);
     rk_stage(g, nl, s, W, 3, dt / (double)ns, ns);
 }
+
+/* One piece of a large step (ORC_PIECE_* of wrfdyn_oracle.h) on S and W. */
+static int run_piece(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,
+                     rstate *S, ws_t *W) {
+    switch (piece) {
+        case ORC_PIECE_ORIGIN: step_origin(g, nl, S, W); return 0;
+        case ORC_PIECE_SETUP: stage_setup(g, nl, S, W, rk_step, dts); return 0;
+        case ORC_PIECE_UV: substep_uv(g, nl, S, W, dts); return 0;
+        case ORC_PIECE_MU_THETA: substep_mu_theta(g, nl, S, W, dts); return 0;
+        case ORC_PIECE_W: substep_w(g, nl, S, W, dts); return 0;
+        case ORC_PIECE_P: substep_p(g, nl, S, W); return 0;
+        case ORC_PIECE_FINISH: stage_finish(g, nl, S, W); return 0;
+        default: return -2;
+    }
+}
+
+/* S and W over caller arrays in the order of array_names. */
+static void bind_arrays(real *const *arrays, rstate *S, ws_t *W) {
+    real **slot[N_ARRAYS];
+    memset(W, 0, sizeof(*W));
+    array_slots(S, W, slot);
+    for (int a = 0; a < N_ARRAYS; ++a) *slot[a] = arrays[a];
+}
+
+int orc_array_count(void) { return N_ARRAYS; }
+const char *orc_array_name(int a) { return a >= 0 && a < N_ARRAYS ? array_names[a] : ""; }
+int orc_array_extent(int a) { return a >= 0 && a < N_ARRAYS ? array_extent[a] : -1; }
 
 #ifndef ORC_AD
 int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps) {
@@ -1484,5 +1590,14 @@ int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps
     for (int step = 0; step < n_steps; ++step) large_step(g, nl, &rs, &W);
     ws_free(&W);
     return 0;
+}
+
+int orc_piece(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,
+              double *const *arrays) {
+    if (g->nx < 4 || g->ny < 1 || g->nz < 4) return -2;
+    rstate S;
+    ws_t W;
+    bind_arrays(arrays, &S, &W);
+    return run_piece(g, nl, piece, rk_step, dts, &S, &W);
 }
 #endif

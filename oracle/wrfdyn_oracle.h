@@ -60,6 +60,29 @@ typedef struct {
 /* Advance n_steps large steps in place.  Returns 0 or a negative error. */
 int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps);
 
+/* The arrays a large step computes on -- the 11 state fields (orc_state's
+ * order), then the workspace -- by index: name, and extent code
+ * 0 (nz,ny,nx), 1 (nz+1,ny,nx), 2 (nz,ny,nx+1), 3 (nz,ny+1,nx), 4 (ny,nx),
+ * 5 (ny,nx+1), 6 (ny+1,nx). */
+int orc_array_count(void);
+const char *orc_array_name(int a);
+int orc_array_extent(int a);
+
+/* A large step is ORC_PIECE_ORIGIN, then per RK stage (rk_step 1..3, dts
+ * dt/3, dt/ns, dt/ns) ORC_PIECE_SETUP, nsub (1, max(ns/2,1), ns) times the
+ * four substep pieces UV, MU_THETA, W, P, and ORC_PIECE_FINISH.  orc_piece
+ * runs one piece in place on `arrays` (orc_array_count() pointers in the
+ * order above).  Returns 0 or a negative error. */
+#define ORC_PIECE_ORIGIN 0
+#define ORC_PIECE_SETUP 1
+#define ORC_PIECE_UV 2
+#define ORC_PIECE_MU_THETA 3
+#define ORC_PIECE_W 4
+#define ORC_PIECE_P 5
+#define ORC_PIECE_FINISH 6
+int orc_piece(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,
+              double *const *arrays);
+
 /* Single-operator entry points used by the oracle's own pinning tests. */
 int orc_calc_coef_w(const orc_grid *g, const double *mut, const double *c2a,
                     double dts, double epssm, int top_lid,

@@ -138,3 +138,32 @@ def test_top_half_w2_sensitivity_to_theta(problem):
             tm[p] -= h
             best = min(best, abs((J(tp) - J(tm)) / (2 * h) - grad["theta"][p]) / abs(grad["theta"][p]))
         assert best <= 1e-5, (p, best, grad["theta"][p])
+
+
+def test_pieces_compose_to_the_step_and_their_vjps_to_the_step_vjp(problem):
+    """A large step run piece by piece (origin, per stage setup, the four
+    substep pieces nsub times, finish) is bit-identical to orc_run, and the
+    piece VJPs chained in reverse over that trace give the whole-step VJP.
+    Adjoint kernels are checked piece by piece against orc_piece_vjp."""
+
+    case, start = problem
+    arrays = oracle.new_arrays(case.metrics, start)
+    pieces = oracle.step_pieces(case.namelist)
+    assert len(pieces) == 1 + 3 * 2 + 4 * (1 + 3 + 6)
+    trace = []
+    for pc in pieces:
+        trace.append({k: v.copy() for k, v in arrays.items()})
+        oracle.piece(case.metrics, case.namelist, arrays, *pc)
+    ref = oracle.run(case.metrics, case.namelist, start, 1)
+    for key, name in oracle._STATE_ARRAY.items():
+        assert np.array_equal(arrays[name], ref[key]), key
+
+    rng = np.random.default_rng(7)
+    bar_state = {k: rng.standard_normal(v.shape) for k, v in ref.items()}
+    whole, _ = oracle.vjp(case.metrics, case.namelist, start, 1, bar_state)
+    bar = {oracle._STATE_ARRAY[k]: v for k, v in bar_state.items()}
+    for pc, before in zip(reversed(pieces), reversed(trace)):
+        bar = oracle.piece_vjp(case.metrics, case.namelist, before, *pc, bar)
+    for key, name in oracle._STATE_ARRAY.items():
+        scale = max(float(np.abs(whole[key]).max()), 1e-300)
+        assert float(np.abs(bar[name] - whole[key]).max()) <= 1e-12 * scale, key
