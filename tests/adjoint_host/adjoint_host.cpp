@@ -51,6 +51,50 @@ wd::DNamelist to_dnl(const HostNamelist& h) {
   return n;
 }
 
+// The stage setup's reverse phases S1 .. S9, each over the whole grid.
+int setup(const wd::DGrid& g, const wd::DNamelist& nl, int rk_step, double dts, const wd::adj::View& V, PlainAdd acc) {
+  using namespace wd::adj;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) adj_setup_s1_column(g, nl, dts, V, j, i, acc);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) adj_setup_s2_column(g, V, j, i);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i <= nx; ++i) adj_setup_s3_face(g, V, false, j, i);
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 0; i < nx; ++i) adj_setup_s3_face(g, V, true, j, i);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) adj_setup_s4_column(g, rk_step, V, j, i);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i <= nx; ++i) adj_setup_s4_clear(g, V, false, j, i);
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 0; i < nx; ++i) adj_setup_s4_clear(g, V, true, j, i);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i <= nx; ++i) adj_pgf_face(g, V, false, j, i, acc);
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 0; i < nx; ++i) adj_pgf_face(g, V, true, j, i, acc);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) adj_setup_s6_column(g, V, j, i, acc);
+  for (int k = 0; k < nz; ++k) {
+    for (int r = 0; r < ny; ++r) adj_advect_copy(g, V, false, k, r);
+    for (int r = 0; r < nx; ++r) adj_advect_copy(g, V, true, k, r);
+  }
+  for (int kind = ADV_SCALAR; kind <= ADV_W; ++kind) {
+    const int kmax = kind == ADV_W ? nz : nz - 1;
+    for (int k = 0; k <= kmax; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) adj_advect_h_point(g, V, kind, k, j, i, acc);
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) adj_advect_v_column(g, V, kind, j, i, acc);
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) adj_couple_column(g, V, j, i, acc);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) adj_couple_point(g, V, k, j, i, acc);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -59,8 +103,8 @@ int adjh_count(void) { return wd::adj::A_count; }
 
 // Adjoint of one substep piece (2 uv, 3 mu_theta, 4 w, 5 p; the numbering of
 // the oracle's ORC_PIECE_*).  Returns 0, or -2 for a piece not built here.
-int adjh_piece(const HostGrid* hg, const HostNamelist* hn, int piece, double dts, const double* const* val,
-               double* const* bar, double* scr) {
+int adjh_piece(const HostGrid* hg, const HostNamelist* hn, int piece, int rk_step, double dts,
+               const double* const* val, double* const* bar, double* scr) {
   const wd::DGrid g = to_dgrid(*hg);
   const wd::DNamelist nl = to_dnl(*hn);
   wd::adj::View V;
@@ -70,7 +114,28 @@ int adjh_piece(const HostGrid* hg, const HostNamelist* hn, int piece, double dts
   }
   V.scr = scr;
   PlainAdd acc;
+  const size_t nx = g.nx, ny = g.ny, nz = g.nz;
   switch (piece) {
+    case 0: {
+      const int pairs[6][2] = {{wd::adj::A_u, wd::adj::A_u0},         {wd::adj::A_v, wd::adj::A_v0},
+                               {wd::adj::A_w, wd::adj::A_w0},         {wd::adj::A_theta, wd::adj::A_th0},
+                               {wd::adj::A_ph_pert, wd::adj::A_ph0},  {wd::adj::A_mu_pert, wd::adj::A_mu0}};
+      const size_t len[6] = {nz * ny * (nx + 1), nz * (ny + 1) * nx, (nz + 1) * ny * nx, nz * ny * nx,
+                             (nz + 1) * ny * nx, ny * nx};
+      for (int p = 0; p < 6; ++p)
+        for (size_t q = 0; q < len[p]; ++q) wd::adj::adj_origin_elem(V, pairs[p][0], pairs[p][1], q);
+      return 0;
+    }
+    case 1:
+      return setup(g, nl, rk_step, dts, V, acc);
+    case 6:
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) wd::adj::adj_finish_column(g, V, j, i);
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i <= g.nx; ++i) wd::adj::adj_finish_face(g, V, false, j, i);
+      for (int j = 0; j <= g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) wd::adj::adj_finish_face(g, V, true, j, i);
+      return 0;
     case 2:
       for (int j = 0; j < g.ny; ++j)
         for (int i = 0; i <= g.nx; ++i) wd::adj::adj_uv_face(g, dts, V, false, j, i, acc);

@@ -44,7 +44,7 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
-def host_piece_adjoint(metrics, namelist, arrays: dict, piece_id: int, dts: float, bar: dict) -> dict:
+def host_piece_adjoint(metrics, namelist, arrays: dict, piece_id: int, rk_step: int, dts: float, bar: dict) -> dict:
     """The adjoint bodies' VJP of one piece, run serially on the host."""
 
     L = lib()
@@ -58,11 +58,24 @@ def host_piece_adjoint(metrics, namelist, arrays: dict, piece_id: int, dts: floa
            for n in names}
     nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
     scr = np.zeros(10 * (nz + 1) * ny * nx)
-    rc = L.adjh_piece(ctypes.byref(g), ctypes.byref(nl), ctypes.c_int(piece_id), ctypes.c_double(dts),
+    rc = L.adjh_piece(ctypes.byref(g), ctypes.byref(nl), ctypes.c_int(piece_id), ctypes.c_int(rk_step),
+                      ctypes.c_double(dts),
                       oracle._pointer_table(names, val), oracle._pointer_table(names, out), oracle._ptr(scr))
     if rc != 0:
         raise RuntimeError(f"adjh_piece failed with code {rc}")
     return out
+
+
+def adjoint_inputs(case, before: dict, piece_id: int, rk_step: int, dts: float) -> dict:
+    """The values a piece's adjoint reads: the arrays after the stage setup
+    for the setup (its work arrays are written, then read back), the arrays
+    before the piece for every other piece."""
+
+    if piece_id != oracle.PIECE_SETUP:
+        return before
+    after = {k: v.copy() for k, v in before.items()}
+    oracle.piece(case.metrics, case.namelist, after, piece_id, rk_step, dts)
+    return after
 
 
 def trace_step(metrics, namelist, state: dict) -> list:

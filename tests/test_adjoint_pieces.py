@@ -1,4 +1,4 @@
-"""The adjoint bodies of the acoustic substep pieces
+"""The adjoint bodies of every piece of the large step
 (wrf_gpu_amd/csrc/wrfdyn_adjoint.h), run serially on the host, against the
 gradient oracle's per-piece VJPs (oracle.piece_vjp).
 
@@ -6,7 +6,10 @@ Each piece's inputs are taken from a real large step of the oracle (piece by
 piece, tests/adjoint_host/harness.trace_step), and a random bar on every
 array after the piece must map to the oracle's bar before it to 1e-12 of
 each array's largest entry (the two differ only in the order in which
-contributions are summed).
+contributions are summed).  The stage setup's adjoint reads the arrays as
+they are after the setup (it writes only work arrays, which it reads back);
+every other piece reads them as they were before it.  Chained in reverse over
+a whole large step, the pieces give the gradient oracle's step VJP.
 """
 
 import numpy as np
@@ -16,7 +19,8 @@ from oracle import oracle
 from tests.adjoint_host import harness
 from wrf_gpu_amd import ic
 
-PIECES = {"uv": oracle.PIECE_UV, "mu_theta": oracle.PIECE_MU_THETA, "w": oracle.PIECE_W, "p": oracle.PIECE_P}
+PIECES = {"setup": oracle.PIECE_SETUP, "uv": oracle.PIECE_UV, "mu_theta": oracle.PIECE_MU_THETA,
+          "w": oracle.PIECE_W, "p": oracle.PIECE_P, "finish": oracle.PIECE_FINISH}
 TOL = 1e-12
 
 
@@ -33,7 +37,8 @@ def _compare(case, before, piece_id, rk, dts, seed):
     rng = np.random.default_rng(seed)
     bar = {k: rng.standard_normal(v.shape) for k, v in before.items()}
     ref = oracle.piece_vjp(case.metrics, case.namelist, before, piece_id, rk, dts, bar)
-    got = harness.host_piece_adjoint(case.metrics, case.namelist, before, piece_id, dts, bar)
+    got = harness.host_piece_adjoint(case.metrics, case.namelist, harness.adjoint_inputs(case, before, piece_id, rk, dts),
+                                     piece_id, rk, dts, bar)
     changed = 0
     for k in ref:
         scale = max(float(np.abs(ref[k]).max()), 1e-300)
@@ -69,3 +74,34 @@ def test_advance_w_adjoint_with_w_damping_active(traced):
                 for f in range(1, nz))
     assert above > 50
     _compare(case, before, pc, rk, dts, seed=99)
+
+
+def test_origin_adjoint_matches_oracle_vjp(traced):
+    case, trace = traced
+    pc, rk, dts, before = trace[0]
+    assert pc == oracle.PIECE_ORIGIN
+    rng = np.random.default_rng(3)
+    bar = {k: rng.standard_normal(v.shape) for k, v in before.items()}
+    ref = oracle.piece_vjp(case.metrics, case.namelist, before, pc, rk, dts, bar)
+    got = harness.host_piece_adjoint(case.metrics, case.namelist, before, pc, rk, dts, bar)
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
+
+
+def test_chained_piece_adjoints_give_the_step_vjp(traced):
+    """Every piece's adjoint, chained from the last piece of a large step to
+    the first, against orc_vjp for that step."""
+
+    case, trace = traced
+    start = {key: trace[0][3][name] for key, name in oracle._STATE_ARRAY.items()}
+    final = oracle.run(case.metrics, case.namelist, start, 1)
+    rng = np.random.default_rng(11)
+    bar_state = {k: rng.standard_normal(v.shape) for k, v in final.items()}
+    whole, _ = oracle.vjp(case.metrics, case.namelist, start, 1, bar_state)
+    bar = {oracle._STATE_ARRAY[k]: v for k, v in bar_state.items()}
+    for pc, rk, dts, before in reversed(trace):
+        bar = harness.host_piece_adjoint(case.metrics, case.namelist,
+                                         harness.adjoint_inputs(case, before, pc, rk, dts), pc, rk, dts, bar)
+    for key, name in oracle._STATE_ARRAY.items():
+        scale = max(float(np.abs(whole[key]).max()), 1e-300)
+        assert float(np.abs(bar[name] - whole[key]).max()) <= 1e-11 * scale, key
