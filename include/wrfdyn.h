@@ -152,6 +152,27 @@ int wd_batch_branches(const wd_batch *b);
 const char *wd_batch_last_error(const wd_batch *b);
 void wd_batch_destroy(wd_batch *b);
 
+/* Adjoint of one large step (the reference has no gradient; this is the
+ * vector-Jacobian product of the step wd_step runs, that of
+ * operational_mode.py:3319-3534, for the FP64 form).  `start` is the state
+ * the step starts from (device buffers, not modified); `bar` holds on entry
+ * the adjoint of every array of the state after the step and on return the
+ * adjoint of every array of `start`, each of the eleven arrays an
+ * independent input.  The step is recomputed from `start` with the forward
+ * kernels, and the arrays each adjoint kernel reads are kept on a tape in
+ * device memory (wd_adjoint_tape_bytes; allocated on the first call, which
+ * therefore must not be inside a graph capture).  Sums of contributions use
+ * atomic adds, so results agree with the CPU gradient to rounding, not bit
+ * for bit.  The handle's workspace is overwritten.  Asynchronous on
+ * `stream`; WD_EINVAL in the mixed form or when a start and a bar buffer
+ * coincide.  For n steps, keep the state at the start of every step and
+ * call this from the last step back to the first.
+ * Currently switched off: it returns WD_EINVAL with a message, because its
+ * first GPU run ended in a memory access fault whose cause is not yet found;
+ * the kernel bodies it runs are checked on the host. */
+int wd_step_adjoint(wd_handle *h, const wd_state *start, wd_state *bar, void *stream);
+size_t wd_adjoint_tape_bytes(const wd_handle *h);
+
 /* Kernel-duration instrumentation for benchmarking (no counterpart in the
  * reference, whose step is one XLA program).  Bit `id` of `kernel_mask`
  * brackets every launch of kernel `id` in subsequent wd_step calls with HIP

@@ -74,7 +74,7 @@ def test_create_rejects_bad_arguments_without_touching_the_device():
     lib.wd_destroy(None)
 
 
-def test_graph_batch_and_autotune_entry_points_reject_bad_arguments():
+def test_graph_batch_autotune_and_adjoint_entry_points_reject_bad_arguments():
     lib = _lib.load()
     assert lib.wd_capture_graph(None, None) == _lib.WD_EINVAL
     assert lib.wd_graph_step(None, 1, None) == _lib.WD_EINVAL
@@ -95,6 +95,8 @@ def test_graph_batch_and_autotune_entry_points_reject_bad_arguments():
     assert lib.wd_batch_branches(None) == _lib.WD_EINVAL
     assert lib.wd_batch_last_error(None) is not None
     lib.wd_batch_destroy(None)
+    assert lib.wd_step_adjoint(None, None, None, None) == _lib.WD_EINVAL
+    assert lib.wd_adjoint_tape_bytes(None) == 0
 
 
 def test_namelist_validation_matches_supported_path():

@@ -27,7 +27,7 @@ EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error
                     "wd_capture_graph", "wd_graph_step", "wd_autotune", "wd_tuned_config",
                     "wd_set_precision", "wd_get_precision",
                     "wd_batch_create", "wd_step_batched", "wd_batch_branch", "wd_batch_branches",
-                    "wd_batch_last_error", "wd_batch_destroy")
+                    "wd_batch_last_error", "wd_batch_destroy", "wd_step_adjoint", "wd_adjoint_tape_bytes")
 
 
 class WdGrid(ctypes.Structure):
@@ -97,6 +97,11 @@ def load() -> ctypes.CDLL:
         lib.wd_capture_graph.argtypes = [ctypes.c_void_p, ctypes.POINTER(WdState)]
         lib.wd_graph_step.restype = ctypes.c_int
         lib.wd_graph_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.wd_step_adjoint.restype = ctypes.c_int
+        lib.wd_step_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(WdState), ctypes.POINTER(WdState),
+                                        ctypes.c_void_p]
+        lib.wd_adjoint_tape_bytes.restype = ctypes.c_size_t
+        lib.wd_adjoint_tape_bytes.argtypes = [ctypes.c_void_p]
         lib.wd_autotune.restype = ctypes.c_int
         lib.wd_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(WdState), ctypes.c_int, ctypes.c_void_p]
         lib.wd_set_precision.restype = ctypes.c_int

@@ -1363,6 +1363,26 @@ hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DS
   return launch_column_tile(column_tile, g, nl, s, w, dts, first, st, prof);
 }
 
+hipError_t launch_substep_part(int part, const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
+                               double dts, int first, hipStream_t st) {
+  if (nl.mixed) return hipErrorInvalidValue;
+  constexpr int BX = 64, BY = 4;
+  const unsigned bs = 256;
+  const size_t C = (size_t)g.ny * g.nx;
+  if (part == 0) {
+    const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 2) / (BX - 1)) * (unsigned)((g.ny + 1 + BY - 1) / BY);
+    k_uv<BX, BY, double><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts);
+  } else if (part == 1) {
+    const unsigned mu_tiles = (unsigned)((g.nx + kMuOwned - 1) / kMuOwned) * (unsigned)((g.ny + kMuRows - 1) / kMuRows);
+    k_column_mu<double><<<mu_tiles, dim3(64, kMuRows), 0, st>>>(g, nl, s, w, dts, first);
+  } else if (part == 2) {
+    k_column_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
   const size_t UVcols = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;

@@ -457,7 +457,11 @@ __host__ __device__ inline void adj_mu_theta_column(const DGrid& g, const DNamel
   const int ip = wrap_h(i + 1, nx), im1 = wrap_h(i - 1, nx), jp = wrap_h(j + 1, ny), jm = wrap_h(j - 1, ny);
   auto T1 = [=](int k, int jj, int ii) { return val[A_theta][im(g, k, jj, ii)] - kT0; };
 
-  // ---- forward recompute: divergence, mu, ww ----
+  // ---- forward recompute: the column divergence d (scratch plane 0), its
+  // sum dmdt, and ww_new (scratch plane 1) ----
+  const size_t P = (size_t)(nz + 1) * ny * nx;
+  double* const d_s = V.scr;
+  double* const wwn_s = V.scr + P;
   double dmdt = 0.0;
   for (int k = 0; k < nz; ++k) {
     const double c1 = g.c1h[k], c2 = g.c2h[k];
@@ -466,30 +470,24 @@ __host__ __device__ inline void adj_mu_theta_column(const DGrid& g, const DNamel
     const double u_e = val[A_u_work][iu(g, k, j, i + 1)] + ((c1 * muu_e + c2) * val[A_u][iu(g, k, j, i + 1)]) / msfuy_e;
     const double u_w = val[A_u_work][iu(g, k, j, i)] + ((c1 * muu_w + c2) * val[A_u][iu(g, k, j, i)]) / msfuy_w;
     const double d = (msftx * msfty) * (g.rdy * (v_n - v_s) + g.rdx * (u_e - u_w));
+    d_s[Q(k)] = d;
     dmdt = (k == 0) ? g.dnw[0] * d : dmdt + g.dnw[k] * d;
   }
   const double muts = val[A_muts][c], mut = val[A_mut][c];
-  // ww_new, from the recurrence (recomputed level by level in the reverse pass)
-  auto d_at = [=](int k) {
-    const double c1 = g.c1h[k], c2 = g.c2h[k];
-    const double v_n = val[A_v_work][iv(g, k, j + 1, i)] + ((c1 * muv_n + c2) * val[A_v][iv(g, k, j + 1, i)]) * rinv_n;
-    const double v_s = val[A_v_work][iv(g, k, j, i)] + ((c1 * muv_s + c2) * val[A_v][iv(g, k, j, i)]) * rinv_s;
-    const double u_e = val[A_u_work][iu(g, k, j, i + 1)] + ((c1 * muu_e + c2) * val[A_u][iu(g, k, j, i + 1)]) / msfuy_e;
-    const double u_w = val[A_u_work][iu(g, k, j, i)] + ((c1 * muu_w + c2) * val[A_u][iu(g, k, j, i)]) / msfuy_w;
-    return (msftx * msfty) * (g.rdy * (v_n - v_s) + g.rdx * (u_e - u_w));
-  };
-  // ww_new[f] for the theta fluxes: wk chain from ww[0]
-  auto ww_new_at = [=](int f) {
-    if (f == 0) return val[A_ww][Q(0)] - val[A_rom][Q(0)];
-    if (f == nz) return val[A_ww][Q(nz)];
+  (void)muts;
+  (void)mut;
+  {
     double wk = val[A_ww][Q(0)];
-    for (int kk = 1; kk <= f; ++kk) {
+    wwn_s[Q(0)] = wk - val[A_rom][Q(0)];
+    for (int kk = 1; kk < nz; ++kk) {
       const int k = kk - 1;
-      const double incr = (g.dnw[kk - 1] * ((g.c1h[k] * dmdt + d_at(kk - 1)) + g.c1h[k] * 0.0)) / msfty;
+      const double incr = (g.dnw[kk - 1] * ((g.c1h[k] * dmdt + d_s[Q(kk - 1)]) + g.c1h[k] * 0.0)) / msfty;
       wk = wk - incr;
+      wwn_s[Q(kk)] = wk - val[A_rom][Q(kk)];
     }
-    return wk - val[A_rom][Q(f)];
-  };
+    wwn_s[Q(nz)] = val[A_ww][Q(nz)];
+  }
+  auto ww_new_at = [=](int f) { return wwn_s[Q(f)]; };
 
   // ---- reverse ----
   // ww = ww_new, mudf = mudf_new (copies); the old mudf is not read

@@ -32,6 +32,9 @@ struct wd_handle {
   // wd_capture_graph: one captured large step and the stream it is captured on
   hipStream_t capture_stream = nullptr;
   hipGraphExec_t graph = nullptr;
+  // wd_step_adjoint: the tape of one large step (allocated on first use)
+  wd::AdjTape* tape = nullptr;
+  size_t tape_bytes = 0;
 };
 
 // An ensemble: member handles (each its own grid, namelist and workspace)
@@ -726,8 +729,40 @@ void wd_destroy(wd_handle* h) {
   if (h->capture_stream) hipStreamDestroy(h->capture_stream);
   if (h->work_buf) hipFree(h->work_buf);
   if (h->metric_buf) hipFree(h->metric_buf);
+  wd::adjoint_tape_destroy(h->tape);
   if (cur != h->device) hipSetDevice(cur);
   delete h;
 }
+
+int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* stream) {
+  if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  wd::DState ds, db;
+  int rc = check_state(h, start, &ds);
+  if (rc == WD_OK) rc = check_state(h, bar, &db);
+  if (rc == WD_OK) rc = check_device(h);
+  if (rc != WD_OK) return rc;
+  if (h->nl.mixed) return fail(h, WD_EINVAL, "the adjoint is built for the FP64 form (WD_PREC_FP64) only");
+  // Switched off: the first GPU run of this path ended in a memory access
+  // fault whose cause is not yet found (its kernel bodies are bounds-checked
+  // on the host).  It stays off the GPU until the cause is shown.
+  return fail(h, WD_EINVAL, "wd_step_adjoint is switched off: its first GPU run faulted, cause not yet found");
+  const double* a[] = {ds.u, ds.v, ds.w, ds.theta, ds.p_total, ds.p_pert, ds.ph_total, ds.ph_pert,
+                       ds.mu_total, ds.mu_pert, ds.mudf};
+  const double* b[] = {db.u, db.v, db.w, db.theta, db.p_total, db.p_pert, db.ph_total, db.ph_pert,
+                       db.mu_total, db.mu_pert, db.mudf};
+  for (const double* x : a)
+    for (const double* y : b)
+      if (x == y) return fail(h, WD_EINVAL, "a start array and a bar array are the same buffer");
+  if (!h->tape) {
+    hipError_t e = wd::adjoint_tape_create(h->g, h->nl, &h->tape, &h->tape_bytes);
+    if (e != hipSuccess)
+      return fail(h, WD_ENOMEM, std::string("allocating the adjoint tape: ") + hipGetErrorString(e));
+  }
+  hipError_t e = wd::adjoint_large_step(h->g, h->nl, h->w, h->tape, ds, db, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(h, e, "adjoint step");
+  return WD_OK;
+}
+
+size_t wd_adjoint_tape_bytes(const wd_handle* h) { return h ? h->tape_bytes : 0; }
 
 }  // extern "C"

@@ -111,4 +111,19 @@ hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DS
 // small_step_finish + grid%p refresh.
 hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof);
 
+// One kernel of a plain-FP64 acoustic substep with the column-march pair:
+// part 0 advance_uv (k_uv), 1 advance_mu_t + theta (k_column_mu), 2 advance_w
+// + calc_p_rho (k_column_w).  The three in order are launch_acoustic_substep.
+hipError_t launch_substep_part(int part, const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
+                               double dts, int first, hipStream_t st);
+
+// Adjoint of one plain-FP64 large step (wrfdyn_adjoint.hip).  The tape is
+// allocated on first use and kept by the caller; the forward recompute runs
+// in w and the state scratch of the tape.
+struct AdjTape;
+hipError_t adjoint_tape_create(const DGrid& g, const DNamelist& nl, AdjTape** out, size_t* bytes);
+void adjoint_tape_destroy(AdjTape* t);
+hipError_t adjoint_large_step(const DGrid& g, const DNamelist& nl, const DWork& w, AdjTape* t, const DState& start,
+                              const DState& bar, hipStream_t st);
+
 }  // namespace wd

@@ -125,6 +125,26 @@ class Dycore:
             raise _lib.WrfdynError(code, _lib.last_error(self._handle))
         return state
 
+    def step_adjoint(self, start: dict, bar: dict, stream=None) -> dict:
+        """Adjoint of one large step from ``start`` (not modified): ``bar``
+        holds the adjoint of the state after the step and is overwritten, in
+        place, with the adjoint of ``start`` (every field an independent
+        input).  FP64 form only.  The first call allocates the tape."""
+
+        self.check_state(start)
+        self.check_state(bar)
+        stream = _stream_of(start, stream)
+        a, b = _state_ptrs(start), _state_ptrs(bar)
+        code = self._lib.wd_step_adjoint(self._handle, ctypes.byref(a), ctypes.byref(b),
+                                         ctypes.c_void_p(stream.cuda_stream))
+        if code != _lib.WD_OK:
+            raise _lib.WrfdynError(code, _lib.last_error(self._handle))
+        return bar
+
+    @property
+    def adjoint_tape_bytes(self) -> int:
+        return int(self._lib.wd_adjoint_tape_bytes(self._handle))
+
     def set_precision(self, precision: str) -> None:
         """Switch the arithmetic form ("fp64" or "mixed") for later steps; drops a captured graph."""
 
