@@ -72,6 +72,14 @@ def load() -> ctypes.CDLL:
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build the HIP extension first "
                 "(python -c 'import __graft_entry__ as e; e.build()')")
+        # torch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7) as the system ROCm: whichever loads first serves
+        # the whole process.  Import torch before the library so that both
+        # use torch's; loaded the other way round, torch finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         lib.wd_create.restype = ctypes.c_int
         lib.wd_create.argtypes = [ctypes.POINTER(WdGrid), ctypes.POINTER(WdNamelist),
