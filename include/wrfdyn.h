@@ -70,6 +70,9 @@ typedef struct {
     int damp_opt;           /* 0 or 3                                 */
     double dampcoef;
     double zdamp;
+    int diff_6th_opt;       /* 0 off; 1 or 2: the 6th-order horizontal
+                               filter, monotonic as the reference applies it */
+    double diff_6th_factor; /* diff_6th_factor (reference default 0.12)   */
 } wd_namelist;
 
 /* Prognostic State (src/gpuwrf/contracts/state.py:811-821) plus the carried

@@ -45,6 +45,7 @@ class _Namelist(ctypes.Structure):
         ("dt", ctypes.c_double), ("ns", ctypes.c_int), ("epssm", ctypes.c_double),
         ("top_lid", ctypes.c_int), ("w_damping", ctypes.c_int), ("damp_opt", ctypes.c_int),
         ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
+        ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
     ]
 
 
@@ -147,7 +148,8 @@ def run(metrics, namelist, state: dict, n_steps: int, threads: int | None = None
 def _namelist(namelist) -> _Namelist:
     return _Namelist(float(namelist.dt_s), int(namelist.acoustic_substeps), float(namelist.epssm),
                      int(bool(namelist.top_lid)), int(namelist.w_damping), int(namelist.damp_opt),
-                     float(namelist.dampcoef), float(namelist.zdamp))
+                     float(namelist.dampcoef), float(namelist.zdamp), int(namelist.diff_6th_opt),
+                     float(namelist.diff_6th_factor))
 
 
 def _state_copy(metrics, state: dict) -> dict:
@@ -226,6 +228,22 @@ def alt_from_theta_p(theta: float, p: float) -> float:
 
 def p_from_theta_alt(theta: float, alt: float) -> float:
     return float(lib().orc_p_from_theta_alt(theta, alt))
+
+
+def sixth_tendency(field: np.ndarray, coef: float) -> np.ndarray:
+    """The uncoupled 6th-order filter tendency (explicit_diffusion.py:200-223)
+    of a (nlev, nrow, ncol) field, periodic in rows and columns."""
+
+    L = lib()
+    L.orc_sixth_tendency.restype = ctypes.c_int
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    assert f.ndim == 3
+    out = np.zeros_like(f)
+    rc = L.orc_sixth_tendency(ctypes.c_int(f.shape[0]), ctypes.c_int(f.shape[1]), ctypes.c_int(f.shape[2]),
+                              _ptr(f), ctypes.c_double(coef), _ptr(out))
+    if rc != 0:
+        raise RuntimeError("orc_sixth_tendency failed")
+    return out
 
 
 def det_pow(x: float, y: float) -> float:

@@ -708,6 +708,63 @@ void orc_thomas(int nzp1, const double *a, const double *alpha, const double *ga
 #endif
 
 /* ------------------------------------------------------------------ */
+/* 6th-order numerical filter, dynamics/explicit_diffusion.py:159-223   */
+/* (_dflux6, _sixth_axis, sixth_order_diffusion_tendency), applied as   */
+/* operational_mode.py:2870-2876 does when diff_6th_opt != 0: periodic */
+/* along each horizontal axis over the field's own extent there, with  */
+/* the monotonic limiter (the reference passes monotonic=True for any  */
+/* nonzero diff_6th_opt).                                              */
+/* ------------------------------------------------------------------ */
+
+/* coef (dflux_p1 - dflux_p0) at index i of a periodic line of n values
+ * spaced `s` apart starting at f. */
+static inline real sixth_axis(const real *f, int i, int n, size_t s, double coef) {
+    const real fm3 = f[(size_t)wrap(i - 3, n) * s], fm2 = f[(size_t)wrap(i - 2, n) * s];
+    const real fm1 = f[(size_t)wrap(i - 1, n) * s], f0 = f[(size_t)i * s];
+    const real fp1 = f[(size_t)wrap(i + 1, n) * s], fp2 = f[(size_t)wrap(i + 2, n) * s];
+    const real fp3 = f[(size_t)wrap(i + 3, n) * s];
+    real d0 = (10.0 * (f0 - fm1) - 5.0 * (fp1 - fm2)) + (fp2 - fm3);
+    real d1 = (10.0 * (fp1 - f0) - 5.0 * (fp2 - fm1)) + (fp3 - fm2);
+    const real g0 = f0 - fm1, g1 = fp1 - f0;
+    if (d0 * g0 <= 0.0) d0 = 0.0;
+    if (d1 * g1 <= 0.0) d1 = 0.0;
+    return coef * (d1 - d0);
+}
+
+/* The uncoupled tendency of f (nlev, nrow, ncol): x part, plus the y part
+ * when there is more than one row. */
+static inline real sixth_at(const real *f, int nrow, int ncol, int k, int j, int i, double coef) {
+    const size_t row = ((size_t)k * nrow + (size_t)j) * ncol;
+    real t = sixth_axis(f + row, i, ncol, 1, coef);
+    if (nrow > 1) t = t + sixth_axis(f + (size_t)k * nrow * ncol + (size_t)i, j, nrow, (size_t)ncol, coef);
+    return t;
+}
+
+/* tend += (c1[k] mu[j, i] + c2[k]) * sixth(f): the coupled tendency. */
+static void sixth_filter(const real *f, int nlev, int nrow, int ncol, double coef, const double *c1,
+                         const double *c2, const real *mu, real *tend) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nlev; ++k)
+        for (int j = 0; j < nrow; ++j)
+            for (int i = 0; i < ncol; ++i) {
+                const size_t q = ((size_t)k * nrow + (size_t)j) * ncol + (size_t)i;
+                const real mass = c1[k] * mu[(size_t)j * ncol + (size_t)i] + c2[k];
+                tend[q] = tend[q] + mass * sixth_at(f, nrow, ncol, k, j, i, coef);
+            }
+}
+
+#ifndef ORC_AD
+int orc_sixth_tendency(int nlev, int nrow, int ncol, const double *f, double coef, double *out) {
+    if (nlev < 1 || nrow < 1 || ncol < 1 || !f || !out) return -1;
+    for (int k = 0; k < nlev; ++k)
+        for (int j = 0; j < nrow; ++j)
+            for (int i = 0; i < ncol; ++i)
+                out[((size_t)k * nrow + (size_t)j) * ncol + (size_t)i] = sixth_at(f, nrow, ncol, k, j, i, coef);
+    return 0;
+}
+#endif
+
+/* ------------------------------------------------------------------ */
 /* Workspace                                                           */
 /* ------------------------------------------------------------------ */
 typedef struct {
@@ -877,6 +934,21 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
         real *thp = W->th_work; /* temporary use before prep overwrites it */
         for (size_t q = 0; q < Mn; ++q) thp[q] = S->theta[q] - T0;
         advect_scalar_flux(g, thp, W->ru, W->rv, W->rom, W->tht);
+    }
+    /* 6th-order filter on the coupled u, v, w, theta tendencies, with the
+     * face masses of the stage mu_total (operational_mode.py:2768-2774,
+     * 2870-2876), before the large-step PGF is added. */
+    if (nl->diff_6th_opt != 0) {
+        const double coef = nl->diff_6th_factor * 0.015625 / (2.0 * nl->dt);
+        real *muu6 = zalloc((size_t)ny * (nx + 1)), *muv6 = zalloc((size_t)(ny + 1) * nx);
+        u_face_avg(nx, ny, S->mu_total, muu6);
+        v_face_avg(nx, ny, S->mu_total, muv6);
+        sixth_filter(S->u, nz, ny, nx + 1, coef, g->c1h, g->c2h, muu6, W->ut);
+        sixth_filter(S->v, nz, ny + 1, nx, coef, g->c1h, g->c2h, muv6, W->vt);
+        sixth_filter(S->w, nz + 1, ny, nx, coef, g->c1f, g->c2f, S->mu_total, W->wt);
+        sixth_filter(S->theta, nz, ny, nx, coef, g->c1h, g->c2h, S->mu_total, W->tht);
+        free(muu6);
+        free(muv6);
     }
     /* EOS alt of the stage-entry state, shared by _absolute_diagnostics
      * (rk_addtend_dry.py:184-186) and diagnose_pressure_al_alt's base_state=None

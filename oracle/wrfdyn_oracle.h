@@ -47,6 +47,8 @@ typedef struct {
     int    damp_opt;    /* 0 or 3                                   */
     double dampcoef;
     double zdamp;
+    int    diff_6th_opt;     /* 0 off; 1 or 2: 6th-order filter (monotonic) */
+    double diff_6th_factor;
 } orc_namelist;
 
 typedef struct {
@@ -96,6 +98,9 @@ int orc_advect_scalar(const orc_grid *g, const double *u, const double *v,
 /* Terrain surface w (advance_w.py:505-556) for every column: phb0 is the
  * surface base geopotential (ny*nx), u and v the staggered 3-D winds. */
 int orc_surface_w(const orc_grid *g, const double *phb0, const double *u, const double *v, double *out);
+/* The uncoupled 6th-order filter tendency (explicit_diffusion.py:200-223)
+ * of a (nlev, nrow, ncol) field, periodic in rows and columns. */
+int orc_sixth_tendency(int nlev, int nrow, int ncol, const double *f, double coef, double *out);
 double orc_pow(double x, double y);
 double orc_sin(double x);
 void orc_set_threads(int n);

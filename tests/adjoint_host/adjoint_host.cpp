@@ -21,6 +21,8 @@ struct HostNamelist {
   double epssm;
   int top_lid, w_damping, damp_opt;
   double dampcoef, zdamp;
+  int diff_6th_opt;
+  double diff_6th_factor;
 };
 
 struct PlainAdd {

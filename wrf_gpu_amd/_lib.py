@@ -46,6 +46,7 @@ class WdNamelist(ctypes.Structure):
         ("dt", ctypes.c_double), ("acoustic_substeps", ctypes.c_int), ("epssm", ctypes.c_double),
         ("top_lid", ctypes.c_int), ("w_damping", ctypes.c_int), ("damp_opt", ctypes.c_int),
         ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
+        ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
     ]
 
 

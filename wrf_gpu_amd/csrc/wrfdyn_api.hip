@@ -24,6 +24,7 @@ struct wd_handle {
   void* metric_buf = nullptr;
   void* work_buf = nullptr;
   size_t work_bytes = 0;
+  void* diff6_buf = nullptr;  // the 6th-order filter's tendencies (diff_6th_opt != 0)
   size_t metric_bytes = 0;
   int device = 0;
   int column_tile = 0;
@@ -259,6 +260,7 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   if (nl->damp_opt != 0 && nl->damp_opt != 3) return fail(nullptr, WD_EINVAL, "damp_opt must be 0 or 3");
   if (nl->damp_opt == 3 && nl->dampcoef > 0.0 && !(nl->zdamp > 0.0))
     return fail(nullptr, WD_EINVAL, "zdamp must be positive with damp_opt=3");
+  if (nl->diff_6th_opt < 0 || nl->diff_6th_opt > 2) return fail(nullptr, WD_EINVAL, "diff_6th_opt must be 0, 1 or 2");
 
   wd_handle* h = new (std::nothrow) wd_handle();
   if (!h) return fail(nullptr, WD_ENOMEM, "host allocation failed");
@@ -349,6 +351,8 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   h->nl.damp_opt = nl->damp_opt;
   h->nl.dampcoef = nl->dampcoef;
   h->nl.zdamp = nl->zdamp;
+  h->nl.diff_6th_opt = nl->diff_6th_opt;
+  h->nl.diff_6th_factor = nl->diff_6th_factor;
 
   // ---- workspace ----
   const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
@@ -394,6 +398,25 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return hip_fail(nullptr, e, "hipMemset(workspace)");
   }
   h->nl.mixed = 0;
+  if (nl->diff_6th_opt != 0) {
+    double** d6[] = {&w.d6u, &w.d6v, &w.d6w, &w.d6t};
+    const size_t d6n[] = {U, V, Wn, M};
+    size_t d6_total = 0;
+    for (size_t n : d6n) d6_total += (n * sizeof(double) + 255) / 256 * 256;
+    e = hipMalloc(&h->diff6_buf, d6_total);
+    if (e != hipSuccess) {
+      hipFree(h->work_buf);
+      hipFree(h->metric_buf);
+      delete h;
+      return fail(nullptr, WD_ENOMEM, std::string("hipMalloc(6th-order filter): ") + hipGetErrorString(e));
+    }
+    size_t o6 = 0;
+    for (int q = 0; q < 4; ++q) {
+      *d6[q] = reinterpret_cast<double*>(static_cast<char*>(h->diff6_buf) + o6);
+      o6 += (d6n[q] * sizeof(double) + 255) / 256 * 256;
+    }
+    h->work_bytes += d6_total;
+  }
   // Acoustic column kernel: the column-march pair (default) or, with
   // WRFDYN_COLUMN_TILE="TXxKT", the LDS-tiled kernel in one of its compiled
   // tile shapes (when that shape holds nz + 1 levels).
@@ -728,6 +751,7 @@ void wd_destroy(wd_handle* h) {
   if (h->graph) hipGraphExecDestroy(h->graph);
   if (h->capture_stream) hipStreamDestroy(h->capture_stream);
   if (h->work_buf) hipFree(h->work_buf);
+  if (h->diff6_buf) hipFree(h->diff6_buf);
   if (h->metric_buf) hipFree(h->metric_buf);
   wd::adjoint_tape_destroy(h->tape);
   if (cur != h->device) hipSetDevice(cur);
@@ -742,6 +766,7 @@ int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* st
   if (rc == WD_OK) rc = check_device(h);
   if (rc != WD_OK) return rc;
   if (h->nl.mixed) return fail(h, WD_EINVAL, "the adjoint is built for the FP64 form (WD_PREC_FP64) only");
+  if (h->nl.diff_6th_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the 6th-order filter");
   // Switched off: the first GPU run of this path ended in a memory access
   // fault whose cause is not yet found (its kernel bodies are bounds-checked
   // on the host).  It stays off the GPU until the cause is shown.

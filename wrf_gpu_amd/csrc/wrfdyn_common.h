@@ -47,6 +47,8 @@ struct DNamelist {
   double epssm;
   int top_lid, w_damping, damp_opt;
   double dampcoef, zdamp;
+  int diff_6th_opt;        // 0 off, 1/2 the 6th-order filter (monotonic)
+  double diff_6th_factor;
   int mixed;  // 1: mixed-precision form (FP32 tendencies and flux interpolants), 0: plain FP64
 };
 
@@ -79,6 +81,9 @@ struct DWork {
   // column-march scratch: phi-equation rhs and the Thomas forward sweep
   // (k_column_w); rhs also carries dvdxi between k_column_mu's two sweeps
   double *rhs, *wfwd;
+  // the 6th-order filter's coupled tendencies of u, v, w, theta (null unless
+  // diff_6th_opt != 0), added to the RK tendencies before the PGF
+  double *d6u, *d6v, *d6w, *d6t;
 };
 
 __host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {

@@ -30,6 +30,7 @@ enum KernelId {
   KID_COLUMN_W,   // column march, advance_w + calc_p_rho kernel
   KID_FIN_UV,
   KID_FIN_COL,
+  KID_DIFF6,
   KID_COUNT
 };
 

@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DState s, D
       t = t - g.rdnw[k] * (vf_hi - vf_lo);
       vf_lo = vf_hi;
     }
-    if (owned) stt<TT>(w.tht, o + c8, t);
+    if (owned) stt<TT>(w.tht, o + c8, w.d6t ? t + ldo(w.d6t, o + c8) : t);
     tm1 = t0;
     t0 = tp1;
     tp1 = tp2;
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
       if (f >= 1 && f < nz) t = t + (-g.rdn[f]) * (vfl_hi - vfl_lo);
       else t = t + 0.0;
     }
-    stt<TT>(w.wt, (unsigned)f * pl8 + c8, t);
+    stt<TT>(w.wt, (unsigned)f * pl8 + c8, w.d6w ? t + ldo(w.d6w, (unsigned)f * pl8 + c8) : t);
     wm1 = w0;
     w0 = wp1;
     wp1 = wp2;
@@ -550,7 +550,10 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
     double dpx = (((msf_u * 0.5) * g.rdx) * mass_u) * ((ph_term + p_term) + pb_term);
     double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_l + mu_r));
     dpx = dpx + ((msf_u * g.rdx) * (cur.phpr - cur.phpl)) * bracket;
-    if (owned) stt<TT>(w.ut, (unsigned)k * upl8 + fo8, t + (-1.0 * dpx));
+    if (owned) {
+      const unsigned off = (unsigned)k * upl8 + fo8;
+      stt<TT>(w.ut, off, (w.d6u ? t + ldo(w.d6u, off) : t) + (-1.0 * dpx));
+    }
     qm1 = q0;
     q0 = qp1;
     qp1 = qp2;
@@ -776,7 +779,10 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, D
     double dpy = (((msf_v * 0.5) * g.rdy) * mass_v) * ((ph_term + p_term) + pb_term);
     double bracket = g.rdnw[k] * (dpn_k1 - dpn_k) - 0.5 * (g.c1h[k] * (mu_s + mu_n));
     dpy = dpy + ((msf_v * g.rdy) * (cur.phpr - cur.phpl)) * bracket;
-    if (owned) stt<TT>(w.vt, (unsigned)k * upl8 + fo8, t + (-1.0 * dpy));
+    if (owned) {
+      const unsigned off = (unsigned)k * upl8 + fo8;
+      stt<TT>(w.vt, off, (w.d6v ? t + ldo(w.d6v, off) : t) + (-1.0 * dpy));
+    }
     qm1 = q0;
     q0 = qp1;
     qp1 = qp2;
@@ -1236,6 +1242,65 @@ inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// 6th-order numerical filter (dynamics/explicit_diffusion.py:159-223 _dflux6,
+// _sixth_axis, sixth_order_diffusion_tendency; applied at
+// runtime/operational_mode.py:2870-2876): out = mass * sixth(f) for one of
+// u (kind 0), v (1), w (2), theta (3), periodic along x and (with more than
+// one row) y over the field's own extents, with the monotonic limiter; mass
+// from the stage mu_total (face averages for u and v, :2768-2774).  The
+// tendency kernels add it before the large-step PGF.
+// ---------------------------------------------------------------------------
+__device__ inline double d6_axis(const double* f, int i, int n, size_t s, double coef) {
+  const double fm3 = f[(size_t)wrapi(i - 3, n) * s], fm2 = f[(size_t)wrapi(i - 2, n) * s];
+  const double fm1 = f[(size_t)wrapi(i - 1, n) * s], f0 = f[(size_t)i * s];
+  const double fp1 = f[(size_t)wrapi(i + 1, n) * s], fp2 = f[(size_t)wrapi(i + 2, n) * s];
+  const double fp3 = f[(size_t)wrapi(i + 3, n) * s];
+  double d0 = (10.0 * (f0 - fm1) - 5.0 * (fp1 - fm2)) + (fp2 - fm3);
+  double d1 = (10.0 * (fp1 - f0) - 5.0 * (fp2 - fm1)) + (fp3 - fm2);
+  const double g0 = f0 - fm1, g1 = fp1 - f0;
+  if (d0 * g0 <= 0.0) d0 = 0.0;
+  if (d1 * g1 <= 0.0) d1 = 0.0;
+  return coef * (d1 - d0);
+}
+
+__global__ void __launch_bounds__(256) k_diff6(DGrid g, const double* f, int kind, int nlev, int nrow, int ncol,
+                                               const double* mu, double coef, double* out) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t plane = (size_t)nrow * ncol;
+  if (q >= (size_t)nlev * plane) return;
+  const int k = (int)(q / plane);
+  const int j = (int)((q % plane) / ncol), i = (int)(q % ncol);
+  const int nx = g.nx, ny = g.ny;
+  double m;
+  if (kind == 0) {
+    m = i == 0 ? mu[(size_t)j * nx] : (i == nx ? mu[(size_t)j * nx + nx - 1]
+                                              : 0.5 * (mu[(size_t)j * nx + i - 1] + mu[(size_t)j * nx + i]));
+  } else if (kind == 1) {
+    m = j == 0 ? mu[i] : (j == ny ? mu[(size_t)(ny - 1) * nx + i]
+                                  : 0.5 * (mu[(size_t)(j - 1) * nx + i] + mu[(size_t)j * nx + i]));
+  } else {
+    m = mu[(size_t)j * nx + i];
+  }
+  const double mass = kind == 2 ? g.c1f[k] * m + g.c2f[k] : g.c1h[k] * m + g.c2h[k];
+  const double* row = f + ((size_t)k * nrow + j) * ncol;
+  double t = d6_axis(row, i, ncol, 1, coef);
+  if (nrow > 1) t = t + d6_axis(f + (size_t)k * plane + i, j, nrow, (size_t)ncol, coef);
+  out[q] = mass * t;
+}
+
+void launch_diff6(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
+  const unsigned bs = 256;
+  const double coef = nl.diff_6th_factor * 0.015625 / (2.0 * nl.dt);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
+  const size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx;
+  k_diff6<<<blocks_for(U, bs), bs, 0, st>>>(g, s.u, 0, nz, ny, nx + 1, s.mu_total, coef, w.d6u);
+  k_diff6<<<blocks_for(V, bs), bs, 0, st>>>(g, s.v, 1, nz, ny + 1, nx, s.mu_total, coef, w.d6v);
+  k_diff6<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s.w, 2, nz + 1, ny, nx, s.mu_total, coef, w.d6w);
+  k_diff6<<<blocks_for(M, bs), bs, 0, st>>>(g, s.theta, 3, nz, ny, nx, s.mu_total, coef, w.d6t);
+}
+
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                               int rk_step, double dts, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
@@ -1246,6 +1311,7 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const size_t F = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;
   const unsigned colgrid = (unsigned)((g.nx + 63) / 64) * (unsigned)g.ny;
   const unsigned kchunks = (unsigned)((g.nz + kChunk - 1) / kChunk);
+  if (w.d6u) WD_TIMED(prof, KID_DIFF6, st, (launch_diff6(g, nl, s, w, st)));
   WD_TIMED(prof, KID_RUV, st,
            (k_ruv<<<dim3(blocks_for(C, bs), kchunks), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
   WD_TIMED(prof, KID_ROM, st, (k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom)));

@@ -139,6 +139,8 @@ class Namelist:
     damp_opt: int = 3
     dampcoef: float = 0.2
     zdamp: float = 3000.0
+    diff_6th_opt: int = 0
+    diff_6th_factor: float = 0.12
     use_flux_advection: bool = True
     hypsometric_opt: int = 1
     h_sca_adv_order: int = 2
@@ -164,6 +166,8 @@ class Namelist:
             raise ValueError("w_damping must be 0 or 1")
         if self.damp_opt not in (0, 3):
             raise ValueError("damp_opt must be 0 or 3")
+        if self.diff_6th_opt not in (0, 1, 2):
+            raise ValueError("diff_6th_opt must be 0, 1 or 2")
         if self.run_physics or self.run_boundary:
             raise ValueError("physics and lateral boundaries are outside the dycore hot path")
 
