@@ -1251,11 +1251,26 @@ inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 
 // from the stage mu_total (face averages for u and v, :2768-2774).  The
 // tendency kernels add it before the large-step PGF.
 // ---------------------------------------------------------------------------
+// i + d wrapped into [0, n) for |d| <= 3 without a division (n >= 3).
+__device__ inline int wrap3(int i, int d, int n) {
+  int r = i + d;
+  r += r < 0 ? n : 0;
+  r -= r >= n ? n : 0;
+  return r;
+}
 __device__ inline double d6_axis(const double* f, int i, int n, size_t s, double coef) {
-  const double fm3 = f[(size_t)wrapi(i - 3, n) * s], fm2 = f[(size_t)wrapi(i - 2, n) * s];
-  const double fm1 = f[(size_t)wrapi(i - 1, n) * s], f0 = f[(size_t)i * s];
-  const double fp1 = f[(size_t)wrapi(i + 1, n) * s], fp2 = f[(size_t)wrapi(i + 2, n) * s];
-  const double fp3 = f[(size_t)wrapi(i + 3, n) * s];
+  int im3, im2, im1, ip1, ip2, ip3;
+  if (n >= 3) {
+    im3 = wrap3(i, -3, n), im2 = wrap3(i, -2, n), im1 = wrap3(i, -1, n);
+    ip1 = wrap3(i, 1, n), ip2 = wrap3(i, 2, n), ip3 = wrap3(i, 3, n);
+  } else {
+    im3 = wrapi(i - 3, n), im2 = wrapi(i - 2, n), im1 = wrapi(i - 1, n);
+    ip1 = wrapi(i + 1, n), ip2 = wrapi(i + 2, n), ip3 = wrapi(i + 3, n);
+  }
+  const double fm3 = f[(size_t)im3 * s], fm2 = f[(size_t)im2 * s];
+  const double fm1 = f[(size_t)im1 * s], f0 = f[(size_t)i * s];
+  const double fp1 = f[(size_t)ip1 * s], fp2 = f[(size_t)ip2 * s];
+  const double fp3 = f[(size_t)ip3 * s];
   double d0 = (10.0 * (f0 - fm1) - 5.0 * (fp1 - fm2)) + (fp2 - fm3);
   double d1 = (10.0 * (fp1 - f0) - 5.0 * (fp2 - fm1)) + (fp3 - fm2);
   const double g0 = f0 - fm1, g1 = fp1 - f0;
