@@ -73,6 +73,10 @@ typedef struct {
     int diff_6th_opt;       /* 0 off; 1 or 2: the 6th-order horizontal
                                filter, monotonic as the reference applies it */
     double diff_6th_factor; /* diff_6th_factor (reference default 0.12)   */
+    int diff_opt;           /* 0 off; 1 with km_opt 4: 2-D Smagorinsky
+                               horizontal diffusion (operational_mode.py:2946-2963) */
+    int km_opt;             /* 0 or 4                                 */
+    double c_s;             /* Smagorinsky constant (reference default 0.25) */
 } wd_namelist;
 
 /* Prognostic State (src/gpuwrf/contracts/state.py:811-821) plus the carried

@@ -46,6 +46,7 @@ class _Namelist(ctypes.Structure):
         ("top_lid", ctypes.c_int), ("w_damping", ctypes.c_int), ("damp_opt", ctypes.c_int),
         ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
+        ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
     ]
 
 
@@ -149,7 +150,8 @@ def _namelist(namelist) -> _Namelist:
     return _Namelist(float(namelist.dt_s), int(namelist.acoustic_substeps), float(namelist.epssm),
                      int(bool(namelist.top_lid)), int(namelist.w_damping), int(namelist.damp_opt),
                      float(namelist.dampcoef), float(namelist.zdamp), int(namelist.diff_6th_opt),
-                     float(namelist.diff_6th_factor))
+                     float(namelist.diff_6th_factor), int(namelist.diff_opt), int(namelist.km_opt),
+                     float(namelist.c_s))
 
 
 def _state_copy(metrics, state: dict) -> dict:
@@ -228,6 +230,28 @@ def alt_from_theta_p(theta: float, p: float) -> float:
 
 def p_from_theta_alt(theta: float, alt: float) -> float:
     return float(lib().orc_p_from_theta_alt(theta, alt))
+
+
+def smag_tendency(metrics, state: dict, c_s: float) -> tuple[dict, np.ndarray]:
+    """The coupled tendencies of the 2-D Smagorinsky horizontal diffusion
+    (diff_opt 1, km_opt 4; operational_mode.py:2946-2963) of ``state``
+    alone, as {"u", "v", "w", "theta"}, and its eddy viscosity xkmh."""
+
+    L = lib()
+    L.orc_smag_tendency.restype = ctypes.c_int
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    st = _state_copy(metrics, state)
+    s = _State(*[_ptr(st[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
+    out = {"u": np.zeros((nz, ny, nx + 1)), "v": np.zeros((nz, ny + 1, nx)), "w": np.zeros((nz + 1, ny, nx)),
+           "theta": np.zeros((nz, ny, nx))}
+    km = np.zeros((nz, ny, nx))
+    rc = L.orc_smag_tendency(ctypes.byref(g), ctypes.c_double(c_s), ctypes.byref(s), _ptr(out["u"]),
+                             _ptr(out["v"]), _ptr(out["w"]), _ptr(out["theta"]), _ptr(km))
+    if rc != 0:
+        raise RuntimeError("orc_smag_tendency failed")
+    return out, km
 
 
 def sixth_tendency(field: np.ndarray, coef: float) -> np.ndarray:

@@ -115,6 +115,10 @@ static inline ad wd_pow(ad x, double y) {
 }
 
 static inline ad wd_sin(ad x) { return ad(wd_sin(x.v), ad_push(x.id, cos(x.v), 0, 0.0)); }
+static inline ad sqrt(ad x) {
+    const double r = sqrt(x.v);
+    return ad(r, ad_push(x.id, 0.5 / r, 0, 0.0));
+}
 
 /* Adjoints of every node, seeded by the caller, swept from the last node back. */
 static inline void ad_reverse(std::vector<double> &adj) {

@@ -911,6 +911,159 @@ static void v_face_avg(int nx, int ny, const real *f, real *out) {
 }
 
 /* ------------------------------------------------------------------ */
+/* 2-D Smagorinsky horizontal diffusion (diff_opt = 1, km_opt = 4),    */
+/* as operational_mode.py:2946-2963 applies it: the eddy viscosity of  */
+/* smag2d_horizontal_km (dynamics/explicit_diffusion.py:1116-1217)     */
+/* from the deformations of horizontal_deformation_2d (:960-1113),     */
+/* called there with no map factors or slopes (so msf = 1, zx = zy =   */
+/* 0 and the slope terms vanish), then the coordinate-surface flux     */
+/* divergence _hdiff_coord_scalar (:1243-1284) of theta - 300 with     */
+/* K / prandtl and of u, v, w with K (:1287-1365).  Periodic in x and  */
+/* y over the mass extents; u and v drop their wrap face and take face */
+/* 0's tendency there.                                                 */
+/* ------------------------------------------------------------------ */
+#define SMAG_PRANDTL (1.0 / 3.0)
+
+/* D12 at the south-west corner of mass cell (j, i): du/dy + dv/dx. */
+static inline real smag_d12(const orc_grid *g, const real *u, const real *v, int k, int j, int i) {
+    const int nx = g->nx, ny = g->ny;
+    const real dudy = (u[IU(k, j, i)] - u[IU(k, wrap(j - 1, ny), i)]) / g->dy;
+    const real dvdx = (v[IV(k, j, i)] - v[IV(k, j, wrap(i - 1, nx))]) / g->dx;
+    return dudy + dvdx;
+}
+
+/* xkmh at mass cell (k, j, i): c_s^2 l^2 sqrt(def2), capped at 10 l, with
+ * l = sqrt(dx dy) (explicit_diffusion.py:1194-1205). */
+static inline real smag_km_at(const orc_grid *g, double c_s, const real *u, const real *v, int k, int j, int i) {
+    const int nx = g->nx, ny = g->ny;
+    const int ip = wrap(i + 1, nx), jp = wrap(j + 1, ny);
+    const real d11 = 2.0 * ((u[IU(k, j, ip)] - u[IU(k, j, i)]) / g->dx);
+    const real d22 = 2.0 * ((v[IV(k, jp, i)] - v[IV(k, j, i)]) / g->dy);
+    const real tmp = 0.25 * (((smag_d12(g, u, v, k, j, i) + smag_d12(g, u, v, k, jp, i)) +
+                              smag_d12(g, u, v, k, j, ip)) + smag_d12(g, u, v, k, jp, ip));
+    const real def2 = 0.25 * (d11 - d22) * (d11 - d22) + tmp * tmp;
+    const double mlen = sqrt(g->dx * g->dy);
+    const real km = c_s * c_s * mlen * mlen * sqrt(def2);
+    const double cap = 10.0 * mlen;
+    return km > cap ? (real)cap : km;
+}
+
+/* rdx (K_e m_e rdx (f_e - f) - K_w m_w rdx (f - f_w)) + the same in y when
+ * nrow > 1, at point (j, i) of one level: f, K and mass are given by their
+ * values at the centre and the four neighbours (index 0 centre, 1 east,
+ * 2 west, 3 north, 4 south). */
+static inline real hdiff_point(const real f[5], const real K[5], const real m[5], int nrow, real rdx, real rdy) {
+    const real k_e = 0.5 * (K[1] + K[0]), k_w = 0.5 * (K[0] + K[2]);
+    const real m_e = 0.5 * (m[1] + m[0]), m_w = 0.5 * (m[0] + m[2]);
+    const real p = k_e * m_e * rdx, q = k_w * m_w * rdx;
+    real t = rdx * (p * (f[1] - f[0]) - q * (f[0] - f[2]));
+    if (nrow > 1) {
+        const real k_n = 0.5 * (K[3] + K[0]), k_s = 0.5 * (K[0] + K[4]);
+        const real m_n = 0.5 * (m[3] + m[0]), m_s = 0.5 * (m[0] + m[4]);
+        const real pn = k_n * m_n * rdy, qs = k_s * m_s * rdy;
+        t = t + rdy * (pn * (f[3] - f[0]) - qs * (f[0] - f[4]));
+    }
+    return t;
+}
+
+/* K of the w level f: the mass-level K averaged to the face, the nearest
+ * level at faces 0 and nz (_xkmhd_to_w_levels, explicit_diffusion.py:1220-1240). */
+static inline real smag_k_w(const real *xkmh, size_t plane, int nz, int f, size_t c) {
+    if (f == 0) return xkmh[c];
+    if (f == nz) return xkmh[(size_t)(nz - 1) * plane + c];
+    return 0.5 * (xkmh[(size_t)f * plane + c] + xkmh[(size_t)(f - 1) * plane + c]);
+}
+
+/* The coupled tendencies of the 2-D Smagorinsky diffusion, added to ut,
+ * vt, wt and tht. */
+static void smagorinsky_hdiff(const orc_grid *g, const orc_namelist *nl, const rstate *S, real *ut, real *vt,
+                              real *wt, real *tht) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t plane = (size_t)ny * nx;
+    const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    real *K = zalloc((size_t)nz * plane);
+    real *muu = zalloc((size_t)ny * (nx + 1)), *muv = zalloc((size_t)(ny + 1) * nx);
+    u_face_avg(nx, ny, S->mu_total, muu);
+    v_face_avg(nx, ny, S->mu_total, muv);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) K[IM(k, j, i)] = smag_km_at(g, nl->c_s, S->u, S->v, k, j, i);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int ie = wrap(i + 1, nx), iw = wrap(i - 1, nx), jn = wrap(j + 1, ny), js = wrap(j - 1, ny);
+                const int jj[5] = {j, j, j, jn, js}, ii[5] = {i, ie, iw, i, i};
+                real f[5], Kp[5], m[5];
+                /* w on level k (faces), K averaged to the face, mass c1f mu + c2f */
+                for (int n = 0; n < 5; ++n) {
+                    const size_t c = (size_t)jj[n] * nx + (size_t)ii[n];
+                    f[n] = S->w[(size_t)k * plane + c];
+                    Kp[n] = smag_k_w(K, plane, nz, k, c);
+                    m[n] = g->c1f[k] * S->mu_total[c] + g->c2f[k];
+                }
+                wt[IM(k, j, i)] = wt[IM(k, j, i)] + hdiff_point(f, Kp, m, ny, rdx, rdy);
+                if (k == nz) continue;
+                /* theta - 300 with K / prandtl */
+                for (int n = 0; n < 5; ++n) {
+                    const size_t c = (size_t)jj[n] * nx + (size_t)ii[n];
+                    f[n] = S->theta[(size_t)k * plane + c] - T0;
+                    Kp[n] = K[(size_t)k * plane + c] / SMAG_PRANDTL;
+                    m[n] = g->c1h[k] * S->mu_total[c] + g->c2h[k];
+                }
+                tht[IM(k, j, i)] = tht[IM(k, j, i)] + hdiff_point(f, Kp, m, ny, rdx, rdy);
+                /* u on face i (faces 0..nx-1, periodic over nx), K of cell i */
+                for (int n = 0; n < 5; ++n) {
+                    f[n] = S->u[IU(k, jj[n], ii[n])];
+                    Kp[n] = K[IM(k, jj[n], ii[n])];
+                    m[n] = g->c1h[k] * muu[I2U(jj[n], ii[n])] + g->c2h[k];
+                }
+                const real du = hdiff_point(f, Kp, m, ny, rdx, rdy);
+                ut[IU(k, j, i)] = ut[IU(k, j, i)] + du;
+                if (i == 0) ut[IU(k, j, nx)] = ut[IU(k, j, nx)] + du;
+                /* v on face j (faces 0..ny-1, periodic over ny), K of cell j */
+                for (int n = 0; n < 5; ++n) {
+                    f[n] = S->v[IV(k, jj[n], ii[n])];
+                    Kp[n] = K[IM(k, jj[n], ii[n])];
+                    m[n] = g->c1h[k] * muv[I2V(jj[n], ii[n])] + g->c2h[k];
+                }
+                const real dv = hdiff_point(f, Kp, m, ny, rdx, rdy);
+                vt[IV(k, j, i)] = vt[IV(k, j, i)] + dv;
+                if (j == 0) vt[IV(k, ny, i)] = vt[IV(k, ny, i)] + dv;
+            }
+    free(K);
+    free(muu);
+    free(muv);
+}
+
+#ifndef ORC_AD
+int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double *ut, double *vt, double *wt,
+                      double *tht, double *xkmh) {
+    if (!g || !s || !ut || !vt || !wt || !tht || g->nx < 4 || g->ny < 1 || g->nz < 1) return -1;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t M = (size_t)nz * ny * nx;
+    memset(ut, 0, (size_t)nz * ny * (nx + 1) * sizeof(double));
+    memset(vt, 0, (size_t)nz * (ny + 1) * nx * sizeof(double));
+    memset(wt, 0, (size_t)(nz + 1) * ny * nx * sizeof(double));
+    memset(tht, 0, M * sizeof(double));
+    orc_namelist nl;
+    memset(&nl, 0, sizeof nl);
+    nl.diff_opt = 1;
+    nl.km_opt = 4;
+    nl.c_s = c_s;
+    const rstate S = {s->u, s->v, s->w, s->theta, s->p_total, s->p_pert, s->ph_total, s->ph_pert,
+                      s->mu_total, s->mu_pert, s->mudf};
+    smagorinsky_hdiff(g, &nl, &S, ut, vt, wt, tht);
+    if (xkmh)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) xkmh[IM(k, j, i)] = smag_km_at(g, c_s, s->u, s->v, k, j, i);
+    return 0;
+}
+#endif
+
+/* ------------------------------------------------------------------ */
 /* One RK stage: operational_mode.py:3331-3515 advance_stage, as its    */
 /* pieces in order: stage_setup, then nsub times (substep_uv,           */
 /* substep_mu_theta, substep_w, substep_p), then stage_finish.  Each    */
@@ -950,6 +1103,9 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
         free(muu6);
         free(muv6);
     }
+    /* 2-D Smagorinsky horizontal diffusion (operational_mode.py:2946-2963), added
+     * after the filter and before the PGF. */
+    if (nl->diff_opt == 1 && nl->km_opt == 4) smagorinsky_hdiff(g, nl, S, W->ut, W->vt, W->wt, W->tht);
     /* EOS alt of the stage-entry state, shared by _absolute_diagnostics
      * (rk_addtend_dry.py:184-186) and diagnose_pressure_al_alt's base_state=None
      * branch used by small_step_prep (acoustic_wrf.py:354-358). */

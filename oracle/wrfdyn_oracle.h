@@ -49,6 +49,9 @@ typedef struct {
     double zdamp;
     int    diff_6th_opt;     /* 0 off; 1 or 2: 6th-order filter (monotonic) */
     double diff_6th_factor;
+    int    diff_opt;    /* 0 off; 1 with km_opt 4: 2-D Smagorinsky horizontal diffusion */
+    int    km_opt;      /* 0 or 4                                   */
+    double c_s;         /* Smagorinsky constant (reference default 0.25) */
 } orc_namelist;
 
 typedef struct {
@@ -101,6 +104,11 @@ int orc_surface_w(const orc_grid *g, const double *phb0, const double *u, const 
 /* The uncoupled 6th-order filter tendency (explicit_diffusion.py:200-223)
  * of a (nlev, nrow, ncol) field, periodic in rows and columns. */
 int orc_sixth_tendency(int nlev, int nrow, int ncol, const double *f, double coef, double *out);
+/* The coupled tendencies of the 2-D Smagorinsky horizontal diffusion
+ * (diff_opt 1, km_opt 4; operational_mode.py:2946-2963) of state s alone,
+ * with its eddy viscosity xkmh (nz*ny*nx) when xkmh is not NULL. */
+int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double *ut, double *vt, double *wt,
+                      double *tht, double *xkmh);
 double orc_pow(double x, double y);
 double orc_sin(double x);
 void orc_set_threads(int n);

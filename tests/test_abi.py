@@ -41,8 +41,9 @@ def test_ctypes_structs_match_c_layout():
     #include "wrfdyn.h"
     int main(void) {
       printf("%zu %zu %zu\n", sizeof(wd_grid), sizeof(wd_namelist), sizeof(wd_state));
-      printf("%zu %zu %zu %zu\n", offsetof(wd_grid, cf1), offsetof(wd_grid, msfvy),
-             offsetof(wd_namelist, dampcoef), offsetof(wd_state, mudf));
+      printf("%zu %zu %zu %zu %zu %zu\n", offsetof(wd_grid, cf1), offsetof(wd_grid, msfvy),
+             offsetof(wd_namelist, dampcoef), offsetof(wd_state, mudf), offsetof(wd_namelist, diff_6th_factor),
+             offsetof(wd_namelist, c_s));
       return 0;
     }
     """
@@ -56,7 +57,7 @@ def test_ctypes_structs_match_c_layout():
     o = [int(x) for x in offs.split()]
     assert s == [ctypes.sizeof(_lib.WdGrid), ctypes.sizeof(_lib.WdNamelist), ctypes.sizeof(_lib.WdState)]
     assert o == [_lib.WdGrid.cf1.offset, _lib.WdGrid.msfvy.offset, _lib.WdNamelist.dampcoef.offset,
-                 _lib.WdState.mudf.offset]
+                 _lib.WdState.mudf.offset, _lib.WdNamelist.diff_6th_factor.offset, _lib.WdNamelist.c_s.offset]
 
 
 def test_create_rejects_bad_arguments_without_touching_the_device():

@@ -24,7 +24,7 @@ struct wd_handle {
   void* metric_buf = nullptr;
   void* work_buf = nullptr;
   size_t work_bytes = 0;
-  void* diff6_buf = nullptr;  // the 6th-order filter's tendencies (diff_6th_opt != 0)
+  void* diff_buf = nullptr;  // explicit-diffusion arrays (diff_6th_opt != 0 or the Smagorinsky diffusion)
   size_t metric_bytes = 0;
   int device = 0;
   int column_tile = 0;
@@ -261,6 +261,11 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   if (nl->damp_opt == 3 && nl->dampcoef > 0.0 && !(nl->zdamp > 0.0))
     return fail(nullptr, WD_EINVAL, "zdamp must be positive with damp_opt=3");
   if (nl->diff_6th_opt < 0 || nl->diff_6th_opt > 2) return fail(nullptr, WD_EINVAL, "diff_6th_opt must be 0, 1 or 2");
+  if ((nl->diff_opt != 0 && nl->diff_opt != 1) || (nl->km_opt != 0 && nl->km_opt != 4) ||
+      (nl->diff_opt == 1 && nl->km_opt != 4))
+    return fail(nullptr, WD_EINVAL,
+                "diffusion: only diff_opt=0, or diff_opt=1 with km_opt=4 (2-D Smagorinsky), is implemented");
+  if (!(nl->c_s >= 0.0)) return fail(nullptr, WD_EINVAL, "c_s must be >= 0");
 
   wd_handle* h = new (std::nothrow) wd_handle();
   if (!h) return fail(nullptr, WD_ENOMEM, "host allocation failed");
@@ -353,6 +358,8 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   h->nl.zdamp = nl->zdamp;
   h->nl.diff_6th_opt = nl->diff_6th_opt;
   h->nl.diff_6th_factor = nl->diff_6th_factor;
+  h->nl.smag = nl->diff_opt == 1 && nl->km_opt == 4;
+  h->nl.c_s = nl->c_s;
 
   // ---- workspace ----
   const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
@@ -398,24 +405,28 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return hip_fail(nullptr, e, "hipMemset(workspace)");
   }
   h->nl.mixed = 0;
-  if (nl->diff_6th_opt != 0) {
-    double** d6[] = {&w.d6u, &w.d6v, &w.d6w, &w.d6t};
-    const size_t d6n[] = {U, V, Wn, M};
-    size_t d6_total = 0;
-    for (size_t n : d6n) d6_total += (n * sizeof(double) + 255) / 256 * 256;
-    e = hipMalloc(&h->diff6_buf, d6_total);
+  if (nl->diff_6th_opt != 0 || h->nl.smag) {
+    // the explicit-diffusion tendencies: the filter's, then the Smagorinsky
+    // viscosity and tendencies, each array 256-byte aligned
+    std::vector<std::pair<double**, size_t>> dslots;
+    if (nl->diff_6th_opt != 0) dslots.insert(dslots.end(), {{&w.d6u, U}, {&w.d6v, V}, {&w.d6w, Wn}, {&w.d6t, M}});
+    if (h->nl.smag)
+      dslots.insert(dslots.end(), {{&w.xkmh, M}, {&w.smu, U}, {&w.smv, V}, {&w.smw, Wn}, {&w.smt, M}});
+    size_t d_total = 0;
+    for (const auto& d : dslots) d_total += (d.second * sizeof(double) + 255) / 256 * 256;
+    e = hipMalloc(&h->diff_buf, d_total);
     if (e != hipSuccess) {
       hipFree(h->work_buf);
       hipFree(h->metric_buf);
       delete h;
-      return fail(nullptr, WD_ENOMEM, std::string("hipMalloc(6th-order filter): ") + hipGetErrorString(e));
+      return fail(nullptr, WD_ENOMEM, std::string("hipMalloc(explicit diffusion): ") + hipGetErrorString(e));
     }
-    size_t o6 = 0;
-    for (int q = 0; q < 4; ++q) {
-      *d6[q] = reinterpret_cast<double*>(static_cast<char*>(h->diff6_buf) + o6);
-      o6 += (d6n[q] * sizeof(double) + 255) / 256 * 256;
+    size_t od = 0;
+    for (const auto& d : dslots) {
+      *d.first = reinterpret_cast<double*>(static_cast<char*>(h->diff_buf) + od);
+      od += (d.second * sizeof(double) + 255) / 256 * 256;
     }
-    h->work_bytes += d6_total;
+    h->work_bytes += d_total;
   }
   // Acoustic column kernel: the column-march pair (default) or, with
   // WRFDYN_COLUMN_TILE="TXxKT", the LDS-tiled kernel in one of its compiled
@@ -751,7 +762,7 @@ void wd_destroy(wd_handle* h) {
   if (h->graph) hipGraphExecDestroy(h->graph);
   if (h->capture_stream) hipStreamDestroy(h->capture_stream);
   if (h->work_buf) hipFree(h->work_buf);
-  if (h->diff6_buf) hipFree(h->diff6_buf);
+  if (h->diff_buf) hipFree(h->diff_buf);
   if (h->metric_buf) hipFree(h->metric_buf);
   wd::adjoint_tape_destroy(h->tape);
   if (cur != h->device) hipSetDevice(cur);
@@ -767,6 +778,7 @@ int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* st
   if (rc != WD_OK) return rc;
   if (h->nl.mixed) return fail(h, WD_EINVAL, "the adjoint is built for the FP64 form (WD_PREC_FP64) only");
   if (h->nl.diff_6th_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the 6th-order filter");
+  if (h->nl.smag) return fail(h, WD_EINVAL, "the adjoint does not cover the Smagorinsky diffusion");
   // Switched off: the first GPU run of this path ended in a memory access
   // fault whose cause is not yet found (its kernel bodies are bounds-checked
   // on the host).  It stays off the GPU until the cause is shown.

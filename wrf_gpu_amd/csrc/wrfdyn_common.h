@@ -49,6 +49,8 @@ struct DNamelist {
   double dampcoef, zdamp;
   int diff_6th_opt;        // 0 off, 1/2 the 6th-order filter (monotonic)
   double diff_6th_factor;
+  int smag;                // 1: 2-D Smagorinsky horizontal diffusion (diff_opt 1, km_opt 4)
+  double c_s;
   int mixed;  // 1: mixed-precision form (FP32 tendencies and flux interpolants), 0: plain FP64
 };
 
@@ -84,6 +86,10 @@ struct DWork {
   // the 6th-order filter's coupled tendencies of u, v, w, theta (null unless
   // diff_6th_opt != 0), added to the RK tendencies before the PGF
   double *d6u, *d6v, *d6w, *d6t;
+  // the 2-D Smagorinsky diffusion: the eddy viscosity on mass cells and the
+  // coupled tendencies of u, v, w, theta (null unless diff_opt 1, km_opt 4),
+  // added after the filter's
+  double *xkmh, *smu, *smv, *smw, *smt;
 };
 
 __host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {
@@ -126,8 +132,18 @@ __device__ inline unsigned tile_id(const DGrid& g) {
 __device__ __forceinline__ double ldo(const double* base, unsigned byte_off) {
   return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
 }
+
 __device__ __forceinline__ void sto(double* base, unsigned byte_off, double v) {
   *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// t plus the explicit-diffusion tendencies present at byte offset off: the
+// 6th-order filter's, then the Smagorinsky diffusion's, in the order the
+// reference adds them (runtime/operational_mode.py:2870-2963)
+__device__ __forceinline__ double add_diff(double t, const double* d6, const double* sm, unsigned off) {
+  if (d6) t = t + ldo(d6, off);
+  if (sm) t = t + ldo(sm, off);
+  return t;
 }
 
 // Tendency arrays are stored as T (double, or float in the mixed-precision

@@ -31,6 +31,7 @@ enum KernelId {
   KID_FIN_UV,
   KID_FIN_COL,
   KID_DIFF6,
+  KID_SMAG,
   KID_COUNT
 };
 

@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DState s, D
       t = t - g.rdnw[k] * (vf_hi - vf_lo);
       vf_lo = vf_hi;
     }
-    if (owned) stt<TT>(w.tht, o + c8, w.d6t ? t + ldo(w.d6t, o + c8) : t);
+    if (owned) stt<TT>(w.tht, o + c8, add_diff(t, w.d6t, w.smt, o + c8));
     tm1 = t0;
     t0 = tp1;
     tp1 = tp2;
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
       if (f >= 1 && f < nz) t = t + (-g.rdn[f]) * (vfl_hi - vfl_lo);
       else t = t + 0.0;
     }
-    stt<TT>(w.wt, (unsigned)f * pl8 + c8, w.d6w ? t + ldo(w.d6w, (unsigned)f * pl8 + c8) : t);
+    stt<TT>(w.wt, (unsigned)f * pl8 + c8, add_diff(t, w.d6w, w.smw, (unsigned)f * pl8 + c8));
     wm1 = w0;
     w0 = wp1;
     wp1 = wp2;
@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
     dpx = dpx + ((msf_u * g.rdx) * (cur.phpr - cur.phpl)) * bracket;
     if (owned) {
       const unsigned off = (unsigned)k * upl8 + fo8;
-      stt<TT>(w.ut, off, (w.d6u ? t + ldo(w.d6u, off) : t) + (-1.0 * dpx));
+      stt<TT>(w.ut, off, add_diff(t, w.d6u, w.smu, off) + (-1.0 * dpx));
     }
     qm1 = q0;
     q0 = qp1;
@@ -781,7 +781,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, D
     dpy = dpy + ((msf_v * g.rdy) * (cur.phpr - cur.phpl)) * bracket;
     if (owned) {
       const unsigned off = (unsigned)k * upl8 + fo8;
-      stt<TT>(w.vt, off, (w.d6v ? t + ldo(w.d6v, off) : t) + (-1.0 * dpy));
+      stt<TT>(w.vt, off, add_diff(t, w.d6v, w.smv, off) + (-1.0 * dpy));
     }
     qm1 = q0;
     q0 = qp1;
@@ -1316,6 +1316,123 @@ void launch_diff6(const DGrid& g, const DNamelist& nl, const DState& s, const DW
   k_diff6<<<blocks_for(M, bs), bs, 0, st>>>(g, s.theta, 3, nz, ny, nx, s.mu_total, coef, w.d6t);
 }
 
+// ---------------------------------------------------------------------------
+// 2-D Smagorinsky horizontal diffusion (diff_opt 1, km_opt 4;
+// runtime/operational_mode.py:2946-2963).  k_smag_km: the eddy viscosity of
+// smag2d_horizontal_km (dynamics/explicit_diffusion.py:1116-1217) from the
+// horizontal deformations of horizontal_deformation_2d (:960-1113), which the
+// reference calls with unit map factors and no slopes.  k_hdiff: the
+// coordinate-surface flux divergence (_hdiff_coord_scalar, :1243-1284) of
+// theta - 300 with K / prandtl and of u, v, w with K (:1287-1365), periodic
+// over the mass extents; the u and v wrap faces take face 0's tendency.  The
+// tendency kernels add it after the filter's, before the large-step PGF.
+// ---------------------------------------------------------------------------
+// D12 at the south-west corner of mass cell (j, i); jm, im its south and west
+// neighbours: du/dy + dv/dx.
+__device__ inline double smag_d12(const DGrid& g, const double* u, const double* v, int k, int j, int jm, int i,
+                                  int im) {
+  const double dudy = (u[iu(g, k, j, i)] - u[iu(g, k, jm, i)]) / g.dy;
+  const double dvdx = (v[iv(g, k, j, i)] - v[iv(g, k, j, im)]) / g.dx;
+  return dudy + dvdx;
+}
+
+__global__ void __launch_bounds__(256) k_smag_km(DGrid g, const double* u, const double* v, double c_s, double* xkmh) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int nx = g.nx, ny = g.ny;
+  const size_t plane = (size_t)ny * nx;
+  if (q >= (size_t)g.nz * plane) return;
+  const int k = (int)(q / plane);
+  const int j = (int)((q % plane) / nx), i = (int)(q % nx);
+  const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+  const int jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+  const double d11 = 2.0 * ((u[iu(g, k, j, ip)] - u[iu(g, k, j, i)]) / g.dx);
+  const double d22 = 2.0 * ((v[iv(g, k, jp, i)] - v[iv(g, k, j, i)]) / g.dy);
+  const double tmp = 0.25 * (((smag_d12(g, u, v, k, j, jm, i, im) + smag_d12(g, u, v, k, jp, j, i, im)) +
+                              smag_d12(g, u, v, k, j, jm, ip, i)) +
+                             smag_d12(g, u, v, k, jp, j, ip, i));
+  const double def2 = 0.25 * (d11 - d22) * (d11 - d22) + tmp * tmp;
+  const double mlen = sqrt(g.dx * g.dy);
+  const double km = c_s * c_s * mlen * mlen * sqrt(def2);
+  const double cap = 10.0 * mlen;
+  xkmh[q] = km > cap ? cap : km;
+}
+
+// K of the w level f at column c: the mass-level K averaged to the face, the
+// nearest level at faces 0 and nz (_xkmhd_to_w_levels, :1220-1240).
+__device__ inline double smag_k_w(const double* xkmh, size_t plane, int nz, int f, size_t c) {
+  if (f == 0) return xkmh[c];
+  if (f == nz) return xkmh[(size_t)(nz - 1) * plane + c];
+  return 0.5 * (xkmh[(size_t)f * plane + c] + xkmh[(size_t)(f - 1) * plane + c]);
+}
+
+// kind 0 u (nz, ny, nx+1), 1 v (nz, ny+1, nx), 2 w (nz+1, ny, nx),
+// 3 theta (nz, ny, nx); out has the field's layout.
+__global__ void __launch_bounds__(256) k_hdiff(DGrid g, const double* f, int kind, const double* xkmh,
+                                               const double* mu, double* out) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int ncol = kind == 0 ? nx + 1 : nx, nrow = kind == 1 ? ny + 1 : ny, nlev = kind == 2 ? nz + 1 : nz;
+  const size_t fplane = (size_t)nrow * ncol, plane = (size_t)ny * nx;
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (size_t)nlev * fplane) return;
+  const int k = (int)(q / fplane);
+  int j = (int)((q % fplane) / ncol), i = (int)(q % ncol);
+  // the u and v wrap faces repeat face 0
+  if (i == nx) i = 0;
+  if (j == ny) j = 0;
+  const int ie = i + 1 == nx ? 0 : i + 1, iw = i == 0 ? nx - 1 : i - 1;
+  const int jn = j + 1 == ny ? 0 : j + 1, js = j == 0 ? ny - 1 : j - 1;
+  const int jj[5] = {j, j, j, jn, js}, ii[5] = {i, ie, iw, i, i};
+  double fv[5], K[5], m[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const size_t c = (size_t)jj[n] * nx + ii[n];
+    const size_t fo = ((size_t)k * nrow + jj[n]) * ncol + ii[n];
+    if (kind == 0) {
+      const double mf = ii[n] == 0 ? mu[(size_t)jj[n] * nx] : 0.5 * (mu[c - 1] + mu[c]);
+      fv[n] = f[fo];
+      K[n] = xkmh[(size_t)k * plane + c];
+      m[n] = g.c1h[k] * mf + g.c2h[k];
+    } else if (kind == 1) {
+      const double mf = jj[n] == 0 ? mu[ii[n]] : 0.5 * (mu[c - nx] + mu[c]);
+      fv[n] = f[fo];
+      K[n] = xkmh[(size_t)k * plane + c];
+      m[n] = g.c1h[k] * mf + g.c2h[k];
+    } else if (kind == 2) {
+      fv[n] = f[fo];
+      K[n] = smag_k_w(xkmh, plane, nz, k, c);
+      m[n] = g.c1f[k] * mu[c] + g.c2f[k];
+    } else {
+      fv[n] = f[fo] - 300.0;
+      K[n] = xkmh[(size_t)k * plane + c] / (1.0 / 3.0);
+      m[n] = g.c1h[k] * mu[c] + g.c2h[k];
+    }
+  }
+  const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
+  const double k_e = 0.5 * (K[1] + K[0]), k_w = 0.5 * (K[0] + K[2]);
+  const double m_e = 0.5 * (m[1] + m[0]), m_w = 0.5 * (m[0] + m[2]);
+  const double p = k_e * m_e * rdx, r = k_w * m_w * rdx;
+  double t = rdx * (p * (fv[1] - fv[0]) - r * (fv[0] - fv[2]));
+  if (ny > 1) {
+    const double k_n = 0.5 * (K[3] + K[0]), k_s = 0.5 * (K[0] + K[4]);
+    const double m_n = 0.5 * (m[3] + m[0]), m_s = 0.5 * (m[0] + m[4]);
+    const double pn = k_n * m_n * rdy, rs = k_s * m_s * rdy;
+    t = t + rdy * (pn * (fv[3] - fv[0]) - rs * (fv[0] - fv[4]));
+  }
+  out[q] = t;
+}
+
+void launch_smag(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
+  const unsigned bs = 256;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
+  const size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx;
+  k_smag_km<<<blocks_for(M, bs), bs, 0, st>>>(g, s.u, s.v, nl.c_s, w.xkmh);
+  k_hdiff<<<blocks_for(U, bs), bs, 0, st>>>(g, s.u, 0, w.xkmh, s.mu_total, w.smu);
+  k_hdiff<<<blocks_for(V, bs), bs, 0, st>>>(g, s.v, 1, w.xkmh, s.mu_total, w.smv);
+  k_hdiff<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s.w, 2, w.xkmh, s.mu_total, w.smw);
+  k_hdiff<<<blocks_for(M, bs), bs, 0, st>>>(g, s.theta, 3, w.xkmh, s.mu_total, w.smt);
+}
+
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                               int rk_step, double dts, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
@@ -1327,6 +1444,7 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const unsigned colgrid = (unsigned)((g.nx + 63) / 64) * (unsigned)g.ny;
   const unsigned kchunks = (unsigned)((g.nz + kChunk - 1) / kChunk);
   if (w.d6u) WD_TIMED(prof, KID_DIFF6, st, (launch_diff6(g, nl, s, w, st)));
+  if (w.xkmh) WD_TIMED(prof, KID_SMAG, st, (launch_smag(g, nl, s, w, st)));
   WD_TIMED(prof, KID_RUV, st,
            (k_ruv<<<dim3(blocks_for(C, bs), kchunks), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
   WD_TIMED(prof, KID_ROM, st, (k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom)));

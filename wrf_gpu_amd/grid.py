@@ -141,6 +141,9 @@ class Namelist:
     zdamp: float = 3000.0
     diff_6th_opt: int = 0
     diff_6th_factor: float = 0.12
+    diff_opt: int = 0
+    km_opt: int = 0
+    c_s: float = 0.25
     use_flux_advection: bool = True
     hypsometric_opt: int = 1
     h_sca_adv_order: int = 2
@@ -168,6 +171,10 @@ class Namelist:
             raise ValueError("damp_opt must be 0 or 3")
         if self.diff_6th_opt not in (0, 1, 2):
             raise ValueError("diff_6th_opt must be 0, 1 or 2")
+        if self.diff_opt not in (0, 1) or self.km_opt not in (0, 4) or (self.diff_opt == 1 and self.km_opt != 4):
+            raise ValueError("diffusion: only diff_opt=0, or diff_opt=1 with km_opt=4 (2-D Smagorinsky), is implemented")
+        if not self.c_s >= 0.0:
+            raise ValueError("c_s must be >= 0")
         if self.run_physics or self.run_boundary:
             raise ValueError("physics and lateral boundaries are outside the dycore hot path")
 
