@@ -20,7 +20,8 @@ per-field error against FP64 after --form-steps large steps, 200 = one
 forecast hour), the launch-bound 64x64x40 warm bubble stepped eagerly and by
 graph replay, the 32-member uneven ensemble (members stepped one after
 another, and the batched graph in FP64 and in the mixed form), and the same
-workload with the 6th-order numerical filter on (diff_6th_opt = 2).
+workload with the 6th-order numerical filter on (diff_6th_opt = 2) and with
+the 2-D Smagorinsky horizontal diffusion on (diff_opt = 1, km_opt = 4).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
   python bench.py --pmc-summary FETCH_DIR WRITE_DIR --calibration-copy-mib 1024 > summary.json
@@ -87,6 +88,8 @@ def parse():
                     help="large steps of the mixed-precision form leg (error measured after them); 0 skips it")
     ap.add_argument("--filter-steps", type=int, default=20,
                     help="large steps of the 6th-order-filter leg on the same workload (0 skips it)")
+    ap.add_argument("--smagorinsky-steps", type=int, default=20,
+                    help="large steps of the 2-D Smagorinsky diffusion leg on the same workload (0 skips it)")
     ap.add_argument("--small-steps", type=int, default=100,
                     help="large steps of the 64x64x40 warm-bubble leg (eager and graph-replayed); 0 skips it")
     ap.add_argument("--ensemble-steps", type=int, default=3,
@@ -213,37 +216,48 @@ def mixed_form_leg(case, steps: int, fp64_ms_per_step: float, dev) -> dict:
                     "error against the oracle"}
 
 
-def filter_leg(case, steps: int, fp64_ms_per_step: float, dev) -> dict:
-    """The same workload with the 6th-order filter on (diff_6th_opt = 2):
-    throughput, and k_diff6's rate over its algorithmic bytes (it reads u, v,
-    w, theta and writes one coupled tendency for each, once per RK stage;
-    the tendency kernels then read those four back, a cost included in the
-    step time)."""
+def diffusion_leg(case, steps: int, fp64_ms_per_step: float, dev, kind: str) -> dict:
+    """The same workload with explicit diffusion on: throughput, and its
+    kernels' rate over their algorithmic bytes per RK stage (each array
+    they must read or write counted once; the tendency kernels then read the
+    diffusion tendencies back, a cost included in the step time).
+
+    kind "filter": the 6th-order filter (diff_6th_opt = 2), k_diff6, which
+    reads u, v, w, theta and writes one coupled tendency for each.
+    kind "smagorinsky": the 2-D Smagorinsky diffusion (diff_opt 1,
+    km_opt 4), k_smag: the viscosity from u and v (read 2 fields, write K),
+    then the four flux divergences (read each field and K, write its
+    tendency)."""
 
     import torch
 
     from wrf_gpu_amd.dycore import Dycore, state_to_device
 
-    m, nl = case.metrics, case.namelist.replace(diff_6th_opt=2)
+    m = case.metrics
+    nx, ny, nz = m.nx, m.ny, m.nz
+    eu, ev, ew, em = nz * ny * (nx + 1), nz * (ny + 1) * nx, (nz + 1) * ny * nx, nz * ny * nx
+    if kind == "filter":
+        nl, kernel, form = case.namelist.replace(diff_6th_opt=2), "k_diff6", "diff_6th_opt=2"
+        alg_bytes = 2 * 8 * (eu + ev + ew + em)
+    else:
+        nl, kernel, form = case.namelist.replace(diff_opt=1, km_opt=4), "k_smag", "diff_opt=1,km_opt=4"
+        alg_bytes = 8 * ((eu + ev + em) + 2 * (eu + ev + ew + em) + 4 * em)
     stream = torch.cuda.current_stream(dev)
     dyn = Dycore(m, nl)
     st = state_to_device(case.state, dev)
     ms = timed_steps(lambda n: dyn.step(st, n), steps, stream) / steps
-    dyn.profile(["k_diff6"])
+    dyn.profile([kernel])
     dyn.step(st, 2)
     torch.cuda.synchronize(dev)
-    tot_ms, launches = dyn.profile_read()["k_diff6"]
+    tot_ms, launches = dyn.profile_read()[kernel]
     dyn.close()
-    nx, ny, nz = m.nx, m.ny, m.nz
-    elems = nz * ny * (nx + 1) + nz * (ny + 1) * nx + (nz + 1) * ny * nx + nz * ny * nx
-    alg_bytes = 2 * 8 * elems  # read the four fields, write the four tendencies
     avg = tot_ms / launches
-    return {"form": "diff_6th_opt=2", "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3),
+    return {"form": form, "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3),
             "unit": "cell-substeps/s", "ms_per_step": ms, "steps": steps,
             "overhead_vs_fp64": ms / fp64_ms_per_step,
-            "k_diff6": {"avg_ms_per_stage": avg, "alg_bytes_per_stage": alg_bytes,
-                        "achieved_GBs": alg_bytes / (avg * 1e-3) / 1e9, "peak_GBs": 8000.0,
-                        "frac": alg_bytes / (avg * 1e-3) / 8e12}}
+            kernel: {"avg_ms_per_stage": avg, "alg_bytes_per_stage": alg_bytes,
+                     "achieved_GBs": alg_bytes / (avg * 1e-3) / 1e9, "peak_GBs": 8000.0,
+                     "frac": alg_bytes / (avg * 1e-3) / 8e12}}
 
 
 def small_workload_leg(steps: int, dev) -> dict:
@@ -451,8 +465,11 @@ def main():
         forms["mixed"] = mixed_form_leg(case, args.form_steps, ms_per_step, dev)
         print(f"[bench] mixed form leg done", file=sys.stderr, flush=True)
     if args.filter_steps > 0:
-        forms["filter"] = filter_leg(case, args.filter_steps, ms_per_step, dev)
+        forms["filter"] = diffusion_leg(case, args.filter_steps, ms_per_step, dev, "filter")
         print(f"[bench] filter leg done", file=sys.stderr, flush=True)
+    if args.smagorinsky_steps > 0:
+        forms["smagorinsky"] = diffusion_leg(case, args.smagorinsky_steps, ms_per_step, dev, "smagorinsky")
+        print(f"[bench] smagorinsky leg done", file=sys.stderr, flush=True)
     if args.small_steps > 0:
         forms["small_workload"] = small_workload_leg(args.small_steps, dev)
         print(f"[bench] small workload leg done", file=sys.stderr, flush=True)

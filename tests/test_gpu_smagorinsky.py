@@ -46,6 +46,7 @@ def test_smagorinsky_on_terrain_matches_oracle(oracle_lib, grid):
     assert not np.array_equal(ref["theta"], off["theta"])  # the diffusion acts
     errs = field_errors(got, ref)
     print("field rel err:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    print("bitwise equal:", {k: bool(np.array_equal(got[k], ref[k])) for k in FIELDS}, flush=True)
     for k in FIELDS:
         assert errs[k] <= 1e-12, (k, errs[k])
 

@@ -1251,26 +1251,26 @@ inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 
 // from the stage mu_total (face averages for u and v, :2768-2774).  The
 // tendency kernels add it before the large-step PGF.
 // ---------------------------------------------------------------------------
-// i + d wrapped into [0, n) for |d| <= 3 without a division (n >= 3).
-__device__ inline int wrap3(int i, int d, int n) {
-  int r = i + d;
-  r += r < 0 ? n : 0;
-  r -= r >= n ? n : 0;
-  return r;
+// The LDS-tiled stencil kernels: a 256-thread block (64 i x 4 j) covers a
+// tile of 64 columns x kTileRows rows of one level, each thread kTileRows/4
+// rows; the tile and its halo are staged in LDS with coalesced loads, each
+// value once, and every stencil reads LDS.  One grid layer per level, so no
+// thread divides to find its point.
+constexpr int kFieldBx = 64, kFieldBy = 4, kTileRows = 16;
+inline dim3 tile_grid(int ncol, int nrow, int nlev) {
+  return dim3((unsigned)((ncol + kFieldBx - 1) / kFieldBx), (unsigned)((nrow + kTileRows - 1) / kTileRows),
+              (unsigned)nlev);
 }
-__device__ inline double d6_axis(const double* f, int i, int n, size_t s, double coef) {
-  int im3, im2, im1, ip1, ip2, ip3;
-  if (n >= 3) {
-    im3 = wrap3(i, -3, n), im2 = wrap3(i, -2, n), im1 = wrap3(i, -1, n);
-    ip1 = wrap3(i, 1, n), ip2 = wrap3(i, 2, n), ip3 = wrap3(i, 3, n);
-  } else {
-    im3 = wrapi(i - 3, n), im2 = wrapi(i - 2, n), im1 = wrapi(i - 1, n);
-    ip1 = wrapi(i + 1, n), ip2 = wrapi(i + 2, n), ip3 = wrapi(i + 3, n);
-  }
-  const double fm3 = f[(size_t)im3 * s], fm2 = f[(size_t)im2 * s];
-  const double fm1 = f[(size_t)im1 * s], f0 = f[(size_t)i * s];
-  const double fp1 = f[(size_t)ip1 * s], fp2 = f[(size_t)ip2 * s];
-  const double fp3 = f[(size_t)ip3 * s];
+// i wrapped into [0, n): the common in-range case without a division.
+__device__ inline int wrapc(int i, int n) {
+  if ((unsigned)i < (unsigned)n) return i;
+  const int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// coef (dflux_p1 - dflux_p0) of the seven values f[-3..3] of a line.
+__device__ inline double d6_line(double fm3, double fm2, double fm1, double f0, double fp1, double fp2, double fp3,
+                                 double coef) {
   double d0 = (10.0 * (f0 - fm1) - 5.0 * (fp1 - fm2)) + (fp2 - fm3);
   double d1 = (10.0 * (fp1 - f0) - 5.0 * (fp2 - fm1)) + (fp3 - fm2);
   const double g0 = f0 - fm1, g1 = fp1 - f0;
@@ -1279,41 +1279,56 @@ __device__ inline double d6_axis(const double* f, int i, int n, size_t s, double
   return coef * (d1 - d0);
 }
 
+constexpr int kD6Halo = 3, kD6W = kFieldBx + 2 * kD6Halo, kD6H = kTileRows + 2 * kD6Halo;
+
 __global__ void __launch_bounds__(256) k_diff6(DGrid g, const double* f, int kind, int nlev, int nrow, int ncol,
                                                const double* mu, double coef, double* out) {
-  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-  const size_t plane = (size_t)nrow * ncol;
-  if (q >= (size_t)nlev * plane) return;
-  const int k = (int)(q / plane);
-  const int j = (int)((q % plane) / ncol), i = (int)(q % ncol);
-  const int nx = g.nx, ny = g.ny;
-  double m;
-  if (kind == 0) {
-    m = i == 0 ? mu[(size_t)j * nx] : (i == nx ? mu[(size_t)j * nx + nx - 1]
-                                              : 0.5 * (mu[(size_t)j * nx + i - 1] + mu[(size_t)j * nx + i]));
-  } else if (kind == 1) {
-    m = j == 0 ? mu[i] : (j == ny ? mu[(size_t)(ny - 1) * nx + i]
-                                  : 0.5 * (mu[(size_t)(j - 1) * nx + i] + mu[(size_t)j * nx + i]));
-  } else {
-    m = mu[(size_t)j * nx + i];
+  __shared__ double tile[kD6H][kD6W];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = blockIdx.x * kFieldBx, j0 = blockIdx.y * kTileRows, k = blockIdx.z;
+  const double* lev = f + (size_t)k * nrow * ncol;
+  for (int e = ty * kFieldBx + tx; e < kD6H * kD6W; e += kFieldBx * kFieldBy) {
+    const int r = e / kD6W, c = e - r * kD6W;
+    tile[r][c] = lev[(size_t)wrapc(j0 - kD6Halo + r, nrow) * ncol + wrapc(i0 - kD6Halo + c, ncol)];
   }
-  const double mass = kind == 2 ? g.c1f[k] * m + g.c2f[k] : g.c1h[k] * m + g.c2h[k];
-  const double* row = f + ((size_t)k * nrow + j) * ncol;
-  double t = d6_axis(row, i, ncol, 1, coef);
-  if (nrow > 1) t = t + d6_axis(f + (size_t)k * plane + i, j, nrow, (size_t)ncol, coef);
-  out[q] = mass * t;
+  __syncthreads();
+  const int i = i0 + tx;
+  if (i >= ncol) return;
+  const int nx = g.nx, ny = g.ny;
+  const int c = tx + kD6Halo;
+  for (int r = ty; r < kTileRows; r += kFieldBy) {
+    const int j = j0 + r;
+    if (j >= nrow) break;
+    double m;
+    if (kind == 0) {
+      m = i == 0 ? mu[(size_t)j * nx] : (i == nx ? mu[(size_t)j * nx + nx - 1]
+                                                : 0.5 * (mu[(size_t)j * nx + i - 1] + mu[(size_t)j * nx + i]));
+    } else if (kind == 1) {
+      m = j == 0 ? mu[i] : (j == ny ? mu[(size_t)(ny - 1) * nx + i]
+                                    : 0.5 * (mu[(size_t)(j - 1) * nx + i] + mu[(size_t)j * nx + i]));
+    } else {
+      m = mu[(size_t)j * nx + i];
+    }
+    const double mass = kind == 2 ? g.c1f[k] * m + g.c2f[k] : g.c1h[k] * m + g.c2h[k];
+    const double* row = tile[r + kD6Halo];
+    double t = d6_line(row[c - 3], row[c - 2], row[c - 1], row[c], row[c + 1], row[c + 2], row[c + 3], coef);
+    if (nrow > 1) {
+      const int q = r + kD6Halo;
+      t = t + d6_line(tile[q - 3][c], tile[q - 2][c], tile[q - 1][c], tile[q][c], tile[q + 1][c], tile[q + 2][c],
+                      tile[q + 3][c], coef);
+    }
+    out[((size_t)k * nrow + j) * ncol + i] = mass * t;
+  }
 }
 
 void launch_diff6(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
-  const unsigned bs = 256;
+  const dim3 bs(kFieldBx, kFieldBy);
   const double coef = nl.diff_6th_factor * 0.015625 / (2.0 * nl.dt);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
-  const size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx;
-  k_diff6<<<blocks_for(U, bs), bs, 0, st>>>(g, s.u, 0, nz, ny, nx + 1, s.mu_total, coef, w.d6u);
-  k_diff6<<<blocks_for(V, bs), bs, 0, st>>>(g, s.v, 1, nz, ny + 1, nx, s.mu_total, coef, w.d6v);
-  k_diff6<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s.w, 2, nz + 1, ny, nx, s.mu_total, coef, w.d6w);
-  k_diff6<<<blocks_for(M, bs), bs, 0, st>>>(g, s.theta, 3, nz, ny, nx, s.mu_total, coef, w.d6t);
+  k_diff6<<<tile_grid(nx + 1, ny, nz), bs, 0, st>>>(g, s.u, 0, nz, ny, nx + 1, s.mu_total, coef, w.d6u);
+  k_diff6<<<tile_grid(nx, ny + 1, nz), bs, 0, st>>>(g, s.v, 1, nz, ny + 1, nx, s.mu_total, coef, w.d6v);
+  k_diff6<<<tile_grid(nx, ny, nz + 1), bs, 0, st>>>(g, s.w, 2, nz + 1, ny, nx, s.mu_total, coef, w.d6w);
+  k_diff6<<<tile_grid(nx, ny, nz), bs, 0, st>>>(g, s.theta, 3, nz, ny, nx, s.mu_total, coef, w.d6t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1327,34 +1342,49 @@ void launch_diff6(const DGrid& g, const DNamelist& nl, const DState& s, const DW
 // over the mass extents; the u and v wrap faces take face 0's tendency.  The
 // tendency kernels add it after the filter's, before the large-step PGF.
 // ---------------------------------------------------------------------------
-// D12 at the south-west corner of mass cell (j, i); jm, im its south and west
-// neighbours: du/dy + dv/dx.
-__device__ inline double smag_d12(const DGrid& g, const double* u, const double* v, int k, int j, int jm, int i,
-                                  int im) {
-  const double dudy = (u[iu(g, k, j, i)] - u[iu(g, k, jm, i)]) / g.dy;
-  const double dvdx = (v[iv(g, k, j, i)] - v[iv(g, k, j, im)]) / g.dx;
-  return dudy + dvdx;
-}
+constexpr int kHdW = kFieldBx + 2, kHdH = kTileRows + 2;  // the tiles with a 1-point halo
 
+// xkmh of the tile's cells from u and v staged with a 1-point halo (u, v
+// periodic over the mass extents; tile (q, c) is cell (j0 + q - 1, i0 + c - 1)).
 __global__ void __launch_bounds__(256) k_smag_km(DGrid g, const double* u, const double* v, double c_s, double* xkmh) {
-  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  __shared__ double tu[kHdH][kHdW], tv[kHdH][kHdW];
+  __shared__ double td12[kHdH][kHdW];  // D12 at the corners (q, c), q, c >= 1
   const int nx = g.nx, ny = g.ny;
-  const size_t plane = (size_t)ny * nx;
-  if (q >= (size_t)g.nz * plane) return;
-  const int k = (int)(q / plane);
-  const int j = (int)((q % plane) / nx), i = (int)(q % nx);
-  const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
-  const int jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
-  const double d11 = 2.0 * ((u[iu(g, k, j, ip)] - u[iu(g, k, j, i)]) / g.dx);
-  const double d22 = 2.0 * ((v[iv(g, k, jp, i)] - v[iv(g, k, j, i)]) / g.dy);
-  const double tmp = 0.25 * (((smag_d12(g, u, v, k, j, jm, i, im) + smag_d12(g, u, v, k, jp, j, i, im)) +
-                              smag_d12(g, u, v, k, j, jm, ip, i)) +
-                             smag_d12(g, u, v, k, jp, j, ip, i));
-  const double def2 = 0.25 * (d11 - d22) * (d11 - d22) + tmp * tmp;
-  const double mlen = sqrt(g.dx * g.dy);
-  const double km = c_s * c_s * mlen * mlen * sqrt(def2);
-  const double cap = 10.0 * mlen;
-  xkmh[q] = km > cap ? cap : km;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = blockIdx.x * kFieldBx, j0 = blockIdx.y * kTileRows, k = blockIdx.z;
+  for (int e = ty * kFieldBx + tx; e < kHdH * kHdW; e += kFieldBx * kFieldBy) {
+    const int r = e / kHdW, c = e - r * kHdW;
+    const int jj = wrapc(j0 - 1 + r, ny), ii = wrapc(i0 - 1 + c, nx);
+    tu[r][c] = u[iu(g, k, jj, ii)];
+    tv[r][c] = v[iv(g, k, jj, ii)];
+  }
+  __syncthreads();
+  const double dx = g.dx, dy = g.dy;
+  // D12 at the south-west corner of tile cell (q, c): du/dy + dv/dx, once
+  // per corner the tile's cells use
+  for (int e = ty * kFieldBx + tx; e < (kHdH - 1) * (kHdW - 1); e += kFieldBx * kFieldBy) {
+    const int q = 1 + e / (kHdW - 1), cc = 1 + e % (kHdW - 1);
+    const double dudy = (tu[q][cc] - tu[q - 1][cc]) / dy;
+    const double dvdx = (tv[q][cc] - tv[q][cc - 1]) / dx;
+    td12[q][cc] = dudy + dvdx;
+  }
+  __syncthreads();
+  const int i = i0 + tx;
+  if (i >= nx) return;
+  const int c = tx + 1;
+  for (int r = ty; r < kTileRows; r += kFieldBy) {
+    const int j = j0 + r;
+    if (j >= ny) break;
+    const int q = r + 1;
+    const double d11 = 2.0 * ((tu[q][c + 1] - tu[q][c]) / dx);
+    const double d22 = 2.0 * ((tv[q + 1][c] - tv[q][c]) / dy);
+    const double tmp = 0.25 * (((td12[q][c] + td12[q + 1][c]) + td12[q][c + 1]) + td12[q + 1][c + 1]);
+    const double def2 = 0.25 * (d11 - d22) * (d11 - d22) + tmp * tmp;
+    const double mlen = sqrt(dx * dy);
+    const double km = c_s * c_s * mlen * mlen * sqrt(def2);
+    const double cap = 10.0 * mlen;
+    xkmh[((size_t)k * ny + j) * nx + i] = km > cap ? cap : km;
+  }
 }
 
 // K of the w level f at column c: the mass-level K averaged to the face, the
@@ -1366,71 +1396,80 @@ __device__ inline double smag_k_w(const double* xkmh, size_t plane, int nz, int 
 }
 
 // kind 0 u (nz, ny, nx+1), 1 v (nz, ny+1, nx), 2 w (nz+1, ny, nx),
-// 3 theta (nz, ny, nx); out has the field's layout.
+// 3 theta (nz, ny, nx); out has the field's layout.  The field (theta - 300
+// for theta), its K and its mass are staged per tile with a 1-point halo,
+// periodic over the mass extents; the u and v wrap faces take face 0's value.
 __global__ void __launch_bounds__(256) k_hdiff(DGrid g, const double* f, int kind, const double* xkmh,
                                                const double* mu, double* out) {
+  __shared__ double tf[kHdH][kHdW], tk[kHdH][kHdW], tm[kHdH][kHdW];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int ncol = kind == 0 ? nx + 1 : nx, nrow = kind == 1 ? ny + 1 : ny, nlev = kind == 2 ? nz + 1 : nz;
-  const size_t fplane = (size_t)nrow * ncol, plane = (size_t)ny * nx;
-  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= (size_t)nlev * fplane) return;
-  const int k = (int)(q / fplane);
-  int j = (int)((q % fplane) / ncol), i = (int)(q % ncol);
-  // the u and v wrap faces repeat face 0
-  if (i == nx) i = 0;
-  if (j == ny) j = 0;
-  const int ie = i + 1 == nx ? 0 : i + 1, iw = i == 0 ? nx - 1 : i - 1;
-  const int jn = j + 1 == ny ? 0 : j + 1, js = j == 0 ? ny - 1 : j - 1;
-  const int jj[5] = {j, j, j, jn, js}, ii[5] = {i, ie, iw, i, i};
-  double fv[5], K[5], m[5];
-#pragma unroll
-  for (int n = 0; n < 5; ++n) {
-    const size_t c = (size_t)jj[n] * nx + ii[n];
-    const size_t fo = ((size_t)k * nrow + jj[n]) * ncol + ii[n];
+  const int ncol = kind == 0 ? nx + 1 : nx, nrow = kind == 1 ? ny + 1 : ny;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = blockIdx.x * kFieldBx, j0 = blockIdx.y * kTileRows, k = blockIdx.z;
+  const size_t plane = (size_t)ny * nx;
+  const double* lev = f + (size_t)k * nrow * ncol;
+  const double c1 = kind == 2 ? g.c1f[k] : g.c1h[k], c2 = kind == 2 ? g.c2f[k] : g.c2h[k];
+  for (int e = ty * kFieldBx + tx; e < kHdH * kHdW; e += kFieldBx * kFieldBy) {
+    const int r = e / kHdW, c = e - r * kHdW;
+    const int jj = wrapc(j0 - 1 + r, ny), ii = wrapc(i0 - 1 + c, nx);
+    const size_t cc = (size_t)jj * nx + ii;
+    double fv, K, mf;
     if (kind == 0) {
-      const double mf = ii[n] == 0 ? mu[(size_t)jj[n] * nx] : 0.5 * (mu[c - 1] + mu[c]);
-      fv[n] = f[fo];
-      K[n] = xkmh[(size_t)k * plane + c];
-      m[n] = g.c1h[k] * mf + g.c2h[k];
+      fv = lev[(size_t)jj * ncol + ii];
+      K = xkmh[(size_t)k * plane + cc];
+      mf = ii == 0 ? mu[(size_t)jj * nx] : 0.5 * (mu[cc - 1] + mu[cc]);
     } else if (kind == 1) {
-      const double mf = jj[n] == 0 ? mu[ii[n]] : 0.5 * (mu[c - nx] + mu[c]);
-      fv[n] = f[fo];
-      K[n] = xkmh[(size_t)k * plane + c];
-      m[n] = g.c1h[k] * mf + g.c2h[k];
+      fv = lev[cc];
+      K = xkmh[(size_t)k * plane + cc];
+      mf = jj == 0 ? mu[ii] : 0.5 * (mu[cc - nx] + mu[cc]);
     } else if (kind == 2) {
-      fv[n] = f[fo];
-      K[n] = smag_k_w(xkmh, plane, nz, k, c);
-      m[n] = g.c1f[k] * mu[c] + g.c2f[k];
+      fv = lev[cc];
+      K = smag_k_w(xkmh, plane, nz, k, cc);
+      mf = mu[cc];
     } else {
-      fv[n] = f[fo] - 300.0;
-      K[n] = xkmh[(size_t)k * plane + c] / (1.0 / 3.0);
-      m[n] = g.c1h[k] * mu[c] + g.c2h[k];
+      fv = lev[cc] - 300.0;
+      K = xkmh[(size_t)k * plane + cc] / (1.0 / 3.0);
+      mf = mu[cc];
     }
+    tf[r][c] = fv;
+    tk[r][c] = K;
+    tm[r][c] = c1 * mf + c2;
   }
+  __syncthreads();
+  const int i = i0 + tx;
+  if (i >= nx) return;
   const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
-  const double k_e = 0.5 * (K[1] + K[0]), k_w = 0.5 * (K[0] + K[2]);
-  const double m_e = 0.5 * (m[1] + m[0]), m_w = 0.5 * (m[0] + m[2]);
-  const double p = k_e * m_e * rdx, r = k_w * m_w * rdx;
-  double t = rdx * (p * (fv[1] - fv[0]) - r * (fv[0] - fv[2]));
-  if (ny > 1) {
-    const double k_n = 0.5 * (K[3] + K[0]), k_s = 0.5 * (K[0] + K[4]);
-    const double m_n = 0.5 * (m[3] + m[0]), m_s = 0.5 * (m[0] + m[4]);
-    const double pn = k_n * m_n * rdy, rs = k_s * m_s * rdy;
-    t = t + rdy * (pn * (fv[3] - fv[0]) - rs * (fv[0] - fv[4]));
+  const int c = tx + 1;
+  for (int r = ty; r < kTileRows; r += kFieldBy) {
+    const int j = j0 + r;
+    if (j >= ny) break;
+    const int q = r + 1;
+    const double k_e = 0.5 * (tk[q][c + 1] + tk[q][c]), k_w = 0.5 * (tk[q][c] + tk[q][c - 1]);
+    const double m_e = 0.5 * (tm[q][c + 1] + tm[q][c]), m_w = 0.5 * (tm[q][c] + tm[q][c - 1]);
+    const double p = k_e * m_e * rdx, pw = k_w * m_w * rdx;
+    const double f0 = tf[q][c];
+    double t = rdx * (p * (tf[q][c + 1] - f0) - pw * (f0 - tf[q][c - 1]));
+    if (ny > 1) {
+      const double k_n = 0.5 * (tk[q + 1][c] + tk[q][c]), k_s = 0.5 * (tk[q][c] + tk[q - 1][c]);
+      const double m_n = 0.5 * (tm[q + 1][c] + tm[q][c]), m_s = 0.5 * (tm[q][c] + tm[q - 1][c]);
+      const double pn = k_n * m_n * rdy, ps = k_s * m_s * rdy;
+      t = t + rdy * (pn * (tf[q + 1][c] - f0) - ps * (f0 - tf[q - 1][c]));
+    }
+    double* o = out + (size_t)k * nrow * ncol;
+    o[(size_t)j * ncol + i] = t;
+    if (kind == 0 && i == 0) o[(size_t)j * ncol + nx] = t;
+    if (kind == 1 && j == 0) o[(size_t)ny * ncol + i] = t;
   }
-  out[q] = t;
 }
 
 void launch_smag(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
-  const unsigned bs = 256;
+  const dim3 bs(kFieldBx, kFieldBy);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
-  const size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx;
-  k_smag_km<<<blocks_for(M, bs), bs, 0, st>>>(g, s.u, s.v, nl.c_s, w.xkmh);
-  k_hdiff<<<blocks_for(U, bs), bs, 0, st>>>(g, s.u, 0, w.xkmh, s.mu_total, w.smu);
-  k_hdiff<<<blocks_for(V, bs), bs, 0, st>>>(g, s.v, 1, w.xkmh, s.mu_total, w.smv);
-  k_hdiff<<<blocks_for(Wn, bs), bs, 0, st>>>(g, s.w, 2, w.xkmh, s.mu_total, w.smw);
-  k_hdiff<<<blocks_for(M, bs), bs, 0, st>>>(g, s.theta, 3, w.xkmh, s.mu_total, w.smt);
+  k_smag_km<<<tile_grid(nx, ny, nz), bs, 0, st>>>(g, s.u, s.v, nl.c_s, w.xkmh);
+  k_hdiff<<<tile_grid(nx, ny, nz), bs, 0, st>>>(g, s.u, 0, w.xkmh, s.mu_total, w.smu);
+  k_hdiff<<<tile_grid(nx, ny, nz), bs, 0, st>>>(g, s.v, 1, w.xkmh, s.mu_total, w.smv);
+  k_hdiff<<<tile_grid(nx, ny, nz + 1), bs, 0, st>>>(g, s.w, 2, w.xkmh, s.mu_total, w.smw);
+  k_hdiff<<<tile_grid(nx, ny, nz), bs, 0, st>>>(g, s.theta, 3, w.xkmh, s.mu_total, w.smt);
 }
 
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
