@@ -20,8 +20,10 @@ per-field error against FP64 after --form-steps large steps, 200 = one
 forecast hour), the launch-bound 64x64x40 warm bubble stepped eagerly and by
 graph replay, the 32-member uneven ensemble (members stepped one after
 another, and the batched graph in FP64 and in the mixed form), and the same
-workload with the 6th-order numerical filter on (diff_6th_opt = 2) and with
-the 2-D Smagorinsky horizontal diffusion on (diff_opt = 1, km_opt = 4).
+workload with the 6th-order numerical filter on (diff_6th_opt = 2), with
+the 2-D Smagorinsky horizontal diffusion on (diff_opt = 1, km_opt = 4) and
+with the monotonic limiter on theta's final-stage advection
+(scalar_adv_opt = 2).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
   python bench.py --pmc-summary FETCH_DIR WRITE_DIR --calibration-copy-mib 1024 > summary.json
@@ -90,6 +92,8 @@ def parse():
                     help="large steps of the 6th-order-filter leg on the same workload (0 skips it)")
     ap.add_argument("--smagorinsky-steps", type=int, default=20,
                     help="large steps of the 2-D Smagorinsky diffusion leg on the same workload (0 skips it)")
+    ap.add_argument("--limiter-steps", type=int, default=20,
+                    help="large steps of the monotonic theta-limiter leg on the same workload (0 skips it)")
     ap.add_argument("--small-steps", type=int, default=100,
                     help="large steps of the 64x64x40 warm-bubble leg (eager and graph-replayed); 0 skips it")
     ap.add_argument("--ensemble-steps", type=int, default=3,
@@ -227,6 +231,12 @@ def diffusion_leg(case, steps: int, fp64_ms_per_step: float, dev, kind: str) -> 
     kind "smagorinsky": the 2-D Smagorinsky diffusion (diff_opt 1,
     km_opt 4), k_smag: the viscosity from u and v (read 2 fields, write K),
     then the four flux divergences (read each field and K, write its
+    tendency).
+    kind "limiter": the monotonic limiter on theta's advection in the final
+    RK stage (scalar_adv_opt 2), k_limiter, once per large step in place of
+    that stage's plain theta advection: the scales (read theta, the origin
+    theta, ru, rv, rom; write the inflow and outflow scales), then the
+    limited divergence (read the same five and the two scales; write the
     tendency)."""
 
     import torch
@@ -239,9 +249,12 @@ def diffusion_leg(case, steps: int, fp64_ms_per_step: float, dev, kind: str) -> 
     if kind == "filter":
         nl, kernel, form = case.namelist.replace(diff_6th_opt=2), "k_diff6", "diff_6th_opt=2"
         alg_bytes = 2 * 8 * (eu + ev + ew + em)
-    else:
+    elif kind == "smagorinsky":
         nl, kernel, form = case.namelist.replace(diff_opt=1, km_opt=4), "k_smag", "diff_opt=1,km_opt=4"
         alg_bytes = 8 * ((eu + ev + em) + 2 * (eu + ev + ew + em) + 4 * em)
+    else:
+        nl, kernel, form = case.namelist.replace(scalar_adv_opt=2), "k_limiter", "scalar_adv_opt=2"
+        alg_bytes = 8 * ((4 * em + ew + 2 * em) + (4 * em + ew + 2 * em + em))
     stream = torch.cuda.current_stream(dev)
     dyn = Dycore(m, nl)
     st = state_to_device(case.state, dev)
@@ -470,6 +483,9 @@ def main():
     if args.smagorinsky_steps > 0:
         forms["smagorinsky"] = diffusion_leg(case, args.smagorinsky_steps, ms_per_step, dev, "smagorinsky")
         print(f"[bench] smagorinsky leg done", file=sys.stderr, flush=True)
+    if args.limiter_steps > 0:
+        forms["limiter"] = diffusion_leg(case, args.limiter_steps, ms_per_step, dev, "limiter")
+        print(f"[bench] limiter leg done", file=sys.stderr, flush=True)
     if args.small_steps > 0:
         forms["small_workload"] = small_workload_leg(args.small_steps, dev)
         print(f"[bench] small workload leg done", file=sys.stderr, flush=True)

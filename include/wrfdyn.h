@@ -77,6 +77,9 @@ typedef struct {
                                horizontal diffusion (operational_mode.py:2946-2963) */
     int km_opt;             /* 0 or 4                                 */
     double c_s;             /* Smagorinsky constant (reference default 0.25) */
+    int scalar_adv_opt;     /* theta advection on the final RK stage: 0 plain
+                               h5/v3, 1 positive-definite, 2 monotonic limiter
+                               (operational_mode.py:2819-2866) */
 } wd_namelist;
 
 /* Prognostic State (src/gpuwrf/contracts/state.py:811-821) plus the carried

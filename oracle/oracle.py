@@ -47,6 +47,7 @@ class _Namelist(ctypes.Structure):
         ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
         ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
+        ("scalar_adv_opt", ctypes.c_int),
     ]
 
 
@@ -151,7 +152,7 @@ def _namelist(namelist) -> _Namelist:
                      int(bool(namelist.top_lid)), int(namelist.w_damping), int(namelist.damp_opt),
                      float(namelist.dampcoef), float(namelist.zdamp), int(namelist.diff_6th_opt),
                      float(namelist.diff_6th_factor), int(namelist.diff_opt), int(namelist.km_opt),
-                     float(namelist.c_s))
+                     float(namelist.c_s), int(namelist.scalar_adv_opt))
 
 
 def _state_copy(metrics, state: dict) -> dict:
@@ -230,6 +231,23 @@ def alt_from_theta_p(theta: float, p: float) -> float:
 
 def p_from_theta_alt(theta: float, alt: float) -> float:
     return float(lib().orc_p_from_theta_alt(theta, alt))
+
+
+def advect_limited(metrics, opt: int, u, v, mu_total, q, qold, mu_old, dt: float) -> np.ndarray:
+    """The coupled tendency of q (nz, ny, nx) with the positive-definite
+    (opt 1) or monotonic (opt 2) limiter (flux_advection.py:736-975)."""
+
+    L = lib()
+    L.orc_advect_limited.restype = ctypes.c_int
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, mu_total, q, qold, mu_old)]
+    out = np.zeros((metrics.nz, metrics.ny, metrics.nx))
+    rc = L.orc_advect_limited(ctypes.byref(g), ctypes.c_int(opt), *[_ptr(a) for a in arrs], ctypes.c_double(dt),
+                              _ptr(out))
+    if rc != 0:
+        raise RuntimeError("orc_advect_limited failed")
+    return out
 
 
 def smag_tendency(metrics, state: dict, c_s: float) -> tuple[dict, np.ndarray]:

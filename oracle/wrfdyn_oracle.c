@@ -788,6 +788,9 @@ typedef struct {
     real *dvdxi, *mudf_new, *ww_new, *wsolve, *rhs;
     /* finish */
     real *tmp_ph_total;
+    /* step-origin mu_total for the limited theta advection; outside the
+     * array table (NULL in the piece API, which refuses scalar_adv_opt) */
+    real *mut0;
 } ws_t;
 
 /* Terrain surface w of advance_w (core/advance_w.py:505-556): the kinematic
@@ -883,7 +886,8 @@ static int ws_alloc(ws_t *w, int nx, int ny, int nz) {
         *slot[a] = zalloc(extent_len(nx, ny, nz, array_extent[a]));
         if (!*slot[a]) return -1;
     }
-    return 0;
+    w->mut0 = zalloc((size_t)ny * nx);
+    return w->mut0 ? 0 : -1;
 }
 
 static void ws_free(ws_t *w) {
@@ -891,6 +895,7 @@ static void ws_free(ws_t *w) {
     real **slot[N_ARRAYS];
     array_slots(&unused, w, slot);
     for (int a = N_STATE; a < N_ARRAYS; ++a) free(*slot[a]);
+    free(w->mut0);
 }
 
 /* _u_face_average_2d / _v_face_average_2d, core/small_step_prep.py:77-88:
@@ -1064,6 +1069,181 @@ int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double 
 #endif
 
 /* ------------------------------------------------------------------ */
+/* Positive-definite (scalar_adv_opt 1) and monotonic (2) limited       */
+/* scalar advection, advect_scalar_flux_limited                         */
+/* (dynamics/flux_advection.py:614-975), which the reference applies to */
+/* theta - 300 on the final RK stage (operational_mode.py:2819-2866):   */
+/* donor-cell fluxes from the step-origin field and the antidiffusive  */
+/* remainder of the h5/v3 fluxes, scaled per cell so that the explicit */
+/* update by dt stays non-negative (1) or within the origin field's    */
+/* local extrema (2).  Periodic in x and y, rigid top and bottom.      */
+/* ------------------------------------------------------------------ */
+#define LIM_EPS 1.0e-20
+
+static inline real lim_pos(real a) { return a < 0.0 ? (real)0.0 : a; }
+static inline real lim_neg(real a) { return a > 0.0 ? (real)0.0 : a; }
+static inline real lim_min(real a, real b) { return b < a ? b : a; }
+static inline real lim_max(real a, real b) { return b > a ? b : a; }
+
+/* Donor-cell flux at the face between q_m (upstream side for cr > 0) and
+ * q_p, mass mu_face, spacing dxy (flux_upwind, flux_advection.py:614-640). */
+static inline real lim_upwind(real q_m, real q_p, real vel, real mu_face, double dxy, double dt) {
+    const real cr = vel * dt / dxy / mu_face;
+    const real acr = fabs(cr);
+    const real w_m = 0.5 * lim_min(1.0, cr + acr);
+    const real w_p = 0.5 * lim_max(-1.0, cr - acr);
+    return mu_face * (dxy / dt) * (w_m * q_m + w_p * q_p);
+}
+
+/* The scale of a cell's outgoing (PD) or incoming/outgoing (monotonic)
+ * antidiffusive flux: 1 unless the flux exceeds the room r. */
+static inline real lim_scale(real flux, real room) {
+    return flux > room ? lim_max(0.0, room / (flux + LIM_EPS)) : (real)1.0;
+}
+
+/* A horizontal face's limited antidiffusive flux a: m the cell on the face's
+ * minus side, p the plus side.  PD: the donor cell's scale; monotonic: the
+ * smaller of the receiving cell's inflow and the donor's outflow scale. */
+static inline real lim_face(int opt, real a, const real *sa, const real *sb, size_t m, size_t p) {
+    if (opt == 1) return a > 0.0 ? sa[m] * a : (a < 0.0 ? sa[p] * a : a);
+    return a > 0.0 ? lim_min(sa[p], sb[m]) * a : lim_min(sb[p], sa[m]) * a;
+}
+/* The same at z face f of column (j, i): the mass coordinate decreases with
+ * k, so a negative flux drains the cell below (f - 1) and a positive one the
+ * cell above (f), each clamped to the column. */
+static inline real lim_zface(int opt, int nz, real a, const real *sa, const real *sb, int f, int j, int i, int nx,
+                             int ny) {
+    const size_t above = IM(f < nz - 1 ? f : nz - 1, j, i), below = IM(f > 0 ? f - 1 : 0, j, i);
+    if (opt == 1) return a < 0.0 ? sa[below] * a : (a > 0.0 ? sa[above] * a : a);
+    return a < 0.0 ? lim_min(sa[above], sb[below]) * a : lim_min(sb[above], sa[below]) * a;
+}
+
+/* tend = the coupled tendency of q (theta - 300 here) with the limiter opt
+ * (1 or 2); qold the step-origin field, mut the stage mu_total, mu_old the
+ * step-origin mu_total, dt the full step. */
+static void advect_scalar_limited(const orc_grid *g, int opt, const real *q, const real *qold, const real *ru,
+                                  const real *rv, const real *rom, const real *mut, const real *mu_old, double dt,
+                                  real *tend) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
+    const double rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
+    const double dxl = 1.0 / rdx, dyl = 1.0 / rdy;
+    real *fxl = zalloc(M), *fx = zalloc(M), *fyl = zalloc(M), *fy = zalloc(M);
+    real *fzl = zalloc(Wn), *fz = zalloc(Wn), *sa = zalloc(M), *sb = zalloc(M);
+    /* face fluxes: x face i and y face j of cell (k, j, i) */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const size_t c = IM(k, j, i);
+                const int im = wrap(i - 1, nx), jm = wrap(j - 1, ny);
+                const real muu = 0.5 * (mut[I2(j, i)] + mut[I2(j, im)]);
+                const real muv = 0.5 * (mut[I2(j, i)] + mut[I2(jm, i)]);
+                const real mu_u = g->c1h[k] * muu + g->c2h[k], mu_v = g->c1h[k] * muv + g->c2h[k];
+#define QX(s) q[IM(k, j, wrap(i + (s), nx))]
+#define QY(s) q[IM(k, wrap(j + (s), ny), i)]
+                fxl[c] = lim_upwind(qold[IM(k, j, im)], qold[c], ru[c], mu_u, dxl, dt);
+                fx[c] = ru[c] * flux5(QX(-3), QX(-2), QX(-1), QX(0), QX(1), QX(2), ru[c]) - fxl[c];
+                fyl[c] = lim_upwind(qold[IM(k, jm, i)], qold[c], rv[c], mu_v, dyl, dt);
+                fy[c] = rv[c] * flux5(QY(-3), QY(-2), QY(-1), QY(0), QY(1), QY(2), rv[c]) - fyl[c];
+#undef QX
+#undef QY
+            }
+    /* z faces 1..nz-1 (0 and nz carry no flux) */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            for (int f = 1; f < nz; ++f) {
+                const size_t c = I2(j, i), o = (size_t)f * ny * nx + c;
+                const real mh = g->c1h[f] * mut[c] + g->c2h[f], mh1 = g->c1h[f - 1] * mut[c] + g->c2h[f - 1];
+                const real dz = 2.0 / (g->rdnw[f] + g->rdnw[f - 1]);
+                const real mu_face = 0.5 * (mh + mh1);
+                const real r = rom[o];
+                const real cr = r * dt / dz / mu_face;
+                const real acr = fabs(cr);
+                const real w_m = 0.5 * lim_min(1.0, cr + acr), w_p = 0.5 * lim_max(-1.0, cr - acr);
+                fzl[o] = mu_face * (dz / dt) * (w_m * qold[IM(f - 1, j, i)] + w_p * qold[IM(f, j, i)]);
+                real hi;
+                if (f == 1)
+                    hi = r * (g->fnm[1] * q[IM(1, j, i)] + g->fnp[1] * q[IM(0, j, i)]);
+                else if (f == nz - 1)
+                    hi = r * (g->fnm[nz - 1] * q[IM(nz - 1, j, i)] + g->fnp[nz - 1] * q[IM(nz - 2, j, i)]);
+                else
+                    hi = r * flux3(q[IM(f - 2, j, i)], q[IM(f - 1, j, i)], q[IM(f, j, i)], q[IM(f + 1, j, i)], -r);
+                fz[o] = hi - fzl[o];
+            }
+    /* per-cell scales: PD sa = scale; monotonic sa = scale_in, sb = scale_out */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const size_t c = IM(k, j, i), cx = IM(k, j, wrap(i + 1, nx)), cy = IM(k, wrap(j + 1, ny), i);
+                const size_t zl = c, zu = IM(k + 1, j, i); /* faces k and k+1 */
+                const real msf = g->msftx[I2(j, i)];
+                const real mass_old = g->c1h[k] * mu_old[I2(j, i)] + g->c2h[k];
+                const real div_low = msf * msf * (rdx * (fxl[cx] - fxl[c]) + rdy * (fyl[cy] - fyl[c])) +
+                                     msf * g->rdnw[k] * (fzl[zu] - fzl[zl]);
+                const real ph_low = mass_old * qold[c] - dt * div_low;
+                const real flux_out = dt * (msf * msf * (rdx * (lim_pos(fx[cx]) - lim_neg(fx[c])) +
+                                                         rdy * (lim_pos(fy[cy]) - lim_neg(fy[c]))) +
+                                            msf * g->rdnw[k] * (lim_neg(fz[zu]) - lim_pos(fz[zl])));
+                if (opt == 1) {
+                    sa[c] = lim_scale(flux_out, ph_low);
+                } else {
+                    real qmax = qold[c], qmin = qold[c];
+                    const real nb[6] = {qold[IM(k, j, wrap(i - 1, nx))], qold[cx], qold[IM(k, wrap(j - 1, ny), i)],
+                                        qold[cy], qold[IM(k > 0 ? k - 1 : 0, j, i)],
+                                        qold[IM(k < nz - 1 ? k + 1 : nz - 1, j, i)]};
+                    for (int n = 0; n < 6; ++n) {
+                        qmax = lim_max(qmax, nb[n]);
+                        qmin = lim_min(qmin, nb[n]);
+                    }
+                    const real flux_in = -dt * (msf * msf * (rdx * (lim_neg(fx[cx]) - lim_pos(fx[c])) +
+                                                             rdy * (lim_neg(fy[cy]) - lim_pos(fy[c]))) +
+                                                msf * g->rdnw[k] * (lim_pos(fz[zu]) - lim_neg(fz[zl])));
+                    const real ph_hi = qmax * mass_old - ph_low, ph_lo = ph_low - qmin * mass_old;
+                    sa[c] = lim_scale(flux_in, ph_hi);
+                    sb[c] = lim_scale(flux_out, ph_lo);
+                }
+            }
+    /* limited face fluxes plus the donor-cell ones, then the divergence */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int ip = wrap(i + 1, nx), jp = wrap(j + 1, ny);
+                const size_t c = IM(k, j, i);
+                /* total flux at x face ii of row (k, j), y face jj of column (k, i), z face f of column (j, i) */
+#define FXT(ii) (lim_face(opt, fx[IM(k, j, ii)], sa, sb, IM(k, j, wrap((ii) - 1, nx)), IM(k, j, ii)) + fxl[IM(k, j, ii)])
+#define FYT(jj) (lim_face(opt, fy[IM(k, jj, i)], sa, sb, IM(k, wrap((jj) - 1, ny), i), IM(k, jj, i)) + fyl[IM(k, jj, i)])
+                const real msf = g->msftx[I2(j, i)];
+                real t = -msf * rdx * (FXT(ip) - FXT(i));
+                t = t - msf * rdy * (FYT(jp) - FYT(j));
+                const real fz_lo = lim_zface(opt, nz, fz[c], sa, sb, k, j, i, nx, ny) + fzl[c];
+                const real fz_hi = lim_zface(opt, nz, fz[IM(k + 1, j, i)], sa, sb, k + 1, j, i, nx, ny) +
+                                   fzl[IM(k + 1, j, i)];
+                tend[c] = t - g->rdnw[k] * (fz_hi - fz_lo);
+#undef FXT
+#undef FYT
+            }
+    free(fxl); free(fx); free(fyl); free(fy); free(fzl); free(fz); free(sa); free(sb);
+}
+
+#ifndef ORC_AD
+int orc_advect_limited(const orc_grid *g, int opt, const double *u, const double *v, const double *mu_total,
+                       const double *q, const double *qold, const double *mu_old, double dt, double *tend) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    if (opt != 1 && opt != 2) return -1;
+    size_t m = (size_t)nz * ny * nx, w = (size_t)(nz + 1) * ny * nx;
+    real *ru = zalloc(m), *rv = zalloc(m), *rom = zalloc(w), *dv = zalloc(m);
+    couple_velocities(g, u, v, mu_total, ru, rv, rom, dv);
+    advect_scalar_limited(g, opt, q, qold, ru, rv, rom, mu_total, mu_old, dt, tend);
+    free(ru); free(rv); free(rom); free(dv);
+    return 0;
+}
+#endif
+
+/* ------------------------------------------------------------------ */
 /* One RK stage: operational_mode.py:3331-3515 advance_stage, as its    */
 /* pieces in order: stage_setup, then nsub times (substep_uv,           */
 /* substep_mu_theta, substep_w, substep_p), then stage_finish.  Each    */
@@ -1086,7 +1266,17 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
     {
         real *thp = W->th_work; /* temporary use before prep overwrites it */
         for (size_t q = 0; q < Mn; ++q) thp[q] = S->theta[q] - T0;
-        advect_scalar_flux(g, thp, W->ru, W->rv, W->rom, W->tht);
+        if (nl->scalar_adv_opt != 0 && rk_step == 3) {
+            /* the limiter on the final stage, from the step origin
+             * (operational_mode.py:2819-2851) */
+            real *tho = zalloc(Mn);
+            for (size_t q = 0; q < Mn; ++q) tho[q] = W->th0[q] - T0;
+            advect_scalar_limited(g, nl->scalar_adv_opt, thp, tho, W->ru, W->rv, W->rom, S->mu_total, W->mut0,
+                                  nl->dt, W->tht);
+            free(tho);
+        } else {
+            advect_scalar_flux(g, thp, W->ru, W->rv, W->rom, W->tht);
+        }
     }
     /* 6th-order filter on the coupled u, v, w, theta tendencies, with the
      * face masses of the stage mu_total (operational_mode.py:2768-2774,
@@ -1763,6 +1953,7 @@ static void step_origin(const orc_grid *g, const orc_namelist *nl, rstate *s, ws
     memcpy(W->th0, s->theta, M * sizeof(real));
     memcpy(W->ph0, s->ph_pert, Wn * sizeof(real));
     memcpy(W->mu0, s->mu_pert, C * sizeof(real));
+    if (W->mut0) memcpy(W->mut0, s->mu_total, C * sizeof(real));
 }
 
 /* One large step in place: the origin, then the solve_em.F cadence of
@@ -1823,6 +2014,7 @@ int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps
 int orc_piece(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,
               double *const *arrays) {
     if (g->nx < 4 || g->ny < 1 || g->nz < 4) return -2;
+    if (nl->scalar_adv_opt != 0) return -4; /* needs the step-origin mu_total */
     rstate S;
     ws_t W;
     bind_arrays(arrays, &S, &W);

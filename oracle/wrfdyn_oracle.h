@@ -52,6 +52,8 @@ typedef struct {
     int    diff_opt;    /* 0 off; 1 with km_opt 4: 2-D Smagorinsky horizontal diffusion */
     int    km_opt;      /* 0 or 4                                   */
     double c_s;         /* Smagorinsky constant (reference default 0.25) */
+    int    scalar_adv_opt; /* theta advection on the final RK stage: 0 plain,
+                              1 positive-definite, 2 monotonic limiter */
 } orc_namelist;
 
 typedef struct {
@@ -107,6 +109,11 @@ int orc_sixth_tendency(int nlev, int nrow, int ncol, const double *f, double coe
 /* The coupled tendencies of the 2-D Smagorinsky horizontal diffusion
  * (diff_opt 1, km_opt 4; operational_mode.py:2946-2963) of state s alone,
  * with its eddy viscosity xkmh (nz*ny*nx) when xkmh is not NULL. */
+/* The coupled tendency of q with the positive-definite (opt 1) or monotonic
+ * (opt 2) limiter (flux_advection.py:736-975), transport from u, v and
+ * mu_total, limits from the origin field qold and mass mu_old over dt. */
+int orc_advect_limited(const orc_grid *g, int opt, const double *u, const double *v, const double *mu_total,
+                       const double *q, const double *qold, const double *mu_old, double dt, double *tend);
 int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double *ut, double *vt, double *wt,
                       double *tht, double *xkmh);
 double orc_pow(double x, double y);

@@ -48,6 +48,7 @@ class WdNamelist(ctypes.Structure):
         ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
         ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
+        ("scalar_adv_opt", ctypes.c_int),
     ]
 
 

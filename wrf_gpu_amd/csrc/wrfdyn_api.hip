@@ -24,7 +24,7 @@ struct wd_handle {
   void* metric_buf = nullptr;
   void* work_buf = nullptr;
   size_t work_bytes = 0;
-  void* diff_buf = nullptr;  // explicit-diffusion arrays (diff_6th_opt != 0 or the Smagorinsky diffusion)
+  void* diff_buf = nullptr;  // arrays of the optional terms (explicit diffusion, the theta limiter)
   size_t metric_bytes = 0;
   int device = 0;
   int column_tile = 0;
@@ -122,6 +122,10 @@ int enqueue_steps(wd_handle* h, const wd::DState& ds, int n_large_steps, hipStre
     size_t tok = prof ? prof->begin(wd::KID_ORIGIN_COPY, s) : 0;
     for (int q = 0; q < 6; ++q) {
       e = hipMemcpyAsync(copies[q].first, copies[q].second, sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync(origin)");
+    }
+    if (w.mut0) {
+      e = hipMemcpyAsync(w.mut0, ds.mu_total, C * sizeof(double), hipMemcpyDeviceToDevice, s);
       if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync(origin)");
     }
     if (prof) prof->end(tok, s);
@@ -266,6 +270,8 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return fail(nullptr, WD_EINVAL,
                 "diffusion: only diff_opt=0, or diff_opt=1 with km_opt=4 (2-D Smagorinsky), is implemented");
   if (!(nl->c_s >= 0.0)) return fail(nullptr, WD_EINVAL, "c_s must be >= 0");
+  if (nl->scalar_adv_opt < 0 || nl->scalar_adv_opt > 2)
+    return fail(nullptr, WD_EINVAL, "scalar_adv_opt must be 0, 1 (positive-definite) or 2 (monotonic)");
 
   wd_handle* h = new (std::nothrow) wd_handle();
   if (!h) return fail(nullptr, WD_ENOMEM, "host allocation failed");
@@ -360,6 +366,7 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   h->nl.diff_6th_factor = nl->diff_6th_factor;
   h->nl.smag = nl->diff_opt == 1 && nl->km_opt == 4;
   h->nl.c_s = nl->c_s;
+  h->nl.scalar_adv_opt = nl->scalar_adv_opt;
 
   // ---- workspace ----
   const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
@@ -405,13 +412,15 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return hip_fail(nullptr, e, "hipMemset(workspace)");
   }
   h->nl.mixed = 0;
-  if (nl->diff_6th_opt != 0 || h->nl.smag) {
-    // the explicit-diffusion tendencies: the filter's, then the Smagorinsky
-    // viscosity and tendencies, each array 256-byte aligned
+  if (nl->diff_6th_opt != 0 || h->nl.smag || nl->scalar_adv_opt != 0) {
+    // the arrays of the optional terms, each 256-byte aligned: the filter's
+    // tendencies, the Smagorinsky viscosity and tendencies, the limiter's
+    // origin mass and scales
     std::vector<std::pair<double**, size_t>> dslots;
     if (nl->diff_6th_opt != 0) dslots.insert(dslots.end(), {{&w.d6u, U}, {&w.d6v, V}, {&w.d6w, Wn}, {&w.d6t, M}});
     if (h->nl.smag)
       dslots.insert(dslots.end(), {{&w.xkmh, M}, {&w.smu, U}, {&w.smv, V}, {&w.smw, Wn}, {&w.smt, M}});
+    if (nl->scalar_adv_opt != 0) dslots.insert(dslots.end(), {{&w.mut0, (size_t)ny * nx}, {&w.lim_sa, M}, {&w.lim_sb, M}});
     size_t d_total = 0;
     for (const auto& d : dslots) d_total += (d.second * sizeof(double) + 255) / 256 * 256;
     e = hipMalloc(&h->diff_buf, d_total);
@@ -419,7 +428,7 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
       hipFree(h->work_buf);
       hipFree(h->metric_buf);
       delete h;
-      return fail(nullptr, WD_ENOMEM, std::string("hipMalloc(explicit diffusion): ") + hipGetErrorString(e));
+      return fail(nullptr, WD_ENOMEM, std::string("hipMalloc(optional terms): ") + hipGetErrorString(e));
     }
     size_t od = 0;
     for (const auto& d : dslots) {
@@ -779,6 +788,7 @@ int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* st
   if (h->nl.mixed) return fail(h, WD_EINVAL, "the adjoint is built for the FP64 form (WD_PREC_FP64) only");
   if (h->nl.diff_6th_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the 6th-order filter");
   if (h->nl.smag) return fail(h, WD_EINVAL, "the adjoint does not cover the Smagorinsky diffusion");
+  if (h->nl.scalar_adv_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the theta limiter");
   // Switched off: the first GPU run of this path ended in a memory access
   // fault whose cause is not yet found (its kernel bodies are bounds-checked
   // on the host).  It stays off the GPU until the cause is shown.

@@ -51,6 +51,7 @@ struct DNamelist {
   double diff_6th_factor;
   int smag;                // 1: 2-D Smagorinsky horizontal diffusion (diff_opt 1, km_opt 4)
   double c_s;
+  int scalar_adv_opt;      // theta on the final RK stage: 0 plain, 1 positive-definite, 2 monotonic
   int mixed;  // 1: mixed-precision form (FP32 tendencies and flux interpolants), 0: plain FP64
 };
 
@@ -90,6 +91,10 @@ struct DWork {
   // coupled tendencies of u, v, w, theta (null unless diff_opt 1, km_opt 4),
   // added after the filter's
   double *xkmh, *smu, *smv, *smw, *smt;
+  // the limited theta advection (null unless scalar_adv_opt != 0): the step
+  // origin's mu_total and the per-cell scales (positive-definite: lim_sa;
+  // monotonic: inflow lim_sa, outflow lim_sb)
+  double *mut0, *lim_sa, *lim_sb;
 };
 
 __host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {

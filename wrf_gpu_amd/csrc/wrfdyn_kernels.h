@@ -32,6 +32,7 @@ enum KernelId {
   KID_FIN_COL,
   KID_DIFF6,
   KID_SMAG,
+  KID_LIMITER,
   KID_COUNT
 };
 

@@ -1472,6 +1472,187 @@ void launch_smag(const DGrid& g, const DNamelist& nl, const DState& s, const DWo
   k_hdiff<<<tile_grid(nx, ny, nz), bs, 0, st>>>(g, s.theta, 3, w.xkmh, s.mu_total, w.smt);
 }
 
+// ---------------------------------------------------------------------------
+// Limited theta advection on the final RK stage (scalar_adv_opt 1
+// positive-definite, 2 monotonic; advect_scalar_flux_limited,
+// dynamics/flux_advection.py:614-975, applied to theta - 300 at
+// runtime/operational_mode.py:2819-2866).  Donor-cell fluxes from the step
+// origin's theta and mass, the antidiffusive remainder of the h5/v3 fluxes,
+// per-cell scales (k_lim_scale), then the divergence of the limited fluxes
+// (k_lim_tend), which replaces k_tend_t on that stage.  One thread per
+// cell; each recomputes the fluxes of its six faces.
+// ---------------------------------------------------------------------------
+constexpr double kLimEps = 1.0e-20;
+__device__ inline double lim_pos(double a) { return a < 0.0 ? 0.0 : a; }
+__device__ inline double lim_neg(double a) { return a > 0.0 ? 0.0 : a; }
+__device__ inline double lim_min(double a, double b) { return b < a ? b : a; }
+__device__ inline double lim_max(double a, double b) { return b > a ? b : a; }
+__device__ inline double lim_upwind(double q_m, double q_p, double vel, double mu_face, double dxy, double dt) {
+  const double cr = vel * dt / dxy / mu_face;
+  const double acr = fabs(cr);
+  const double w_m = 0.5 * lim_min(1.0, cr + acr);
+  const double w_p = 0.5 * lim_max(-1.0, cr - acr);
+  return mu_face * (dxy / dt) * (w_m * q_m + w_p * q_p);
+}
+__device__ inline double lim_scale(double flux, double room) {
+  return flux > room ? lim_max(0.0, room / (flux + kLimEps)) : 1.0;
+}
+// Antidiffusive (a) and donor-cell (lo) flux at x face ii of row (k, j), or
+// y face jj of column (k, i).
+__device__ inline void lim_xface(const DGrid& g, const double* th, const double* th0, const double* ru,
+                                 const double* mut, int k, int j, int ii, double dxl, double dt, double& a,
+                                 double& lo) {
+  const int nx = g.nx;
+  const int iw = ii == 0 ? nx - 1 : ii - 1;
+  auto Q = [&](int d) { return th[im(g, k, j, wrapi(ii + d, nx))] - kT0; };
+  const double muu = 0.5 * (mut[i2(g, j, ii)] + mut[i2(g, j, iw)]);
+  const double mu_u = g.c1h[k] * muu + g.c2h[k];
+  const double vel = ru[im(g, k, j, ii)];
+  lo = lim_upwind(th0[im(g, k, j, iw)] - kT0, th0[im(g, k, j, ii)] - kT0, vel, mu_u, dxl, dt);
+  a = vel * flux5(Q(-3), Q(-2), Q(-1), Q(0), Q(1), Q(2), vel) - lo;
+}
+__device__ inline void lim_yface(const DGrid& g, const double* th, const double* th0, const double* rv,
+                                 const double* mut, int k, int jj, int i, double dyl, double dt, double& a,
+                                 double& lo) {
+  const int ny = g.ny;
+  const int js = jj == 0 ? ny - 1 : jj - 1;
+  auto Q = [&](int d) { return th[im(g, k, wrapi(jj + d, ny), i)] - kT0; };
+  const double muv = 0.5 * (mut[i2(g, jj, i)] + mut[i2(g, js, i)]);
+  const double mu_v = g.c1h[k] * muv + g.c2h[k];
+  const double vel = rv[im(g, k, jj, i)];
+  lo = lim_upwind(th0[im(g, k, js, i)] - kT0, th0[im(g, k, jj, i)] - kT0, vel, mu_v, dyl, dt);
+  a = vel * flux5(Q(-3), Q(-2), Q(-1), Q(0), Q(1), Q(2), vel) - lo;
+}
+// The same at z face f of column (j, i); faces 0 and nz carry no flux.
+__device__ inline void lim_zface(const DGrid& g, const double* th, const double* th0, const double* rom,
+                                 const double* mut, int f, int j, int i, double dt, double& a, double& lo) {
+  const int nz = g.nz;
+  if (f == 0 || f == nz) {
+    a = 0.0;
+    lo = 0.0;
+    return;
+  }
+  const size_t c = i2(g, j, i);
+  const double mh = g.c1h[f] * mut[c] + g.c2h[f], mh1 = g.c1h[f - 1] * mut[c] + g.c2h[f - 1];
+  const double dz = 2.0 / (g.rdnw[f] + g.rdnw[f - 1]);
+  const double mu_face = 0.5 * (mh + mh1);
+  const double r = rom[im(g, f, j, i)];
+  const double cr = r * dt / dz / mu_face;
+  const double acr = fabs(cr);
+  const double w_m = 0.5 * lim_min(1.0, cr + acr), w_p = 0.5 * lim_max(-1.0, cr - acr);
+  lo = mu_face * (dz / dt) * (w_m * (th0[im(g, f - 1, j, i)] - kT0) + w_p * (th0[im(g, f, j, i)] - kT0));
+  auto Q = [&](int kk) { return th[im(g, kk, j, i)] - kT0; };
+  double hi;
+  if (f == 1)
+    hi = r * (g.fnm[1] * Q(1) + g.fnp[1] * Q(0));
+  else if (f == nz - 1)
+    hi = r * (g.fnm[nz - 1] * Q(nz - 1) + g.fnp[nz - 1] * Q(nz - 2));
+  else
+    hi = r * flux3(Q(f - 2), Q(f - 1), Q(f), Q(f + 1), -r);
+  a = hi - lo;
+}
+struct LimFaces {
+  double ax0, lx0, ax1, lx1, ay0, ly0, ay1, ly1, az0, lz0, az1, lz1;
+};
+__device__ inline LimFaces lim_faces(const DGrid& g, const DState& s, const DWork& w, int k, int j, int i, double dt) {
+  const double dxl = 1.0 / (1.0 / g.dx), dyl = 1.0 / (1.0 / g.dy);
+  const int ip = i + 1 == g.nx ? 0 : i + 1, jp = j + 1 == g.ny ? 0 : j + 1;
+  LimFaces F;
+  lim_xface(g, s.theta, w.th0, w.ru, s.mu_total, k, j, i, dxl, dt, F.ax0, F.lx0);
+  lim_xface(g, s.theta, w.th0, w.ru, s.mu_total, k, j, ip, dxl, dt, F.ax1, F.lx1);
+  lim_yface(g, s.theta, w.th0, w.rv, s.mu_total, k, j, i, dyl, dt, F.ay0, F.ly0);
+  lim_yface(g, s.theta, w.th0, w.rv, s.mu_total, k, jp, i, dyl, dt, F.ay1, F.ly1);
+  lim_zface(g, s.theta, w.th0, w.rom, s.mu_total, k, j, i, dt, F.az0, F.lz0);
+  lim_zface(g, s.theta, w.th0, w.rom, s.mu_total, k + 1, j, i, dt, F.az1, F.lz1);
+  return F;
+}
+
+__global__ void __launch_bounds__(256) k_lim_scale(DGrid g, DState s, DWork w, int opt, double dt) {
+  const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
+  const LimFaces F = lim_faces(g, s, w, k, j, i, dt);
+  const size_t c = im(g, k, j, i), c2 = i2(g, j, i);
+  const double msf = g.msftx[c2];
+  const double mass_old = g.c1h[k] * w.mut0[c2] + g.c2h[k];
+  const double qo = w.th0[c] - kT0;
+  const double div_low =
+      msf * msf * (rdx * (F.lx1 - F.lx0) + rdy * (F.ly1 - F.ly0)) + msf * g.rdnw[k] * (F.lz1 - F.lz0);
+  const double ph_low = mass_old * qo - dt * div_low;
+  const double flux_out = dt * (msf * msf * (rdx * (lim_pos(F.ax1) - lim_neg(F.ax0)) +
+                                             rdy * (lim_pos(F.ay1) - lim_neg(F.ay0))) +
+                                msf * g.rdnw[k] * (lim_neg(F.az1) - lim_pos(F.az0)));
+  if (opt == 1) {
+    w.lim_sa[c] = lim_scale(flux_out, ph_low);
+    return;
+  }
+  const int ip = i + 1 == nx ? 0 : i + 1, iw = i == 0 ? nx - 1 : i - 1;
+  const int jp = j + 1 == ny ? 0 : j + 1, js = j == 0 ? ny - 1 : j - 1;
+  const double nb[6] = {w.th0[im(g, k, j, iw)] - kT0, w.th0[im(g, k, j, ip)] - kT0,
+                        w.th0[im(g, k, js, i)] - kT0, w.th0[im(g, k, jp, i)] - kT0,
+                        w.th0[im(g, k > 0 ? k - 1 : 0, j, i)] - kT0,
+                        w.th0[im(g, k < nz - 1 ? k + 1 : nz - 1, j, i)] - kT0};
+  double qmax = qo, qmin = qo;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    qmax = lim_max(qmax, nb[n]);
+    qmin = lim_min(qmin, nb[n]);
+  }
+  const double flux_in = -dt * (msf * msf * (rdx * (lim_neg(F.ax1) - lim_pos(F.ax0)) +
+                                             rdy * (lim_neg(F.ay1) - lim_pos(F.ay0))) +
+                                msf * g.rdnw[k] * (lim_pos(F.az1) - lim_neg(F.az0)));
+  w.lim_sa[c] = lim_scale(flux_in, qmax * mass_old - ph_low);
+  w.lim_sb[c] = lim_scale(flux_out, ph_low - qmin * mass_old);
+}
+
+// A horizontal face's limited antidiffusive flux a, m the cell on its minus
+// side and p on its plus side; at a z face, m is the cell below, p above.
+__device__ inline double lim_hface(int opt, double a, const double* sa, const double* sb, size_t m, size_t p) {
+  if (opt == 1) return a > 0.0 ? sa[m] * a : (a < 0.0 ? sa[p] * a : a);
+  return a > 0.0 ? lim_min(sa[p], sb[m]) * a : lim_min(sb[p], sa[m]) * a;
+}
+__device__ inline double lim_vface(int opt, double a, const double* sa, const double* sb, size_t below,
+                                   size_t above) {
+  if (opt == 1) return a < 0.0 ? sa[below] * a : (a > 0.0 ? sa[above] * a : a);
+  return a < 0.0 ? lim_min(sa[above], sb[below]) * a : lim_min(sb[above], sa[below]) * a;
+}
+
+template <class TT>
+__global__ void __launch_bounds__(256) k_lim_tend(DGrid g, DState s, DWork w, int opt, double dt) {
+  const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
+  const LimFaces F = lim_faces(g, s, w, k, j, i, dt);
+  const int ip = i + 1 == nx ? 0 : i + 1, iw = i == 0 ? nx - 1 : i - 1;
+  const int jp = j + 1 == ny ? 0 : j + 1, js = j == 0 ? ny - 1 : j - 1;
+  const size_t c = im(g, k, j, i);
+  const double* sa = w.lim_sa;
+  const double* sb = w.lim_sb;
+  const double fx0 = lim_hface(opt, F.ax0, sa, sb, im(g, k, j, iw), c) + F.lx0;
+  const double fx1 = lim_hface(opt, F.ax1, sa, sb, c, im(g, k, j, ip)) + F.lx1;
+  const double fy0 = lim_hface(opt, F.ay0, sa, sb, im(g, k, js, i), c) + F.ly0;
+  const double fy1 = lim_hface(opt, F.ay1, sa, sb, c, im(g, k, jp, i)) + F.ly1;
+  const double fz0 = lim_vface(opt, F.az0, sa, sb, im(g, k > 0 ? k - 1 : 0, j, i), c) + F.lz0;
+  const double fz1 = lim_vface(opt, F.az1, sa, sb, c, im(g, k + 1 < nz ? k + 1 : nz - 1, j, i)) + F.lz1;
+  const double msf = g.msftx[i2(g, j, i)];
+  double t = -msf * rdx * (fx1 - fx0);
+  t = t - msf * rdy * (fy1 - fy0);
+  t = t - g.rdnw[k] * (fz1 - fz0);
+  const unsigned off = (unsigned)c * 8u;
+  stt<TT>(w.tht, off, add_diff(t, w.d6t, w.smt, off));
+}
+
+template <class TT>
+void launch_limiter(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
+  const dim3 bs(kFieldBx, kFieldBy);
+  const dim3 grid((unsigned)((g.nx + kFieldBx - 1) / kFieldBx), (unsigned)((g.ny + kFieldBy - 1) / kFieldBy),
+                  (unsigned)g.nz);
+  k_lim_scale<<<grid, bs, 0, st>>>(g, s, w, nl.scalar_adv_opt, nl.dt);
+  k_lim_tend<TT><<<grid, bs, 0, st>>>(g, s, w, nl.scalar_adv_opt, nl.dt);
+}
+
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                               int rk_step, double dts, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
@@ -1484,6 +1665,8 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const unsigned kchunks = (unsigned)((g.nz + kChunk - 1) / kChunk);
   if (w.d6u) WD_TIMED(prof, KID_DIFF6, st, (launch_diff6(g, nl, s, w, st)));
   if (w.xkmh) WD_TIMED(prof, KID_SMAG, st, (launch_smag(g, nl, s, w, st)));
+  // theta's limited advection replaces the plain one on the final stage
+  const bool limit_theta = nl.scalar_adv_opt != 0 && rk_step == 3;
   WD_TIMED(prof, KID_RUV, st,
            (k_ruv<<<dim3(blocks_for(C, bs), kchunks), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
   WD_TIMED(prof, KID_ROM, st, (k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom)));
@@ -1499,12 +1682,18 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
     WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
-    WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<float><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
+    if (limit_theta)
+      WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<float>(g, nl, s, w, st)));
+    else
+      WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<float><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
   } else {
     WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
-    WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
+    if (limit_theta)
+      WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<double>(g, nl, s, w, st)));
+    else
+      WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
   }
   WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
   WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<dim3(blocks_for(CU, bs), kchunks), bs, 0, st>>>(g, s, w)));

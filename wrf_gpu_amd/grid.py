@@ -144,6 +144,7 @@ class Namelist:
     diff_opt: int = 0
     km_opt: int = 0
     c_s: float = 0.25
+    scalar_adv_opt: int = 0
     use_flux_advection: bool = True
     hypsometric_opt: int = 1
     h_sca_adv_order: int = 2
@@ -175,6 +176,8 @@ class Namelist:
             raise ValueError("diffusion: only diff_opt=0, or diff_opt=1 with km_opt=4 (2-D Smagorinsky), is implemented")
         if not self.c_s >= 0.0:
             raise ValueError("c_s must be >= 0")
+        if self.scalar_adv_opt not in (0, 1, 2):
+            raise ValueError("scalar_adv_opt must be 0, 1 (positive-definite) or 2 (monotonic)")
         if self.run_physics or self.run_boundary:
             raise ValueError("physics and lateral boundaries are outside the dycore hot path")
 
