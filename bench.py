@@ -233,10 +233,11 @@ def diffusion_leg(case, steps: int, fp64_ms_per_step: float, dev, kind: str) -> 
     then the four flux divergences (read each field and K, write its
     tendency).
     kind "limiter": the monotonic limiter on theta's advection in the final
-    RK stage (scalar_adv_opt 2), k_limiter, once per large step in place of
-    that stage's plain theta advection: the scales (read theta, the origin
-    theta, ru, rv, rom; write the inflow and outflow scales), then the
-    limited divergence (read the same five and the two scales; write the
+    RK stage (scalar_adv_opt 2), k_limiter (three kernels), once per large
+    step in place of that stage's plain theta advection, against the bytes
+    of the two-pass scheme: the scales (read theta, the origin theta, ru,
+    rv, rom; write the inflow and outflow scales), then the limited
+    divergence (read the same five and the two scales; write the
     tendency)."""
 
     import torch

@@ -420,7 +420,9 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     if (nl->diff_6th_opt != 0) dslots.insert(dslots.end(), {{&w.d6u, U}, {&w.d6v, V}, {&w.d6w, Wn}, {&w.d6t, M}});
     if (h->nl.smag)
       dslots.insert(dslots.end(), {{&w.xkmh, M}, {&w.smu, U}, {&w.smv, V}, {&w.smw, Wn}, {&w.smt, M}});
-    if (nl->scalar_adv_opt != 0) dslots.insert(dslots.end(), {{&w.mut0, (size_t)ny * nx}, {&w.lim_sa, M}, {&w.lim_sb, M}});
+    if (nl->scalar_adv_opt != 0)
+      dslots.insert(dslots.end(), {{&w.mut0, (size_t)ny * nx}, {&w.lim_sa, M}, {&w.lim_sb, M}, {&w.lim_ax, M},
+                                   {&w.lim_lx, M}, {&w.lim_ay, M}, {&w.lim_ly, M}, {&w.lim_az, Wn}, {&w.lim_lz, Wn}});
     size_t d_total = 0;
     for (const auto& d : dslots) d_total += (d.second * sizeof(double) + 255) / 256 * 256;
     e = hipMalloc(&h->diff_buf, d_total);

@@ -92,9 +92,12 @@ struct DWork {
   // added after the filter's
   double *xkmh, *smu, *smv, *smw, *smt;
   // the limited theta advection (null unless scalar_adv_opt != 0): the step
-  // origin's mu_total and the per-cell scales (positive-definite: lim_sa;
-  // monotonic: inflow lim_sa, outflow lim_sb)
+  // origin's mu_total, the per-cell scales (positive-definite: lim_sa;
+  // monotonic: inflow lim_sa, outflow lim_sb) and
   double *mut0, *lim_sa, *lim_sb;
+  // its face fluxes: antidiffusive (a) and donor-cell (l) at each cell's
+  // x, y and z minus face (z: nz + 1 faces)
+  double *lim_ax, *lim_lx, *lim_ay, *lim_ly, *lim_az, *lim_lz;
 };
 
 __host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {

@@ -1478,9 +1478,9 @@ void launch_smag(const DGrid& g, const DNamelist& nl, const DState& s, const DWo
 // dynamics/flux_advection.py:614-975, applied to theta - 300 at
 // runtime/operational_mode.py:2819-2866).  Donor-cell fluxes from the step
 // origin's theta and mass, the antidiffusive remainder of the h5/v3 fluxes,
-// per-cell scales (k_lim_scale), then the divergence of the limited fluxes
-// (k_lim_tend), which replaces k_tend_t on that stage.  One thread per
-// cell; each recomputes the fluxes of its six faces.
+// the face fluxes (k_lim_faces, each face once), the per-cell scales
+// (k_lim_scale), then the divergence of the limited fluxes (k_lim_tend),
+// which replaces k_tend_t on that stage.  One thread per cell.
 // ---------------------------------------------------------------------------
 constexpr double kLimEps = 1.0e-20;
 __device__ inline double lim_pos(double a) { return a < 0.0 ? 0.0 : a; }
@@ -1554,16 +1554,38 @@ __device__ inline void lim_zface(const DGrid& g, const double* th, const double*
 struct LimFaces {
   double ax0, lx0, ax1, lx1, ay0, ly0, ay1, ly1, az0, lz0, az1, lz1;
 };
-__device__ inline LimFaces lim_faces(const DGrid& g, const DState& s, const DWork& w, int k, int j, int i, double dt) {
+// Each cell's minus faces (x face i, y face j, z face k; and face nz above
+// the top cell), computed once and stored.
+__global__ void __launch_bounds__(256) k_lim_faces(DGrid g, DState s, DWork w, double dt) {
+  const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
   const double dxl = 1.0 / (1.0 / g.dx), dyl = 1.0 / (1.0 / g.dy);
+  const size_t c = im(g, k, j, i);
+  lim_xface(g, s.theta, w.th0, w.ru, s.mu_total, k, j, i, dxl, dt, w.lim_ax[c], w.lim_lx[c]);
+  lim_yface(g, s.theta, w.th0, w.rv, s.mu_total, k, j, i, dyl, dt, w.lim_ay[c], w.lim_ly[c]);
+  lim_zface(g, s.theta, w.th0, w.rom, s.mu_total, k, j, i, dt, w.lim_az[c], w.lim_lz[c]);
+  if (k == g.nz - 1) {
+    const size_t top = im(g, g.nz, j, i);
+    lim_zface(g, s.theta, w.th0, w.rom, s.mu_total, g.nz, j, i, dt, w.lim_az[top], w.lim_lz[top]);
+  }
+}
+// The six faces of cell (k, j, i) from the stored face fluxes.
+__device__ inline LimFaces lim_faces(const DGrid& g, const DWork& w, int k, int j, int i) {
   const int ip = i + 1 == g.nx ? 0 : i + 1, jp = j + 1 == g.ny ? 0 : j + 1;
+  const size_t c = im(g, k, j, i), cx = im(g, k, j, ip), cy = im(g, k, jp, i), cz = im(g, k + 1, j, i);
   LimFaces F;
-  lim_xface(g, s.theta, w.th0, w.ru, s.mu_total, k, j, i, dxl, dt, F.ax0, F.lx0);
-  lim_xface(g, s.theta, w.th0, w.ru, s.mu_total, k, j, ip, dxl, dt, F.ax1, F.lx1);
-  lim_yface(g, s.theta, w.th0, w.rv, s.mu_total, k, j, i, dyl, dt, F.ay0, F.ly0);
-  lim_yface(g, s.theta, w.th0, w.rv, s.mu_total, k, jp, i, dyl, dt, F.ay1, F.ly1);
-  lim_zface(g, s.theta, w.th0, w.rom, s.mu_total, k, j, i, dt, F.az0, F.lz0);
-  lim_zface(g, s.theta, w.th0, w.rom, s.mu_total, k + 1, j, i, dt, F.az1, F.lz1);
+  F.ax0 = w.lim_ax[c];
+  F.lx0 = w.lim_lx[c];
+  F.ax1 = w.lim_ax[cx];
+  F.lx1 = w.lim_lx[cx];
+  F.ay0 = w.lim_ay[c];
+  F.ly0 = w.lim_ly[c];
+  F.ay1 = w.lim_ay[cy];
+  F.ly1 = w.lim_ly[cy];
+  F.az0 = w.lim_az[c];
+  F.lz0 = w.lim_lz[c];
+  F.az1 = w.lim_az[cz];
+  F.lz1 = w.lim_lz[cz];
   return F;
 }
 
@@ -1572,7 +1594,7 @@ __global__ void __launch_bounds__(256) k_lim_scale(DGrid g, DState s, DWork w, i
   if (i >= g.nx || j >= g.ny) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
-  const LimFaces F = lim_faces(g, s, w, k, j, i, dt);
+  const LimFaces F = lim_faces(g, w, k, j, i);
   const size_t c = im(g, k, j, i), c2 = i2(g, j, i);
   const double msf = g.msftx[c2];
   const double mass_old = g.c1h[k] * w.mut0[c2] + g.c2h[k];
@@ -1624,7 +1646,7 @@ __global__ void __launch_bounds__(256) k_lim_tend(DGrid g, DState s, DWork w, in
   if (i >= g.nx || j >= g.ny) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
-  const LimFaces F = lim_faces(g, s, w, k, j, i, dt);
+  const LimFaces F = lim_faces(g, w, k, j, i);
   const int ip = i + 1 == nx ? 0 : i + 1, iw = i == 0 ? nx - 1 : i - 1;
   const int jp = j + 1 == ny ? 0 : j + 1, js = j == 0 ? ny - 1 : j - 1;
   const size_t c = im(g, k, j, i);
@@ -1649,6 +1671,7 @@ void launch_limiter(const DGrid& g, const DNamelist& nl, const DState& s, const 
   const dim3 bs(kFieldBx, kFieldBy);
   const dim3 grid((unsigned)((g.nx + kFieldBx - 1) / kFieldBx), (unsigned)((g.ny + kFieldBy - 1) / kFieldBy),
                   (unsigned)g.nz);
+  k_lim_faces<<<grid, bs, 0, st>>>(g, s, w, nl.dt);
   k_lim_scale<<<grid, bs, 0, st>>>(g, s, w, nl.scalar_adv_opt, nl.dt);
   k_lim_tend<TT><<<grid, bs, 0, st>>>(g, s, w, nl.scalar_adv_opt, nl.dt);
 }
