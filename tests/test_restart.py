@@ -87,6 +87,17 @@ def test_schema_changes_are_refused(case, tmp_path, what):
         restart.read_restart(bad)
 
 
+def test_numpy_scalars_in_the_metrics_and_namelist_are_written(case, tmp_path):
+    import dataclasses
+
+    m = dataclasses.replace(case.metrics, nx=np.int64(case.metrics.nx), cf1=np.float32(case.metrics.cf1))
+    nl = case.namelist.replace(acoustic_substeps=np.int32(case.namelist.acoustic_substeps))
+    p = restart.write_restart(_state(case), m, nl, np.int64(3), tmp_path / "r.npz")
+    _, m2, nl2, step = restart.read_restart(p)
+    assert step == 3 and m2.nx == case.metrics.nx and nl2.acoustic_substeps == case.namelist.acoustic_substeps
+    assert m2.cf1 == float(np.float32(case.metrics.cf1))
+
+
 def test_write_refuses_a_wrong_state(case, tmp_path):
     st = _state(case)
     st["u"] = st["u"][..., :-1]

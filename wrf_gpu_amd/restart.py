@@ -37,6 +37,14 @@ _METRIC_SCALARS = ("nx", "ny", "nz", "dx", "dy", "p_top", "cf1", "cf2", "cf3")
 _METRIC_ARRAYS = tuple(f.name for f in dataclasses.fields(Metrics) if f.name not in _METRIC_SCALARS)
 
 
+def _json_default(o: Any) -> Any:
+    """numpy scalars in the metrics or the namelist as Python numbers."""
+
+    if isinstance(o, np.generic):
+        return o.item()
+    raise TypeError(f"{type(o).__name__} is not serialisable in a restart header")
+
+
 def _host(a: Any) -> np.ndarray:
     """A float64 C-contiguous host copy of a numpy array or a torch tensor."""
 
@@ -74,7 +82,7 @@ def write_restart(state: dict, metrics: Metrics, namelist: Namelist, step_index:
         "step_index": int(step_index),
         "metadata": dict(extra_metadata) if extra_metadata else {},
     }
-    arrays["header"] = np.frombuffer(json.dumps(header).encode(), dtype=np.uint8)
+    arrays["header"] = np.frombuffer(json.dumps(header, default=_json_default).encode(), dtype=np.uint8)
     target = Path(path)
     target.parent.mkdir(parents=True, exist_ok=True)
     tmp = target.with_name(target.name + ".tmp")
