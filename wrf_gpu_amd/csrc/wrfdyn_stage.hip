@@ -1679,9 +1679,6 @@ void launch_limiter(const DGrid& g, const DNamelist& nl, const DState& s, const 
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                               int rk_step, double dts, hipStream_t st, Prof* prof) {
   const unsigned bs = 256;
-  const size_t M = (size_t)g.nz * g.ny * g.nx;
-  const size_t U = (size_t)g.nz * g.ny * (g.nx + 1);
-  const size_t V = (size_t)g.nz * (g.ny + 1) * g.nx;
   const size_t C = (size_t)g.ny * g.nx;
   const size_t F = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;
   const unsigned colgrid = (unsigned)((g.nx + 63) / 64) * (unsigned)g.ny;
