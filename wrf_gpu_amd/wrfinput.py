@@ -95,8 +95,11 @@ def read_wrfinput(path: str | Path, time_index: int = 0, device: str | None = No
     for k in STATE_FIELDS:
         if state[k].shape != shapes[k]:
             raise ValueError(f"{k} has shape {state[k].shape}, the grid needs {shapes[k]}")
-        if metrics.msftx.shape != (ny, nx) or metrics.msfux.shape != (ny, nx + 1) or metrics.msfvx.shape != (ny + 1, nx):
-            raise ValueError("the map factors do not match the grid")
+    want = {"msftx": (ny, nx), "msfty": (ny, nx), "msfux": (ny, nx + 1), "msfuy": (ny, nx + 1),
+            "msfvx": (ny + 1, nx), "msfvy": (ny + 1, nx)}
+    for k, shape in want.items():
+        if getattr(metrics, k).shape != shape:
+            raise ValueError(f"{_MAPFAC[k]} has shape {getattr(metrics, k).shape}, the grid needs {shape}")
     if device is not None:
         import torch
 
