@@ -57,6 +57,12 @@ typedef struct {
     const double *msftx, *msfty;          /* ny*nx     */
     const double *msfux, *msfuy;          /* ny*(nx+1) */
     const double *msfvx, *msfvy;          /* (ny+1)*nx */
+    /* dry-pressure coefficients p = c3 mu + c4 + p_top (DycoreMetrics
+     * c3h/c4h/c3f/c4f, contracts/grid.py:84-89), read with hypsometric_opt 2
+     * only (NULL otherwise) */
+    const double *c3h, *c4h;              /* nz   */
+    const double *c3f, *c4f;              /* nz+1 */
+    double p_top;
 } wd_grid;
 
 /* Hot-path fields of OperationalNamelist
@@ -80,6 +86,8 @@ typedef struct {
     int scalar_adv_opt;     /* theta advection on the final RK stage: 0 plain
                                h5/v3, 1 positive-definite, 2 monotonic limiter
                                (operational_mode.py:2819-2866) */
+    int hypsometric_opt;    /* 1 (or 0) linear al, 2 log-pressure al
+                               (rk_addtend_dry.py:196-222, acoustic_wrf.py:394-427) */
 } wd_namelist;
 
 /* Prognostic State (src/gpuwrf/contracts/state.py:811-821) plus the carried

@@ -1,6 +1,7 @@
 /*
  * Deterministic FP64 elementary functions for the dycore's equation of state
- * (pow) and Rayleigh damping profile (sin).
+ * (pow), the log-pressure specific volume (log) and the Rayleigh damping
+ * profile (sin).
  *
  * The same source is compiled for the GPU kernels (hipcc) and for host C code
  * (gcc), so a given input gives the same bits on both: every step is a plain
@@ -111,6 +112,13 @@ WD_MATH_FN double wd_exp_dd(double hi, double lo) {
     q = q * r + 0.5;
     double em1 = r + (r * r) * q;
     return ldexp(1.0 + em1, (int)k);
+}
+
+/* ln(x) for finite x > 0: the double-double log rounded to double */
+WD_MATH_FN double wd_log(double x) {
+    double lh, ll;
+    wd_log_dd(x, &lh, &ll);
+    return lh + ll;
 }
 
 /* x^y for finite x > 0 */

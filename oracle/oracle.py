@@ -37,6 +37,7 @@ class _Grid(ctypes.Structure):
         ("cf1", ctypes.c_double), ("cf2", ctypes.c_double), ("cf3", ctypes.c_double),
         ("msftx", _dp), ("msfty", _dp), ("msfux", _dp), ("msfuy", _dp),
         ("msfvx", _dp), ("msfvy", _dp),
+        ("c3h", _dp), ("c4h", _dp), ("c3f", _dp), ("c4f", _dp), ("p_top", ctypes.c_double),
     ]
 
 
@@ -47,7 +48,7 @@ class _Namelist(ctypes.Structure):
         ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
         ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
-        ("scalar_adv_opt", ctypes.c_int),
+        ("scalar_adv_opt", ctypes.c_int), ("hypsometric_opt", ctypes.c_int),
     ]
 
 
@@ -126,6 +127,7 @@ def _grid(metrics, keep: _Keep) -> _Grid:
         keep(m.c1h), keep(m.c2h), keep(m.c1f), keep(m.c2f), keep(m.dnw), keep(m.rdnw),
         keep(m.dn), keep(m.rdn), keep(m.fnm), keep(m.fnp), m.cf1, m.cf2, m.cf3,
         keep(m.msftx), keep(m.msfty), keep(m.msfux), keep(m.msfuy), keep(m.msfvx), keep(m.msfvy),
+        keep(m.c3h), keep(m.c4h), keep(m.c3f), keep(m.c4f), float(m.p_top),
     )
 
 
@@ -152,7 +154,7 @@ def _namelist(namelist) -> _Namelist:
                      int(bool(namelist.top_lid)), int(namelist.w_damping), int(namelist.damp_opt),
                      float(namelist.dampcoef), float(namelist.zdamp), int(namelist.diff_6th_opt),
                      float(namelist.diff_6th_factor), int(namelist.diff_opt), int(namelist.km_opt),
-                     float(namelist.c_s), int(namelist.scalar_adv_opt))
+                     float(namelist.c_s), int(namelist.scalar_adv_opt), int(namelist.hypsometric_opt))
 
 
 def _state_copy(metrics, state: dict) -> dict:

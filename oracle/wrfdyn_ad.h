@@ -115,6 +115,7 @@ static inline ad wd_pow(ad x, double y) {
 }
 
 static inline ad wd_sin(ad x) { return ad(wd_sin(x.v), ad_push(x.id, cos(x.v), 0, 0.0)); }
+static inline ad wd_log(ad x) { return ad(wd_log(x.v), ad_push(x.id, 1.0 / x.v, 0, 0.0)); }
 static inline ad sqrt(ad x) {
     const double r = sqrt(x.v);
     return ad(r, ad_push(x.id, 0.5 / r, 0, 0.0));

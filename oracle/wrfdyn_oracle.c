@@ -503,7 +503,18 @@ static void advect_w_flux(const orc_grid *g, const real *w, const real *ru, cons
 /* core/rk_addtend_dry.py:127-229 (_absolute_diagnostics, hyps opt 1)   */
 /* core/rk_addtend_dry.py:232-361 (large_step_horizontal_pgf)          */
 /* ------------------------------------------------------------------ */
-static void large_step_pgf(const orc_grid *g, const rstate *S, const real *alt,
+/* hypsometric_opt 2 (WRF calc_p_rho_phi log form, rk_addtend_dry.py:196-222,
+ * acoustic_wrf.py:394-427): the specific volume of layer k from its
+ * geopotential depth dph and the column mass: dph / phm / ln(pfd / pfu), with
+ * the face and half-level dry pressures c3 mass + c4 + p_top. */
+static inline real log_alpha(const orc_grid *g, int k, real dph, real mass) {
+    const real pfu = g->c3f[k + 1] * mass + g->c4f[k + 1] + g->p_top;
+    const real pfd = g->c3f[k] * mass + g->c4f[k] + g->p_top;
+    const real phm = g->c3h[k] * mass + g->c4h[k] + g->p_top;
+    return dph / phm / wd_log(pfd / pfu);
+}
+
+static void large_step_pgf(const orc_grid *g, int hypsometric_opt, const rstate *S, const real *alt,
                            real *al, real *php, real *ut, real *vt, real *dpn_col) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
@@ -527,12 +538,18 @@ static void large_step_pgf(const orc_grid *g, const rstate *S, const real *alt,
                 real mass_base = g->c1h[k] * mub + g->c2h[k];
                 real safe_base = safe_pos(mass_base, 1.0e-12);
                 real alb = ((-g->rdnw[k]) * (phb_k1 - phb_k)) / safe_base;
-                real mass_t = g->c1h[k] * muts + g->c2h[k];
-                real safe_t = safe_pos(mass_t, 1.0e-12);
-                real mu_term = g->c1h[k] * mu_pert;
-                al[IM(k, j, i)] =
-                    -(alb * mu_term + g->rdnw[k] * (S->ph_pert[IM(k + 1, j, i)] - S->ph_pert[IM(k, j, i)])) /
-                    safe_t;
+                if (hypsometric_opt == 2) {
+                    /* muts here is the state's mu_total (rk_addtend_dry.py:207) */
+                    real dph_tot = (phb_k1 + S->ph_pert[IM(k + 1, j, i)]) - (phb_k + S->ph_pert[IM(k, j, i)]);
+                    al[IM(k, j, i)] = log_alpha(g, k, dph_tot, mu_total) - log_alpha(g, k, phb_k1 - phb_k, mub);
+                } else {
+                    real mass_t = g->c1h[k] * muts + g->c2h[k];
+                    real safe_t = safe_pos(mass_t, 1.0e-12);
+                    real mu_term = g->c1h[k] * mu_pert;
+                    al[IM(k, j, i)] =
+                        -(alb * mu_term + g->rdnw[k] * (S->ph_pert[IM(k + 1, j, i)] - S->ph_pert[IM(k, j, i)])) /
+                        safe_t;
+                }
                 real pht_k = phb_k + S->ph_pert[IM(k, j, i)];
                 real pht_k1 = phb_k1 + S->ph_pert[IM(k + 1, j, i)];
                 php[IM(k, j, i)] = 0.5 * (pht_k + pht_k1);
@@ -1301,7 +1318,7 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
      * branch used by small_step_prep (acoustic_wrf.py:354-358). */
 #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < Mn; ++q) W->alt_raw[q] = alt_from_theta_p(S->theta[q], S->p_total[q]);
-    large_step_pgf(g, S, W->alt_raw, W->al_ls, W->php_ls, W->ut, W->vt, NULL);
+    large_step_pgf(g, nl->hypsometric_opt, S, W->alt_raw, W->al_ls, W->php_ls, W->ut, W->vt, NULL);
 
     /* --- small_step_prep_wrf, core/small_step_prep.py:216-394 --- */
     for (int j = 0; j < ny; ++j)
@@ -1913,12 +1930,20 @@ static void stage_finish(const orc_grid *g, const orc_namelist *nl, rstate *S, w
                 real denom_alb = g->dnw[k] * mass_h_base;
                 real safe_da = safe_pos(denom_alb, 1.0e-12);
                 real alb = -(phb_k1 - phb_k) / safe_da;
-                real mass_w = g->c1h[k] * muts + g->c2h[k];
-                real safe_mass = safe_pos(mass_w, 1.0e-12);
-                real mu_term = (alb * g->c1h[k]) * S->mu_pert[c];
-                real geo_term = g->rdnw[k] * (S->ph_pert[q1] - S->ph_pert[q]);
-                real al = -(mu_term + geo_term) / safe_mass;
-                real alt = al + alb;
+                real al, alt;
+                if (nl->hypsometric_opt == 2) {
+                    real dph_tot = (phb_k1 + S->ph_pert[q1]) - (phb_k + S->ph_pert[q]);
+                    real alb_used = log_alpha(g, k, phb_k1 - phb_k, mub);
+                    al = log_alpha(g, k, dph_tot, muts) - alb_used;
+                    alt = al + alb_used;
+                } else {
+                    real mass_w = g->c1h[k] * muts + g->c2h[k];
+                    real safe_mass = safe_pos(mass_w, 1.0e-12);
+                    real mu_term = (alb * g->c1h[k]) * S->mu_pert[c];
+                    real geo_term = g->rdnw[k] * (S->ph_pert[q1] - S->ph_pert[q]);
+                    al = -(mu_term + geo_term) / safe_mass;
+                    alt = al + alb;
+                }
                 real ptot = p_from_theta_alt(S->theta[q], alt);
                 real p_pert = ptot - W->pb[q];
                 real p_base = S->p_total[q] - S->p_pert[q];
@@ -1999,6 +2024,7 @@ int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     if (nx < 4 || ny < 1 || nz < 4) return -2;
     if (!nl->top_lid) return -3;
+    if (nl->hypsometric_opt == 2 && (!g->c3h || !g->c4h || !g->c3f || !g->c4f)) return -5;
     ws_t W;
     if (ws_alloc(&W, nx, ny, nz) != 0) {
         ws_free(&W);

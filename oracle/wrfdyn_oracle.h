@@ -36,6 +36,11 @@ typedef struct {
     const double *msftx, *msfty;             /* ny*nx      */
     const double *msfux, *msfuy;             /* ny*(nx+1)  */
     const double *msfvx, *msfvy;             /* (ny+1)*nx  */
+    /* dry-pressure coefficients p = c3 mu + c4 + p_top, read only with
+     * hypsometric_opt 2 (may be NULL otherwise) */
+    const double *c3h, *c4h;                 /* nz   */
+    const double *c3f, *c4f;                 /* nz+1 */
+    double p_top;
 } orc_grid;
 
 typedef struct {
@@ -54,6 +59,7 @@ typedef struct {
     double c_s;         /* Smagorinsky constant (reference default 0.25) */
     int    scalar_adv_opt; /* theta advection on the final RK stage: 0 plain,
                               1 positive-definite, 2 monotonic limiter */
+    int    hypsometric_opt; /* 1 linear al (dp = mu d eta), 2 log-pressure al */
 } orc_namelist;
 
 typedef struct {

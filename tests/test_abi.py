@@ -103,7 +103,7 @@ def test_graph_batch_autotune_and_adjoint_entry_points_reject_bad_arguments():
 def test_namelist_validation_matches_supported_path():
     nl = Namelist(dt_s=6.0, acoustic_substeps=6)
     nl.validate()
-    for bad in (dict(top_lid=False), dict(hypsometric_opt=2), dict(damp_opt=1), dict(w_damping=2),
+    for bad in (dict(top_lid=False), dict(hypsometric_opt=3), dict(damp_opt=1), dict(w_damping=2),
                 dict(acoustic_substeps=0), dict(run_physics=True), dict(h_sca_adv_order=5)):
         with pytest.raises(ValueError):
             nl.replace(**bad).validate()

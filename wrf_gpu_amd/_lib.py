@@ -38,6 +38,7 @@ class WdGrid(ctypes.Structure):
         ("dnw", _dp), ("rdnw", _dp), ("dn", _dp), ("rdn", _dp), ("fnm", _dp), ("fnp", _dp),
         ("cf1", ctypes.c_double), ("cf2", ctypes.c_double), ("cf3", ctypes.c_double),
         ("msftx", _dp), ("msfty", _dp), ("msfux", _dp), ("msfuy", _dp), ("msfvx", _dp), ("msfvy", _dp),
+        ("c3h", _dp), ("c4h", _dp), ("c3f", _dp), ("c4f", _dp), ("p_top", ctypes.c_double),
     ]
 
 
@@ -48,7 +49,7 @@ class WdNamelist(ctypes.Structure):
         ("dampcoef", ctypes.c_double), ("zdamp", ctypes.c_double),
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
         ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
-        ("scalar_adv_opt", ctypes.c_int),
+        ("scalar_adv_opt", ctypes.c_int), ("hypsometric_opt", ctypes.c_int),
     ]
 
 

@@ -1209,6 +1209,7 @@ __global__ void __launch_bounds__(256) k_fin_col(DGrid g, DState s, DWork w) {
   const double msfty = g.msfty[c];
   const double muts = w.muts[c], mut = w.mut[c];
   const double mub = w.mub[c], mu_pert_new = w.mu[c];
+  const bool hyps2 = g.hypsometric_opt == 2;
   {
     double mu_base = s.mu_total[c] - s.mu_pert[c];
     s.mu_pert[c] = mu_pert_new;
@@ -1269,12 +1270,19 @@ __global__ void __launch_bounds__(256) k_fin_col(DGrid g, DState s, DWork w) {
       double denom_alb = g.dnw[k] * mass_h_base;
       double safe_da = safe_pos(denom_alb, 1.0e-12);
       double alb = -(phb_k1 - phb_k) / safe_da;
-      double mass_w = g.c1h[k] * muts_r + g.c2h[k];
-      double safe_mass = safe_pos(mass_w, 1.0e-12);
-      double mu_term = (alb * g.c1h[k]) * mu_pert_new;
-      double geo_term = g.rdnw[k] * (ph_pert - php_prev);
-      double al = -(mu_term + geo_term) / safe_mass;
-      double alt = al + alb;
+      double alt;
+      if (hyps2) {
+        const double dph_tot = (phb_k1 + ph_pert) - (phb_k + php_prev);
+        const double alb_used = log_alpha(g, k, phb_k1 - phb_k, mub);
+        alt = (log_alpha(g, k, dph_tot, muts_r) - alb_used) + alb_used;
+      } else {
+        double mass_w = g.c1h[k] * muts_r + g.c2h[k];
+        double safe_mass = safe_pos(mass_w, 1.0e-12);
+        double mu_term = (alb * g.c1h[k]) * mu_pert_new;
+        double geo_term = g.rdnw[k] * (ph_pert - php_prev);
+        double al = -(mu_term + geo_term) / safe_mass;
+        alt = al + alb;
+      }
       double ptot = p_from_theta_alt(theta, alt);
       double p_pert_new = ptot - prev.pb;
       double p_base2 = p_total - p_pert;

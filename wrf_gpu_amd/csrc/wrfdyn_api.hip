@@ -270,6 +270,10 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return fail(nullptr, WD_EINVAL,
                 "diffusion: only diff_opt=0, or diff_opt=1 with km_opt=4 (2-D Smagorinsky), is implemented");
   if (!(nl->c_s >= 0.0)) return fail(nullptr, WD_EINVAL, "c_s must be >= 0");
+  if (nl->hypsometric_opt < 0 || nl->hypsometric_opt > 2)
+    return fail(nullptr, WD_EINVAL, "hypsometric_opt must be 1 (linear) or 2 (log-pressure)");
+  if (nl->hypsometric_opt == 2 && (!grid->c3h || !grid->c4h || !grid->c3f || !grid->c4f))
+    return fail(nullptr, WD_EINVAL, "hypsometric_opt=2 needs the c3h, c4h, c3f and c4f metrics");
   if (nl->scalar_adv_opt < 0 || nl->scalar_adv_opt > 2)
     return fail(nullptr, WD_EINVAL, "scalar_adv_opt must be 0, 1 (positive-definite) or 2 (monotonic)");
 
@@ -285,7 +289,7 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   const size_t n1 = (size_t)nz, n1f = (size_t)nz + 1;
   const size_t c = (size_t)ny * nx, cu = (size_t)ny * (nx + 1), cv = (size_t)(ny + 1) * nx;
   std::vector<double> host;
-  host.reserve(8 * n1 + 2 * n1f + 2 * c + 2 * cu + 2 * cv);
+  host.reserve(10 * n1 + 4 * n1f + 2 * c + 2 * cu + 2 * cv);
   std::vector<size_t> offs;
   auto put = [&](const double* src, size_t n) {
     offs.push_back(host.size());
@@ -307,6 +311,13 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   put(grid->msfuy, cu);
   put(grid->msfvx, cv);
   put(grid->msfvy, cv);
+  // the dry-pressure coefficients; without them (hypsometric_opt 1) the
+  // idealized c3 = 1, c4 = 0, which nothing reads
+  const std::vector<double> one_h(n1, 1.0), zero_h(n1, 0.0), one_f(n1f, 1.0), zero_f(n1f, 0.0);
+  put(grid->c3h ? grid->c3h : one_h.data(), n1);
+  put(grid->c4h ? grid->c4h : zero_h.data(), n1);
+  put(grid->c3f ? grid->c3f : one_f.data(), n1f);
+  put(grid->c4f ? grid->c4f : zero_f.data(), n1f);
   h->metric_bytes = host.size() * sizeof(double);
   e = hipMalloc(&h->metric_buf, h->metric_bytes);
   if (e != hipSuccess) {
@@ -345,6 +356,12 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   g.msfuy = mb + offs[o++];
   g.msfvx = mb + offs[o++];
   g.msfvy = mb + offs[o++];
+  g.c3h = mb + offs[o++];
+  g.c4h = mb + offs[o++];
+  g.c3f = mb + offs[o++];
+  g.c4f = mb + offs[o++];
+  g.p_top = grid->p_top;
+  g.hypsometric_opt = nl->hypsometric_opt;
   g.cf1 = grid->cf1;
   g.cf2 = grid->cf2;
   g.cf3 = grid->cf3;
@@ -791,6 +808,7 @@ int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* st
   if (h->nl.diff_6th_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the 6th-order filter");
   if (h->nl.smag) return fail(h, WD_EINVAL, "the adjoint does not cover the Smagorinsky diffusion");
   if (h->nl.scalar_adv_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the theta limiter");
+  if (h->g.hypsometric_opt == 2) return fail(h, WD_EINVAL, "the adjoint does not cover hypsometric_opt=2");
   // Switched off: the first GPU run of this path ended in a memory access
   // fault whose cause is not yet found (its kernel bodies are bounds-checked
   // on the host).  It stays off the GPU until the cause is shown.

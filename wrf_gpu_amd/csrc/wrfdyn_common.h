@@ -38,6 +38,9 @@ struct DGrid {
   const double *dnw, *rdnw, *dn, *rdn, *fnm, *fnp;
   double cf1, cf2, cf3, cfn, cfn1;
   const double *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy;
+  const double *c3h, *c4h, *c3f, *c4f;  // dry-pressure coefficients (hypsometric_opt 2)
+  double p_top;
+  int hypsometric_opt;  // 2: log-pressure al (WRF calc_p_rho_phi), else linear
   int xcd_remap;  // 1: walk tiles XCD-contiguously (xcd_tile), 0: hardware order
 };
 
@@ -196,6 +199,16 @@ __device__ inline double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 
 __device__ inline double safe_pos(double x, double floor) { return fabs(x) > floor ? x : floor; }
 
 // _inverse_density_from_theta_pressure, dynamics/acoustic_wrf.py:178-195 (qvf=1).
+// hypsometric_opt 2 (WRF calc_p_rho_phi log form, rk_addtend_dry.py:196-222,
+// acoustic_wrf.py:394-427): the specific volume of layer k from its
+// geopotential depth dph and the column mass, dph / phm / ln(pfd / pfu), with
+// the face and half-level dry pressures c3 mass + c4 + p_top.
+__device__ inline double log_alpha(const DGrid& g, int k, double dph, double mass) {
+  const double pfu = g.c3f[k + 1] * mass + g.c4f[k + 1] + g.p_top;
+  const double pfd = g.c3f[k] * mass + g.c4f[k] + g.p_top;
+  const double phm = g.c3h[k] * mass + g.c4h[k] + g.p_top;
+  return dph / phm / wd_log(pfd / pfu);
+}
 __device__ inline double alt_from_theta_p(double theta, double p) {
   double ps = p > kMinPressure ? p : kMinPressure;
   return (((kRd / kP0) * theta) * 1.0) * wd_pow(ps / kP0, kCvpm);

@@ -912,6 +912,7 @@ __global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
   const double mu_pert = s.mu_pert[c], mu_total = s.mu_total[c];
   const double mub_d = mu_total - mu_pert;
   const double muts_d = mu_total + mu_pert;
+  const bool hyps2 = g.hypsometric_opt == 2;
   struct Lv {
     double w0, wc, ph0, php, pht;   // w level f
     double th, ptot, ppert, th0;    // mass level f
@@ -957,10 +958,16 @@ __global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
         double mass_base = g.c1h[k] * mub_d + g.c2h[k];
         double safe_base = safe_pos(mass_base, 1.0e-12);
         double alb = ((-g.rdnw[k]) * (phb_k1 - phb_k)) / safe_base;
-        double mass_t = g.c1h[k] * muts_d + g.c2h[k];
-        double safe_t = safe_pos(mass_t, 1.0e-12);
-        double mu_term = g.c1h[k] * mu_pert;
-        sto(w.al_ls, qk, -(alb * mu_term + g.rdnw[k] * (cur.php - prev.php)) / safe_t);
+        if (hyps2) {
+          // the state's mu_total is the column mass here (rk_addtend_dry.py:207)
+          const double dph_tot = (phb_k1 + cur.php) - (phb_k + prev.php);
+          sto(w.al_ls, qk, log_alpha(g, k, dph_tot, mu_total) - log_alpha(g, k, phb_k1 - phb_k, mub_d));
+        } else {
+          double mass_t = g.c1h[k] * muts_d + g.c2h[k];
+          double safe_t = safe_pos(mass_t, 1.0e-12);
+          double mu_term = g.c1h[k] * mu_pert;
+          sto(w.al_ls, qk, -(alb * mu_term + g.rdnw[k] * (cur.php - prev.php)) / safe_t);
+        }
         double pht_k = phb_k + prev.php;
         double pht_k1 = phb_k1 + cur.php;
         sto(w.php_ls, qk, 0.5 * (pht_k + pht_k1));

@@ -68,13 +68,15 @@ class Dycore:
             _c_array(m.fnm, keep), _c_array(m.fnp, keep), m.cf1, m.cf2, m.cf3,
             _c_array(m.msftx, keep), _c_array(m.msfty, keep), _c_array(m.msfux, keep),
             _c_array(m.msfuy, keep), _c_array(m.msfvx, keep), _c_array(m.msfvy, keep),
+            _c_array(m.c3h, keep), _c_array(m.c4h, keep), _c_array(m.c3f, keep), _c_array(m.c4f, keep),
+            float(m.p_top),
         )
         nl = namelist
         self._nl = _lib.WdNamelist(float(nl.dt_s), int(nl.acoustic_substeps), float(nl.epssm),
                                    int(bool(nl.top_lid)), int(nl.w_damping), int(nl.damp_opt),
                                    float(nl.dampcoef), float(nl.zdamp), int(nl.diff_6th_opt),
                                    float(nl.diff_6th_factor), int(nl.diff_opt), int(nl.km_opt),
-                                   float(nl.c_s), int(nl.scalar_adv_opt))
+                                   float(nl.c_s), int(nl.scalar_adv_opt), int(nl.hypsometric_opt))
         handle = ctypes.c_void_p()
         code = lib.wd_create(ctypes.byref(self._grid), ctypes.byref(self._nl), ctypes.byref(handle))
         if code != _lib.WD_OK:

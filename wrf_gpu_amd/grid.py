@@ -48,6 +48,19 @@ class Metrics:
     msfuy: np.ndarray
     msfvx: np.ndarray
     msfvy: np.ndarray
+    # dry-pressure coefficients p_d = c3 mu + c4 + p_top (read with
+    # hypsometric_opt 2); None gives the idealized override c3 = 1, c4 = 0
+    # (ic_generators/idealized.py:430-431)
+    c3h: np.ndarray | None = None
+    c4h: np.ndarray | None = None
+    c3f: np.ndarray | None = None
+    c4f: np.ndarray | None = None
+
+    def __post_init__(self) -> None:
+        for name, n, v in (("c3h", self.nz, 1.0), ("c4h", self.nz, 0.0), ("c3f", self.nz + 1, 1.0),
+                           ("c4f", self.nz + 1, 0.0)):
+            if getattr(self, name) is None:
+                object.__setattr__(self, name, np.full(n, v, dtype=np.float64))
 
     @classmethod
     def flat(cls, *, nx: int, ny: int, nz: int, dx: float, dy: float, eta: np.ndarray,
@@ -81,11 +94,13 @@ class Metrics:
             c2h = np.zeros(nz)
             c1f = np.ones(nz + 1)
             c2f = np.zeros(nz + 1)
+            c3h, c3f = np.ones(nz), np.ones(nz + 1)
         else:
             c1h = eta_mass.copy()
             c2h = np.zeros(nz)
             c1f = eta.copy()
             c2f = np.zeros(nz + 1)
+            c3h, c3f = eta_mass.copy(), eta.copy()
         one = lambda shape: np.ones(shape, dtype=np.float64)  # noqa: E731
         return cls(
             nx=int(nx), ny=int(ny), nz=int(nz), dx=float(dx), dy=float(dy), eta=eta.copy(),
@@ -93,6 +108,7 @@ class Metrics:
             rdn=rdn, fnm=fnm, fnp=fnp, cf1=float(fnp[1] + cof1), cf2=float(fnm[1] - cof1 - cof2),
             cf3=float(cof2), msftx=one((ny, nx)), msfty=one((ny, nx)), msfux=one((ny, nx + 1)),
             msfuy=one((ny, nx + 1)), msfvx=one((ny + 1, nx)), msfvy=one((ny + 1, nx)),
+            c3h=c3h, c4h=np.zeros(nz), c3f=c3f, c4f=np.zeros(nz + 1),
         )
 
     def shapes(self) -> dict[str, tuple[int, ...]]:
@@ -162,8 +178,8 @@ class Namelist:
             raise ValueError("only top_lid=True (rigid lid) is implemented")
         if not self.use_flux_advection:
             raise ValueError("only use_flux_advection=True is implemented")
-        if self.hypsometric_opt != 1:
-            raise ValueError("only hypsometric_opt=1 is implemented")
+        if self.hypsometric_opt not in (1, 2):
+            raise ValueError("hypsometric_opt must be 1 (linear) or 2 (log-pressure)")
         if self.h_sca_adv_order > 2:
             raise ValueError("only the order-2 rhs_ph horizontal advection (h_sca_adv_order<=2) is implemented")
         if self.w_damping not in (0, 1):

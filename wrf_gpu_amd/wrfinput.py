@@ -15,7 +15,8 @@ of the file it builds the state in the reference layout
 
 The metrics come from the vertical coordinate arrays WRF writes (``ZNW``,
 ``C1H``, ``C2H``, ``C1F``, ``C2F``, ``DNW``, ``RDNW``, ``DN``, ``RDN``,
-``FNM``, ``FNP``, ``CF1``, ``CF2``, ``CF3``, ``P_TOP``), the six map factors
+``FNM``, ``FNP``, ``CF1``, ``CF2``, ``CF3``, ``P_TOP``, and ``C3H``, ``C4H``,
+``C3F``, ``C4F`` when present, for hypsometric_opt 2), the six map factors
 ``MAPFAC_{M,U,V}{X,Y}`` and the ``DX``/``DY`` global attributes, with WRF's
 signs (contracts/grid.py:64-164).  Files must be classic netCDF (CDF-1 or
 CDF-2), which ``scipy.io.netcdf_file`` reads; netCDF4/HDF5 files need
@@ -81,6 +82,14 @@ def read_wrfinput(path: str | Path, time_index: int = 0, device: str | None = No
                 raise ValueError(f"{name} has {a.shape[0]} levels, the grid has {nz} (+1 staggered)")
             vert[k] = a
         maps = {k: _var(f, name, t) for k, name in _MAPFAC.items()}
+        # the dry-pressure coefficients, for hypsometric_opt 2, when the file has them
+        for k, (name, stag) in (("c3h", ("C3H", False)), ("c4h", ("C4H", False)), ("c3f", ("C3F", True)),
+                                ("c4f", ("C4F", True))):
+            if name in f.variables:
+                a = _var(f, name, t).reshape(-1)
+                if a.shape[0] != (nz + 1 if stag else nz):
+                    raise ValueError(f"{name} has {a.shape[0]} levels, the grid has {nz} (+1 staggered)")
+                vert[k] = a
         scal = {k: float(_var(f, name, t).reshape(-1)[0]) for k, name in _SCALARS.items()}
         metrics = Metrics(nx=nx, ny=ny, nz=nz, dx=_gattr(f, "DX"), dy=_gattr(f, "DY"), **scal, **vert, **maps)
         u, v, w = _var(f, "U", t), _var(f, "V", t), _var(f, "W", t)
