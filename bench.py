@@ -94,6 +94,8 @@ def parse():
                     help="large steps of the 2-D Smagorinsky diffusion leg on the same workload (0 skips it)")
     ap.add_argument("--limiter-steps", type=int, default=20,
                     help="large steps of the monotonic theta-limiter leg on the same workload (0 skips it)")
+    ap.add_argument("--constk-steps", type=int, default=20,
+                    help="large steps of the constant-K diffusion leg on the same workload (0 skips it)")
     ap.add_argument("--small-steps", type=int, default=100,
                     help="large steps of the 64x64x40 warm-bubble leg (eager and graph-replayed); 0 skips it")
     ap.add_argument("--ensemble-steps", type=int, default=3,
@@ -238,6 +240,10 @@ def diffusion_leg(case, steps: int, fp64_ms_per_step: float, dev, kind: str) -> 
     of the two-pass scheme: the scales (read theta, the origin theta, ru,
     rv, rom; write the inflow and outflow scales), then the limited
     divergence (read the same five and the two scales; write the
+    tendency).
+    kind "constk": the constant-K diffusion (const_nu_m2_s = 75), k_constk:
+    the grid-mean layer thickness (read ph_total, write and read one sum per
+    column), then the four flux divergences (read each field, write its
     tendency)."""
 
     import torch
@@ -253,6 +259,9 @@ def diffusion_leg(case, steps: int, fp64_ms_per_step: float, dev, kind: str) -> 
     elif kind == "smagorinsky":
         nl, kernel, form = case.namelist.replace(diff_opt=1, km_opt=4), "k_smag", "diff_opt=1,km_opt=4"
         alg_bytes = 8 * ((eu + ev + em) + 2 * (eu + ev + ew + em) + 4 * em)
+    elif kind == "constk":
+        nl, kernel, form = case.namelist.replace(const_nu_m2_s=75.0), "k_constk", "const_nu_m2_s=75"
+        alg_bytes = 8 * (ew + 2 * ny * nx + 2 * (eu + ev + ew + em))
     else:
         nl, kernel, form = case.namelist.replace(scalar_adv_opt=2), "k_limiter", "scalar_adv_opt=2"
         alg_bytes = 8 * ((4 * em + ew + 2 * em) + (4 * em + ew + 2 * em + em))
@@ -487,6 +496,9 @@ def main():
     if args.limiter_steps > 0:
         forms["limiter"] = diffusion_leg(case, args.limiter_steps, ms_per_step, dev, "limiter")
         print(f"[bench] limiter leg done", file=sys.stderr, flush=True)
+    if args.constk_steps > 0:
+        forms["constk"] = diffusion_leg(case, args.constk_steps, ms_per_step, dev, "constk")
+        print(f"[bench] constant-K leg done", file=sys.stderr, flush=True)
     if args.small_steps > 0:
         forms["small_workload"] = small_workload_leg(args.small_steps, dev)
         print(f"[bench] small workload leg done", file=sys.stderr, flush=True)

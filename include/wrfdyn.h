@@ -88,6 +88,10 @@ typedef struct {
                                (operational_mode.py:2819-2866) */
     int hypsometric_opt;    /* 1 (or 0) linear al, 2 log-pressure al
                                (rk_addtend_dry.py:196-222, acoustic_wrf.py:394-427) */
+    double const_nu_m2_s;   /* > 0: constant-K flux-form diffusion of u, v, w
+                               and theta with this K in m^2/s
+                               (operational_mode.py:2878-2929); not together
+                               with the Smagorinsky option */
 } wd_namelist;
 
 /* Prognostic State (src/gpuwrf/contracts/state.py:811-821) plus the carried

@@ -49,6 +49,7 @@ class _Namelist(ctypes.Structure):
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
         ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
         ("scalar_adv_opt", ctypes.c_int), ("hypsometric_opt", ctypes.c_int),
+        ("const_nu_m2_s", ctypes.c_double),
     ]
 
 
@@ -154,7 +155,8 @@ def _namelist(namelist) -> _Namelist:
                      int(bool(namelist.top_lid)), int(namelist.w_damping), int(namelist.damp_opt),
                      float(namelist.dampcoef), float(namelist.zdamp), int(namelist.diff_6th_opt),
                      float(namelist.diff_6th_factor), int(namelist.diff_opt), int(namelist.km_opt),
-                     float(namelist.c_s), int(namelist.scalar_adv_opt), int(namelist.hypsometric_opt))
+                     float(namelist.c_s), int(namelist.scalar_adv_opt), int(namelist.hypsometric_opt),
+                     float(namelist.const_nu_m2_s))
 
 
 def _state_copy(metrics, state: dict) -> dict:
@@ -272,6 +274,28 @@ def smag_tendency(metrics, state: dict, c_s: float) -> tuple[dict, np.ndarray]:
     if rc != 0:
         raise RuntimeError("orc_smag_tendency failed")
     return out, km
+
+
+def constk_tendency(metrics, state: dict, nu: float) -> tuple[dict, float]:
+    """The coupled tendencies of the constant-K diffusion (const_nu_m2_s = nu;
+    operational_mode.py:2878-2929) of ``state`` alone, as {"u", "v", "w",
+    "theta"}, and the layer thickness dz it uses."""
+
+    L = lib()
+    L.orc_constk_tendency.restype = ctypes.c_int
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    st = _state_copy(metrics, state)
+    s = _State(*[_ptr(st[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
+    out = {"u": np.zeros((nz, ny, nx + 1)), "v": np.zeros((nz, ny + 1, nx)), "w": np.zeros((nz + 1, ny, nx)),
+           "theta": np.zeros((nz, ny, nx))}
+    dz = ctypes.c_double(0.0)
+    rc = L.orc_constk_tendency(ctypes.byref(g), ctypes.c_double(nu), ctypes.byref(s), _ptr(out["u"]),
+                               _ptr(out["v"]), _ptr(out["w"]), _ptr(out["theta"]), ctypes.byref(dz))
+    if rc != 0:
+        raise RuntimeError("orc_constk_tendency failed")
+    return out, dz.value
 
 
 def sixth_tendency(field: np.ndarray, coef: float) -> np.ndarray:

@@ -1059,6 +1059,117 @@ static void smagorinsky_hdiff(const orc_grid *g, const orc_namelist *nl, const r
     free(muv);
 }
 
+/* ------------------------------------------------------------------ */
+/* Constant-K diffusion (const_nu_m2_s > 0, operational_mode.py:2878-2929): */
+/* conservative_constant_k_diffusion_tendency                           */
+/* (dynamics/explicit_diffusion.py:282-331) of u, v, w and theta: the   */
+/* divergence of mass K grad f, each face mass the mean of its two      */
+/* neighbours, periodic over each array's own extent (u over its nx + 1 */
+/* faces, v over its ny + 1), no flux through the first and last level, */
+/* and one layer thickness for the whole grid, the mean of              */
+/* (ph_total[k+1] - ph_total[k]) / g and at least 1 m                   */
+/* (operational_mode.py:2756-2758).                                     */
+/* ------------------------------------------------------------------ */
+#define CK_PARTS 1024
+
+/* dz in a fixed summation order: each column summed upward, the columns in
+ * CK_PARTS interleaved partial sums, the partial sums in order. */
+static real constk_dz(const orc_grid *g, const real *ph) {
+    const int nz = g->nz;
+    const size_t C = (size_t)g->ny * g->nx;
+    real *part = zalloc(CK_PARTS);
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < CK_PARTS; ++t) {
+        real s = 0.0;
+        for (size_t c = (size_t)t; c < C; c += CK_PARTS) {
+            real col = 0.0;
+            for (int k = 0; k < nz; ++k) col = col + (ph[(size_t)(k + 1) * C + c] - ph[(size_t)k * C + c]) / G;
+            s = s + col;
+        }
+        part[t] = s;
+    }
+    real tot = 0.0;
+    for (int t = 0; t < CK_PARTS; ++t) tot = tot + part[t];
+    free(part);
+    const real mean = tot / (double)((size_t)nz * C);
+    return mean < 1.0 ? (real)1.0 : mean; /* NaN stays NaN, as jnp.maximum */
+}
+
+/* The diffusion tendency of f (nlev x nrow x ncol) at (k, j, i), with mass
+ * c1[k] mf[j, i] + c2[k]. */
+static inline real constk_point(const real *f, int nlev, int nrow, int ncol, const double *c1, const double *c2,
+                                const real *mf, double K, double dx, double dy, real dz, int k, int j, int i) {
+    const size_t P = (size_t)nrow * ncol;
+#define CKF(kk, jj, ii) f[(size_t)(kk) * P + (size_t)(jj) * ncol + (size_t)(ii)]
+#define CKM(kk, jj, ii) (c1[kk] * mf[(size_t)(jj) * ncol + (size_t)(ii)] + c2[kk])
+    const int ie = wrap(i + 1, ncol), iw = wrap(i - 1, ncol);
+    const real fe = (0.5 * (CKM(k, j, i) + CKM(k, j, ie))) * K * (CKF(k, j, ie) - CKF(k, j, i)) / dx;
+    const real fw = (0.5 * (CKM(k, j, iw) + CKM(k, j, i))) * K * (CKF(k, j, i) - CKF(k, j, iw)) / dx;
+    real t = (fe - fw) / dx;
+    if (nrow > 1) {
+        const int jn = wrap(j + 1, nrow), js = wrap(j - 1, nrow);
+        const real fn = (0.5 * (CKM(k, j, i) + CKM(k, jn, i))) * K * (CKF(k, jn, i) - CKF(k, j, i)) / dy;
+        const real fs = (0.5 * (CKM(k, js, i) + CKM(k, j, i))) * K * (CKF(k, j, i) - CKF(k, js, i)) / dy;
+        t = t + (fn - fs) / dy;
+    }
+    if (nlev > 2) {
+        const real fu = k + 1 < nlev
+                            ? (0.5 * (CKM(k, j, i) + CKM(k + 1, j, i))) * K * (CKF(k + 1, j, i) - CKF(k, j, i)) / dz
+                            : (real)0.0;
+        const real fd = k > 0 ? (0.5 * (CKM(k - 1, j, i) + CKM(k, j, i))) * K * (CKF(k, j, i) - CKF(k - 1, j, i)) / dz
+                              : (real)0.0;
+        t = t + (fu - fd) / dz;
+    }
+#undef CKF
+#undef CKM
+    return t;
+}
+
+static void constk_field(const real *f, int nlev, int nrow, int ncol, const double *c1, const double *c2,
+                         const real *mf, double K, double dx, double dy, real dz, real *tend) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nlev; ++k)
+        for (int j = 0; j < nrow; ++j)
+            for (int i = 0; i < ncol; ++i) {
+                const size_t q = ((size_t)k * nrow + j) * ncol + i;
+                tend[q] = tend[q] + constk_point(f, nlev, nrow, ncol, c1, c2, mf, K, dx, dy, dz, k, j, i);
+            }
+}
+
+/* The coupled constant-K tendencies added to ut, vt, wt and tht; the layer
+ * thickness used in *dz_out when given. */
+static void constk_diffusion(const orc_grid *g, double K, const rstate *S, real *ut, real *vt, real *wt, real *tht,
+                             real *dz_out) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const real dz = constk_dz(g, S->ph_total);
+    real *muu = zalloc((size_t)ny * (nx + 1)), *muv = zalloc((size_t)(ny + 1) * nx);
+    u_face_avg(nx, ny, S->mu_total, muu);
+    v_face_avg(nx, ny, S->mu_total, muv);
+    constk_field(S->theta, nz, ny, nx, g->c1h, g->c2h, S->mu_total, K, g->dx, g->dy, dz, tht);
+    constk_field(S->u, nz, ny, nx + 1, g->c1h, g->c2h, muu, K, g->dx, g->dy, dz, ut);
+    constk_field(S->v, nz, ny + 1, nx, g->c1h, g->c2h, muv, K, g->dx, g->dy, dz, vt);
+    constk_field(S->w, nz + 1, ny, nx, g->c1f, g->c2f, S->mu_total, K, g->dx, g->dy, dz, wt);
+    if (dz_out) *dz_out = dz;
+    free(muu);
+    free(muv);
+}
+
+#ifndef ORC_AD
+int orc_constk_tendency(const orc_grid *g, double nu, const orc_state *s, double *ut, double *vt, double *wt,
+                        double *tht, double *dz) {
+    if (!g || !s || !ut || !vt || !wt || !tht || g->nx < 1 || g->ny < 1 || g->nz < 1 || !(nu >= 0.0)) return -1;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    memset(ut, 0, (size_t)nz * ny * (nx + 1) * sizeof(double));
+    memset(vt, 0, (size_t)nz * (ny + 1) * nx * sizeof(double));
+    memset(wt, 0, (size_t)(nz + 1) * ny * nx * sizeof(double));
+    memset(tht, 0, (size_t)nz * ny * nx * sizeof(double));
+    const rstate S = {s->u, s->v, s->w, s->theta, s->p_total, s->p_pert, s->ph_total, s->ph_pert,
+                      s->mu_total, s->mu_pert, s->mudf};
+    constk_diffusion(g, nu, &S, ut, vt, wt, tht, dz);
+    return 0;
+}
+#endif
+
 #ifndef ORC_AD
 int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double *ut, double *vt, double *wt,
                       double *tht, double *xkmh) {
@@ -1312,6 +1423,8 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
     }
     /* 2-D Smagorinsky horizontal diffusion (operational_mode.py:2946-2963), added
      * after the filter and before the PGF. */
+    /* constant-K diffusion (operational_mode.py:2878-2929), after the filter */
+    if (nl->const_nu_m2_s > 0.0) constk_diffusion(g, nl->const_nu_m2_s, S, W->ut, W->vt, W->wt, W->tht, NULL);
     if (nl->diff_opt == 1 && nl->km_opt == 4) smagorinsky_hdiff(g, nl, S, W->ut, W->vt, W->wt, W->tht);
     /* EOS alt of the stage-entry state, shared by _absolute_diagnostics
      * (rk_addtend_dry.py:184-186) and diagnose_pressure_al_alt's base_state=None

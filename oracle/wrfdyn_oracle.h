@@ -60,6 +60,7 @@ typedef struct {
     int    scalar_adv_opt; /* theta advection on the final RK stage: 0 plain,
                               1 positive-definite, 2 monotonic limiter */
     int    hypsometric_opt; /* 1 linear al (dp = mu d eta), 2 log-pressure al */
+    double const_nu_m2_s;   /* > 0: constant-K flux-form diffusion, K in m^2/s */
 } orc_namelist;
 
 typedef struct {
@@ -120,6 +121,10 @@ int orc_sixth_tendency(int nlev, int nrow, int ncol, const double *f, double coe
  * mu_total, limits from the origin field qold and mass mu_old over dt. */
 int orc_advect_limited(const orc_grid *g, int opt, const double *u, const double *v, const double *mu_total,
                        const double *q, const double *qold, const double *mu_old, double dt, double *tend);
+/* The constant-K diffusion tendencies alone (zero + the diffusion) of
+ * state s with K = nu, and the layer thickness dz it uses; 0 on success. */
+int orc_constk_tendency(const orc_grid *g, double nu, const orc_state *s, double *ut, double *vt, double *wt,
+                        double *tht, double *dz);
 int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double *ut, double *vt, double *wt,
                       double *tht, double *xkmh);
 double orc_pow(double x, double y);

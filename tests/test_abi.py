@@ -104,7 +104,9 @@ def test_namelist_validation_matches_supported_path():
     nl = Namelist(dt_s=6.0, acoustic_substeps=6)
     nl.validate()
     for bad in (dict(top_lid=False), dict(hypsometric_opt=3), dict(damp_opt=1), dict(w_damping=2),
-                dict(acoustic_substeps=0), dict(run_physics=True), dict(h_sca_adv_order=5)):
+                dict(acoustic_substeps=0), dict(run_physics=True), dict(h_sca_adv_order=5),
+                dict(const_nu_m2_s=-1.0), dict(const_nu_m2_s=float("inf")),
+                dict(const_nu_m2_s=75.0, diff_opt=1, km_opt=4)):
         with pytest.raises(ValueError):
             nl.replace(**bad).validate()
 

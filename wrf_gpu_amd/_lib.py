@@ -50,6 +50,7 @@ class WdNamelist(ctypes.Structure):
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
         ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
         ("scalar_adv_opt", ctypes.c_int), ("hypsometric_opt", ctypes.c_int),
+        ("const_nu_m2_s", ctypes.c_double),
     ]
 
 

@@ -270,6 +270,10 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return fail(nullptr, WD_EINVAL,
                 "diffusion: only diff_opt=0, or diff_opt=1 with km_opt=4 (2-D Smagorinsky), is implemented");
   if (!(nl->c_s >= 0.0)) return fail(nullptr, WD_EINVAL, "c_s must be >= 0");
+  if (!(nl->const_nu_m2_s >= 0.0 && nl->const_nu_m2_s <= 1.0e300))
+    return fail(nullptr, WD_EINVAL, "const_nu_m2_s must be finite and >= 0");
+  if (nl->const_nu_m2_s > 0.0 && nl->diff_opt == 1)
+    return fail(nullptr, WD_EINVAL, "const_nu_m2_s > 0 and the 2-D Smagorinsky diffusion do not run together");
   if (nl->hypsometric_opt < 0 || nl->hypsometric_opt > 2)
     return fail(nullptr, WD_EINVAL, "hypsometric_opt must be 1 (linear) or 2 (log-pressure)");
   if (nl->hypsometric_opt == 2 && (!grid->c3h || !grid->c4h || !grid->c3f || !grid->c4f))
@@ -384,6 +388,7 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   h->nl.smag = nl->diff_opt == 1 && nl->km_opt == 4;
   h->nl.c_s = nl->c_s;
   h->nl.scalar_adv_opt = nl->scalar_adv_opt;
+  h->nl.const_nu = nl->const_nu_m2_s;
 
   // ---- workspace ----
   const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
@@ -429,7 +434,7 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return hip_fail(nullptr, e, "hipMemset(workspace)");
   }
   h->nl.mixed = 0;
-  if (nl->diff_6th_opt != 0 || h->nl.smag || nl->scalar_adv_opt != 0) {
+  if (nl->diff_6th_opt != 0 || h->nl.smag || nl->scalar_adv_opt != 0 || h->nl.const_nu > 0.0) {
     // the arrays of the optional terms, each 256-byte aligned: the filter's
     // tendencies, the Smagorinsky viscosity and tendencies, the limiter's
     // origin mass and scales
@@ -437,6 +442,9 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     if (nl->diff_6th_opt != 0) dslots.insert(dslots.end(), {{&w.d6u, U}, {&w.d6v, V}, {&w.d6w, Wn}, {&w.d6t, M}});
     if (h->nl.smag)
       dslots.insert(dslots.end(), {{&w.xkmh, M}, {&w.smu, U}, {&w.smv, V}, {&w.smw, Wn}, {&w.smt, M}});
+    if (h->nl.const_nu > 0.0)
+      dslots.insert(dslots.end(), {{&w.smu, U}, {&w.smv, V}, {&w.smw, Wn}, {&w.smt, M}, {&w.ck_col, (size_t)ny * nx},
+                                   {&w.ck_dz, 1}});
     if (nl->scalar_adv_opt != 0)
       dslots.insert(dslots.end(), {{&w.mut0, (size_t)ny * nx}, {&w.lim_sa, M}, {&w.lim_sb, M}, {&w.lim_ax, M},
                                    {&w.lim_lx, M}, {&w.lim_ay, M}, {&w.lim_ly, M}, {&w.lim_az, Wn}, {&w.lim_lz, Wn}});
@@ -807,6 +815,7 @@ int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* st
   if (h->nl.mixed) return fail(h, WD_EINVAL, "the adjoint is built for the FP64 form (WD_PREC_FP64) only");
   if (h->nl.diff_6th_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the 6th-order filter");
   if (h->nl.smag) return fail(h, WD_EINVAL, "the adjoint does not cover the Smagorinsky diffusion");
+  if (h->nl.const_nu > 0.0) return fail(h, WD_EINVAL, "the adjoint does not cover the constant-K diffusion");
   if (h->nl.scalar_adv_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the theta limiter");
   if (h->g.hypsometric_opt == 2) return fail(h, WD_EINVAL, "the adjoint does not cover hypsometric_opt=2");
   // Switched off: the first GPU run of this path ended in a memory access

@@ -55,6 +55,7 @@ struct DNamelist {
   int smag;                // 1: 2-D Smagorinsky horizontal diffusion (diff_opt 1, km_opt 4)
   double c_s;
   int scalar_adv_opt;      // theta on the final RK stage: 0 plain, 1 positive-definite, 2 monotonic
+  double const_nu;         // > 0: constant-K diffusion with this K (m^2/s)
   int mixed;  // 1: mixed-precision form (FP32 tendencies and flux interpolants), 0: plain FP64
 };
 
@@ -101,6 +102,9 @@ struct DWork {
   // its face fluxes: antidiffusive (a) and donor-cell (l) at each cell's
   // x, y and z minus face (z: nz + 1 faces)
   double *lim_ax, *lim_lx, *lim_ay, *lim_ly, *lim_az, *lim_lz;
+  // the constant-K diffusion (null unless const_nu > 0; its tendencies go to
+  // smu..smt): each column's summed layer thickness and the grid mean dz
+  double *ck_col, *ck_dz;
 };
 
 __host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {

@@ -33,6 +33,7 @@ enum KernelId {
   KID_DIFF6,
   KID_SMAG,
   KID_LIMITER,
+  KID_CONSTK,
   KID_COUNT
 };
 

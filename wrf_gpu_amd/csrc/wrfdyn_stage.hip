@@ -1288,6 +1288,24 @@ __device__ inline double d6_line(double fm3, double fm2, double fm1, double f0, 
 
 constexpr int kD6Halo = 3, kD6W = kFieldBx + 2 * kD6Halo, kD6H = kTileRows + 2 * kD6Halo;
 
+// The dry mass of a point of a filtered or diffused field: u faces (kind 0)
+// and v faces (1) take mu_total averaged to the face with the edge value at
+// the first and last face, w levels (2) c1f mu + c2f, mass points (3) mu.
+__device__ inline double face_mass(const DGrid& g, int kind, const double* mu, int k, int j, int i) {
+  const int nx = g.nx, ny = g.ny;
+  double m;
+  if (kind == 0) {
+    m = i == 0 ? mu[(size_t)j * nx] : (i == nx ? mu[(size_t)j * nx + nx - 1]
+                                              : 0.5 * (mu[(size_t)j * nx + i - 1] + mu[(size_t)j * nx + i]));
+  } else if (kind == 1) {
+    m = j == 0 ? mu[i] : (j == ny ? mu[(size_t)(ny - 1) * nx + i]
+                                  : 0.5 * (mu[(size_t)(j - 1) * nx + i] + mu[(size_t)j * nx + i]));
+  } else {
+    m = mu[(size_t)j * nx + i];
+  }
+  return kind == 2 ? g.c1f[k] * m + g.c2f[k] : g.c1h[k] * m + g.c2h[k];
+}
+
 __global__ void __launch_bounds__(256) k_diff6(DGrid g, const double* f, int kind, int nlev, int nrow, int ncol,
                                                const double* mu, double coef, double* out) {
   __shared__ double tile[kD6H][kD6W];
@@ -1301,22 +1319,11 @@ __global__ void __launch_bounds__(256) k_diff6(DGrid g, const double* f, int kin
   __syncthreads();
   const int i = i0 + tx;
   if (i >= ncol) return;
-  const int nx = g.nx, ny = g.ny;
   const int c = tx + kD6Halo;
   for (int r = ty; r < kTileRows; r += kFieldBy) {
     const int j = j0 + r;
     if (j >= nrow) break;
-    double m;
-    if (kind == 0) {
-      m = i == 0 ? mu[(size_t)j * nx] : (i == nx ? mu[(size_t)j * nx + nx - 1]
-                                                : 0.5 * (mu[(size_t)j * nx + i - 1] + mu[(size_t)j * nx + i]));
-    } else if (kind == 1) {
-      m = j == 0 ? mu[i] : (j == ny ? mu[(size_t)(ny - 1) * nx + i]
-                                    : 0.5 * (mu[(size_t)(j - 1) * nx + i] + mu[(size_t)j * nx + i]));
-    } else {
-      m = mu[(size_t)j * nx + i];
-    }
-    const double mass = kind == 2 ? g.c1f[k] * m + g.c2f[k] : g.c1h[k] * m + g.c2h[k];
+    const double mass = face_mass(g, kind, mu, k, j, i);
     const double* row = tile[r + kD6Halo];
     double t = d6_line(row[c - 3], row[c - 2], row[c - 1], row[c], row[c + 1], row[c + 2], row[c + 3], coef);
     if (nrow > 1) {
@@ -1336,6 +1343,88 @@ void launch_diff6(const DGrid& g, const DNamelist& nl, const DState& s, const DW
   k_diff6<<<tile_grid(nx, ny + 1, nz), bs, 0, st>>>(g, s.v, 1, nz, ny + 1, nx, s.mu_total, coef, w.d6v);
   k_diff6<<<tile_grid(nx, ny, nz + 1), bs, 0, st>>>(g, s.w, 2, nz + 1, ny, nx, s.mu_total, coef, w.d6w);
   k_diff6<<<tile_grid(nx, ny, nz), bs, 0, st>>>(g, s.theta, 3, nz, ny, nx, s.mu_total, coef, w.d6t);
+}
+
+// ---------------------------------------------------------------------------
+// Constant-K diffusion (const_nu_m2_s > 0; runtime/operational_mode.py:2878-2929):
+// conservative_constant_k_diffusion_tendency (dynamics/explicit_diffusion.py:282-331)
+// of u, v, w and theta into the smu..smt slots, which the tendency kernels
+// add after the filter.  The divergence of mass K grad f, each face mass the
+// mean of its two neighbours, periodic over each array's own extent (u over
+// its nx + 1 faces, v over its ny + 1), no flux through the first and last
+// level.  The layer thickness is one number for the grid, the mean of
+// (ph_total[k+1] - ph_total[k]) / g and at least 1 m (:2756-2758), summed in a
+// fixed order: k_ck_col sums each column upward, k_ck_dz (one block) folds
+// the columns into kCkParts interleaved partial sums and adds those in order.
+constexpr int kCkParts = 1024;
+
+__global__ void __launch_bounds__(256) k_ck_col(int nz, size_t C, const double* ph, double* colsum) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  double s = 0.0;
+  for (int k = 0; k < nz; ++k) s = s + (ph[(size_t)(k + 1) * C + c] - ph[(size_t)k * C + c]) / kG;
+  colsum[c] = s;
+}
+
+__global__ void __launch_bounds__(kCkParts) k_ck_dz(size_t C, double n, const double* colsum, double* dz) {
+  __shared__ double part[kCkParts];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (size_t c = (size_t)t; c < C; c += kCkParts) s = s + colsum[c];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    double tot = 0.0;
+    for (int q = 0; q < kCkParts; ++q) tot = tot + part[q];
+    const double mean = tot / n;
+    dz[0] = mean < 1.0 ? 1.0 : mean;  // NaN stays NaN, as jnp.maximum
+  }
+}
+
+// One point of f (nlev x nrow x ncol) per thread; the neighbours come
+// through the caches.
+__global__ void __launch_bounds__(256) k_constk(DGrid g, const double* f, int kind, int nlev, int nrow, int ncol,
+                                                const double* mu, double K, const double* dzp, double* out) {
+  const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
+  if (i >= ncol || j >= nrow) return;
+  const size_t P = (size_t)nrow * ncol;
+  auto F = [&](int kk, int jj, int ii) { return f[(size_t)kk * P + (size_t)jj * ncol + ii]; };
+  auto Ms = [&](int kk, int jj, int ii) { return face_mass(g, kind, mu, kk, jj, ii); };
+  const double dx = g.dx, dy = g.dy, dz = dzp[0];
+  const int ie = wrapc(i + 1, ncol), iw = wrapc(i - 1, ncol);
+  const double fe = (0.5 * (Ms(k, j, i) + Ms(k, j, ie))) * K * (F(k, j, ie) - F(k, j, i)) / dx;
+  const double fw = (0.5 * (Ms(k, j, iw) + Ms(k, j, i))) * K * (F(k, j, i) - F(k, j, iw)) / dx;
+  double t = (fe - fw) / dx;
+  if (nrow > 1) {
+    const int jn = wrapc(j + 1, nrow), js = wrapc(j - 1, nrow);
+    const double fn = (0.5 * (Ms(k, j, i) + Ms(k, jn, i))) * K * (F(k, jn, i) - F(k, j, i)) / dy;
+    const double fs = (0.5 * (Ms(k, js, i) + Ms(k, j, i))) * K * (F(k, j, i) - F(k, js, i)) / dy;
+    t = t + (fn - fs) / dy;
+  }
+  if (nlev > 2) {
+    const double fu =
+        k + 1 < nlev ? (0.5 * (Ms(k, j, i) + Ms(k + 1, j, i))) * K * (F(k + 1, j, i) - F(k, j, i)) / dz : 0.0;
+    const double fd = k > 0 ? (0.5 * (Ms(k - 1, j, i) + Ms(k, j, i))) * K * (F(k, j, i) - F(k - 1, j, i)) / dz : 0.0;
+    t = t + (fu - fd) / dz;
+  }
+  out[(size_t)k * P + (size_t)j * ncol + i] = t;
+}
+
+void launch_constk(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t C = (size_t)ny * nx;
+  k_ck_col<<<blocks_for(C, 256), 256, 0, st>>>(nz, C, s.ph_total, w.ck_col);
+  k_ck_dz<<<1, kCkParts, 0, st>>>(C, (double)((size_t)nz * C), w.ck_col, w.ck_dz);
+  const dim3 bs(kFieldBx, kFieldBy);
+  auto grid = [](int ncol, int nrow, int nlev) {
+    return dim3((unsigned)((ncol + kFieldBx - 1) / kFieldBx), (unsigned)((nrow + kFieldBy - 1) / kFieldBy),
+                (unsigned)nlev);
+  };
+  const double K = nl.const_nu;
+  k_constk<<<grid(nx + 1, ny, nz), bs, 0, st>>>(g, s.u, 0, nz, ny, nx + 1, s.mu_total, K, w.ck_dz, w.smu);
+  k_constk<<<grid(nx, ny + 1, nz), bs, 0, st>>>(g, s.v, 1, nz, ny + 1, nx, s.mu_total, K, w.ck_dz, w.smv);
+  k_constk<<<grid(nx, ny, nz + 1), bs, 0, st>>>(g, s.w, 2, nz + 1, ny, nx, s.mu_total, K, w.ck_dz, w.smw);
+  k_constk<<<grid(nx, ny, nz), bs, 0, st>>>(g, s.theta, 3, nz, ny, nx, s.mu_total, K, w.ck_dz, w.smt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1692,6 +1781,7 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const unsigned kchunks = (unsigned)((g.nz + kChunk - 1) / kChunk);
   if (w.d6u) WD_TIMED(prof, KID_DIFF6, st, (launch_diff6(g, nl, s, w, st)));
   if (w.xkmh) WD_TIMED(prof, KID_SMAG, st, (launch_smag(g, nl, s, w, st)));
+  if (w.ck_dz) WD_TIMED(prof, KID_CONSTK, st, (launch_constk(g, nl, s, w, st)));
   // theta's limited advection replaces the plain one on the final stage
   const bool limit_theta = nl.scalar_adv_opt != 0 && rk_step == 3;
   WD_TIMED(prof, KID_RUV, st,

@@ -76,7 +76,8 @@ class Dycore:
                                    int(bool(nl.top_lid)), int(nl.w_damping), int(nl.damp_opt),
                                    float(nl.dampcoef), float(nl.zdamp), int(nl.diff_6th_opt),
                                    float(nl.diff_6th_factor), int(nl.diff_opt), int(nl.km_opt),
-                                   float(nl.c_s), int(nl.scalar_adv_opt), int(nl.hypsometric_opt))
+                                   float(nl.c_s), int(nl.scalar_adv_opt), int(nl.hypsometric_opt),
+                                   float(nl.const_nu_m2_s))
         handle = ctypes.c_void_p()
         code = lib.wd_create(ctypes.byref(self._grid), ctypes.byref(self._nl), ctypes.byref(handle))
         if code != _lib.WD_OK:

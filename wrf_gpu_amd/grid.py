@@ -13,6 +13,7 @@ v-point ``(ny+1, nx)``.
 from __future__ import annotations
 
 import dataclasses
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -161,6 +162,7 @@ class Namelist:
     km_opt: int = 0
     c_s: float = 0.25
     scalar_adv_opt: int = 0
+    const_nu_m2_s: float = 0.0
     use_flux_advection: bool = True
     hypsometric_opt: int = 1
     h_sca_adv_order: int = 2
@@ -192,6 +194,10 @@ class Namelist:
             raise ValueError("diffusion: only diff_opt=0, or diff_opt=1 with km_opt=4 (2-D Smagorinsky), is implemented")
         if not self.c_s >= 0.0:
             raise ValueError("c_s must be >= 0")
+        if not (self.const_nu_m2_s >= 0.0 and math.isfinite(self.const_nu_m2_s)):
+            raise ValueError("const_nu_m2_s must be finite and >= 0")
+        if self.const_nu_m2_s > 0.0 and self.diff_opt == 1:
+            raise ValueError("const_nu_m2_s > 0 and the 2-D Smagorinsky diffusion do not run together")
         if self.scalar_adv_opt not in (0, 1, 2):
             raise ValueError("scalar_adv_opt must be 0, 1 (positive-definite) or 2 (monotonic)")
         if self.run_physics or self.run_boundary:
