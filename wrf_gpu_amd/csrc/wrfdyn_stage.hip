@@ -1288,21 +1288,23 @@ __device__ inline double d6_line(double fm3, double fm2, double fm1, double f0, 
 
 constexpr int kD6Halo = 3, kD6W = kFieldBx + 2 * kD6Halo, kD6H = kTileRows + 2 * kD6Halo;
 
-// The dry mass of a point of a filtered or diffused field: u faces (kind 0)
-// and v faces (1) take mu_total averaged to the face with the edge value at
-// the first and last face, w levels (2) c1f mu + c2f, mass points (3) mu.
-__device__ inline double face_mass(const DGrid& g, int kind, const double* mu, int k, int j, int i) {
+// The mu of a point of a filtered or diffused field: u faces (kind 0) and v
+// faces (1) take mu_total averaged to the face with the edge value at the
+// first and last face, w levels (2) and mass points (3) mu itself.
+__device__ inline double face_mu(const DGrid& g, int kind, const double* mu, int j, int i) {
   const int nx = g.nx, ny = g.ny;
-  double m;
-  if (kind == 0) {
-    m = i == 0 ? mu[(size_t)j * nx] : (i == nx ? mu[(size_t)j * nx + nx - 1]
-                                              : 0.5 * (mu[(size_t)j * nx + i - 1] + mu[(size_t)j * nx + i]));
-  } else if (kind == 1) {
-    m = j == 0 ? mu[i] : (j == ny ? mu[(size_t)(ny - 1) * nx + i]
-                                  : 0.5 * (mu[(size_t)(j - 1) * nx + i] + mu[(size_t)j * nx + i]));
-  } else {
-    m = mu[(size_t)j * nx + i];
-  }
+  if (kind == 0)
+    return i == 0 ? mu[(size_t)j * nx] : (i == nx ? mu[(size_t)j * nx + nx - 1]
+                                                 : 0.5 * (mu[(size_t)j * nx + i - 1] + mu[(size_t)j * nx + i]));
+  if (kind == 1)
+    return j == 0 ? mu[i] : (j == ny ? mu[(size_t)(ny - 1) * nx + i]
+                                     : 0.5 * (mu[(size_t)(j - 1) * nx + i] + mu[(size_t)j * nx + i]));
+  return mu[(size_t)j * nx + i];
+}
+
+// Its dry mass on level k: c1f mu + c2f on w levels, c1h mu + c2h elsewhere.
+__device__ inline double face_mass(const DGrid& g, int kind, const double* mu, int k, int j, int i) {
+  const double m = face_mu(g, kind, mu, j, i);
   return kind == 2 ? g.c1f[k] * m + g.c2f[k] : g.c1h[k] * m + g.c2h[k];
 }
 
@@ -1381,33 +1383,64 @@ __global__ void __launch_bounds__(kCkParts) k_ck_dz(size_t C, double n, const do
   }
 }
 
-// One point of f (nlev x nrow x ncol) per thread; the neighbours come
-// through the caches.
+// LDS-tiled like k_diff6: the level's field and face mu over the tile and a
+// one-point halo, staged once with coalesced loads; the levels above and
+// below are read at the thread's own point.
+constexpr int kCkW = kFieldBx + 2, kCkH = kTileRows + 2;
+
 __global__ void __launch_bounds__(256) k_constk(DGrid g, const double* f, int kind, int nlev, int nrow, int ncol,
                                                 const double* mu, double K, const double* dzp, double* out) {
-  const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
-  if (i >= ncol || j >= nrow) return;
+  __shared__ double tf[kCkH][kCkW], tm[kCkH][kCkW];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = blockIdx.x * kFieldBx, j0 = blockIdx.y * kTileRows, k = blockIdx.z;
   const size_t P = (size_t)nrow * ncol;
-  auto F = [&](int kk, int jj, int ii) { return f[(size_t)kk * P + (size_t)jj * ncol + ii]; };
-  auto Ms = [&](int kk, int jj, int ii) { return face_mass(g, kind, mu, kk, jj, ii); };
+  const double* lev = f + (size_t)k * P;
+  for (int e = ty * kFieldBx + tx; e < kCkH * kCkW; e += kFieldBx * kFieldBy) {
+    const int r = e / kCkW, c = e - r * kCkW;
+    const int jj = wrapc(j0 - 1 + r, nrow), ii = wrapc(i0 - 1 + c, ncol);
+    tf[r][c] = lev[(size_t)jj * ncol + ii];
+    tm[r][c] = face_mu(g, kind, mu, jj, ii);
+  }
+  __syncthreads();
+  const int i = i0 + tx;
+  if (i >= ncol) return;
+  const bool wl = kind == 2;
+  const double* c1 = wl ? g.c1f : g.c1h;
+  const double* c2 = wl ? g.c2f : g.c2h;
+  const double c1k = c1[k], c2k = c2[k];
   const double dx = g.dx, dy = g.dy, dz = dzp[0];
-  const int ie = wrapc(i + 1, ncol), iw = wrapc(i - 1, ncol);
-  const double fe = (0.5 * (Ms(k, j, i) + Ms(k, j, ie))) * K * (F(k, j, ie) - F(k, j, i)) / dx;
-  const double fw = (0.5 * (Ms(k, j, iw) + Ms(k, j, i))) * K * (F(k, j, i) - F(k, j, iw)) / dx;
-  double t = (fe - fw) / dx;
-  if (nrow > 1) {
-    const int jn = wrapc(j + 1, nrow), js = wrapc(j - 1, nrow);
-    const double fn = (0.5 * (Ms(k, j, i) + Ms(k, jn, i))) * K * (F(k, jn, i) - F(k, j, i)) / dy;
-    const double fs = (0.5 * (Ms(k, js, i) + Ms(k, j, i))) * K * (F(k, j, i) - F(k, js, i)) / dy;
-    t = t + (fn - fs) / dy;
+  const int c = tx + 1;
+  for (int r = ty; r < kTileRows; r += kFieldBy) {
+    const int j = j0 + r;
+    if (j >= nrow) break;
+    const int q = r + 1;
+    const double f0 = tf[q][c], mu0 = tm[q][c];
+    const double m0 = c1k * mu0 + c2k;
+    const double me = c1k * tm[q][c + 1] + c2k, mw = c1k * tm[q][c - 1] + c2k;
+    const double fe = (0.5 * (m0 + me)) * K * (tf[q][c + 1] - f0) / dx;
+    const double fw = (0.5 * (mw + m0)) * K * (f0 - tf[q][c - 1]) / dx;
+    double t = (fe - fw) / dx;
+    if (nrow > 1) {
+      const double mn = c1k * tm[q + 1][c] + c2k, ms = c1k * tm[q - 1][c] + c2k;
+      const double fn = (0.5 * (m0 + mn)) * K * (tf[q + 1][c] - f0) / dy;
+      const double fs = (0.5 * (ms + m0)) * K * (f0 - tf[q - 1][c]) / dy;
+      t = t + (fn - fs) / dy;
+    }
+    if (nlev > 2) {
+      const size_t p = (size_t)j * ncol + i;
+      double fu = 0.0, fd = 0.0;
+      if (k + 1 < nlev) {
+        const double mup = c1[k + 1] * mu0 + c2[k + 1];
+        fu = (0.5 * (m0 + mup)) * K * (f[(size_t)(k + 1) * P + p] - f0) / dz;
+      }
+      if (k > 0) {
+        const double mdn = c1[k - 1] * mu0 + c2[k - 1];
+        fd = (0.5 * (mdn + m0)) * K * (f0 - f[(size_t)(k - 1) * P + p]) / dz;
+      }
+      t = t + (fu - fd) / dz;
+    }
+    out[(size_t)k * P + (size_t)j * ncol + i] = t;
   }
-  if (nlev > 2) {
-    const double fu =
-        k + 1 < nlev ? (0.5 * (Ms(k, j, i) + Ms(k + 1, j, i))) * K * (F(k + 1, j, i) - F(k, j, i)) / dz : 0.0;
-    const double fd = k > 0 ? (0.5 * (Ms(k - 1, j, i) + Ms(k, j, i))) * K * (F(k, j, i) - F(k - 1, j, i)) / dz : 0.0;
-    t = t + (fu - fd) / dz;
-  }
-  out[(size_t)k * P + (size_t)j * ncol + i] = t;
 }
 
 void launch_constk(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
@@ -1416,15 +1449,11 @@ void launch_constk(const DGrid& g, const DNamelist& nl, const DState& s, const D
   k_ck_col<<<blocks_for(C, 256), 256, 0, st>>>(nz, C, s.ph_total, w.ck_col);
   k_ck_dz<<<1, kCkParts, 0, st>>>(C, (double)((size_t)nz * C), w.ck_col, w.ck_dz);
   const dim3 bs(kFieldBx, kFieldBy);
-  auto grid = [](int ncol, int nrow, int nlev) {
-    return dim3((unsigned)((ncol + kFieldBx - 1) / kFieldBx), (unsigned)((nrow + kFieldBy - 1) / kFieldBy),
-                (unsigned)nlev);
-  };
   const double K = nl.const_nu;
-  k_constk<<<grid(nx + 1, ny, nz), bs, 0, st>>>(g, s.u, 0, nz, ny, nx + 1, s.mu_total, K, w.ck_dz, w.smu);
-  k_constk<<<grid(nx, ny + 1, nz), bs, 0, st>>>(g, s.v, 1, nz, ny + 1, nx, s.mu_total, K, w.ck_dz, w.smv);
-  k_constk<<<grid(nx, ny, nz + 1), bs, 0, st>>>(g, s.w, 2, nz + 1, ny, nx, s.mu_total, K, w.ck_dz, w.smw);
-  k_constk<<<grid(nx, ny, nz), bs, 0, st>>>(g, s.theta, 3, nz, ny, nx, s.mu_total, K, w.ck_dz, w.smt);
+  k_constk<<<tile_grid(nx + 1, ny, nz), bs, 0, st>>>(g, s.u, 0, nz, ny, nx + 1, s.mu_total, K, w.ck_dz, w.smu);
+  k_constk<<<tile_grid(nx, ny + 1, nz), bs, 0, st>>>(g, s.v, 1, nz, ny + 1, nx, s.mu_total, K, w.ck_dz, w.smv);
+  k_constk<<<tile_grid(nx, ny, nz + 1), bs, 0, st>>>(g, s.w, 2, nz + 1, ny, nx, s.mu_total, K, w.ck_dz, w.smw);
+  k_constk<<<tile_grid(nx, ny, nz), bs, 0, st>>>(g, s.theta, 3, nz, ny, nx, s.mu_total, K, w.ck_dz, w.smt);
 }
 
 // ---------------------------------------------------------------------------
