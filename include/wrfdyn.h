@@ -63,6 +63,14 @@ typedef struct {
     const double *c3h, *c4h;              /* nz   */
     const double *c3f, *c4f;              /* nz+1 */
     double p_top;
+    /* Coriolis parameters on mass points (DycoreMetrics f, e, sina, cosa,
+     * contracts/grid.py:65-73): f = 2 Omega sin(lat), e = 2 Omega cos(lat)
+     * and the map-projection rotation.  They drive large_step_coriolis
+     * (core/rk_addtend_dry.py:364-483) and the cosine-Coriolis term of
+     * rw_tend (operational_mode.py:1818-1821).  All four NULL, or f and e
+     * zero everywhere, means no rotation (f = e = sina = 0, cosa = 1): the
+     * terms vanish and no kernel evaluates them.  f or e given needs all four. */
+    const double *f, *e, *sina, *cosa;    /* ny*nx */
 } wd_grid;
 
 /* Hot-path fields of OperationalNamelist

@@ -38,6 +38,7 @@ class _Grid(ctypes.Structure):
         ("msftx", _dp), ("msfty", _dp), ("msfux", _dp), ("msfuy", _dp),
         ("msfvx", _dp), ("msfvy", _dp),
         ("c3h", _dp), ("c4h", _dp), ("c3f", _dp), ("c4f", _dp), ("p_top", ctypes.c_double),
+        ("f", _dp), ("e", _dp), ("sina", _dp), ("cosa", _dp),
     ]
 
 
@@ -129,6 +130,7 @@ def _grid(metrics, keep: _Keep) -> _Grid:
         keep(m.dn), keep(m.rdn), keep(m.fnm), keep(m.fnp), m.cf1, m.cf2, m.cf3,
         keep(m.msftx), keep(m.msfty), keep(m.msfux), keep(m.msfuy), keep(m.msfvx), keep(m.msfvy),
         keep(m.c3h), keep(m.c4h), keep(m.c3f), keep(m.c4f), float(m.p_top),
+        *((keep(m.f), keep(m.e), keep(m.sina), keep(m.cosa)) if m.rotating else (None,) * 4),
     )
 
 
@@ -274,6 +276,25 @@ def smag_tendency(metrics, state: dict, c_s: float) -> tuple[dict, np.ndarray]:
     if rc != 0:
         raise RuntimeError("orc_smag_tendency failed")
     return out, km
+
+
+def coriolis_tendency(metrics, state: dict) -> dict:
+    """The Coriolis terms of ``state`` alone: the coupled u and v tendencies
+    of large_step_coriolis (rk_addtend_dry.py:364-483) as "u", "v", and the
+    cosine-Coriolis part of rw_tend (operational_mode.py:1818-1821) as "w"."""
+
+    L = lib()
+    L.orc_coriolis_tendency.restype = ctypes.c_int
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    st = _state_copy(metrics, state)
+    s = _State(*[_ptr(st[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
+    out = {"u": np.zeros((nz, ny, nx + 1)), "v": np.zeros((nz, ny + 1, nx)), "w": np.zeros((nz + 1, ny, nx))}
+    rc = L.orc_coriolis_tendency(ctypes.byref(g), ctypes.byref(s), _ptr(out["u"]), _ptr(out["v"]), _ptr(out["w"]))
+    if rc != 0:
+        raise RuntimeError("orc_coriolis_tendency failed")
+    return out
 
 
 def constk_tendency(metrics, state: dict, nu: float) -> tuple[dict, float]:

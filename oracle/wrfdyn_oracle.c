@@ -650,6 +650,89 @@ static void large_step_pgf(const orc_grid *g, int hypsometric_opt, const rstate 
 }
 
 /* ------------------------------------------------------------------ */
+/* large_step_coriolis, dynamics/core/rk_addtend_dry.py:364-483, added  */
+/* to the coupled u/v tendencies right after the large-step PGF        */
+/* (operational_mode.py:3052-3058) with specified = run_boundary =     */
+/* false.  The coupled momenta are rebuilt here from the stage state   */
+/* with edge-padded face masses of mu_total (:415-419), not taken from */
+/* couple_velocities (periodic face masses):                           */
+/*   ru_cor = (msfux/msfuy) f_u rv_quad - e_u cosa_u rw_quad_x         */
+/*   rv_cor = -(msfvy/msfvx) f_v ru_quad + (msfvy/msfvx) e_v sina_v     */
+/*            rw_quad_y                                                 */
+/* with the f/e/sina/cosa face averages of the two mass points and the */
+/* off-axis momenta averaged over the four surrounding faces, x and y  */
+/* neighbours clamped at the array edges (:436-470).                   */
+/* ------------------------------------------------------------------ */
+static inline real cor_ru(const orc_grid *g, const rstate *S, int k, int j, int i) {
+    const int nx = g->nx, ny = g->ny;
+    (void)ny;
+    const int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
+    real muu = 0.5 * (S->mu_total[I2(j, il)] + S->mu_total[I2(j, ir)]);
+    return (S->u[IU(k, j, i)] * (g->c1h[k] * muu + g->c2h[k])) / g->msfuy[I2U(j, i)];
+}
+static inline real cor_rv(const orc_grid *g, const rstate *S, int k, int j, int i) {
+    const int nx = g->nx, ny = g->ny;
+    const int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
+    real muv = 0.5 * (S->mu_total[I2(js, i)] + S->mu_total[I2(jn, i)]);
+    return (S->v[IV(k, j, i)] * (g->c1h[k] * muv + g->c2h[k])) / g->msfvx[I2V(j, i)];
+}
+static inline real cor_rw(const orc_grid *g, const rstate *S, int k, int j, int i) {
+    const int nx = g->nx, ny = g->ny;
+    (void)ny;
+    return (S->w[IM(k, j, i)] * (g->c1f[k] * S->mu_total[I2(j, i)] + g->c2f[k])) / g->msfty[I2(j, i)];
+}
+
+static void large_step_coriolis(const orc_grid *g, const rstate *S, real *ut, real *vt) {
+    if (!g->f) return;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k) {
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
+                real msf_u = g->msfux[I2U(j, i)] / g->msfuy[I2U(j, i)];
+                real f_u = 0.5 * (g->f[I2(j, il)] + g->f[I2(j, ir)]);
+                real e_u = 0.5 * (g->e[I2(j, il)] + g->e[I2(j, ir)]);
+                real cosa_u = 0.5 * (g->cosa[I2(j, il)] + g->cosa[I2(j, ir)]);
+                real rv_quad = 0.25 * (((cor_rv(g, S, k, j, il) + cor_rv(g, S, k, j, ir)) +
+                                        cor_rv(g, S, k, j + 1, il)) +
+                                       cor_rv(g, S, k, j + 1, ir));
+                real ru_cor = (msf_u * f_u) * rv_quad;
+                real rw_quad = 0.25 * (((cor_rw(g, S, k, j, il) + cor_rw(g, S, k + 1, j, il)) +
+                                        cor_rw(g, S, k, j, ir)) +
+                                       cor_rw(g, S, k + 1, j, ir));
+                ru_cor = ru_cor - (e_u * cosa_u) * rw_quad;
+                ut[IU(k, j, i)] = ut[IU(k, j, i)] + ru_cor;
+            }
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
+                real msf_v = g->msfvy[I2V(j, i)] / g->msfvx[I2V(j, i)];
+                real f_v = 0.5 * (g->f[I2(js, i)] + g->f[I2(jn, i)]);
+                real e_v = 0.5 * (g->e[I2(js, i)] + g->e[I2(jn, i)]);
+                real sina_v = 0.5 * (g->sina[I2(js, i)] + g->sina[I2(jn, i)]);
+                real ru_quad = 0.25 * (((cor_ru(g, S, k, js, i) + cor_ru(g, S, k, js, i + 1)) +
+                                        cor_ru(g, S, k, jn, i)) +
+                                       cor_ru(g, S, k, jn, i + 1));
+                real rv_cor = ((-msf_v) * f_v) * ru_quad;
+                real rw_quad = 0.25 * (((cor_rw(g, S, k, js, i) + cor_rw(g, S, k + 1, js, i)) +
+                                        cor_rw(g, S, k, jn, i)) +
+                                       cor_rw(g, S, k + 1, jn, i));
+                rv_cor = rv_cor + ((msf_v * e_v) * sina_v) * rw_quad;
+                vt[IV(k, j, i)] = vt[IV(k, j, i)] + rv_cor;
+            }
+    }
+}
+
+/* The cosine-Coriolis term of the stage rw_tend at interior face k
+ * (operational_mode.py:1818-1821): e (cosa ru_f - (msftx/msfty) sina rv_f)
+ * with the face-averaged coupled momenta ru_f, rv_f the curvature term uses. */
+static inline real cor_w(const orc_grid *g, size_t c, real mxy, real ru_f, real rv_f) {
+    if (!g->f) return 0.0;
+    return g->e[c] * (g->cosa[c] * ru_f - (mxy * g->sina[c]) * rv_f);
+}
+
+/* ------------------------------------------------------------------ */
 /* calc_coef_w_wrf_coefficients, dynamics/acoustic_wrf.py:783-852       */
 /* cqw is the post-pg_buoy_w field: 0 at faces 0 and nz, 1 inside       */
 /* (core/advance_w.py:266-290 with qtot = 0).                           */
@@ -1194,6 +1277,51 @@ int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double 
                 for (int i = 0; i < nx; ++i) xkmh[IM(k, j, i)] = smag_km_at(g, c_s, s->u, s->v, k, j, i);
     return 0;
 }
+
+/* The Coriolis terms alone, for tests: ut, vt get large_step_coriolis and
+ * rwt (nz+1 faces) the cosine-Coriolis part of rw_tend, with the face
+ * masses of the stage mut = (mu_total - mu_pert) + mu_pert as
+ * small_step_prep forms them (0 at faces 0 and nz). */
+int orc_coriolis_tendency(const orc_grid *g, const orc_state *s, double *ut, double *vt, double *rwt) {
+    if (!g || !s || !ut || !vt || !rwt || g->nx < 4 || g->ny < 1 || g->nz < 2) return -1;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    memset(ut, 0, (size_t)nz * ny * (nx + 1) * sizeof(double));
+    memset(vt, 0, (size_t)nz * (ny + 1) * nx * sizeof(double));
+    memset(rwt, 0, (size_t)(nz + 1) * ny * nx * sizeof(double));
+    const rstate S = {s->u, s->v, s->w, s->theta, s->p_total, s->p_pert, s->ph_total, s->ph_pert,
+                      s->mu_total, s->mu_pert, s->mudf};
+    large_step_coriolis(g, &S, ut, vt);
+    real *mut = zalloc((size_t)ny * nx), *muu = zalloc((size_t)ny * (nx + 1)), *muv = zalloc((size_t)(ny + 1) * nx);
+    for (size_t c = 0; c < (size_t)ny * nx; ++c) mut[c] = (s->mu_total[c] - s->mu_pert[c]) + s->mu_pert[c];
+    u_face_avg(nx, ny, mut, muu);
+    v_face_avg(nx, ny, mut, muv);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t c = I2(j, i);
+            real mxy = g->msftx[c] / g->msfty[c];
+            real ru_m[2], rv_m[2];
+            for (int k = 1; k < nz; ++k) {
+                for (int q = 0; q < 2; ++q) {
+                    int kk = k - q;
+                    real ru_l = ((g->c1h[kk] * muu[I2U(j, i)] + g->c2h[kk]) * s->u[IU(kk, j, i)]) / g->msfuy[I2U(j, i)];
+                    real ru_r = ((g->c1h[kk] * muu[I2U(j, i + 1)] + g->c2h[kk]) * s->u[IU(kk, j, i + 1)]) /
+                                g->msfuy[I2U(j, i + 1)];
+                    ru_m[q] = 0.5 * (ru_l + ru_r);
+                    real rv_s = ((g->c1h[kk] * muv[I2V(j, i)] + g->c2h[kk]) * s->v[IV(kk, j, i)]) / g->msfvx[I2V(j, i)];
+                    real rv_n = ((g->c1h[kk] * muv[I2V(j + 1, i)] + g->c2h[kk]) * s->v[IV(kk, j + 1, i)]) /
+                                g->msfvx[I2V(j + 1, i)];
+                    rv_m[q] = 0.5 * (rv_s + rv_n);
+                }
+                real ru_f = g->fnm[k] * ru_m[0] + g->fnp[k] * ru_m[1];
+                real rv_f = g->fnm[k] * rv_m[0] + g->fnp[k] * rv_m[1];
+                rwt[IM(k, j, i)] = cor_w(g, c, mxy, ru_f, rv_f);
+            }
+        }
+    free(mut);
+    free(muu);
+    free(muv);
+    return 0;
+}
 #endif
 
 /* ------------------------------------------------------------------ */
@@ -1432,6 +1560,7 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
 #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < Mn; ++q) W->alt_raw[q] = alt_from_theta_p(S->theta[q], S->p_total[q]);
     large_step_pgf(g, nl->hypsometric_opt, S, W->alt_raw, W->al_ls, W->php_ls, W->ut, W->vt, NULL);
+    large_step_coriolis(g, S, W->ut, W->vt);
 
     /* --- small_step_prep_wrf, core/small_step_prep.py:216-394 --- */
     for (int j = 0; j < ny; ++j)
@@ -1582,7 +1711,7 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
                 real rv_f = g->fnm[k] * rv_m[0] + g->fnp[k] * rv_m[1];
                 real v_f = g->fnm[k] * v_m[0] + g->fnp[k] * v_m[1];
                 real curv = RERADIUS * (ru_f * u_f + (mxy * rv_f) * v_f);
-                W->rw_tend[IM(k, j, i)] = W->rw_tend[IM(k, j, i)] + (0.0 + curv);
+                W->rw_tend[IM(k, j, i)] = W->rw_tend[IM(k, j, i)] + (cor_w(g, c, mxy, ru_f, rv_f) + curv);
             }
         }
     /* rhs_ph_wrf order-2 periodic (core/rhs_ph.py:175-398), top_lid */

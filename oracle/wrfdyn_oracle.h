@@ -41,6 +41,11 @@ typedef struct {
     const double *c3h, *c4h;                 /* nz   */
     const double *c3f, *c4f;                 /* nz+1 */
     double p_top;
+    /* Coriolis parameters on mass points (DycoreMetrics f, e, sina, cosa,
+     * contracts/grid.py:65-73): f = 2 Omega sin(lat), e = 2 Omega cos(lat)
+     * and the projection rotation.  NULL (all four) means no rotation:
+     * f = e = sina = 0, cosa = 1, and every Coriolis term is skipped. */
+    const double *f, *e, *sina, *cosa;       /* ny*nx      */
 } orc_grid;
 
 typedef struct {
@@ -125,6 +130,7 @@ int orc_advect_limited(const orc_grid *g, int opt, const double *u, const double
  * state s with K = nu, and the layer thickness dz it uses; 0 on success. */
 int orc_constk_tendency(const orc_grid *g, double nu, const orc_state *s, double *ut, double *vt, double *wt,
                         double *tht, double *dz);
+int orc_coriolis_tendency(const orc_grid *g, const orc_state *s, double *ut, double *vt, double *rwt);
 int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double *ut, double *vt, double *wt,
                       double *tht, double *xkmh);
 double orc_pow(double x, double y);

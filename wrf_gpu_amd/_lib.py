@@ -39,6 +39,7 @@ class WdGrid(ctypes.Structure):
         ("cf1", ctypes.c_double), ("cf2", ctypes.c_double), ("cf3", ctypes.c_double),
         ("msftx", _dp), ("msfty", _dp), ("msfux", _dp), ("msfuy", _dp), ("msfvx", _dp), ("msfvy", _dp),
         ("c3h", _dp), ("c4h", _dp), ("c3f", _dp), ("c4f", _dp), ("p_top", ctypes.c_double),
+        ("f", _dp), ("e", _dp), ("sina", _dp), ("cosa", _dp),
     ]
 
 

@@ -1406,7 +1406,7 @@ const char* kernel_name(int id) {
       "origin_copy", "k_ruv", "k_rom", "k_prep_col", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t",
       "k_prep_2d", "k_prep_faces", "k_prep_u", "k_prep_v", "k_col_setup",
       "k_uv", "k_column", "k_column_mu", "k_column_w", "k_fin_uv", "k_fin_col", "k_diff6", "k_smag", "k_limiter",
-      "k_constk"};
+      "k_constk", "k_coriolis"};
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 

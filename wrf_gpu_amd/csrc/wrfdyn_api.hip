@@ -280,6 +280,13 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return fail(nullptr, WD_EINVAL, "hypsometric_opt=2 needs the c3h, c4h, c3f and c4f metrics");
   if (nl->scalar_adv_opt < 0 || nl->scalar_adv_opt > 2)
     return fail(nullptr, WD_EINVAL, "scalar_adv_opt must be 0, 1 (positive-definite) or 2 (monotonic)");
+  // rotation: all four Coriolis arrays or none; it runs when f or e is nonzero
+  bool rot = false;
+  if (grid->f || grid->e || grid->sina || grid->cosa) {
+    if (!grid->f || !grid->e || !grid->sina || !grid->cosa)
+      return fail(nullptr, WD_EINVAL, "rotation needs all four of f, e, sina and cosa (or none)");
+    for (size_t q = 0; q < (size_t)ny * nx && !rot; ++q) rot = grid->f[q] != 0.0 || grid->e[q] != 0.0;
+  }
 
   wd_handle* h = new (std::nothrow) wd_handle();
   if (!h) return fail(nullptr, WD_ENOMEM, "host allocation failed");
@@ -322,6 +329,12 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   put(grid->c4h ? grid->c4h : zero_h.data(), n1);
   put(grid->c3f ? grid->c3f : one_f.data(), n1f);
   put(grid->c4f ? grid->c4f : zero_f.data(), n1f);
+  if (rot) {
+    put(grid->f, c);
+    put(grid->e, c);
+    put(grid->sina, c);
+    put(grid->cosa, c);
+  }
   h->metric_bytes = host.size() * sizeof(double);
   e = hipMalloc(&h->metric_buf, h->metric_bytes);
   if (e != hipSuccess) {
@@ -364,6 +377,11 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   g.c4h = mb + offs[o++];
   g.c3f = mb + offs[o++];
   g.c4f = mb + offs[o++];
+  g.rot = rot ? 1 : 0;
+  g.fcor = rot ? mb + offs[o++] : nullptr;
+  g.ecor = rot ? mb + offs[o++] : nullptr;
+  g.sina = rot ? mb + offs[o++] : nullptr;
+  g.cosa = rot ? mb + offs[o++] : nullptr;
   g.p_top = grid->p_top;
   g.hypsometric_opt = nl->hypsometric_opt;
   g.cf1 = grid->cf1;
@@ -818,6 +836,7 @@ int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* st
   if (h->nl.const_nu > 0.0) return fail(h, WD_EINVAL, "the adjoint does not cover the constant-K diffusion");
   if (h->nl.scalar_adv_opt != 0) return fail(h, WD_EINVAL, "the adjoint does not cover the theta limiter");
   if (h->g.hypsometric_opt == 2) return fail(h, WD_EINVAL, "the adjoint does not cover hypsometric_opt=2");
+  if (h->g.rot) return fail(h, WD_EINVAL, "the adjoint does not cover the Coriolis terms (f or e nonzero)");
   // Switched off: the first GPU run of this path ended in a memory access
   // fault whose cause is not yet found (its kernel bodies are bounds-checked
   // on the host).  It stays off the GPU until the cause is shown.

@@ -40,6 +40,9 @@ struct DGrid {
   const double *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy;
   const double *c3h, *c4h, *c3f, *c4f;  // dry-pressure coefficients (hypsometric_opt 2)
   double p_top;
+  // Coriolis parameters on mass points (null unless rot): f, e, sina, cosa
+  const double *fcor, *ecor, *sina, *cosa;
+  int rot;              // 1: f or e nonzero somewhere, the Coriolis terms run
   int hypsometric_opt;  // 2: log-pressure al (WRF calc_p_rho_phi), else linear
   int xcd_remap;  // 1: walk tiles XCD-contiguously (xcd_tile), 0: hardware order
 };

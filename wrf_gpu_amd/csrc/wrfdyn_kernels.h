@@ -34,6 +34,7 @@ enum KernelId {
   KID_SMAG,
   KID_LIMITER,
   KID_CONSTK,
+  KID_CORIOLIS,
   KID_COUNT
 };
 

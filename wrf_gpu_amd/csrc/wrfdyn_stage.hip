@@ -1057,6 +1057,7 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
   const double mut = w.mut[c], mub = w.mub[c];
   const double mu_p = mut - mub;
   const double mxy = g.msftx[c] / msfty;
+  const double e_c = g.rot ? g.ecor[c] : 0.0, cosa_c = g.rot ? g.cosa[c] : 1.0, sina_c = g.rot ? g.sina[c] : 0.0;
   const double muu_w = w.muu[fuw], muu_e = w.muu[fue];
   const double msfuy_w = g.msfuy[fuw], msfuy_e = g.msfuy[fue];
   const double muv_s = w.muv[fvs], muv_n = w.muv[fvn];
@@ -1161,7 +1162,9 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
       double rv_f = g.fnm[f] * cv0.rv_m + g.fnp[f] * cv_m.rv_m;
       double v_f = g.fnm[f] * cv0.v_m + g.fnp[f] * cv_m.v_m;
       double curv = kReradius * (ru_f * u_f + (mxy * rv_f) * v_f);
-      if (owned) sto(w.rw_tend, q, rw + (0.0 + curv));
+      // cosine-Coriolis (operational_mode.py:1818-1821)
+      const double cor = g.rot ? e_c * (cosa_c * ru_f - (mxy * sina_c) * rv_f) : 0.0;
+      if (owned) sto(w.rw_tend, q, rw + (cor + curv));
       cv_m = cv0;
     }
     // ---- ph_tend (rhs_ph) ----
@@ -1791,6 +1794,101 @@ __global__ void __launch_bounds__(256) k_lim_tend(DGrid g, DState s, DWork w, in
   stt<TT>(w.tht, off, add_diff(t, w.d6t, w.smt, off));
 }
 
+// ---------------------------------------------------------------------------
+// large_step_coriolis (core/rk_addtend_dry.py:364-483), added to the coupled
+// u and v tendencies after the large-step PGF (operational_mode.py:3052-3058),
+// periodic run (no specified-boundary edge mask).  One thread per u-face
+// column (blocks [0, nbu)) or v-face column (the rest), marching k:
+//   u face: ut += (msfux/msfuy) f_u rv_quad - (e_u cosa_u) rw_quad
+//   v face: vt += (-(msfvy/msfvx) f_v) ru_quad + ((msfvy/msfvx) e_v sina_v) rw_quad
+// rv_quad / ru_quad average the coupled momenta of the four surrounding
+// faces, rw_quad the coupled w of the two mass columns at levels k and k+1;
+// x and y neighbours are clamped at the array edges, and the face masses are
+// the edge-padded face averages of the stage mu_total.  Only launched when
+// the grid rotates (f or e nonzero).
+// ---------------------------------------------------------------------------
+template <class TT>
+__global__ void __launch_bounds__(256) k_coriolis(DGrid g, DState s, DWork w, unsigned nbu) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t pl = (size_t)ny * nx, upl = (size_t)ny * (nx + 1), vpl = (size_t)(ny + 1) * nx;
+  const bool ufaces = blockIdx.x < nbu;
+  const size_t t = (size_t)(ufaces ? blockIdx.x : blockIdx.x - nbu) * blockDim.x + threadIdx.x;
+  if (ufaces ? t >= upl : t >= vpl) return;
+  auto rw_at = [&](int k, size_t c) {
+    return (s.w[(size_t)k * pl + c] * (g.c1f[k] * s.mu_total[c] + g.c2f[k])) / g.msfty[c];
+  };
+  if (ufaces) {
+    const int j = (int)(t / (size_t)(nx + 1)), i = (int)(t % (size_t)(nx + 1));
+    const int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
+    const size_t cl = i2(g, j, il), cr = i2(g, j, ir);
+    const double msf_u = g.msfux[t] / g.msfuy[t];
+    const double f_u = 0.5 * (g.fcor[cl] + g.fcor[cr]);
+    const double e_u = 0.5 * (g.ecor[cl] + g.ecor[cr]);
+    const double cosa_u = 0.5 * (g.cosa[cl] + g.cosa[cr]);
+    // the four v faces (rows j, j+1; columns il, ir), their face mu and msfvx
+    size_t fv[4];
+    double muv[4], mvx[4];
+    for (int q = 0; q < 4; ++q) {
+      const int jj = j + (q >> 1), ii = (q & 1) ? ir : il;
+      fv[q] = i2v(g, jj, ii);
+      muv[q] = 0.5 * (s.mu_total[i2(g, clampi(jj - 1, 0, ny - 1), ii)] + s.mu_total[i2(g, clampi(jj, 0, ny - 1), ii)]);
+      mvx[q] = g.msfvx[fv[q]];
+    }
+    double rwl = rw_at(0, cl), rwr = rw_at(0, cr);
+    for (int k = 0; k < nz; ++k) {
+      double rv[4];
+      for (int q = 0; q < 4; ++q) rv[q] = (s.v[(size_t)k * vpl + fv[q]] * (g.c1h[k] * muv[q] + g.c2h[k])) / mvx[q];
+      const double rv_quad = 0.25 * (((rv[0] + rv[1]) + rv[2]) + rv[3]);
+      const double rwl1 = rw_at(k + 1, cl), rwr1 = rw_at(k + 1, cr);
+      const double rw_quad = 0.25 * (((rwl + rwl1) + rwr) + rwr1);
+      double cor = (msf_u * f_u) * rv_quad;
+      cor = cor - (e_u * cosa_u) * rw_quad;
+      const unsigned off = (unsigned)((size_t)k * upl + t) * 8u;
+      stt<TT>(w.ut, off, ldt<TT>(w.ut, off) + cor);
+      rwl = rwl1;
+      rwr = rwr1;
+    }
+  } else {
+    const int j = (int)(t / (size_t)nx), i = (int)(t % (size_t)nx);
+    const int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
+    const size_t cs = i2(g, js, i), cn = i2(g, jn, i);
+    const double msf_v = g.msfvy[t] / g.msfvx[t];
+    const double f_v = 0.5 * (g.fcor[cs] + g.fcor[cn]);
+    const double e_v = 0.5 * (g.ecor[cs] + g.ecor[cn]);
+    const double sina_v = 0.5 * (g.sina[cs] + g.sina[cn]);
+    // the four u faces (rows js, jn; columns i, i+1), their face mu and msfuy
+    size_t fu[4];
+    double muu[4], muy[4];
+    for (int q = 0; q < 4; ++q) {
+      const int jj = (q >> 1) ? jn : js, ii = i + (q & 1);
+      fu[q] = i2u(g, jj, ii);
+      muu[q] = 0.5 * (s.mu_total[i2(g, jj, clampi(ii - 1, 0, nx - 1))] + s.mu_total[i2(g, jj, clampi(ii, 0, nx - 1))]);
+      muy[q] = g.msfuy[fu[q]];
+    }
+    double rws = rw_at(0, cs), rwn = rw_at(0, cn);
+    for (int k = 0; k < nz; ++k) {
+      double ru[4];
+      for (int q = 0; q < 4; ++q) ru[q] = (s.u[(size_t)k * upl + fu[q]] * (g.c1h[k] * muu[q] + g.c2h[k])) / muy[q];
+      const double ru_quad = 0.25 * (((ru[0] + ru[1]) + ru[2]) + ru[3]);
+      const double rws1 = rw_at(k + 1, cs), rwn1 = rw_at(k + 1, cn);
+      const double rw_quad = 0.25 * (((rws + rws1) + rwn) + rwn1);
+      double cor = ((-msf_v) * f_v) * ru_quad;
+      cor = cor + ((msf_v * e_v) * sina_v) * rw_quad;
+      const unsigned off = (unsigned)((size_t)k * vpl + t) * 8u;
+      stt<TT>(w.vt, off, ldt<TT>(w.vt, off) + cor);
+      rws = rws1;
+      rwn = rwn1;
+    }
+  }
+}
+
+template <class TT>
+void launch_coriolis(const DGrid& g, const DState& s, const DWork& w, hipStream_t st) {
+  const unsigned bs = 256;
+  const unsigned nbu = blocks_for((size_t)g.ny * (g.nx + 1), bs), nbv = blocks_for((size_t)(g.ny + 1) * g.nx, bs);
+  k_coriolis<TT><<<nbu + nbv, bs, 0, st>>>(g, s, w, nbu);
+}
+
 template <class TT>
 void launch_limiter(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
   const dim3 bs(kFieldBx, kFieldBy);
@@ -1840,6 +1938,13 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
       WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<double>(g, nl, s, w, st)));
     else
       WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
+  }
+  // the Coriolis force joins the coupled u/v tendencies after the PGF
+  if (g.rot) {
+    if (nl.mixed)
+      WD_TIMED(prof, KID_CORIOLIS, st, (launch_coriolis<float>(g, s, w, st)));
+    else
+      WD_TIMED(prof, KID_CORIOLIS, st, (launch_coriolis<double>(g, s, w, st)));
   }
   WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
   WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<dim3(blocks_for(CU, bs), kchunks), bs, 0, st>>>(g, s, w)));

@@ -70,6 +70,8 @@ class Dycore:
             _c_array(m.msfuy, keep), _c_array(m.msfvx, keep), _c_array(m.msfvy, keep),
             _c_array(m.c3h, keep), _c_array(m.c4h, keep), _c_array(m.c3f, keep), _c_array(m.c4f, keep),
             float(m.p_top),
+            *((_c_array(m.f, keep), _c_array(m.e, keep), _c_array(m.sina, keep), _c_array(m.cosa, keep))
+              if m.rotating else (None,) * 4),
         )
         nl = namelist
         self._nl = _lib.WdNamelist(float(nl.dt_s), int(nl.acoustic_substeps), float(nl.epssm),

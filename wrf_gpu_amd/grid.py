@@ -56,12 +56,39 @@ class Metrics:
     c4h: np.ndarray | None = None
     c3f: np.ndarray | None = None
     c4f: np.ndarray | None = None
+    # Coriolis parameters on mass points (ny, nx) (contracts/grid.py:65-73):
+    # f = 2 Omega sin(lat), e = 2 Omega cos(lat), and the projection rotation
+    # sina/cosa; None gives no rotation (f = e = sina = 0, cosa = 1), the
+    # idealized default under which every Coriolis term vanishes
+    f: np.ndarray | None = None
+    e: np.ndarray | None = None
+    sina: np.ndarray | None = None
+    cosa: np.ndarray | None = None
 
     def __post_init__(self) -> None:
         for name, n, v in (("c3h", self.nz, 1.0), ("c4h", self.nz, 0.0), ("c3f", self.nz + 1, 1.0),
                            ("c4f", self.nz + 1, 0.0)):
             if getattr(self, name) is None:
                 object.__setattr__(self, name, np.full(n, v, dtype=np.float64))
+        for name, v in (("f", 0.0), ("e", 0.0), ("sina", 0.0), ("cosa", 1.0)):
+            a = getattr(self, name)
+            if a is None:
+                a = np.full((self.ny, self.nx), v, dtype=np.float64)
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                if a.shape != (self.ny, self.nx):
+                    # a no-rotation default carried over by dataclasses.replace to another grid
+                    if np.all(a == v):
+                        a = np.full((self.ny, self.nx), v, dtype=np.float64)
+                    else:
+                        raise ValueError(f"{name} must have the mass-point shape {(self.ny, self.nx)}, not {a.shape}")
+            object.__setattr__(self, name, a)
+
+    @property
+    def rotating(self) -> bool:
+        """True when any Coriolis term can be nonzero (f or e nonzero somewhere)."""
+
+        return bool(np.any(self.f != 0.0) or np.any(self.e != 0.0))
 
     @classmethod
     def flat(cls, *, nx: int, ny: int, nz: int, dx: float, dy: float, eta: np.ndarray,
