@@ -34,30 +34,80 @@ def test_library_exports_every_declared_symbol():
         assert re.search(rf"\bT {name}\b", out), name
 
 
-def test_ctypes_structs_match_c_layout():
-    src = r"""
-    #include <stdio.h>
-    #include <stddef.h>
-    #include "wrfdyn.h"
-    int main(void) {
-      printf("%zu %zu %zu\n", sizeof(wd_grid), sizeof(wd_namelist), sizeof(wd_state));
-      printf("%zu %zu %zu %zu %zu %zu\n", offsetof(wd_grid, cf1), offsetof(wd_grid, msfvy),
-             offsetof(wd_namelist, dampcoef), offsetof(wd_state, mudf), offsetof(wd_namelist, diff_6th_factor),
-             offsetof(wd_namelist, c_s));
-      return 0;
-    }
-    """
+def _c_layout(structs):
+    """sizeof and every field offset of the named C structs, from the header."""
+
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "wrfdyn.h"', "int main(void) {"]
+    for cname, fields in structs.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in fields:
+            lines.append(f'  printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "t.c")
         exe = os.path.join(d, "t")
-        open(c, "w").write(src)
+        open(c, "w").write("\n".join(lines))
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
-        sizes, offs = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.splitlines()
-    s = [int(x) for x in sizes.split()]
-    o = [int(x) for x in offs.split()]
-    assert s == [ctypes.sizeof(_lib.WdGrid), ctypes.sizeof(_lib.WdNamelist), ctypes.sizeof(_lib.WdState)]
-    assert o == [_lib.WdGrid.cf1.offset, _lib.WdGrid.msfvy.offset, _lib.WdNamelist.dampcoef.offset,
-                 _lib.WdState.mudf.offset, _lib.WdNamelist.diff_6th_factor.offset, _lib.WdNamelist.c_s.offset]
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    return {k: int(v) for k, v in (ln.split() for ln in out.splitlines())}
+
+
+def _py_layout(structs):
+    out = {}
+    for cname, cls in structs.items():
+        out[cname] = ctypes.sizeof(cls)
+        for f, _ in cls._fields_:
+            out[f"{cname}.{f}"] = getattr(cls, f).offset
+    return out
+
+
+def _header_fields(cname):
+    """The member names of a typedef struct in include/wrfdyn.h, in order."""
+
+    text = open(HEADER).read()
+    body = re.search(r"typedef struct \{([^}]*)\}\s*" + cname + ";", text).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        for part in decl.split(","):
+            names.append(re.findall(r"[A-Za-z_][A-Za-z0-9_]*", part)[-1])
+    return names
+
+
+def test_ctypes_structs_match_c_layout():
+    """_lib.py's structs have the header's members, in order, at the header's offsets."""
+
+    py = {"wd_grid": _lib.WdGrid, "wd_namelist": _lib.WdNamelist, "wd_state": _lib.WdState}
+    for cname, cls in py.items():
+        assert [f for f, _ in cls._fields_] == _header_fields(cname), cname
+    c = _c_layout({k: [f for f, _ in v._fields_] for k, v in py.items()})
+    assert _py_layout(py) == c
+
+
+def _documented_binding():
+    """The ctypes structs of the reference-side binding shown in INTEGRATION.md."""
+
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    code = re.search(r"```python\n(.*?)```", text, re.S).group(1)
+    defs = code[code.index("_dp = "):code.index("_lib.wd_create.argtypes")]
+    ns = {"ctypes": ctypes}
+    exec(defs, ns)
+    return code, {k: ns[k] for k in ("wd_grid", "wd_namelist", "wd_state")}
+
+
+def test_documented_binding_matches_c_layout():
+    """A binding copied from INTEGRATION.md has the header's members and layout, and its
+    create() fills every namelist member."""
+
+    code, doc = _documented_binding()
+    for cname, cls in doc.items():
+        assert [f for f, _ in cls._fields_] == _header_fields(cname), cname
+    assert _py_layout(doc) == _c_layout({k: [f for f, _ in v._fields_] for k, v in doc.items()})
+    call = re.search(r"nl = wd_namelist\((.*?)\)\n", code, re.S).group(1)
+    assert set(re.findall(r"(\w+)=", call)) == {f for f, _ in doc["wd_namelist"]._fields_}
 
 
 def test_create_rejects_bad_arguments_without_touching_the_device():

@@ -19,7 +19,7 @@ _MAPS = {"MAPFAC_MX": "msftx", "MAPFAC_MY": "msfty", "MAPFAC_UX": "msfux", "MAPF
          "MAPFAC_VX": "msfvx", "MAPFAC_VY": "msfvy"}
 
 
-def _write(path, case, dtype="f8", drop=None):
+def _write(path, case, dtype="f8", drop=None, extra=None, attrs=None):
     from scipy.io import netcdf_file
 
     m, s = case.metrics, case.state
@@ -54,8 +54,12 @@ def _write(path, case, dtype="f8", drop=None):
     put("PHB", ("Time", "bottom_top_stag", "south_north", "west_east"), s["ph_total"] - s["ph_perturbation"])
     put("MU", ("Time", "south_north", "west_east"), s["mu_perturbation"])
     put("MUB", ("Time", "south_north", "west_east"), s["mu_total"] - s["mu_perturbation"])
+    for name, a in (extra or {}).items():
+        put(name, ("Time", "south_north", "west_east"), a, "f8")
     f.DX = np.float32(m.dx)
     f.DY = np.float32(m.dy)
+    for k, v in (attrs or {}).items():
+        setattr(f, k, v)
     f.close()
     return path
 
@@ -101,3 +105,33 @@ def test_a_file_without_the_vertical_coordinate_is_refused(case, tmp_path):
 def test_missing_variables_are_refused(case, tmp_path, drop):
     with pytest.raises(ValueError, match=drop):
         wrfinput.read_wrfinput(_write(tmp_path / "wrfinput_d01", case, drop=drop))
+
+
+def test_coriolis_parameters_are_read(case, tmp_path):
+    ny, nx = case.metrics.ny, case.metrics.nx
+    rng = np.random.default_rng(3)
+    F, E = 1e-4 + 1e-6 * rng.random((ny, nx)), 5e-5 + 1e-6 * rng.random((ny, nx))
+    a = 0.1 * rng.random((ny, nx))
+    p = _write(tmp_path / "wrfinput_d01", case, extra={"F": F, "E": E, "SINALPHA": np.sin(a), "COSALPHA": np.cos(a)},
+               attrs={"MAP_PROJ": np.int32(0)})
+    _, m = wrfinput.read_wrfinput(p)
+    assert m.rotating
+    for k, want in (("f", F), ("e", E), ("sina", np.sin(a)), ("cosa", np.cos(a))):
+        assert np.array_equal(getattr(m, k), want), k
+    # without F and E the reference's fallback: 2 Omega sin / cos of XLAT
+    lat = 30.0 + rng.random((ny, nx))
+    _, m = wrfinput.read_wrfinput(_write(tmp_path / "wrfinput_d02", case, extra={"XLAT": lat}))
+    assert np.allclose(m.f, 2 * 7.2921e-5 * np.sin(np.deg2rad(lat)), rtol=1e-15)
+    assert np.allclose(m.e, 2 * 7.2921e-5 * np.cos(np.deg2rad(lat)), rtol=1e-15)
+    assert not np.any(m.sina) and np.all(m.cosa == 1.0)
+    # no rotation data at all: no rotation
+    _, m = wrfinput.read_wrfinput(_write(tmp_path / "wrfinput_d03", case))
+    assert not m.rotating
+
+
+def test_a_real_data_file_is_refused_unless_stepped_as_periodic(case, tmp_path):
+    p = _write(tmp_path / "wrfinput_d01", case, attrs={"MAP_PROJ": np.int32(1)})
+    with pytest.raises(ValueError, match="MAP_PROJ"):
+        wrfinput.read_wrfinput(p)
+    st, m = wrfinput.read_wrfinput(p, assume_periodic=True)
+    assert st["u"].shape == case.state["u"].shape

@@ -18,9 +18,18 @@ The metrics come from the vertical coordinate arrays WRF writes (``ZNW``,
 ``FNM``, ``FNP``, ``CF1``, ``CF2``, ``CF3``, ``P_TOP``, and ``C3H``, ``C4H``,
 ``C3F``, ``C4F`` when present, for hypsometric_opt 2), the six map factors
 ``MAPFAC_{M,U,V}{X,Y}`` and the ``DX``/``DY`` global attributes, with WRF's
-signs (contracts/grid.py:64-164).  Files must be classic netCDF (CDF-1 or
+signs (contracts/grid.py:64-164).  The Coriolis parameters ``F``, ``E``,
+``SINALPHA`` and ``COSALPHA`` are read as the reference reads them
+(dynamics/metrics.py:47-79): a missing ``F`` or ``E`` falls back to
+2 Omega sin/cos of ``XLAT`` when the file has it, else to 0, and a missing
+``SINALPHA``/``COSALPHA`` to 0 and 1.  Files must be classic netCDF (CDF-1 or
 CDF-2), which ``scipy.io.netcdf_file`` reads; netCDF4/HDF5 files need
 converting first (``nccopy -k classic``).
+
+The step runs periodic lateral boundaries.  A real-data file (global
+attribute ``MAP_PROJ`` nonzero: a map projection, whose lateral boundaries
+WRF specifies from ``wrfbdy``) is refused unless the caller asks for it to be
+stepped as a periodic domain (``assume_periodic=True``).
 """
 
 from __future__ import annotations
@@ -41,6 +50,10 @@ _VERTICAL = {  # Metrics field: (WRF variable, length is nz + 1)
 _MAPFAC = {"msftx": "MAPFAC_MX", "msfty": "MAPFAC_MY", "msfux": "MAPFAC_UX", "msfuy": "MAPFAC_UY",
            "msfvx": "MAPFAC_VX", "msfvy": "MAPFAC_VY"}
 _SCALARS = {"cf1": "CF1", "cf2": "CF2", "cf3": "CF3", "p_top": "P_TOP"}
+# Metrics field: (WRF variable, value when absent)
+_CORIOLIS = {"f": ("F", 0.0), "e": ("E", 0.0), "sina": ("SINALPHA", 0.0), "cosa": ("COSALPHA", 1.0)}
+# 2 Omega of WRF's EARTH_OMEGA, for F and E from XLAT (dynamics/metrics.py:44)
+_TWO_OMEGA = 2.0 * 7.2921e-5
 
 
 def _var(f, name: str, t: int) -> np.ndarray:
@@ -62,13 +75,21 @@ def _gattr(f, name: str) -> float:
     return float(np.asarray(v).reshape(-1)[0])
 
 
-def read_wrfinput(path: str | Path, time_index: int = 0, device: str | None = None) -> tuple[dict, Metrics]:
+def read_wrfinput(path: str | Path, time_index: int = 0, device: str | None = None,
+                  assume_periodic: bool = False) -> tuple[dict, Metrics]:
     """``(state, metrics)`` from time ``time_index`` of a WRF netCDF file;
-    the state as float64 numpy arrays, or tensors on ``device``."""
+    the state as float64 numpy arrays, or tensors on ``device``.  A real-data
+    file (``MAP_PROJ`` nonzero) raises ValueError unless ``assume_periodic``."""
 
     from scipy.io import netcdf_file
 
     with netcdf_file(str(path), "r", mmap=False) as f:
+        map_proj = f._attributes.get("MAP_PROJ")
+        if map_proj is not None and int(np.asarray(map_proj).reshape(-1)[0]) != 0 and not assume_periodic:
+            raise ValueError(
+                "the file is a real-data case (MAP_PROJ != 0), whose lateral boundaries WRF specifies from "
+                "wrfbdy; this step runs periodic lateral boundaries only (pass assume_periodic=True to step "
+                "it as a periodic domain anyway)")
         dims = f.dimensions
         try:
             nx, ny, nz = int(dims["west_east"]), int(dims["south_north"]), int(dims["bottom_top"])
@@ -91,7 +112,20 @@ def read_wrfinput(path: str | Path, time_index: int = 0, device: str | None = No
                     raise ValueError(f"{name} has {a.shape[0]} levels, the grid has {nz} (+1 staggered)")
                 vert[k] = a
         scal = {k: float(_var(f, name, t).reshape(-1)[0]) for k, name in _SCALARS.items()}
-        metrics = Metrics(nx=nx, ny=ny, nz=nz, dx=_gattr(f, "DX"), dy=_gattr(f, "DY"), **scal, **vert, **maps)
+        lat = np.deg2rad(_var(f, "XLAT", t)) if "XLAT" in f.variables else None
+        rot = {}
+        for k, (name, default) in _CORIOLIS.items():
+            if name in f.variables:
+                a = _var(f, name, t)
+            elif k in ("f", "e") and lat is not None:
+                a = _TWO_OMEGA * (np.sin(lat) if k == "f" else np.cos(lat))
+            else:
+                a = np.full((ny, nx), default)
+            if a.shape != (ny, nx):
+                raise ValueError(f"{name} has shape {a.shape}, the grid needs {(ny, nx)}")
+            rot[k] = np.ascontiguousarray(a, dtype=np.float64)
+        metrics = Metrics(nx=nx, ny=ny, nz=nz, dx=_gattr(f, "DX"), dy=_gattr(f, "DY"), **scal, **vert, **maps,
+                          **rot)
         u, v, w = _var(f, "U", t), _var(f, "V", t), _var(f, "W", t)
         T, P, PB = _var(f, "T", t), _var(f, "P", t), _var(f, "PB", t)
         PH, PHB, MU, MUB = _var(f, "PH", t), _var(f, "PHB", t), _var(f, "MU", t), _var(f, "MUB", t)
