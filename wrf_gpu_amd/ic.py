@@ -153,6 +153,37 @@ def warm_bubble_2d() -> Case:
                 {"x_m": x, "z_m": z, "snapshot_steps": (1000, 2500, 5000)})
 
 
+def density_current_2d() -> Case:
+    """The reference's 2-D Straka density current slab (idealized.py:277-330, :566-602).
+
+    500 x 60 (ny = 1), dx = dz = 100 m, x from -25 km; a -15 K cos^2 cold bubble of
+    radii 4 km (x) and 2 km (z) centred at (0, 3 km) on the neutral theta0 = 300 K
+    base; dt = 0.1 s, 10 acoustic substeps, rigid lid, w_damping=1, damp_opt=3,
+    dampcoef=0.2, zdamp=3000 m, constant-K diffusion with K = 75 m^2/s.
+    """
+
+    dx = 100.0
+    nx, nz = 500, 60
+    x = -25000.0 + (np.arange(nx) + 0.5) * dx
+    z = (np.arange(nz) + 0.5) * dx
+    z_face = np.arange(nz + 1) * dx
+    xx, zz = np.meshgrid(x, z)
+    radius = np.sqrt((xx / 4000.0) ** 2 + ((zz - 3000.0) / 2000.0) ** 2)
+    theta_prime = -15.0 * centered_cosine(radius)
+    theta_prime *= 15.0 / abs(float(np.min(theta_prime)))
+    eta, p_mass, _ph_face, mu = uniform_z_hydrostatic_base(z_face, THETA0)
+    p_top = float(P0 - mu)
+    metrics = Metrics.flat(nx=nx, ny=1, nz=nz, dx=dx, dy=dx, eta=eta, p_top=p_top)
+    theta = (THETA0 + theta_prime)[:, None, :]
+    pb = np.broadcast_to(p_mass[:, None, None], (nz, 1, nx)).copy()
+    alb = np.broadcast_to(alpha_dry(np.full(nz, THETA0), p_mass)[:, None, None], (nz, 1, nx)).copy()
+    state = _balanced_state(metrics, mub=np.full((1, nx), mu), pb=pb, alb=alb, theta_full=theta,
+                            phb_surface=np.zeros((1, nx)))
+    nl = Namelist(dt_s=0.1, acoustic_substeps=10, zdamp=3000.0, const_nu_m2_s=75.0)
+    return Case("density_current_2d", metrics, nl, state, 9000,
+                {"x_m": x, "z_m": z, "snapshot_steps": (9000,)})
+
+
 def _column_base(metrics: Metrics, mub, phb_surface, theta_of_z, iterations=8, cp=CP_DRY):
     """Hydrostatic base columns on the eta levels for a theta(z) profile.
 
