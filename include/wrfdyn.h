@@ -190,16 +190,17 @@ void wd_batch_destroy(wd_batch *b);
  * adjoint of every array of `start`, each of the eleven arrays an
  * independent input.  The step is recomputed from `start` with the forward
  * kernels, and the arrays each adjoint kernel reads are kept on a tape in
- * device memory (wd_adjoint_tape_bytes; allocated on the first call, which
- * therefore must not be inside a graph capture).  Sums of contributions use
+ * device memory (wd_adjoint_tape_bytes; allocated on the first call).  Sums of contributions use
  * atomic adds, so results agree with the CPU gradient to rounding, not bit
  * for bit.  The handle's workspace is overwritten.  Asynchronous on
  * `stream`; WD_EINVAL in the mixed form or when a start and a bar buffer
  * coincide.  For n steps, keep the state at the start of every step and
- * call this from the last step back to the first.
+ * call this from the last step back to the first.  It enqueues memsets, so
+ * it must not be captured into a graph.
  * Currently switched off: it returns WD_EINVAL with a message, because its
  * first GPU run ended in a memory access fault whose cause is not yet found;
- * the kernel bodies it runs are checked on the host. */
+ * the kernel bodies and the whole reverse launch sequence are checked on the
+ * host. */
 int wd_step_adjoint(wd_handle *h, const wd_state *start, wd_state *bar, void *stream);
 size_t wd_adjoint_tape_bytes(const wd_handle *h);
 

@@ -20,56 +20,19 @@
 // is the same to rounding, not bit for bit, from run to run.
 #include <vector>
 
-#include "wrfdyn_adjoint.h"
+#include "wrfdyn_adjoint_glue.h"
 #include "wrfdyn_kernels.h"
 
 namespace wd {
 
 using namespace adj;
+using namespace adjg;
 
 namespace {
 
 struct AtomicAdd {
   __device__ void operator()(double* p, double v) const { atomicAdd(p, v); }
 };
-
-enum Ext : int { E_M, E_W, E_U, E_V, E_C, E_CU, E_CV };
-// extents of the AdjArr arrays, in their order
-constexpr int kExtent[A_count] = {
-    E_U, E_V, E_W, E_M, E_M, E_M, E_W, E_W, E_C, E_C, E_C,
-    E_U, E_V, E_W, E_M, E_W, E_C, E_M, E_M, E_W, E_M, E_U, E_V, E_W, E_M,
-    E_M, E_M, E_M, E_C, E_C, E_C, E_C, E_CU, E_CU, E_CV, E_CV,
-    E_U, E_V, E_W, E_M, E_W, E_M, E_M, E_M, E_W, E_M, E_M, E_M,
-    E_M, E_W, E_W, E_W, E_W, E_W, E_C, E_C, E_C, E_W, E_M,
-    E_M, E_C, E_W, E_W, E_W, E_W};
-
-size_t ext_len(const DGrid& g, int e) {
-  const size_t nx = g.nx, ny = g.ny, nz = g.nz;
-  switch (e) {
-    case E_M: return nz * ny * nx;
-    case E_W: return (nz + 1) * ny * nx;
-    case E_U: return nz * ny * (nx + 1);
-    case E_V: return nz * (ny + 1) * nx;
-    case E_C: return ny * nx;
-    case E_CU: return ny * (nx + 1);
-    default: return (ny + 1) * nx;
-  }
-}
-
-// The state fields kept per stage (mudf changes inside a stage and is carried).
-constexpr int kStageState[] = {A_u, A_v, A_w, A_theta, A_p_total, A_p_pert, A_ph_total, A_ph_pert, A_mu_total,
-                               A_mu_pert};
-// Work arrays that the setup computes and the substeps and finish only read.
-constexpr int kStageConst[] = {A_u0, A_v0, A_w0, A_th0, A_ph0, A_mu0, A_ru, A_rv, A_rom, A_ut, A_vt, A_wt,
-                               A_tht, A_alt_raw, A_al_ls, A_php_ls, A_mub, A_mut, A_muts0, A_mu_work0, A_muu,
-                               A_muus, A_muv, A_muvs, A_c2a, A_alt, A_pb, A_php, A_rw_tend, A_ph_tend, A_ca,
-                               A_calpha, A_cgamma, A_phb};
-// Arrays that change from substep to substep.
-constexpr int kCarried[] = {A_u_work, A_v_work, A_w_work, A_th_work, A_ph_work, A_p, A_al, A_pm1,
-                            A_mu, A_muts, A_muave, A_ww, A_t2ave, A_mudf};
-constexpr int kNStageState = sizeof(kStageState) / sizeof(int);
-constexpr int kNStageConst = sizeof(kStageConst) / sizeof(int);
-constexpr int kNCarried = sizeof(kCarried) / sizeof(int);
 
 double* dwork_ptr(const DWork& w, int a) {
   switch (a) {
@@ -140,190 +103,64 @@ double* state_ptr(const DState& s, int a) {
   }
 }
 
-constexpr unsigned kBs = 256;
-unsigned nblocks(size_t n) { return (unsigned)((n + kBs - 1) / kBs); }
-
-// ---- kernels: one thread per column / face column / point ----
 __global__ void __launch_bounds__(kBs) k_adj_phb(size_t n, const double* pht, const double* php, double* out) {
   const size_t q = (size_t)blockIdx.x * kBs + threadIdx.x;
   if (q < n) out[q] = pht[q] - php[q];
 }
-__global__ void __launch_bounds__(kBs) k_adj_origin(View V, int src, int dst, size_t n) {
-  const size_t q = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (q < n) adj_origin_elem(V, src, dst, q);
+
+// One reverse kernel: thread t of the launch runs rev_thread (the per-thread
+// index map shared with the host replay), bars shared by neighbouring
+// threads through atomic adds.
+template <AK K>
+__global__ void __launch_bounds__(kBs) k_adj_rev(RevArgs a) {
+  rev_thread(K, a, (size_t)blockIdx.x * kBs + threadIdx.x, AtomicAdd{});
 }
-__global__ void __launch_bounds__(kBs) k_adj_finish_col(DGrid g, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_finish_column(g, V, (int)(c / g.nx), (int)(c % g.nx));
-}
-// u faces (ny x (nx+1)) then v faces ((ny+1) x nx)
-__device__ inline bool face_of(const DGrid& g, size_t f, bool* ydir, int* j, int* i) {
-  const size_t nu = (size_t)g.ny * (g.nx + 1);
-  if (f < nu) {
-    *ydir = false;
-    *j = (int)(f / (g.nx + 1));
-    *i = (int)(f % (g.nx + 1));
-    return true;
+
+struct GpuLaunch {
+  hipStream_t st;
+  hipError_t err = hipSuccess;
+  bool operator()(AK k, unsigned nb, const RevArgs& a) {
+    switch (k) {
+      case AK::origin: k_adj_rev<AK::origin><<<nb, kBs, 0, st>>>(a); break;
+      case AK::finish_col: k_adj_rev<AK::finish_col><<<nb, kBs, 0, st>>>(a); break;
+      case AK::finish_face: k_adj_rev<AK::finish_face><<<nb, kBs, 0, st>>>(a); break;
+      case AK::p: k_adj_rev<AK::p><<<nb, kBs, 0, st>>>(a); break;
+      case AK::w: k_adj_rev<AK::w><<<nb, kBs, 0, st>>>(a); break;
+      case AK::mu_theta: k_adj_rev<AK::mu_theta><<<nb, kBs, 0, st>>>(a); break;
+      case AK::uv: k_adj_rev<AK::uv><<<nb, kBs, 0, st>>>(a); break;
+      case AK::s1: k_adj_rev<AK::s1><<<nb, kBs, 0, st>>>(a); break;
+      case AK::s2: k_adj_rev<AK::s2><<<nb, kBs, 0, st>>>(a); break;
+      case AK::s3: k_adj_rev<AK::s3><<<nb, kBs, 0, st>>>(a); break;
+      case AK::s4: k_adj_rev<AK::s4><<<nb, kBs, 0, st>>>(a); break;
+      case AK::s4_clear: k_adj_rev<AK::s4_clear><<<nb, kBs, 0, st>>>(a); break;
+      case AK::pgf: k_adj_rev<AK::pgf><<<nb, kBs, 0, st>>>(a); break;
+      case AK::s6: k_adj_rev<AK::s6><<<nb, kBs, 0, st>>>(a); break;
+      case AK::adv_copy: k_adj_rev<AK::adv_copy><<<nb, kBs, 0, st>>>(a); break;
+      case AK::adv_h: k_adj_rev<AK::adv_h><<<nb, kBs, 0, st>>>(a); break;
+      case AK::adv_v: k_adj_rev<AK::adv_v><<<nb, kBs, 0, st>>>(a); break;
+      case AK::couple_col: k_adj_rev<AK::couple_col><<<nb, kBs, 0, st>>>(a); break;
+      case AK::couple_pt: k_adj_rev<AK::couple_pt><<<nb, kBs, 0, st>>>(a); break;
+    }
+    err = hipGetLastError();
+    return err == hipSuccess;
   }
-  f -= nu;
-  if (f >= (size_t)(g.ny + 1) * g.nx) return false;
-  *ydir = true;
-  *j = (int)(f / g.nx);
-  *i = (int)(f % g.nx);
-  return true;
-}
-size_t n_faces(const DGrid& g) { return (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx; }
-
-__global__ void __launch_bounds__(kBs) k_adj_finish_face(DGrid g, View V) {
-  bool yd;
-  int j, i;
-  if (face_of(g, (size_t)blockIdx.x * kBs + threadIdx.x, &yd, &j, &i)) adj_finish_face(g, V, yd, j, i);
-}
-__global__ void __launch_bounds__(kBs) k_adj_p(DGrid g, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_p_column(g, V, (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_w(DGrid g, DNamelist nl, double dts, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_w_column(g, nl, dts, V, (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_mu_theta(DGrid g, DNamelist nl, double dts, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_mu_theta_column(g, nl, dts, V, (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_uv(DGrid g, double dts, View V) {
-  bool yd;
-  int j, i;
-  if (face_of(g, (size_t)blockIdx.x * kBs + threadIdx.x, &yd, &j, &i)) adj_uv_face(g, dts, V, yd, j, i, AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_s1(DGrid g, DNamelist nl, double dts, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_setup_s1_column(g, nl, dts, V, (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_s2(DGrid g, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_setup_s2_column(g, V, (int)(c / g.nx), (int)(c % g.nx));
-}
-__global__ void __launch_bounds__(kBs) k_adj_s3(DGrid g, View V) {
-  bool yd;
-  int j, i;
-  if (face_of(g, (size_t)blockIdx.x * kBs + threadIdx.x, &yd, &j, &i)) adj_setup_s3_face(g, V, yd, j, i);
-}
-__global__ void __launch_bounds__(kBs) k_adj_s4(DGrid g, int rk_step, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_setup_s4_column(g, rk_step, V, (int)(c / g.nx), (int)(c % g.nx));
-}
-__global__ void __launch_bounds__(kBs) k_adj_s4_clear(DGrid g, View V) {
-  bool yd;
-  int j, i;
-  if (face_of(g, (size_t)blockIdx.x * kBs + threadIdx.x, &yd, &j, &i)) adj_setup_s4_clear(g, V, yd, j, i);
-}
-__global__ void __launch_bounds__(kBs) k_adj_pgf(DGrid g, View V) {
-  bool yd;
-  int j, i;
-  if (face_of(g, (size_t)blockIdx.x * kBs + threadIdx.x, &yd, &j, &i)) adj_pgf_face(g, V, yd, j, i, AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_s6(DGrid g, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_setup_s6_column(g, V, (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-// periodic tendency copies: u rows (nz x ny) then v columns (nz x nx)
-__global__ void __launch_bounds__(kBs) k_adj_adv_copy(DGrid g, View V) {
-  const size_t t = (size_t)blockIdx.x * kBs + threadIdx.x;
-  const size_t nu = (size_t)g.nz * g.ny, nv = (size_t)g.nz * g.nx;
-  if (t < nu) adj_advect_copy(g, V, false, (int)(t / g.ny), (int)(t % g.ny));
-  else if (t < nu + nv) adj_advect_copy(g, V, true, (int)((t - nu) / g.nx), (int)((t - nu) % g.nx));
-}
-__global__ void __launch_bounds__(kBs) k_adj_adv_h(DGrid g, View V, int kind, int nlev) {
-  const size_t t = (size_t)blockIdx.x * kBs + threadIdx.x;
-  const size_t plane = (size_t)g.ny * g.nx;
-  if (t >= (size_t)nlev * plane) return;
-  const int k = (int)(t / plane);
-  const size_t c = t % plane;
-  adj_advect_h_point(g, V, kind, k, (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_adv_v(DGrid g, View V, int kind) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_advect_v_column(g, V, kind, (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_couple_col(DGrid g, View V) {
-  const size_t c = (size_t)blockIdx.x * kBs + threadIdx.x;
-  if (c < (size_t)g.ny * g.nx) adj_couple_column(g, V, (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-__global__ void __launch_bounds__(kBs) k_adj_couple_pt(DGrid g, View V) {
-  const size_t t = (size_t)blockIdx.x * kBs + threadIdx.x;
-  const size_t plane = (size_t)g.ny * g.nx;
-  if (t >= (size_t)g.nz * plane) return;
-  const size_t c = t % plane;
-  adj_couple_point(g, V, (int)(t / plane), (int)(c / g.nx), (int)(c % g.nx), AtomicAdd{});
-}
-
-int n_boundaries(const DNamelist& nl, int* nsub) {
-  const int ns = nl.ns;
-  nsub[0] = 1;
-  nsub[1] = ns / 2 > 1 ? ns / 2 : 1;
-  nsub[2] = ns;
-  return 3 + 3 * (nsub[0] + nsub[1] + nsub[2]);
-}
+};
 
 }  // namespace
 
 struct AdjTape {
   void* buf = nullptr;
   size_t bytes = 0;
-  DState scratch{};                                   // forward recompute state
-  std::vector<std::vector<double*>> stage_state;      // [3][kNStageState]
-  std::vector<std::vector<double*>> stage_const;      // [3][kNStageConst]
-  std::vector<std::vector<double*>> carried;          // [boundary][kNCarried]
-  double* bar[A_count] = {};
-  double* scr = nullptr;                              // kScrSlots planes of (nz+1, ny, nx)
-  double* dummy = nullptr;                            // a zero array of the largest extent
+  TapePtrs p;  // every array carved from buf
 };
 
 hipError_t adjoint_tape_create(const DGrid& g, const DNamelist& nl, AdjTape** out, size_t* bytes) {
   *out = nullptr;
-  int nsub[3];
-  const int nb = n_boundaries(nl, nsub);
   AdjTape* t = new (std::nothrow) AdjTape();
   if (!t) return hipErrorOutOfMemory;
-  struct Want {
-    double** p;
-    size_t n;
-  };
-  std::vector<Want> want;
-  const size_t wlen = ext_len(g, E_W), big = std::max(ext_len(g, E_U), std::max(ext_len(g, E_V), wlen));
-  for (int a = 0; a < kNStageState + 1; ++a) {
-    const int arr = a < kNStageState ? kStageState[a] : A_mudf;
-    double** slot = &t->scratch.u;
-    switch (arr) {
-      case A_u: slot = &t->scratch.u; break;
-      case A_v: slot = &t->scratch.v; break;
-      case A_w: slot = &t->scratch.w; break;
-      case A_theta: slot = &t->scratch.theta; break;
-      case A_p_total: slot = &t->scratch.p_total; break;
-      case A_p_pert: slot = &t->scratch.p_pert; break;
-      case A_ph_total: slot = &t->scratch.ph_total; break;
-      case A_ph_pert: slot = &t->scratch.ph_pert; break;
-      case A_mu_total: slot = &t->scratch.mu_total; break;
-      case A_mu_pert: slot = &t->scratch.mu_pert; break;
-      default: slot = &t->scratch.mudf; break;
-    }
-    want.push_back({slot, ext_len(g, kExtent[arr])});
-  }
-  t->stage_state.assign(3, std::vector<double*>(kNStageState));
-  t->stage_const.assign(3, std::vector<double*>(kNStageConst));
-  t->carried.assign(nb, std::vector<double*>(kNCarried));
-  for (int s = 0; s < 3; ++s) {
-    for (int a = 0; a < kNStageState; ++a) want.push_back({&t->stage_state[s][a], ext_len(g, kExtent[kStageState[a]])});
-    for (int a = 0; a < kNStageConst; ++a) want.push_back({&t->stage_const[s][a], ext_len(g, kExtent[kStageConst[a]])});
-  }
-  for (int b = 0; b < nb; ++b)
-    for (int a = 0; a < kNCarried; ++a) want.push_back({&t->carried[b][a], ext_len(g, kExtent[kCarried[a]])});
-  for (int a = 0; a < A_count; ++a) want.push_back({&t->bar[a], ext_len(g, kExtent[a])});
-  want.push_back({&t->scr, (size_t)kScrSlots * wlen});
-  want.push_back({&t->dummy, big});
+  const std::vector<TapeSlot> want = tape_slots(g, nl, &t->p);
   size_t total = 0;
-  for (const Want& x : want) total += (x.n * sizeof(double) + 255) / 256 * 256;
+  for (const TapeSlot& x : want) total += slot_bytes(x);
   hipError_t e = hipMalloc(&t->buf, total);
   if (e != hipSuccess) {
     delete t;
@@ -338,9 +175,9 @@ hipError_t adjoint_tape_create(const DGrid& g, const DNamelist& nl, AdjTape** ou
   t->bytes = total;
   char* base = static_cast<char*>(t->buf);
   size_t off = 0;
-  for (const Want& x : want) {
+  for (const TapeSlot& x : want) {
     *x.p = reinterpret_cast<double*>(base + off);
-    off += (x.n * sizeof(double) + 255) / 256 * 256;
+    off += slot_bytes(x);
   }
   *out = t;
   *bytes = total;
@@ -361,48 +198,35 @@ hipError_t copy_arr(const DGrid& g, int arr, double* dst, const double* src, hip
 
 // Copy the carried arrays into boundary b; first: the stage's first substep
 // has not yet run advance_w, so t_2ave is 0 and pm1 is the step-0 p.
-hipError_t snap_carried(const DGrid& g, const DWork& w, const DState& s, AdjTape* t, int b, bool first,
+hipError_t snap_carried(const DGrid& g, const DWork& w, const DState& s, TapePtrs& t, int b, bool first,
                         hipStream_t st) {
   for (int a = 0; a < kNCarried; ++a) {
     const int arr = kCarried[a];
     const double* src = arr == A_mudf ? s.mudf : dwork_ptr(w, arr);
     hipError_t e;
     if (first && arr == A_t2ave)
-      e = hipMemsetAsync(t->carried[b][a], 0, ext_len(g, E_M) * sizeof(double), st);
+      e = hipMemsetAsync(t.carried[b][a], 0, ext_len(g, E_M) * sizeof(double), st);
     else if (first && arr == A_pm1)
-      e = copy_arr(g, arr, t->carried[b][a], w.p, st);
+      e = copy_arr(g, arr, t.carried[b][a], w.p, st);
     else
-      e = copy_arr(g, arr, t->carried[b][a], src, st);
+      e = copy_arr(g, arr, t.carried[b][a], src, st);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
-// The values view of stage s at carried boundary b.
-View make_view(const AdjTape* t, int s, int b) {
-  View V;
-  for (int a = 0; a < A_count; ++a) {
-    V.val[a] = t->dummy;
-    V.bar[a] = t->bar[a];
-  }
-  for (int a = 0; a < kNStageState; ++a) V.val[kStageState[a]] = t->stage_state[s][a];
-  for (int a = 0; a < kNStageConst; ++a) V.val[kStageConst[a]] = t->stage_const[s][a];
-  for (int a = 0; a < kNCarried; ++a) V.val[kCarried[a]] = t->carried[b][a];
-  V.scr = t->scr;
-  return V;
-}
-
 }  // namespace
 
-hipError_t adjoint_large_step(const DGrid& g, const DNamelist& nl, const DWork& w, AdjTape* t, const DState& start,
+hipError_t adjoint_large_step(const DGrid& g, const DNamelist& nl, const DWork& w, AdjTape* tape, const DState& start,
                               const DState& bar_io, hipStream_t st) {
   if (nl.mixed) return hipErrorInvalidValue;
+  TapePtrs& t = tape->p;
   int nsub[3];
   n_boundaries(nl, nsub);
   const double dt = nl.dt;
   const int rk_of[3] = {1, 2, 3};
   const double dts_of[3] = {dt / 3.0, dt / (double)nl.ns, dt / (double)nl.ns};
-  const DState& S = t->scratch;
+  const DState& S = t.scratch;
   hipError_t e;
 #define ADJ_TRY(x)             \
   do {                         \
@@ -417,23 +241,21 @@ hipError_t adjoint_large_step(const DGrid& g, const DNamelist& nl, const DWork& 
     const int pairs[6][2] = {{A_u0, A_u}, {A_v0, A_v}, {A_w0, A_w}, {A_th0, A_theta}, {A_ph0, A_ph_pert}, {A_mu0, A_mu_pert}};
     for (const auto& pr : pairs) ADJ_TRY(copy_arr(g, pr[0], dwork_ptr(w, pr[0]), state_ptr(S, pr[1]), st));
   }
-  std::vector<int> first_b(3);
   int b = 0;
   for (int s = 0; s < 3; ++s) {
     for (int a = 0; a < kNStageState; ++a)
-      ADJ_TRY(copy_arr(g, kStageState[a], t->stage_state[s][a], state_ptr(S, kStageState[a]), st));
+      ADJ_TRY(copy_arr(g, kStageState[a], t.stage_state[s][a], state_ptr(S, kStageState[a]), st));
     ADJ_TRY(launch_stage_setup(g, nl, S, w, rk_of[s], dts_of[s], st, nullptr));
     for (int a = 0; a < kNStageConst; ++a) {
       const int arr = kStageConst[a];
       if (arr == A_phb) {
         const size_t n = ext_len(g, E_W);
-        k_adj_phb<<<nblocks(n), kBs, 0, st>>>(n, S.ph_total, S.ph_pert, t->stage_const[s][a]);
+        k_adj_phb<<<nblocks(n), kBs, 0, st>>>(n, S.ph_total, S.ph_pert, t.stage_const[s][a]);
         ADJ_TRY(hipGetLastError());
       } else {
-        ADJ_TRY(copy_arr(g, arr, t->stage_const[s][a], dwork_ptr(w, arr), st));
+        ADJ_TRY(copy_arr(g, arr, t.stage_const[s][a], dwork_ptr(w, arr), st));
       }
     }
-    first_b[s] = b;
     ADJ_TRY(snap_carried(g, w, S, t, b++, true, st));
     for (int it = 0; it < nsub[s]; ++it) {
       for (int part = 0; part < 3; ++part) {
@@ -444,56 +266,11 @@ hipError_t adjoint_large_step(const DGrid& g, const DNamelist& nl, const DWork& 
     ADJ_TRY(launch_stage_finish(g, S, w, st, nullptr));
   }
   // ---- reverse ----
-  ADJ_TRY(hipMemsetAsync(t->bar[0], 0,
-                         (size_t)((char*)t->scr - (char*)t->bar[0]), st));  // every bar array (contiguous)
-  for (int a = 0; a <= A_mudf; ++a) ADJ_TRY(copy_arr(g, a, t->bar[a], state_ptr(bar_io, a), st));
-  const size_t C = (size_t)g.ny * g.nx;
-  const unsigned cb = nblocks(C), fb = nblocks(n_faces(g));
-  for (int s = 2; s >= 0; --s) {
-    const int b0 = first_b[s];
-    const int b_last = b0 + 3 * nsub[s];
-    {
-      View V = make_view(t, s, b_last);
-      k_adj_finish_col<<<cb, kBs, 0, st>>>(g, V);
-      k_adj_finish_face<<<fb, kBs, 0, st>>>(g, V);
-    }
-    for (int it = nsub[s] - 1; it >= 0; --it) {
-      const int bs0 = b0 + 3 * it;  // before uv; +1 after uv; +2 after mu; +3 after w
-      k_adj_p<<<cb, kBs, 0, st>>>(g, make_view(t, s, bs0 + 3));
-      k_adj_w<<<cb, kBs, 0, st>>>(g, nl, dts_of[s], make_view(t, s, bs0 + 2));
-      k_adj_mu_theta<<<cb, kBs, 0, st>>>(g, nl, dts_of[s], make_view(t, s, bs0 + 1));
-      k_adj_uv<<<fb, kBs, 0, st>>>(g, dts_of[s], make_view(t, s, bs0));
-    }
-    {
-      View V = make_view(t, s, b0);  // the arrays after the setup
-      k_adj_s1<<<cb, kBs, 0, st>>>(g, nl, dts_of[s], V);
-      k_adj_s2<<<cb, kBs, 0, st>>>(g, V);
-      k_adj_s3<<<fb, kBs, 0, st>>>(g, V);
-      k_adj_s4<<<cb, kBs, 0, st>>>(g, rk_of[s], V);
-      k_adj_s4_clear<<<fb, kBs, 0, st>>>(g, V);
-      k_adj_pgf<<<fb, kBs, 0, st>>>(g, V);
-      k_adj_s6<<<cb, kBs, 0, st>>>(g, V);
-      k_adj_adv_copy<<<nblocks((size_t)g.nz * (g.ny + g.nx)), kBs, 0, st>>>(g, V);
-      for (int kind = ADV_SCALAR; kind <= ADV_W; ++kind) {
-        const int nlev = kind == ADV_W ? g.nz + 1 : g.nz;
-        k_adj_adv_h<<<nblocks((size_t)nlev * C), kBs, 0, st>>>(g, V, kind, nlev);
-        k_adj_adv_v<<<cb, kBs, 0, st>>>(g, V, kind);
-      }
-      k_adj_couple_col<<<cb, kBs, 0, st>>>(g, V);
-      k_adj_couple_pt<<<nblocks((size_t)g.nz * C), kBs, 0, st>>>(g, V);
-    }
-    ADJ_TRY(hipGetLastError());
-  }
-  {
-    View V = make_view(t, 0, 0);
-    const int pairs[6][2] = {{A_u, A_u0}, {A_v, A_v0}, {A_w, A_w0}, {A_theta, A_th0}, {A_ph_pert, A_ph0}, {A_mu_pert, A_mu0}};
-    for (const auto& pr : pairs) {
-      const size_t n = ext_len(g, kExtent[pr[0]]);
-      k_adj_origin<<<nblocks(n), kBs, 0, st>>>(V, pr[0], pr[1], n);
-    }
-    ADJ_TRY(hipGetLastError());
-  }
-  for (int a = 0; a <= A_mudf; ++a) ADJ_TRY(copy_arr(g, a, state_ptr(bar_io, a), t->bar[a], st));
+  ADJ_TRY(hipMemsetAsync(t.bar[0], 0, (size_t)((char*)t.scr - (char*)t.bar[0]), st));  // every bar array
+  for (int a = 0; a <= A_mudf; ++a) ADJ_TRY(copy_arr(g, a, t.bar[a], state_ptr(bar_io, a), st));
+  GpuLaunch L{st};
+  if (!reverse_sequence(g, nl, t, L)) return L.err;
+  for (int a = 0; a <= A_mudf; ++a) ADJ_TRY(copy_arr(g, a, state_ptr(bar_io, a), t.bar[a], st));
 #undef ADJ_TRY
   return hipSuccess;
 }
