@@ -204,6 +204,25 @@ void wd_batch_destroy(wd_batch *b);
 int wd_step_adjoint(wd_handle *h, const wd_state *start, wd_state *bar, void *stream);
 size_t wd_adjoint_tape_bytes(const wd_handle *h);
 
+/* A large step that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It is
+ * wd_create + wd_step + wd_destroy in one call, with the same checks, kernels
+ * and results (bit for bit those of wd_step), but every device buffer it
+ * needs (the metrics and the workspace) comes from one fixed 64 MiB device
+ * array inside the library, one per GPU that calls it, held until the process
+ * ends.  That holds the 64x64x40 warm bubble with any one optional term,
+ * about 1.8e5 cells; a grid and namelist that need more return WD_EINVAL
+ * with the size in the message (use wd_create and wd_step).  No hipMalloc,
+ * hipFree or other allocating call is made.  The grid's host arrays are
+ * copied in on `stream`, which is synchronised once before the steps are
+ * enqueued; the steps themselves run asynchronously.  The array keeps the
+ * last call's contents and the workspace is cleared at every call; calls
+ * that overlap on one GPU (from several threads or streams) race on it.
+ * Errors: wd_last_error(NULL). */
+int wd_step_noalloc(const wd_grid *grid, const wd_namelist *nl, const wd_state *state, int n_large_steps,
+                    void *stream);
+
 /* Kernel-duration instrumentation for benchmarking (no counterpart in the
  * reference, whose step is one XLA program).  Bit `id` of `kernel_mask`
  * brackets every launch of kernel `id` in subsequent wd_step calls with HIP

@@ -168,3 +168,21 @@ def test_metrics_shapes():
     assert m.dnw[0] < 0 and m.rdnw[0] < 0  # WRF-signed metric
     with pytest.raises(ValueError):
         Metrics.flat(nx=5, ny=4, nz=3, dx=1.0, dy=1.0, eta=[1.0, 0.6, 0.3, 0.0], p_top=5000.0)
+
+
+def test_noalloc_refuses_a_grid_beyond_its_scratch_before_any_device_call():
+    """wd_step_noalloc checks its fixed scratch size on the host: a grid that needs more is
+    refused with the size before any HIP call (here without a GPU)."""
+
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import _abi_structs
+
+    lib = _lib.load()
+    case = ic.bubble_3d(nx=96, ny=96, nz=40, noise=0.0)
+    keep: list = []
+    grid, nl = _abi_structs(case.metrics, case.namelist, keep)
+    fake = _lib.WdState(*[ctypes.c_void_p(4096 * (q + 1)) for q in range(11)])  # never dereferenced
+    assert lib.wd_step_noalloc(ctypes.byref(grid), ctypes.byref(nl), ctypes.byref(fake), 1, None) == _lib.WD_EINVAL
+    msg = _lib.last_error(None)
+    assert "scratch" in msg and "wd_create" in msg
+    assert lib.wd_step_noalloc(None, None, None, 1, None) == _lib.WD_EINVAL

@@ -27,7 +27,8 @@ EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error
                     "wd_capture_graph", "wd_graph_step", "wd_autotune", "wd_tuned_config",
                     "wd_set_precision", "wd_get_precision",
                     "wd_batch_create", "wd_step_batched", "wd_batch_branch", "wd_batch_branches",
-                    "wd_batch_last_error", "wd_batch_destroy", "wd_step_adjoint", "wd_adjoint_tape_bytes")
+                    "wd_batch_last_error", "wd_batch_destroy", "wd_step_adjoint", "wd_adjoint_tape_bytes",
+                    "wd_step_noalloc")
 
 
 class WdGrid(ctypes.Structure):
@@ -135,6 +136,9 @@ def load() -> ctypes.CDLL:
         lib.wd_batch_branches.argtypes = [ctypes.c_void_p]
         lib.wd_batch_last_error.restype = ctypes.c_char_p
         lib.wd_batch_last_error.argtypes = [ctypes.c_void_p]
+        lib.wd_step_noalloc.restype = ctypes.c_int
+        lib.wd_step_noalloc.argtypes = [ctypes.POINTER(WdGrid), ctypes.POINTER(WdNamelist), ctypes.POINTER(WdState),
+                                        ctypes.c_int, ctypes.c_void_p]
         lib.wd_batch_destroy.restype = None
         lib.wd_batch_destroy.argtypes = [ctypes.c_void_p]
         _lib = lib

@@ -12,10 +12,18 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "wrfdyn.h"
 #include "wrfdyn_kernels.h"
+
+// Device memory of wd_step_noalloc: the metrics and the workspace of one
+// call, at most kScratchBytes.  One copy per GPU that runs the library, held
+// until the process ends; in host code `scratch` only names the symbol, and
+// its device address comes from hipGetSymbolAddress (scratch_base).
+constexpr size_t kScratchBytes = (size_t)64 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
 
 struct wd_handle {
   wd::DGrid g{};
@@ -234,13 +242,8 @@ int capture_batch(wd_batch* b, const std::vector<wd::DState>& states, const std:
   return WD_OK;
 }
 
-}  // namespace
 
-extern "C" {
-
-int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
-  if (!out) return fail(nullptr, WD_EINVAL, "out is NULL");
-  *out = nullptr;
+int validate_create(const wd_grid* grid, const wd_namelist* nl, bool* rot_out) {
   if (!grid || !nl) return fail(nullptr, WD_EINVAL, "grid or namelist is NULL");
   const int nx = grid->nx, ny = grid->ny, nz = grid->nz;
   if (nx < 4 || ny < 1 || nz < 4)
@@ -281,12 +284,164 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   if (nl->scalar_adv_opt < 0 || nl->scalar_adv_opt > 2)
     return fail(nullptr, WD_EINVAL, "scalar_adv_opt must be 0, 1 (positive-definite) or 2 (monotonic)");
   // rotation: all four Coriolis arrays or none; it runs when f or e is nonzero
-  bool rot = false;
+  bool& rot = *rot_out;
+  rot = false;
   if (grid->f || grid->e || grid->sina || grid->cosa) {
     if (!grid->f || !grid->e || !grid->sina || !grid->cosa)
       return fail(nullptr, WD_EINVAL, "rotation needs all four of f, e, sina and cosa (or none)");
     for (size_t q = 0; q < (size_t)ny * nx && !rot; ++q) rot = grid->f[q] != 0.0 || grid->e[q] != 0.0;
   }
+
+  return WD_OK;
+}
+
+// The scalar parameters of the grid and namelist (everything but the device
+// pointers of the metrics).
+void init_params(const wd_grid* grid, const wd_namelist* nl, bool rot, wd::DGrid& g, wd::DNamelist& n) {
+  const int nz = grid->nz;
+  g.nx = grid->nx;
+  g.ny = grid->ny;
+  g.nz = nz;
+  g.dx = grid->dx;
+  g.dy = grid->dy;
+  g.rdx = 1.0 / grid->dx;
+  g.rdy = 1.0 / grid->dy;
+  g.rot = rot ? 1 : 0;
+  g.p_top = grid->p_top;
+  g.hypsometric_opt = nl->hypsometric_opt;
+  g.cf1 = grid->cf1;
+  g.cf2 = grid->cf2;
+  g.cf3 = grid->cf3;
+  // WRF top-face extrapolation weights cfn/cfn1 (core/rk_addtend_dry.py:305-312).
+  double dn_top = grid->dn[nz - 1];
+  double dn_safe = std::fabs(dn_top) > 1.0e-30 ? dn_top : 1.0;
+  g.cfn = (0.5 * grid->dnw[nz - 1] + grid->dn[nz - 1]) / dn_safe;
+  g.cfn1 = -0.5 * grid->dnw[nz - 1] / dn_safe;
+  // XCD-contiguous tile order (default on); WRFDYN_XCD_REMAP=0 keeps the
+  // hardware dispatch order, for measurement.
+  g.xcd_remap = 1;
+  if (const char* env = std::getenv("WRFDYN_XCD_REMAP")) g.xcd_remap = std::atoi(env) != 0;
+
+  n.dt = nl->dt;
+  n.ns = nl->acoustic_substeps;
+  n.epssm = nl->epssm;
+  n.top_lid = nl->top_lid;
+  n.w_damping = nl->w_damping;
+  n.damp_opt = nl->damp_opt;
+  n.dampcoef = nl->dampcoef;
+  n.zdamp = nl->zdamp;
+  n.diff_6th_opt = nl->diff_6th_opt;
+  n.diff_6th_factor = nl->diff_6th_factor;
+  n.smag = nl->diff_opt == 1 && nl->km_opt == 4;
+  n.c_s = nl->c_s;
+  n.scalar_adv_opt = nl->scalar_adv_opt;
+  n.const_nu = nl->const_nu_m2_s;
+  n.mixed = 0;
+}
+
+struct WsSlot {
+  double** p;
+  size_t n;  // doubles
+};
+constexpr int kMaxWsSlots = 64;
+size_t slot_bytes(size_t n) { return (n * sizeof(double) + 255) / 256 * 256; }
+
+// The workspace arrays of a handle: the base set (optional = false), or the
+// arrays of the optional terms the namelist turns on (optional = true): the
+// filter's tendencies, the Smagorinsky viscosity and tendencies or the
+// constant-K tendencies and layer sums, the limiter's origin mass, scales
+// and face fluxes.  Returns the count written to out.
+int workspace_slots(const wd::DGrid& g, const wd::DNamelist& n, bool optional, wd::DWork& w, WsSlot* out) {
+  const size_t nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t M = nz * ny * nx, Wn = (nz + 1) * ny * nx, U = nz * ny * (nx + 1), V = nz * (ny + 1) * nx;
+  const size_t c = ny * nx, cu = ny * (nx + 1), cv = (ny + 1) * nx;
+  int k = 0;
+  auto add = [&](double** p, size_t len) { out[k++] = WsSlot{p, len}; };
+  if (!optional) {
+    const WsSlot base[] = {
+        {&w.u0, U},       {&w.v0, V},        {&w.w0, Wn},      {&w.th0, M},      {&w.ph0, Wn},
+        {&w.mu0, c},      {&w.ru, M},        {&w.rv, M},       {&w.rom, Wn},     {&w.ut, U},
+        {&w.vt, V},       {&w.wt, Wn},       {&w.tht, M},      {&w.alt_raw, M},  {&w.al_ls, M},
+        {&w.php_ls, M},   {&w.mub, c},       {&w.mut, c},      {&w.muts0, c},    {&w.mu_work0, c},
+        {&w.mu, c},       {&w.muts, c},      {&w.muave, c},    {&w.muu, cu},     {&w.muus, cu},
+        {&w.muv, cv},     {&w.muvs, cv},     {&w.u_work, U},   {&w.v_work, V},   {&w.w_work, Wn},
+        {&w.th_work, M},  {&w.ph_work, Wn},  {&w.c2a, M},      {&w.alt, M},      {&w.pb, M},
+        {&w.ph1t, Wn},    {&w.php, M},       {&w.p, M},        {&w.al, M},       {&w.pm1, M},
+        {&w.rw_tend, Wn}, {&w.ph_tend, Wn},  {&w.ca, Wn},      {&w.calpha, Wn},  {&w.cgamma, Wn},
+        {&w.ww, Wn},      {&w.t2ave, M},     {&w.rhs, Wn},     {&w.wfwd, Wn},
+    };
+    for (const WsSlot& s : base) add(s.p, s.n);
+    return k;
+  }
+  if (n.diff_6th_opt != 0) {
+    add(&w.d6u, U);
+    add(&w.d6v, V);
+    add(&w.d6w, Wn);
+    add(&w.d6t, M);
+  }
+  if (n.smag) {
+    add(&w.xkmh, M);
+    add(&w.smu, U);
+    add(&w.smv, V);
+    add(&w.smw, Wn);
+    add(&w.smt, M);
+  }
+  if (n.const_nu > 0.0) {
+    add(&w.smu, U);
+    add(&w.smv, V);
+    add(&w.smw, Wn);
+    add(&w.smt, M);
+    add(&w.ck_col, c);
+    add(&w.ck_dz, 1);
+  }
+  if (n.scalar_adv_opt != 0) {
+    add(&w.mut0, c);
+    add(&w.lim_sa, M);
+    add(&w.lim_sb, M);
+    add(&w.lim_ax, M);
+    add(&w.lim_lx, M);
+    add(&w.lim_ay, M);
+    add(&w.lim_ly, M);
+    add(&w.lim_az, Wn);
+    add(&w.lim_lz, Wn);
+  }
+  return k;
+}
+
+// The device address of this GPU's `scratch`, looked up once per device with
+// that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_base(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, WD_EINVAL, "HIP device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return WD_OK;
+}
+
+__global__ void __launch_bounds__(256) k_fill(double* p, size_t n, double v) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
+  if (!out) return fail(nullptr, WD_EINVAL, "out is NULL");
+  *out = nullptr;
+  bool rot = false;
+  const int vrc = validate_create(grid, nl, &rot);
+  if (vrc != WD_OK) return vrc;
+  const int nx = grid->nx, ny = grid->ny, nz = grid->nz;
 
   wd_handle* h = new (std::nothrow) wd_handle();
   if (!h) return fail(nullptr, WD_ENOMEM, "host allocation failed");
@@ -349,13 +504,6 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   }
   const double* mb = static_cast<const double*>(h->metric_buf);
   wd::DGrid& g = h->g;
-  g.nx = nx;
-  g.ny = ny;
-  g.nz = nz;
-  g.dx = grid->dx;
-  g.dy = grid->dy;
-  g.rdx = 1.0 / grid->dx;
-  g.rdy = 1.0 / grid->dy;
   int o = 0;
   g.c1h = mb + offs[o++];
   g.c2h = mb + offs[o++];
@@ -377,60 +525,19 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   g.c4h = mb + offs[o++];
   g.c3f = mb + offs[o++];
   g.c4f = mb + offs[o++];
-  g.rot = rot ? 1 : 0;
   g.fcor = rot ? mb + offs[o++] : nullptr;
   g.ecor = rot ? mb + offs[o++] : nullptr;
   g.sina = rot ? mb + offs[o++] : nullptr;
   g.cosa = rot ? mb + offs[o++] : nullptr;
-  g.p_top = grid->p_top;
-  g.hypsometric_opt = nl->hypsometric_opt;
-  g.cf1 = grid->cf1;
-  g.cf2 = grid->cf2;
-  g.cf3 = grid->cf3;
-  // WRF top-face extrapolation weights cfn/cfn1 (core/rk_addtend_dry.py:305-312).
-  double dn_top = grid->dn[nz - 1];
-  double dn_safe = std::fabs(dn_top) > 1.0e-30 ? dn_top : 1.0;
-  g.cfn = (0.5 * grid->dnw[nz - 1] + grid->dn[nz - 1]) / dn_safe;
-  g.cfn1 = -0.5 * grid->dnw[nz - 1] / dn_safe;
-
-  h->nl.dt = nl->dt;
-  h->nl.ns = nl->acoustic_substeps;
-  h->nl.epssm = nl->epssm;
-  h->nl.top_lid = nl->top_lid;
-  h->nl.w_damping = nl->w_damping;
-  h->nl.damp_opt = nl->damp_opt;
-  h->nl.dampcoef = nl->dampcoef;
-  h->nl.zdamp = nl->zdamp;
-  h->nl.diff_6th_opt = nl->diff_6th_opt;
-  h->nl.diff_6th_factor = nl->diff_6th_factor;
-  h->nl.smag = nl->diff_opt == 1 && nl->km_opt == 4;
-  h->nl.c_s = nl->c_s;
-  h->nl.scalar_adv_opt = nl->scalar_adv_opt;
-  h->nl.const_nu = nl->const_nu_m2_s;
+  init_params(grid, nl, rot, g, h->nl);
 
   // ---- workspace ----
-  const size_t M = (size_t)nz * ny * nx, Wn = (size_t)(nz + 1) * ny * nx;
-  const size_t U = (size_t)nz * ny * (nx + 1), V = (size_t)nz * (ny + 1) * nx;
-  struct Slot {
-    double** p;
-    size_t n;
-  };
   wd::DWork& w = h->w;
-  const Slot slots[] = {
-      {&w.u0, U},       {&w.v0, V},        {&w.w0, Wn},      {&w.th0, M},      {&w.ph0, Wn},
-      {&w.mu0, c},      {&w.ru, M},        {&w.rv, M},       {&w.rom, Wn},     {&w.ut, U},
-      {&w.vt, V},       {&w.wt, Wn},       {&w.tht, M},      {&w.alt_raw, M},  {&w.al_ls, M},
-      {&w.php_ls, M},   {&w.mub, c},       {&w.mut, c},      {&w.muts0, c},    {&w.mu_work0, c},
-      {&w.mu, c},       {&w.muts, c},      {&w.muave, c},    {&w.muu, cu},     {&w.muus, cu},
-      {&w.muv, cv},     {&w.muvs, cv},     {&w.u_work, U},   {&w.v_work, V},   {&w.w_work, Wn},
-      {&w.th_work, M},  {&w.ph_work, Wn},  {&w.c2a, M},      {&w.alt, M},      {&w.pb, M},
-      {&w.ph1t, Wn},     {&w.php, M},       {&w.p, M},        {&w.al, M},       {&w.pm1, M},
-      {&w.rw_tend, Wn}, {&w.ph_tend, Wn},  {&w.ca, Wn},      {&w.calpha, Wn},  {&w.cgamma, Wn},
-      {&w.ww, Wn},      {&w.t2ave, M},     {&w.rhs, Wn},     {&w.wfwd, Wn},
-  };
+  WsSlot slots[kMaxWsSlots];
+  const int nslots = workspace_slots(g, h->nl, false, w, slots);
   // 256-byte aligned sub-buffers
   size_t total = 0;
-  for (const Slot& s : slots) total += (s.n * sizeof(double) + 255) / 256 * 256;
+  for (int q = 0; q < nslots; ++q) total += slot_bytes(slots[q].n);
   e = hipMalloc(&h->work_buf, total);
   if (e != hipSuccess) {
     hipFree(h->metric_buf);
@@ -440,9 +547,9 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
   h->work_bytes = total;
   char* base = static_cast<char*>(h->work_buf);
   size_t off = 0;
-  for (const Slot& s : slots) {
-    *s.p = reinterpret_cast<double*>(base + off);
-    off += (s.n * sizeof(double) + 255) / 256 * 256;
+  for (int q = 0; q < nslots; ++q) {
+    *slots[q].p = reinterpret_cast<double*>(base + off);
+    off += slot_bytes(slots[q].n);
   }
   e = hipMemset(h->work_buf, 0, total);
   if (e != hipSuccess) {
@@ -452,22 +559,14 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
     return hip_fail(nullptr, e, "hipMemset(workspace)");
   }
   h->nl.mixed = 0;
-  if (nl->diff_6th_opt != 0 || h->nl.smag || nl->scalar_adv_opt != 0 || h->nl.const_nu > 0.0) {
+  WsSlot dslots[kMaxWsSlots];
+  const int ndslots = workspace_slots(g, h->nl, true, w, dslots);
+  if (ndslots > 0) {
     // the arrays of the optional terms, each 256-byte aligned: the filter's
     // tendencies, the Smagorinsky viscosity and tendencies, the limiter's
     // origin mass and scales
-    std::vector<std::pair<double**, size_t>> dslots;
-    if (nl->diff_6th_opt != 0) dslots.insert(dslots.end(), {{&w.d6u, U}, {&w.d6v, V}, {&w.d6w, Wn}, {&w.d6t, M}});
-    if (h->nl.smag)
-      dslots.insert(dslots.end(), {{&w.xkmh, M}, {&w.smu, U}, {&w.smv, V}, {&w.smw, Wn}, {&w.smt, M}});
-    if (h->nl.const_nu > 0.0)
-      dslots.insert(dslots.end(), {{&w.smu, U}, {&w.smv, V}, {&w.smw, Wn}, {&w.smt, M}, {&w.ck_col, (size_t)ny * nx},
-                                   {&w.ck_dz, 1}});
-    if (nl->scalar_adv_opt != 0)
-      dslots.insert(dslots.end(), {{&w.mut0, (size_t)ny * nx}, {&w.lim_sa, M}, {&w.lim_sb, M}, {&w.lim_ax, M},
-                                   {&w.lim_lx, M}, {&w.lim_ay, M}, {&w.lim_ly, M}, {&w.lim_az, Wn}, {&w.lim_lz, Wn}});
     size_t d_total = 0;
-    for (const auto& d : dslots) d_total += (d.second * sizeof(double) + 255) / 256 * 256;
+    for (int q = 0; q < ndslots; ++q) d_total += slot_bytes(dslots[q].n);
     e = hipMalloc(&h->diff_buf, d_total);
     if (e != hipSuccess) {
       hipFree(h->work_buf);
@@ -476,9 +575,9 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
       return fail(nullptr, WD_ENOMEM, std::string("hipMalloc(optional terms): ") + hipGetErrorString(e));
     }
     size_t od = 0;
-    for (const auto& d : dslots) {
-      *d.first = reinterpret_cast<double*>(static_cast<char*>(h->diff_buf) + od);
-      od += (d.second * sizeof(double) + 255) / 256 * 256;
+    for (int q = 0; q < ndslots; ++q) {
+      *dslots[q].p = reinterpret_cast<double*>(static_cast<char*>(h->diff_buf) + od);
+      od += slot_bytes(dslots[q].n);
     }
     h->work_bytes += d_total;
   }
@@ -496,10 +595,6 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
           h->column_tile = t;
     }
   }
-  // XCD-contiguous tile order (default on); WRFDYN_XCD_REMAP=0 keeps the
-  // hardware dispatch order, for measurement.
-  h->g.xcd_remap = 1;
-  if (const char* env = std::getenv("WRFDYN_XCD_REMAP")) h->g.xcd_remap = std::atoi(env) != 0;
   *out = h;
   return WD_OK;
 }
@@ -859,5 +954,91 @@ int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* st
 }
 
 size_t wd_adjoint_tape_bytes(const wd_handle* h) { return h ? h->tape_bytes : 0; }
+
+int wd_step_noalloc(const wd_grid* grid, const wd_namelist* nl, const wd_state* st, int n_large_steps, void* stream) {
+  bool rot = false;
+  int rc = validate_create(grid, nl, &rot);
+  if (rc != WD_OK) return rc;
+  if (n_large_steps < 0) return fail(nullptr, WD_EINVAL, "n_large_steps must be >= 0");
+  wd::DState ds;
+  rc = check_state(nullptr, st, &ds);
+  if (rc != WD_OK) return rc;
+  wd_handle h;  // on the stack: its device buffers all come from `scratch`
+  init_params(grid, nl, rot, h.g, h.nl);
+  h.column_tile = wd::kColumnMarch;
+  // the layout in `scratch`: the metrics, then the workspace (base, then the
+  // optional terms' arrays), each 256-byte aligned
+  const size_t n1 = grid->nz, n1f = (size_t)grid->nz + 1;
+  const size_t c = (size_t)grid->ny * grid->nx, cu = (size_t)grid->ny * (grid->nx + 1),
+               cv = (size_t)(grid->ny + 1) * grid->nx;
+  struct Metric {
+    const double* src;  // host array, or NULL: filled with `value`
+    size_t n;
+    double value;
+    const double** dst;
+  };
+  wd::DGrid& g = h.g;
+  const Metric metrics[] = {
+      {grid->c1h, n1, 0.0, &g.c1h},     {grid->c2h, n1, 0.0, &g.c2h},     {grid->c1f, n1f, 0.0, &g.c1f},
+      {grid->c2f, n1f, 0.0, &g.c2f},    {grid->dnw, n1, 0.0, &g.dnw},     {grid->rdnw, n1, 0.0, &g.rdnw},
+      {grid->dn, n1, 0.0, &g.dn},       {grid->rdn, n1, 0.0, &g.rdn},     {grid->fnm, n1, 0.0, &g.fnm},
+      {grid->fnp, n1, 0.0, &g.fnp},     {grid->msftx, c, 0.0, &g.msftx},  {grid->msfty, c, 0.0, &g.msfty},
+      {grid->msfux, cu, 0.0, &g.msfux}, {grid->msfuy, cu, 0.0, &g.msfuy}, {grid->msfvx, cv, 0.0, &g.msfvx},
+      {grid->msfvy, cv, 0.0, &g.msfvy}, {grid->c3h, n1, 1.0, &g.c3h},     {grid->c4h, n1, 0.0, &g.c4h},
+      {grid->c3f, n1f, 1.0, &g.c3f},    {grid->c4f, n1f, 0.0, &g.c4f},    {grid->f, c, 0.0, &g.fcor},
+      {grid->e, c, 0.0, &g.ecor},       {grid->sina, c, 0.0, &g.sina},    {grid->cosa, c, 1.0, &g.cosa},
+  };
+  const int n_metrics = rot ? 24 : 20;  // the rotation arrays only when the grid rotates
+  WsSlot ws[2 * kMaxWsSlots];
+  const int nbase = workspace_slots(g, h.nl, false, h.w, ws);
+  const int nws = nbase + workspace_slots(g, h.nl, true, h.w, ws + nbase);
+  size_t total = 0;
+  for (int q = 0; q < n_metrics; ++q) total += slot_bytes(metrics[q].n);
+  for (int q = 0; q < nws; ++q) total += slot_bytes(ws[q].n);
+  if (total > kScratchBytes)
+    return fail(nullptr, WD_EINVAL,
+                "wd_step_noalloc: this grid and namelist need " + std::to_string(total) +
+                    " bytes of device scratch, above its fixed " + std::to_string(kScratchBytes) +
+                    " (use wd_create and wd_step)");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDevice");
+  double* base = nullptr;
+  rc = scratch_base(dev, &base);
+  if (rc != WD_OK) return rc;
+  h.device = dev;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  char* cur = reinterpret_cast<char*>(base);
+  for (int q = 0; q < n_metrics; ++q) {
+    const Metric& m = metrics[q];
+    double* d = reinterpret_cast<double*>(cur);
+    *m.dst = d;
+    if (m.src)
+      e = hipMemcpyAsync(d, m.src, m.n * sizeof(double), hipMemcpyHostToDevice, s);
+    else {
+      k_fill<<<(unsigned)std::min<size_t>((m.n + 255) / 256, 1024), 256, 0, s>>>(d, m.n, m.value);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) return hip_fail(nullptr, e, "wd_step_noalloc: metrics to the device");
+    cur += slot_bytes(m.n);
+  }
+  // the caller's host metric arrays are read by the copies above: wait for them
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipStreamSynchronize");
+  // the workspace starts at zero, as wd_create leaves it
+  char* ws_begin = cur;
+  for (int q = 0; q < nws; ++q) {
+    *ws[q].p = reinterpret_cast<double*>(cur);
+    cur += slot_bytes(ws[q].n);
+  }
+  const size_t ws_doubles = (size_t)(cur - ws_begin) / sizeof(double);
+  k_fill<<<(unsigned)std::min<size_t>((ws_doubles + 255) / 256, 4096), 256, 0, s>>>(
+      reinterpret_cast<double*>(ws_begin), ws_doubles, 0.0);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(nullptr, e, "wd_step_noalloc: clearing the workspace");
+  rc = enqueue_steps(&h, ds, n_large_steps, s);
+  if (rc != WD_OK) return fail(nullptr, rc, h.err);
+  return WD_OK;
+}
 
 }  // extern "C"

@@ -40,6 +40,33 @@ def _c_array(a: np.ndarray, keep: list):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _abi_structs(metrics: Metrics, namelist: Namelist, keep: list):
+    """The C-ABI wd_grid and wd_namelist of a (grid, namelist); ``keep`` holds
+    the host arrays the grid points to."""
+
+    m = metrics
+    grid = _lib.WdGrid(
+        m.nx, m.ny, m.nz, m.dx, m.dy,
+        _c_array(m.c1h, keep), _c_array(m.c2h, keep), _c_array(m.c1f, keep), _c_array(m.c2f, keep),
+        _c_array(m.dnw, keep), _c_array(m.rdnw, keep), _c_array(m.dn, keep), _c_array(m.rdn, keep),
+        _c_array(m.fnm, keep), _c_array(m.fnp, keep), m.cf1, m.cf2, m.cf3,
+        _c_array(m.msftx, keep), _c_array(m.msfty, keep), _c_array(m.msfux, keep),
+        _c_array(m.msfuy, keep), _c_array(m.msfvx, keep), _c_array(m.msfvy, keep),
+        _c_array(m.c3h, keep), _c_array(m.c4h, keep), _c_array(m.c3f, keep), _c_array(m.c4f, keep),
+        float(m.p_top),
+        *((_c_array(m.f, keep), _c_array(m.e, keep), _c_array(m.sina, keep), _c_array(m.cosa, keep))
+          if m.rotating else (None,) * 4),
+    )
+    nl = namelist
+    nls = _lib.WdNamelist(float(nl.dt_s), int(nl.acoustic_substeps), float(nl.epssm),
+                          int(bool(nl.top_lid)), int(nl.w_damping), int(nl.damp_opt),
+                          float(nl.dampcoef), float(nl.zdamp), int(nl.diff_6th_opt),
+                          float(nl.diff_6th_factor), int(nl.diff_opt), int(nl.km_opt),
+                          float(nl.c_s), int(nl.scalar_adv_opt), int(nl.hypsometric_opt),
+                          float(nl.const_nu_m2_s))
+    return grid, nls
+
+
 class Dycore:
     """A compiled dycore for one (grid, namelist): the jit-cache entry analogue.
 
@@ -60,26 +87,7 @@ class Dycore:
         self.namelist = namelist
         lib = _lib.load()
         keep: list = []
-        m = metrics
-        self._grid = _lib.WdGrid(
-            m.nx, m.ny, m.nz, m.dx, m.dy,
-            _c_array(m.c1h, keep), _c_array(m.c2h, keep), _c_array(m.c1f, keep), _c_array(m.c2f, keep),
-            _c_array(m.dnw, keep), _c_array(m.rdnw, keep), _c_array(m.dn, keep), _c_array(m.rdn, keep),
-            _c_array(m.fnm, keep), _c_array(m.fnp, keep), m.cf1, m.cf2, m.cf3,
-            _c_array(m.msftx, keep), _c_array(m.msfty, keep), _c_array(m.msfux, keep),
-            _c_array(m.msfuy, keep), _c_array(m.msfvx, keep), _c_array(m.msfvy, keep),
-            _c_array(m.c3h, keep), _c_array(m.c4h, keep), _c_array(m.c3f, keep), _c_array(m.c4f, keep),
-            float(m.p_top),
-            *((_c_array(m.f, keep), _c_array(m.e, keep), _c_array(m.sina, keep), _c_array(m.cosa, keep))
-              if m.rotating else (None,) * 4),
-        )
-        nl = namelist
-        self._nl = _lib.WdNamelist(float(nl.dt_s), int(nl.acoustic_substeps), float(nl.epssm),
-                                   int(bool(nl.top_lid)), int(nl.w_damping), int(nl.damp_opt),
-                                   float(nl.dampcoef), float(nl.zdamp), int(nl.diff_6th_opt),
-                                   float(nl.diff_6th_factor), int(nl.diff_opt), int(nl.km_opt),
-                                   float(nl.c_s), int(nl.scalar_adv_opt), int(nl.hypsometric_opt),
-                                   float(nl.const_nu_m2_s))
+        self._grid, self._nl = _abi_structs(metrics, namelist, keep)
         handle = ctypes.c_void_p()
         code = lib.wd_create(ctypes.byref(self._grid), ctypes.byref(self._nl), ctypes.byref(handle))
         if code != _lib.WD_OK:
@@ -346,3 +354,31 @@ def dycore_step(state: dict, namelist: Namelist, metrics: Metrics, n_steps: int 
     torch.cuda.synchronize(state["u"].device)
     dyn.close()
     return out
+
+
+def step_noalloc(state: dict, namelist: Namelist, metrics: Metrics, n_steps: int = 1, stream=None) -> dict:
+    """``n_steps`` large steps of ``state`` (float64 device tensors, advanced in
+    place) through wd_step_noalloc: no handle and no device allocation, the
+    library's fixed device scratch holds the metrics and the workspace.  Same
+    results as :meth:`Dycore.step`; raises for a grid too large for the scratch."""
+
+    import torch
+
+    namelist.validate()
+    shapes = metrics.shapes()
+    for name in STATE_FIELDS:
+        t = state.get(name)
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or tuple(t.shape) != shapes[name]:
+            raise ValueError(f"state[{name!r}] must be a float64 tensor of shape {shapes[name]}")
+        if not t.is_contiguous() or t.device != state["u"].device:
+            raise ValueError(f"state[{name!r}] must be contiguous and on the same device as state['u']")
+    lib = _lib.load()
+    keep: list = []
+    grid, nls = _abi_structs(metrics, namelist, keep)
+    s = _state_ptrs(state)
+    stream = _stream_of(state, stream)
+    code = lib.wd_step_noalloc(ctypes.byref(grid), ctypes.byref(nls), ctypes.byref(s), int(n_steps),
+                               ctypes.c_void_p(stream.cuda_stream))
+    if code != _lib.WD_OK:
+        raise _lib.WrfdynError(code, _lib.last_error(None))
+    return state
