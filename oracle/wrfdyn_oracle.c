@@ -18,23 +18,23 @@
  *   CVPM=-(CP_D-R_D)/CP_D, g=9.81, MIN_ACOUSTIC_ALT=0.1, re=6370e3,
  *   emdiv=0.01, smdiv=0.1 (core/acoustic.py:845-846), t0=300.
  *
- * How it is pinned (tests/test_oracle_golden.py, test_oracle_invariants.py):
+ * How it is pinned (tests/test_oracle_golden.py, test_oracle_invariants.py,
+ * test_oracle_terrain_pins.py, test_oracle_coriolis.py, test_oracle_straka.py):
  * built with the reference's v0.13 constants (ORC_LEGACY_V013) it reproduces
  * the reference-held warm-bubble golden scalars (2-D, 80x40, 5000 steps) to
- * <= 6e-8, which pins the whole x-z large step incl. advection, PGF, prep,
- * advance_uv/mu_t/w, Thomas, damp_opt=3, calc_p_rho and finish; the y path
- * is pinned against it by x<->y transposition; Thomas, calc_coef_w, the EOS,
- * flux5 and the shared pow/sin have closed-form or library checks; the
- * terrain surface w (surface_w) converges at second order to the kinematic
- * condition u dh/dx + v dh/dy on stretched eta, which also pins the cf1..cf3
- * surface extrapolation.
+ * <= 4.9e-7, which pins the whole x-z large step incl. advection, PGF, prep,
+ * advance_uv/mu_t/w, Thomas, damp_opt=3, calc_p_rho and finish, and the
+ * Straka density-current golden scalars (9000 steps) to <= 4e-4, which pins
+ * the constant-K diffusion; the y path is pinned against the x path by x<->y
+ * transposition; Thomas, calc_coef_w, the EOS, flux5 and the shared pow/sin
+ * have closed-form or library checks; the terrain and stretched-eta terms
+ * (the large-step PGF and rhs_ph with a varying base geopotential, the
+ * terrain surface w) converge at second order to their continuous forms;
+ * the w curvature term, w_damping above a vertical CFL of 1, the
+ * hypsometric_opt=2 refresh mass and the Coriolis terms have closed forms.
  *
- * Parity unpinned (restated from the cited lines, checked only for internal
- * consistency -- symmetry, transposition -- because no reference vector or
- * closed form covers them): horizontally varying base geopotential in the
- * large-step PGF and rhs_ph, the stretched-eta metrics other than cf1..cf3
- * inside the large step, the w curvature term and cp = 7R/2 (both newer than
- * the golden runs), and the w_damping branch for a vertical CFL above 1.
+ * Parity unpinned: none of the step's terms.  What no reference file holds
+ * is the reference's own field-level FP64 output beyond the 2-D slabs.
  */
 #include "wrfdyn_oracle.h"
 #include "wrfdyn_math.h"

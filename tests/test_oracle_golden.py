@@ -12,6 +12,14 @@ dynamics/acoustic_wrf.py:45-54, the w curvature term in
 runtime/operational_mode.py:1784-1822).  The oracle built with exactly those
 two constants reverted (liboracle_v013.so) must reproduce the golden
 scalars; the current oracle must stay within the drift those changes cause.
+
+The remaining gap (2e-9 to 3e-8 at 100 s) is not rounding: the legacy build with
+libm pow in place of the shared one, with the vertical column sums taken from the
+top down, or with pg_buoy_w fed the stage-entry EOS pressure instead of the
+carried one, moves none of the scalars by more than 1e-12 relative.  It is a
+small formula-level difference between the v0.13 code path and the current one
+that the oracle follows (v0.14 also reworked advance_w's lid and pg_buoy
+handling); the bars below sit just above what is observed.
 """
 
 import json
@@ -76,20 +84,22 @@ def test_initial_state_matches_reference_ic():
 
 
 def test_legacy_oracle_reproduces_golden_scalars(oracle_lib):
-    """100 s and 250 s snapshots to 1e-7 relative (observed ~2e-9)."""
+    """100 s and 250 s snapshots to 5e-8 relative (observed at most 3.0e-8, on theta' min, a value
+    of 1e-2 K; 1.5e-9 to 1.5e-8 elsewhere)."""
 
     from wrf_gpu_amd import ic
 
     case = ic.warm_bubble_2d()
     gold = _golden()["jax_cpu_v013"]
     s = oracle_lib.run(case.metrics, case.namelist, case.state, 1000, legacy_v013=True)
-    _compare(_snapshot(case, s), gold[0], 1e-7)
+    _compare(_snapshot(case, s), gold[0], 5e-8)
     s = oracle_lib.run(case.metrics, case.namelist, s, 1500, legacy_v013=True)
-    _compare(_snapshot(case, s), gold[1], 1e-7)
+    _compare(_snapshot(case, s), gold[1], 5e-8)
 
 
 def test_legacy_oracle_reproduces_golden_500s(oracle_lib):
-    """Full 5000 steps (500 s): 1e-6 relative (observed ~6e-8)."""
+    """Full 5000 steps (500 s): 7.5e-7 relative (observed 4.9e-7 on theta' min, at most 8.4e-8
+    elsewhere)."""
 
     from wrf_gpu_amd import ic
 
@@ -97,8 +107,8 @@ def test_legacy_oracle_reproduces_golden_500s(oracle_lib):
     gold = _golden()
     s = oracle_lib.run(case.metrics, case.namelist, case.state, 5000, legacy_v013=True)
     snap = _snapshot(case, s)
-    _compare(snap, gold["jax_cpu_v013"][2], 1e-6)
-    _compare(snap, gold["gpu_v0100"][2], 1e-6)
+    _compare(snap, gold["jax_cpu_v013"][2], 7.5e-7)
+    _compare(snap, gold["gpu_v0100"][2], 7.5e-7)
 
 
 def test_current_oracle_within_documented_drift(oracle_lib):
