@@ -50,21 +50,32 @@ class _Namelist(ctypes.Structure):
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
         ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
         ("scalar_adv_opt", ctypes.c_int), ("hypsometric_opt", ctypes.c_int),
-        ("const_nu_m2_s", ctypes.c_double),
+        ("const_nu_m2_s", ctypes.c_double), ("moist_adv_opt", ctypes.c_int),
     ]
+
+
+MOIST_SPECIES = ("qv", "qc", "qr", "qi", "qs", "qg")
 
 
 class _State(ctypes.Structure):
     _fields_ = [(n, _dp) for n in (
         "u", "v", "w", "theta", "p_total", "p_pert", "ph_total", "ph_pert",
-        "mu_total", "mu_pert", "mudf")]
+        "mu_total", "mu_pert", "mudf") + MOIST_SPECIES]
 
 
 _STATE_KEYS = {
     "u": "u", "v": "v", "w": "w", "theta": "theta", "p_total": "p_total",
     "p_pert": "p_perturbation", "ph_total": "ph_total", "ph_pert": "ph_perturbation",
     "mu_total": "mu_total", "mu_pert": "mu_perturbation", "mudf": "mudf",
+    **{q: q for q in MOIST_SPECIES},
 }
+
+
+def _mk_state(d: dict) -> "_State":
+    """The orc_state of a state dict; the moisture pointers NULL when the dict has none."""
+
+    return _State(*[_ptr(d[_STATE_KEYS[n]]) if _STATE_KEYS[n] in d else None for n, _ in _State._fields_])
+
 
 _libs: dict = {}
 
@@ -145,7 +156,7 @@ def run(metrics, namelist, state: dict, n_steps: int, threads: int | None = None
     g = _grid(metrics, keep)
     nl = _namelist(namelist)
     out = _state_copy(metrics, state)
-    s = _State(*[_ptr(out[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    s = _mk_state(out)
     rc = L.orc_run(ctypes.byref(g), ctypes.byref(nl), ctypes.byref(s), int(n_steps))
     if rc != 0:
         raise RuntimeError(f"orc_run failed with code {rc}")
@@ -158,7 +169,7 @@ def _namelist(namelist) -> _Namelist:
                      float(namelist.dampcoef), float(namelist.zdamp), int(namelist.diff_6th_opt),
                      float(namelist.diff_6th_factor), int(namelist.diff_opt), int(namelist.km_opt),
                      float(namelist.c_s), int(namelist.scalar_adv_opt), int(namelist.hypsometric_opt),
-                     float(namelist.const_nu_m2_s))
+                     float(namelist.const_nu_m2_s), int(getattr(namelist, "moist_adv_opt", 0)))
 
 
 def _state_copy(metrics, state: dict) -> dict:
@@ -201,7 +212,7 @@ def vjp(metrics, namelist, state: dict, n_steps: int, bar: dict) -> tuple:
             for k, v in start.items()}
     grad = {k: np.zeros_like(v) for k, v in start.items()}
     final = {k: np.zeros_like(v) for k, v in start.items()}
-    mk = lambda d: _State(*[_ptr(d[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    mk = _mk_state
     s_in, s_bar, s_grad, s_out = mk(start), mk(seed), mk(grad), mk(final)
     rc = L.orc_vjp(ctypes.byref(g), ctypes.byref(nl), ctypes.byref(s_in), int(n_steps), ctypes.byref(s_bar),
                    ctypes.byref(s_grad), ctypes.byref(s_out))
@@ -210,15 +221,20 @@ def vjp(metrics, namelist, state: dict, n_steps: int, bar: dict) -> tuple:
     return grad, final
 
 
-def calc_coef_w(metrics, mut: np.ndarray, c2a: np.ndarray, dts: float, epssm: float, top_lid=True):
+def calc_coef_w(metrics, mut: np.ndarray, c2a: np.ndarray, dts: float, epssm: float, top_lid=True,
+                cqw_raw: np.ndarray | None = None):
+    """The implicit w operator's (a, alpha, gamma); ``cqw_raw`` (nz+1, ny, nx) the raw
+    moisture face factor (None: the dry state)."""
+
     L = lib()
     keep = _Keep()
     g = _grid(metrics, keep)
     shape = (metrics.nz + 1, metrics.ny, metrics.nx)
     a, alpha, gamma = (np.zeros(shape) for _ in range(3))
-    L.orc_calc_coef_w(ctypes.byref(g), keep(mut), keep(c2a), ctypes.c_double(dts),
-                      ctypes.c_double(epssm), ctypes.c_int(int(top_lid)), _ptr(a), _ptr(alpha),
-                      _ptr(gamma))
+    L.orc_calc_coef_w_cq.restype = ctypes.c_int
+    L.orc_calc_coef_w_cq(ctypes.byref(g), keep(mut), keep(c2a), ctypes.c_double(dts),
+                         ctypes.c_double(epssm), ctypes.c_int(int(top_lid)),
+                         None if cqw_raw is None else keep(cqw_raw), _ptr(a), _ptr(alpha), _ptr(gamma))
     return a, alpha, gamma
 
 
@@ -266,7 +282,7 @@ def smag_tendency(metrics, state: dict, c_s: float) -> tuple[dict, np.ndarray]:
     keep = _Keep()
     g = _grid(metrics, keep)
     st = _state_copy(metrics, state)
-    s = _State(*[_ptr(st[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    s = _mk_state(st)
     nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
     out = {"u": np.zeros((nz, ny, nx + 1)), "v": np.zeros((nz, ny + 1, nx)), "w": np.zeros((nz + 1, ny, nx)),
            "theta": np.zeros((nz, ny, nx))}
@@ -288,12 +304,34 @@ def coriolis_tendency(metrics, state: dict) -> dict:
     keep = _Keep()
     g = _grid(metrics, keep)
     st = _state_copy(metrics, state)
-    s = _State(*[_ptr(st[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    s = _mk_state(st)
     nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
     out = {"u": np.zeros((nz, ny, nx + 1)), "v": np.zeros((nz, ny + 1, nx)), "w": np.zeros((nz + 1, ny, nx))}
     rc = L.orc_coriolis_tendency(ctypes.byref(g), ctypes.byref(s), _ptr(out["u"]), _ptr(out["v"]), _ptr(out["w"]))
     if rc != 0:
         raise RuntimeError("orc_coriolis_tendency failed")
+    return out
+
+
+def moist_factors(metrics, state: dict) -> dict:
+    """The moisture coupling factors of ``state`` (which holds the six species):
+    qtot on mass levels, cqu and cqv on u and v faces
+    (acoustic_wrf.py:303-319) and the raw calc_cq cqw on w faces
+    (core/advance_w.py:195-219)."""
+
+    L = lib()
+    L.orc_moist_factors.restype = ctypes.c_int
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    st = _state_copy(metrics, state)
+    s = _mk_state(st)
+    nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
+    out = {"qtot": np.zeros((nz, ny, nx)), "cqu": np.zeros((nz, ny, nx + 1)), "cqv": np.zeros((nz, ny + 1, nx)),
+           "cqw": np.zeros((nz + 1, ny, nx))}
+    rc = L.orc_moist_factors(ctypes.byref(g), ctypes.byref(s), _ptr(out["qtot"]), _ptr(out["cqu"]),
+                             _ptr(out["cqv"]), _ptr(out["cqw"]))
+    if rc != 0:
+        raise RuntimeError("orc_moist_factors failed")
     return out
 
 
@@ -307,7 +345,7 @@ def constk_tendency(metrics, state: dict, nu: float) -> tuple[dict, float]:
     keep = _Keep()
     g = _grid(metrics, keep)
     st = _state_copy(metrics, state)
-    s = _State(*[_ptr(st[_STATE_KEYS[n]]) for n, _ in _State._fields_])
+    s = _mk_state(st)
     nx, ny, nz = metrics.nx, metrics.ny, metrics.nz
     out = {"u": np.zeros((nz, ny, nx + 1)), "v": np.zeros((nz, ny + 1, nx)), "w": np.zeros((nz + 1, ny, nx)),
            "theta": np.zeros((nz, ny, nx))}

@@ -514,8 +514,11 @@ static inline real log_alpha(const orc_grid *g, int k, real dph, real mass) {
     return dph / phm / wd_log(pfd / pfu);
 }
 
+/* cqu, cqv: the moisture factors on u and v faces (ru_tend -= cqu dpx,
+ * rk_addtend_dry.py:278, :359-360); NULL for the dry state's 1. */
 static void large_step_pgf(const orc_grid *g, int hypsometric_opt, const rstate *S, const real *alt,
-                           real *al, real *php, real *ut, real *vt, real *dpn_col) {
+                           real *al, real *php, real *ut, real *vt, real *dpn_col, const real *cqu,
+                           const real *cqv) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const real rdx = 1.0 / g->dx, rdy = 1.0 / g->dy;
     real dn_top = g->dn[nz - 1];
@@ -597,7 +600,7 @@ static void large_step_pgf(const orc_grid *g, int hypsometric_opt, const rstate 
                 real dpx = (((msf_u * 0.5) * rdx) * mass_u) * ((ph_term + p_term) + pb_term);
                 real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (mu_l + mu_r));
                 dpx = dpx + ((msf_u * rdx) * (php[IM(k, j, ir)] - php[IM(k, j, il)])) * bracket;
-                ut[IU(k, j, i)] = ut[IU(k, j, i)] + (-1.0 * dpx);
+                ut[IU(k, j, i)] = ut[IU(k, j, i)] + (-(cqu ? cqu[IU(k, j, i)] : (real)1.0)) * dpx;
             }
         }
         free(dpn);
@@ -642,7 +645,7 @@ static void large_step_pgf(const orc_grid *g, int hypsometric_opt, const rstate 
                 real dpy = (((msf_v * 0.5) * rdy) * mass_v) * ((ph_term + p_term) + pb_term);
                 real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (mu_s + mu_n));
                 dpy = dpy + ((msf_v * rdy) * (php[IM(k, jn, i)] - php[IM(k, js, i)])) * bracket;
-                vt[IV(k, j, i)] = vt[IV(k, j, i)] + (-1.0 * dpy);
+                vt[IV(k, j, i)] = vt[IV(k, j, i)] + (-(cqv ? cqv[IV(k, j, i)] : (real)1.0)) * dpy;
             }
         }
         free(dpn);
@@ -738,11 +741,17 @@ static inline real cor_w(const orc_grid *g, size_t c, real mxy, real ru_f, real 
 /* (core/advance_w.py:266-290 with qtot = 0).                           */
 /* ------------------------------------------------------------------ */
 static inline real cqw_at(int kf, int nz) { return (kf >= 1 && kf < nz) ? 1.0 : 0.0; }
+/* cq1 = 1/(1 + cqw) of pg_buoy_w_moist (core/advance_w.py:266-290) from the
+ * raw calc_cq face value; 1 for the dry state's cqw = 0. */
+static inline real cq1_of(real raw) { return 1.0 / (1.0 + raw); }
 
 static void calc_coef_w_column(const orc_grid *g, real mut, const real *c2a, size_t sc,
                                double dts, double epssm, int top_lid, real *a, real *alpha,
-                               real *gamma, size_t sw) {
+                               real *gamma, size_t sw, const real *cqwr, size_t scq) {
     const int nz = g->nz;
+    /* the post-pg_buoy_w cqw: cq1 inside, 0 at faces 0 and nz; cqwr (the raw
+     * calc_cq field of the column, stride scq) NULL for the dry state */
+#define CQW(k) (((k) >= 1 && (k) < nz) ? (cqwr ? cq1_of(cqwr[(size_t)(k) * scq]) : cqw_at((k), nz)) : (real)0.0)
     real cof = (0.5 * dts * G * (1.0 + epssm));
     cof = cof * cof;
     real lid_flag = top_lid ? 0.0 : 1.0;
@@ -764,15 +773,15 @@ static void calc_coef_w_column(const orc_grid *g, real mut, const real *c2a, siz
         int k = kk - 1;
         real denom = MH(k) * MF(k);
         a[(size_t)kk * sw] =
-            (((((-cqw_at(kk, nz)) * cof) * g->rdn[kk]) * g->rdnw[kk - 1]) * C2A(kk - 1)) / denom;
+            (((((-CQW(kk)) * cof) * g->rdn[kk]) * g->rdnw[kk - 1]) * C2A(kk - 1)) / denom;
     }
     for (int k = 1; k < nz; ++k) {
         real du = MH(k) * MF(k);
         real dl = MH(k - 1) * MF(k);
         real dc = MH(k) * MF(k + 1);
-        real b = 1.0 + ((cqw_at(k, nz) * cof) * g->rdn[k]) *
+        real b = 1.0 + ((CQW(k) * cof) * g->rdn[k]) *
                              ((g->rdnw[k] * C2A(k)) / du + (g->rdnw[k - 1] * C2A(k - 1)) / dl);
-        real c = (((((-cqw_at(k, nz)) * cof) * g->rdn[k]) * g->rdnw[k]) * C2A(k)) / dc;
+        real c = (((((-CQW(k)) * cof) * g->rdn[k]) * g->rdnw[k]) * C2A(k)) / dc;
         real al = 1.0 / (b - a[(size_t)k * sw] * gamma[(size_t)(k - 1) * sw]);
         alpha[(size_t)k * sw] = al;
         gamma[(size_t)k * sw] = c * al;
@@ -783,18 +792,24 @@ static void calc_coef_w_column(const orc_grid *g, real mut, const real *c2a, siz
 #undef MH
 #undef MF
 #undef C2A
+#undef CQW
 }
 
 #ifndef ORC_AD
-int orc_calc_coef_w(const orc_grid *g, const double *mut, const double *c2a, double dts,
-                    double epssm, int top_lid, double *a, double *alpha, double *gamma) {
-    const int nx = g->nx, ny = g->ny, nz = g->nz;
+int orc_calc_coef_w_cq(const orc_grid *g, const double *mut, const double *c2a, double dts, double epssm,
+                       int top_lid, const double *cqw_raw, double *a, double *alpha, double *gamma) {
+    const int nx = g->nx, ny = g->ny;
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i)
             calc_coef_w_column(g, mut[I2(j, i)], c2a + I2(j, i), (size_t)ny * nx, dts, epssm, top_lid,
-                               a + I2(j, i), alpha + I2(j, i), gamma + I2(j, i), (size_t)ny * nx);
-    (void)nz;
+                               a + I2(j, i), alpha + I2(j, i), gamma + I2(j, i), (size_t)ny * nx,
+                               cqw_raw ? cqw_raw + I2(j, i) : NULL, (size_t)ny * nx);
     return 0;
+}
+
+int orc_calc_coef_w(const orc_grid *g, const double *mut, const double *c2a, double dts,
+                    double epssm, int top_lid, double *a, double *alpha, double *gamma) {
+    return orc_calc_coef_w_cq(g, mut, c2a, dts, epssm, top_lid, NULL, a, alpha, gamma);
 }
 
 /* Thomas sweeps, core/advance_w.py:558-592 and dynamics/tridiag_solve.py:15-57. */
@@ -891,7 +906,53 @@ typedef struct {
     /* step-origin mu_total for the limited theta advection; outside the
      * array table (NULL in the piece API, which refuses scalar_adv_opt) */
     real *mut0;
+    /* the moisture species and their work arrays (NULL: the dry state; always
+     * NULL in the piece API) */
+    struct moist_s *moist;
 } ws_t;
+
+/* The six moisture species qv, qc, qr, qi, qs, qg (operational_mode.py:3100)
+ * and the work arrays of their coupling and transport. */
+#define N_MOIST 6
+typedef struct moist_s {
+    real *q[N_MOIST];   /* the state's species, advanced in place */
+    real *q0[N_MOIST];  /* step-origin copies (WRF moist_old) */
+    real *qt[N_MOIST];  /* the stage's coupled tendencies d(mu q)/dt */
+    real *qtot;         /* total moisture on mass levels */
+    real *cqu, *cqv;    /* moist factors on u and v faces */
+    real *cqwr;         /* raw calc_cq cqw on w faces (0 at faces 0 and nz) */
+} moist_t;
+
+/* moisture_coupling_factors (dynamics/acoustic_wrf.py:303-319) and
+ * moist_cqw_calc_face (core/advance_w.py:195-219) of the stage-entry
+ * moisture: qtot = qv + qc + qr + qi + qs + qg on mass levels;
+ * cqu = 1/(1 + (qtot_left + qtot_right)/2) on u faces and cqv likewise on v
+ * faces, the neighbours clamped at the array edges; the raw cqw =
+ * (qtot(k) + qtot(k-1))/2 on the interior w faces. */
+static void moist_factors(const orc_grid *g, moist_t *m) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t Mn = (size_t)nz * ny * nx;
+    for (size_t q = 0; q < Mn; ++q)
+        m->qtot[q] = ((((m->q[0][q] + m->q[1][q]) + m->q[2][q]) + m->q[3][q]) + m->q[4][q]) + m->q[5][q];
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k) {
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const int il = clampi(i - 1, 0, nx - 1), ir = clampi(i, 0, nx - 1);
+                m->cqu[IU(k, j, i)] = 1.0 / (1.0 + 0.5 * (m->qtot[IM(k, j, il)] + m->qtot[IM(k, j, ir)]));
+            }
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int js = clampi(j - 1, 0, ny - 1), jn = clampi(j, 0, ny - 1);
+                m->cqv[IV(k, j, i)] = 1.0 / (1.0 + 0.5 * (m->qtot[IM(k, js, i)] + m->qtot[IM(k, jn, i)]));
+            }
+    }
+    for (int f = 0; f <= nz; ++f)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                m->cqwr[IM(f, j, i)] =
+                    (f >= 1 && f < nz) ? 0.5 * (m->qtot[IM(f, j, i)] + m->qtot[IM(f - 1, j, i)]) : (real)0.0;
+}
 
 /* Terrain surface w of advance_w (core/advance_w.py:505-556): the kinematic
  * condition from u and v extrapolated to the surface (cf1..cf3 over mass
@@ -1278,6 +1339,23 @@ int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double 
     return 0;
 }
 
+/* The moisture coupling factors alone, for tests: qtot, cqu, cqv and the
+ * raw cqw of the state's six species. */
+int orc_moist_factors(const orc_grid *g, const orc_state *s, double *qtot, double *cqu, double *cqv, double *cqwr) {
+    if (!g || !s || !s->qv || !s->qc || !s->qr || !s->qi || !s->qs || !s->qg || g->nx < 1 || g->ny < 1 || g->nz < 2)
+        return -1;
+    moist_t m;
+    memset(&m, 0, sizeof m);
+    double *qs[N_MOIST] = {s->qv, s->qc, s->qr, s->qi, s->qs, s->qg};
+    for (int sp = 0; sp < N_MOIST; ++sp) m.q[sp] = qs[sp];
+    m.qtot = qtot;
+    m.cqu = cqu;
+    m.cqv = cqv;
+    m.cqwr = cqwr;
+    moist_factors(g, &m);
+    return 0;
+}
+
 /* The Coriolis terms alone, for tests: ut, vt get large_step_coriolis and
  * rwt (nz+1 faces) the cosine-Coriolis part of rw_tend, with the face
  * masses of the stage mut = (mu_total - mu_pert) + mu_pert as
@@ -1534,6 +1612,22 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
             advect_scalar_flux(g, thp, W->ru, W->rv, W->rom, W->tht);
         }
     }
+    /* the moisture's coupling factors and, with moist_adv_opt, its coupled
+     * tendencies with the same transport velocities: the plain h5/v3 flux
+     * advection, the limiter on the final stage from the step origin
+     * (operational_mode.py:3162-3230, flux_advection.py:1002-1099) */
+    if (W->moist) {
+        moist_t *m = W->moist;
+        moist_factors(g, m);
+        if (nl->moist_adv_opt != 0)
+            for (int sp = 0; sp < N_MOIST; ++sp) {
+                if (rk_step == 3)
+                    advect_scalar_limited(g, nl->moist_adv_opt, m->q[sp], m->q0[sp], W->ru, W->rv, W->rom,
+                                          S->mu_total, W->mut0, nl->dt, m->qt[sp]);
+                else
+                    advect_scalar_flux(g, m->q[sp], W->ru, W->rv, W->rom, m->qt[sp]);
+            }
+    }
     /* 6th-order filter on the coupled u, v, w, theta tendencies, with the
      * face masses of the stage mu_total (operational_mode.py:2768-2774,
      * 2870-2876), before the large-step PGF is added. */
@@ -1559,7 +1653,8 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
      * branch used by small_step_prep (acoustic_wrf.py:354-358). */
 #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < Mn; ++q) W->alt_raw[q] = alt_from_theta_p(S->theta[q], S->p_total[q]);
-    large_step_pgf(g, nl->hypsometric_opt, S, W->alt_raw, W->al_ls, W->php_ls, W->ut, W->vt, NULL);
+    large_step_pgf(g, nl->hypsometric_opt, S, W->alt_raw, W->al_ls, W->php_ls, W->ut, W->vt, NULL,
+                   W->moist ? W->moist->cqu : NULL, W->moist ? W->moist->cqv : NULL);
     large_step_coriolis(g, S, W->ut, W->vt);
 
     /* --- small_step_prep_wrf, core/small_step_prep.py:216-394 --- */
@@ -1666,7 +1761,8 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
             real mub = W->mub[c];
             W->rw_tend[IM(0, j, i)] = 0.0;
             for (int k = 1; k < nz; ++k) {
-                real cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
+                real raw = W->moist ? W->moist->cqwr[IM(k, j, i)] : (real)0.0;
+                real cq1 = 1.0 / (1.0 + raw), cq2 = raw * cq1;
                 real rw = (msft_inv * G) *
                             (((cq1 * g->rdn[k]) * (S->p_pert[IM(k, j, i)] - S->p_pert[IM(k - 1, j, i)]) -
                               g->c1f[k] * mu_p) -
@@ -1674,7 +1770,9 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
                 W->rw_tend[IM(k, j, i)] = rw;
             }
             {
-                real cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
+                /* the top face takes the raw cqw of face nz - 1 */
+                real raw = W->moist ? W->moist->cqwr[IM(nz - 1, j, i)] : (real)0.0;
+                real cq1 = 1.0 / (1.0 + raw), cq2 = raw * cq1;
                 real rw = (msft_inv * G) *
                             (((((cq1 * 2.0) * g->rdnw[nz - 1]) * (-S->p_pert[IM(nz - 1, j, i)])) -
                               g->c1f[nz] * mu_p) -
@@ -1760,7 +1858,7 @@ static void stage_setup(const orc_grid *g, const orc_namelist *nl, rstate *S, ws
         for (int i = 0; i < nx; ++i)
             calc_coef_w_column(g, W->mut[I2(j, i)], W->c2a + I2(j, i), (size_t)ny * nx, dts, nl->epssm,
                                nl->top_lid, W->ca + I2(j, i), W->calpha + I2(j, i), W->cgamma + I2(j, i),
-                               (size_t)ny * nx);
+                               (size_t)ny * nx, W->moist ? W->moist->cqwr + I2(j, i) : NULL, (size_t)ny * nx);
 
     /* acoustic carried state initialisation */
     memcpy(W->muts, W->muts0, sizeof(real) * (size_t)ny * nx);
@@ -1813,7 +1911,7 @@ static void substep_uv(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_
                 real dpx = (((msf * 0.5) * rdx) * mass_x) * ((ph_term + p_term) + pb_term);
                 real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_l + muw_r));
                 dpx = dpx + ((msf * rdx) * (W->php[IM(k, j, ir)] - W->php[IM(k, j, il)])) * bracket;
-                u = u - (dts * 1.0) * dpx;
+                u = u - (dts * (W->moist ? W->moist->cqu[q] : (real)1.0)) * dpx;
                 u = u + g->c1h[k] * mudf_xy;
                 W->u_work[q] = u;
             }
@@ -1858,7 +1956,7 @@ static void substep_uv(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_
                 real dpy = (((msf * 0.5) * rdy) * mass_y) * ((ph_term + p_term) + pb_term);
                 real bracket = g->rdnw[k] * (dpn[k + 1] - dpn[k]) - 0.5 * (g->c1h[k] * (muw_s + muw_n));
                 dpy = dpy + ((msf * rdy) * (W->php[IM(k, jn, i)] - W->php[IM(k, js, i)])) * bracket;
-                v = v - (dts * 1.0) * dpy;
+                v = v - (dts * (W->moist ? W->moist->cqv[q] : (real)1.0)) * dpy;
                 v = v + g->c1h[k] * mudf_xy;
                 W->v_work[q] = v;
             }
@@ -2021,7 +2119,8 @@ static void substep_w(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t
                        phkm = W->ph_work[IM(f - 1, j, i)];
                 real tA_u = coef_u * (eps_p * (rhs[f + 1] - rhs[f]) + eps_m * (phk1 - phk));
                 real tA_l = coef_l * (eps_p * (rhs[f] - rhs[f - 1]) + eps_m * (phk - phkm));
-                real termA = ((msft_inv * cqw_at(f, nz)) * (c05dtg * g->rdn[f])) * (tA_u - tA_l);
+                real cqw_f = W->moist ? cq1_of(W->moist->cqwr[IM(f, j, i)]) : cqw_at(f, nz);
+                real termA = ((msft_inv * cqw_f) * (c05dtg * g->rdn[f])) * (tA_u - tA_l);
                 real bu = (W->c2a[IM(f, j, i)] * W->alt[IM(f, j, i)]) * t2n[f];
                 real bl = (W->c2a[IM(f - 1, j, i)] * W->alt[IM(f - 1, j, i)]) * t2n[f - 1];
                 real termB = ((dts * G) * msft_inv) * (g->rdn[f] * (bu - bl) - (g->c1f[f] * muave));
@@ -2195,6 +2294,26 @@ static void stage_finish(const orc_grid *g, const orc_namelist *nl, rstate *S, w
         }
 }
 
+/* The moisture's large-step update after the acoustic loop
+ * (_apply_moisture_large_step, operational_mode.py:3273-3316):
+ * q = (m_origin q_origin + dt_rk qt) / m_new with the column masses
+ * c1h mu + c2h of the step origin and of the finished stage. */
+static void moist_update(const orc_grid *g, const rstate *S, ws_t *W, double dt_rk) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    moist_t *m = W->moist;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const size_t c = I2(j, i), q = IM(k, j, i);
+                real mass_old = g->c1h[k] * W->mut0[c] + g->c2h[k];
+                real mass_new = g->c1h[k] * S->mu_total[c] + g->c2h[k];
+                real inv_mass_new = 1.0 / mass_new;
+                for (int sp = 0; sp < N_MOIST; ++sp)
+                    m->q[sp][q] = (mass_old * m->q0[sp][q] + dt_rk * m->qt[sp][q]) * inv_mass_new;
+            }
+}
+
 static void rk_stage(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t *W, int rk_step,
                      double dts, int nsub) {
     stage_setup(g, nl, S, W, rk_step, dts);
@@ -2205,6 +2324,9 @@ static void rk_stage(const orc_grid *g, const orc_namelist *nl, rstate *S, ws_t 
         substep_p(g, nl, S, W);
     }
     stage_finish(g, nl, S, W);
+    /* the stage's dt_rk: dt/3, dt/2, dt (operational_mode.py:3527-3529) */
+    if (W->moist && nl->moist_adv_opt != 0)
+        moist_update(g, S, W, rk_step == 1 ? nl->dt / 3.0 : (rk_step == 2 ? 0.5 * nl->dt : nl->dt));
 }
 
 /* The rk1 reference of a large step: copies of the state it starts from
@@ -2221,6 +2343,8 @@ static void step_origin(const orc_grid *g, const orc_namelist *nl, rstate *s, ws
     memcpy(W->ph0, s->ph_pert, Wn * sizeof(real));
     memcpy(W->mu0, s->mu_pert, C * sizeof(real));
     if (W->mut0) memcpy(W->mut0, s->mu_total, C * sizeof(real));
+    if (W->moist)
+        for (int sp = 0; sp < N_MOIST; ++sp) memcpy(W->moist->q0[sp], W->moist->q[sp], M * sizeof(real));
 }
 
 /* One large step in place: the origin, then the solve_em.F cadence of
@@ -2274,9 +2398,46 @@ int orc_run(const orc_grid *g, const orc_namelist *nl, orc_state *s, int n_steps
     }
     rstate rs = {s->u, s->v, s->w, s->theta, s->p_total, s->p_pert,
                  s->ph_total, s->ph_pert, s->mu_total, s->mu_pert, s->mudf};
-    for (int step = 0; step < n_steps; ++step) large_step(g, nl, &rs, &W);
+    /* the moisture: all six species or none */
+    double *qs[N_MOIST] = {s->qv, s->qc, s->qr, s->qi, s->qs, s->qg};
+    int nq = 0;
+    for (int sp = 0; sp < N_MOIST; ++sp) nq += qs[sp] != NULL;
+    if (nq != 0 && nq != N_MOIST) {
+        ws_free(&W);
+        return -6;
+    }
+    moist_t moist;
+    memset(&moist, 0, sizeof moist);
+    int rc = 0;
+    if (nq == N_MOIST) {
+        const size_t Mn = (size_t)nz * ny * nx;
+        for (int sp = 0; sp < N_MOIST; ++sp) {
+            moist.q[sp] = qs[sp];
+            moist.q0[sp] = zalloc(Mn);
+            moist.qt[sp] = zalloc(Mn);
+            if (!moist.q0[sp] || !moist.qt[sp]) rc = -1;
+        }
+        moist.qtot = zalloc(Mn);
+        moist.cqu = zalloc((size_t)nz * ny * (nx + 1));
+        moist.cqv = zalloc((size_t)nz * (ny + 1) * nx);
+        moist.cqwr = zalloc((size_t)(nz + 1) * ny * nx);
+        if (!moist.qtot || !moist.cqu || !moist.cqv || !moist.cqwr) rc = -1;
+        W.moist = &moist;
+    }
+    if (rc == 0)
+        for (int step = 0; step < n_steps; ++step) large_step(g, nl, &rs, &W);
+    if (nq == N_MOIST) {
+        for (int sp = 0; sp < N_MOIST; ++sp) {
+            free(moist.q0[sp]);
+            free(moist.qt[sp]);
+        }
+        free(moist.qtot);
+        free(moist.cqu);
+        free(moist.cqv);
+        free(moist.cqwr);
+    }
     ws_free(&W);
-    return 0;
+    return rc;
 }
 
 int orc_piece(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,

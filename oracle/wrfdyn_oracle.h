@@ -66,6 +66,9 @@ typedef struct {
                               1 positive-definite, 2 monotonic limiter */
     int    hypsometric_opt; /* 1 linear al (dp = mu d eta), 2 log-pressure al */
     double const_nu_m2_s;   /* > 0: constant-K flux-form diffusion, K in m^2/s */
+    int    moist_adv_opt;   /* moisture transport: 0 off, 1 positive-definite,
+                               2 monotonic limiter on the final stage (plain
+                               h5/v3 on the others) */
 } orc_namelist;
 
 typedef struct {
@@ -74,6 +77,9 @@ typedef struct {
     double *ph_total, *ph_pert;
     double *mu_total, *mu_pert;
     double *mudf;
+    /* the six moisture species (mixing ratios, nz*ny*nx each), all six or
+     * none: NULL is the dry state */
+    double *qv, *qc, *qr, *qi, *qs, *qg;
 } orc_state;
 
 /* Advance n_steps large steps in place.  Returns 0 or a negative error. */
@@ -106,6 +112,10 @@ int orc_piece(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step,
 int orc_calc_coef_w(const orc_grid *g, const double *mut, const double *c2a,
                     double dts, double epssm, int top_lid,
                     double *a, double *alpha, double *gamma);
+/* the same with the moisture: cqw_raw (nz+1, ny, nx) the raw calc_cq face
+ * field (0.5 (qtot(k) + qtot(k-1)) inside), NULL for the dry state */
+int orc_calc_coef_w_cq(const orc_grid *g, const double *mut, const double *c2a, double dts, double epssm,
+                       int top_lid, const double *cqw_raw, double *a, double *alpha, double *gamma);
 void orc_thomas(int nzp1, const double *a, const double *alpha, const double *gamma,
                 const double *rhs, double *out);
 double orc_alt_from_theta_p(double theta, double p);
@@ -130,6 +140,7 @@ int orc_advect_limited(const orc_grid *g, int opt, const double *u, const double
  * state s with K = nu, and the layer thickness dz it uses; 0 on success. */
 int orc_constk_tendency(const orc_grid *g, double nu, const orc_state *s, double *ut, double *vt, double *wt,
                         double *tht, double *dz);
+int orc_moist_factors(const orc_grid *g, const orc_state *s, double *qtot, double *cqu, double *cqv, double *cqwr);
 int orc_coriolis_tendency(const orc_grid *g, const orc_state *s, double *ut, double *vt, double *rwt);
 int orc_smag_tendency(const orc_grid *g, double c_s, const orc_state *s, double *ut, double *vt, double *wt,
                       double *tht, double *xkmh);

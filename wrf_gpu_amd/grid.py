@@ -164,6 +164,8 @@ STATE_FIELDS = (
     "u", "v", "w", "theta", "p_total", "p_perturbation", "ph_total", "ph_perturbation",
     "mu_total", "mu_perturbation", "mudf",
 )
+# the moisture species a moist state adds, each (nz, ny, nx) (operational_mode.py:3100)
+MOIST_FIELDS = ("qv", "qc", "qr", "qi", "qs", "qg")
 
 
 @dataclass(frozen=True)
@@ -190,6 +192,12 @@ class Namelist:
     c_s: float = 0.25
     scalar_adv_opt: int = 0
     const_nu_m2_s: float = 0.0
+    # the state carries the six moisture species qv, qc, qr, qi, qs, qg
+    # (the reference's State always does; a dry state leaves them out)
+    moist: bool = False
+    # moisture transport (operational_mode.py:495-522): 0 off, 1 positive-definite,
+    # 2 monotonic limiter on the final RK stage, plain h5/v3 on the others
+    moist_adv_opt: int = 0
     use_flux_advection: bool = True
     hypsometric_opt: int = 1
     h_sca_adv_order: int = 2
@@ -223,6 +231,10 @@ class Namelist:
             raise ValueError("c_s must be >= 0")
         if not (self.const_nu_m2_s >= 0.0 and math.isfinite(self.const_nu_m2_s)):
             raise ValueError("const_nu_m2_s must be finite and >= 0")
+        if self.moist_adv_opt not in (0, 1, 2):
+            raise ValueError("moist_adv_opt must be 0 (off), 1 (positive-definite) or 2 (monotonic)")
+        if self.moist_adv_opt != 0 and not self.moist:
+            raise ValueError("moist_adv_opt transports moisture: it needs moist=True (a state with qv..qg)")
         if self.const_nu_m2_s > 0.0 and self.diff_opt == 1:
             raise ValueError("const_nu_m2_s > 0 and the 2-D Smagorinsky diffusion do not run together")
         if self.scalar_adv_opt not in (0, 1, 2):

@@ -153,6 +153,36 @@ def warm_bubble_2d() -> Case:
                 {"x_m": x, "z_m": z, "snapshot_steps": (1000, 2500, 5000)})
 
 
+def with_moisture(case: Case, seed: int = 7, qv_surface: float = 0.012, moist_adv_opt: int = 1) -> Case:
+    """``case`` with the six moisture species added (a moist state): qv decaying with height
+    from ``qv_surface`` (2.5 km scale) with seeded 5 % noise, a cloud-water blob of 1 g/kg and
+    small rain, ice, snow and graupel blobs beside it, all non-negative, and the namelist's
+    ``moist`` flag and ``moist_adv_opt`` set."""
+
+    m = case.metrics
+    nz, ny, nx = m.nz, m.ny, m.nx
+    ph = case.state["ph_total"]
+    z = 0.5 * (ph[:-1] + ph[1:]) / GRAVITY
+    rng = np.random.default_rng(seed)
+    qv = qv_surface * np.exp(-np.maximum(z - z[0:1], 0.0) / 2500.0) * (1.0 + 0.05 * rng.uniform(-1, 1, (nz, ny, nx)))
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    kk = np.arange(nz)[:, None, None]
+
+    def blob(ci, cj, ck, r, amp):
+        d = np.sqrt(((ii[None] - ci) / r) ** 2 + ((jj[None] - cj) / max(r * ny / nx, 0.5)) ** 2 + ((kk - ck) / 2.0) ** 2)
+        return amp * np.where(d < 1.0, np.cos(0.5 * np.pi * d) ** 2, 0.0)
+
+    state = dict(case.state)
+    state["qv"] = np.ascontiguousarray(qv)
+    state["qc"] = np.ascontiguousarray(blob(nx * 0.4, ny * 0.5, nz * 0.3, max(nx / 6, 1.5), 1.0e-3))
+    state["qr"] = np.ascontiguousarray(blob(nx * 0.6, ny * 0.4, nz * 0.2, max(nx / 8, 1.5), 4.0e-4))
+    state["qi"] = np.ascontiguousarray(blob(nx * 0.5, ny * 0.6, nz * 0.7, max(nx / 7, 1.5), 2.0e-4))
+    state["qs"] = np.ascontiguousarray(blob(nx * 0.3, ny * 0.3, nz * 0.6, max(nx / 9, 1.5), 1.0e-4))
+    state["qg"] = np.ascontiguousarray(blob(nx * 0.7, ny * 0.7, nz * 0.4, max(nx / 9, 1.5), 1.5e-4))
+    nl = case.namelist.replace(moist=True, moist_adv_opt=int(moist_adv_opt))
+    return Case(case.name + "_moist", m, nl, state, case.n_steps, dict(case.info, moist_seed=seed))
+
+
 def density_current_2d() -> Case:
     """The reference's 2-D Straka density current slab (idealized.py:277-330, :566-602).
 
