@@ -23,7 +23,8 @@ another, and the batched graph in FP64 and in the mixed form), and the same
 workload with the 6th-order numerical filter on (diff_6th_opt = 2), with
 the 2-D Smagorinsky horizontal diffusion on (diff_opt = 1, km_opt = 4) and
 with the monotonic limiter on theta's final-stage advection
-(scalar_adv_opt = 2).
+(scalar_adv_opt = 2), and carrying the six moisture species, coupled and
+transported (moist = 1, moist_adv_opt = 2).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
   python bench.py --pmc-summary FETCH_DIR WRITE_DIR --calibration-copy-mib 1024 > summary.json
@@ -96,6 +97,9 @@ def parse():
                     help="large steps of the monotonic theta-limiter leg on the same workload (0 skips it)")
     ap.add_argument("--constk-steps", type=int, default=20,
                     help="large steps of the constant-K diffusion leg on the same workload (0 skips it)")
+    ap.add_argument("--moist-steps", type=int, default=20,
+                    help="large steps of the moist leg (six species, coupled and transported) on the same workload "
+                         "(0 skips it)")
     ap.add_argument("--small-steps", type=int, default=100,
                     help="large steps of the 64x64x40 warm-bubble leg (eager and graph-replayed); 0 skips it")
     ap.add_argument("--ensemble-steps", type=int, default=3,
@@ -281,6 +285,56 @@ def diffusion_leg(case, steps: int, fp64_ms_per_step: float, dev, kind: str) -> 
             kernel: {"avg_ms_per_stage": avg, "alg_bytes_per_stage": alg_bytes,
                      "achieved_GBs": alg_bytes / (avg * 1e-3) / 1e9, "peak_GBs": 8000.0,
                      "frac": alg_bytes / (avg * 1e-3) / 8e12}}
+
+
+def moist_leg(case, steps: int, fp64_ms_per_step: float, dev) -> dict:
+    """The same workload carrying the six moisture species (ic.with_moisture:
+    qv decaying with height, cloud and precipitation blobs), coupled into the
+    dynamics and transported with the monotonic limiter on the final stage
+    (moist = 1, moist_adv_opt = 2): throughput, and per large step the
+    moisture kernels against their algorithmic bytes.
+
+    k_moist per step: three times the factors (read six species, write cqu,
+    cqv and the raw cqw), two stages of plain flux advection of each species
+    (read it, ru, rv, rom; write its tendency) and one stage of the two-pass
+    limiter per species (the theta limiter's bytes).  k_moist_update per
+    step: three updates (read six origin species and six tendencies, write
+    six species; the 2-D masses are negligible)."""
+
+    import torch
+
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import Dycore, state_to_device
+
+    mcase = ic.with_moisture(case, moist_adv_opt=2)
+    m, nl = mcase.metrics, mcase.namelist
+    nx, ny, nz = m.nx, m.ny, m.nz
+    eu, ev, ew, em = nz * ny * (nx + 1), nz * (ny + 1) * nx, (nz + 1) * ny * nx, nz * ny * nx
+    factors = 8 * (6 * em + eu + ev + ew)
+    plain = 8 * (4 * em + ew)
+    limited = 8 * ((4 * em + ew + 2 * em) + (4 * em + ew + 2 * em + em))
+    alg = {"k_moist": 3 * factors + 6 * (2 * plain + limited), "k_moist_update": 3 * 8 * 18 * em}
+    stream = torch.cuda.current_stream(dev)
+    dyn = Dycore(m, nl)
+    st = state_to_device(mcase.state, dev)
+    ms = timed_steps(lambda n: dyn.step(st, n), steps, stream) / steps
+    finite = all(bool(torch.isfinite(t).all()) for t in st.values())
+    dyn.profile(list(alg))
+    dyn.step(st, 2)
+    torch.cuda.synchronize(dev)
+    prof = dyn.profile_read()
+    dyn.close()
+    out = {"form": "moist=1,moist_adv_opt=2", "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3),
+           "unit": "cell-substeps/s", "ms_per_step": ms, "steps": steps,
+           "overhead_vs_fp64": ms / fp64_ms_per_step, "finite": finite}
+    for k, b in alg.items():
+        tot_ms, _ = prof[k]
+        per_step = tot_ms / 2
+        out[k] = {"ms_per_step": per_step, "alg_bytes_per_step": b, "achieved_GBs": b / (per_step * 1e-3) / 1e9,
+                  "peak_GBs": 8000.0, "frac": b / (per_step * 1e-3) / 8e12}
+    del st
+    torch.cuda.empty_cache()
+    return out
 
 
 def small_workload_leg(steps: int, dev) -> dict:
@@ -499,6 +553,9 @@ def main():
     if args.constk_steps > 0:
         forms["constk"] = diffusion_leg(case, args.constk_steps, ms_per_step, dev, "constk")
         print(f"[bench] constant-K leg done", file=sys.stderr, flush=True)
+    if args.moist_steps > 0:
+        forms["moist"] = moist_leg(case, args.moist_steps, ms_per_step, dev)
+        print(f"[bench] moist leg done", file=sys.stderr, flush=True)
     if args.small_steps > 0:
         forms["small_workload"] = small_workload_leg(args.small_steps, dev)
         print(f"[bench] small workload leg done", file=sys.stderr, flush=True)

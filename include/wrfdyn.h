@@ -100,6 +100,19 @@ typedef struct {
                                and theta with this K in m^2/s
                                (operational_mode.py:2878-2929); not together
                                with the Smagorinsky option */
+    int moist;              /* 1: the state carries the six moisture species
+                               (wd_state qv..qg) and they couple into the
+                               dynamics: cqu/cqv in the large-step and acoustic
+                               u/v pressure gradients, the moist rw_tend and
+                               implicit-w cqw (acoustic_wrf.py:303-319,
+                               core/advance_w.py:195-290,
+                               core/rk_addtend_dry.py:278, :359-360);
+                               0: a dry state */
+    int moist_adv_opt;      /* with moist: 0 the species are not transported,
+                               1 or 2 flux-form h5/v3 transport with the
+                               positive-definite (1) or monotonic (2) limiter
+                               on the final RK stage and the large-step update
+                               (operational_mode.py:3162-3316, :3379-3393) */
 } wd_namelist;
 
 /* Prognostic State (src/gpuwrf/contracts/state.py:811-821) plus the carried
@@ -111,6 +124,10 @@ typedef struct {
     double *ph_total, *ph_pert;
     double *mu_total, *mu_pert;
     double *mudf;
+    /* the moisture species (State qv, qc, qr, qi, qs, qg,
+     * contracts/state.py:98-117), each nz*ny*nx on mass points: all six with a
+     * handle created with moist = 1, all NULL otherwise */
+    double *qv, *qc, *qr, *qi, *qs, *qg;
 } wd_state;
 
 /* Create a handle for one grid + namelist; allocates the device workspace on

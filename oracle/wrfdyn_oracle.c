@@ -1,6 +1,6 @@
 /*
  * CPU oracle: FP64 restatement of the reference's split-explicit ARW large
- * step (dry, periodic idealized path).
+ * step (periodic idealized path, dry or with the six moisture species).
  *
  * TEST INFRASTRUCTURE ONLY -- the checker for the HIP product path, and the
  * timed CPU baseline.  Never linked into or called by the product.
@@ -31,10 +31,19 @@
  * (the large-step PGF and rhs_ph with a varying base geopotential, the
  * terrain surface w) converge at second order to their continuous forms;
  * the w curvature term, w_damping above a vertical CFL of 1, the
- * hypsometric_opt=2 refresh mass and the Coriolis terms have closed forms.
+ * hypsometric_opt=2 refresh mass and the Coriolis terms have closed forms;
+ * the moisture factors have closed forms, zero moisture reduces to the dry
+ * step bit for bit, the moist rw_tend and the large-step cqu are pinned at
+ * first order in dt, the implicit-w rows by their per-face cq1 scaling and
+ * the species' transport by conservation and positivity
+ * (test_oracle_moist.py).
  *
- * Parity unpinned: none of the step's terms.  What no reference file holds
- * is the reference's own field-level FP64 output beyond the 2-D slabs.
+ * Parity unpinned: the moisture factor of the explicit part of the implicit
+ * w equation (substep_w termA) and of the acoustic u/v pressure gradient
+ * (substep_uv).  Both act on work arrays that start each stage at zero, so
+ * they enter a step at second order in dt only, which no closed form above
+ * isolates.  What no reference file holds is the reference's own
+ * field-level FP64 output beyond the 2-D slabs.
  */
 #include "wrfdyn_oracle.h"
 #include "wrfdyn_math.h"

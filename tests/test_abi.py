@@ -125,6 +125,24 @@ def test_create_rejects_bad_arguments_without_touching_the_device():
     lib.wd_destroy(None)
 
 
+def test_create_validates_the_moisture_options_on_the_host():
+    """moist must be 0 or 1, moist_adv_opt 0..2 and only with moist: refused before any
+    device call (here without a GPU)."""
+
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import _abi_structs
+
+    lib = _lib.load()
+    case = ic.bubble_3d(nx=8, ny=6, nz=6, noise=0.0)
+    keep: list = []
+    grid, nl = _abi_structs(case.metrics, case.namelist, keep)
+    h = ctypes.c_void_p()
+    for moist, opt, msg in ((2, 0, "moist must be"), (1, 3, "moist_adv_opt must be"), (0, 1, "needs moist")):
+        nl.moist, nl.moist_adv_opt = moist, opt
+        assert lib.wd_create(ctypes.byref(grid), ctypes.byref(nl), ctypes.byref(h)) == _lib.WD_EINVAL
+        assert msg in _lib.last_error(None)
+
+
 def test_graph_batch_autotune_and_adjoint_entry_points_reject_bad_arguments():
     lib = _lib.load()
     assert lib.wd_capture_graph(None, None) == _lib.WD_EINVAL
@@ -156,7 +174,8 @@ def test_namelist_validation_matches_supported_path():
     for bad in (dict(top_lid=False), dict(hypsometric_opt=3), dict(damp_opt=1), dict(w_damping=2),
                 dict(acoustic_substeps=0), dict(run_physics=True), dict(h_sca_adv_order=5),
                 dict(const_nu_m2_s=-1.0), dict(const_nu_m2_s=float("inf")),
-                dict(const_nu_m2_s=75.0, diff_opt=1, km_opt=4)):
+                dict(const_nu_m2_s=75.0, diff_opt=1, km_opt=4), dict(moist_adv_opt=1),
+                dict(moist=True, moist_adv_opt=3)):
         with pytest.raises(ValueError):
             nl.replace(**bad).validate()
 

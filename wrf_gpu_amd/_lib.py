@@ -52,13 +52,14 @@ class WdNamelist(ctypes.Structure):
         ("diff_6th_opt", ctypes.c_int), ("diff_6th_factor", ctypes.c_double),
         ("diff_opt", ctypes.c_int), ("km_opt", ctypes.c_int), ("c_s", ctypes.c_double),
         ("scalar_adv_opt", ctypes.c_int), ("hypsometric_opt", ctypes.c_int),
-        ("const_nu_m2_s", ctypes.c_double),
+        ("const_nu_m2_s", ctypes.c_double), ("moist", ctypes.c_int), ("moist_adv_opt", ctypes.c_int),
     ]
 
 
 class WdState(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
-        "u", "v", "w", "theta", "p_total", "p_pert", "ph_total", "ph_pert", "mu_total", "mu_pert", "mudf")]
+        "u", "v", "w", "theta", "p_total", "p_pert", "ph_total", "ph_pert", "mu_total", "mu_pert", "mudf",
+        "qv", "qc", "qr", "qi", "qs", "qg")]
 
 
 _lib = None

@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
       double dpx = (((msf_u * 0.5) * g.rdx) * mass_x) * ((ph_term + p_term) + pb_term);
       double bracket = g.rdnw[k] * (dpn_u1 - dpn_u) - 0.5 * (g.c1h[k] * muwsum_u);
       dpx = dpx + ((msf_u * g.rdx) * (phpc - cur.phpw)) * bracket;
-      u = u - (dts * 1.0) * dpx;
+      u = u - (dts * (w.cqu ? ldo(w.cqu, q) : 1.0)) * dpx;
       u = u + g.c1h[k] * mudf_u;
       sto(w.u_work, q, u);
       phw = phw1;
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
       double dpy = (((msf_v * 0.5) * g.rdy) * mass_y) * ((ph_term + p_term) + pb_term);
       double bracket = g.rdnw[k] * (dpn_v1 - dpn_v) - 0.5 * (g.c1h[k] * muwsum_v);
       dpy = dpy + ((msf_v * g.rdy) * (phpc - cur.phps)) * bracket;
-      v = v - (dts * 1.0) * dpy;
+      v = v - (dts * (w.cqv ? ldo(w.cqv, q) : 1.0)) * dpy;
       v = v + g.c1h[k] * mudf_v;
       sto(w.v_work, q, v);
       phs = phs1;
@@ -1011,8 +1011,8 @@ __global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState 
 
   const double c05geps = (0.5 * kG) * eps_m;
   const double c05dtg = (0.5 * dts) * kG;
-  struct Wv {  // w level data (c2a, alt, t2n: mass level f)
-    double ph1t, pht, wold, phw, rwt, ca, cal, ww, c2a, alt, t2n;
+  struct Wv {  // w level data (c2a, alt, t2n: mass level f; cqw: the moisture factor of face f)
+    double ph1t, pht, wold, phw, rwt, ca, cal, ww, c2a, alt, t2n, cqw;
   };
   Wv w_1{}, w_2{}, w_3{};                   // levels f-1, f-2, f-3
   double d_1 = 0.0, d_2 = 0.0;              // phi rhs at f-2, f-3
@@ -1030,6 +1030,8 @@ __global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState 
       L.ca = ldo(w.ca, q);
       L.cal = ldo(w.calpha, q);
       L.ww = ldo(w.ww, q);
+      // the post-pg_buoy_w cqw inside: cq1 = 1/(1 + raw cqw) with moisture
+      L.cqw = w.cqwr ? 1.0 / (1.0 + ldo(w.cqwr, q)) : cqw_at(f, nz);
       if (f < nz) {
         L.c2a = ldo(w.c2a, q);
         L.alt = ldo(w.alt, q);
@@ -1088,7 +1090,7 @@ __global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState 
         double phk1 = w_1.phw, phk = w_2.phw, phkm = w_3.phw;
         double tA_u = coef_u * (eps_p * (d0 - d_1) + eps_m * (phk1 - phk));
         double tA_l = coef_l * (eps_p * (d_1 - d_2) + eps_m * (phk - phkm));
-        double termA = ((msft_inv * cqw_at(fa, nz)) * (c05dtg * g.rdn[fa])) * (tA_u - tA_l);
+        double termA = ((msft_inv * w_2.cqw) * (c05dtg * g.rdn[fa])) * (tA_u - tA_l);
         double bu = (c2a_u * w_2.alt) * w_2.t2n;
         double bl = (c2a_l * w_3.alt) * w_3.t2n;
         double termB = ((dts * kG) * msft_inv) * (g.rdn[fa] * (bu - bl) - (g.c1f[fa] * muave));
@@ -1368,7 +1370,8 @@ hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DS
     WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, float><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
   else
     WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, double><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
-  return launch_column_tile(column_tile, g, nl, s, w, dts, first, st, prof);
+  // the LDS-tiled kernel has no moisture factor: a moist state runs the column march
+  return launch_column_tile(w.cqwr ? kColumnMarch : column_tile, g, nl, s, w, dts, first, st, prof);
 }
 
 hipError_t launch_substep_part(int part, const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
@@ -1406,7 +1409,7 @@ const char* kernel_name(int id) {
       "origin_copy", "k_ruv", "k_rom", "k_prep_col", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t",
       "k_prep_2d", "k_prep_faces", "k_prep_u", "k_prep_v", "k_col_setup",
       "k_uv", "k_column", "k_column_mu", "k_column_w", "k_fin_uv", "k_fin_col", "k_diff6", "k_smag", "k_limiter",
-      "k_constk", "k_coriolis"};
+      "k_constk", "k_coriolis", "k_moist", "k_moist_update"};
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 

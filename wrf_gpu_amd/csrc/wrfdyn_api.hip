@@ -88,7 +88,8 @@ int check_device(wd_handle* h) {
   return WD_OK;
 }
 
-int check_state(wd_handle* h, const wd_state* st, wd::DState* out) {
+// moist: whether the state must carry the six species (-1: the handle's namelist says)
+int check_state(wd_handle* h, const wd_state* st, wd::DState* out, int moist_flag = -1) {
   if (!st) return fail(h, WD_EINVAL, "state is NULL");
   double* ptrs[] = {st->u, st->v, st->w, st->theta, st->p_total, st->p_pert, st->ph_total,
                     st->ph_pert, st->mu_total, st->mu_pert, st->mudf};
@@ -96,6 +97,17 @@ int check_state(wd_handle* h, const wd_state* st, wd::DState* out) {
     if (!p) return fail(h, WD_EINVAL, "a state pointer is NULL");
   *out = wd::DState{st->u, st->v, st->w, st->theta, st->p_total, st->p_pert,
                     st->ph_total, st->ph_pert, st->mu_total, st->mu_pert, st->mudf};
+  // the moisture species: all six for a moist handle, none for a dry one
+  double* q[wd::kNumMoist] = {st->qv, st->qc, st->qr, st->qi, st->qs, st->qg};
+  int nq = 0;
+  for (double* p : q) nq += p != nullptr;
+  const bool moist = moist_flag >= 0 ? moist_flag != 0 : (h && h->nl.moist);
+  if (moist && nq != wd::kNumMoist)
+    return fail(h, WD_EINVAL, "a moist handle needs all six moisture species (qv, qc, qr, qi, qs, qg)");
+  if (!moist && nq != 0)
+    return fail(h, WD_EINVAL, "the state carries moisture species but the handle was created with moist = 0");
+  if (moist)
+    for (int sp = 0; sp < wd::kNumMoist; ++sp) out->q[sp] = q[sp];
   return WD_OK;
 }
 
@@ -136,6 +148,11 @@ int enqueue_steps(wd_handle* h, const wd::DState& ds, int n_large_steps, hipStre
       e = hipMemcpyAsync(w.mut0, ds.mu_total, C * sizeof(double), hipMemcpyDeviceToDevice, s);
       if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync(origin)");
     }
+    if (w.q0[0])  // the species at the step origin (transport on)
+      for (int sp = 0; sp < wd::kNumMoist; ++sp) {
+        e = hipMemcpyAsync(w.q0[sp], ds.q[sp], M * sizeof(double), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync(origin)");
+      }
     if (prof) prof->end(tok, s);
     for (const StageDesc& sd : stages) {
       e = wd::launch_stage_setup(g, h->nl, ds, w, sd.rk_step, sd.dts, s, prof);
@@ -146,6 +163,13 @@ int enqueue_steps(wd_handle* h, const wd::DState& ds, int n_large_steps, hipStre
       }
       e = wd::launch_stage_finish(g, ds, w, s, prof);
       if (e != hipSuccess) return hip_fail(h, e, "stage finish launch");
+      // the species' update with the stage's dt_rk: dt/3, dt/2, dt
+      // (operational_mode.py:3527-3529)
+      if (w.q0[0]) {
+        const double dt_rk = sd.rk_step == 1 ? dt / 3.0 : (sd.rk_step == 2 ? 0.5 * dt : dt);
+        e = wd::launch_moist_update(g, ds, w, dt_rk, s, prof);
+        if (e != hipSuccess) return hip_fail(h, e, "moisture update launch");
+      }
     }
   }
   return WD_OK;
@@ -164,16 +188,20 @@ size_t state_doubles(const wd::DGrid& g, int field) {
     case 1: return nz * (ny + 1) * nx;                  // v
     case 2: case 6: case 7: return (nz + 1) * ny * nx;  // w, ph_total, ph_pert
     case 3: case 4: case 5: return nz * ny * nx;        // theta, p_total, p_pert
-    default: return ny * nx;                            // mu_total, mu_pert, mudf
+    case 8: case 9: case 10: return ny * nx;            // mu_total, mu_pert, mudf
+    default: return nz * ny * nx;                       // the moisture species
   }
 }
 
+// fields 0..10 of the dry state, then the six moisture species
 double** state_field(wd::DState& s, int field) {
   double** f[] = {&s.u, &s.v, &s.w, &s.theta, &s.p_total, &s.p_pert,
-                  &s.ph_total, &s.ph_pert, &s.mu_total, &s.mu_pert, &s.mudf};
+                  &s.ph_total, &s.ph_pert, &s.mu_total, &s.mu_pert, &s.mudf,
+                  &s.q[0], &s.q[1], &s.q[2], &s.q[3], &s.q[4], &s.q[5]};
   return f[field];
 }
 constexpr int kStateFields = 11;
+int state_fields(const wd::DNamelist& nl) { return nl.moist ? kStateFields + wd::kNumMoist : kStateFields; }
 
 }  // namespace
 
@@ -283,6 +311,11 @@ int validate_create(const wd_grid* grid, const wd_namelist* nl, bool* rot_out) {
     return fail(nullptr, WD_EINVAL, "hypsometric_opt=2 needs the c3h, c4h, c3f and c4f metrics");
   if (nl->scalar_adv_opt < 0 || nl->scalar_adv_opt > 2)
     return fail(nullptr, WD_EINVAL, "scalar_adv_opt must be 0, 1 (positive-definite) or 2 (monotonic)");
+  if (nl->moist != 0 && nl->moist != 1) return fail(nullptr, WD_EINVAL, "moist must be 0 (dry) or 1");
+  if (nl->moist_adv_opt < 0 || nl->moist_adv_opt > 2)
+    return fail(nullptr, WD_EINVAL, "moist_adv_opt must be 0 (off), 1 (positive-definite) or 2 (monotonic)");
+  if (nl->moist_adv_opt != 0 && !nl->moist)
+    return fail(nullptr, WD_EINVAL, "moist_adv_opt transports the moisture: it needs moist = 1");
   // rotation: all four Coriolis arrays or none; it runs when f or e is nonzero
   bool& rot = *rot_out;
   rot = false;
@@ -336,6 +369,8 @@ void init_params(const wd_grid* grid, const wd_namelist* nl, bool rot, wd::DGrid
   n.c_s = nl->c_s;
   n.scalar_adv_opt = nl->scalar_adv_opt;
   n.const_nu = nl->const_nu_m2_s;
+  n.moist = nl->moist;
+  n.moist_adv_opt = nl->moist ? nl->moist_adv_opt : 0;
   n.mixed = 0;
 }
 
@@ -394,7 +429,18 @@ int workspace_slots(const wd::DGrid& g, const wd::DNamelist& n, bool optional, w
     add(&w.ck_col, c);
     add(&w.ck_dz, 1);
   }
-  if (n.scalar_adv_opt != 0) {
+  if (n.moist) {
+    add(&w.cqu, U);
+    add(&w.cqv, V);
+    add(&w.cqwr, Wn);
+    if (n.moist_adv_opt != 0)
+      for (int sp = 0; sp < wd::kNumMoist; ++sp) {
+        add(&w.q0[sp], M);
+        add(&w.qt[sp], M);
+      }
+  }
+  // the limiter's arrays serve theta's and the species' limited advection
+  if (n.scalar_adv_opt != 0 || n.moist_adv_opt != 0) {
     add(&w.mut0, c);
     add(&w.lim_sa, M);
     add(&w.lim_sb, M);
@@ -595,6 +641,8 @@ int wd_create(const wd_grid* grid, const wd_namelist* nl, wd_handle** out) {
           h->column_tile = t;
     }
   }
+  // the LDS-tiled kernel has no moisture factor
+  if (h->nl.moist) h->column_tile = wd::kColumnMarch;
   *out = h;
   return WD_OK;
 }
@@ -663,14 +711,15 @@ int wd_autotune(wd_handle* h, const wd_state* st, int trial_steps, void* stream)
   hipStream_t s = static_cast<hipStream_t>(stream);
   // scratch copy of the state: trials advance it, the caller's state is untouched
   size_t total = 0;
-  for (int f = 0; f < kStateFields; ++f) total += (state_doubles(h->g, f) * sizeof(double) + 255) / 256 * 256;
+  const int nfields = state_fields(h->nl);
+  for (int f = 0; f < nfields; ++f) total += (state_doubles(h->g, f) * sizeof(double) + 255) / 256 * 256;
   void* buf = nullptr;
   hipError_t e = hipMalloc(&buf, total);
   if (e != hipSuccess) return fail(h, WD_ENOMEM, std::string("hipMalloc(autotune state): ") + hipGetErrorString(e));
   wd::DState scratch = src;
   {
     size_t off = 0;
-    for (int f = 0; f < kStateFields; ++f) {
+    for (int f = 0; f < nfields; ++f) {
       *state_field(scratch, f) = reinterpret_cast<double*>(static_cast<char*>(buf) + off);
       off += (state_doubles(h->g, f) * sizeof(double) + 255) / 256 * 256;
     }
@@ -682,13 +731,13 @@ int wd_autotune(wd_handle* h, const wd_state* st, int trial_steps, void* stream)
   std::vector<std::pair<int, int>> cand;
   for (int remap = 1; remap >= 0; --remap) {
     cand.emplace_back(wd::kColumnMarch, remap);
-    for (int t = 0; t < wd::kNumColumnTiles && !h->nl.mixed; ++t)
+    for (int t = 0; t < wd::kNumColumnTiles && !h->nl.mixed && !h->nl.moist; ++t)
       if (h->g.nz + 1 <= wd::kColumnTiles[t].kt * wd::kMaxLevelsPerThread) cand.emplace_back(t, remap);
   }
   int best = -1;
   float best_ms = 0.0f;
   auto reset = [&]() -> hipError_t {
-    for (int f = 0; f < kStateFields; ++f) {
+    for (int f = 0; f < nfields; ++f) {
       hipError_t r = hipMemcpyAsync(*state_field(scratch, f), *state_field(src, f),
                                     state_doubles(h->g, f) * sizeof(double), hipMemcpyDeviceToDevice, s);
       if (r != hipSuccess) return r;
@@ -740,6 +789,7 @@ int wd_autotune(wd_handle* h, const wd_state* st, int trial_steps, void* stream)
 int wd_set_precision(wd_handle* h, int mode) {
   if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
   if (mode != WD_PREC_FP64 && mode != WD_PREC_MIXED) return fail(h, WD_EINVAL, "precision must be WD_PREC_FP64 or WD_PREC_MIXED");
+  if (mode == WD_PREC_MIXED && h->nl.moist) return fail(h, WD_EINVAL, "the mixed form runs dry states only");
   h->nl.mixed = mode == WD_PREC_MIXED;
   // the mixed form runs the column-march kernels (the LDS-tiled kernel reads FP64 tendencies)
   if (h->nl.mixed) h->column_tile = wd::kColumnMarch;
@@ -783,7 +833,7 @@ int wd_batch_create(wd_handle* const* members, const wd_state* states, int n_mem
   b->members.assign(members, members + n_members);
   b->states.resize(n_members);
   for (int m = 0; m < n_members; ++m) {
-    int rc = check_state(nullptr, &states[m], &b->states[m]);
+    int rc = check_state(nullptr, &states[m], &b->states[m], members[m]->nl.moist);
     if (rc != WD_OK) {
       delete b;
       return rc;
@@ -920,6 +970,7 @@ void wd_destroy(wd_handle* h) {
 
 int wd_step_adjoint(wd_handle* h, const wd_state* start, wd_state* bar, void* stream) {
   if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  if (h->nl.moist) return fail(h, WD_EINVAL, "the adjoint does not cover the moisture (moist = 1)");
   wd::DState ds, db;
   int rc = check_state(h, start, &ds);
   if (rc == WD_OK) rc = check_state(h, bar, &db);
@@ -961,7 +1012,7 @@ int wd_step_noalloc(const wd_grid* grid, const wd_namelist* nl, const wd_state* 
   if (rc != WD_OK) return rc;
   if (n_large_steps < 0) return fail(nullptr, WD_EINVAL, "n_large_steps must be >= 0");
   wd::DState ds;
-  rc = check_state(nullptr, st, &ds);
+  rc = check_state(nullptr, st, &ds, nl->moist);
   if (rc != WD_OK) return rc;
   wd_handle h;  // on the stack: its device buffers all come from `scratch`
   init_params(grid, nl, rot, h.g, h.nl);

@@ -59,12 +59,16 @@ struct DNamelist {
   double c_s;
   int scalar_adv_opt;      // theta on the final RK stage: 0 plain, 1 positive-definite, 2 monotonic
   double const_nu;         // > 0: constant-K diffusion with this K (m^2/s)
+  int moist;               // 1: the state carries the six moisture species
+  int moist_adv_opt;       // with moist: 0 no transport, 1/2 transport, limited on the final stage
   int mixed;  // 1: mixed-precision form (FP32 tendencies and flux interpolants), 0: plain FP64
 };
 
 // Caller state (device pointers).
+constexpr int kNumMoist = 6;  // qv, qc, qr, qi, qs, qg
 struct DState {
   double *u, *v, *w, *theta, *p_total, *p_pert, *ph_total, *ph_pert, *mu_total, *mu_pert, *mudf;
+  double* q[kNumMoist];  // the moisture species (all null for a dry state)
 };
 
 // Per-handle device workspace.
@@ -108,6 +112,13 @@ struct DWork {
   // the constant-K diffusion (null unless const_nu > 0; its tendencies go to
   // smu..smt): each column's summed layer thickness and the grid mean dz
   double *ck_col, *ck_dz;
+  // the moisture (null for a dry state): the coupling factors of the stage
+  // species on u faces, v faces and w faces (the raw calc_cq field, 0 at the
+  // surface and the lid), and with transport the step-origin species and the
+  // stage's coupled tendencies d(mu q)/dt
+  double *cqu, *cqv, *cqwr;
+  double* q0[kNumMoist];
+  double* qt[kNumMoist];
 };
 
 __host__ __device__ inline size_t im(const DGrid& g, int k, int j, int i) {

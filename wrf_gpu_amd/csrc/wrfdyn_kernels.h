@@ -35,6 +35,8 @@ enum KernelId {
   KID_LIMITER,
   KID_CONSTK,
   KID_CORIOLIS,
+  KID_MOIST,         // the moisture factors and the species' tendencies
+  KID_MOIST_UPDATE,  // the species' large-step update after a stage
   KID_COUNT
 };
 
@@ -112,6 +114,11 @@ extern const int kNumColumnTiles;
 // which starts from t_2ave = 0 and pm1 = the step-0 pressure.
 hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                                    double dts, int first, int column_tile, hipStream_t st, Prof* prof);
+
+// The moisture's large-step update after the acoustic loop of a stage with
+// transport on (moist_adv_opt != 0): q = (m_origin q0 + dt_rk qt) / m_new.
+hipError_t launch_moist_update(const DGrid& g, const DState& s, const DWork& w, double dt_rk, hipStream_t st,
+                               Prof* prof);
 
 // small_step_finish + grid%p refresh.
 hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, hipStream_t st, Prof* prof);

@@ -127,7 +127,9 @@ __device__ inline double dpn_ls_bot(const DGrid& g, double ps0, double ps1, doub
 
 constexpr int kTendURows = 4;
 
-// advect_scalar_flux of theta-300 (flux_advection.py:482-568) on mass levels.
+// advect_scalar_flux (flux_advection.py:482-568) on mass levels of q - qoff:
+// theta - 300 into tht (plus the explicit-diffusion tendencies d6 and sm), or a
+// moisture species (qoff 0, no diffusion) into its qt.
 // TT: the tendency storage type; float selects the mixed-precision form.
 // Blocks of kTendURows rows of one 64-lane wave each, waves overlapping by one
 // lane either side (lanes 1..62 own columns): the +-1 columns and ru at i+1
@@ -135,7 +137,9 @@ constexpr int kTendURows = 4;
 // rv at j+1 from the adjacent waves (LDS); the +-2, +-3 columns and rows
 // outside the block (or wrapping periodically) are loaded.
 template <class TT>
-__global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DState s, DWork w) {
+__global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DWork w, const double* __restrict__ th,
+                                                             double qoff, double* dst, const double* d6,
+                                                             const double* sm) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const unsigned nbx = (unsigned)(nx + 61) / 62;
@@ -157,8 +161,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DState s, D
     lds_y[d] = r >= 0 && r < kTendURows && jj >= 0 && jj < ny;
   }
   const double msftx = g.msftx[c8 / 8u];
-  const double* th = s.theta;
-  __shared__ double sh[2][kTendURows][2][64];  // theta and rv at level k
+  __shared__ double sh[2][kTendURows][2][64];  // the scalar and rv at level k
   struct Lv {
     double x[7], y[7];  // theta at columns i-3..i+3 and rows j-3..j+3 (x[3], y[3]: centre)
     double ru0, ru1, rv0, rv1;
@@ -204,9 +207,9 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DState s, D
         if (d != 3 && lds_y[d]) cur.y[d] = sh[k & 1][ty + d - 3][0][tx];
       if (lds_y[4]) cur.rv1 = sh[k & 1][ty + 1][1][tx];
     }
-    const double qm1 = tm1 - kT0, q0 = t0 - kT0, qp1 = tp1 - kT0, qp2 = tp2 - kT0;
-    auto QX = [&](int sft) { return cur.x[sft + 3] - kT0; };
-    auto QY = [&](int d) { return cur.y[d + 3] - kT0; };
+    const double qm1 = tm1 - qoff, q0 = t0 - qoff, qp1 = tp1 - qoff, qp2 = tp2 - qoff;
+    auto QX = [&](int sft) { return cur.x[sft + 3] - qoff; };
+    auto QY = [&](int d) { return cur.y[d + 3] - qoff; };
     double t;
     if constexpr (MX) {
       t = ((-msftx) * g.rdx) * hdiv5_mixed(cur.x, QX(0), cur.ru0, cur.ru1);
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DState s, D
       t = t - g.rdnw[k] * (vf_hi - vf_lo);
       vf_lo = vf_hi;
     }
-    if (owned) stt<TT>(w.tht, o + c8, add_diff(t, w.d6t, w.smt, o + c8));
+    if (owned) stt<TT>(dst, o + c8, add_diff(t, d6, sm, o + c8));
     tm1 = t0;
     t0 = tp1;
     tp1 = tp2;
@@ -552,7 +555,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
     dpx = dpx + ((msf_u * g.rdx) * (cur.phpr - cur.phpl)) * bracket;
     if (owned) {
       const unsigned off = (unsigned)k * upl8 + fo8;
-      stt<TT>(w.ut, off, add_diff(t, w.d6u, w.smu, off) + (-1.0 * dpx));
+      stt<TT>(w.ut, off, add_diff(t, w.d6u, w.smu, off) + (-(w.cqu ? ldo(w.cqu, off) : 1.0)) * dpx);
     }
     qm1 = q0;
     q0 = qp1;
@@ -781,7 +784,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, D
     dpy = dpy + ((msf_v * g.rdy) * (cur.phpr - cur.phpl)) * bracket;
     if (owned) {
       const unsigned off = (unsigned)k * upl8 + fo8;
-      stt<TT>(w.vt, off, add_diff(t, w.d6v, w.smv, off) + (-1.0 * dpy));
+      stt<TT>(w.vt, off, add_diff(t, w.d6v, w.smv, off) + (-(w.cqv ? ldo(w.cqv, off) : 1.0)) * dpy);
     }
     qm1 = q0;
     q0 = qp1;
@@ -1062,7 +1065,9 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
   const double msfuy_w = g.msfuy[fuw], msfuy_e = g.msfuy[fue];
   const double muv_s = w.muv[fvs], muv_n = w.muv[fvn];
   const double msfvx_s = g.msfvx[fvs], msfvx_n = g.msfvx[fvn];
-  const double cq1 = 1.0 / (1.0 + 0.0), cq2 = 0.0 * cq1;
+  // moisture: the raw calc_cq cqw of face f (0 for a dry state) gives
+  // cq1 = 1/(1 + cqw) and cq2 = cqw cq1 in rw_tend (pg_buoy_w,
+  // core/advance_w.py:266-290) and cq1 in the implicit-w rows (calc_coef_w)
   double cof = (0.5 * dts * kG * (1.0 + nl.epssm));
   cof = cof * cof;
   const double lid_flag = nl.top_lid ? 0.0 : 1.0;
@@ -1092,6 +1097,7 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
   struct Lv {
     double pp, wt, uw, ue, vs, vn, romp, phpp, phtp, ww, c2a;
     double pnp, pnt, psp, pst;  // ph' and ph total at f on rows j+1, j-1 (edge waves only)
+    double cqr;                 // raw moisture cqw at face f
   };
   auto load = [&](int f, Lv& L) {
     const unsigned o = (unsigned)f * pl8, ou = (unsigned)f * upl8, ov = (unsigned)f * vpl8;
@@ -1105,6 +1111,7 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
     L.phtp = ldo(s.ph_total, o + pl8 + c8);
     L.ww = ldo(s.w, o + c8);
     L.c2a = ldo(w.c2a, o + c8);
+    L.cqr = w.cqwr ? ldo(w.cqwr, o + c8) : 0.0;
     if (load_n) {
       L.pnp = ldo(s.ph_pert, o + cjp8);
       L.pnt = ldo(s.ph_total, o + cjp8);
@@ -1131,6 +1138,7 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
     sto(w.cgamma, c8, 0.0);
   }
   double gamma_prev = 0.0;
+  double cqr_m = 0.0;  // raw moisture cqw of the previous face
   Lv cur{};
   load(1, cur);
   for (int f = 1; f < nz; ++f) {
@@ -1151,6 +1159,7 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
       pht_s = load_s ? PHT(cur.psp, cur.pst) : sh[f & 1][ty - 1][0][tx];
       if (!load_vn) cur.vn = sh[f & 1][ty + 1][1][tx];
     }
+    const double cq1 = 1.0 / (1.0 + cur.cqr), cq2 = cur.cqr * cq1;
     // ---- rw_tend ----
     {
       double rw = (msft_inv * kG) *
@@ -1203,13 +1212,13 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
       } else {
         int kb = k - 1;
         double denom = MH(kb) * MF(kb);
-        ak = (((((-cqw_at(k, nz)) * cof) * g.rdn[k]) * g.rdnw[k - 1]) * c2a_m) / denom;
+        ak = (((((-cq1) * cof) * g.rdn[k]) * g.rdnw[k - 1]) * c2a_m) / denom;
       }
       double du = MH(k) * MF(k);
       double dl = MH(k - 1) * MF(k);
       double dc = MH(k) * MF(k + 1);
-      double b = 1.0 + ((cqw_at(k, nz) * cof) * g.rdn[k]) * ((g.rdnw[k] * cur.c2a) / du + (g.rdnw[k - 1] * c2a_m) / dl);
-      double cc = (((((-cqw_at(k, nz)) * cof) * g.rdn[k]) * g.rdnw[k]) * cur.c2a) / dc;
+      double b = 1.0 + ((cq1 * cof) * g.rdn[k]) * ((g.rdnw[k] * cur.c2a) / du + (g.rdnw[k - 1] * c2a_m) / dl);
+      double cc = (((((-cq1) * cof) * g.rdn[k]) * g.rdnw[k]) * cur.c2a) / dc;
       double al = 1.0 / (b - ak * gamma_prev);
       double gm = cc * al;
       if (owned) {
@@ -1229,10 +1238,13 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
     pht_m = pht_0;
     pht_0 = pht_p;
     c2a_m = cur.c2a;
+    cqr_m = cur.cqr;
     cur = nxt;
   }
-  // top face nz (p_m, c2a_m now hold level nz-1)
+  // top face nz (p_m, c2a_m, cqr_m now hold level nz-1): the raw cqw of face
+  // nz - 1 (advance_w.py:266-290)
   if (!owned) return;  // no barrier follows
+  const double cq1 = 1.0 / (1.0 + cqr_m), cq2 = cqr_m * cq1;
   const unsigned qt = c8 + (unsigned)nz * pl8;
   double rwt = (msft_inv * kG) *
                (((((cq1 * 2.0) * g.rdnw[nz - 1]) * (-p_m)) - g.c1f[nz] * mu_p) - cq2 * (g.c1f[nz] * mub + g.c2f[nz]));
@@ -1601,16 +1613,27 @@ void launch_smag(const DGrid& g, const DNamelist& nl, const DState& s, const DWo
 }
 
 // ---------------------------------------------------------------------------
-// Limited theta advection on the final RK stage (scalar_adv_opt 1
-// positive-definite, 2 monotonic; advect_scalar_flux_limited,
-// dynamics/flux_advection.py:614-975, applied to theta - 300 at
-// runtime/operational_mode.py:2819-2866).  Donor-cell fluxes from the step
+// Limited scalar advection on the final RK stage (advect_scalar_flux_limited,
+// dynamics/flux_advection.py:614-975): theta - 300 with scalar_adv_opt 1
+// positive-definite or 2 monotonic (runtime/operational_mode.py:2819-2866),
+// and each moisture species with moist_adv_opt (:3162-3230).  Donor-cell fluxes from the step
 // origin's theta and mass, the antidiffusive remainder of the h5/v3 fluxes,
 // the face fluxes (k_lim_faces, each face once), the per-cell scales
 // (k_lim_scale), then the divergence of the limited fluxes (k_lim_tend),
 // which replaces k_tend_t on that stage.  One thread per cell.
 // ---------------------------------------------------------------------------
 constexpr double kLimEps = 1.0e-20;
+// The scalar a limiter launch advects: q - qoff with the step origin's q0,
+// its tendency into dst plus the explicit-diffusion tendencies d6 and sm
+// (theta: theta - 300 into tht; a moisture species: qoff 0, no diffusion).
+struct LimScalar {
+  const double* q;
+  const double* q0;
+  double qoff;
+  double* dst;
+  const double* d6;
+  const double* sm;
+};
 __device__ inline double lim_pos(double a) { return a < 0.0 ? 0.0 : a; }
 __device__ inline double lim_neg(double a) { return a > 0.0 ? 0.0 : a; }
 __device__ inline double lim_min(double a, double b) { return b < a ? b : a; }
@@ -1627,33 +1650,34 @@ __device__ inline double lim_scale(double flux, double room) {
 }
 // Antidiffusive (a) and donor-cell (lo) flux at x face ii of row (k, j), or
 // y face jj of column (k, i).
-__device__ inline void lim_xface(const DGrid& g, const double* th, const double* th0, const double* ru,
-                                 const double* mut, int k, int j, int ii, double dxl, double dt, double& a,
-                                 double& lo) {
+__device__ inline void lim_xface(const DGrid& g, const double* th, const double* th0, double qoff,
+                                 const double* ru, const double* mut, int k, int j, int ii, double dxl, double dt,
+                                 double& a, double& lo) {
   const int nx = g.nx;
   const int iw = ii == 0 ? nx - 1 : ii - 1;
-  auto Q = [&](int d) { return th[im(g, k, j, wrapi(ii + d, nx))] - kT0; };
+  auto Q = [&](int d) { return th[im(g, k, j, wrapi(ii + d, nx))] - qoff; };
   const double muu = 0.5 * (mut[i2(g, j, ii)] + mut[i2(g, j, iw)]);
   const double mu_u = g.c1h[k] * muu + g.c2h[k];
   const double vel = ru[im(g, k, j, ii)];
-  lo = lim_upwind(th0[im(g, k, j, iw)] - kT0, th0[im(g, k, j, ii)] - kT0, vel, mu_u, dxl, dt);
+  lo = lim_upwind(th0[im(g, k, j, iw)] - qoff, th0[im(g, k, j, ii)] - qoff, vel, mu_u, dxl, dt);
   a = vel * flux5(Q(-3), Q(-2), Q(-1), Q(0), Q(1), Q(2), vel) - lo;
 }
-__device__ inline void lim_yface(const DGrid& g, const double* th, const double* th0, const double* rv,
-                                 const double* mut, int k, int jj, int i, double dyl, double dt, double& a,
-                                 double& lo) {
+__device__ inline void lim_yface(const DGrid& g, const double* th, const double* th0, double qoff,
+                                 const double* rv, const double* mut, int k, int jj, int i, double dyl, double dt,
+                                 double& a, double& lo) {
   const int ny = g.ny;
   const int js = jj == 0 ? ny - 1 : jj - 1;
-  auto Q = [&](int d) { return th[im(g, k, wrapi(jj + d, ny), i)] - kT0; };
+  auto Q = [&](int d) { return th[im(g, k, wrapi(jj + d, ny), i)] - qoff; };
   const double muv = 0.5 * (mut[i2(g, jj, i)] + mut[i2(g, js, i)]);
   const double mu_v = g.c1h[k] * muv + g.c2h[k];
   const double vel = rv[im(g, k, jj, i)];
-  lo = lim_upwind(th0[im(g, k, js, i)] - kT0, th0[im(g, k, jj, i)] - kT0, vel, mu_v, dyl, dt);
+  lo = lim_upwind(th0[im(g, k, js, i)] - qoff, th0[im(g, k, jj, i)] - qoff, vel, mu_v, dyl, dt);
   a = vel * flux5(Q(-3), Q(-2), Q(-1), Q(0), Q(1), Q(2), vel) - lo;
 }
 // The same at z face f of column (j, i); faces 0 and nz carry no flux.
-__device__ inline void lim_zface(const DGrid& g, const double* th, const double* th0, const double* rom,
-                                 const double* mut, int f, int j, int i, double dt, double& a, double& lo) {
+__device__ inline void lim_zface(const DGrid& g, const double* th, const double* th0, double qoff,
+                                 const double* rom, const double* mut, int f, int j, int i, double dt, double& a,
+                                 double& lo) {
   const int nz = g.nz;
   if (f == 0 || f == nz) {
     a = 0.0;
@@ -1668,8 +1692,8 @@ __device__ inline void lim_zface(const DGrid& g, const double* th, const double*
   const double cr = r * dt / dz / mu_face;
   const double acr = fabs(cr);
   const double w_m = 0.5 * lim_min(1.0, cr + acr), w_p = 0.5 * lim_max(-1.0, cr - acr);
-  lo = mu_face * (dz / dt) * (w_m * (th0[im(g, f - 1, j, i)] - kT0) + w_p * (th0[im(g, f, j, i)] - kT0));
-  auto Q = [&](int kk) { return th[im(g, kk, j, i)] - kT0; };
+  lo = mu_face * (dz / dt) * (w_m * (th0[im(g, f - 1, j, i)] - qoff) + w_p * (th0[im(g, f, j, i)] - qoff));
+  auto Q = [&](int kk) { return th[im(g, kk, j, i)] - qoff; };
   double hi;
   if (f == 1)
     hi = r * (g.fnm[1] * Q(1) + g.fnp[1] * Q(0));
@@ -1684,17 +1708,17 @@ struct LimFaces {
 };
 // Each cell's minus faces (x face i, y face j, z face k; and face nz above
 // the top cell), computed once and stored.
-__global__ void __launch_bounds__(256) k_lim_faces(DGrid g, DState s, DWork w, double dt) {
+__global__ void __launch_bounds__(256) k_lim_faces(DGrid g, DState s, DWork w, LimScalar a, double dt) {
   const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
   if (i >= g.nx || j >= g.ny) return;
   const double dxl = 1.0 / (1.0 / g.dx), dyl = 1.0 / (1.0 / g.dy);
   const size_t c = im(g, k, j, i);
-  lim_xface(g, s.theta, w.th0, w.ru, s.mu_total, k, j, i, dxl, dt, w.lim_ax[c], w.lim_lx[c]);
-  lim_yface(g, s.theta, w.th0, w.rv, s.mu_total, k, j, i, dyl, dt, w.lim_ay[c], w.lim_ly[c]);
-  lim_zface(g, s.theta, w.th0, w.rom, s.mu_total, k, j, i, dt, w.lim_az[c], w.lim_lz[c]);
+  lim_xface(g, a.q, a.q0, a.qoff, w.ru, s.mu_total, k, j, i, dxl, dt, w.lim_ax[c], w.lim_lx[c]);
+  lim_yface(g, a.q, a.q0, a.qoff, w.rv, s.mu_total, k, j, i, dyl, dt, w.lim_ay[c], w.lim_ly[c]);
+  lim_zface(g, a.q, a.q0, a.qoff, w.rom, s.mu_total, k, j, i, dt, w.lim_az[c], w.lim_lz[c]);
   if (k == g.nz - 1) {
     const size_t top = im(g, g.nz, j, i);
-    lim_zface(g, s.theta, w.th0, w.rom, s.mu_total, g.nz, j, i, dt, w.lim_az[top], w.lim_lz[top]);
+    lim_zface(g, a.q, a.q0, a.qoff, w.rom, s.mu_total, g.nz, j, i, dt, w.lim_az[top], w.lim_lz[top]);
   }
 }
 // The six faces of cell (k, j, i) from the stored face fluxes.
@@ -1717,7 +1741,7 @@ __device__ inline LimFaces lim_faces(const DGrid& g, const DWork& w, int k, int 
   return F;
 }
 
-__global__ void __launch_bounds__(256) k_lim_scale(DGrid g, DState s, DWork w, int opt, double dt) {
+__global__ void __launch_bounds__(256) k_lim_scale(DGrid g, DWork w, LimScalar a, int opt, double dt) {
   const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
   if (i >= g.nx || j >= g.ny) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -1726,7 +1750,9 @@ __global__ void __launch_bounds__(256) k_lim_scale(DGrid g, DState s, DWork w, i
   const size_t c = im(g, k, j, i), c2 = i2(g, j, i);
   const double msf = g.msftx[c2];
   const double mass_old = g.c1h[k] * w.mut0[c2] + g.c2h[k];
-  const double qo = w.th0[c] - kT0;
+  const double qoff = a.qoff;
+  const double* th0 = a.q0;
+  const double qo = th0[c] - qoff;
   const double div_low =
       msf * msf * (rdx * (F.lx1 - F.lx0) + rdy * (F.ly1 - F.ly0)) + msf * g.rdnw[k] * (F.lz1 - F.lz0);
   const double ph_low = mass_old * qo - dt * div_low;
@@ -1739,10 +1765,10 @@ __global__ void __launch_bounds__(256) k_lim_scale(DGrid g, DState s, DWork w, i
   }
   const int ip = i + 1 == nx ? 0 : i + 1, iw = i == 0 ? nx - 1 : i - 1;
   const int jp = j + 1 == ny ? 0 : j + 1, js = j == 0 ? ny - 1 : j - 1;
-  const double nb[6] = {w.th0[im(g, k, j, iw)] - kT0, w.th0[im(g, k, j, ip)] - kT0,
-                        w.th0[im(g, k, js, i)] - kT0, w.th0[im(g, k, jp, i)] - kT0,
-                        w.th0[im(g, k > 0 ? k - 1 : 0, j, i)] - kT0,
-                        w.th0[im(g, k < nz - 1 ? k + 1 : nz - 1, j, i)] - kT0};
+  const double nb[6] = {th0[im(g, k, j, iw)] - qoff, th0[im(g, k, j, ip)] - qoff,
+                        th0[im(g, k, js, i)] - qoff, th0[im(g, k, jp, i)] - qoff,
+                        th0[im(g, k > 0 ? k - 1 : 0, j, i)] - qoff,
+                        th0[im(g, k < nz - 1 ? k + 1 : nz - 1, j, i)] - qoff};
   double qmax = qo, qmin = qo;
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
@@ -1769,7 +1795,7 @@ __device__ inline double lim_vface(int opt, double a, const double* sa, const do
 }
 
 template <class TT>
-__global__ void __launch_bounds__(256) k_lim_tend(DGrid g, DState s, DWork w, int opt, double dt) {
+__global__ void __launch_bounds__(256) k_lim_tend(DGrid g, DWork w, LimScalar a, int opt) {
   const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
   if (i >= g.nx || j >= g.ny) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -1791,7 +1817,7 @@ __global__ void __launch_bounds__(256) k_lim_tend(DGrid g, DState s, DWork w, in
   t = t - msf * rdy * (fy1 - fy0);
   t = t - g.rdnw[k] * (fz1 - fz0);
   const unsigned off = (unsigned)c * 8u;
-  stt<TT>(w.tht, off, add_diff(t, w.d6t, w.smt, off));
+  stt<TT>(a.dst, off, add_diff(t, a.d6, a.sm, off));
 }
 
 // ---------------------------------------------------------------------------
@@ -1889,14 +1915,73 @@ void launch_coriolis(const DGrid& g, const DState& s, const DWork& w, hipStream_
   k_coriolis<TT><<<nbu + nbv, bs, 0, st>>>(g, s, w, nbu);
 }
 
+inline dim3 cell_grid(const DGrid& g) {
+  return dim3((unsigned)((g.nx + kFieldBx - 1) / kFieldBx), (unsigned)((g.ny + kFieldBy - 1) / kFieldBy),
+              (unsigned)g.nz);
+}
+
+// One limited advection (opt 1 or 2) of the scalar `a`; the face and scale
+// arrays are shared, so limiter launches on one stream run one after another.
 template <class TT>
-void launch_limiter(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, hipStream_t st) {
+void launch_limiter(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, const LimScalar& a,
+                    int opt, hipStream_t st) {
   const dim3 bs(kFieldBx, kFieldBy);
-  const dim3 grid((unsigned)((g.nx + kFieldBx - 1) / kFieldBx), (unsigned)((g.ny + kFieldBy - 1) / kFieldBy),
-                  (unsigned)g.nz);
-  k_lim_faces<<<grid, bs, 0, st>>>(g, s, w, nl.dt);
-  k_lim_scale<<<grid, bs, 0, st>>>(g, s, w, nl.scalar_adv_opt, nl.dt);
-  k_lim_tend<TT><<<grid, bs, 0, st>>>(g, s, w, nl.scalar_adv_opt, nl.dt);
+  const dim3 grid = cell_grid(g);
+  k_lim_faces<<<grid, bs, 0, st>>>(g, s, w, a, nl.dt);
+  k_lim_scale<<<grid, bs, 0, st>>>(g, w, a, opt, nl.dt);
+  k_lim_tend<TT><<<grid, bs, 0, st>>>(g, w, a, opt);
+}
+
+// ---------------------------------------------------------------------------
+// The moisture.  moisture_coupling_factors (dynamics/acoustic_wrf.py:303-319)
+// and moist_cqw_calc_face (core/advance_w.py:195-219) of the stage-entry
+// species, one thread per mass cell: qtot = qv + qc + qr + qi + qs + qg;
+// cqu = 1/(1 + (qtot_west + qtot_east)/2) on the cell's west u face (and the
+// last face of the row, the neighbour clamped at the edge), cqv likewise on
+// the south v face; the raw cqw = (qtot(k) + qtot(k-1))/2 on w face k, 0 at
+// the surface and the lid.  k_tend_u/v scale the large-step PGF by cqu/cqv,
+// k_uv the acoustic one, k_col_setup forms rw_tend and the implicit-w
+// coefficients with cq1 = 1/(1 + cqw) and cq2 = cqw cq1, and k_column_w the
+// explicit part of the w equation with cq1.
+// ---------------------------------------------------------------------------
+__device__ inline double qtot_at(const DState& s, size_t c) {
+  return ((((s.q[0][c] + s.q[1][c]) + s.q[2][c]) + s.q[3][c]) + s.q[4][c]) + s.q[5][c];
+}
+__global__ void __launch_bounds__(256) k_moist_factors(DGrid g, DState s, DWork w) {
+  const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const double qc = qtot_at(s, im(g, k, j, i));
+  const double qw = i > 0 ? qtot_at(s, im(g, k, j, i - 1)) : qc;
+  const double qs = j > 0 ? qtot_at(s, im(g, k, j - 1, i)) : qc;
+  w.cqu[iu(g, k, j, i)] = 1.0 / (1.0 + 0.5 * (qw + qc));
+  if (i == nx - 1) w.cqu[iu(g, k, j, nx)] = 1.0 / (1.0 + 0.5 * (qc + qc));
+  w.cqv[iv(g, k, j, i)] = 1.0 / (1.0 + 0.5 * (qs + qc));
+  if (j == ny - 1) w.cqv[iv(g, k, ny, i)] = 1.0 / (1.0 + 0.5 * (qc + qc));
+  w.cqwr[im(g, k, j, i)] = k > 0 ? 0.5 * (qc + qtot_at(s, im(g, k - 1, j, i))) : 0.0;
+  if (k == nz - 1) w.cqwr[im(g, nz, j, i)] = 0.0;
+}
+
+// _apply_moisture_large_step (runtime/operational_mode.py:3273-3316) after a
+// stage's acoustic loop: q = (m_origin q_origin + dt_rk qt) / m_new with the
+// column masses c1h mu + c2h of the step origin and of the finished stage,
+// every species of the cell in one thread.
+__global__ void __launch_bounds__(256) k_moist_update(DGrid g, DState s, DWork w, double dt_rk) {
+  const int i = blockIdx.x * kFieldBx + threadIdx.x, j = blockIdx.y * kFieldBy + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const size_t c2 = i2(g, j, i), c = im(g, k, j, i);
+  const double mass_old = g.c1h[k] * w.mut0[c2] + g.c2h[k];
+  const double mass_new = g.c1h[k] * s.mu_total[c2] + g.c2h[k];
+  const double inv_mass_new = 1.0 / mass_new;
+#pragma unroll
+  for (int sp = 0; sp < kNumMoist; ++sp) s.q[sp][c] = (mass_old * w.q0[sp][c] + dt_rk * w.qt[sp][c]) * inv_mass_new;
+}
+
+hipError_t launch_moist_update(const DGrid& g, const DState& s, const DWork& w, double dt_rk, hipStream_t st,
+                               Prof* prof) {
+  WD_TIMED(prof, KID_MOIST_UPDATE, st,
+           (k_moist_update<<<cell_grid(g), dim3(kFieldBx, kFieldBy), 0, st>>>(g, s, w, dt_rk)));
+  return hipGetLastError();
 }
 
 hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
@@ -1911,6 +1996,11 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   if (w.ck_dz) WD_TIMED(prof, KID_CONSTK, st, (launch_constk(g, nl, s, w, st)));
   // theta's limited advection replaces the plain one on the final stage
   const bool limit_theta = nl.scalar_adv_opt != 0 && rk_step == 3;
+  const LimScalar theta_lim{s.theta, w.th0, kT0, w.tht, w.d6t, w.smt};
+  // the moisture factors of the stage-entry species (k_tend_u/v read cqu/cqv)
+  if (nl.moist)
+    WD_TIMED(prof, KID_MOIST, st,
+             (k_moist_factors<<<cell_grid(g), dim3(kFieldBx, kFieldBy), 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_RUV, st,
            (k_ruv<<<dim3(blocks_for(C, bs), kchunks), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
   WD_TIMED(prof, KID_ROM, st, (k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom)));
@@ -1927,18 +2017,33 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
     if (limit_theta)
-      WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<float>(g, nl, s, w, st)));
+      WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<float>(g, nl, s, w, theta_lim, nl.scalar_adv_opt, st)));
     else
-      WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<float><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
+      WD_TIMED(prof, KID_TEND_T, st,
+               (k_tend_t<float><<<tt_tiles, tu_block, 0, st>>>(g, w, s.theta, kT0, w.tht, w.d6t, w.smt)));
   } else {
     WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
     if (limit_theta)
-      WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<double>(g, nl, s, w, st)));
+      WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<double>(g, nl, s, w, theta_lim, nl.scalar_adv_opt, st)));
     else
-      WD_TIMED(prof, KID_TEND_T, st, (k_tend_t<double><<<tt_tiles, tu_block, 0, st>>>(g, s, w)));
+      WD_TIMED(prof, KID_TEND_T, st,
+               (k_tend_t<double><<<tt_tiles, tu_block, 0, st>>>(g, w, s.theta, kT0, w.tht, w.d6t, w.smt)));
   }
+  // the species' coupled tendencies with the same transport velocities: the
+  // plain h5/v3 advection, the limiter on the final stage from the step
+  // origin (operational_mode.py:3162-3230); FP64 form only
+  if (nl.moist && nl.moist_adv_opt != 0)
+    for (int sp = 0; sp < kNumMoist; ++sp) {
+      if (rk_step == 3)
+        WD_TIMED(prof, KID_MOIST, st,
+                 (launch_limiter<double>(g, nl, s, w, LimScalar{s.q[sp], w.q0[sp], 0.0, w.qt[sp], nullptr, nullptr},
+                                         nl.moist_adv_opt, st)));
+      else
+        WD_TIMED(prof, KID_MOIST, st,
+                 (k_tend_t<double><<<tt_tiles, tu_block, 0, st>>>(g, w, s.q[sp], 0.0, w.qt[sp], nullptr, nullptr)));
+    }
   // the Coriolis force joins the coupled u/v tendencies after the PGF
   if (g.rot) {
     if (nl.mixed)

@@ -24,7 +24,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .grid import STATE_FIELDS, Metrics, Namelist
+from .grid import MOIST_FIELDS, STATE_FIELDS, Metrics, Namelist
 
 # Reference State field names -> C-ABI wd_state member.
 _ABI_FIELDS = (
@@ -63,8 +63,41 @@ def _abi_structs(metrics: Metrics, namelist: Namelist, keep: list):
                           float(nl.dampcoef), float(nl.zdamp), int(nl.diff_6th_opt),
                           float(nl.diff_6th_factor), int(nl.diff_opt), int(nl.km_opt),
                           float(nl.c_s), int(nl.scalar_adv_opt), int(nl.hypsometric_opt),
-                          float(nl.const_nu_m2_s))
+                          float(nl.const_nu_m2_s), int(bool(nl.moist)), int(nl.moist_adv_opt))
     return grid, nls
+
+
+def _state_fields(namelist: Namelist) -> tuple:
+    """The state dict keys a step reads: the dry fields, plus the six species when moist."""
+
+    return STATE_FIELDS + (MOIST_FIELDS if namelist.moist else ())
+
+
+def _check_tensors(state: dict, metrics: Metrics, namelist: Namelist) -> None:
+    import torch
+
+    shapes = metrics.shapes()
+    fields = _state_fields(namelist)
+    missing = [k for k in fields if k not in state]
+    if missing:
+        raise ValueError(f"state is missing fields {missing}")
+    dev = None
+    for name in fields:
+        t = state[name]
+        want = shapes.get(name, (metrics.nz, metrics.ny, metrics.nx))
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"state[{name!r}] must be a torch tensor")
+        if t.dtype != torch.float64:
+            raise ValueError(f"state[{name!r}] must be float64 (force_fp64 path), got {t.dtype}")
+        if tuple(t.shape) != want:
+            raise ValueError(f"state[{name!r}] has shape {tuple(t.shape)}, expected {want}")
+        if not t.is_cuda:
+            raise ValueError(f"state[{name!r}] must live on the GPU")
+        if not t.is_contiguous():
+            raise ValueError(f"state[{name!r}] must be contiguous")
+        dev = t.device if dev is None else dev
+        if t.device != dev:
+            raise ValueError("all state tensors must be on the same device")
 
 
 class Dycore:
@@ -103,28 +136,10 @@ class Dycore:
         return int(self._lib.wd_workspace_bytes(self._handle))
 
     def check_state(self, state: dict) -> None:
-        import torch
+        """The fields a step reads (the six species too when the namelist is moist) are
+        contiguous float64 device tensors of the grid's shapes, on one device."""
 
-        shapes = self.metrics.shapes()
-        missing = [k for k in STATE_FIELDS if k not in state]
-        if missing:
-            raise ValueError(f"state is missing fields {missing}")
-        dev = None
-        for name in STATE_FIELDS:
-            t = state[name]
-            if not isinstance(t, torch.Tensor):
-                raise ValueError(f"state[{name!r}] must be a torch tensor")
-            if t.dtype != torch.float64:
-                raise ValueError(f"state[{name!r}] must be float64 (force_fp64 path), got {t.dtype}")
-            if tuple(t.shape) != shapes[name]:
-                raise ValueError(f"state[{name!r}] has shape {tuple(t.shape)}, expected {shapes[name]}")
-            if not t.is_cuda:
-                raise ValueError(f"state[{name!r}] must live on the GPU")
-            if not t.is_contiguous():
-                raise ValueError(f"state[{name!r}] must be contiguous")
-            dev = t.device if dev is None else dev
-            if t.device != dev:
-                raise ValueError("all state tensors must be on the same device")
+        _check_tensors(state, self.metrics, self.namelist)
 
     def step(self, state: dict, n_steps: int = 1, stream=None) -> dict:
         """Advance ``n_steps`` large steps in place on ``stream`` (default: torch's current)."""
@@ -133,7 +148,7 @@ class Dycore:
         if n_steps < 0:
             raise ValueError("n_steps must be >= 0")
         stream = _stream_of(state, stream)
-        ptrs = _state_ptrs(state)
+        ptrs = _state_ptrs(state, self.namelist.moist)
         code = self._lib.wd_step(self._handle, ctypes.byref(ptrs), int(n_steps),
                                  ctypes.c_void_p(stream.cuda_stream))
         if code != _lib.WD_OK:
@@ -149,7 +164,7 @@ class Dycore:
         self.check_state(start)
         self.check_state(bar)
         stream = _stream_of(start, stream)
-        a, b = _state_ptrs(start), _state_ptrs(bar)
+        a, b = _state_ptrs(start, self.namelist.moist), _state_ptrs(bar, self.namelist.moist)
         code = self._lib.wd_step_adjoint(self._handle, ctypes.byref(a), ctypes.byref(b),
                                          ctypes.c_void_p(stream.cuda_stream))
         if code != _lib.WD_OK:
@@ -176,7 +191,7 @@ class Dycore:
         """
 
         self.check_state(state)
-        ptrs = _state_ptrs(state)
+        ptrs = _state_ptrs(state, self.namelist.moist)
         _lib.check(self._lib.wd_capture_graph(self._handle, ctypes.byref(ptrs)), self._handle)
         self._graph_state = state
 
@@ -201,7 +216,7 @@ class Dycore:
 
         self.check_state(state)
         stream = _stream_of(state, stream)
-        ptrs = _state_ptrs(state)
+        ptrs = _state_ptrs(state, self.namelist.moist)
         _lib.check(self._lib.wd_autotune(self._handle, ctypes.byref(ptrs), int(trial_steps),
                                          ctypes.c_void_p(stream.cuda_stream)), self._handle)
         self._graph_state = None
@@ -269,7 +284,7 @@ class Ensemble:
                 raise ValueError("all members must live on the same device")
         n = len(self.members)
         handles = (ctypes.c_void_p * n)(*[d._handle for d, _ in self.members])
-        states = (_lib.WdState * n)(*[_state_ptrs(s) for _, s in self.members])
+        states = (_lib.WdState * n)(*[_state_ptrs(s, d.namelist.moist) for d, s in self.members])
         out = ctypes.c_void_p()
         code = lib.wd_batch_create(handles, states, n, int(branches), ctypes.byref(out))
         if code != _lib.WD_OK:
@@ -317,8 +332,13 @@ class Ensemble:
             pass
 
 
-def _state_ptrs(state: dict):
-    return _lib.WdState(*[ctypes.c_void_p(state[k].data_ptr()) for k, _ in _ABI_FIELDS])
+def _state_ptrs(state: dict, moist: bool = False):
+    """The wd_state of a state dict: the species' pointers only for a moist handle (NULL otherwise)."""
+
+    ptrs = [ctypes.c_void_p(state[k].data_ptr()) for k, _ in _ABI_FIELDS]
+    if moist:
+        ptrs += [ctypes.c_void_p(state[k].data_ptr()) for k in MOIST_FIELDS]
+    return _lib.WdState(*ptrs)
 
 
 def _stream_of(state: dict, stream):
@@ -362,20 +382,12 @@ def step_noalloc(state: dict, namelist: Namelist, metrics: Metrics, n_steps: int
     library's fixed device scratch holds the metrics and the workspace.  Same
     results as :meth:`Dycore.step`; raises for a grid too large for the scratch."""
 
-    import torch
-
     namelist.validate()
-    shapes = metrics.shapes()
-    for name in STATE_FIELDS:
-        t = state.get(name)
-        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or tuple(t.shape) != shapes[name]:
-            raise ValueError(f"state[{name!r}] must be a float64 tensor of shape {shapes[name]}")
-        if not t.is_contiguous() or t.device != state["u"].device:
-            raise ValueError(f"state[{name!r}] must be contiguous and on the same device as state['u']")
+    _check_tensors(state, metrics, namelist)
     lib = _lib.load()
     keep: list = []
     grid, nls = _abi_structs(metrics, namelist, keep)
-    s = _state_ptrs(state)
+    s = _state_ptrs(state, namelist.moist)
     stream = _stream_of(state, stream)
     code = lib.wd_step_noalloc(ctypes.byref(grid), ctypes.byref(nls), ctypes.byref(s), int(n_steps),
                                ctypes.c_void_p(stream.cuda_stream))
