@@ -375,7 +375,8 @@ def small_workload_leg(steps: int, dev) -> dict:
 def ensemble_leg(steps: int, dev) -> dict:
     """32 members (8 each of 129^2, 201^2, 257^2, 321^2 columns, 50 levels):
     members stepped one after another with the plain step, then the batched
-    graph (4 branches) in FP64 and in the mixed form."""
+    graph (4 branches) in FP64 and in the mixed form, and in FP64 with each
+    member's autotuned column kernel and workgroup order."""
 
     import torch
 
@@ -411,9 +412,26 @@ def ensemble_leg(steps: int, dev) -> dict:
         out[f"batched_graph_{prec}"] = {"ms_per_step": ms, "value": cells * ns / (ms * 1e-3),
                                         "branches": ens.branches}
         ens.close()
+    # the batched FP64 graph with every member's autotuned column kernel and workgroup order
+    for d, _ in members:
+        d.set_precision("fp64")
+    t1 = time.time()
+    ens = Ensemble(members, branches=4, autotune=True, trial_steps=1)
+    tune_s = time.time() - t1
+    ens.step(1)
+    ms = timed_steps(ens.step, steps, stream) / steps
+    picks = {}
+    for cfg in ens.tuned:
+        key = f"column_tile={cfg['column_tile']},xcd_remap={cfg['xcd_remap']}"
+        picks[key] = picks.get(key, 0) + 1
+    out["batched_graph_fp64_autotuned"] = {"ms_per_step": ms, "value": cells * ns / (ms * 1e-3),
+                                           "branches": ens.branches, "member_picks": picks,
+                                           "autotune_s": tune_s}
+    ens.close()
     base = out["sequential_fp64"]["ms_per_step"]
     out["speedup_vs_sequential_fp64"] = {k: base / out[k]["ms_per_step"]
-                                         for k in ("batched_graph_fp64", "batched_graph_mixed")}
+                                         for k in ("batched_graph_fp64", "batched_graph_mixed",
+                                                   "batched_graph_fp64_autotuned")}
     for d, _ in members:
         d.close()
     return out

@@ -108,6 +108,41 @@ def test_batched_members_match_individual_steps(oracle_lib, branches):
         d.close()
 
 
+def test_autotuned_ensemble_matches_members_alone(oracle_lib):
+    """Ensemble(autotune=True) tunes every member alone, then captures each with its pick
+    (the LDS-tiled column kernel or the column march, either workgroup order); the result is
+    still bitwise each member stepped alone, a moist member included."""
+
+    import torch
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device, state_to_host
+
+    cases = [
+        ic.bubble_3d(nx=24, ny=20, nz=10, dx=1000.0, u0=10.0),
+        ic.stratified_terrain_3d(nx=20, ny=18, nz=12, dx=3000.0, u_sigma=1.0, sigma=0.5),
+        ic.with_moisture(ic.stratified_terrain_3d(nx=16, ny=12, nz=8, dx=2000.0, u_sigma=0.5, sigma=0.2),
+                         moist_adv_opt=1),
+    ]
+    print(f"[batch] autotuned ensemble of {len(cases)} members", flush=True)
+    alone = [(Dycore(c.metrics, c.namelist), state_to_device(c.state)) for c in cases]
+    batched = [(Dycore(c.metrics, c.namelist), state_to_device(c.state)) for c in cases]
+    ens = Ensemble(batched, branches=2, autotune=True, trial_steps=1)
+    assert len(ens.tuned) == len(cases) and ens.tuned[2]["column_tile"] == -1  # moist: the column march
+    ens.step(2)
+    for d, s in alone:
+        d.step(s, 2)
+    torch.cuda.synchronize()
+    for m, ((_, sa), (_, sb)) in enumerate(zip(alone, batched)):
+        _assert_bitwise(state_to_host(sb), state_to_host(sa), f"member {m}")
+    ref = oracle_lib.run(cases[2].metrics, cases[2].namelist, cases[2].state, 2)
+    got = state_to_host(batched[2][1])
+    for k in ("qv", "qc", "qr", "qi", "qs", "qg"):
+        assert np.array_equal(got[k], ref[k]), k
+    ens.close()
+    for d, _ in alone + batched:
+        d.close()
+
+
 def test_batch_rejects_duplicate_member():
     from wrf_gpu_amd import _lib, ic
     from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device

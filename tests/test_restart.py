@@ -11,7 +11,7 @@ import pytest
 
 from oracle import oracle
 from wrf_gpu_amd import ic, restart
-from wrf_gpu_amd.grid import STATE_FIELDS
+from wrf_gpu_amd.grid import MOIST_FIELDS, STATE_FIELDS
 
 
 @pytest.fixture
@@ -106,3 +106,34 @@ def test_write_refuses_a_wrong_state(case, tmp_path):
     del st["u"]
     with pytest.raises(ValueError):
         restart.write_restart(st, case.metrics, case.namelist, 0, tmp_path / "r.npz")
+
+
+def test_moist_restart_carries_the_species_and_resumes_bit_for_bit(case, tmp_path):
+    mcase = ic.with_moisture(case, moist_adv_opt=2)
+    straight = oracle.run(mcase.metrics, mcase.namelist, _state(mcase), 4)
+    half = oracle.run(mcase.metrics, mcase.namelist, _state(mcase), 2)
+    p = restart.write_restart(half, mcase.metrics, mcase.namelist, 2, tmp_path / "r.npz")
+    st, m, nl, step = restart.read_restart(p)
+    assert nl.moist and nl.moist_adv_opt == 2
+    resumed = oracle.run(m, nl, st, 4 - step)
+    for k in STATE_FIELDS + MOIST_FIELDS:
+        assert np.array_equal(resumed[k], straight[k]), k
+    # a moist namelist needs the species
+    dry = {k: v for k, v in half.items() if k not in MOIST_FIELDS}
+    with pytest.raises(ValueError):
+        restart.write_restart(dry, mcase.metrics, mcase.namelist, 2, tmp_path / "r2.npz")
+
+
+def test_a_version_1_restart_reads_as_a_dry_state(case, tmp_path):
+    p = restart.write_restart(_state(case), case.metrics, case.namelist, 1, tmp_path / "r.npz")
+
+    def as_v1(h, arrays):
+        h["format_version"] = 1
+        for k in ("moist", "moist_adv_opt"):
+            del h["namelist"][k]
+
+    old = tmp_path / "v1.npz"
+    _rewrite_header(p, old, as_v1)
+    st, _, nl, step = restart.read_restart(old)
+    assert step == 1 and nl == case.namelist and not nl.moist
+    assert set(st) == set(STATE_FIELDS)

@@ -135,3 +135,24 @@ def test_a_real_data_file_is_refused_unless_stepped_as_periodic(case, tmp_path):
         wrfinput.read_wrfinput(p)
     st, m = wrfinput.read_wrfinput(p, assume_periodic=True)
     assert st["u"].shape == case.state["u"].shape
+
+
+def test_moisture_species_are_read(case, tmp_path):
+    """QVAPOR, QCLOUD and QRAIN present (a warm-rain microphysics): the state carries all
+    six species, the absent ice species as zeros; without QVAPOR it carries none."""
+
+    from scipy.io import netcdf_file
+
+    mcase = ic.with_moisture(case)
+    p = _write(tmp_path / "wrfinput_d01", case)
+    with netcdf_file(str(p), "a") as f:
+        for name, key in (("QVAPOR", "qv"), ("QCLOUD", "qc"), ("QRAIN", "qr")):
+            v = f.createVariable(name, "f8", ("Time", "bottom_top", "south_north", "west_east"))
+            v[:] = mcase.state[key][None]
+    st, _ = wrfinput.read_wrfinput(p)
+    for k in ("qv", "qc", "qr"):
+        assert np.array_equal(st[k], mcase.state[k]), k
+    for k in ("qi", "qs", "qg"):
+        assert st[k].shape == mcase.state[k].shape and not np.any(st[k]), k
+    st, _ = wrfinput.read_wrfinput(_write(tmp_path / "wrfinput_d02", case))
+    assert "qv" not in st

@@ -71,3 +71,23 @@ def test_disagreeing_layers_and_schema_are_refused(case, tmp_path):
         f.GPUWRF_STATE_FIELD_ORDER = "u,v"
     with pytest.raises(ValueError, match="field order"):
         wrfrst.read_wrfrst(p2)
+
+
+def test_moist_restart_has_wrf_q_variables_and_resumes_bit_for_bit(case, tmp_path):
+    from scipy.io import netcdf_file
+
+    mcase = ic.with_moisture(case, moist_adv_opt=1)
+    straight = oracle.run(mcase.metrics, mcase.namelist, mcase.state, 3)
+    one = oracle.run(mcase.metrics, mcase.namelist, mcase.state, 1)
+    p = wrfrst.write_wrfrst(one, mcase.metrics, mcase.namelist, 1, tmp_path / "wrfrst_d01")
+    with netcdf_file(str(p), "r", mmap=False) as f:
+        for name, key in (("QVAPOR", "qv"), ("QCLOUD", "qc"), ("QRAIN", "qr"), ("QICE", "qi"), ("QSNOW", "qs"),
+                          ("QGRAUP", "qg")):
+            v = f.variables[name]
+            assert v.dimensions == ("Time", "bottom_top", "south_north", "west_east") and v.units == b"kg kg-1"
+            assert np.array_equal(v[0], one[key].astype(np.float32)), name
+    st, m, nl, step = wrfrst.read_wrfrst(p)
+    assert nl == mcase.namelist
+    resumed = oracle.run(m, nl, st, 3 - step)
+    for k in STATE_FIELDS + ("qv", "qc", "qr", "qi", "qs", "qg"):
+        assert np.array_equal(resumed[k], straight[k]), k

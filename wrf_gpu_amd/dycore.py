@@ -269,9 +269,16 @@ class Ensemble:
     once per large step.  ``members`` is a list of (Dycore, state) pairs; the
     handles and tensors must outlive the ensemble.  Each member's result is
     identical to ``Dycore.step`` on that member alone.
+
+    ``autotune=True`` first runs each member's autotuner (``Dycore.autotune``,
+    on a copy of its state, ``trial_steps`` timed steps per candidate) and
+    captures every member with the column kernel and workgroup order it
+    picked alone; ``tuned`` then lists those choices.  Tuned alone, a member
+    does not see the other branches running beside it, so its pick is not
+    necessarily the fastest inside the ensemble.
     """
 
-    def __init__(self, members, branches: int = 4):
+    def __init__(self, members, branches: int = 4, autotune: bool = False, trial_steps: int = 1):
         if not members:
             raise ValueError("an ensemble needs at least one member")
         self.members = list(members)
@@ -282,6 +289,7 @@ class Ensemble:
             dev = state["u"].device if dev is None else dev
             if state["u"].device != dev:
                 raise ValueError("all members must live on the same device")
+        self.tuned = [dyn.autotune(state, trial_steps) for dyn, state in self.members] if autotune else None
         n = len(self.members)
         handles = (ctypes.c_void_p * n)(*[d._handle for d, _ in self.members])
         states = (_lib.WdState * n)(*[_state_ptrs(s, d.namelist.moist) for d, s in self.members])

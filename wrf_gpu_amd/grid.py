@@ -140,8 +140,11 @@ class Metrics:
         )
 
     def shapes(self) -> dict[str, tuple[int, ...]]:
+        """Shapes of the state fields, the moisture species included."""
+
         nx, ny, nz = self.nx, self.ny, self.nz
         return {
+            **{q: (nz, ny, nx) for q in ("qv", "qc", "qr", "qi", "qs", "qg")},
             "u": (nz, ny, nx + 1),
             "v": (nz, ny + 1, nx),
             "w": (nz + 1, ny, nx),
@@ -166,6 +169,13 @@ STATE_FIELDS = (
 )
 # the moisture species a moist state adds, each (nz, ny, nx) (operational_mode.py:3100)
 MOIST_FIELDS = ("qv", "qc", "qr", "qi", "qs", "qg")
+
+
+def state_fields(namelist) -> tuple:
+    """The state dict keys a step reads and a restart carries: the dry fields, and the six
+    species when the namelist is moist."""
+
+    return STATE_FIELDS + (MOIST_FIELDS if getattr(namelist, "moist", False) else ())
 
 
 @dataclass(frozen=True)

@@ -27,11 +27,14 @@ from typing import Any
 
 import numpy as np
 
-from .grid import STATE_FIELDS, Metrics, Namelist
+from .grid import STATE_FIELDS, Metrics, Namelist, state_fields
 
 FORMAT = "wrf_gpu_amd.dycore_restart"
-FORMAT_VERSION = 1
-SUPPORTED_FORMAT_VERSIONS = (1,)
+# 2: a moist state carries its six species and the namelist its moisture
+# options; a version-1 file is a dry state whose namelist predates them
+FORMAT_VERSION = 2
+SUPPORTED_FORMAT_VERSIONS = (1, 2)
+_V1_NAMELIST_DEFAULTS = {"moist": False, "moist_adv_opt": 0}
 
 _METRIC_SCALARS = ("nx", "ny", "nz", "dx", "dy", "p_top", "cf1", "cf2", "cf3")
 _METRIC_ARRAYS = tuple(f.name for f in dataclasses.fields(Metrics) if f.name not in _METRIC_SCALARS)
@@ -55,17 +58,18 @@ def _host(a: Any) -> np.ndarray:
 
 def write_restart(state: dict, metrics: Metrics, namelist: Namelist, step_index: int, path: str | Path, *,
                   extra_metadata: dict | None = None) -> Path:
-    """Write the carry ``state`` (every field of ``STATE_FIELDS``, numpy arrays
-    or device tensors) after ``step_index`` large steps; returns the path.
-    The file is written beside ``path`` and renamed over it, so a reader never
-    sees a partial restart."""
+    """Write the carry ``state`` (every field of ``STATE_FIELDS``, and the six
+    species when the namelist is moist; numpy arrays or device tensors) after
+    ``step_index`` large steps; returns the path.  The file is written beside
+    ``path`` and renamed over it, so a reader never sees a partial restart."""
 
-    missing = [k for k in STATE_FIELDS if k not in state]
+    fields = state_fields(namelist)
+    missing = [k for k in fields if k not in state]
     if missing:
         raise ValueError(f"state lacks {missing}")
     shapes = metrics.shapes()
     arrays: dict[str, np.ndarray] = {}
-    for k in STATE_FIELDS:
+    for k in fields:
         a = _host(state[k])
         if a.shape != shapes[k]:
             raise ValueError(f"state[{k!r}] has shape {a.shape}, the grid needs {shapes[k]}")
@@ -75,7 +79,7 @@ def write_restart(state: dict, metrics: Metrics, namelist: Namelist, step_index:
     header = {
         "format": FORMAT,
         "format_version": FORMAT_VERSION,
-        "state_field_order": list(STATE_FIELDS),
+        "state_field_order": list(fields),
         "metric_scalars": {k: getattr(metrics, k) for k in _METRIC_SCALARS},
         "metric_array_order": list(_METRIC_ARRAYS),
         "namelist": dataclasses.asdict(namelist),
@@ -98,14 +102,22 @@ def _read_header(z) -> dict:
     header = json.loads(bytes(z["header"]).decode())
     if header.get("format") != FORMAT:
         raise ValueError(f"unsupported restart format {header.get('format')!r}")
-    if int(header.get("format_version", -1)) not in SUPPORTED_FORMAT_VERSIONS:
+    version = int(header.get("format_version", -1))
+    if version not in SUPPORTED_FORMAT_VERSIONS:
         raise ValueError(f"unsupported restart version {header.get('format_version')}; "
                          f"supported {SUPPORTED_FORMAT_VERSIONS}")
-    if tuple(header.get("state_field_order", ())) != STATE_FIELDS:
+    nl = header.get("namelist")
+    if not isinstance(nl, dict):
+        raise ValueError("restart header has no namelist")
+    if version == 1:
+        nl = {**_V1_NAMELIST_DEFAULTS, **nl}
+        header["namelist"] = nl
+    fields = STATE_FIELDS + (("qv", "qc", "qr", "qi", "qs", "qg") if nl.get("moist") else ())
+    if tuple(header.get("state_field_order", ())) != fields:
         raise ValueError("restart state field order does not match the current state schema")
     if tuple(header.get("metric_array_order", ())) != _METRIC_ARRAYS:
         raise ValueError("restart metric fields do not match the current metrics schema")
-    want = {f"state.{k}" for k in STATE_FIELDS} | {f"metrics.{k}" for k in _METRIC_ARRAYS} | {"header"}
+    want = {f"state.{k}" for k in fields} | {f"metrics.{k}" for k in _METRIC_ARRAYS} | {"header"}
     if set(z.files) != want:
         raise ValueError("restart arrays do not match its header")
     return header
@@ -122,7 +134,7 @@ def read_restart(path: str | Path, device: str | None = None) -> tuple[dict, Met
                           **{k: np.array(z[f"metrics.{k}"]) for k in _METRIC_ARRAYS})
         shapes = metrics.shapes()
         state = {}
-        for k in STATE_FIELDS:
+        for k in header["state_field_order"]:
             a = np.array(z[f"state.{k}"])
             if a.shape != shapes[k] or a.dtype != np.float64:
                 raise ValueError(f"restart state[{k!r}] has shape {a.shape} / {a.dtype}, the grid needs {shapes[k]}")

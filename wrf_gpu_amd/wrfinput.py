@@ -11,7 +11,11 @@ of the file it builds the state in the reference layout
   ``P``, ``PH`` and ``MU`` the perturbations;
 * ``u``, ``v``, ``w`` as stored (staggered ``west_east_stag``,
   ``south_north_stag``, ``bottom_top_stag``);
-* ``mudf`` zero (a cold start; the carried value only exists in a restart).
+* ``mudf`` zero (a cold start; the carried value only exists in a restart);
+* the moisture species ``qv, qc, qr, qi, qs, qg`` from ``QVAPOR``, ``QCLOUD``,
+  ``QRAIN``, ``QICE``, ``QSNOW``, ``QGRAUP`` when the file has ``QVAPOR``
+  (a species its microphysics does not carry is zero, as in the reference's
+  State); step such a state with a moist namelist (``moist=True``).
 
 The metrics come from the vertical coordinate arrays WRF writes (``ZNW``,
 ``C1H``, ``C2H``, ``C1F``, ``C2F``, ``DNW``, ``RDNW``, ``DN``, ``RDN``,
@@ -52,6 +56,7 @@ _MAPFAC = {"msftx": "MAPFAC_MX", "msfty": "MAPFAC_MY", "msfux": "MAPFAC_UX", "ms
 _SCALARS = {"cf1": "CF1", "cf2": "CF2", "cf3": "CF3", "p_top": "P_TOP"}
 # Metrics field: (WRF variable, value when absent)
 _CORIOLIS = {"f": ("F", 0.0), "e": ("E", 0.0), "sina": ("SINALPHA", 0.0), "cosa": ("COSALPHA", 1.0)}
+_MOIST = {"qv": "QVAPOR", "qc": "QCLOUD", "qr": "QRAIN", "qi": "QICE", "qs": "QSNOW", "qg": "QGRAUP"}
 # 2 Omega of WRF's EARTH_OMEGA, for F and E from XLAT (dynamics/metrics.py:44)
 _TWO_OMEGA = 2.0 * 7.2921e-5
 
@@ -129,13 +134,19 @@ def read_wrfinput(path: str | Path, time_index: int = 0, device: str | None = No
         u, v, w = _var(f, "U", t), _var(f, "V", t), _var(f, "W", t)
         T, P, PB = _var(f, "T", t), _var(f, "P", t), _var(f, "PB", t)
         PH, PHB, MU, MUB = _var(f, "PH", t), _var(f, "PHB", t), _var(f, "MU", t), _var(f, "MUB", t)
+        # the moisture species (State qv..qg): with QVAPOR present, all six,
+        # a species the file lacks (its microphysics does not carry it) as zeros
+        moist = {}
+        if "QVAPOR" in f.variables:
+            for key, name in _MOIST.items():
+                moist[key] = _var(f, name, t) if name in f.variables else np.zeros((nz, ny, nx))
     state = {
         "u": u, "v": v, "w": w, "theta": T + THETA_BASE_OFFSET_K, "p_total": P + PB, "p_perturbation": P,
         "ph_total": PH + PHB, "ph_perturbation": PH, "mu_total": MU + MUB, "mu_perturbation": MU,
-        "mudf": np.zeros((ny, nx)),
+        "mudf": np.zeros((ny, nx)), **moist,
     }
     shapes = metrics.shapes()
-    for k in STATE_FIELDS:
+    for k in state:
         if state[k].shape != shapes[k]:
             raise ValueError(f"{k} has shape {state[k].shape}, the grid needs {shapes[k]}")
     want = {"msftx": (ny, nx), "msfty": (ny, nx), "msfux": (ny, nx + 1), "msfuy": (ny, nx + 1),

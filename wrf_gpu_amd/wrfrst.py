@@ -29,11 +29,13 @@ from pathlib import Path
 
 import numpy as np
 
-from .grid import STATE_FIELDS, Metrics, Namelist
-from .restart import _METRIC_ARRAYS, _METRIC_SCALARS, _host, _json_default
+from .grid import MOIST_FIELDS, STATE_FIELDS, Metrics, Namelist, state_fields
+from .restart import _METRIC_ARRAYS, _METRIC_SCALARS, _V1_NAMELIST_DEFAULTS, _host, _json_default
 
 FORMAT = "wrf_gpu_amd.wrfrst"
-FORMAT_VERSION = 1
+# 2: a moist state adds WRF's QVAPOR..QGRAUP and their exact layer
+FORMAT_VERSION = 2
+SUPPORTED_FORMAT_VERSIONS = (1, 2)
 STATE_EXTENSION_PREFIX = "GPUWRF_STATE_"
 CARRY_EXTENSION_PREFIX = "GPUWRF_CARRY_"
 METRIC_EXTENSION_PREFIX = "GPUWRF_METRIC_"
@@ -60,10 +62,30 @@ _STANDARD = {
     "MU": (XY, "XY ", "perturbation dry air mass in column", "Pa", "", lambda s: s["mu_perturbation"]),
     "MUB": (XY, "XY ", "base state dry air mass in column", "Pa", "", lambda s: s["mu_total"] - s["mu_perturbation"]),
 }
+# the moisture species under WRF's names (written only for a moist state)
+_MOIST_STANDARD = {
+    "QVAPOR": ("qv", "Water vapor mixing ratio"),
+    "QCLOUD": ("qc", "Cloud water mixing ratio"),
+    "QRAIN": ("qr", "Rain water mixing ratio"),
+    "QICE": ("qi", "Ice mixing ratio"),
+    "QSNOW": ("qs", "Snow mixing ratio"),
+    "QGRAUP": ("qg", "Graupel mixing ratio"),
+}
 _STATE_DIMS = {
     "u": U_XYZ, "v": V_XYZ, "w": W_XYZ, "theta": XYZ, "p_total": XYZ, "p_perturbation": XYZ,
     "ph_total": W_XYZ, "ph_perturbation": W_XYZ, "mu_total": XY, "mu_perturbation": XY, "mudf": XY,
+    **{q: XYZ for q in MOIST_FIELDS},
 }
+
+
+def _standard(moist: bool) -> dict:
+    """The WRF variables of the standard layer: the dry ones, and QVAPOR..QGRAUP when moist."""
+
+    out = dict(_STANDARD)
+    if moist:
+        for name, (key, desc) in _MOIST_STANDARD.items():
+            out[name] = (XYZ, "XYZ", desc, "kg kg-1", "", lambda s, key=key: s[key])
+    return out
 
 
 def _token(name: str) -> str:
@@ -94,14 +116,15 @@ def write_wrfrst(state: dict, metrics: Metrics, namelist: Namelist, step_index: 
 
     from scipy.io import netcdf_file
 
-    missing = [k for k in STATE_FIELDS if k not in state]
+    fields = state_fields(namelist)
+    missing = [k for k in fields if k not in state]
     if missing:
         raise ValueError(f"state lacks {missing}")
     if len(date) != 19:
         raise ValueError("date must be WRF's 19-character YYYY-MM-DD_hh:mm:ss")
     shapes = metrics.shapes()
     host = {}
-    for k in STATE_FIELDS:
+    for k in fields:
         a = _host(state[k])
         if a.shape != shapes[k]:
             raise ValueError(f"state[{k!r}] has shape {a.shape}, the grid needs {shapes[k]}")
@@ -119,7 +142,7 @@ def write_wrfrst(state: dict, metrics: Metrics, namelist: Namelist, step_index: 
             f.createDimension(name, n)
         times = f.createVariable("Times", "c", ("Time", "DateStrLen"))
         times[0, :] = np.frombuffer(date.encode(), dtype="S1")
-        for name, (dims, order, desc, units, stagger, value) in _STANDARD.items():
+        for name, (dims, order, desc, units, stagger, value) in _standard(namelist.moist).items():
             v = f.createVariable(name, "f4", dims)
             v[0] = np.asarray(value(host), dtype=np.float32)
             v.FieldType = np.int32(104)
@@ -127,7 +150,7 @@ def write_wrfrst(state: dict, metrics: Metrics, namelist: Namelist, step_index: 
             v.description = desc
             v.units = units
             v.stagger = stagger
-        for k in STATE_FIELDS:
+        for k in fields:
             name = CARRY_EXTENSION_PREFIX + "MUDF" if k == "mudf" else state_extension_name(k)
             v = f.createVariable(name, "f8", _STATE_DIMS[k])
             v[0] = host[k]
@@ -147,7 +170,7 @@ def write_wrfrst(state: dict, metrics: Metrics, namelist: Namelist, step_index: 
         f.GPUWRF_FORMAT = FORMAT
         f.GPUWRF_FORMAT_VERSION = np.int32(FORMAT_VERSION)
         f.GPUWRF_STEP_INDEX = np.int32(step_index)
-        f.GPUWRF_STATE_FIELD_ORDER = ",".join(STATE_FIELDS)
+        f.GPUWRF_STATE_FIELD_ORDER = ",".join(fields)
         f.GPUWRF_METRIC_SCALARS = json.dumps({k: getattr(m, k) for k in _METRIC_SCALARS}, default=_json_default)
         f.GPUWRF_NAMELIST = json.dumps(dataclasses.asdict(namelist), default=_json_default)
     finally:
@@ -173,9 +196,15 @@ def read_wrfrst(path: str | Path, device: str | None = None) -> tuple[dict, Metr
     with netcdf_file(str(path), "r", mmap=False) as f:
         if _attr(f, "GPUWRF_FORMAT") != FORMAT:
             raise ValueError("unsupported wrfrst format")
-        if int(f._attributes.get("GPUWRF_FORMAT_VERSION", -1)) != FORMAT_VERSION:
+        version = int(f._attributes.get("GPUWRF_FORMAT_VERSION", -1))
+        if version not in SUPPORTED_FORMAT_VERSIONS:
             raise ValueError("unsupported wrfrst version")
-        if tuple(_attr(f, "GPUWRF_STATE_FIELD_ORDER").split(",")) != STATE_FIELDS:
+        nl = json.loads(_attr(f, "GPUWRF_NAMELIST"))
+        if version == 1:
+            nl = {**_V1_NAMELIST_DEFAULTS, **nl}
+        moist = bool(nl.get("moist", False))
+        fields = STATE_FIELDS + (MOIST_FIELDS if moist else ())
+        if tuple(_attr(f, "GPUWRF_STATE_FIELD_ORDER").split(",")) != fields:
             raise ValueError("wrfrst state field order does not match the current state schema")
         scal = json.loads(_attr(f, "GPUWRF_METRIC_SCALARS"))
         arrays = {}
@@ -187,7 +216,7 @@ def read_wrfrst(path: str | Path, device: str | None = None) -> tuple[dict, Metr
         metrics = Metrics(**{k: scal[k] for k in _METRIC_SCALARS}, **arrays)
         shapes = metrics.shapes()
         state = {}
-        for k in STATE_FIELDS:
+        for k in fields:
             name = CARRY_EXTENSION_PREFIX + "MUDF" if k == "mudf" else state_extension_name(k)
             if name not in f.variables:
                 raise ValueError(f"wrfrst lacks {name}")
@@ -195,13 +224,12 @@ def read_wrfrst(path: str | Path, device: str | None = None) -> tuple[dict, Metr
             if a.shape != shapes[k]:
                 raise ValueError(f"wrfrst {name} has shape {a.shape}, the grid needs {shapes[k]}")
             state[k] = np.ascontiguousarray(a)
-        for name, (_, _, _, _, _, value) in _STANDARD.items():
+        for name, (_, _, _, _, _, value) in _standard(moist).items():
             if name not in f.variables:
                 raise ValueError(f"wrfrst lacks the WRF variable {name}")
             want = np.asarray(value(state), dtype=np.float32)
             if not np.array_equal(np.array(f.variables[name][0]), want):
                 raise ValueError(f"the WRF variable {name} disagrees with the exact layer")
-        nl = json.loads(_attr(f, "GPUWRF_NAMELIST"))
         step = int(f._attributes["GPUWRF_STEP_INDEX"])
     if set(nl) != {fl.name for fl in dataclasses.fields(Namelist)}:
         raise ValueError("wrfrst namelist fields do not match the current namelist schema")
