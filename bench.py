@@ -87,6 +87,9 @@ def parse():
     ap.add_argument("--cpu-baseline-steps", type=int, default=8,
                     help="large steps of the oracle timed on the host (0 skips)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all cores in this process's affinity")
+    ap.add_argument("--cpu-small-steps", type=int, default=100,
+                    help="large steps of the oracle on the 64x64x40 warm bubble, timed beside the main sample "
+                         "(0 skips)")
     ap.add_argument("--form-steps", type=int, default=200,
                     help="large steps of the mixed-precision form leg (error measured after them); 0 skips it")
     ap.add_argument("--filter-steps", type=int, default=20,
@@ -553,6 +556,18 @@ def main():
                "sample": f"{args.cpu_baseline_steps} large step(s) of the same {m.nx}x{m.ny}x{m.nz} workload "
                          f"from the initial state, CPU oracle (C, OpenMP), {cpu_s:.1f} s"}
         print(f"[bench] cpu baseline {cpu_s:.1f}s on {threads} threads", file=sys.stderr, flush=True)
+        if args.cpu_small_steps > 0:
+            # the CPU-runnable 64x64x40 warm bubble, over the oracle as well
+            small = ic.bubble_3d()
+            t1 = time.time()
+            oracle.run(small.metrics, small.namelist, small.state, args.cpu_small_steps, threads=threads)
+            small_s = time.time() - t1
+            cpu["small_workload"] = {
+                "value": small.metrics.n_cells * small.namelist.acoustic_substeps * args.cpu_small_steps / small_s,
+                "unit": "cell-substeps/s", "cores": threads,
+                "sample": f"{args.cpu_small_steps} large steps of the 64x64x40 warm bubble (dt=6 s, ns=6), "
+                          f"CPU oracle (C, OpenMP), {small_s:.1f} s"}
+            print(f"[bench] cpu baseline, 64x64x40: {small_s:.1f}s", file=sys.stderr, flush=True)
 
     # the layered forms: mixed precision on this workload, and the batched ensemble
     forms = {}
