@@ -100,6 +100,8 @@ def parse():
                     help="large steps of the monotonic theta-limiter leg on the same workload (0 skips it)")
     ap.add_argument("--constk-steps", type=int, default=20,
                     help="large steps of the constant-K diffusion leg on the same workload (0 skips it)")
+    ap.add_argument("--graph-steps", type=int, default=20,
+                    help="large steps of the same workload replayed from a captured graph (0 skips it)")
     ap.add_argument("--moist-steps", type=int, default=20,
                     help="large steps of the moist leg (six species, coupled and transported) on the same workload "
                          "(0 skips it)")
@@ -571,6 +573,19 @@ def main():
 
     # the layered forms: mixed precision on this workload, and the batched ensemble
     forms = {}
+    if args.graph_steps > 0:
+        # the same step captured once into a HIP graph and replayed (wd_capture_graph / wd_graph_step)
+        gdyn = Dycore(m, nl)
+        gst = state_to_device(case.state, dev)
+        gdyn.capture_graph(gst)
+        gdyn.graph_step(1)
+        gms = timed_steps(lambda n: gdyn.graph_step(n), args.graph_steps, torch.cuda.current_stream(dev))
+        gms /= args.graph_steps
+        forms["graph_replay"] = {"ms_per_step": gms, "value": n_cells * ns / (gms * 1e-3), "steps": args.graph_steps,
+                                 "vs_eager": ms_per_step / gms}
+        gdyn.close()
+        del gst
+        print(f"[bench] graph replay leg done", file=sys.stderr, flush=True)
     if args.form_steps > 0:
         forms["mixed"] = mixed_form_leg(case, args.form_steps, ms_per_step, dev)
         print(f"[bench] mixed form leg done", file=sys.stderr, flush=True)
