@@ -130,3 +130,30 @@ def test_moist_full_size_step_matches_oracle_bitwise(case, oracle_lib):
     print("[full] max |gpu - oracle|:", {k: f"{v:.1e}" for k, v in diff.items()}, flush=True)
     for k in keys:
         assert np.array_equal(got[k], ref[k]), (k, diff[k])
+
+
+@pytest.mark.skipif(not __import__("os").environ.get("WRFDYN_SLOW"),
+                    reason="about 3 minutes of CPU oracle; set WRFDYN_SLOW=1 to run it")
+def test_hundred_full_size_steps_match_oracle_bitwise(case, oracle_lib):
+    """The plain form's specified length at the workload's size: 100 large steps of the
+    433x433x50 terrain case, every field bit for bit against the oracle."""
+
+    import torch
+    from wrf_gpu_amd.dycore import Dycore, state_to_device, state_to_host
+
+    m = case.metrics
+    print(f"[full] {m.nx}x{m.ny}x{m.nz}, 100 steps against the oracle", flush=True)
+    ref = case.state
+    for chunk in range(5):  # the oracle's state is its whole carry, so chunks continue exactly
+        ref = oracle_lib.run(m, case.namelist, ref, 20)
+        print(f"[full] oracle at step {20 * (chunk + 1)}", flush=True)
+    dyn = Dycore(m, case.namelist)
+    st = state_to_device(case.state)
+    dyn.step(st, 100)
+    torch.cuda.synchronize()
+    got = state_to_host(st)
+    dyn.close()
+    for k in FIELDS:
+        print(f"[full] {k}: bitwise {np.array_equal(got[k], ref[k])}", flush=True)
+    for k in FIELDS:
+        assert np.array_equal(got[k], ref[k]), k
