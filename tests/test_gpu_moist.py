@@ -5,8 +5,9 @@ core/rk_addtend_dry.py:278, :359-360) and, with moist_adv_opt 1 or 2, are transp
 (runtime/operational_mode.py:3162-3316).  Covered: terrain with each moist_adv_opt, the
 2-D slab, the theta limiter and the species' limiter sharing their scratch on the final
 stage, the diffusions, a captured graph, wd_step_noalloc, and the autotuner (which must
-leave the caller's species untouched and keep the column march); a state whose species
-are zero steps as the dry state; and the handle refuses mismatched states."""
+leave the caller's species untouched and keep the column march); every option at once
+(rotation, hypsometric_opt 2, both limiters, the filter); a state whose species are zero
+steps as the dry state; and the handle refuses mismatched states."""
 
 import ctypes
 
@@ -52,6 +53,20 @@ def _gpu(case, n_steps, mode="step"):
         dyn.close()
     torch.cuda.synchronize()
     return state_to_host(st)
+
+
+def _rotating(m, lat0=45.0):
+    """A beta-plane f and e = 2 Omega cos(lat) about lat0, and a projection rotation varying in x."""
+
+    import dataclasses
+
+    two_omega = 2.0 * 7.2921e-5
+    y = (np.arange(m.ny) + 0.5) * m.dy
+    x = (np.arange(m.nx) + 0.5) * m.dx
+    lat = np.deg2rad(lat0 + (y[:, None] - y.mean()) / 111e3 + 0.0 * x[None, :])
+    alpha = np.deg2rad(8.0) * np.sin(2.0 * np.pi * x[None, :] / (m.nx * m.dx)) + 0.0 * y[:, None]
+    return dataclasses.replace(m, f=two_omega * np.sin(lat), e=two_omega * np.cos(lat), sina=np.sin(alpha),
+                               cosa=np.cos(alpha))
 
 
 def _check_bitwise(got, ref, keys):
@@ -145,3 +160,22 @@ def test_handles_refuse_mismatched_states():
         moist.set_precision("mixed")
     moist.close()
     dry.close()
+
+
+@pytest.mark.parametrize("grid", [(40, 36, 16), (21, 7, 9)])
+def test_moist_with_rotation_log_pressure_and_filter(oracle_lib, grid):
+    """Every option at once: moisture with the monotonic species limiter, a rotating grid
+    (beta-plane f, e, a varying projection rotation), hypsometric_opt 2 with c3 = eta,
+    the positive-definite theta limiter and the 6th-order filter."""
+
+    import dataclasses
+
+    nx, ny, nz = grid
+    case = _moist_case(nx, ny, nz, 2, hypsometric_opt=2, scalar_adv_opt=1, diff_6th_opt=2)
+    m = case.metrics
+    eta_mass = 0.5 * (m.eta[:-1] + m.eta[1:])
+    m = _rotating(dataclasses.replace(m, c3h=eta_mass, c4h=np.zeros(nz), c3f=m.eta.copy(), c4f=np.zeros(nz + 1)))
+    case = case.__class__(**{**case.__dict__, "metrics": m})
+    ref = oracle_lib.run(case.metrics, case.namelist, case.state, 3)
+    got = _gpu(case, 3)
+    _check_bitwise(got, ref, FIELDS + SPECIES)
