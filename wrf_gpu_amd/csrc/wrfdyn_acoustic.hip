@@ -42,7 +42,8 @@ __device__ inline double dpn_ac(const DGrid& g, const double* __restrict__ p, in
 // sums of the work pressure are carried up the column (interior dpn(k) uses
 // levels k and k-1; the bottom and rigid-lid top faces use three levels).
 // ---------------------------------------------------------------------------
-template <int BX, int BY, class TT>
+// WET: the state carries moisture (cqu/cqv scale the pressure gradient); dry launches fold them to 1
+template <int BX, int BY, class TT, bool WET>
 __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   // waves overlap by one lane: lanes 1..BX-1 own faces, lane 0 holds the
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
       double dpx = (((msf_u * 0.5) * g.rdx) * mass_x) * ((ph_term + p_term) + pb_term);
       double bracket = g.rdnw[k] * (dpn_u1 - dpn_u) - 0.5 * (g.c1h[k] * muwsum_u);
       dpx = dpx + ((msf_u * g.rdx) * (phpc - cur.phpw)) * bracket;
-      u = u - (dts * (w.cqu ? ldo(w.cqu, q) : 1.0)) * dpx;
+      u = u - (dts * (WET ? ldo(w.cqu, q) : 1.0)) * dpx;
       u = u + g.c1h[k] * mudf_u;
       sto(w.u_work, q, u);
       phw = phw1;
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
       double dpy = (((msf_v * 0.5) * g.rdy) * mass_y) * ((ph_term + p_term) + pb_term);
       double bracket = g.rdnw[k] * (dpn_v1 - dpn_v) - 0.5 * (g.c1h[k] * muwsum_v);
       dpy = dpy + ((msf_v * g.rdy) * (phpc - cur.phps)) * bracket;
-      v = v - (dts * (w.cqv ? ldo(w.cqv, q) : 1.0)) * dpy;
+      v = v - (dts * (WET ? ldo(w.cqv, q) : 1.0)) * dpy;
       v = v + g.c1h[k] * mudf_v;
       sto(w.v_work, q, v);
       phs = phs1;
@@ -1366,10 +1367,13 @@ hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DS
                                    double dts, int first, int column_tile, hipStream_t st, Prof* prof) {
   constexpr int BX = 64, BY = 4;
   const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 2) / (BX - 1)) * (unsigned)((g.ny + 1 + BY - 1) / BY);
-  if (nl.mixed)
-    WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, float><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
-  else
-    WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, double><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
+  if (nl.mixed) {
+    WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, float, false><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
+  } else if (w.cqu) {
+    WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, double, true><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
+  } else {
+    WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, double, false><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
+  }
   // the LDS-tiled kernel has no moisture factor: a moist state runs the column march
   return launch_column_tile(w.cqwr ? kColumnMarch : column_tile, g, nl, s, w, dts, first, st, prof);
 }
@@ -1382,7 +1386,7 @@ hipError_t launch_substep_part(int part, const DGrid& g, const DNamelist& nl, co
   const size_t C = (size_t)g.ny * g.nx;
   if (part == 0) {
     const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 2) / (BX - 1)) * (unsigned)((g.ny + 1 + BY - 1) / BY);
-    k_uv<BX, BY, double><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts);
+    k_uv<BX, BY, double, false><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts);
   } else if (part == 1) {
     const unsigned mu_tiles = (unsigned)((g.nx + kMuOwned - 1) / kMuOwned) * (unsigned)((g.ny + kMuRows - 1) / kMuRows);
     k_column_mu<double><<<mu_tiles, dim3(64, kMuRows), 0, st>>>(g, nl, s, w, dts, first);

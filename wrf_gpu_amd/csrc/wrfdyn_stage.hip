@@ -368,7 +368,8 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
 // advect_u_flux (flux_advection.py:1131-1202) + large_step_horizontal_pgf x
 // (rk_addtend_dry.py:329-343, 357-361) on u faces (nz, ny, nx+1).  Face nx
 // repeats face 0's advection (periodic); the PGF uses edge-padded pairs.
-template <class TT>
+// WET: the moisture factor cqu scales the PGF; dry launches fold it to 1.
+template <class TT, bool WET>
 __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, DWork w) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -555,7 +556,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
     dpx = dpx + ((msf_u * g.rdx) * (cur.phpr - cur.phpl)) * bracket;
     if (owned) {
       const unsigned off = (unsigned)k * upl8 + fo8;
-      stt<TT>(w.ut, off, add_diff(t, w.d6u, w.smu, off) + (-(w.cqu ? ldo(w.cqu, off) : 1.0)) * dpx);
+      stt<TT>(w.ut, off, add_diff(t, w.d6u, w.smu, off) + (-(WET ? ldo(w.cqu, off) : 1.0)) * dpx);
     }
     qm1 = q0;
     q0 = qp1;
@@ -575,7 +576,8 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
 // advect_v_flux (flux_advection.py:1205-1274) + large_step_horizontal_pgf y
 // (rk_addtend_dry.py:345-361) on v faces (nz, ny+1, nx).  Face ny repeats
 // face 0's advection (periodic); the PGF uses edge-padded pairs.
-template <class TT>
+// WET: the moisture factor cqv scales the PGF; dry launches fold it to 1.
+template <class TT, bool WET>
 __global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, DWork w) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -784,7 +786,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, D
     dpy = dpy + ((msf_v * g.rdy) * (cur.phpr - cur.phpl)) * bracket;
     if (owned) {
       const unsigned off = (unsigned)k * upl8 + fo8;
-      stt<TT>(w.vt, off, add_diff(t, w.d6v, w.smv, off) + (-(w.cqv ? ldo(w.cqv, off) : 1.0)) * dpy);
+      stt<TT>(w.vt, off, add_diff(t, w.d6v, w.smv, off) + (-(WET ? ldo(w.cqv, off) : 1.0)) * dpy);
     }
     qm1 = q0;
     q0 = qp1;
@@ -1029,7 +1031,8 @@ __global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
 constexpr int kColRows = 4;
 constexpr int kColOwned = 62;
 
-template <class TT>
+// WET: the state carries moisture (the raw cqw is read); a dry launch folds cq1 = 1, cq2 = 0
+template <class TT, bool WET>
 __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const unsigned nbx = (unsigned)(nx + kColOwned - 1) / kColOwned;
@@ -1111,7 +1114,7 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
     L.phtp = ldo(s.ph_total, o + pl8 + c8);
     L.ww = ldo(s.w, o + c8);
     L.c2a = ldo(w.c2a, o + c8);
-    L.cqr = w.cqwr ? ldo(w.cqwr, o + c8) : 0.0;
+    if constexpr (WET) L.cqr = ldo(w.cqwr, o + c8);
     if (load_n) {
       L.pnp = ldo(s.ph_pert, o + cjp8);
       L.pnt = ldo(s.ph_total, o + cjp8);
@@ -1159,7 +1162,8 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
       pht_s = load_s ? PHT(cur.psp, cur.pst) : sh[f & 1][ty - 1][0][tx];
       if (!load_vn) cur.vn = sh[f & 1][ty + 1][1][tx];
     }
-    const double cq1 = 1.0 / (1.0 + cur.cqr), cq2 = cur.cqr * cq1;
+    // (dry: cq1 = 1/(1 + 0) = 1 and cq2 = 0 exactly)
+    const double cq1 = WET ? 1.0 / (1.0 + cur.cqr) : 1.0, cq2 = WET ? cur.cqr * cq1 : 0.0;
     // ---- rw_tend ----
     {
       double rw = (msft_inv * kG) *
@@ -1238,13 +1242,13 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
     pht_m = pht_0;
     pht_0 = pht_p;
     c2a_m = cur.c2a;
-    cqr_m = cur.cqr;
+    if constexpr (WET) cqr_m = cur.cqr;
     cur = nxt;
   }
   // top face nz (p_m, c2a_m, cqr_m now hold level nz-1): the raw cqw of face
   // nz - 1 (advance_w.py:266-290)
   if (!owned) return;  // no barrier follows
-  const double cq1 = 1.0 / (1.0 + cqr_m), cq2 = cqr_m * cq1;
+  const double cq1 = WET ? 1.0 / (1.0 + cqr_m) : 1.0, cq2 = WET ? cqr_m * cq1 : 0.0;
   const unsigned qt = c8 + (unsigned)nz * pl8;
   double rwt = (msft_inv * kG) *
                (((((cq1 * 2.0) * g.rdnw[nz - 1]) * (-p_m)) - g.c1f[nz] * mu_p) - cq2 * (g.c1f[nz] * mub + g.c2f[nz]));
@@ -2013,8 +2017,8 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const unsigned tt_tiles = (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
   const unsigned tv_tiles = (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + 1 + kTendURows - 1) / kTendURows);
   if (nl.mixed) {
-    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
-    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float, false><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
+    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float, false><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<float><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
     if (limit_theta)
       WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<float>(g, nl, s, w, theta_lim, nl.scalar_adv_opt, st)));
@@ -2022,8 +2026,13 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
       WD_TIMED(prof, KID_TEND_T, st,
                (k_tend_t<float><<<tt_tiles, tu_block, 0, st>>>(g, w, s.theta, kT0, w.tht, w.d6t, w.smt)));
   } else {
-    WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
-    WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
+    if (w.cqu) {
+      WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double, true><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
+      WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double, true><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
+    } else {
+      WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<double, false><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
+      WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<double, false><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
+    }
     WD_TIMED(prof, KID_TEND_W, st, (k_tend_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
     if (limit_theta)
       WD_TIMED(prof, KID_LIMITER, st, (launch_limiter<double>(g, nl, s, w, theta_lim, nl.scalar_adv_opt, st)));
@@ -2057,10 +2066,13 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const unsigned col_tiles =
       (unsigned)((g.nx + kColOwned - 1) / kColOwned) * (unsigned)((g.ny + kColRows - 1) / kColRows);
   const dim3 col_block(64, kColRows);
-  if (nl.mixed)
-    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<float><<<col_tiles, col_block, 0, st>>>(g, nl, s, w, dts)));
-  else
-    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<double><<<col_tiles, col_block, 0, st>>>(g, nl, s, w, dts)));
+  if (nl.mixed) {
+    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<float, false><<<col_tiles, col_block, 0, st>>>(g, nl, s, w, dts)));
+  } else if (w.cqwr) {
+    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<double, true><<<col_tiles, col_block, 0, st>>>(g, nl, s, w, dts)));
+  } else {
+    WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<double, false><<<col_tiles, col_block, 0, st>>>(g, nl, s, w, dts)));
+  }
   return hipGetLastError();
 }
 
