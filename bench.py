@@ -380,8 +380,9 @@ def small_workload_leg(steps: int, dev) -> dict:
 def ensemble_leg(steps: int, dev) -> dict:
     """32 members (8 each of 129^2, 201^2, 257^2, 321^2 columns, 50 levels):
     members stepped one after another with the plain step, then the batched
-    graph (4 branches) in FP64 and in the mixed form, and in FP64 with each
-    member's autotuned column kernel and workgroup order."""
+    graph (4 branches) and the persistent step (one launch per large step),
+    each in FP64 and in the mixed form, and the batched graph in FP64 with
+    each member's autotuned column kernel and workgroup order."""
 
     import torch
 
@@ -417,6 +418,15 @@ def ensemble_leg(steps: int, dev) -> dict:
         out[f"batched_graph_{prec}"] = {"ms_per_step": ms, "value": cells * ns / (ms * 1e-3),
                                         "branches": ens.branches}
         ens.close()
+        # one persistent launch per large step (the barrier's report is checked after the timing)
+        ens = Ensemble(members, persistent=True)
+        ens.step(1)
+        ens.status()
+        ms = timed_steps(ens.step, steps, stream) / steps
+        ens.status()
+        out[f"persistent_{prec}"] = {"ms_per_step": ms, "value": cells * ns / (ms * 1e-3),
+                                     "workgroups": ens.workgroups}
+        ens.close()
     # the batched FP64 graph with every member's autotuned column kernel and workgroup order
     for d, _ in members:
         d.set_precision("fp64")
@@ -436,6 +446,7 @@ def ensemble_leg(steps: int, dev) -> dict:
     base = out["sequential_fp64"]["ms_per_step"]
     out["speedup_vs_sequential_fp64"] = {k: base / out[k]["ms_per_step"]
                                          for k in ("batched_graph_fp64", "batched_graph_mixed",
+                                                   "persistent_fp64", "persistent_mixed",
                                                    "batched_graph_fp64_autotuned")}
     for d, _ in members:
         d.close()

@@ -196,6 +196,35 @@ int wd_step_batched(wd_batch *b, int n_large_steps, void *stream);
 int wd_batch_branch(const wd_batch *b, int member);
 int wd_batch_branches(const wd_batch *b);
 
+/* The same ensemble as one persistent launch per large step (no counterpart
+ * in the reference).  One large step of every member is a program of phases
+ * (the stage setup's kernels grouped where they touch disjoint arrays, each
+ * acoustic kernel, the finish), and a single launch of as many 256-thread
+ * workgroups as the GPU holds at once runs it: in every phase the
+ * workgroups take the blocks the standalone launches of that phase's
+ * kernels would have, for every member, and run the same kernel code on
+ * them, then meet at a grid barrier.  The launch is captured in the batch's
+ * graph, so wd_step_batched runs it n_large_steps times; each member's
+ * result is identical to wd_step on that member alone.  Members must share
+ * the precision form and the substep count and use none of the explicit
+ * diffusions, the limiter or the moisture (WD_EINVAL); the column march
+ * serves every member whatever its tuned column form, and each keeps its
+ * tuned workgroup order.  The grid barrier needs every workgroup resident,
+ * so nothing else may run on the GPU during a persistent step: a barrier
+ * wait that runs out of polls (wd_batch_set_spin_limit) abandons the step
+ * and every later one until wd_batch_status reports it.  wd_batch_branches
+ * is 1 and wd_batch_workgroups gives the launch's workgroup count. */
+int wd_batch_create_persistent(wd_handle *const *members, const wd_state *states, int n_members,
+                               wd_batch **out);
+/* Synchronises `stream`; for a persistent batch, WD_EDEVICE if a grid
+ * barrier wait ran out since the last call (the steps since then did not
+ * complete), after which the flag is cleared and the batch can step again. */
+int wd_batch_status(wd_batch *b, void *stream);
+/* The bound on one grid barrier wait, in polls of about a microsecond each
+ * (default 4,194,304).  Synchronises the device. */
+int wd_batch_set_spin_limit(wd_batch *b, unsigned polls);
+int wd_batch_workgroups(const wd_batch *b);
+
 const char *wd_batch_last_error(const wd_batch *b);
 void wd_batch_destroy(wd_batch *b);
 

@@ -166,6 +166,15 @@ def test_graph_batch_autotune_and_adjoint_entry_points_reject_bad_arguments():
     lib.wd_batch_destroy(None)
     assert lib.wd_step_adjoint(None, None, None, None) == _lib.WD_EINVAL
     assert lib.wd_adjoint_tape_bytes(None) == 0
+    # the persistent batch: refused before any device call
+    assert lib.wd_batch_create_persistent(None, None, 0, ctypes.byref(b)) == _lib.WD_EINVAL
+    assert "member" in _lib.last_error(None)
+    assert lib.wd_batch_create_persistent(handles, states, 1, ctypes.byref(b)) == _lib.WD_EINVAL
+    assert "NULL" in _lib.last_error(None)
+    assert lib.wd_batch_create_persistent(handles, states, 1, None) == _lib.WD_EINVAL
+    assert lib.wd_batch_status(None, None) == _lib.WD_EINVAL
+    assert lib.wd_batch_set_spin_limit(None, 0) == _lib.WD_EINVAL
+    assert lib.wd_batch_workgroups(None) == _lib.WD_EINVAL
 
 
 def test_namelist_validation_matches_supported_path():

@@ -1,0 +1,167 @@
+"""GPU checks of the persistent ensemble step (wd_batch_create_persistent).
+
+One launch per large step runs every kernel body of every member, the
+workgroups meeting at a grid barrier between phases.  Each virtual block is
+run by the same kernel code, with the block index, grid and thread layout of
+the standalone launch, as wd_step, so every member must come out bit for bit
+as that member stepped alone: FP64 and mixed, with rotation, at the uneven
+ensemble's member sizes, over several launches.  A barrier wait that runs out
+(forced here with a zero poll budget) must end every workgroup, be reported
+by wd_batch_status and leave the states as they were; the batch then steps
+normally.  Members the persistent step cannot run are refused.
+"""
+
+import numpy as np
+import pytest
+
+from tests.parity_util import FIELDS, field_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def _assert_bitwise(a: dict, b: dict, what: str):
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), (what, k, float(np.max(np.abs(a[k] - b[k]))))
+
+
+def _small_cases():
+    from wrf_gpu_amd import ic
+
+    return [
+        ic.bubble_3d(nx=16, ny=12, nz=8, dx=1000.0, u0=5.0),
+        ic.stratified_terrain_3d(nx=20, ny=18, nz=12, dx=3000.0, u_sigma=1.0, sigma=0.5),
+        ic.bubble_3d(nx=70, ny=19, nz=12, dx=1000.0, u0=10.0),
+        ic.stratified_terrain_3d(nx=12, ny=10, nz=8, dx=2000.0, u_sigma=0.5, sigma=0.2),
+        ic.stratified_terrain_3d(nx=9, ny=1, nz=10, dx=2000.0, u_sigma=0.5, sigma=0.2),
+    ]
+
+
+def _rotating(case, lat0):
+    """The case on a beta plane (f, e about lat0) with a projection rotation varying in x."""
+
+    import dataclasses
+
+    m = case.metrics
+    two_omega = 2.0 * 7.2921e-5
+    y = (np.arange(m.ny) + 0.5) * m.dy
+    x = (np.arange(m.nx) + 0.5) * m.dx
+    lat = np.deg2rad(lat0 + (y[:, None] - y.mean()) / 111e3 + 0.0 * x[None, :])
+    alpha = np.deg2rad(6.0) * np.sin(2.0 * np.pi * x[None, :] / (m.nx * m.dx)) + 0.0 * y[:, None]
+    m = dataclasses.replace(m, f=two_omega * np.sin(lat), e=two_omega * np.cos(lat), sina=np.sin(alpha),
+                            cosa=np.cos(alpha))
+    return case.__class__(**{**case.__dict__, "metrics": m})
+
+
+def _run_pair(cases, steps, precision="fp64"):
+    """Each case stepped alone and in one persistent ensemble; returns both host states per case."""
+
+    import torch
+    from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device, state_to_host
+
+    alone = [(Dycore(c.metrics, c.namelist, precision=precision), state_to_device(c.state)) for c in cases]
+    batched = [(Dycore(c.metrics, c.namelist, precision=precision), state_to_device(c.state)) for c in cases]
+    ens = Ensemble(batched, persistent=True)
+    assert ens.branches == 1 and ens.workgroups > 0 and ens.workgroups % 8 == 0
+    print(f"[persistent] {len(cases)} members, {precision}, {ens.workgroups} workgroups, steps {steps}", flush=True)
+    out = []
+    done = 0
+    for n in steps:
+        ens.step(n)
+        for d, s in alone:
+            d.step(s, n)
+        done += n
+        ens.status()
+        torch.cuda.synchronize()
+        out = [(state_to_host(sa), state_to_host(sb)) for (_, sa), (_, sb) in zip(alone, batched)]
+        for m, (a, b) in enumerate(out):
+            _assert_bitwise(b, a, f"member {m} ({cases[m].name}) after {done} steps")
+    ens.close()
+    for d, _ in alone + batched:
+        d.close()
+    return out
+
+
+def test_persistent_members_match_individual_steps(oracle_lib):
+    cases = _small_cases()
+    out = _run_pair(cases, (1, 2))
+    ref = oracle_lib.run(cases[1].metrics, cases[1].namelist, cases[1].state, 3)
+    errs = field_errors(out[1][1], ref)
+    for k in FIELDS:
+        assert errs[k] <= 1e-12, (k, errs[k])
+
+
+def test_persistent_mixed_members_match_individual_steps():
+    _run_pair(_small_cases()[:4], (2, 1), precision="mixed")
+
+
+def test_persistent_rotating_members_match_individual_steps():
+    cases = _small_cases()
+    cases = [_rotating(cases[1], 45.0), cases[0], _rotating(cases[3], 30.0)]
+    _run_pair(cases, (2,))
+
+
+def test_persistent_at_the_ensemble_member_sizes():
+    """One member of each of the uneven ensemble's sizes (129^2 .. 321^2 columns, 50 levels),
+    FP64 and mixed, three large steps."""
+
+    from wrf_gpu_amd import ic
+
+    cases = [ic.ensemble_member(nx, ny, 50, member=8 * q) for q, (nx, ny) in enumerate(ic.ENSEMBLE_SIZES)]
+    for precision in ("fp64", "mixed"):
+        _run_pair(cases, (3,), precision=precision)
+
+
+def test_barrier_wait_that_runs_out_is_reported_and_leaves_the_state():
+    """With a zero poll budget every workgroup but the last to arrive gives up at the first
+    barrier, and the last gives up at the second: every workgroup returns, the later launches
+    return at once, the step is reported and the states are untouched (the first two phases
+    write only work arrays); with the budget restored the same batch steps to the members'
+    results alone."""
+
+    import torch
+    from wrf_gpu_amd import _lib
+    from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device, state_to_host
+
+    cases = _small_cases()[:3]
+    alone = [(Dycore(c.metrics, c.namelist), state_to_device(c.state)) for c in cases]
+    batched = [(Dycore(c.metrics, c.namelist), state_to_device(c.state)) for c in cases]
+    ens = Ensemble(batched, persistent=True)
+    print("[persistent] forced barrier time-out", flush=True)
+    ens.set_spin_limit(0)
+    ens.step(2)
+    with pytest.raises(_lib.WrfdynError, match="grid barrier"):
+        ens.status()
+    for m, (c, (_, s)) in enumerate(zip(cases, batched)):
+        _assert_bitwise(state_to_host(s), c.state, f"member {m} after the abandoned steps")
+    ens.status()  # the flag was cleared
+    ens.set_spin_limit(1 << 22)
+    ens.step(2)
+    for d, s in alone:
+        d.step(s, 2)
+    ens.status()
+    torch.cuda.synchronize()
+    for m, ((_, sa), (_, sb)) in enumerate(zip(alone, batched)):
+        _assert_bitwise(state_to_host(sb), state_to_host(sa), f"member {m} after recovery")
+    ens.close()
+    for d, _ in alone + batched:
+        d.close()
+
+
+def test_persistent_refuses_members_it_cannot_run():
+    from wrf_gpu_amd import _lib, ic
+    from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device
+
+    a = ic.bubble_3d(nx=16, ny=12, nz=8, dx=1000.0, u0=5.0)
+    filt = a.__class__(**{**a.__dict__, "namelist": a.namelist.replace(diff_6th_opt=2)})
+    b = ic.bubble_3d(nx=12, ny=10, nz=8, dx=1000.0, u0=5.0, ns=4)
+    da, dfilt, db = (Dycore(c.metrics, c.namelist) for c in (a, filt, b))
+    sa, sfilt, sb = (state_to_device(c.state) for c in (a, filt, b))
+    with pytest.raises(_lib.WrfdynError, match="diffusion"):
+        Ensemble([(da, sa), (dfilt, sfilt)], persistent=True)
+    with pytest.raises(_lib.WrfdynError, match="substeps"):
+        Ensemble([(da, sa), (db, sb)], persistent=True)
+    dm = Dycore(a.metrics, a.namelist, precision="mixed")
+    with pytest.raises(_lib.WrfdynError, match="precision"):
+        Ensemble([(da, sa), (dm, state_to_device(a.state))], persistent=True)
+    for d in (da, dfilt, db, dm):
+        d.close()

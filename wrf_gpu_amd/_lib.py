@@ -28,7 +28,8 @@ EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error
                     "wd_set_precision", "wd_get_precision",
                     "wd_batch_create", "wd_step_batched", "wd_batch_branch", "wd_batch_branches",
                     "wd_batch_last_error", "wd_batch_destroy", "wd_step_adjoint", "wd_adjoint_tape_bytes",
-                    "wd_step_noalloc")
+                    "wd_step_noalloc", "wd_batch_create_persistent", "wd_batch_status", "wd_batch_set_spin_limit",
+                    "wd_batch_workgroups")
 
 
 class WdGrid(ctypes.Structure):
@@ -142,6 +143,15 @@ def load() -> ctypes.CDLL:
                                         ctypes.c_int, ctypes.c_void_p]
         lib.wd_batch_destroy.restype = None
         lib.wd_batch_destroy.argtypes = [ctypes.c_void_p]
+        lib.wd_batch_create_persistent.restype = ctypes.c_int
+        lib.wd_batch_create_persistent.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(WdState),
+                                                   ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        lib.wd_batch_status.restype = ctypes.c_int
+        lib.wd_batch_status.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.wd_batch_set_spin_limit.restype = ctypes.c_int
+        lib.wd_batch_set_spin_limit.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        lib.wd_batch_workgroups.restype = ctypes.c_int
+        lib.wd_batch_workgroups.argtypes = [ctypes.c_void_p]
         _lib = lib
     return _lib
 

@@ -19,7 +19,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB_PATH = os.path.join(HERE, "libwrfdyn.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ("wrfdyn_api.hip", "wrfdyn_stage.hip", "wrfdyn_acoustic.hip", "wrfdyn_adjoint.hip")
+# wrfdyn_device.hip includes wrfdyn_stage.hip, wrfdyn_acoustic.hip and wrfdyn_persistent.h
+SOURCES = ("wrfdyn_api.hip", "wrfdyn_device.hip", "wrfdyn_adjoint.hip")
 ARCH = os.environ.get("WRFDYN_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -14,13 +14,6 @@ namespace {
 __device__ __forceinline__ double wave_shr1(double old, double src) { return dpp_move<kDppWaveShr1>(old, src); }
 __device__ __forceinline__ double wave_shl1(double old, double src) { return dpp_move<kDppWaveShl1>(old, src); }
 
-__device__ inline void decompose(size_t idx, int nxx, int nyy, int& k, int& j, int& i) {
-  i = (int)(idx % (size_t)nxx);
-  size_t t = idx / (size_t)nxx;
-  j = (int)(t % (size_t)nyy);
-  k = (int)(t / (size_t)nyy);
-}
-
 // dpn face value for the acoustic PGF (core/acoustic.py:582-623): pair sums of
 // the work pressure; the rigid-lid top face uses cf1/cf2/cf3 on the top three levels.
 __device__ inline double dpn_ac(const DGrid& g, const double* __restrict__ p, int kk, size_t ca, size_t cb,
@@ -44,18 +37,19 @@ __device__ inline double dpn_ac(const DGrid& g, const double* __restrict__ p, in
 // ---------------------------------------------------------------------------
 // WET: the state carries moisture (cqu/cqv scale the pressure gradient); dry launches fold them to 1
 template <int BX, int BY, class TT, bool WET>
-__global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, double dts) {
+__device__ __forceinline__ void b_uv(const Blk& B, double* lds, DGrid g, DState s, DWork w,
+                                     double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   // waves overlap by one lane: lanes 1..BX-1 own faces, lane 0 holds the
   // column west of the first owned face
   const unsigned nbx = (nx + 1 + BX - 2) / (BX - 1);
-  const unsigned t = tile_id(g);
-  const int i = (int)(t % nbx) * (BX - 1) + (int)threadIdx.x - 1;
-  const int j = (int)(t / nbx) * BY + threadIdx.y;
+  const unsigned t = tile_id(g, B);
+  const int i = (int)(t % nbx) * (BX - 1) + (int)B.tx - 1;
+  const int j = (int)(t / nbx) * BY + B.ty;
   // threads past the face grid, and lane 0, stay resident for the per-level
   // barrier; their mass-column indices are clamped and their face work is
   // switched off
-  const bool active = threadIdx.x >= 1 && i <= nx && j <= ny;
+  const bool active = B.tx >= 1 && i <= nx && j <= ny;
   const bool do_u = active && j < ny;
   const bool do_v = active && i < nx;
   const size_t plane = (size_t)ny * nx;
@@ -109,11 +103,11 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
   };
   const unsigned pl8 = (unsigned)plane * 8u;
   const unsigned cc8 = (unsigned)cc * 8u, cs8 = (unsigned)cs * 8u;
-  // a wave is one row of BX faces (blockDim.x == BX == wavefront size)
+  // a wave is one row of BX faces (the block is BX wide, one wavefront)
   static_assert(BX == 64, "k_uv maps one wavefront to one row of faces");
   // row j - 1 is in this block unless this is its first row or j = 0 (whose
   // south column is clamped to row 0 itself)
-  const bool row_below_outside = threadIdx.y == 0 || j == 0;
+  const bool row_below_outside = B.ty == 0 || j == 0;
   // face offsets (used only where do_u / do_v hold, i.e. for in-range faces)
   const int ifc = clampi(i, 0, nx), jfc = clampi(j, 0, ny);
   const unsigned uq8 = (unsigned)iu(g, 0, jfc, ifc) * 8u, upl8 = (unsigned)((size_t)ny * (nx + 1)) * 8u;
@@ -153,9 +147,9 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
   // The south column of row j is the centre column of row j - 1, which the
   // wave below holds (rows j and j - 1 of one block): each level's centre
   // values go through LDS, double-buffered, with one barrier per level.
-  __shared__ double sh[2][BY][7][BX];
+  double (*sh)[BY][7][BX] = reinterpret_cast<double (*)[BY][7][BX]>(lds);
   auto south = [&](int k, Lv& L) {
-    double* mine = &sh[k & 1][threadIdx.y][0][threadIdx.x];
+    double* mine = &sh[k & 1][B.ty][0][B.tx];
     mine[0 * BX] = L.phc1;
     mine[1 * BX] = L.altc;
     mine[2 * BX] = L.alc;
@@ -165,7 +159,7 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     mine[6 * BX] = L.p2c;
     __syncthreads();
     if (!row_below_outside) {
-      const double* below = &sh[k & 1][threadIdx.y - 1][0][threadIdx.x];
+      const double* below = &sh[k & 1][B.ty - 1][0][B.tx];
       L.phs1 = below[0 * BX];
       L.alts = below[1 * BX];
       L.als = below[2 * BX];
@@ -254,6 +248,12 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
     }
     cur = nxt;
   }
+}
+
+template <int BX, int BY, class TT, bool WET>
+__global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, double dts) {
+  __shared__ double sh[2][BY][7][BX];
+  b_uv<BX, BY, TT, WET>(hw_blk(), &sh[0][0][0][0], g, s, w, dts);
 }
 
 // ---------------------------------------------------------------------------
@@ -757,13 +757,13 @@ __global__ void __launch_bounds__(TX * KT, (R <= 4 ? 3 : 1))
 __device__ inline double safe_den(double ms) { return fabs(ms) > 1.0e-12 ? ms : (ms >= 0.0 ? 1.0e-12 : -1.0e-12); }
 
 // One thread per face column (u faces first, then v faces) and a chunk of
-// kFinChunk levels (blockIdx.y); the 2-D factors are loaded once per chunk.
+// kFinChunk levels (the block's y index); the 2-D factors are loaded once per chunk.
 constexpr int kFinChunk = 8;
 
-__device__ inline void fin_face_column(const DGrid& g, unsigned f, unsigned fplane8, double mus, double mu,
-                                       double msf, const double* __restrict__ xw, double* __restrict__ x) {
+__device__ inline void fin_face_column(const DGrid& g, unsigned chunk, unsigned f, unsigned fplane8, double mus,
+                                       double mu, double msf, const double* __restrict__ xw, double* __restrict__ x) {
   const unsigned f8 = f * 8u;
-  const int k0 = blockIdx.y * kFinChunk, k1 = k0 + kFinChunk < g.nz ? k0 + kFinChunk : g.nz;
+  const int k0 = chunk * kFinChunk, k1 = k0 + kFinChunk < g.nz ? k0 + kFinChunk : g.nz;
 #pragma unroll 4
   for (int k = k0; k < k1; ++k) {
     const unsigned q = (unsigned)k * fplane8 + f8;
@@ -773,14 +773,18 @@ __device__ inline void fin_face_column(const DGrid& g, unsigned f, unsigned fpla
   }
 }
 
-__global__ void __launch_bounds__(256) k_fin_uv(DGrid g, DState s, DWork w) {
+__device__ __forceinline__ void b_fin_uv(const Blk& B, DGrid g, DState s, DWork w) {
   const unsigned ncu = (unsigned)(g.ny * (g.nx + 1)), ncv = (unsigned)((g.ny + 1) * g.nx);
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ncu) fin_face_column(g, t, ncu * 8u, w.muus[t], w.muu[t], g.msfuy[t], w.u_work, s.u);
+  const unsigned t = B.x * B.bdx + B.tx;
+  if (t < ncu) fin_face_column(g, B.y, t, ncu * 8u, w.muus[t], w.muu[t], g.msfuy[t], w.u_work, s.u);
   else if (t < ncu + ncv) {
     const unsigned f = t - ncu;
-    fin_face_column(g, f, ncv * 8u, w.muvs[f], w.muv[f], g.msfvx[f], w.v_work, s.v);
+    fin_face_column(g, B.y, f, ncv * 8u, w.muvs[f], w.muv[f], g.msfvx[f], w.v_work, s.v);
   }
+}
+
+__global__ void __launch_bounds__(256) k_fin_uv(DGrid g, DState s, DWork w) {
+  b_fin_uv(hw_blk(), g, s, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -807,12 +811,12 @@ constexpr int kMuRows = 4;
 constexpr int kMuOwned = 62;  // columns owned per wave
 
 template <class TT>
-__global__ void __launch_bounds__(64 * kMuRows) k_column_mu(DGrid g, DNamelist nl, DState s, DWork w, double dts,
-                                                           int first) {
+__device__ __forceinline__ void b_column_mu(const Blk& B, double* lds, DGrid g, DNamelist nl,
+                                            DState s, DWork w, double dts, int first) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const unsigned nbx = (unsigned)(nx + kMuOwned - 1) / kMuOwned;
-  const unsigned t = tile_id(g);
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  const unsigned t = tile_id(g, B);
+  const int tx = B.tx, ty = B.ty;
   const int i = (int)(t % nbx) * kMuOwned + tx - 1;  // -1 and >= nx on the halo lanes
   const int j = (int)(t / nbx) * kMuRows + ty;
   const bool owned = tx >= 1 && tx <= kMuOwned && i >= 0 && i < nx && j < ny;
@@ -885,7 +889,7 @@ __global__ void __launch_bounds__(64 * kMuRows) k_column_mu(DGrid g, DNamelist n
     double vwn, vws, uwe, uww, dv;
     double thc, thn, ths, the, thw, thwk, tht, t2o;
   };
-  __shared__ double sh[2][kMuRows][2][64];
+  double (*sh)[kMuRows][2][64] = reinterpret_cast<double (*)[kMuRows][2][64]>(lds);
   Mv m_1{};                           // level f-1
   double thc_2 = 0.0;                 // theta at f-2
   double b_1 = 0.0;                   // omega at f-1
@@ -969,17 +973,24 @@ __global__ void __launch_bounds__(64 * kMuRows) k_column_mu(DGrid g, DNamelist n
   }
 }
 
+template <class TT>
+__global__ void __launch_bounds__(64 * kMuRows) k_column_mu(DGrid g, DNamelist nl, DState s, DWork w, double dts,
+                                                           int first) {
+  __shared__ double sh[2][kMuRows][2][64];
+  b_column_mu<TT>(hw_blk(), &sh[0][0][0][0], g, nl, s, w, dts, first);
+}
+
 // k_column_w (advance_w + calc_p_rho): sweep up for the phi-equation rhs at
 // f-1 (stored to w.rhs) and the explicit w and Thomas forward sweep at f-2
 // (stored to w.wfwd), reading omega and the new t_2ave from k_column_mu; then
 // sweep down for the back substitution, damping, the phi finish and
 // calc_p_rho (which needs the finished phi of levels k and k+1).
 template <class TT>
-__global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState s, DWork w, double dts,
-                                                   int first) {
+__device__ __forceinline__ void b_column_w(const Blk& B, DGrid g, DNamelist nl, DState s,
+                                           DWork w, double dts, int first) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t ncol = (size_t)ny * nx;
-  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  const size_t c = (size_t)tile_id(g, B) * B.bdx + B.tx;
   if (c >= ncol) return;
   const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
   const unsigned plane = (unsigned)ncol * 8u;
@@ -1197,15 +1208,21 @@ __global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState 
   }
 }
 
+template <class TT>
+__global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState s, DWork w, double dts,
+                                                   int first) {
+  b_column_w<TT>(hw_blk(), g, nl, s, w, dts, first);
+}
+
 // small_step_finish for w, ph, theta, p and mu (core/small_step_finish.py:22-101)
 // followed by _refresh_grid_p_from_finished (operational_mode.py:2169-2231,
 // diagnose_pressure_al_alt's base branch, acoustic_wrf.py:360-438).  One
 // thread per column marches up the w levels: the finished ph of levels f-1
 // and f stay in registers for mass level f-1, and the next level's loads are
 // issued before the current level is computed.
-__global__ void __launch_bounds__(256) k_fin_col(DGrid g, DState s, DWork w) {
+__device__ __forceinline__ void b_fin_col(const Blk& B, DGrid g, DState s, DWork w) {
   const size_t ncol = (size_t)g.ny * g.nx;
-  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  const size_t c = (size_t)tile_id(g, B) * B.bdx + B.tx;
   if (c >= ncol) return;
   const int nz = g.nz;
   const unsigned pl8 = (unsigned)ncol * 8u, c8 = (unsigned)c * 8u;
@@ -1299,7 +1316,20 @@ __global__ void __launch_bounds__(256) k_fin_col(DGrid g, DState s, DWork w) {
   }
 }
 
-inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+__global__ void __launch_bounds__(256) k_fin_col(DGrid g, DState s, DWork w) {
+  b_fin_col(hw_blk(), g, s, w);
+}
+
+// Launch grids of the acoustic kernels, which the persistent ensemble step
+// gives its virtual blocks too (64 x 4 tiles, or 256-thread blocks).
+constexpr int kUvBx = 64, kUvBy = 4;  // k_uv's block: one wave per row of faces
+inline unsigned uv_tiles(const DGrid& g) {
+  return (unsigned)((g.nx + 1 + kUvBx - 2) / (kUvBx - 1)) * (unsigned)((g.ny + 1 + kUvBy - 1) / kUvBy);
+}
+inline unsigned column_mu_tiles(const DGrid& g) {
+  return (unsigned)((g.nx + kMuOwned - 1) / kMuOwned) * (unsigned)((g.ny + kMuRows - 1) / kMuRows);
+}
+inline unsigned fin_chunks(const DGrid& g) { return (unsigned)((g.nz + kFinChunk - 1) / kFinChunk); }
 
 template <int TX, int KT, int R>
 hipError_t launch_column(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w, double dts,
@@ -1340,7 +1370,7 @@ hipError_t launch_column_tile(int tile, const DGrid& g, const DNamelist& nl, con
   if (tile == kColumnMarch) {
     const unsigned bs = 256;
     const size_t C = (size_t)g.ny * g.nx;
-    const unsigned mu_tiles = (unsigned)((g.nx + kMuOwned - 1) / kMuOwned) * (unsigned)((g.ny + kMuRows - 1) / kMuRows);
+    const unsigned mu_tiles = column_mu_tiles(g);
     const dim3 mu_block(64, kMuRows);
     if (nl.mixed) {
       WD_TIMED(prof, KID_COLUMN_MU, st, (k_column_mu<float><<<mu_tiles, mu_block, 0, st>>>(g, nl, s, w, dts, first)));
@@ -1365,8 +1395,8 @@ hipError_t launch_column_tile(int tile, const DGrid& g, const DNamelist& nl, con
 
 hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                                    double dts, int first, int column_tile, hipStream_t st, Prof* prof) {
-  constexpr int BX = 64, BY = 4;
-  const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 2) / (BX - 1)) * (unsigned)((g.ny + 1 + BY - 1) / BY);
+  constexpr int BX = kUvBx, BY = kUvBy;
+  const unsigned uvgrid = uv_tiles(g);
   if (nl.mixed) {
     WD_TIMED(prof, KID_UV, st, (k_uv<BX, BY, float, false><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts)));
   } else if (w.cqu) {
@@ -1381,15 +1411,14 @@ hipError_t launch_acoustic_substep(const DGrid& g, const DNamelist& nl, const DS
 hipError_t launch_substep_part(int part, const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                                double dts, int first, hipStream_t st) {
   if (nl.mixed) return hipErrorInvalidValue;
-  constexpr int BX = 64, BY = 4;
+  constexpr int BX = kUvBx, BY = kUvBy;
   const unsigned bs = 256;
   const size_t C = (size_t)g.ny * g.nx;
   if (part == 0) {
-    const unsigned uvgrid = (unsigned)((g.nx + 1 + BX - 2) / (BX - 1)) * (unsigned)((g.ny + 1 + BY - 1) / BY);
+    const unsigned uvgrid = uv_tiles(g);
     k_uv<BX, BY, double, false><<<uvgrid, dim3(BX, BY), 0, st>>>(g, s, w, dts);
   } else if (part == 1) {
-    const unsigned mu_tiles = (unsigned)((g.nx + kMuOwned - 1) / kMuOwned) * (unsigned)((g.ny + kMuRows - 1) / kMuRows);
-    k_column_mu<double><<<mu_tiles, dim3(64, kMuRows), 0, st>>>(g, nl, s, w, dts, first);
+    k_column_mu<double><<<column_mu_tiles(g), dim3(64, kMuRows), 0, st>>>(g, nl, s, w, dts, first);
   } else if (part == 2) {
     k_column_w<double><<<blocks_for(C, bs), bs, 0, st>>>(g, nl, s, w, dts, first);
   } else {
@@ -1402,7 +1431,7 @@ hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, 
   const unsigned bs = 256;
   const size_t UVcols = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;
   const size_t C = (size_t)g.ny * g.nx;
-  const unsigned kchunks = (unsigned)((g.nz + kFinChunk - 1) / kFinChunk);
+  const unsigned kchunks = fin_chunks(g);
   WD_TIMED(prof, KID_FIN_UV, st, (k_fin_uv<<<dim3(blocks_for(UVcols, bs), kchunks), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_FIN_COL, st, (k_fin_col<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   return hipGetLastError();

@@ -59,6 +59,13 @@ struct wd_batch {
   hipGraphExec_t graph = nullptr;
   int device = 0;
   std::string err;
+  // wd_batch_create_persistent: the graph holds one launch of the persistent
+  // step over `grid` workgroups; its member table, program and grid barrier
+  // share one device allocation
+  bool persistent = false;
+  unsigned grid = 0;
+  void* pbuf = nullptr;
+  wd::PBarrier* bar = nullptr;
 };
 
 namespace {
@@ -862,6 +869,144 @@ int wd_batch_create(wd_handle* const* members, const wd_state* states, int n_mem
   return WD_OK;
 }
 
+// A grid barrier wait of the persistent step gives up after this many polls
+// (a poll is one device-scope load and a short sleep, about a microsecond),
+// seconds beyond the longest phase of any ensemble that fits on the GPU.
+constexpr unsigned kDefaultSpinLimit = 1u << 22;
+
+int wd_batch_create_persistent(wd_handle* const* members, const wd_state* states, int n_members, wd_batch** out) {
+  if (!out) return fail(nullptr, WD_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!members || !states || n_members < 1) return fail(nullptr, WD_EINVAL, "need at least one member");
+  for (int m = 0; m < n_members; ++m) {
+    if (!members[m]) return fail(nullptr, WD_EINVAL, "a member handle is NULL");
+    for (int q = 0; q < m; ++q)
+      if (members[q] == members[m]) return fail(nullptr, WD_EINVAL, "a handle appears twice (members share no workspace)");
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDevice");
+  for (int m = 0; m < n_members; ++m)
+    if (members[m]->device != dev) return fail(nullptr, WD_EINVAL, "members must live on the current device");
+  std::vector<wd::PMember> pm(n_members);
+  for (int m = 0; m < n_members; ++m) {
+    wd::DState ds;
+    int rc = check_state(nullptr, &states[m], &ds, members[m]->nl.moist);
+    if (rc != WD_OK) return rc;
+    pm[m] = wd::PMember{members[m]->g, members[m]->nl, ds, members[m]->w};
+  }
+  std::vector<wd::PSeg> segs;
+  std::vector<wd::PPhase> phases;
+  std::string why;
+  if (!wd::persistent_program(pm, &segs, &phases, &why)) return fail(nullptr, WD_EINVAL, why);
+  const bool mixed = pm[0].nl.mixed != 0;
+  // as many workgroups as the GPU holds at once (the grid barrier needs every
+  // one resident), a multiple of the 8 XCDs
+  int per_cu = 0, cus = 0;
+  e = wd::persistent_blocks_per_cu(mixed, &per_cu);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "occupancy of the persistent step");
+  const unsigned grid = (unsigned)(per_cu * cus) / wd::kNumXcd * wd::kNumXcd;
+  if (grid == 0) return fail(nullptr, WD_EDEVICE, "the persistent step does not fit on this GPU");
+  wd_batch* b = new (std::nothrow) wd_batch();
+  if (!b) return fail(nullptr, WD_ENOMEM, "host allocation failed");
+  b->device = dev;
+  b->persistent = true;
+  b->grid = grid;
+  b->members.assign(members, members + n_members);
+  b->states.resize(n_members);
+  for (int m = 0; m < n_members; ++m) b->states[m] = pm[m].s;
+  b->branches = 1;
+  b->branch_of.assign(n_members, 0);
+  // one allocation: the barrier, the members, the segments, the phases
+  const wd::PBarrier bar0{0u, 0u, 0u, kDefaultSpinLimit};
+  const size_t o_mem = 256, o_seg = o_mem + slot_bytes((sizeof(wd::PMember) * pm.size() + 7) / 8),
+               o_ph = o_seg + slot_bytes((sizeof(wd::PSeg) * segs.size() + 7) / 8),
+               total = o_ph + slot_bytes((sizeof(wd::PPhase) * phases.size() + 7) / 8);
+  e = hipMalloc(&b->pbuf, total);
+  char* base = static_cast<char*>(b->pbuf);
+  if (e == hipSuccess) e = hipMemcpy(base, &bar0, sizeof(bar0), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + o_mem, pm.data(), sizeof(wd::PMember) * pm.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + o_seg, segs.data(), sizeof(wd::PSeg) * segs.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(base + o_ph, phases.data(), sizeof(wd::PPhase) * phases.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    wd_batch_destroy(b);
+    return hip_fail(nullptr, e, "persistent step tables");
+  }
+  b->bar = reinterpret_cast<wd::PBarrier*>(base);
+  // capture the one launch per large step
+  b->streams.assign(1, nullptr);
+  e = hipStreamCreateWithFlags(&b->streams[0], hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    wd_batch_destroy(b);
+    return hip_fail(nullptr, e, "batch stream creation");
+  }
+  hipStream_t s0 = b->streams[0];
+  e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    wd_batch_destroy(b);
+    return hip_fail(nullptr, e, "hipStreamBeginCapture");
+  }
+  e = wd::launch_persistent_step(mixed, grid, reinterpret_cast<const wd::PMember*>(base + o_mem),
+                                 reinterpret_cast<const wd::PSeg*>(base + o_seg),
+                                 reinterpret_cast<const wd::PPhase*>(base + o_ph), (int)phases.size(), b->bar, s0);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s0, &graph);
+  if (e == hipSuccess) e = ec;
+  if (e == hipSuccess) e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+  if (graph) hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    wd_batch_destroy(b);
+    return hip_fail(nullptr, e, "capturing the persistent step");
+  }
+  *out = b;
+  return WD_OK;
+}
+
+int wd_batch_status(wd_batch* b, void* stream) {
+  if (!b) return fail(nullptr, WD_EINVAL, "batch is NULL");
+  hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) {
+    b->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+    return WD_EDEVICE;
+  }
+  if (!b->persistent) return WD_OK;
+  wd::PBarrier bar{};
+  e = hipMemcpy(&bar, b->bar, sizeof(bar), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    b->err = std::string("reading the grid barrier: ") + hipGetErrorString(e);
+    return WD_EDEVICE;
+  }
+  if (!bar.abort) return WD_OK;
+  // nothing runs now: clear the flag and the arrivals for the next launches
+  bar.count = 0;
+  bar.abort = 0;
+  e = hipMemcpy(b->bar, &bar, sizeof(bar), hipMemcpyHostToDevice);
+  b->err = "a grid barrier wait of the persistent step ran out (not every workgroup was resident at once, "
+           "e.g. beside other work on the GPU): the large steps since the last check did not complete";
+  if (e != hipSuccess) b->err += std::string("; clearing the flag: ") + hipGetErrorString(e);
+  return WD_EDEVICE;
+}
+
+int wd_batch_set_spin_limit(wd_batch* b, unsigned polls) {
+  if (!b) return fail(nullptr, WD_EINVAL, "batch is NULL");
+  if (!b->persistent) {
+    b->err = "only a persistent batch has a grid barrier";
+    return WD_EINVAL;
+  }
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(&b->bar->spin_limit, &polls, sizeof(polls), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    b->err = std::string("setting the spin limit: ") + hipGetErrorString(e);
+    return WD_EDEVICE;
+  }
+  return WD_OK;
+}
+
+int wd_batch_workgroups(const wd_batch* b) { return b ? (int)b->grid : WD_EINVAL; }
+
 int wd_step_batched(wd_batch* b, int n_large_steps, void* stream) {
   if (!b) return fail(nullptr, WD_EINVAL, "batch is NULL");
   if (n_large_steps < 0) {
@@ -898,6 +1043,7 @@ void wd_batch_destroy(wd_batch* b) {
   if (hipGetDevice(&cur) == hipSuccess && cur != b->device) hipSetDevice(b->device);
   (void)hipDeviceSynchronize();
   if (b->graph) hipGraphExecDestroy(b->graph);
+  if (b->pbuf) hipFree(b->pbuf);
   for (hipEvent_t x : b->events)
     if (x) hipEventDestroy(x);
   for (hipStream_t x : b->streams)

@@ -148,9 +148,24 @@ __device__ inline unsigned xcd_tile(unsigned b, unsigned n) {
   const unsigned x = b % kNumXcd, s = b / kNumXcd, q = n / kNumXcd, r = n % kNumXcd;
   return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
 }
-__device__ inline unsigned tile_id(const DGrid& g) {
-  return g.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+// The block a kernel body runs as: its place in the launch grid (x, y, and
+// the grid's width gx), the thread's place in the block (tx, ty) and the
+// block's width bdx.  A __global__ kernel hands its body the hardware's
+// (hw_blk); the persistent ensemble step hands it the virtual block it gave
+// to one of its resident workgroups, so the same body serves both.
+struct Blk {
+  unsigned x, y, gx;
+  unsigned tx, ty, bdx;
+};
+__device__ __forceinline__ Blk hw_blk() {
+  return Blk{blockIdx.x, blockIdx.y, gridDim.x, threadIdx.x, threadIdx.y, blockDim.x};
 }
+__device__ inline unsigned tile_id(const DGrid& g, const Blk& b) {
+  return g.xcd_remap ? xcd_tile(b.x, b.gx) : b.x;
+}
+__device__ inline unsigned tile_id(const DGrid& g) { return tile_id(g, hw_blk()); }
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Loads and stores at a 32-bit byte offset from a wave-uniform base pointer.
 // The compiler then addresses with the base in SGPRs plus one 32-bit VGPR

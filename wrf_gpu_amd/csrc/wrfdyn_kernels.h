@@ -1,7 +1,8 @@
-// Launchers for the wrfdyn kernels (one translation unit per phase) and the
+// Launchers for the wrfdyn kernels and the
 // optional per-kernel event timing used for benchmarking.
 #pragma once
 
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -128,6 +129,49 @@ hipError_t launch_stage_finish(const DGrid& g, const DState& s, const DWork& w, 
 // + calc_p_rho (k_column_w).  The three in order are launch_acoustic_substep.
 hipError_t launch_substep_part(int part, const DGrid& g, const DNamelist& nl, const DState& s, const DWork& w,
                                double dts, int first, hipStream_t st);
+
+// ---- The persistent ensemble step (wrfdyn_persistent.h) ----
+// One large step of every member of an ensemble from a single launch: a
+// program of phases, each a set of kernel bodies that read nothing another
+// body of the phase writes, run for every member, with a grid barrier
+// between phases.  A segment is one kernel body for one member; its virtual
+// blocks are the blocks of that kernel's standalone launch.
+struct PMember {
+  DGrid g;
+  DNamelist nl;
+  DState s;
+  DWork w;
+};
+struct PSeg {
+  int kind;      // PKind
+  int member;
+  int rk_step;   // k_prep_2d
+  int first;     // the column kernels: a stage's first substep
+  double dts;    // the stage's acoustic step
+  unsigned vb0;  // the segment's first virtual block in its phase (a multiple of 8)
+  unsigned nvb;  // its virtual blocks (256-thread workgroups)
+  unsigned gx, gy;  // the standalone launch's grid
+  unsigned aux;     // k_coriolis: its u-face block count
+};
+struct PPhase {
+  unsigned seg0, nseg;  // segments [seg0, seg0 + nseg), sorted by vb0
+  unsigned total;       // virtual blocks of the phase
+};
+// The grid barrier's state (device memory, zeroed at creation): arrivals,
+// the generation every arrival waits to see change, the abort flag a timed
+// out wait raises, and the bound on a wait in polls.
+struct PBarrier {
+  unsigned count, gen, abort, spin_limit;
+};
+// The program of one large step of the members (all of one precision form
+// and one substep count; no optional term but rotation).  Returns false with
+// a message for a member it cannot run.
+bool persistent_program(const std::vector<PMember>& members, std::vector<PSeg>* segs,
+                        std::vector<PPhase>* phases, std::string* why);
+// Workgroups of the persistent kernel one CU holds at once.
+hipError_t persistent_blocks_per_cu(bool mixed, int* out);
+hipError_t launch_persistent_step(bool mixed, unsigned grid, const PMember* members, const PSeg* segs,
+                                  const PPhase* phases, int n_phases, PBarrier* bar, hipStream_t st);
 
 // Adjoint of one plain-FP64 large step (wrfdyn_adjoint.hip).  The tape is
 // allocated on first use and kept by the caller; the forward recompute runs

@@ -10,27 +10,20 @@ namespace wd {
 
 namespace {
 
-__device__ inline void decompose(size_t idx, int nxx, int nyy, int& k, int& j, int& i) {
-  i = (int)(idx % (size_t)nxx);
-  size_t t = idx / (size_t)nxx;
-  j = (int)(t % (size_t)nyy);
-  k = (int)(t / (size_t)nyy);
-}
-
 // ---------------------------------------------------------------------------
 // couple_velocities_periodic ru/rv, dynamics/flux_advection.py:394-428
 // ---------------------------------------------------------------------------
 // The element-wise kernels below give each thread one column (mass, u-face
-// or v-face) and a chunk of kChunk levels (blockIdx.y): the 2-D factors are
+// or v-face) and a chunk of kChunk levels (the block's y index): the 2-D factors are
 // loaded once per chunk, the 3-D index is k * plane + column, and a wave's
 // loads cover 64 consecutive elements of one level.
 constexpr int kChunk = 8;
 
-__global__ void __launch_bounds__(256) k_ruv(DGrid g, const double* __restrict__ u, const double* __restrict__ v,
-                                             const double* __restrict__ mu, double* __restrict__ ru,
-                                             double* __restrict__ rv) {
+__device__ __forceinline__ void b_ruv(const Blk& B, DGrid g, const double* __restrict__ u,
+                                      const double* __restrict__ v, const double* __restrict__ mu,
+                                      double* __restrict__ ru, double* __restrict__ rv) {
   const unsigned ncol = (unsigned)(g.ny * g.nx);
-  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned c = B.x * B.bdx + B.tx;
   if (c >= ncol) return;
   const int j = (int)(c / (unsigned)g.nx), i = (int)(c - (unsigned)j * (unsigned)g.nx);
   const double muu = 0.5 * (mu[c] + mu[i2(g, j, wrapi(i - 1, g.nx))]);
@@ -38,7 +31,7 @@ __global__ void __launch_bounds__(256) k_ruv(DGrid g, const double* __restrict__
   const double msfuy = g.msfuy[i2u(g, j, i)], msfvx = g.msfvx[i2v(g, j, i)];
   const unsigned pl8 = ncol * 8u, upl8 = (unsigned)(g.ny * (g.nx + 1)) * 8u, vpl8 = (unsigned)((g.ny + 1) * g.nx) * 8u;
   const unsigned c8 = c * 8u, cu8 = (unsigned)i2u(g, j, i) * 8u;
-  const int k0 = blockIdx.y * kChunk, k1 = k0 + kChunk < g.nz ? k0 + kChunk : g.nz;
+  const int k0 = B.y * kChunk, k1 = k0 + kChunk < g.nz ? k0 + kChunk : g.nz;
 #pragma unroll 4
   for (int k = k0; k < k1; ++k) {
     const double mass_u = g.c1h[k] * muu + g.c2h[k];
@@ -48,12 +41,18 @@ __global__ void __launch_bounds__(256) k_ruv(DGrid g, const double* __restrict__
   }
 }
 
+__global__ void __launch_bounds__(256) k_ruv(DGrid g, const double* __restrict__ u, const double* __restrict__ v,
+                                             const double* __restrict__ mu, double* __restrict__ ru,
+                                             double* __restrict__ rv) {
+  b_ruv(hw_blk(), g, u, v, mu, ru, rv);
+}
+
 // calc_ww_cp omega, dynamics/flux_advection.py:430-458 (one thread per column)
-__global__ void k_rom(DGrid g, const double* __restrict__ ru, const double* __restrict__ rv,
-                      double* __restrict__ rom) {
-  const unsigned nbx = (g.nx + blockDim.x - 1) / blockDim.x;
-  const unsigned t = tile_id(g);
-  int i = (int)(t % nbx * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void b_rom(const Blk& B, DGrid g, const double* __restrict__ ru,
+                                      const double* __restrict__ rv, double* __restrict__ rom) {
+  const unsigned nbx = (g.nx + B.bdx - 1) / B.bdx;
+  const unsigned t = tile_id(g, B);
+  int i = (int)(t % nbx * B.bdx + B.tx);
   int j = (int)(t / nbx);
   if (i >= g.nx) return;
   int ip = wrapi(i + 1, g.nx), jp = wrapi(j + 1, g.ny);
@@ -74,6 +73,11 @@ __global__ void k_rom(DGrid g, const double* __restrict__ ru, const double* __re
     if (k + 1 < g.nz) rom[im(g, k + 1, j, i)] = cum;
   }
   rom[im(g, g.nz, j, i)] = 0.0;
+}
+
+__global__ void k_rom(DGrid g, const double* __restrict__ ru, const double* __restrict__ rv,
+                      double* __restrict__ rom) {
+  b_rom(hw_blk(), g, ru, rv, rom);
 }
 
 // _vertical_flux_div_3 face flux (flux_advection.py:1362-1407) from a column
@@ -137,14 +141,14 @@ constexpr int kTendURows = 4;
 // rv at j+1 from the adjacent waves (LDS); the +-2, +-3 columns and rows
 // outside the block (or wrapping periodically) are loaded.
 template <class TT>
-__global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DWork w, const double* __restrict__ th,
-                                                             double qoff, double* dst, const double* d6,
-                                                             const double* sm) {
+__device__ __forceinline__ void b_tend_t(const Blk& B, double* lds, DGrid g, DWork w,
+                                         const double* __restrict__ th, double qoff, double* dst, const double* d6,
+                                         const double* sm) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const unsigned nbx = (unsigned)(nx + 61) / 62;
-  const unsigned tile = tile_id(g);
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  const unsigned tile = tile_id(g, B);
+  const int tx = B.tx, ty = B.ty;
   const int i = (int)(tile % nbx) * 62 + tx - 1;  // outside 0..nx-1 on the halo lanes
   const int j = (int)(tile / nbx) * kTendURows + ty;
   const bool owned = tx >= 1 && tx <= 62 && i < nx && j < ny;
@@ -161,7 +165,8 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DWork w, co
     lds_y[d] = r >= 0 && r < kTendURows && jj >= 0 && jj < ny;
   }
   const double msftx = g.msftx[c8 / 8u];
-  __shared__ double sh[2][kTendURows][2][64];  // the scalar and rv at level k
+  // the scalar and rv at level k
+  double (*sh)[kTendURows][2][64] = reinterpret_cast<double (*)[kTendURows][2][64]>(lds);
   struct Lv {
     double x[7], y[7];  // theta at columns i-3..i+3 and rows j-3..j+3 (x[3], y[3]: centre)
     double ru0, ru1, rv0, rv1;
@@ -174,7 +179,9 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DWork w, co
   double rom0 = 0.0;   // mixed form: rom at face k, and its face interpolant (deviation from level k-1)
   float fdev_lo = 0.0f;
   Lv cur{};
-  for (int k = 0; k < nz; ++k) {
+  // One mass level.  Level 0 is peeled off the loop, which folds the
+  // surface-face branch out of it (measured faster).
+  auto level = [&](int k) __attribute__((always_inline)) {
     const unsigned o = (unsigned)k * pl8;
     const double tp3 = k + 3 < nz ? ldo(th, (unsigned)(k + 3) * pl8 + c8) : 0.0;
     const double rom2 = k + 2 <= nz ? ldo(w.rom, (unsigned)(k + 2) * pl8 + c8) : 0.0;
@@ -237,18 +244,29 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DWork w, co
     tp1 = tp2;
     tp2 = tp3;
     rom1 = rom2;
-  }
+  };
+  level(0);
+  for (int k = 1; k < nz; ++k) level(k);
+
+}
+
+template <class TT>
+__global__ void __launch_bounds__(64 * kTendURows) k_tend_t(DGrid g, DWork w, const double* __restrict__ th,
+                                                             double qoff, double* dst, const double* d6,
+                                                             const double* sm) {
+  __shared__ double sh[2][kTendURows][2][64];
+  b_tend_t<TT>(hw_blk(), &sh[0][0][0][0], g, w, th, qoff, dst, d6, sm);
 }
 
 // advect_w_flux + _vertical_flux_div_w with a rigid lid
 // (flux_advection.py:1277-1329, 1410-1482) on w levels (nz+1, ny, nx); the
 // transport is _mass_to_full_levels of ru/rv (flux_advection.py:1332-1348).
 template <class TT>
-__global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
+__device__ __forceinline__ void b_tend_w(const Blk& B, DGrid g, DState s, DWork w) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t ncol = (size_t)ny * nx;
-  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  const size_t c = (size_t)tile_id(g, B) * B.bdx + B.tx;
   if (c >= ncol) return;
   const int j = (int)(c / (size_t)nx), i = (int)(c % (size_t)nx);
   const unsigned pl8 = (unsigned)ncol * 8u;
@@ -292,7 +310,9 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
   double ru0m = 0.0, ru1m = 0.0, rv0m = 0.0, rv1m = 0.0;  // ru/rv at mass level f-1
   Lv cur{};
   load(0, cur);
-  for (int f = 0; f <= nz; ++f) {
+  // One w level.  Levels 0 and 1 are peeled off the loop, which folds the
+  // surface-face and first-face branches out of it (measured faster).
+  auto level = [&](int f) __attribute__((always_inline)) {
     const double wp3 = f + 3 <= nz ? ldo(W, (unsigned)(f + 3) * pl8 + c8) : 0.0;
     const double rom2 = f + 2 <= nz ? ldo(w.rom, (unsigned)(f + 2) * pl8 + c8) : 0.0;
     Lv nxt{};
@@ -362,7 +382,16 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
       rv1m = cur.rv1;
     }
     cur = nxt;
-  }
+  };
+  level(0);
+  level(1);
+  for (int f = 2; f <= nz; ++f) level(f);
+
+}
+
+template <class TT>
+__global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
+  b_tend_w<TT>(hw_blk(), g, s, w);
 }
 
 // advect_u_flux (flux_advection.py:1131-1202) + large_step_horizontal_pgf x
@@ -370,7 +399,7 @@ __global__ void __launch_bounds__(256) k_tend_w(DGrid g, DState s, DWork w) {
 // repeats face 0's advection (periodic); the PGF uses edge-padded pairs.
 // WET: the moisture factor cqu scales the PGF; dry launches fold it to 1.
 template <class TT, bool WET>
-__global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, DWork w) {
+__device__ __forceinline__ void b_tend_u(const Blk& B, double* lds, DGrid g, DState s, DWork w) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   // Blocks of kTendURows rows of one 64-lane wave each; waves overlap by one
@@ -381,8 +410,8 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
   // from the adjacent waves (LDS); the +-2, +-3 faces and rows outside the
   // block (or wrapping periodically) are loaded.
   const unsigned nbx = (unsigned)(nx + 1 + 61) / 62;
-  const unsigned tile = tile_id(g);
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  const unsigned tile = tile_id(g, B);
+  const int tx = B.tx, ty = B.ty;
   const int i = (int)(tile % nbx) * 62 + tx - 1;  // -1 and > nx on the halo lanes
   const int j = (int)(tile / nbx) * kTendURows + ty;
   const bool owned = tx >= 1 && tx <= 62 && i <= nx && j < ny;
@@ -418,7 +447,8 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
   const double mu_l = s.mu_pert[cl], mu_r = s.mu_pert[cr];
   const double* U = s.u;
   const double* PP = s.p_pert;
-  __shared__ double sh[2][kTendURows][2][64];  // u at face ia and rv at column ia, level k
+  // u at face ia and rv at column ia, level k
+  double (*sh)[kTendURows][2][64] = reinterpret_cast<double (*)[kTendURows][2][64]>(lds);
   struct Lv {
     double x[7], y[7];                          // u at faces ia-3..ia+3 / rows j-3..j+3
     double ru_a, ru_m, ru_p, rv_a, rv_m, rv_pa, rv_pm;  // transport pieces
@@ -573,12 +603,21 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, D
   }
 }
 
+template <class TT, bool WET>
+__global__ void __launch_bounds__(64 * kTendURows) k_tend_u(DGrid g, DState s, DWork w) {
+  __shared__ double sh[2][kTendURows][2][64];
+  b_tend_u<TT, WET>(hw_blk(), &sh[0][0][0][0], g, s, w);
+}
+
 // advect_v_flux (flux_advection.py:1205-1274) + large_step_horizontal_pgf y
 // (rk_addtend_dry.py:345-361) on v faces (nz, ny+1, nx).  Face ny repeats
 // face 0's advection (periodic); the PGF uses edge-padded pairs.
 // WET: the moisture factor cqv scales the PGF; dry launches fold it to 1.
+// k_tend_v's LDS fields: v, ru, rv, and the six PGF north-row values
+constexpr int kTendVFields = 9;
+
 template <class TT, bool WET>
-__global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, DWork w) {
+__device__ __forceinline__ void b_tend_v(const Blk& B, double* lds, DGrid g, DState s, DWork w) {
   constexpr bool MX = sizeof(TT) == sizeof(float);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   // Blocks of kTendURows v-face rows of one 64-lane wave each; waves overlap
@@ -589,8 +628,8 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, D
   // the PGF's south row from the adjacent waves (LDS); the +-2, +-3 columns
   // and rows outside the block (or wrapping differently) are loaded.
   const unsigned nbx = (unsigned)(nx + 61) / 62;
-  const unsigned tile = tile_id(g);
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  const unsigned tile = tile_id(g, B);
+  const int tx = B.tx, ty = B.ty;
   const int i = (int)(tile % nbx) * 62 + tx - 1;  // outside 0..nx-1 on the halo lanes
   const int j = (int)(tile / nbx) * kTendURows + ty;
   const bool owned = tx >= 1 && tx <= 62 && i < nx && j <= ny;
@@ -631,8 +670,7 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, D
   const double mu_s = s.mu_pert[cl], mu_n = s.mu_pert[cr];
   const double* U = s.v;
   const double* PP = s.p_pert;
-  constexpr int kF = 9;  // LDS fields: v, ru, rv, and the six PGF north-row values
-  __shared__ double sh[2][kTendURows][kF][64];
+  double (*sh)[kTendURows][kTendVFields][64] = reinterpret_cast<double (*)[kTendURows][kTendVFields][64]>(lds);
   struct Lv {
     double x[7], y[7];                          // v at columns i-3..i+3 / rows ja-3..ja+3
     double ru_a, ru_m, ru_p, ru_mp, rv_a, rv_m, rv_p;   // transport pieces
@@ -803,12 +841,18 @@ __global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, D
   }
 }
 
+template <class TT, bool WET>
+__global__ void __launch_bounds__(64 * kTendURows) k_tend_v(DGrid g, DState s, DWork w) {
+  __shared__ double sh[2][kTendURows][kTendVFields][64];
+  b_tend_v<TT, WET>(hw_blk(), &sh[0][0][0][0], g, s, w);
+}
+
 // ---------------------------------------------------------------------------
 // small_step_prep_wrf, core/small_step_prep.py:216-394
 // ---------------------------------------------------------------------------
-__global__ void k_prep_2d(DGrid g, DState s, DWork w, int rk_step) {
+__device__ __forceinline__ void b_prep_2d(const Blk& B, DGrid g, DState s, DWork w, int rk_step) {
   size_t n = (size_t)g.ny * g.nx;
-  size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  size_t c = (size_t)tile_id(g, B) * B.bdx + B.tx;
   if (c >= n) return;
   double mub = s.mu_total[c] - s.mu_pert[c];
   double mu_cur = s.mu_pert[c];
@@ -825,11 +869,15 @@ __global__ void k_prep_2d(DGrid g, DState s, DWork w, int rk_step) {
   w.muave[c] = 0.0;
 }
 
+__global__ void k_prep_2d(DGrid g, DState s, DWork w, int rk_step) {
+  b_prep_2d(hw_blk(), g, s, w, rk_step);
+}
+
 // _u_face_average_2d / _v_face_average_2d (small_step_prep.py:77-88)
-__global__ void k_prep_faces(DGrid g, DWork w) {
+__device__ __forceinline__ void b_prep_faces(const Blk& B, DGrid g, DWork w) {
   const int nx = g.nx, ny = g.ny;
   size_t nu = (size_t)ny * (nx + 1), nv = (size_t)(ny + 1) * nx;
-  size_t idx = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  size_t idx = (size_t)tile_id(g, B) * B.bdx + B.tx;
   if (idx < nu) {
     int i = (int)(idx % (size_t)(nx + 1)), j = (int)(idx / (size_t)(nx + 1));
     double a, b;
@@ -864,13 +912,17 @@ __global__ void k_prep_faces(DGrid g, DWork w) {
   }
 }
 
+__global__ void k_prep_faces(DGrid g, DWork w) {
+  b_prep_faces(hw_blk(), g, w);
+}
+
 // coupled work velocity (small_step_prep.py:216-394) on one face column
 // (u faces: f = j * (nx + 1) + i; v faces: f = j * nx + i)
-__device__ inline void prep_face_column(const DGrid& g, unsigned f, unsigned fplane8, double mus, double mu,
-                                        double msf, const double* __restrict__ x0, const double* __restrict__ x,
-                                        double* __restrict__ xw) {
+__device__ inline void prep_face_column(const DGrid& g, unsigned chunk, unsigned f, unsigned fplane8, double mus,
+                                        double mu, double msf, const double* __restrict__ x0,
+                                        const double* __restrict__ x, double* __restrict__ xw) {
   const unsigned f8 = f * 8u;
-  const int k0 = blockIdx.y * kChunk, k1 = k0 + kChunk < g.nz ? k0 + kChunk : g.nz;
+  const int k0 = chunk * kChunk, k1 = k0 + kChunk < g.nz ? k0 + kChunk : g.nz;
 #pragma unroll 4
   for (int k = k0; k < k1; ++k) {
     const unsigned q = (unsigned)k * fplane8 + f8;
@@ -880,18 +932,26 @@ __device__ inline void prep_face_column(const DGrid& g, unsigned f, unsigned fpl
   }
 }
 
-__global__ void __launch_bounds__(256) k_prep_u(DGrid g, DState s, DWork w) {
+__device__ __forceinline__ void b_prep_u(const Blk& B, DGrid g, DState s, DWork w) {
   const unsigned ncol = (unsigned)(g.ny * (g.nx + 1));
-  const unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned f = B.x * B.bdx + B.tx;
   if (f >= ncol) return;
-  prep_face_column(g, f, ncol * 8u, w.muus[f], w.muu[f], g.msfuy[f], w.u0, s.u, w.u_work);
+  prep_face_column(g, B.y, f, ncol * 8u, w.muus[f], w.muu[f], g.msfuy[f], w.u0, s.u, w.u_work);
+}
+
+__global__ void __launch_bounds__(256) k_prep_u(DGrid g, DState s, DWork w) {
+  b_prep_u(hw_blk(), g, s, w);
+}
+
+__device__ __forceinline__ void b_prep_v(const Blk& B, DGrid g, DState s, DWork w) {
+  const unsigned ncol = (unsigned)((g.ny + 1) * g.nx);
+  const unsigned f = B.x * B.bdx + B.tx;
+  if (f >= ncol) return;
+  prep_face_column(g, B.y, f, ncol * 8u, w.muvs[f], w.muv[f], g.msfvx[f], w.v0, s.v, w.v_work);
 }
 
 __global__ void __launch_bounds__(256) k_prep_v(DGrid g, DState s, DWork w) {
-  const unsigned ncol = (unsigned)((g.ny + 1) * g.nx);
-  const unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= ncol) return;
-  prep_face_column(g, f, ncol * 8u, w.muvs[f], w.muv[f], g.msfvx[f], w.v0, s.v, w.v_work);
+  b_prep_v(hw_blk(), g, s, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -906,9 +966,9 @@ __global__ void __launch_bounds__(256) k_prep_v(DGrid g, DState s, DWork w) {
 //                c2a, php and calc_p_rho_wrf step 0 al, p
 //                (core/calc_p_rho.py:48-139)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
+__device__ __forceinline__ void b_prep_col(const Blk& B, DGrid g, DState s, DWork w) {
   const size_t ncol = (size_t)g.ny * g.nx;
-  const size_t c = (size_t)tile_id(g) * blockDim.x + threadIdx.x;
+  const size_t c = (size_t)tile_id(g, B) * B.bdx + B.tx;
   if (c >= ncol) return;
   const int nz = g.nz;
   const unsigned pl8 = (unsigned)ncol * 8u, c8 = (unsigned)c * 8u;
@@ -1012,6 +1072,10 @@ __global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_prep_col(DGrid g, DState s, DWork w) {
+  b_prep_col(hw_blk(), g, s, w);
+}
+
 // ---------------------------------------------------------------------------
 // Stage column setup, one thread per column marching up the faces:
 //  rw_tend = pg_buoy_w_moist(qtot=0) + advect_w + curvature
@@ -1033,11 +1097,12 @@ constexpr int kColOwned = 62;
 
 // WET: the state carries moisture (the raw cqw is read); a dry launch folds cq1 = 1, cq2 = 0
 template <class TT, bool WET>
-__global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
+__device__ __forceinline__ void b_col_setup(const Blk& B, double* lds, DGrid g, DNamelist nl,
+                                            DState s, DWork w, double dts) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const unsigned nbx = (unsigned)(nx + kColOwned - 1) / kColOwned;
-  const unsigned tile = tile_id(g);
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  const unsigned tile = tile_id(g, B);
+  const int tx = B.tx, ty = B.ty;
   const int i = (int)(tile % nbx) * kColOwned + tx - 1;  // -1 and >= nx on the halo lanes
   const int j = (int)(tile / nbx) * kColRows + ty;
   const bool owned = tx >= 1 && tx <= kColOwned && i >= 0 && i < nx && j < ny;
@@ -1124,7 +1189,7 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
       L.pst = ldo(s.ph_total, o + cjm8);
     }
   };
-  __shared__ double sh[2][kColRows][2][64];
+  double (*sh)[kColRows][2][64] = reinterpret_cast<double (*)[kColRows][2][64]>(lds);
   // level 0
   double p_m = ldo(s.p_pert, c8);  // p' at f-1
   double uw_m = ldo(s.u, uw8), ue_m = ldo(s.u, ue8), vs_m = ldo(s.v, vs8), vn_m = ldo(s.v, vn8);
@@ -1264,7 +1329,11 @@ __global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist 
   sto(w.cgamma, qt, 0.0);
 }
 
-inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+template <class TT, bool WET>
+__global__ void __launch_bounds__(64 * kColRows) k_col_setup(DGrid g, DNamelist nl, DState s, DWork w, double dts) {
+  __shared__ double sh[2][kColRows][2][64];
+  b_col_setup<TT, WET>(hw_blk(), &sh[0][0][0][0], g, nl, s, w, dts);
+}
 
 }  // namespace
 
@@ -1838,11 +1907,12 @@ __global__ void __launch_bounds__(256) k_lim_tend(DGrid g, DWork w, LimScalar a,
 // the grid rotates (f or e nonzero).
 // ---------------------------------------------------------------------------
 template <class TT>
-__global__ void __launch_bounds__(256) k_coriolis(DGrid g, DState s, DWork w, unsigned nbu) {
+__device__ __forceinline__ void b_coriolis(const Blk& B, DGrid g, DState s, DWork w,
+                                           unsigned nbu) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t pl = (size_t)ny * nx, upl = (size_t)ny * (nx + 1), vpl = (size_t)(ny + 1) * nx;
-  const bool ufaces = blockIdx.x < nbu;
-  const size_t t = (size_t)(ufaces ? blockIdx.x : blockIdx.x - nbu) * blockDim.x + threadIdx.x;
+  const bool ufaces = B.x < nbu;
+  const size_t t = (size_t)(ufaces ? B.x : B.x - nbu) * B.bdx + B.tx;
   if (ufaces ? t >= upl : t >= vpl) return;
   auto rw_at = [&](int k, size_t c) {
     return (s.w[(size_t)k * pl + c] * (g.c1f[k] * s.mu_total[c] + g.c2f[k])) / g.msfty[c];
@@ -1913,10 +1983,35 @@ __global__ void __launch_bounds__(256) k_coriolis(DGrid g, DState s, DWork w, un
 }
 
 template <class TT>
+__global__ void __launch_bounds__(256) k_coriolis(DGrid g, DState s, DWork w, unsigned nbu) {
+  b_coriolis<TT>(hw_blk(), g, s, w, nbu);
+}
+
+// Launch grids of the stage kernels, which the persistent ensemble step
+// gives its virtual blocks too.  Blocks are 256 threads (64 x 4 for the
+// tiles) except k_rom's 64.
+inline unsigned level_chunks(const DGrid& g) { return (unsigned)((g.nz + kChunk - 1) / kChunk); }
+inline unsigned rom_blocks(const DGrid& g) { return (unsigned)((g.nx + 63) / 64) * (unsigned)g.ny; }
+inline unsigned tend_u_tiles(const DGrid& g) {
+  return (unsigned)((g.nx + 1 + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
+}
+inline unsigned tend_v_tiles(const DGrid& g) {
+  return (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + 1 + kTendURows - 1) / kTendURows);
+}
+inline unsigned tend_t_tiles(const DGrid& g) {
+  return (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
+}
+inline unsigned col_setup_tiles(const DGrid& g) {
+  return (unsigned)((g.nx + kColOwned - 1) / kColOwned) * (unsigned)((g.ny + kColRows - 1) / kColRows);
+}
+inline unsigned coriolis_u_blocks(const DGrid& g) { return blocks_for((size_t)g.ny * (g.nx + 1), 256); }
+inline unsigned coriolis_blocks(const DGrid& g) {
+  return coriolis_u_blocks(g) + blocks_for((size_t)(g.ny + 1) * g.nx, 256);
+}
+
+template <class TT>
 void launch_coriolis(const DGrid& g, const DState& s, const DWork& w, hipStream_t st) {
-  const unsigned bs = 256;
-  const unsigned nbu = blocks_for((size_t)g.ny * (g.nx + 1), bs), nbv = blocks_for((size_t)(g.ny + 1) * g.nx, bs);
-  k_coriolis<TT><<<nbu + nbv, bs, 0, st>>>(g, s, w, nbu);
+  k_coriolis<TT><<<coriolis_blocks(g), 256, 0, st>>>(g, s, w, coriolis_u_blocks(g));
 }
 
 inline dim3 cell_grid(const DGrid& g) {
@@ -1993,8 +2088,7 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   const unsigned bs = 256;
   const size_t C = (size_t)g.ny * g.nx;
   const size_t F = (size_t)g.ny * (g.nx + 1) + (size_t)(g.ny + 1) * g.nx;
-  const unsigned colgrid = (unsigned)((g.nx + 63) / 64) * (unsigned)g.ny;
-  const unsigned kchunks = (unsigned)((g.nz + kChunk - 1) / kChunk);
+  const unsigned kchunks = level_chunks(g);
   if (w.d6u) WD_TIMED(prof, KID_DIFF6, st, (launch_diff6(g, nl, s, w, st)));
   if (w.xkmh) WD_TIMED(prof, KID_SMAG, st, (launch_smag(g, nl, s, w, st)));
   if (w.ck_dz) WD_TIMED(prof, KID_CONSTK, st, (launch_constk(g, nl, s, w, st)));
@@ -2007,15 +2101,13 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
              (k_moist_factors<<<cell_grid(g), dim3(kFieldBx, kFieldBy), 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_RUV, st,
            (k_ruv<<<dim3(blocks_for(C, bs), kchunks), bs, 0, st>>>(g, s.u, s.v, s.mu_total, w.ru, w.rv)));
-  WD_TIMED(prof, KID_ROM, st, (k_rom<<<colgrid, 64, 0, st>>>(g, w.ru, w.rv, w.rom)));
+  WD_TIMED(prof, KID_ROM, st, (k_rom<<<rom_blocks(g), 64, 0, st>>>(g, w.ru, w.rv, w.rom)));
   WD_TIMED(prof, KID_PREP_2D, st, (k_prep_2d<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w, rk_step)));
   // k_prep_col writes the large-step diagnostics the tendency kernels read
   WD_TIMED(prof, KID_PREP_COL, st, (k_prep_col<<<blocks_for(C, bs), bs, 0, st>>>(g, s, w)));
   const size_t CU = (size_t)g.ny * (g.nx + 1), CV = (size_t)(g.ny + 1) * g.nx;
-  const unsigned tu_tiles = (unsigned)((g.nx + 1 + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
+  const unsigned tu_tiles = tend_u_tiles(g), tv_tiles = tend_v_tiles(g), tt_tiles = tend_t_tiles(g);
   const dim3 tu_block(64, kTendURows);
-  const unsigned tt_tiles = (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + kTendURows - 1) / kTendURows);
-  const unsigned tv_tiles = (unsigned)((g.nx + 61) / 62) * (unsigned)((g.ny + 1 + kTendURows - 1) / kTendURows);
   if (nl.mixed) {
     WD_TIMED(prof, KID_TEND_U, st, (k_tend_u<float, false><<<tu_tiles, tu_block, 0, st>>>(g, s, w)));
     WD_TIMED(prof, KID_TEND_V, st, (k_tend_v<float, false><<<tv_tiles, tu_block, 0, st>>>(g, s, w)));
@@ -2063,8 +2155,7 @@ hipError_t launch_stage_setup(const DGrid& g, const DNamelist& nl, const DState&
   WD_TIMED(prof, KID_PREP_FACES, st, (k_prep_faces<<<blocks_for(F, bs), bs, 0, st>>>(g, w)));
   WD_TIMED(prof, KID_PREP_U, st, (k_prep_u<<<dim3(blocks_for(CU, bs), kchunks), bs, 0, st>>>(g, s, w)));
   WD_TIMED(prof, KID_PREP_V, st, (k_prep_v<<<dim3(blocks_for(CV, bs), kchunks), bs, 0, st>>>(g, s, w)));
-  const unsigned col_tiles =
-      (unsigned)((g.nx + kColOwned - 1) / kColOwned) * (unsigned)((g.ny + kColRows - 1) / kColRows);
+  const unsigned col_tiles = col_setup_tiles(g);
   const dim3 col_block(64, kColRows);
   if (nl.mixed) {
     WD_TIMED(prof, KID_COL_SETUP, st, (k_col_setup<float, false><<<col_tiles, col_block, 0, st>>>(g, nl, s, w, dts)));

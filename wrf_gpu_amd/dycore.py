@@ -276,9 +276,18 @@ class Ensemble:
     picked alone; ``tuned`` then lists those choices.  Tuned alone, a member
     does not see the other branches running beside it, so its pick is not
     necessarily the fastest inside the ensemble.
+
+    ``persistent=True`` captures instead one launch of the persistent step
+    per large step (``wd_batch_create_persistent``): every kernel body of
+    every member runs from one launch of ``workgroups`` resident workgroups
+    that meet at a grid barrier between phases.  Members must share the
+    precision form and substep count and use no explicit diffusion, limiter
+    or moisture; nothing else may run on the GPU during a step.  ``status``
+    synchronises and raises if a barrier wait ran out since the last check.
     """
 
-    def __init__(self, members, branches: int = 4, autotune: bool = False, trial_steps: int = 1):
+    def __init__(self, members, branches: int = 4, autotune: bool = False, trial_steps: int = 1,
+                 persistent: bool = False):
         if not members:
             raise ValueError("an ensemble needs at least one member")
         self.members = list(members)
@@ -294,12 +303,33 @@ class Ensemble:
         handles = (ctypes.c_void_p * n)(*[d._handle for d, _ in self.members])
         states = (_lib.WdState * n)(*[_state_ptrs(s, d.namelist.moist) for d, s in self.members])
         out = ctypes.c_void_p()
-        code = lib.wd_batch_create(handles, states, n, int(branches), ctypes.byref(out))
+        if persistent:
+            code = lib.wd_batch_create_persistent(handles, states, n, ctypes.byref(out))
+        else:
+            code = lib.wd_batch_create(handles, states, n, int(branches), ctypes.byref(out))
         if code != _lib.WD_OK:
             raise _lib.WrfdynError(code, _lib.last_error(None))
         self._batch = out
         self._lib = lib
         self.device = dev
+        self.persistent = bool(persistent)
+
+    @property
+    def workgroups(self) -> int:
+        """Workgroups of the persistent step's launch (0 for the graph of kernel launches)."""
+
+        return int(self._lib.wd_batch_workgroups(self._batch))
+
+    def status(self, stream=None) -> None:
+        """Synchronise ``stream``; raise if a persistent step's barrier wait ran out since the last call."""
+
+        stream = self._stream(stream)
+        self._check(self._lib.wd_batch_status(self._batch, ctypes.c_void_p(stream.cuda_stream)))
+
+    def set_spin_limit(self, polls: int) -> None:
+        """Bound one grid barrier wait of the persistent step to ``polls`` polls."""
+
+        self._check(self._lib.wd_batch_set_spin_limit(self._batch, int(polls)))
 
     def branch(self, member: int) -> int:
         return int(self._lib.wd_batch_branch(self._batch, int(member)))
