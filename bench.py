@@ -344,12 +344,13 @@ def moist_leg(case, steps: int, fp64_ms_per_step: float, dev) -> dict:
 
 def small_workload_leg(steps: int, dev) -> dict:
     """The 64x64x40 warm bubble (dt 6 s, ns 6): a launch-bound size, stepped
-    eagerly and by replaying the graph-captured large step."""
+    eagerly, by replaying the graph-captured large step, and as a one-member
+    persistent step."""
 
     import torch
 
     from wrf_gpu_amd import ic
-    from wrf_gpu_amd.dycore import Dycore, state_to_device
+    from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device
 
     case = ic.bubble_3d()
     m, nl = case.metrics, case.namelist
@@ -366,6 +367,19 @@ def small_workload_leg(steps: int, dev) -> dict:
         out[name] = {"ms_per_step": ms, "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3)}
     out["graph_speedup"] = out["eager"]["ms_per_step"] / out["graph"]["ms_per_step"]
     out["graph_equals_eager"] = all(bool(torch.equal(se[k], sg[k])) for k in se)
+    # the persistent step with this one member: one launch per large step instead of ~85
+    pers = Dycore(m, nl)
+    sp = state_to_device(case.state, dev)
+    ens = Ensemble([(pers, sp)], persistent=True)
+    ens.step(1)
+    ens.status()
+    ms = timed_steps(ens.step, steps, stream) / steps
+    ens.status()
+    out["persistent"] = {"ms_per_step": ms, "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3),
+                         "workgroups": ens.workgroups}
+    out["persistent_equals_eager"] = all(bool(torch.equal(se[k], sp[k])) for k in se)
+    ens.close()
+    pers.close()
     # the autotuner's pick for this size (k-marching kernels have one wave per 64 columns here)
     out["autotuned_config"] = graphed.autotune(sg, trial_steps=2)
     graphed.capture_graph(sg)

@@ -474,6 +474,22 @@ def piece(metrics, namelist, arrays: dict, piece_id: int, rk_step: int, dts: flo
         raise RuntimeError(f"orc_piece failed with code {rc}")
 
 
+def piece_moist(metrics, namelist, arrays: dict, piece_id: int, rk_step: int, dts: float, cqu, cqv, cqwr) -> None:
+    """``piece`` for PIECE_UV or PIECE_W with the moisture's coupling factors on the u, v and w faces."""
+
+    L = lib()
+    L.orc_piece_moist.restype = ctypes.c_int
+    keep = _Keep()
+    g = _grid(metrics, keep)
+    nl = _namelist(namelist)
+    table = _pointer_table(array_names(), arrays)
+    factors = [np.ascontiguousarray(x, dtype=np.float64) for x in (cqu, cqv, cqwr)]
+    rc = L.orc_piece_moist(ctypes.byref(g), ctypes.byref(nl), ctypes.c_int(piece_id), ctypes.c_int(rk_step),
+                           ctypes.c_double(dts), table, *[_ptr(x) for x in factors])
+    if rc != 0:
+        raise RuntimeError(f"orc_piece_moist failed with code {rc}")
+
+
 def piece_vjp(metrics, namelist, arrays: dict, piece_id: int, rk_step: int, dts: float, bar: dict) -> dict:
     """Adjoint of every array before one piece, from ``bar`` (the adjoint of
     the arrays after it; missing names count as zero).  ``arrays`` holds the

@@ -36,14 +36,13 @@
  * step bit for bit, the moist rw_tend and the large-step cqu are pinned at
  * first order in dt, the implicit-w rows by their per-face cq1 scaling and
  * the species' transport by conservation and positivity
- * (test_oracle_moist.py).
- *
- * Parity unpinned: the moisture factor of the explicit part of the implicit
- * w equation (substep_w termA) and of the acoustic u/v pressure gradient
- * (substep_uv).  Both act on work arrays that start each stage at zero, so
- * they enter a step at second order in dt only, which no closed form above
- * isolates.  What no reference file holds is the reference's own
- * field-level FP64 output beyond the 2-D slabs.
+ * (test_oracle_moist.py); the factors inside the acoustic loop (cqu/cqv on
+ * the u/v pressure gradient, cq1 on the explicit part termA of the implicit
+ * w equation), which enter a whole step at second order in dt only, are
+ * pinned on the acoustic pieces: with the term's inputs zero the output does
+ * not depend on the factor, and otherwise it is affine in each face's own
+ * factor (test_oracle_moist_acoustic.py).  What no reference file holds is
+ * the reference's own field-level FP64 output beyond the 2-D slabs.
  */
 #include "wrfdyn_oracle.h"
 #include "wrfdyn_math.h"
@@ -2456,6 +2455,25 @@ int orc_piece(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step,
     rstate S;
     ws_t W;
     bind_arrays(arrays, &S, &W);
+    return run_piece(g, nl, piece, rk_step, dts, &S, &W);
+}
+
+/* An acoustic piece that reads the moisture's coupling factors (advance_uv:
+ * cqu, cqv; advance_w: the raw cqw) run with the given factor arrays. */
+int orc_piece_moist(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,
+                    double *const *arrays, const double *cqu, const double *cqv, const double *cqwr) {
+    if (g->nx < 4 || g->ny < 1 || g->nz < 4) return -2;
+    if (piece != ORC_PIECE_UV && piece != ORC_PIECE_W) return -2;
+    if (!cqu || !cqv || !cqwr) return -2;
+    rstate S;
+    ws_t W;
+    moist_t m;
+    memset(&m, 0, sizeof m);
+    m.cqu = (real *)cqu;
+    m.cqv = (real *)cqv;
+    m.cqwr = (real *)cqwr;
+    bind_arrays(arrays, &S, &W);
+    W.moist = &m;
     return run_piece(g, nl, piece, rk_step, dts, &S, &W);
 }
 #endif

@@ -107,6 +107,11 @@ int orc_array_extent(int a);
 #define ORC_PIECE_FINISH 6
 int orc_piece(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,
               double *const *arrays);
+/* ORC_PIECE_UV or ORC_PIECE_W with the moisture's coupling factors: cqu
+ * (nz, ny, nx+1), cqv (nz, ny+1, nx) and the raw cqw (nz+1, ny, nx), as
+ * orc_moist_factors gives them; for the pins of those two uses. */
+int orc_piece_moist(const orc_grid *g, const orc_namelist *nl, int piece, int rk_step, double dts,
+                    double *const *arrays, const double *cqu, const double *cqv, const double *cqwr);
 
 /* Single-operator entry points used by the oracle's own pinning tests. */
 int orc_calc_coef_w(const orc_grid *g, const double *mut, const double *c2a,
