@@ -901,12 +901,18 @@ int wd_batch_create_persistent(wd_handle* const* members, const wd_state* states
   if (!wd::persistent_program(pm, &segs, &phases, &why)) return fail(nullptr, WD_EINVAL, why);
   const bool mixed = pm[0].nl.mixed != 0;
   // as many workgroups as the GPU holds at once (the grid barrier needs every
-  // one resident), a multiple of the 8 XCDs
+  // one resident), a multiple of the 8 XCDs,
   int per_cu = 0, cus = 0;
   e = wd::persistent_blocks_per_cu(mixed, &per_cu);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return hip_fail(nullptr, e, "occupancy of the persistent step");
-  const unsigned grid = (unsigned)(per_cu * cus) / wd::kNumXcd * wd::kNumXcd;
+  // and no more than the largest phase has virtual blocks: fewer workgroups
+  // meet at each barrier, and none of them would find work
+  unsigned most = 0;
+  for (const wd::PPhase& p : phases) most = p.total > most ? p.total : most;
+  const unsigned fit = (unsigned)(per_cu * cus) / wd::kNumXcd * wd::kNumXcd;
+  const unsigned need = (most + wd::kNumXcd - 1) / wd::kNumXcd * wd::kNumXcd;
+  const unsigned grid = need < fit ? need : fit;
   if (grid == 0) return fail(nullptr, WD_EDEVICE, "the persistent step does not fit on this GPU");
   wd_batch* b = new (std::nothrow) wd_batch();
   if (!b) return fail(nullptr, WD_ENOMEM, "host allocation failed");

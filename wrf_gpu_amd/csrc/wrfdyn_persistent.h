@@ -55,7 +55,9 @@ static_assert(64 * kTendURows == 256 && 64 * kColRows == 256 && 64 * kMuRows == 
               "every tile body runs on a 256-thread workgroup");
 
 // wd_step's step-origin copies (u0 = u, v0 = v, w0 = w, th0 = theta,
-// ph0 = ph', mu0 = mu'), element e of each.
+// ph0 = ph', mu0 = mu'): a virtual block copies kOriginChunk runs of 256
+// elements of each array.
+constexpr unsigned kOriginChunk = 8;
 __device__ inline void origin_copy(const PMember& M, size_t e) {
   const DGrid& g = M.g;
   const size_t C = (size_t)g.ny * g.nx, Mn = C * g.nz, Wn = C * (g.nz + 1);
@@ -81,7 +83,7 @@ __device__ __forceinline__ void run_segment(const PSeg& S, const PMember& M, uns
   const Blk tile{local, 0u, S.gx, tid & 63u, tid >> 6, 64u};
   switch (S.kind) {
     case PK_ORIGIN:
-      origin_copy(M, (size_t)local * 256u + tid);
+      for (unsigned r = 0; r < kOriginChunk; ++r) origin_copy(M, ((size_t)local * kOriginChunk + r) * 256u + tid);
       break;
     case PK_RUV:
       b_ruv(flat, M.g, M.s.u, M.s.v, M.s.mu_total, M.w.ru, M.w.rv);
@@ -145,19 +147,26 @@ __device__ __forceinline__ void run_segment(const PSeg& S, const PMember& M, uns
   }
 }
 
-// The grid barrier.  Every thread releases its stores at device scope (each
-// XCD has its own L2), the workgroup's first thread arrives, and the last to
-// arrive resets the count and advances the generation the others wait on.
-// A wait is bounded (spin_limit polls): one that runs out raises the abort
-// flag, which every other wait and the next launches see, so each workgroup
-// returns and the host reports the failure; no wait lasts for ever.
+// s_waitcnt vmcnt(0) in the gfx9 encoding (vmcnt 0, expcnt 7, lgkmcnt 15):
+// every vector memory access the wave issued, stores included, has completed.
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+// The grid barrier.  Each wave waits for its own stores of the phase; the
+// workgroup's first thread then arrives with a device-scope release, which
+// writes its XCD's L2 back (each XCD has its own L2), and the last to
+// arrive resets the count and advances the generation the others poll.
+// Polls are relaxed device-scope loads; one device-scope acquire after the
+// wait drops the stale lines from the CU's L1 and the XCD's L2.  A wait is
+// bounded (spin_limit polls): one that runs out raises the abort flag, which
+// every other wait and the next launches see, so each workgroup returns and
+// the host reports the failure; no wait lasts for ever.
 __device__ bool grid_barrier(PBarrier* bar, int* ok) {
-  __threadfence();
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   __syncthreads();
   if (threadIdx.x == 0) {
     int good = 1;
     const unsigned gen = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived + 1u == gridDim.x) {
       __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -169,15 +178,15 @@ __device__ bool grid_barrier(PBarrier* bar, int* ok) {
           good = 0;
           break;
         }
-        if (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) break;
+        if (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen) break;
         __builtin_amdgcn_s_sleep(2);
       }
       if (!good) __hip_atomic_store(&bar->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     *ok = good;
   }
   __syncthreads();
-  __threadfence();
   return *ok != 0;
 }
 
@@ -268,7 +277,7 @@ bool persistent_program(const std::vector<PMember>& members, std::vector<PSeg>* 
         switch (jb.kind) {
           case PK_ORIGIN: {
             const size_t U = (size_t)g.nz * CU, V = (size_t)g.nz * CV, Wn = (size_t)(g.nz + 1) * C;
-            S.gx = blocks_for(std::max(U, std::max(V, Wn)), 256);
+            S.gx = blocks_for(std::max(U, std::max(V, Wn)), 256 * kOriginChunk);
             break;
           }
           case PK_RUV: S.gx = blocks_for(C, 256); S.gy = level_chunks(g); break;
