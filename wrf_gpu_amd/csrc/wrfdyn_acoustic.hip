@@ -1051,12 +1051,21 @@ __device__ __forceinline__ void b_column_w(const Blk& B, DGrid g, DNamelist nl, 
       }
     }
   };
+  // damp_opt 3: the damping layer starts at fdamp, the lowest face f >= 1
+  // with ph1t(f) / g >= hbot, found on the way up; below it the damping
+  // weight is 0 and w passes through unchanged, so the down sweep skips it
+  const bool damp = (nl.damp_opt == 3 && nl.dampcoef > 0.0);
+  const double dampmag = dts * nl.dampcoef, hdepth = nl.zdamp;
+  const double htop = ldo(w.ph1t, c8 + (unsigned)nz * plane) / kG;
+  const double hbot = htop - hdepth;
+  int fdamp = nz + 1;
   Wv wv_next{};
   load_w(0, wv_next);
   for (int f = 0; f <= nz + 2; ++f) {
     const Wv wv = wv_next;
     wv_next = Wv{};
     load_w(f + 1, wv_next);
+    if (damp && f >= 1 && f <= nz && fdamp > nz && wv.ph1t / kG >= hbot) fdamp = f;
     // phi rhs at fd = f-1 (advance_w.py:460-503)
     double d0 = 0.0;
     if (f >= 1 && f <= nz + 1) {
@@ -1122,10 +1131,6 @@ __device__ __forceinline__ void b_column_w(const Blk& B, DGrid g, DNamelist nl, 
 
   // ---- sweep 3: back substitution, damping, phi finish, calc_p_rho ----
   {
-    const bool damp = (nl.damp_opt == 3 && nl.dampcoef > 0.0);
-    const double dampmag = dts * nl.dampcoef, hdepth = nl.zdamp;
-    const double htop = ldo(w.ph1t, c8 + (unsigned)nz * plane) / kG;
-    const double hbot = htop - hdepth;
     const double mu_work = muts - mut;
     double w_next = 0.0, phn_next = 0.0;
     struct Dv {  // level f loads of the down sweep (mass-level fields for f < nz)
@@ -1138,7 +1143,7 @@ __device__ __forceinline__ void b_column_w(const Blk& B, DGrid g, DNamelist nl, 
       L.wf = ldo(w.wfwd, q);
       if (f >= 1 && f < nz) L.cg = ldo(w.cgamma, q);
       if (f >= 1) {
-        L.ph1t = ldo(w.ph1t, q);
+        if (f >= fdamp) L.ph1t = ldo(w.ph1t, q);
         L.rhs = ldo(w.rhs, q);
       } else {
         L.phw0 = ldo(w.ph_work, q);  // the surface ph work is unchanged
@@ -1170,7 +1175,7 @@ __device__ __forceinline__ void b_column_w(const Blk& B, DGrid g, DNamelist nl, 
         phn = dv.phw0;
       } else {
         double wd = wv;
-        if (damp) {
+        if (f >= fdamp) {  // damp_opt 3 (fdamp stays past the lid without it)
           double hk = dv.ph1t / kG;
           double sn = wd_sin(((0.5 * kPi) * (hk - hbot)) / hdepth);
           double ramp = sn * sn;
@@ -1208,9 +1213,11 @@ __device__ __forceinline__ void b_column_w(const Blk& B, DGrid g, DNamelist nl, 
   }
 }
 
+// 3 waves per SIMD: the kernel fits 168 VGPRs without spilling
 template <class TT>
-__global__ void __launch_bounds__(256) k_column_w(DGrid g, DNamelist nl, DState s, DWork w, double dts,
-                                                   int first) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_column_w(DGrid g, DNamelist nl,
+                                                                                         DState s, DWork w,
+                                                                                         double dts, int first) {
   b_column_w<TT>(hw_blk(), g, nl, s, w, dts, first);
 }
 

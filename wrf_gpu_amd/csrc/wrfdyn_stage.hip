@@ -47,7 +47,10 @@ __global__ void __launch_bounds__(256) k_ruv(DGrid g, const double* __restrict__
   b_ruv(hw_blk(), g, u, v, mu, ru, rv);
 }
 
-// calc_ww_cp omega, dynamics/flux_advection.py:430-458 (one thread per column)
+// calc_ww_cp omega, dynamics/flux_advection.py:430-458 (one thread per column).
+// Up to kRomRegLevels levels the column's divergence is read once and kept in
+// registers for the second sweep; deeper grids read it twice.
+constexpr int kRomRegLevels = 64;
 __device__ __forceinline__ void b_rom(const Blk& B, DGrid g, const double* __restrict__ ru,
                                       const double* __restrict__ rv, double* __restrict__ rom) {
   const unsigned nbx = (g.nx + B.bdx - 1) / B.bdx;
@@ -57,26 +60,42 @@ __device__ __forceinline__ void b_rom(const Blk& B, DGrid g, const double* __res
   if (i >= g.nx) return;
   int ip = wrapi(i + 1, g.nx), jp = wrapi(j + 1, g.ny);
   double msftx = g.msftx[i2(g, j, i)];
-  double dmdt = 0.0;
-  for (int k = 0; k < g.nz; ++k) {
-    double d = (g.dnw[k] * msftx) * (g.rdx * (ru[im(g, k, j, ip)] - ru[im(g, k, j, i)]) +
-                                     g.rdy * (rv[im(g, k, jp, i)] - rv[im(g, k, j, i)]));
-    dmdt += d;
-  }
-  double cum = 0.0;
+  auto div = [&](int k) {
+    return (g.dnw[k] * msftx) * (g.rdx * (ru[im(g, k, j, ip)] - ru[im(g, k, j, i)]) +
+                                 g.rdy * (rv[im(g, k, jp, i)] - rv[im(g, k, j, i)]));
+  };
+  double dmdt = 0.0, cum = 0.0;
   rom[im(g, 0, j, i)] = 0.0;
-  for (int k = 0; k < g.nz; ++k) {
-    double d = (g.dnw[k] * msftx) * (g.rdx * (ru[im(g, k, j, ip)] - ru[im(g, k, j, i)]) +
-                                     g.rdy * (rv[im(g, k, jp, i)] - rv[im(g, k, j, i)]));
-    double incr = -((g.dnw[k] * g.c1h[k]) * dmdt) - d;
-    cum = (k == 0) ? incr : cum + incr;
-    if (k + 1 < g.nz) rom[im(g, k + 1, j, i)] = cum;
+  if (g.nz <= kRomRegLevels) {
+    // one pass over ru and rv: the column's divergence stays in registers
+    // (every index is fixed once the loops are unrolled)
+    double d[kRomRegLevels];
+#pragma unroll
+    for (int k = 0; k < kRomRegLevels; ++k)
+      if (k < g.nz) {
+        d[k] = div(k);
+        dmdt += d[k];
+      }
+#pragma unroll
+    for (int k = 0; k < kRomRegLevels; ++k)
+      if (k < g.nz) {
+        double incr = -((g.dnw[k] * g.c1h[k]) * dmdt) - d[k];
+        cum = (k == 0) ? incr : cum + incr;
+        if (k + 1 < g.nz) rom[im(g, k + 1, j, i)] = cum;
+      }
+  } else {
+    for (int k = 0; k < g.nz; ++k) dmdt += div(k);
+    for (int k = 0; k < g.nz; ++k) {
+      double incr = -((g.dnw[k] * g.c1h[k]) * dmdt) - div(k);
+      cum = (k == 0) ? incr : cum + incr;
+      if (k + 1 < g.nz) rom[im(g, k + 1, j, i)] = cum;
+    }
   }
   rom[im(g, g.nz, j, i)] = 0.0;
 }
 
-__global__ void k_rom(DGrid g, const double* __restrict__ ru, const double* __restrict__ rv,
-                      double* __restrict__ rom) {
+__global__ void __launch_bounds__(64) k_rom(DGrid g, const double* __restrict__ ru, const double* __restrict__ rv,
+                                            double* __restrict__ rom) {
   b_rom(hw_blk(), g, ru, rv, rom);
 }
 
