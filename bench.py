@@ -345,7 +345,8 @@ def moist_leg(case, steps: int, fp64_ms_per_step: float, dev) -> dict:
 def small_workload_leg(steps: int, dev) -> dict:
     """The 64x64x40 warm bubble (dt 6 s, ns 6): a launch-bound size, stepped
     eagerly, by replaying the graph-captured large step, and as a one-member
-    persistent step."""
+    persistent step, then by the autotuned graph and the autotuned persistent
+    step."""
 
     import torch
 
@@ -386,6 +387,15 @@ def small_workload_leg(steps: int, dev) -> dict:
     graphed.graph_step(1)
     ms = timed_steps(lambda n: graphed.graph_step(n), steps, stream) / steps
     out["autotuned_graph"] = {"ms_per_step": ms, "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3)}
+    # the persistent step with the autotuned handle: its column tile runs inside the launch
+    ens = Ensemble([(graphed, sg)], persistent=True)
+    ens.step(1)
+    ens.status()
+    ms = timed_steps(ens.step, steps, stream) / steps
+    ens.status()
+    out["persistent_autotuned"] = {"ms_per_step": ms, "value": m.n_cells * nl.acoustic_substeps / (ms * 1e-3),
+                                   "workgroups": ens.workgroups, "column_kernel": ens.column_kernel(0)}
+    ens.close()
     eager.close()
     graphed.close()
     return out
@@ -395,8 +405,9 @@ def ensemble_leg(steps: int, dev) -> dict:
     """32 members (8 each of 129^2, 201^2, 257^2, 321^2 columns, 50 levels):
     members stepped one after another with the plain step, then the batched
     graph (4 branches) and the persistent step (one launch per large step),
-    each in FP64 and in the mixed form, and the batched graph in FP64 with
-    each member's autotuned column kernel and workgroup order."""
+    each in FP64 and in the mixed form, and the batched graph and the
+    persistent step in FP64 with each member's autotuned column kernel and
+    workgroup order."""
 
     import torch
 
@@ -457,11 +468,22 @@ def ensemble_leg(steps: int, dev) -> dict:
                                            "branches": ens.branches, "member_picks": picks,
                                            "autotune_s": tune_s}
     ens.close()
+    # the persistent step with the same autotuned members: tuned 16x16 column tiles run inside the launch
+    ens = Ensemble(members, persistent=True)
+    ens.step(1)
+    ens.status()
+    ms = timed_steps(ens.step, steps, stream) / steps
+    ens.status()
+    kernels = [ens.column_kernel(q) for q in range(len(members))]
+    out["persistent_fp64_autotuned"] = {"ms_per_step": ms, "value": cells * ns / (ms * 1e-3),
+                                        "workgroups": ens.workgroups,
+                                        "members_on_column_tile": sum(k >= 0 for k in kernels)}
+    ens.close()
     base = out["sequential_fp64"]["ms_per_step"]
     out["speedup_vs_sequential_fp64"] = {k: base / out[k]["ms_per_step"]
                                          for k in ("batched_graph_fp64", "batched_graph_mixed",
                                                    "persistent_fp64", "persistent_mixed",
-                                                   "batched_graph_fp64_autotuned")}
+                                                   "batched_graph_fp64_autotuned", "persistent_fp64_autotuned")}
     for d, _ in members:
         d.close()
     return out

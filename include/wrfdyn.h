@@ -207,9 +207,10 @@ int wd_batch_branches(const wd_batch *b);
  * graph, so wd_step_batched runs it n_large_steps times; each member's
  * result is identical to wd_step on that member alone.  Members must share
  * the precision form and the substep count and use none of the explicit
- * diffusions, the limiter or the moisture (WD_EINVAL); the column march
- * serves every member whatever its tuned column form, and each keeps its
- * tuned workgroup order.  The grid barrier needs every workgroup resident,
+ * diffusions, the limiter or the moisture (WD_EINVAL).  Each member keeps
+ * its tuned workgroup order and, in the FP64 form up to 63 levels, a tuned
+ * 16 x 16 LDS column tile; any other column form runs as the column march,
+ * which gives the same bits (wd_batch_column_kernel tells which runs).  The grid barrier needs every workgroup resident,
  * so nothing else may run on the GPU during a persistent step: a barrier
  * wait that runs out of polls (wd_batch_set_spin_limit) abandons the step
  * and every later one until wd_batch_status reports it.  wd_batch_branches
@@ -224,6 +225,12 @@ int wd_batch_status(wd_batch *b, void *stream);
  * (default 4,194,304).  Synchronises the device. */
 int wd_batch_set_spin_limit(wd_batch *b, unsigned polls);
 int wd_batch_workgroups(const wd_batch *b);
+/* The acoustic column kernel that `member` runs in the batch's step: -1 for
+ * the column march, else the index of its LDS tile shape (as
+ * wd_tuned_config).  For a graph batch this is the member's own choice; a
+ * persistent batch reports what its step runs.  No reference counterpart: a
+ * query of this build's batched forms. */
+int wd_batch_column_kernel(const wd_batch *b, int member, int *column_tile);
 
 const char *wd_batch_last_error(const wd_batch *b);
 void wd_batch_destroy(wd_batch *b);

@@ -5,7 +5,8 @@ workgroups meeting at a grid barrier between phases.  Each virtual block is
 run by the same kernel code, with the block index, grid and thread layout of
 the standalone launch, as wd_step, so every member must come out bit for bit
 as that member stepped alone: FP64 and mixed, with rotation, at the uneven
-ensemble's member sizes, over several launches.  A barrier wait that runs out
+ensemble's member sizes, with members tuned to the LDS column tile, over
+several launches.  A barrier wait that runs out
 (forced here with a zero poll budget) must end every workgroup, be reported
 by wd_batch_status and leave the states as they were; the batch then steps
 normally.  Members the persistent step cannot run are refused.
@@ -52,17 +53,41 @@ def _rotating(case, lat0):
     return case.__class__(**{**case.__dict__, "metrics": m})
 
 
-def _run_pair(cases, steps, precision="fp64"):
-    """Each case stepped alone and in one persistent ensemble; returns both host states per case."""
+def _dycore(case, precision, column=None):
+    """A handle for the case; ``column`` ("16x16", "march", ...) picks its acoustic column kernel."""
+
+    import os
+    from wrf_gpu_amd.dycore import Dycore
+
+    keep = os.environ.pop("WRFDYN_COLUMN_TILE", None)
+    if column is not None:
+        os.environ["WRFDYN_COLUMN_TILE"] = column
+    try:
+        return Dycore(case.metrics, case.namelist, precision=precision)
+    finally:
+        os.environ.pop("WRFDYN_COLUMN_TILE", None)
+        if keep is not None:
+            os.environ["WRFDYN_COLUMN_TILE"] = keep
+
+
+def _run_pair(cases, steps, precision="fp64", columns=None, expect_kernels=None):
+    """Each case stepped alone and in one persistent ensemble; returns both host states per case.
+    ``columns`` gives each member's column kernel choice, ``expect_kernels`` the column kernel
+    the persistent step must then run for each (-1 the column march, 0 the 16x16 tile)."""
 
     import torch
-    from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device, state_to_host
+    from wrf_gpu_amd.dycore import Ensemble, state_to_device, state_to_host
 
-    alone = [(Dycore(c.metrics, c.namelist, precision=precision), state_to_device(c.state)) for c in cases]
-    batched = [(Dycore(c.metrics, c.namelist, precision=precision), state_to_device(c.state)) for c in cases]
+    columns = columns or [None] * len(cases)
+    alone = [(_dycore(c, precision, col), state_to_device(c.state)) for c, col in zip(cases, columns)]
+    batched = [(_dycore(c, precision, col), state_to_device(c.state)) for c, col in zip(cases, columns)]
     ens = Ensemble(batched, persistent=True)
     assert ens.branches == 1 and ens.workgroups > 0 and ens.workgroups % 8 == 0
-    print(f"[persistent] {len(cases)} members, {precision}, {ens.workgroups} workgroups, steps {steps}", flush=True)
+    kernels = [ens.column_kernel(m) for m in range(len(cases))]
+    if expect_kernels is not None:
+        assert kernels == list(expect_kernels), kernels
+    print(f"[persistent] {len(cases)} members, {precision}, {ens.workgroups} workgroups, steps {steps}, "
+          f"column kernels {kernels}", flush=True)
     out = []
     done = 0
     for n in steps:
@@ -98,6 +123,42 @@ def test_persistent_rotating_members_match_individual_steps():
     cases = _small_cases()
     cases = [_rotating(cases[1], 45.0), cases[0], _rotating(cases[3], 30.0)]
     _run_pair(cases, (2,))
+
+
+def test_persistent_runs_tuned_column_tiles(oracle_lib):
+    """Members tuned to the 16x16 LDS column tile run it inside the launch at 1, 3 and 4 levels
+    per thread (8, 40, 50 and 63 levels; the damping layer active at 50 and 63), beside a member
+    on the column march, one tuned to another tile shape and one too deep for the persistent
+    tile (64 levels), which both run as the column march.  Every member comes out bit for bit
+    as stepped alone with its own choice; the 50-level tile member also against the oracle."""
+
+    from wrf_gpu_amd import ic
+
+    terrain = dict(dx=3000.0, u_sigma=1.0, sigma=0.5)
+    cases = [
+        ic.bubble_3d(nx=16, ny=12, nz=8, dx=1000.0, u0=5.0),
+        ic.stratified_terrain_3d(nx=20, ny=18, nz=40, **terrain),
+        ic.stratified_terrain_3d(nx=20, ny=18, nz=50, **terrain),
+        ic.stratified_terrain_3d(nx=21, ny=7, nz=63, **terrain),
+        ic.bubble_3d(nx=70, ny=19, nz=12, dx=1000.0, u0=10.0),
+        ic.stratified_terrain_3d(nx=12, ny=10, nz=8, dx=2000.0, u_sigma=0.5, sigma=0.2),
+        ic.stratified_terrain_3d(nx=17, ny=9, nz=64, **terrain),
+    ]
+    columns = ["16x16", "16x16", "16x16", "16x16", "march", "8x16", "16x16"]
+    out = _run_pair(cases, (2, 1), columns=columns, expect_kernels=[0, 0, 0, 0, -1, -1, -1])
+    ref = oracle_lib.run(cases[2].metrics, cases[2].namelist, cases[2].state, 3)
+    errs = field_errors(out[2][1], ref)
+    for k in FIELDS:
+        assert errs[k] <= 1e-12, (k, errs[k])
+
+
+def test_persistent_mixed_form_keeps_the_column_march():
+    """The LDS column tile is FP64 only: a mixed-form member runs the column march."""
+
+    from wrf_gpu_amd import ic
+
+    cases = [ic.stratified_terrain_3d(nx=20, ny=18, nz=40, dx=3000.0, u_sigma=1.0, sigma=0.5)]
+    _run_pair(cases, (2,), precision="mixed", columns=["16x16"], expect_kernels=[-1])
 
 
 def test_persistent_at_the_ensemble_member_sizes():

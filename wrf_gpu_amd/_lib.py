@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error
                     "wd_batch_create", "wd_step_batched", "wd_batch_branch", "wd_batch_branches",
                     "wd_batch_last_error", "wd_batch_destroy", "wd_step_adjoint", "wd_adjoint_tape_bytes",
                     "wd_step_noalloc", "wd_batch_create_persistent", "wd_batch_status", "wd_batch_set_spin_limit",
-                    "wd_batch_workgroups")
+                    "wd_batch_workgroups", "wd_batch_column_kernel")
 
 
 class WdGrid(ctypes.Structure):
@@ -152,6 +152,8 @@ def load() -> ctypes.CDLL:
         lib.wd_batch_set_spin_limit.argtypes = [ctypes.c_void_p, ctypes.c_uint]
         lib.wd_batch_workgroups.restype = ctypes.c_int
         lib.wd_batch_workgroups.argtypes = [ctypes.c_void_p]
+        lib.wd_batch_column_kernel.restype = ctypes.c_int
+        lib.wd_batch_column_kernel.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _lib = lib
     return _lib
 

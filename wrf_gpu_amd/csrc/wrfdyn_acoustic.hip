@@ -284,14 +284,10 @@ __global__ void __launch_bounds__(BX * BY) k_uv(DGrid g, DState s, DWork w, doub
 //   F: old ph work -> gamma
 // ---------------------------------------------------------------------------
 template <int TX, int KT, int R>
-__global__ void __launch_bounds__(TX * KT, (R <= 4 ? 3 : 1))
-    k_column(DGrid g, DNamelist nl, DState s, DWork w, double dts, int first) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void b_column(const int tx, const int kt, const unsigned t, double* lds, DGrid g,
+                                         DNamelist nl, DState s, DWork w, double dts, int first) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int tx = threadIdx.x % TX;
-  const int kt = threadIdx.x / TX;
   const unsigned nbx = (nx + TX - 1) / TX;
-  const unsigned t = tile_id(g);
   const int i = (int)(t % nbx) * TX + tx;
   const int j = (int)(t / nbx);
   const bool valid = i < nx;
@@ -749,6 +745,13 @@ __global__ void __launch_bounds__(TX * KT, (R <= 4 ? 3 : 1))
     }
   }
 #undef L
+}
+
+template <int TX, int KT, int R>
+__global__ void __launch_bounds__(TX * KT, (R <= 4 ? 3 : 1))
+    k_column(DGrid g, DNamelist nl, DState s, DWork w, double dts, int first) {
+  extern __shared__ double lds[];
+  b_column<TX, KT, R>(threadIdx.x % TX, threadIdx.x / TX, tile_id(g), lds, g, nl, s, w, dts, first);
 }
 
 // ---------------------------------------------------------------------------

@@ -64,6 +64,7 @@ struct wd_batch {
   // share one device allocation
   bool persistent = false;
   unsigned grid = 0;
+  std::vector<int> column_kernel;  // per member: kColumnMarch or the tile it runs
   void* pbuf = nullptr;
   wd::PBarrier* bar = nullptr;
 };
@@ -889,21 +890,27 @@ int wd_batch_create_persistent(wd_handle* const* members, const wd_state* states
   for (int m = 0; m < n_members; ++m)
     if (members[m]->device != dev) return fail(nullptr, WD_EINVAL, "members must live on the current device");
   std::vector<wd::PMember> pm(n_members);
+  std::vector<int> tiles(n_members);
   for (int m = 0; m < n_members; ++m) {
     wd::DState ds;
     int rc = check_state(nullptr, &states[m], &ds, members[m]->nl.moist);
     if (rc != WD_OK) return rc;
     pm[m] = wd::PMember{members[m]->g, members[m]->nl, ds, members[m]->w};
+    tiles[m] = members[m]->column_tile;
   }
+  std::vector<int> column_kernel(n_members);
+  for (int m = 0; m < n_members; ++m)
+    column_kernel[m] = wd::persistent_column_levels(pm[m].g, pm[m].nl, tiles[m]) > 0 ? tiles[m] : wd::kColumnMarch;
   std::vector<wd::PSeg> segs;
   std::vector<wd::PPhase> phases;
   std::string why;
-  if (!wd::persistent_program(pm, &segs, &phases, &why)) return fail(nullptr, WD_EINVAL, why);
+  if (!wd::persistent_program(pm, tiles, &segs, &phases, &why)) return fail(nullptr, WD_EINVAL, why);
   const bool mixed = pm[0].nl.mixed != 0;
+  const bool tile = std::any_of(column_kernel.begin(), column_kernel.end(), [](int k) { return k >= 0; });
   // as many workgroups as the GPU holds at once (the grid barrier needs every
   // one resident), a multiple of the 8 XCDs,
   int per_cu = 0, cus = 0;
-  e = wd::persistent_blocks_per_cu(mixed, &per_cu);
+  e = wd::persistent_blocks_per_cu(mixed, tile, &per_cu);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return hip_fail(nullptr, e, "occupancy of the persistent step");
   // and no more than the largest phase has virtual blocks: fewer workgroups
@@ -919,6 +926,7 @@ int wd_batch_create_persistent(wd_handle* const* members, const wd_state* states
   b->device = dev;
   b->persistent = true;
   b->grid = grid;
+  b->column_kernel = column_kernel;
   b->members.assign(members, members + n_members);
   b->states.resize(n_members);
   for (int m = 0; m < n_members; ++m) b->states[m] = pm[m].s;
@@ -954,7 +962,7 @@ int wd_batch_create_persistent(wd_handle* const* members, const wd_state* states
     wd_batch_destroy(b);
     return hip_fail(nullptr, e, "hipStreamBeginCapture");
   }
-  e = wd::launch_persistent_step(mixed, grid, reinterpret_cast<const wd::PMember*>(base + o_mem),
+  e = wd::launch_persistent_step(mixed, tile, grid, reinterpret_cast<const wd::PMember*>(base + o_mem),
                                  reinterpret_cast<const wd::PSeg*>(base + o_seg),
                                  reinterpret_cast<const wd::PPhase*>(base + o_ph), (int)phases.size(), b->bar, s0);
   hipGraph_t graph = nullptr;
@@ -1012,6 +1020,12 @@ int wd_batch_set_spin_limit(wd_batch* b, unsigned polls) {
 }
 
 int wd_batch_workgroups(const wd_batch* b) { return b ? (int)b->grid : WD_EINVAL; }
+
+int wd_batch_column_kernel(const wd_batch* b, int member, int* column_tile) {
+  if (!b || !column_tile || member < 0 || member >= (int)b->members.size()) return WD_EINVAL;
+  *column_tile = b->persistent ? b->column_kernel[member] : b->members[member]->column_tile;
+  return WD_OK;
+}
 
 int wd_step_batched(wd_batch* b, int n_large_steps, void* stream) {
   if (!b) return fail(nullptr, WD_EINVAL, "batch is NULL");

@@ -164,13 +164,21 @@ struct PBarrier {
   unsigned count, gen, abort, spin_limit;
 };
 // The program of one large step of the members (all of one precision form
-// and one substep count; no optional term but rotation).  Returns false with
-// a message for a member it cannot run.
-bool persistent_program(const std::vector<PMember>& members, std::vector<PSeg>* segs,
-                        std::vector<PPhase>* phases, std::string* why);
-// Workgroups of the persistent kernel one CU holds at once.
-hipError_t persistent_blocks_per_cu(bool mixed, int* out);
-hipError_t launch_persistent_step(bool mixed, unsigned grid, const PMember* members, const PSeg* segs,
+// and one substep count; no optional term but rotation).  column_tiles holds
+// each member's acoustic column kernel (kColumnMarch or a kColumnTiles index):
+// the 16 x 16 tile runs as such for up to 63 levels in the FP64 form, any
+// other choice as the column march, which gives the same bits.  Returns false
+// with a message for a member it cannot run.
+bool persistent_program(const std::vector<PMember>& members, const std::vector<int>& column_tiles,
+                        std::vector<PSeg>* segs, std::vector<PPhase>* phases, std::string* why);
+// Levels per thread with which the persistent step runs a member's column
+// tile, 0 where it runs the column march.
+int persistent_column_levels(const DGrid& g, const DNamelist& nl, int column_tile);
+// Workgroups of the persistent kernel one CU holds at once.  `tile`: the
+// FP64 kernel that also carries the column tile, for a program with a
+// PK_COLUMN_TILE segment.
+hipError_t persistent_blocks_per_cu(bool mixed, bool tile, int* out);
+hipError_t launch_persistent_step(bool mixed, bool tile, unsigned grid, const PMember* members, const PSeg* segs,
                                   const PPhase* phases, int n_phases, PBarrier* bar, hipStream_t st);
 
 // Adjoint of one plain-FP64 large step (wrfdyn_adjoint.hip).  The tape is

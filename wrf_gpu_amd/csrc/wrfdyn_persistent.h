@@ -41,12 +41,25 @@ enum PKind : int {
   PK_COLUMN_W,
   PK_FIN_UV,
   PK_FIN_COL,
+  PK_COLUMN_TILE,
 };
 
 namespace {
 
-// The tile bodies share one LDS buffer, the size of the largest (k_tend_v's).
+// The LDS-tiled acoustic column kernel the persistent step carries: the
+// 16 x 16 shape (16 columns, 16 threads per column, 256 threads), with up to
+// kPersistTileLevels levels per thread, so for nz + 1 <= 64 w levels.  Its LDS
+// is six (nz + 1) x 16 planes and three rows of 16.
+constexpr int kPersistTile = 0;  // index of the 16 x 16 shape in kColumnTiles
+constexpr int kPersistTileTx = 16, kPersistTileKt = 16, kPersistTileLevels = 4;
+constexpr int kPersistTileLds = (6 * kPersistTileKt * kPersistTileLevels + 3) * kPersistTileTx;
+
+// The tile bodies share one LDS buffer, the size of the largest (k_tend_v's);
+// the kernel that also carries the column tile sizes it for that tile's
+// deepest grid.
 constexpr int kPersistLds = 2 * kTendURows * kTendVFields * 64;
+constexpr int kPersistLdsTile = std::max(kPersistLds, kPersistTileLds);
+static_assert(kPersistTileTx * kPersistTileKt == 256, "the column tile runs on a 256-thread workgroup");
 static_assert(2 * kTendURows * 2 * 64 <= kPersistLds, "k_tend_u / k_tend_t tile");
 static_assert(2 * kColRows * 2 * 64 <= kPersistLds, "k_col_setup tile");
 static_assert(2 * kMuRows * 2 * 64 <= kPersistLds, "k_column_mu tile");
@@ -74,7 +87,7 @@ __device__ inline void origin_copy(const PMember& M, size_t e) {
 
 // Virtual block `local` of segment S, on the thread `tid` of a 256-thread
 // workgroup.
-template <bool MX>
+template <bool MX, bool TILE>
 __device__ __forceinline__ void run_segment(const PSeg& S, const PMember& M, unsigned local, unsigned tid,
                                             double* lds) {
   using TT = typename std::conditional<MX, float, double>::type;
@@ -142,6 +155,21 @@ __device__ __forceinline__ void run_segment(const PSeg& S, const PMember& M, uns
     case PK_FIN_COL:
       b_fin_col(flat, M.g, M.s, M.w);
       break;
+    case PK_COLUMN_TILE:
+      // FP64 only (the host never emits it for the mixed form); aux = levels per thread
+      if constexpr (!MX && TILE) {
+        constexpr int TX = kPersistTileTx, KT = kPersistTileKt;
+        const int tx = (int)(tid % TX), kt = (int)(tid / TX);
+        const unsigned t = tile_id(M.g, flat);
+        switch (S.aux) {
+          case 1: b_column<TX, KT, 1>(tx, kt, t, lds, M.g, M.nl, M.s, M.w, S.dts, S.first); break;
+          case 2: b_column<TX, KT, 2>(tx, kt, t, lds, M.g, M.nl, M.s, M.w, S.dts, S.first); break;
+          case 3: b_column<TX, KT, 3>(tx, kt, t, lds, M.g, M.nl, M.s, M.w, S.dts, S.first); break;
+          case 4: b_column<TX, KT, 4>(tx, kt, t, lds, M.g, M.nl, M.s, M.w, S.dts, S.first); break;
+          default: break;
+        }
+      }
+      break;
     default:
       break;
   }
@@ -190,17 +218,16 @@ __device__ bool grid_barrier(PBarrier* bar, int* ok) {
   return *ok != 0;
 }
 
-template <bool MX>
-__global__ void __launch_bounds__(256) k_step_persistent(const PMember* __restrict__ members,
-                                                        const PSeg* __restrict__ segs,
-                                                        const PPhase* __restrict__ phases, int n_phases,
-                                                        PBarrier* bar) {
-  __shared__ double lds[kPersistLds];
-  __shared__ int ok;
+// One large step: the phases in order, the workgroup taking every
+// gridDim.x-th virtual block of each, with a grid barrier between phases.
+template <bool MX, bool TILE>
+__device__ __forceinline__ void step_phases(const PMember* __restrict__ members, const PSeg* __restrict__ segs,
+                                            const PPhase* __restrict__ phases, int n_phases, PBarrier* bar,
+                                            double* lds, int* ok) {
   if (threadIdx.x == 0)
-    ok = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+    *ok = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
   __syncthreads();
-  if (!ok) return;  // an earlier launch timed out: the host must see it first
+  if (!*ok) return;  // an earlier launch timed out: the host must see it first
   const unsigned tid = threadIdx.x;
   for (int p = 0; p < n_phases; ++p) {
     const PPhase P = phases[p];
@@ -216,23 +243,56 @@ __global__ void __launch_bounds__(256) k_step_persistent(const PMember* __restri
       const unsigned local = v - S.vb0;
       if (local >= S.nvb) continue;  // the padding that aligns the next segment
       __syncthreads();               // the previous body's LDS reads are done
-      run_segment<MX>(S, members[S.member], local, tid, lds);
+      run_segment<MX, TILE>(S, members[S.member], local, tid, lds);
     }
-    if (p + 1 < n_phases && !grid_barrier(bar, &ok)) return;
+    if (p + 1 < n_phases && !grid_barrier(bar, ok)) return;
   }
+}
+
+// The step without the column tile (every member on the column march).
+template <bool MX>
+__global__ void __launch_bounds__(256) k_step_persistent(const PMember* __restrict__ members,
+                                                        const PSeg* __restrict__ segs,
+                                                        const PPhase* __restrict__ phases, int n_phases,
+                                                        PBarrier* bar) {
+  __shared__ double lds[kPersistLds];
+  __shared__ int ok;
+  step_phases<MX, false>(members, segs, phases, n_phases, bar, lds, &ok);
+}
+
+// The FP64 step with the 16 x 16 column tile among its bodies, held to two
+// waves per SIMD: the tile's body would otherwise take the whole register
+// file and leave one.  The extra registers and LDS slow the other bodies, so
+// a batch runs this kernel only when a member runs the tile.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    k_step_persistent_tile(const PMember* __restrict__ members, const PSeg* __restrict__ segs,
+                           const PPhase* __restrict__ phases, int n_phases, PBarrier* bar) {
+  __shared__ double lds[kPersistLdsTile];
+  __shared__ int ok;
+  step_phases<false, true>(members, segs, phases, n_phases, bar, lds, &ok);
 }
 
 }  // namespace
 
-bool persistent_program(const std::vector<PMember>& members, std::vector<PSeg>* segs, std::vector<PPhase>* phases,
-                        std::string* why) {
+int persistent_column_levels(const DGrid& g, const DNamelist& nl, int column_tile) {
+  if (column_tile != kPersistTile || nl.mixed) return 0;
+  const int r = (g.nz + 1 + kPersistTileKt - 1) / kPersistTileKt;
+  return r <= kPersistTileLevels ? r : 0;
+}
+
+bool persistent_program(const std::vector<PMember>& members, const std::vector<int>& column_tiles,
+                        std::vector<PSeg>* segs, std::vector<PPhase>* phases, std::string* why) {
   segs->clear();
   phases->clear();
-  if (members.empty()) {
-    *why = "no members";
+  if (members.empty() || column_tiles.size() != members.size()) {
+    *why = members.empty() ? "no members" : "one column kernel choice per member";
     return false;
   }
   const int nm = (int)members.size();
+  // levels per thread of the member's LDS column tile, 0 for the column march
+  std::vector<int> tile_levels(nm);
+  for (int m = 0; m < nm; ++m) tile_levels[m] = persistent_column_levels(members[m].g, members[m].nl, column_tiles[m]);
+  const bool any_march = std::count(tile_levels.begin(), tile_levels.end(), 0) > 0;
   const DNamelist& n0 = members[0].nl;
   for (int m = 0; m < nm; ++m) {
     const DNamelist& nl = members[m].nl;
@@ -266,6 +326,8 @@ bool persistent_program(const std::vector<PMember>& members, std::vector<PSeg>* 
         const DGrid& g = members[m].g;
         const DNamelist& nl = members[m].nl;
         if (jb.kind == PK_CORIOLIS && !g.rot) continue;
+        const bool march_kind = jb.kind == PK_COLUMN_MU || jb.kind == PK_COLUMN_W;
+        if ((march_kind && tile_levels[m] > 0) || (jb.kind == PK_COLUMN_TILE && tile_levels[m] == 0)) continue;
         const size_t C = (size_t)g.ny * g.nx, CU = (size_t)g.ny * (g.nx + 1), CV = (size_t)(g.ny + 1) * g.nx;
         PSeg S{};
         S.kind = jb.kind;
@@ -298,6 +360,10 @@ bool persistent_program(const std::vector<PMember>& members, std::vector<PSeg>* 
           case PK_COLUMN_W: S.gx = blocks_for(C, 256); break;
           case PK_FIN_UV: S.gx = blocks_for(CU + CV, 256); S.gy = fin_chunks(g); break;
           case PK_FIN_COL: S.gx = blocks_for(C, 256); break;
+          case PK_COLUMN_TILE:
+            S.gx = (unsigned)((g.nx + kPersistTileTx - 1) / kPersistTileTx) * (unsigned)g.ny;
+            S.aux = (unsigned)tile_levels[m];
+            break;
           default: break;
         }
         S.nvb = jb.kind == PK_ROM ? (S.gx + 3u) / 4u : S.gx * S.gy;
@@ -315,8 +381,8 @@ bool persistent_program(const std::vector<PMember>& members, std::vector<PSeg>* 
   // (rom) and k_prep_faces (face mu); k_prep_col (the column diagnostics)
   // and k_prep_u/v (u_work, v_work); the four tendencies (ut, vt, wt, tht);
   // the Coriolis terms (ut, vt) and k_col_setup (rw_tend, ph_tend and the
-  // implicit-w coefficients); the two finish kernels (u, v and the rest of
-  // the state).
+  // implicit-w coefficients); one member's k_column_mu and another's column
+  // tile; the two finish kernels (u, v and the rest of the state).
   add_phase({{PK_ORIGIN, 1, 0}});
   for (int st = 0; st < 3; ++st) {
     const int rk = st + 1;
@@ -328,25 +394,33 @@ bool persistent_program(const std::vector<PMember>& members, std::vector<PSeg>* 
     for (int it = 0; it < nsub[st]; ++it) {
       const int first = it == 0 ? 1 : 0;
       add_phase({{PK_UV, rk, first}});
-      add_phase({{PK_COLUMN_MU, rk, first}});
-      add_phase({{PK_COLUMN_W, rk, first}});
+      // a member runs either the column-march pair or the LDS column tile
+      add_phase({{PK_COLUMN_MU, rk, first}, {PK_COLUMN_TILE, rk, first}});
+      if (any_march) add_phase({{PK_COLUMN_W, rk, first}});
     }
     add_phase({{PK_FIN_UV, rk, 0}, {PK_FIN_COL, rk, 0}});
   }
   return true;
 }
 
-hipError_t persistent_blocks_per_cu(bool mixed, int* out) {
+hipError_t persistent_blocks_per_cu(bool mixed, bool tile, int* out) {
+  if (tile)
+    return mixed ? hipErrorInvalidValue
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_step_persistent_tile, 256, 0);
   return mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_step_persistent<true>, 256, 0)
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_step_persistent<false>, 256, 0);
 }
 
-hipError_t launch_persistent_step(bool mixed, unsigned grid, const PMember* members, const PSeg* segs,
+hipError_t launch_persistent_step(bool mixed, bool tile, unsigned grid, const PMember* members, const PSeg* segs,
                                   const PPhase* phases, int n_phases, PBarrier* bar, hipStream_t st) {
-  if (mixed)
+  if (tile) {
+    if (mixed) return hipErrorInvalidValue;
+    k_step_persistent_tile<<<grid, 256, 0, st>>>(members, segs, phases, n_phases, bar);
+  } else if (mixed) {
     k_step_persistent<true><<<grid, 256, 0, st>>>(members, segs, phases, n_phases, bar);
-  else
+  } else {
     k_step_persistent<false><<<grid, 256, 0, st>>>(members, segs, phases, n_phases, bar);
+  }
   return hipGetLastError();
 }
 

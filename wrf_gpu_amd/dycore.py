@@ -280,7 +280,9 @@ class Ensemble:
     ``persistent=True`` captures instead one launch of the persistent step
     per large step (``wd_batch_create_persistent``): every kernel body of
     every member runs from one launch of ``workgroups`` resident workgroups
-    that meet at a grid barrier between phases.  Members must share the
+    that meet at a grid barrier between phases.  With ``autotune=True`` a
+    member tuned to the 16x16 LDS column tile runs that tile inside the
+    launch (FP64, up to 63 levels; ``column_kernel`` tells).  Members must share the
     precision form and substep count and use no explicit diffusion, limiter
     or moisture; nothing else may run on the GPU during a step.  ``status``
     synchronises and raises if a barrier wait ran out since the last check.
@@ -319,6 +321,13 @@ class Ensemble:
         """Workgroups of the persistent step's launch (0 for the graph of kernel launches)."""
 
         return int(self._lib.wd_batch_workgroups(self._batch))
+
+    def column_kernel(self, member: int) -> int:
+        """The acoustic column kernel ``member`` runs in the ensemble's step: -1 the column march, else a tile index."""
+
+        out = ctypes.c_int(0)
+        self._check(self._lib.wd_batch_column_kernel(self._batch, int(member), ctypes.byref(out)))
+        return out.value
 
     def status(self, stream=None) -> None:
         """Synchronise ``stream``; raise if a persistent step's barrier wait ran out since the last call."""
