@@ -178,6 +178,11 @@ int wd_get_precision(const wd_handle *h);
  * form only the workgroup order is tuned. */
 int wd_autotune(wd_handle *h, const wd_state *state, int trial_steps, void *stream);
 int wd_tuned_config(const wd_handle *h, int *column_tile, int *xcd_remap);
+/* Applies a tuning (a saved wd_tuned_config, or a choice of the caller's):
+ * WD_EINVAL for a tile shape that does not hold the grid's levels, or any
+ * tile in the mixed form or with moisture, which run the column march.  A
+ * captured graph is dropped.  No reference counterpart. */
+int wd_set_tuned_config(wd_handle *h, int column_tile, int xcd_remap);
 
 /* Ensembles of independent members with differing grids (no counterpart in
  * the reference, which steps one domain per call).  wd_batch_create captures

@@ -177,6 +177,7 @@ def test_graph_batch_autotune_and_adjoint_entry_points_reject_bad_arguments():
     assert lib.wd_batch_workgroups(None) == _lib.WD_EINVAL
     tile = ctypes.c_int(5)
     assert lib.wd_batch_column_kernel(None, 0, ctypes.byref(tile)) == _lib.WD_EINVAL and tile.value == 5
+    assert lib.wd_set_tuned_config(None, -1, 1) == _lib.WD_EINVAL
 
 
 def test_namelist_validation_matches_supported_path():

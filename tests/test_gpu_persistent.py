@@ -152,6 +152,71 @@ def test_persistent_runs_tuned_column_tiles(oracle_lib):
         assert errs[k] <= 1e-12, (k, errs[k])
 
 
+def test_persistent_autotune_keeps_the_faster_column_choice():
+    """Ensemble(autotune=True, persistent=True): each member is tuned alone, then the persistent
+    step is timed with the members' tuned tiles and with every member on the column march, on
+    copies of the states, and the faster is kept.  The members' column kernels in the step follow
+    that choice, the states are not touched by the trials, and every member is bit for bit as
+    stepped alone."""
+
+    import torch
+    from wrf_gpu_amd import ic
+    from wrf_gpu_amd.dycore import Dycore, Ensemble, state_to_device, state_to_host
+
+    cases = [ic.bubble_3d(nx=64, ny=64, nz=40, dx=1000.0), ic.bubble_3d(nx=48, ny=40, nz=40, dx=1000.0, u0=5.0)]
+    alone = [(Dycore(c.metrics, c.namelist), state_to_device(c.state)) for c in cases]
+    batched = [(Dycore(c.metrics, c.namelist), state_to_device(c.state)) for c in cases]
+    ens = Ensemble(batched, persistent=True, autotune=True, trial_steps=1)
+    print(f"[persistent] autotuned {ens.tuned}, trial {ens.persistent_trial}", flush=True)
+    for m, (c, (_, s)) in enumerate(zip(cases, batched)):
+        _assert_bitwise(state_to_host(s), c.state, f"member {m} after the trials")
+    kernels = [ens.column_kernel(m) for m in range(len(cases))]
+    trial = ens.persistent_trial
+    if trial is None:  # no member picked a tile: nothing to compare
+        assert all(cfg["column_tile"] < 0 for cfg in ens.tuned) and kernels == [-1] * len(cases)
+    else:
+        assert trial["kept"] in ("tuned", "march") and trial["tuned_ms"] > 0 and trial["march_ms"] > 0
+        want = [cfg["column_tile"] if trial["kept"] == "tuned" and cfg["column_tile"] == 0 else -1 for cfg in ens.tuned]
+        assert kernels == want, (kernels, want)
+        for (d, _), cfg in zip(batched, ens.tuned):
+            now = d.tuned_config()
+            assert now["xcd_remap"] == cfg["xcd_remap"]
+            assert now["column_tile"] == (cfg["column_tile"] if trial["kept"] == "tuned" else -1)
+    ens.step(2)
+    for d, s in alone:
+        d.step(s, 2)
+    ens.status()
+    torch.cuda.synchronize()
+    for m, ((_, sa), (_, sb)) in enumerate(zip(alone, batched)):
+        _assert_bitwise(state_to_host(sb), state_to_host(sa), f"member {m}")
+    ens.close()
+    for d, _ in alone + batched:
+        d.close()
+
+
+def test_set_tuned_config_round_trip_and_refusals():
+    """wd_set_tuned_config applies a tuning as wd_tuned_config reports it and refuses a shape that
+    does not hold the levels, an unknown shape or order, and any tile in the mixed form."""
+
+    from wrf_gpu_amd import _lib, ic
+    from wrf_gpu_amd.dycore import Dycore
+
+    c = ic.stratified_terrain_3d(nx=20, ny=18, nz=70, dx=3000.0, u_sigma=1.0, sigma=0.5)
+    d = Dycore(c.metrics, c.namelist)
+    for cfg in ((0, 1), (1, 0), (-1, 1), (-1, 0)):
+        d.set_tuned_config(*cfg)
+        assert d.tuned_config() == {"column_tile": cfg[0], "xcd_remap": cfg[1]}
+    for bad in ((2, 1), (3, 0), (4, 1), (-2, 1), (0, 2)):  # 32x8 and 16x8 hold 64 levels, not 71
+        with pytest.raises(_lib.WrfdynError):
+            d.set_tuned_config(*bad)
+    assert d.tuned_config() == {"column_tile": -1, "xcd_remap": 0}
+    d.set_precision("mixed")
+    with pytest.raises(_lib.WrfdynError, match="column march"):
+        d.set_tuned_config(0, 1)
+    d.set_tuned_config(-1, 1)
+    d.close()
+
+
 def test_persistent_mixed_form_keeps_the_column_march():
     """The LDS column tile is FP64 only: a mixed-form member runs the column march."""
 

@@ -24,7 +24,7 @@ WD_PREC_MIXED = 1
 
 EXPORTED_SYMBOLS = ("wd_create", "wd_step", "wd_workspace_bytes", "wd_last_error", "wd_destroy",
                     "wd_profile_enable", "wd_profile_read", "wd_kernel_count", "wd_kernel_name",
-                    "wd_capture_graph", "wd_graph_step", "wd_autotune", "wd_tuned_config",
+                    "wd_capture_graph", "wd_graph_step", "wd_autotune", "wd_tuned_config", "wd_set_tuned_config",
                     "wd_set_precision", "wd_get_precision",
                     "wd_batch_create", "wd_step_batched", "wd_batch_branch", "wd_batch_branches",
                     "wd_batch_last_error", "wd_batch_destroy", "wd_step_adjoint", "wd_adjoint_tape_bytes",
@@ -127,6 +127,8 @@ def load() -> ctypes.CDLL:
         lib.wd_get_precision.argtypes = [ctypes.c_void_p]
         lib.wd_tuned_config.restype = ctypes.c_int
         lib.wd_tuned_config.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        lib.wd_set_tuned_config.restype = ctypes.c_int
+        lib.wd_set_tuned_config.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         lib.wd_batch_create.restype = ctypes.c_int
         lib.wd_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(WdState), ctypes.c_int,
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]

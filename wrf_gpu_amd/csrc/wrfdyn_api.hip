@@ -794,6 +794,26 @@ int wd_autotune(wd_handle* h, const wd_state* st, int trial_steps, void* stream)
   return WD_OK;
 }
 
+int wd_set_tuned_config(wd_handle* h, int column_tile, int xcd_remap) {
+  if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
+  if (xcd_remap != 0 && xcd_remap != 1) return fail(h, WD_EINVAL, "xcd_remap must be 0 or 1");
+  if (column_tile != wd::kColumnMarch) {
+    if (column_tile < 0 || column_tile >= wd::kNumColumnTiles)
+      return fail(h, WD_EINVAL, "column_tile must be -1 or a compiled tile shape index");
+    if (h->g.nz + 1 > wd::kColumnTiles[column_tile].kt * wd::kMaxLevelsPerThread)
+      return fail(h, WD_EINVAL, "the tile shape does not hold the grid's levels");
+    if (h->nl.mixed || h->nl.moist)
+      return fail(h, WD_EINVAL, "the mixed form and moist states run the column march (column_tile -1)");
+  }
+  h->column_tile = column_tile;
+  h->g.xcd_remap = xcd_remap;
+  if (h->graph) {  // a captured graph holds the old launch shapes
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return WD_OK;
+}
+
 int wd_set_precision(wd_handle* h, int mode) {
   if (!h) return fail(nullptr, WD_EINVAL, "handle is NULL");
   if (mode != WD_PREC_FP64 && mode != WD_PREC_MIXED) return fail(h, WD_EINVAL, "precision must be WD_PREC_FP64 or WD_PREC_MIXED");

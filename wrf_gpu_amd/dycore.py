@@ -227,6 +227,12 @@ class Dycore:
         _lib.check(self._lib.wd_tuned_config(self._handle, ctypes.byref(t), ctypes.byref(r)), self._handle)
         return {"column_tile": t.value, "xcd_remap": r.value}
 
+    def set_tuned_config(self, column_tile: int, xcd_remap: int) -> None:
+        """Apply a tuning (as ``tuned_config`` returns it) without timing anything."""
+
+        _lib.check(self._lib.wd_set_tuned_config(self._handle, int(column_tile), int(xcd_remap)), self._handle)
+        self._graph_state = None
+
     def profile(self, kernels) -> None:
         """Time the named kernels (see _lib.kernel_names()) with HIP events in later steps."""
 
@@ -282,7 +288,10 @@ class Ensemble:
     every member runs from one launch of ``workgroups`` resident workgroups
     that meet at a grid barrier between phases.  With ``autotune=True`` a
     member tuned to the 16x16 LDS column tile runs that tile inside the
-    launch (FP64, up to 63 levels; ``column_kernel`` tells).  Members must share the
+    launch (FP64, up to 63 levels; ``column_kernel`` tells), unless the
+    persistent step with every member on the column march is faster: both
+    are timed on copies of the states (``persistent_trial``) and the faster
+    kept.  Members must share the
     precision form and substep count and use no explicit diffusion, limiter
     or moisture; nothing else may run on the GPU during a step.  ``status``
     synchronises and raises if a barrier wait ran out since the last check.
@@ -301,6 +310,9 @@ class Ensemble:
             if state["u"].device != dev:
                 raise ValueError("all members must live on the same device")
         self.tuned = [dyn.autotune(state, trial_steps) for dyn, state in self.members] if autotune else None
+        self.persistent_trial = None
+        if persistent and autotune and any(cfg["column_tile"] >= 0 for cfg in self.tuned):
+            self.persistent_trial = self._persistent_trial(trial_steps)
         n = len(self.members)
         handles = (ctypes.c_void_p * n)(*[d._handle for d, _ in self.members])
         states = (_lib.WdState * n)(*[_state_ptrs(s, d.namelist.moist) for d, s in self.members])
@@ -315,6 +327,41 @@ class Ensemble:
         self._lib = lib
         self.device = dev
         self.persistent = bool(persistent)
+
+    def _persistent_trial(self, trial_steps: int) -> dict:
+        """Time the persistent step with the members' own tuned column tiles and with every member on
+        the column march (same bits), on copies of the states, and keep the faster: each member was
+        tuned alone, without the other members or the persistent kernel's occupancy."""
+
+        import torch
+
+        clones = [(d, {k: v.clone() for k, v in s.items()}) for d, s in self.members]
+        stream = torch.cuda.current_stream(self.members[0][1]["u"].device)
+
+        def timed() -> float:
+            ens = Ensemble(clones, persistent=True)
+            try:
+                ens.step(1, stream)
+                ens.status(stream)
+                t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                t0.record(stream)
+                ens.step(max(int(trial_steps), 1), stream)
+                t1.record(stream)
+                ens.status(stream)
+                return t0.elapsed_time(t1)
+            finally:
+                ens.close()
+
+        tuned_ms = timed()
+        for (d, _), cfg in zip(self.members, self.tuned):
+            d.set_tuned_config(-1, cfg["xcd_remap"])
+        march_ms = timed()
+        kept = "tuned" if tuned_ms < march_ms else "march"
+        if kept == "tuned":
+            for (d, _), cfg in zip(self.members, self.tuned):
+                d.set_tuned_config(cfg["column_tile"], cfg["xcd_remap"])
+        del clones
+        return {"tuned_ms": tuned_ms, "march_ms": march_ms, "kept": kept}
 
     @property
     def workgroups(self) -> int:
