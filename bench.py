@@ -461,14 +461,17 @@ def ensemble_leg(steps: int, dev) -> dict:
     ens.step(1)
     ms = timed_steps(ens.step, steps, stream) / steps
     picks = {}
+    tuned = list(ens.tuned)
     for cfg in ens.tuned:
         key = f"column_tile={cfg['column_tile']},xcd_remap={cfg['xcd_remap']}"
         picks[key] = picks.get(key, 0) + 1
     out["batched_graph_fp64_autotuned"] = {"ms_per_step": ms, "value": cells * ns / (ms * 1e-3),
                                            "branches": ens.branches, "member_picks": picks,
-                                           "autotune_s": tune_s}
+                                           "trial": ens.batch_trial, "autotune_s": tune_s}
     ens.close()
-    # the persistent step with the same autotuned members: tuned 16x16 column tiles run inside the launch
+    # the persistent step with the members' picks made alone: tuned 16x16 column tiles run inside the launch
+    for (d, _), cfg in zip(members, tuned):
+        d.set_tuned_config(cfg["column_tile"], cfg["xcd_remap"])
     ens = Ensemble(members, persistent=True)
     ens.step(1)
     ens.status()

@@ -109,9 +109,10 @@ def test_batched_members_match_individual_steps(oracle_lib, branches):
 
 
 def test_autotuned_ensemble_matches_members_alone(oracle_lib):
-    """Ensemble(autotune=True) tunes every member alone, then captures each with its pick
-    (the LDS-tiled column kernel or the column march, either workgroup order); the result is
-    still bitwise each member stepped alone, a moist member included."""
+    """Ensemble(autotune=True) tunes every member alone, times the batch with those picks and
+    with every member on the column march, and captures the faster (the LDS-tiled column kernel
+    or the column march, either workgroup order); the states are untouched by the trials, and the
+    result is still bitwise each member stepped alone, a moist member included."""
 
     import torch
     from wrf_gpu_amd import ic
@@ -128,6 +129,17 @@ def test_autotuned_ensemble_matches_members_alone(oracle_lib):
     batched = [(Dycore(c.metrics, c.namelist), state_to_device(c.state)) for c in cases]
     ens = Ensemble(batched, branches=2, autotune=True, trial_steps=1)
     assert len(ens.tuned) == len(cases) and ens.tuned[2]["column_tile"] == -1  # moist: the column march
+    print(f"[batch] picks {ens.tuned}, trial {ens.batch_trial}", flush=True)
+    for m, (c, (_, s)) in enumerate(zip(cases, batched)):  # tuning and the batch trial ran on copies
+        _assert_bitwise(state_to_host(s), c.state, f"member {m} after tuning")
+    trial = ens.batch_trial
+    if trial is None:
+        assert all(cfg["column_tile"] < 0 for cfg in ens.tuned)
+    else:
+        assert trial["kept"] in ("tuned", "march") and ens.persistent_trial is None
+    for m, cfg in enumerate(ens.tuned):  # the batch runs the kept choice
+        want = cfg["column_tile"] if trial is None or trial["kept"] == "tuned" else -1
+        assert ens.column_kernel(m) == want and batched[m][0].tuned_config()["column_tile"] == want
     ens.step(2)
     for d, s in alone:
         d.step(s, 2)

@@ -277,11 +277,13 @@ class Ensemble:
     identical to ``Dycore.step`` on that member alone.
 
     ``autotune=True`` first runs each member's autotuner (``Dycore.autotune``,
-    on a copy of its state, ``trial_steps`` timed steps per candidate) and
-    captures every member with the column kernel and workgroup order it
-    picked alone; ``tuned`` then lists those choices.  Tuned alone, a member
-    does not see the other branches running beside it, so its pick is not
-    necessarily the fastest inside the ensemble.
+    on a copy of its state, ``trial_steps`` timed steps per candidate);
+    ``tuned`` then lists the choices made alone.  Tuned alone, a member does
+    not see the other members running beside it, so if some member picked an
+    LDS column tile, the batched step is then timed on copies of the states
+    with those picks and with every member on the column march (same bits),
+    and the faster is kept (``batch_trial``).  Each member keeps its tuned
+    workgroup order.
 
     ``persistent=True`` captures instead one launch of the persistent step
     per large step (``wd_batch_create_persistent``): every kernel body of
@@ -310,9 +312,9 @@ class Ensemble:
             if state["u"].device != dev:
                 raise ValueError("all members must live on the same device")
         self.tuned = [dyn.autotune(state, trial_steps) for dyn, state in self.members] if autotune else None
-        self.persistent_trial = None
-        if persistent and autotune and any(cfg["column_tile"] >= 0 for cfg in self.tuned):
-            self.persistent_trial = self._persistent_trial(trial_steps)
+        self.batch_trial = None
+        if autotune and any(cfg["column_tile"] >= 0 for cfg in self.tuned):
+            self.batch_trial = self._batch_trial(trial_steps, persistent, branches)
         n = len(self.members)
         handles = (ctypes.c_void_p * n)(*[d._handle for d, _ in self.members])
         states = (_lib.WdState * n)(*[_state_ptrs(s, d.namelist.moist) for d, s in self.members])
@@ -328,10 +330,17 @@ class Ensemble:
         self.device = dev
         self.persistent = bool(persistent)
 
-    def _persistent_trial(self, trial_steps: int) -> dict:
-        """Time the persistent step with the members' own tuned column tiles and with every member on
-        the column march (same bits), on copies of the states, and keep the faster: each member was
-        tuned alone, without the other members or the persistent kernel's occupancy."""
+    @property
+    def persistent_trial(self):
+        """The batch trial of a persistent ensemble (None for a graph ensemble or without a trial)."""
+
+        return self.batch_trial if self.persistent else None
+
+    def _batch_trial(self, trial_steps: int, persistent: bool, branches: int) -> dict:
+        """Time the batched step (the persistent step, or the graph on ``branches`` branches) with the
+        members' own tuned column tiles and with every member on the column march (same bits), on
+        copies of the states, and keep the faster: each member was tuned alone, without the other
+        members beside it or the persistent kernel's occupancy."""
 
         import torch
 
@@ -339,7 +348,7 @@ class Ensemble:
         stream = torch.cuda.current_stream(self.members[0][1]["u"].device)
 
         def timed() -> float:
-            ens = Ensemble(clones, persistent=True)
+            ens = Ensemble(clones, branches=branches, persistent=persistent)
             try:
                 ens.step(1, stream)
                 ens.status(stream)
