@@ -70,10 +70,11 @@ def _dycore(case, precision, column=None):
             os.environ["WRFDYN_COLUMN_TILE"] = keep
 
 
-def _run_pair(cases, steps, precision="fp64", columns=None, expect_kernels=None):
+def _run_pair(cases, steps, precision="fp64", columns=None, expect_kernels=None, remaps=None):
     """Each case stepped alone and in one persistent ensemble; returns both host states per case.
     ``columns`` gives each member's column kernel choice, ``expect_kernels`` the column kernel
-    the persistent step must then run for each (-1 the column march, 0 the 16x16 tile)."""
+    the persistent step must then run for each (-1 the column march, 0 the 16x16 tile), and
+    ``remaps`` each member's workgroup order (1 XCD-contiguous, 0 hardware order)."""
 
     import torch
     from wrf_gpu_amd.dycore import Ensemble, state_to_device, state_to_host
@@ -81,6 +82,9 @@ def _run_pair(cases, steps, precision="fp64", columns=None, expect_kernels=None)
     columns = columns or [None] * len(cases)
     alone = [(_dycore(c, precision, col), state_to_device(c.state)) for c, col in zip(cases, columns)]
     batched = [(_dycore(c, precision, col), state_to_device(c.state)) for c, col in zip(cases, columns)]
+    for m, remap in enumerate(remaps or []):
+        for d, _ in (alone[m], batched[m]):
+            d.set_tuned_config(d.tuned_config()["column_tile"], remap)
     ens = Ensemble(batched, persistent=True)
     assert ens.branches == 1 and ens.workgroups > 0 and ens.workgroups % 8 == 0
     kernels = [ens.column_kernel(m) for m in range(len(cases))]
@@ -150,6 +154,19 @@ def test_persistent_runs_tuned_column_tiles(oracle_lib):
     errs = field_errors(out[2][1], ref)
     for k in FIELDS:
         assert errs[k] <= 1e-12, (k, errs[k])
+
+
+def test_persistent_column_tile_with_rotation_and_hardware_order():
+    """The column tile inside the launch beside the Coriolis phase (a rotating member), and in
+    hardware workgroup order as well as the XCD-contiguous order: bit for bit as stepped alone."""
+
+    from wrf_gpu_amd import ic
+
+    terrain = dict(dx=3000.0, u_sigma=1.0, sigma=0.5)
+    cases = [_rotating(ic.stratified_terrain_3d(nx=20, ny=18, nz=30, **terrain), 45.0),
+             ic.stratified_terrain_3d(nx=33, ny=17, nz=50, **terrain),
+             _rotating(ic.bubble_3d(nx=24, ny=20, nz=10, dx=1000.0, u0=10.0), 30.0)]
+    _run_pair(cases, (2,), columns=["16x16", "16x16", "march"], expect_kernels=[0, 0, -1], remaps=[1, 0, 0])
 
 
 def test_persistent_autotune_keeps_the_faster_column_choice():
